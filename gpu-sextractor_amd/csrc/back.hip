@@ -1,0 +1,648 @@
+/*
+ * back.hip -- background estimation and subtraction on gfx950 (back_estimate,
+ * the reference's src/cuda/cudaback.cu:1212), bit-exact with the CPU code src/back.c.
+ *
+ * Kernels:
+ *   k_meshstat   one lane per mesh: backstat (back.c:214-290).  The double sums of pix
+ *                and pix^2 are sequential in raster order inside the mesh, as back.c
+ *                does, so each lane runs the chain; float4 loads cut the load count.
+ *   k_meshmode   one workgroup per mesh: histogram in LDS (back.c:297-341), then the
+ *                clipped-mode iteration of backguess (back.c:347-426).  Inside the
+ *                iteration every sum is an exact integer (< 2^53), so the sequential
+ *                double sums equal prefix-sum differences; the two-pointer median walk
+ *                is a merge of two non-decreasing sequences and its end point is found
+ *                by a binary search on prefix sums (merge-path), which reproduces the
+ *                walk exactly.
+ *   k_filterback one workgroup: bad-mesh interpolation and median filtering
+ *                (back.c:434-560), global medians, y second derivatives
+ *                (back.c:739-780).
+ *   k_rownodes   one lane per frame row: node interpolation along y and the x
+ *                tridiagonal solve of subbackline (back.c:790-880).
+ *   k_subback    one lane per pixel: bicubic evaluation with the column tables of the
+ *                cumulative dx recurrence (back.c:880-908), computed once on the host
+ *                with the same float additions, and in-place subtraction.
+ */
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sxg_internal.h"
+
+#define QUANTIF_NSIGMA 5
+#define QUANTIF_AMIN 4
+#define BACK_MINGOODFRAC 0.5
+#define BG_EPS (1e-4)
+
+struct MeshStat {
+    float mean, sigma, lcut, hcut, qzero, qscale;
+    int nlevels, npix;
+};
+
+/* ---------------------------------------------------------------- backstat ---- */
+__global__ void __launch_bounds__(64) k_meshstat(const float *__restrict__ img, int W, int H,
+                                                 int bs, int nbx, int nmesh, MeshStat *ms)
+{
+    const int m = blockIdx.x * 64 + threadIdx.x;
+    if (m >= nmesh) return;
+    const int mx = m % nbx, my = m / nbx;
+    const int x0 = mx * bs, y0 = my * bs;
+    const int bw = (x0 + bs <= W) ? bs : W - x0;
+    const int bh = (y0 + bs <= H) ? bs : H - y0;
+    const double step = sqrt(2 / SXG_PI) * QUANTIF_NSIGMA / QUANTIF_AMIN;
+    double mean = 0.0, sigma = 0.0, pix, sig;
+    int npix = 0;
+    const bool vec = ((x0 & 3) == 0) && ((bw & 3) == 0) && ((W & 3) == 0);
+    for (int y = 0; y < bh; y++) {
+        const float *r = img + (size_t)(y0 + y) * W + x0;
+        if (vec) {
+            for (int x = 0; x < bw; x += 4) {
+                float4 v = *reinterpret_cast<const float4 *>(r + x);
+                float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if ((pix = vv[k]) > -SXG_BIG) {
+                        mean += pix;
+                        sigma += pix * pix;
+                        npix++;
+                    }
+            }
+        } else {
+            for (int x = 0; x < bw; x++)
+                if ((pix = r[x]) > -SXG_BIG) {
+                    mean += pix;
+                    sigma += pix * pix;
+                    npix++;
+                }
+        }
+    }
+    MeshStat o;
+    o.lcut = o.hcut = o.qzero = o.qscale = 0.0f;
+    o.nlevels = o.npix = 0;
+    if ((float)npix < (float)(bw * bh * BACK_MINGOODFRAC)) {
+        o.mean = o.sigma = (float)-SXG_BIG;
+        ms[m] = o;
+        return;
+    }
+    mean /= (double)npix;
+    sigma = (sig = sigma / npix - mean * mean) > 0.0 ? sqrt(sig) : 0.0;
+    const float lcut = (float)(mean - 2.0 * sigma);
+    const float hcut = (float)(mean + 2.0 * sigma);
+    o.lcut = lcut;
+    o.hcut = hcut;
+    mean = sigma = 0.0;
+    npix = 0;
+    for (int y = 0; y < bh; y++) {
+        const float *r = img + (size_t)(y0 + y) * W + x0;
+        if (vec) {
+            for (int x = 0; x < bw; x += 4) {
+                float4 v = *reinterpret_cast<const float4 *>(r + x);
+                float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    pix = vv[k];
+                    if (pix <= hcut && pix >= lcut) {
+                        mean += pix;
+                        sigma += pix * pix;
+                        npix++;
+                    }
+                }
+            }
+        } else {
+            for (int x = 0; x < bw; x++) {
+                pix = r[x];
+                if (pix <= hcut && pix >= lcut) {
+                    mean += pix;
+                    sigma += pix * pix;
+                    npix++;
+                }
+            }
+        }
+    }
+    o.npix = npix;
+    mean /= (double)npix;
+    sig = sigma / npix - mean * mean;
+    sigma = sig > 0.0 ? sqrt(sig) : 0.0;
+    o.mean = (float)mean;
+    o.sigma = (float)sigma;
+    if ((o.nlevels = (int)(step * npix + 1)) > SXG_NMAXLEVELS) o.nlevels = SXG_NMAXLEVELS;
+    o.qscale = (float)(sigma > 0.0 ? 2 * QUANTIF_NSIGMA * sigma / o.nlevels : 1.0);
+    o.qzero = (float)(mean - QUANTIF_NSIGMA * sigma);
+    ms[m] = o;
+}
+
+/* --------------------------------------------------------------- backguess ---- */
+#define MODE_THREADS 256
+
+__device__ inline int d2i_sat(double v)
+{
+    if (v != v) return 0;
+    if (v >= 2147483647.0) return 2147483647;
+    if (v <= -2147483648.0) return (int)-2147483647 - 1;
+    return (int)v;
+}
+
+/* workgroup sum of 64-bit values, MODE_THREADS threads (every thread gets the total) */
+__device__ inline long long wg_sum64(long long v, long long *sm)
+{
+    const int t = threadIdx.x;
+    sm[t] = v;
+    __syncthreads();
+    for (int off = MODE_THREADS / 2; off > 0; off >>= 1) {
+        if (t < off) sm[t] += sm[t + off];
+        __syncthreads();
+    }
+    long long r = sm[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restrict__ img, int W,
+                                                           int H, int bs, int nbx,
+                                                           const MeshStat *ms, float *back,
+                                                           float *sigma)
+{
+    /* h[i]: histogram; P0[k] = sum_{j<k} h[j] for k in [0, 4096] */
+    __shared__ int h[SXG_NMAXLEVELS];
+    __shared__ int P0[SXG_NMAXLEVELS + 1];
+    __shared__ long long sm[MODE_THREADS];
+    const int m = blockIdx.x;
+    const MeshStat st = ms[m];
+    if (st.mean <= -SXG_BIG) {
+        if (threadIdx.x == 0) back[m] = sigma[m] = (float)-SXG_BIG;
+        return;
+    }
+    const int mx = m % nbx, my = m / nbx;
+    const int x0 = mx * bs, y0 = my * bs;
+    const int bw = (x0 + bs <= W) ? bs : W - x0;
+    const int bh = (y0 + bs <= H) ? bs : H - y0;
+    const int nlev = st.nlevels;
+    for (int i = threadIdx.x; i < SXG_NMAXLEVELS; i += MODE_THREADS) h[i] = 0;
+    __syncthreads();
+    const float qscale = st.qscale;
+    const float cste = (float)(0.499999 - st.qzero / qscale);
+    const int npx = bw * bh;
+    for (int k = threadIdx.x; k < npx; k += MODE_THREADS) {
+        const int y = k / bw, x = k - y * bw;
+        float t = img[(size_t)(y0 + y) * W + x0 + x] / qscale + cste;
+        if (t > -1.0f && t < (float)nlev) atomicAdd(&h[(int)t], 1);
+    }
+    __syncthreads();
+    /* prefix counts: thread t owns bins [t*16, t*16+16) */
+    const int per = SXG_NMAXLEVELS / MODE_THREADS;
+    {
+        const int b0 = threadIdx.x * per;
+        int s0 = 0;
+        for (int j = b0; j < b0 + per; j++) s0 += h[j];
+        sm[threadIdx.x] = s0;
+        __syncthreads();
+        for (int off = 1; off < MODE_THREADS; off <<= 1) {
+            long long add = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
+            __syncthreads();
+            sm[threadIdx.x] += add;
+            __syncthreads();
+        }
+        int e0 = (int)(sm[threadIdx.x] - s0);
+        for (int j = b0; j < b0 + per; j++) {
+            P0[j] = e0;
+            e0 += h[j];
+        }
+        if (threadIdx.x == MODE_THREADS - 1) P0[SXG_NMAXLEVELS] = e0;
+        __syncthreads();
+    }
+
+    /* The iteration of backguess.  Every thread runs the same control flow on the same
+       shared values; the window sums are collective. */
+    auto hget = [&](long i) -> long { return (i >= 0 && i < nlev) ? h[i] : 0; };
+    unsigned long lowsum, highsum, sum;
+    double ftemp, mea, sig, sig1, med;
+    int n, lcut, hcut, nlevelsm1;
+    hcut = nlevelsm1 = nlev - 1;
+    lcut = 0;
+    sig = 10.0 * nlevelsm1;
+    sig1 = 1.0;
+    mea = med = st.mean;
+    for (n = 100; n-- && (sig >= 0.1) && (fabs(sig / sig1 - 1.0) > BG_EPS);) {
+        long lo, hi;
+        sig1 = sig;
+        const long nb = (long)hcut - lcut + 1;
+        if (nb <= 0) {
+            sum = 0;
+            mea = sig = 0.0;
+            lowsum = highsum = 0;
+            lo = lcut;
+            hi = hcut;
+        } else {
+            /* exact integer window sums (each below 2^53, so the sequential double sums
+               of back.c equal them) */
+            long long s1 = 0, s2 = 0;
+            for (int j = lcut + threadIdx.x; j <= hcut; j += MODE_THREADS) {
+                long long c = h[j];
+                s1 += c * j;
+                s2 += c * j * j;
+            }
+            s1 = wg_sum64(s1, sm);
+            s2 = wg_sum64(s2, sm);
+            sum = (unsigned long)(P0[hcut + 1] - P0[lcut]);
+            mea = (double)s1;
+            sig = (double)s2;
+            /* a = #{i in [0,nb): L(i) < H(nb-1-i)},
+               L(i) = P0[lcut+i]-P0[lcut], H(nb-1-i) = P0[hcut+1]-P0[lcut+i+1];
+               the predicate holds for i < a and fails from a on */
+            const int base = P0[lcut], top = P0[hcut + 1];
+            long a_lo = 0, a_hi = nb;
+            while (a_lo < a_hi) {
+                long mid = (a_lo + a_hi) >> 1;
+                int L = P0[lcut + mid] - base;
+                int Hh = top - P0[lcut + mid + 1];
+                if (L < Hh) a_lo = mid + 1; else a_hi = mid;
+            }
+            lo = lcut + a_lo;
+            hi = lo - 1;
+            lowsum = (unsigned long)(P0[lo] - base);
+            highsum = (unsigned long)(top - P0[lo]);
+        }
+        if (hi >= 0) {
+            long hl = hget(lo), hh = hget(hi);
+            med = hi + 0.5 + ((double)highsum - lowsum) / (2.0 * (hl > hh ? hl : hh));
+        } else
+            med = 0.0;
+        if (sum) {
+            mea /= (double)sum;
+            sig = sig / sum - mea * mea;
+        }
+        sig = sig > 0.0 ? sqrt(sig) : 0.0;
+        lcut = (ftemp = med - 3.0 * sig) > 0.0 ? d2i_sat(ftemp > 0.0 ? ftemp + 0.5 : ftemp - 0.5) : 0;
+        hcut = (ftemp = med + 3.0 * sig) < nlevelsm1
+                   ? d2i_sat(ftemp > 0.0 ? ftemp + 0.5 : ftemp - 0.5)
+                   : nlevelsm1;
+    }
+    if (threadIdx.x != 0) return;
+    float mode;
+    if (fabs(sig) > 0.0) {
+        if (fabs(st.sigma / (sig * st.qscale) - 1) < 0.0)
+            mode = (float)(st.qzero + mea * st.qscale);
+        else if (fabs((mea - med) / sig) < 0.3)
+            mode = (float)(st.qzero + (2.5 * med - 1.5 * mea) * st.qscale);
+        else
+            mode = (float)(st.qzero + med * st.qscale);
+    } else
+        mode = (float)(st.qzero + mea * st.qscale);
+    back[m] = mode;
+    sigma[m] = (float)(sig * st.qscale);
+}
+
+/* -------------------------------------------------------------- filterback ---- */
+#define FB_THREADS 1024
+
+/* median of a small array (sorted in place), hmedian's result (misc.c:30-72) */
+__device__ inline float small_median(float *a, int n)
+{
+    for (int i = 1; i < n; i++) {
+        float v = a[i];
+        int j = i - 1;
+        while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
+        a[j + 1] = v;
+    }
+    if (n < 2) return a[0];
+    return (n & 1) ? a[n / 2] : (float)((a[n / 2 - 1] + a[n / 2]) / 2.0);
+}
+
+/* ascending bitonic sort of n2 (power of two) floats in global memory, one workgroup */
+__device__ void wg_bitonic(float *a, int n2)
+{
+    for (int k = 2; k <= n2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += FB_THREADS) {
+                int l = i ^ j;
+                if (l > i) {
+                    float x = a[i], y = a[l];
+                    bool up = (i & k) == 0;
+                    if ((x > y) == up) { a[i] = y; a[l] = x; }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+/* median of a sorted array, hmedian's even-n rule */
+__device__ inline float sorted_median(const float *a, int n)
+{
+    if (n < 2) return a[0];
+    return (n & 1) ? a[n / 2] : (float)((a[n / 2 - 1] + a[n / 2]) / 2.0);
+}
+
+/* maps: back, sigma (unfiltered in, filtered out), dback, dsigma; work: 4*n2 floats;
+   out[0..1]: backmean, backsig */
+__global__ void __launch_bounds__(FB_THREADS) k_filterback(float *back, float *sigma,
+                                                           float *dback, float *dsigma,
+                                                           int nx, int ny, int nfx, int nfy,
+                                                           double backfthresh, float *work,
+                                                           int n2, float *out)
+{
+    const int np = nx * ny;
+    float *back2 = work, *sigma2 = work + n2, *sbuf = work + 2 * n2, *u = work + 3 * n2;
+    const float fthresh = (float)backfthresh;
+    /* bad meshes: average of the nearest valid meshes (back.c:461-490) */
+    for (int i = threadIdx.x; i < np; i += FB_THREADS) {
+        back2[i] = back[i];
+        sigma2[i] = sigma[i];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += FB_THREADS) {
+        if (back2[i] > -SXG_BIG) continue;
+        const int px = i % nx, py = i / nx;
+        float d2min = (float)SXG_BIG, val = 0.0f, sval = 0.0f;
+        int nmin = 0;
+        for (int j = 0, y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++, j++)
+                if (back2[j] > -SXG_BIG) {
+                    float d2 = (float)(x - px) * (x - px) + (y - py) * (y - py);
+                    if (d2 < d2min) {
+                        val = back2[j];
+                        sval = sigma2[j];
+                        nmin = 1;
+                        d2min = d2;
+                    } else if (d2 == d2min) {
+                        val += back2[j];
+                        sval += sigma2[j];
+                        nmin++;
+                    }
+                }
+        back[i] = nmin ? val / nmin : 0.0f;
+        sigma[i] = nmin ? sval / nmin : 1.0f;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += FB_THREADS)
+        if (back2[i] > -SXG_BIG) { back[i] = back2[i]; sigma[i] = sigma2[i]; }
+    __syncthreads();
+    /* median filtering with a window clipped symmetrically at the edges */
+    const int npx = nfx / 2, npy = nfy / 2;
+    for (int i = threadIdx.x; i < np; i += FB_THREADS) {
+        const int px = i % nx, py = i / nx;
+        int npx2 = nx - px - 1, npy2 = ny - py - 1;
+        if (npx2 > npx) npx2 = npx;
+        if (npx2 > px) npx2 = px;
+        if (npy2 > npy) npy2 = npy;
+        if (npy2 > py) npy2 = py;
+        float bm[SXG_MAXBACKF * SXG_MAXBACKF], sm_[SXG_MAXBACKF * SXG_MAXBACKF];
+        int k = 0;
+        for (int dy = -npy2; dy <= npy2; dy++)
+            for (int dx = -npx2; dx <= npx2; dx++) {
+                bm[k] = back[(py + dy) * nx + px + dx];
+                sm_[k++] = sigma[(py + dy) * nx + px + dx];
+            }
+        float med = small_median(bm, k);
+        if (fabs(med - back[i]) >= fthresh) {
+            back2[i] = med;
+            sigma2[i] = small_median(sm_, k);
+        } else {
+            back2[i] = back[i];
+            sigma2[i] = sigma[i];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += FB_THREADS) {
+        back[i] = back2[i];
+        sigma[i] = sigma2[i];
+    }
+    /* global medians: sort copies padded with +inf */
+    for (int i = threadIdx.x; i < n2; i += FB_THREADS) {
+        sbuf[i] = i < np ? back2[i] : __builtin_inff();
+        u[i] = i < np ? sigma2[i] : __builtin_inff();
+    }
+    __syncthreads();
+    wg_bitonic(sbuf, n2);
+    wg_bitonic(u, n2);
+    if (threadIdx.x == 0) {
+        out[0] = sorted_median(sbuf, np);
+        float bsig = sorted_median(u, np);
+        if (bsig <= 0.0) {
+            /* median of the strictly positive sigmas (back.c:542-555) */
+            int i = np;
+            while (i-- && u[i] > 0.0f) {}
+            if (i >= 0 && i < np - 1)
+                bsig = sorted_median(u + i + 1, np - 1 - i);
+            else
+                bsig = 1.0f;
+        }
+        out[1] = bsig;
+    }
+    __syncthreads();
+    /* y second derivatives, one lane per mesh column (back.c:739-780); u reused */
+    for (int c = threadIdx.x; c < 2 * nx; c += FB_THREADS) {
+        const int x = c % nx;
+        const float *map = c < nx ? back : sigma;
+        float *dmap = c < nx ? dback : dsigma;
+        float *uu = (c < nx ? back2 : sigma2) + x * (ny > 1 ? ny - 1 : 1);
+        if (ny > 1) {
+            float temp;
+            dmap[x] = uu[0] = 0.0f;
+            for (int y = 1; y < ny - 1; y++) {
+                temp = -1 / (dmap[(y - 1) * nx + x] + 4);
+                dmap[y * nx + x] = temp;
+                temp *= uu[y - 1] - 6 * (map[(y + 1) * nx + x] + map[(y - 1) * nx + x]
+                                         - 2 * map[y * nx + x]);
+                uu[y] = temp;
+            }
+            dmap[(ny - 1) * nx + x] = 0.0f;
+            for (int y = ny - 2; y >= 1; y--)
+                dmap[y * nx + x] = (float)((dmap[y * nx + x] * dmap[(y + 1) * nx + x] + uu[y]) / 6.0);
+        } else
+            dmap[x] = 0.0f;
+    }
+}
+
+/* ------------------------------------------------------------ subtraction ---- */
+/* per row: node[x] (y interpolation) and dnode[x] (x second derivatives) */
+__global__ void __launch_bounds__(256) k_rownodes(const float *back, const float *dback, int nbx,
+                                                  int nby, int bs, int H, float *nodes,
+                                                  float *dnodes, float *utmp)
+{
+    const int y = blockIdx.x * 256 + threadIdx.x;
+    if (y >= H) return;
+    float *node = nodes + (size_t)y * nbx, *dnode = dnodes + (size_t)y * nbx;
+    float *u = utmp + (size_t)y * nbx;
+    if (nby > 1) {
+        int yl;
+        float dy = (float)((float)y / bs - 0.5);
+        dy -= (yl = (int)dy);
+        if (yl < 0) {
+            yl = 0;
+            dy = (float)(dy - 1.0);
+        } else if (yl >= nby - 1) {
+            yl = nby < 2 ? 0 : nby - 2;
+            dy = (float)(dy + 1.0);
+        }
+        const float cdy = 1 - dy;
+        const float dy3 = (dy * dy * dy - dy);
+        const float cdy3 = (cdy * cdy * cdy - cdy);
+        const float *blo = back + nbx * yl, *bhi = blo + nbx;
+        const float *dblo = dback + nbx * yl, *dbhi = dblo + nbx;
+        for (int x = 0; x < nbx; x++)
+            node[x] = cdy * blo[x] + dy * bhi[x] + cdy3 * dblo[x] + dy3 * dbhi[x];
+        if (nbx > 1) {
+            float temp;
+            dnode[0] = u[0] = 0.0f;
+            for (int x = 1; x < nbx - 1; x++) {
+                temp = -1 / (dnode[x - 1] + 4);
+                dnode[x] = temp;
+                temp *= u[x - 1] - 6 * (node[x + 1] + node[x - 1] - 2 * node[x]);
+                u[x] = temp;
+            }
+            dnode[nbx - 1] = 0.0f;
+            for (int x = nbx - 2; x >= 1; x--)
+                dnode[x] = (float)((dnode[x] * dnode[x + 1] + u[x]) / 6.0);
+        }
+    } else {
+        for (int x = 0; x < nbx; x++) {
+            node[x] = back[x];
+            dnode[x] = dback[x];
+        }
+    }
+}
+
+/* colseg[x]: index of the left node, coldx[x]: dx of the cumulative recurrence */
+__global__ void __launch_bounds__(256) k_subback(float *img, int W, int H, int nbx,
+                                                 const float *nodes, const float *dnodes,
+                                                 const int *colseg, const float *coldx)
+{
+    const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= (long)W * H) return;
+    const int y = (int)(k / W), x = (int)(k - (long)y * W);
+    const float *node = nodes + (size_t)y * nbx, *dnode = dnodes + (size_t)y * nbx;
+    float bval;
+    if (nbx > 1) {
+        const int s = colseg[x];
+        const float dx = coldx[x];
+        const float cdx = 1 - dx;
+        bval = cdx * (node[s] + (cdx * cdx - 1) * dnode[s]) + dx * (node[s + 1] + (dx * dx - 1) * dnode[s + 1]);
+    } else
+        bval = node[0];
+    img[k] -= bval;
+}
+
+__global__ void __launch_bounds__(256) k_subconst(float *img, long n, float v)
+{
+    const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) img[k] -= v;
+}
+
+/* The column tables replay the x loop of subbackline (back.c:880-908) on the host: the
+   float additions dx += xstep and the resets to dx0 are the same IEEE operations. */
+static void column_tables(int W, int nbx, int bs, std::vector<int> &seg, std::vector<float> &dxs)
+{
+    seg.assign(W, 0);
+    dxs.assign(W, 0.0f);
+    if (nbx <= 1) return;
+    const int nx = bs, nbxm1 = nbx - 1;
+    volatile float xstep = (float)(1.0 / nx);   /* volatile: each float operation as written */
+    const int changepoint = nx / 2;
+    volatile float dx = (xstep - 1) / 2;
+    const float dx0 = ((nx + 1) % 2) * xstep / 2;
+    int s = 0;
+    for (int x = 0, i = 0, j = 0; j < W; j++, i++, dx = dx + xstep) {
+        if (i == changepoint && x > 0 && x < nbxm1) {
+            s++;
+            dx = dx0;
+        }
+        seg[j] = s;
+        dxs[j] = dx;
+        if (i == nx) {
+            x++;
+            i = 0;
+        }
+    }
+}
+
+extern "C" void back_estimate(int back_size, float backfthresh, float *backmean, float *backsig,
+                              int nbackfx, int nbackfy, int back_type)
+{
+    SxgState &S = sxg_state();
+    if (!S.have_frame) {
+        sxg_set_error("back_estimate: no frame (call init_device first)");
+        return;
+    }
+    const int W = S.W, H = S.H;
+    if (back_size < 1) {
+        sxg_set_error("back_estimate: BACK_SIZE must be >= 1");
+        return;
+    }
+    const int bs = back_size;
+    const int nbx = (W - 1) / bs + 1, nby = (H - 1) / bs + 1, nmesh = nbx * nby;
+    /* field.c:139-140: a single mesh along an axis disables filtering along it */
+    const int nfx = nbx > 1 ? nbackfx : 1, nfy = nby > 1 ? nbackfy : 1;
+    if (nfx < 1 || nfy < 1 || nfx > SXG_MAXBACKF || nfy > SXG_MAXBACKF) {
+        sxg_set_error("back_estimate: BACK_FILTERSIZE out of range (1..11)");
+        return;
+    }
+    S.bs = bs;
+    S.nbx = nbx;
+    S.nby = nby;
+    int n2 = 1;
+    while (n2 < nmesh) n2 <<= 1;
+    /* mesh scratch: MeshStat[nmesh] + 4*n2 floats of work + 2 outputs */
+    const size_t ms_bytes = ((sizeof(MeshStat) * nmesh + 255) / 256) * 256;
+    if (!S.mesh.ensure(ms_bytes + sizeof(float) * (4 * (size_t)n2 + 64)) ||
+        !S.maps.ensure(sizeof(float) * 4 * (size_t)nmesh) ||
+        !S.rowspl.ensure(sizeof(float) * 3 * (size_t)H * nbx + sizeof(float) * 2 * W + 256))
+        return;
+    MeshStat *ms = S.mesh.as<MeshStat>();
+    float *work = reinterpret_cast<float *>(reinterpret_cast<char *>(S.mesh.ptr) + ms_bytes);
+    float *outv = work + 4 * (size_t)n2;
+    float *back = S.maps.as<float>(), *sigma = back + nmesh, *dback = sigma + nmesh,
+          *dsigma = dback + nmesh;
+    float *img = S.img.as<float>();
+
+    sxg_stage_begin(ST_BACK);
+    hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nmesh, 64)), dim3(64), 0, S.stream, img, W, H,
+                       bs, nbx, nmesh, ms);
+    hipLaunchKernelGGL(k_meshmode, dim3(nmesh), dim3(MODE_THREADS), 0, S.stream, img, W, H, bs,
+                       nbx, ms, back, sigma);
+    hipLaunchKernelGGL(k_filterback, dim3(1), dim3(FB_THREADS), 0, S.stream, back, sigma, dback,
+                       dsigma, nbx, nby, nfx, nfy, (double)backfthresh, work, n2, outv);
+    float hv[2];
+    SXG_HIP(hipMemcpyAsync(hv, outv, sizeof(hv), hipMemcpyDeviceToHost, S.stream));
+    SXG_HIP(hipStreamSynchronize(S.stream));
+    S.backmean = hv[0];
+    S.backsig = hv[1];
+    if (back_type == 1) {
+        /* BACK_TYPE MANUAL: subtract BACK_VALUE (back.c:148-149, :807-813) */
+        S.backmean = S.back_value;
+        hipLaunchKernelGGL(k_subconst, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0, S.stream,
+                           img, (long)S.N, S.back_value);
+    } else {
+        float *nodes = S.rowspl.as<float>(), *dnodes = nodes + (size_t)H * nbx,
+              *utmp = dnodes + (size_t)H * nbx;
+        int *colseg = reinterpret_cast<int *>(utmp + (size_t)H * nbx);
+        float *coldx = reinterpret_cast<float *>(colseg + W);
+        std::vector<int> seg;
+        std::vector<float> dxs;
+        column_tables(W, nbx, bs, seg, dxs);
+        SXG_HIP(hipMemcpyAsync(colseg, seg.data(), sizeof(int) * W, hipMemcpyHostToDevice, S.stream));
+        SXG_HIP(hipMemcpyAsync(coldx, dxs.data(), sizeof(float) * W, hipMemcpyHostToDevice, S.stream));
+        hipLaunchKernelGGL(k_rownodes, dim3(sxg_div_up(H, 256)), dim3(256), 0, S.stream, back,
+                           dback, nbx, nby, bs, H, nodes, dnodes, utmp);
+        hipLaunchKernelGGL(k_subback, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0, S.stream,
+                           img, W, H, nbx, nodes, dnodes, colseg, coldx);
+        SXG_HIP(hipStreamSynchronize(S.stream));   /* host tables must outlive the copies */
+    }
+    sxg_stage_end(ST_BACK);
+    SXG_HIP(hipGetLastError());
+    if (backmean) *backmean = S.backmean;
+    if (backsig) *backsig = S.backsig;
+}
+
+extern "C" int sxg_get_back_maps(float *back, float *sigma, float *dback, float *dsigma)
+{
+    SxgState &S = sxg_state();
+    const int nm = S.nbx * S.nby;
+    if (nm <= 0) return -1;
+    float *d = S.maps.as<float>();
+    float *dst[4] = {back, sigma, dback, dsigma};
+    for (int i = 0; i < 4; i++)
+        if (dst[i])
+            SXG_HIP_RET(hipMemcpyAsync(dst[i], d + (size_t)i * nm, sizeof(float) * nm,
+                                       hipMemcpyDeviceToHost, S.stream), -1);
+    SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
+    return 0;
+}

@@ -1,0 +1,122 @@
+/*
+ * sxg_internal.h -- shared runtime state, error recording and device helpers of the
+ * MI355X detection path.  Every kernel is compiled for gfx950 with FP contraction off
+ * (see the Makefile) so that FP32/FP64 expressions round exactly as written.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "sexgpu.h"
+#include "sx_cmath.h"
+
+#define SXG_BIG 1e+30
+#define SXG_PI 3.1415926535898          /* src/define.h:56 */
+#define SXG_OBJ_MERGED 0x0002
+#define SXG_OBJ_SATUR 0x0004
+#define SXG_OBJ_TRUNC 0x0008
+#define SXG_NISO 8
+#define SXG_MAXPICSIZE 1048576
+#define SXG_NMAXLEVELS 4096              /* QUANTIF_NMAXLEVELS, src/back.h:23 */
+#define SXG_MAXNTHRESH 64                /* DEBLEND_NTHRESH upper bound, preflist.h:79 */
+#define SXG_MAXMASK 31                   /* largest convolution mask width handled */
+#define SXG_MAXBACKF 11                  /* BACK_FILTERSIZE upper bound, preflist.h:48 */
+
+/* ---------------------------------------------------------------- errors ---- */
+void sxg_set_error(const std::string &msg);
+void sxg_clear_error();
+bool sxg_has_error();
+
+#define SXG_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            sxg_set_error(std::string(#call) + ": " + hipGetErrorString(_e) + " (" +   \
+                          __FILE__ + ":" + std::to_string(__LINE__) + ")");            \
+            return;                                                                    \
+        }                                                                              \
+    } while (0)
+#define SXG_HIP_RET(call, ret)                                                         \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            sxg_set_error(std::string(#call) + ": " + hipGetErrorString(_e) + " (" +   \
+                          __FILE__ + ":" + std::to_string(__LINE__) + ")");            \
+            return ret;                                                                \
+        }                                                                              \
+    } while (0)
+
+/* ------------------------------------------------------------ workspace ---- */
+/* A grow-only device buffer: reallocated only when a larger size is requested. */
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t want);   /* false (error recorded) on allocation failure */
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+enum StageId { ST_BACK = 0, ST_FILTER, ST_DETECT, ST_DEBLEND, ST_ANALYSE, ST_CLEAN, ST_N };
+
+struct SxgState {
+    int W = 0, H = 0;
+    size_t N = 0;
+    bool have_frame = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2 * ST_N] = {};
+    bool ev_used[ST_N] = {};
+    float back_value = 0.0f;
+
+    /* frame buffers */
+    DevBuf img;      /* float[N]: input, background-subtracted in place */
+    DevBuf cd;       /* float[N]: filtered (detection) frame */
+    DevBuf lab;      /* int[N]: labels, then the per-pixel object index */
+    DevBuf aux;      /* int[N]: union-find parents / counters */
+
+    /* background */
+    int bs = 0, nbx = 0, nby = 0;
+    float backmean = 0.0f, backsig = 0.0f;
+    DevBuf mesh;     /* per-mesh scratch */
+    DevBuf maps;     /* back, sigma, dback, dsigma */
+    DevBuf rowspl;   /* per-row node / dnode tables */
+    DevBuf small;    /* small host-visible results */
+
+    /* detection */
+    float dthresh = 0.0f, thresh = 0.0f;
+    int ext_minarea = 5;
+    int64_t npix_above = 0, npix_raw = 0;
+    int nraw = 0;
+    DevBuf fg;       /* int[P_all]: foreground pixel indices, raster order */
+    DevBuf rawobj;   /* per raw object: first pixel, count, offset, fdflux, fdpeak, bbox */
+    DevBuf fpia;     /* int[P0]: pixels grouped by raw object, raster order inside */
+    DevBuf scan_tmp; /* scan scratch */
+
+    /* deblending and objects */
+    int nthresh = 0;
+    int nobj = 0;
+    DevBuf thr;      /* float[nraw * nthresh] sub-thresholds */
+    DevBuf lvl;      /* int[nthresh * P0] component of each raw pixel per level */
+    DevBuf comps;    /* component table */
+    DevBuf objs;     /* object columns (SoA) */
+    DevBuf rootobj;  /* per root: first object, object count */
+    int nsurv = 0;
+    bool cleaned = false;
+    float stage_ms[ST_N] = {};
+};
+
+SxgState &sxg_state();
+void sxg_stage_begin(StageId s);
+void sxg_stage_end(StageId s);
+
+/* ---------------------------------------------------------- primitives ---- */
+/* Exclusive prefix sum of n int32 values (out may alias in); *total (device) receives
+   the sum when total != nullptr. */
+void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s);
+/* Stream compaction of the indices i in [0,n) with flag[i] != 0, in increasing order;
+   *count (device) receives the number kept. */
+void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s);
+
+static inline int sxg_div_up(long a, long b) { return (int)((a + b - 1) / b); }

@@ -1,0 +1,318 @@
+/*
+ * sxg_state.hip -- library state, errors, workspace, stage timing, scan/compaction
+ * primitives, and the frame entry points init_device / clear_device (the reference's
+ * src/cuda/cudainit.cu:16 and :160).
+ */
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "sxg_internal.h"
+
+/* ---------------------------------------------------------------- errors ---- */
+static std::string g_err;
+static bool g_has_err = false;
+
+void sxg_set_error(const std::string &msg)
+{
+    if (!g_has_err) g_err = msg;   /* keep the first failure of a call sequence */
+    g_has_err = true;
+}
+void sxg_clear_error() { g_has_err = false; g_err.clear(); }
+bool sxg_has_error() { return g_has_err; }
+
+extern "C" const char *sxg_last_error(void) { return g_has_err ? g_err.c_str() : nullptr; }
+
+/* ------------------------------------------------------------ workspace ---- */
+bool DevBuf::ensure(size_t want)
+{
+    if (want <= bytes && ptr) return true;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    size_t alloc = want < 256 ? 256 : want;
+    alloc = (alloc + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&ptr, alloc);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        sxg_set_error(std::string("device allocation of ") + std::to_string(alloc) +
+                      " bytes failed: " + hipGetErrorString(e));
+        return false;
+    }
+    bytes = alloc;
+    return true;
+}
+
+void DevBuf::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+SxgState &sxg_state()
+{
+    static SxgState st;
+    return st;
+}
+
+static bool ensure_stream()
+{
+    SxgState &S = sxg_state();
+    if (S.stream) return true;
+    /* default priority stream; events for per-stage timing */
+    if (hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking) != hipSuccess) {
+        sxg_set_error("hipStreamCreate failed");
+        return false;
+    }
+    for (int i = 0; i < 2 * ST_N; i++)
+        if (hipEventCreate(&S.ev[i]) != hipSuccess) {
+            sxg_set_error("hipEventCreate failed");
+            return false;
+        }
+    return true;
+}
+
+void sxg_stage_begin(StageId s)
+{
+    SxgState &S = sxg_state();
+    (void)hipEventRecord(S.ev[2 * s], S.stream);
+}
+
+void sxg_stage_end(StageId s)
+{
+    SxgState &S = sxg_state();
+    (void)hipEventRecord(S.ev[2 * s + 1], S.stream);
+    S.ev_used[s] = true;
+}
+
+extern "C" int sxg_stage_times(float *ms, int n)
+{
+    SxgState &S = sxg_state();
+    int k = 0;
+    for (; k < n && k < ST_N; k++) {
+        ms[k] = 0.0f;
+        if (S.ev_used[k]) {
+            (void)hipEventSynchronize(S.ev[2 * k + 1]);
+            (void)hipEventElapsedTime(&ms[k], S.ev[2 * k], S.ev[2 * k + 1]);
+        }
+    }
+    return k;
+}
+
+extern "C" void sxg_release(void)
+{
+    SxgState &S = sxg_state();
+    if (S.stream) (void)hipStreamSynchronize(S.stream);
+    DevBuf *bufs[] = {&S.img, &S.cd, &S.lab, &S.aux, &S.mesh, &S.maps, &S.rowspl, &S.small,
+                      &S.fg, &S.rawobj, &S.fpia, &S.scan_tmp, &S.thr, &S.lvl, &S.comps,
+                      &S.objs, &S.rootobj};
+    for (DevBuf *b : bufs) b->release();
+    S.have_frame = false;
+}
+
+/* ------------------------------------------------------------------ scan ---- */
+#define SCAN_BLOCK 256
+#define SCAN_ITEMS 4
+#define SCAN_TILE (SCAN_BLOCK * SCAN_ITEMS)
+
+/* workgroup-wide exclusive scan of one value per thread; returns the total */
+__device__ inline int block_excl_scan(int v, int *sm, int *total)
+{
+    const int t = threadIdx.x;
+    sm[t] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+        int add = t >= off ? sm[t - off] : 0;
+        __syncthreads();
+        sm[t] += add;
+        __syncthreads();
+    }
+    int incl = sm[t];
+    *total = sm[SCAN_BLOCK - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const int *in, int n, int *bsum)
+{
+    __shared__ int sm[SCAN_BLOCK];
+    const long base = (long)blockIdx.x * SCAN_TILE;
+    int s = 0;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        long k = base + (long)i * SCAN_BLOCK + threadIdx.x;
+        if (k < n) s += in[k];
+    }
+    int tot;
+    block_excl_scan(s, sm, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+/* single workgroup: exclusive scan of nb block sums in place, total to *total */
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_blocks(int *bsum, int nb, int *total)
+{
+    __shared__ int sm[SCAN_BLOCK];
+    int carry = 0;
+    for (int base = 0; base < nb; base += SCAN_BLOCK) {
+        int k = base + threadIdx.x;
+        int v = k < nb ? bsum[k] : 0;
+        int tot;
+        int ex = block_excl_scan(v, sm, &tot);
+        if (k < nb) bsum[k] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const int *in, int *out, int n,
+                                                           const int *bsum)
+{
+    __shared__ int sm[SCAN_BLOCK];
+    const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS], s = 0;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        long k = base + i;
+        v[i] = k < n ? in[k] : 0;
+        s += v[i];
+    }
+    int tot;
+    int ex = block_excl_scan(s, sm, &tot) + bsum[blockIdx.x];
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        long k = base + i;
+        if (k < n) out[k] = ex;
+        ex += v[i];
+    }
+}
+
+void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s)
+{
+    SxgState &S = sxg_state();
+    if (n <= 0) {
+        if (total) (void)hipMemsetAsync(total, 0, sizeof(int), s);
+        return;
+    }
+    const int nb = sxg_div_up(n, SCAN_TILE);
+    if (!S.scan_tmp.ensure(sizeof(int) * (nb + 64))) return;
+    int *bsum = S.scan_tmp.as<int>();
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, n, bsum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_BLOCK), 0, s, bsum, nb, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, out, n, bsum);
+}
+
+/* ------------------------------------------------------------ compaction ---- */
+/* Contiguous per-thread chunks keep the output in increasing index order. */
+__global__ void __launch_bounds__(SCAN_BLOCK) k_compact_count(const int *flag, int n, int *bsum)
+{
+    __shared__ int sm[SCAN_BLOCK];
+    const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        long k = base + i;
+        if (k < n && flag[k]) s++;
+    }
+    int tot;
+    block_excl_scan(s, sm, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_compact_write(const int *flag, int n,
+                                                              const int *bsum, int *out)
+{
+    __shared__ int sm[SCAN_BLOCK];
+    const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+    int f[SCAN_ITEMS], s = 0;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        long k = base + i;
+        f[i] = (k < n && flag[k]) ? 1 : 0;
+        s += f[i];
+    }
+    int tot;
+    int pos = block_excl_scan(s, sm, &tot) + bsum[blockIdx.x];
+    for (int i = 0; i < SCAN_ITEMS; i++)
+        if (f[i]) out[pos++] = (int)(base + i);
+}
+
+void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s)
+{
+    SxgState &S = sxg_state();
+    if (n <= 0) {
+        (void)hipMemsetAsync(count, 0, sizeof(int), s);
+        return;
+    }
+    const int nb = sxg_div_up(n, SCAN_TILE);
+    if (!S.scan_tmp.ensure(sizeof(int) * (nb + 64))) return;
+    int *bsum = S.scan_tmp.as<int>();
+    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_BLOCK), 0, s, bsum, nb, count);
+    hipLaunchKernelGGL(k_compact_write, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum, out);
+}
+
+/* -------------------------------------------------------- frame entry points ---- */
+extern "C" void init_device(int width, int height, float *imgbuf)
+{
+    sxg_clear_error();
+    SxgState &S = sxg_state();
+    if (width <= 0 || height <= 0 || (long)width * height >= 2147483647L || !imgbuf) {
+        sxg_set_error("init_device: invalid frame size or null image");
+        return;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        sxg_set_error("init_device: no HIP device visible");
+        return;
+    }
+    if (!ensure_stream()) return;
+    S.W = width;
+    S.H = height;
+    S.N = (size_t)width * height;
+    S.nraw = S.nobj = S.nsurv = 0;
+    S.cleaned = false;
+    for (int i = 0; i < ST_N; i++) S.ev_used[i] = false;
+    if (!S.img.ensure(S.N * sizeof(float)) || !S.cd.ensure(S.N * sizeof(float)) ||
+        !S.lab.ensure(S.N * sizeof(int)) || !S.aux.ensure(S.N * sizeof(int)))
+        return;
+    SXG_HIP(hipMemcpyAsync(S.img.ptr, imgbuf, S.N * sizeof(float), hipMemcpyHostToDevice,
+                           S.stream));
+    SXG_HIP(hipStreamSynchronize(S.stream));
+    S.have_frame = true;
+}
+
+extern "C" void clear_device(float *img)
+{
+    SxgState &S = sxg_state();
+    if (!S.have_frame) {
+        sxg_set_error("clear_device: no frame");
+        return;
+    }
+    if (img) {
+        SXG_HIP(hipMemcpyAsync(img, S.img.ptr, S.N * sizeof(float), hipMemcpyDeviceToHost,
+                               S.stream));
+    }
+    SXG_HIP(hipStreamSynchronize(S.stream));
+    S.have_frame = false;
+}
+
+extern "C" void init_detection(void)
+{
+    SxgState &S = sxg_state();
+    if (!S.have_frame || S.W == 0 || S.H == 0) sxg_set_error("init_detection: no frame");
+}
+
+extern "C" void sxg_set_back_value(float value) { sxg_state().back_value = value; }
+
+extern "C" void *sxg_device_frame(int which)
+{
+    SxgState &S = sxg_state();
+    if (!S.have_frame) return nullptr;
+    return which == 0 ? S.img.ptr : S.cd.ptr;
+}
+
+extern "C" int sxg_get_filtered(float *out)
+{
+    SxgState &S = sxg_state();
+    if (!S.have_frame) return -1;
+    SXG_HIP_RET(hipMemcpyAsync(out, S.cd.ptr, S.N * sizeof(float), hipMemcpyDeviceToHost,
+                               S.stream), -1);
+    SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
+    return 0;
+}

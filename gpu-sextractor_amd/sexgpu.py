@@ -1,0 +1,369 @@
+"""Python binding of the MI355X detection path.
+
+The product is the C-ABI library lib/libsexgpu.so (HIP kernels for gfx950 behind the
+entry points the reference ``sex`` program calls: init_device, back_estimate, cudaFilter,
+run_detection, init_deblend, parcel_out, clear_deblend, analyse_gpu, run_clean,
+endobject_gpu, clear_detection, clear_device).  This module loads it with ctypes and
+offers a numpy-level view for tests and benchmarks.  It also exposes the host helper
+library lib/libsxhost.so (deterministic synthetic frames, FITS I/O).
+
+Nothing here falls back to a CPU implementation: if the HIP library is missing or no GPU
+is visible, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GPU_LIB = os.path.join(HERE, "lib", "libsexgpu.so")
+HOST_LIB = os.path.join(HERE, "lib", "libsxhost.so")
+
+_host = None
+_gpu = None
+
+
+# ------------------------------------------------------------------ host side ----
+class SxsParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("nsrc", C.c_int),
+        ("star_frac", C.c_double), ("fmin", C.c_double), ("fmax", C.c_double),
+        ("bkg", C.c_double), ("grad", C.c_double), ("sigma", C.c_double),
+        ("moffat_beta", C.c_double), ("fwhm", C.c_double),
+        ("re_min", C.c_double), ("re_max", C.c_double), ("q_min", C.c_double),
+        ("group_frac", C.c_double), ("eps", C.c_double), ("maxrad", C.c_int),
+        ("seed", C.c_uint64), ("nblob", C.c_int),
+        ("blob_x", C.c_double * 8), ("blob_y", C.c_double * 8),
+        ("blob_sigma", C.c_double), ("blob_amp", C.c_double),
+        ("nsat", C.c_int), ("sat_beta", C.c_double), ("sat_peak", C.c_double),
+        ("clip_level", C.c_double),
+    ]
+
+
+def host():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError("host library not built: run `make -C gpu-sextractor_amd host`")
+        _host = C.CDLL(HOST_LIB)
+        _host.sxs_defaults.argtypes = [C.POINTER(SxsParams)]
+        _host.sxs_render.argtypes = [C.POINTER(SxsParams), C.c_void_p, C.c_void_p, C.c_int]
+        _host.sxs_weight.argtypes = [C.POINTER(SxsParams), C.c_double, C.c_double, C.c_void_p]
+        _host.sxs_hash.restype = C.c_uint64
+        _host.sxs_hash.argtypes = [C.c_void_p, C.c_size_t]
+        _host.sxf_read_image.restype = C.POINTER(C.c_float)
+        _host.sxf_read_image.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_char_p)]
+        _host.sxf_write_image.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
+    return _host
+
+
+def synth_params(preset="plain1024", **kw):
+    """Parameters of the synthetic workloads.
+
+    plain1024   1024x1024, 3000 sources (70% Moffat stars), dN/dF~F^-2 on [1e2,1e5]
+    crowded4096 4096x4096, ~167k sources (60/40), F up to 1e6, 8% in close pairs/triples
+    wide8192    8192x8192, ~400k sources, large-scale background structure (3 Gaussians,
+                500 px, 3 sigma); weight map from synth_weight()
+    hdr4096     4096x4096 high-dynamic-range: 200 saturated Moffat stars (beta 2.5,
+                peak 1e6) + 100k faint sources (1-5 sigma) + groups of 5-20 components
+    """
+    p = SxsParams()
+    host().sxs_defaults(C.byref(p))
+    if preset == "plain1024":
+        pass
+    elif preset == "crowded4096":
+        p.width = p.height = 4096
+        p.nsrc = 167000
+        p.star_frac = 0.6
+        p.fmax = 1e6
+        p.group_frac = 0.08
+        p.seed = 1001
+    elif preset == "wide8192":
+        p.width = p.height = 8192
+        p.nsrc = 400000
+        p.star_frac = 0.6
+        p.fmax = 1e6
+        p.group_frac = 0.08
+        p.seed = 1002
+        p.nblob = 3
+        for i, (x, y) in enumerate([(0.25, 0.3), (0.7, 0.55), (0.4, 0.8)]):
+            p.blob_x[i], p.blob_y[i] = x, y
+        p.blob_sigma = 500.0
+        p.blob_amp = 30.0
+    elif preset == "hdr4096":
+        p.width = p.height = 4096
+        p.nsrc = 100000
+        p.star_frac = 0.5
+        p.fmin, p.fmax = 30.0, 400.0
+        p.group_frac = 0.5
+        p.seed = 1004
+        p.nsat = 200
+        p.sat_beta = 2.5
+        p.sat_peak = 1e6
+    else:
+        raise ValueError("unknown preset " + preset)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def synth(preset="plain1024", **kw):
+    """Render a synthetic float32 frame (deterministic for a given preset and seed)."""
+    p = synth_params(preset, **kw)
+    img = np.empty((p.height, p.width), dtype=np.float32)
+    rc = host().sxs_render(C.byref(p), img.ctypes.data, None, 0)
+    if rc:
+        raise RuntimeError("sxs_render failed (%d)" % rc)
+    return img
+
+
+def synth_weight(preset="wide8192", vign=0.2, badcol=0.01, **kw):
+    p = synth_params(preset, **kw)
+    w = np.empty((p.height, p.width), dtype=np.float32)
+    host().sxs_weight(C.byref(p), vign, badcol, w.ctypes.data)
+    return w
+
+
+def frame_hash(img):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    return "%016x" % host().sxs_hash(img.ctypes.data, img.size)
+
+
+def read_fits(path):
+    W, H = C.c_int(), C.c_int()
+    gain, satur = C.c_double(0.0), C.c_double(0.0)
+    err = C.c_char_p()
+    ptr = host().sxf_read_image(path.encode(), C.byref(W), C.byref(H), C.byref(gain),
+                                C.byref(satur), C.byref(err))
+    if not ptr:
+        raise RuntimeError("cannot read %s: %s" % (path, err.value))
+    arr = np.ctypeslib.as_array(ptr, shape=(H.value * W.value,)).copy().reshape(H.value, W.value)
+    C.CDLL(None).free(C.cast(ptr, C.c_void_p))
+    return arr
+
+
+def write_fits(path, arr):
+    arr = np.ascontiguousarray(arr)
+    bitpix = -32 if arr.dtype == np.float32 else 32
+    if arr.dtype not in (np.float32, np.int32):
+        raise TypeError("write_fits takes float32 or int32 arrays")
+    rc = host().sxf_write_image(path.encode(), arr.ctypes.data, arr.shape[1], arr.shape[0], bitpix)
+    if rc:
+        raise RuntimeError("cannot write %s" % path)
+
+
+# ------------------------------------------------------------------- GPU side ----
+class _Lazy:
+    """Wraps a CDLL so that declaring the signature of a symbol the library does not
+    export is deferred until the symbol is used (AttributeError at the call)."""
+
+    def __init__(self, lib):
+        object.__setattr__(self, "_lib", lib)
+        object.__setattr__(self, "_sig", {})
+
+    def __getattr__(self, name):
+        fn = getattr(self._lib, name, None)
+        if fn is None:
+            sig = self._sig.setdefault(name, _Sig())
+            return sig
+        sig = self._sig.get(name)
+        if sig is not None:
+            if sig.argtypes is not None:
+                fn.argtypes = sig.argtypes
+            if sig.restype is not None:
+                fn.restype = sig.restype
+        return fn
+
+    def has(self, name):
+        return hasattr(self._lib, name)
+
+
+class _Sig:
+    def __init__(self):
+        object.__setattr__(self, "argtypes", None)
+        object.__setattr__(self, "restype", None)
+
+    def __call__(self, *a):
+        raise AttributeError("libsexgpu.so does not export this entry point")
+
+
+class FrameInfo(C.Structure):
+    _fields_ = [("backmean", C.c_float), ("backsig", C.c_float), ("dthresh", C.c_float),
+                ("thresh", C.c_float), ("npix_above", C.c_int64), ("npix_raw", C.c_int64),
+                ("nraw", C.c_int32), ("nobj", C.c_int32), ("nsurv", C.c_int32),
+                ("nbx", C.c_int32), ("nby", C.c_int32)]
+
+
+def gpu():
+    """Load lib/libsexgpu.so (raises if it is missing: there is no CPU fallback)."""
+    global _gpu
+    if _gpu is None:
+        if not os.path.exists(GPU_LIB):
+            raise RuntimeError("HIP library not built: run `make -C gpu-sextractor_amd gpu`")
+        L = _Lazy(C.CDLL(GPU_LIB))
+        vp, i, f, d, sz = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t
+        L.init_device.argtypes = [i, i, vp]
+        L.back_estimate.argtypes = [i, f, C.POINTER(f), C.POINTER(f), i, i, i]
+        L.init_detection.argtypes = []
+        L.cudaFilter.argtypes = [vp, C.c_uint]
+        L.run_detection.argtypes = [f, f, i, C.POINTER(sz), C.POINTER(sz)]
+        L.init_deblend.argtypes = [f, i, sz, sz]
+        L.parcel_out.argtypes = [f, i, i, sz, sz, d]
+        L.parcel_out.restype = i
+        L.clear_deblend.argtypes = [i]
+        L.analyse_gpu.argtypes = [d, i, i, f, f, d, d, f, f, d, i, i, i, i, i, i, i, i, i, i,
+                                  i, i]
+        L.run_clean.argtypes = [vp, d, i, i]
+        L.endobject_gpu.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, f, f, d, d, d, d, i]
+        L.clear_detection.argtypes = []
+        L.clear_device.argtypes = [vp]
+        L.sxg_last_error.restype = C.c_char_p
+        L.sxg_set_back_value.argtypes = [f]
+        L.sxg_get_segmap.argtypes = [vp]
+        L.sxg_get_column.argtypes = [i, vp, sz]
+        L.sxg_get_frame_info.argtypes = [C.POINTER(FrameInfo)]
+        L.sxg_get_back_maps.argtypes = [vp, vp, vp, vp]
+        L.sxg_get_filtered.argtypes = [vp]
+        L.sxg_get_alloc_map.argtypes = [vp]
+        L.sxg_device_frame.argtypes = [i]
+        L.sxg_device_frame.restype = vp
+        L.sxg_stage_times.argtypes = [vp, i]
+        L.sxg_release.argtypes = []
+        _gpu = L
+    return _gpu
+
+
+def check():
+    err = gpu().sxg_last_error()
+    if err:
+        raise RuntimeError("libsexgpu: " + err.decode())
+
+
+EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run_detection",
+            "init_deblend", "parcel_out", "clear_deblend", "analyse_gpu", "run_clean",
+            "endobject_gpu", "clear_detection", "clear_device", "sxg_last_error",
+            "sxg_set_back_value", "sxg_get_segmap", "sxg_get_column", "sxg_get_frame_info",
+            "sxg_get_back_maps", "sxg_get_filtered", "sxg_get_alloc_map", "sxg_device_frame",
+            "sxg_stage_times", "sxg_release"]
+
+COLUMNS = ["level", "root", "firstpix", "flag", "fdnpix", "dnpix", "npix", "xmin", "xmax",
+           "ymin", "ymax", "peakx", "peaky", "iso", "mx", "my", "mx2", "my2", "mxy",
+           "poserr_mx2", "poserr_my2", "poserr_mxy", "a", "b", "theta", "cxx", "cyy", "cxy",
+           "abcor", "fdflux", "dflux", "flux", "fluxerr", "fdpeak", "dpeak", "peak", "bkg",
+           "thresh_obj", "mthresh", "fwhm", "master"]
+
+
+def _coltype(name):
+    if name in ("mx", "my", "mx2", "my2", "mxy", "poserr_mx2", "poserr_my2", "poserr_mxy"):
+        return np.float64
+    if name == "master":
+        return np.uint32
+    if name in ("level", "root", "firstpix", "flag", "fdnpix", "dnpix", "npix", "xmin", "xmax",
+                "ymin", "ymax", "peakx", "peaky", "iso"):
+        return np.int32
+    return np.float32
+
+
+def frame_info():
+    fi = FrameInfo()
+    gpu().sxg_get_frame_info(C.byref(fi))
+    check()
+    return {k: getattr(fi, k) for k, _ in FrameInfo._fields_}
+
+
+def get_column(name, nobj):
+    t = _coltype(name)
+    n = nobj * (8 if name == "iso" else 1)
+    out = np.zeros(max(n, 1), dtype=t)
+    rc = gpu().sxg_get_column(COLUMNS.index(name), out.ctypes.data, out.nbytes)
+    if rc:
+        check()
+        raise RuntimeError("sxg_get_column(%s) failed" % name)
+    out = out[:n]
+    return out.reshape(nobj, 8) if name == "iso" else out
+
+
+class Params:
+    """Configuration of the path (the .sex keys the reference front-end passes on)."""
+
+    def __init__(self, **kw):
+        self.back_size = 64
+        self.back_fsize = 3
+        self.back_fthresh = 0.0
+        self.back_type = 0
+        self.back_value = 0.0
+        self.thresh_type = 0
+        self.detect_thresh = 1.5
+        self.analysis_thresh = 1.5
+        self.ext_minarea = 5
+        self.deblend_nthresh = 32
+        self.deblend_mincont = 0.005
+        self.clean = 1
+        self.clean_param = 1.0
+        self.clean_stacksize = 3000
+        self.satur_level = 50000.0
+        self.gain = 0.0
+        self.mask = None
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise KeyError(k)
+            setattr(self, k, v)
+
+
+def run_background(img, p):
+    """init_device + back_estimate; returns (backmean, backsig)."""
+    L = gpu()
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    H, W = img.shape
+    L.init_device(W, H, img.ctypes.data)
+    check()
+    bw = min(p.back_size, W)
+    bh = min(p.back_size, H)
+    if bw != bh:
+        raise ValueError("only square background meshes are supported (back.c:99-103)")
+    nbx, nby = (W - 1) // bw + 1, (H - 1) // bh + 1
+    L.sxg_set_back_value(p.back_value)
+    bm, bs = C.c_float(), C.c_float()
+    L.back_estimate(bw, p.back_fthresh, C.byref(bm), C.byref(bs),
+                    p.back_fsize if nbx > 1 else 1, p.back_fsize if nby > 1 else 1, p.back_type)
+    check()
+    return bm.value, bs.value
+
+
+def thresholds(p, backsig):
+    """back.c:155-192: detection and analysis thresholds (double product -> float)."""
+    f32 = np.float32
+    if p.thresh_type == 1:
+        return f32(p.detect_thresh), f32(p.analysis_thresh)
+    return f32(p.detect_thresh * float(np.float64(backsig))), f32(p.analysis_thresh * float(np.float64(backsig)))
+
+
+def back_maps(nbx, nby):
+    maps = [np.zeros((nby, nbx), np.float32) for _ in range(4)]
+    gpu().sxg_get_back_maps(*[m.ctypes.data for m in maps])
+    check()
+    return maps
+
+
+def filtered(shape):
+    out = np.zeros(shape, np.float32)
+    gpu().sxg_get_filtered(out.ctypes.data)
+    check()
+    return out
+
+
+def subtracted(shape):
+    out = np.zeros(shape, np.float32)
+    gpu().clear_device(out.ctypes.data)
+    check()
+    return out
+
+
+def apply_filter(mask):
+    mask = np.ascontiguousarray(mask, dtype=np.float32)
+    if mask.ndim != 2 or mask.shape[0] != mask.shape[1]:
+        raise ValueError("the C-ABI takes square masks (cudafilter.cu:150)")
+    gpu().cudaFilter(mask.ctypes.data, mask.shape[0])
+    check()

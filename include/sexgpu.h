@@ -1,0 +1,175 @@
+/*
+ * sexgpu.h -- C-ABI of the MI355X SExtractor detection path (libsexgpu.so).
+ *
+ * The first group of entry points has exactly the names, argument lists and calling
+ * order of the reference's CUDA library (the src/cuda sources, called from src/makeit.c:189,
+ * src/back.c:104 and src/scan.c:78-146), so that the reference front-end can link this
+ * library instead.  Semantics follow the reference:
+ *   - a stateful singleton: width, height and every device buffer live in the library;
+ *     one frame at a time, calls in the order listed; not re-entrant, not thread-safe;
+ *   - every call returns synchronously;
+ *   - the library owns device memory; init_device copies the caller's frame in and
+ *     clear_device overwrites the caller's buffer with the background-subtracted frame;
+ *   - run_clean fills the caller's masterIndex[numobj] (0xffffffff = kept).
+ * The reference exits on any error; this library records the error instead: the void
+ * entry points keep their signatures, sxg_last_error() returns the message (NULL when
+ * the last call succeeded), and the int-returning ones return a negative value.
+ */
+#ifndef SEXGPU_H
+#define SEXGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference entry points ---------------------------------------------------- */
+
+/* src/cuda/cudainit.cu:16 -- copy the width x height float frame to the device. */
+void init_device(int width, int height, float *img);
+
+/* src/cuda/cudaback.cu:1212 -- background mesh statistics, mode, median filter,
+   splines, global medians (written to *backmean, *backsig) and in-place subtraction.
+   back_type 1 (MANUAL) subtracts the value set by sxg_set_back_value(). */
+void back_estimate(int back_size, float backfthresh, float *backmean, float *backsig,
+                   int nbackfx, int nbackfy, int back_type);
+
+/* src/cuda/cudadetection.cu:492 */
+void init_detection(void);
+
+/* src/cuda/cudafilter.cu:150 -- convolve the subtracted frame with a square
+   mask_width x mask_width mask (any odd or even width up to 31, row-major). */
+void cudaFilter(float *mask, unsigned int mask_width);
+
+/* src/cuda/cudadetection.cu:1107 -- threshold (cd >= dthresh), 8-connected labelling,
+   DETECT_MINAREA pruning; returns the number of raw objects and of their pixels. */
+void run_detection(float dthresh, float thresh, int ext_minarea, size_t *numValidObj,
+                   size_t *numValidPix);
+
+/* src/cuda/cudadeblend.cu:508 -- per-object sub-thresholds. */
+void init_deblend(float basethresh, int nthresh, size_t numValidObj, size_t numValidPix);
+
+/* src/cuda/cudadeblend.cu:579 -- multi-threshold deblending, tree pruning,
+   preanalysis of the kept objects and gather-up; returns the number of objects. */
+int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_t numValidObj,
+               size_t numValidPix, double deblend_mincont);
+
+/* src/cuda/cudadeblend.cu:1937 */
+void clear_deblend(int nthresh);
+
+/* src/cuda/cudaanalyse.cu:3171 -- robust preanalysis and isophotal analysis. The
+   flagobj_* arguments carry a boolean in their first byte (src/define.h:283). */
+void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int flagobj_iso_0,
+                 float flagobj_fwhm, float backsig, double gain, double ngamma, float thresh,
+                 float dthresh, double satur_level, int plistexist_var, int plistoff_var,
+                 int prefs_weightgain_flag, int prefs_ext_minarea, int prefs_clean_flag,
+                 int detect_type, int nbackx, int nbacky, int backw, int backh, int width,
+                 int numobj);
+
+/* src/cuda/cudaclean.cu:260 -- masterIndex[i] = index of the object that makes object
+   i spurious, or 0xffffffff.  (The reference header cudaclean.h:4 declares an int
+   return; the definition is void, as here.) */
+void run_clean(int *masterIndex, double clean_param, int clean_stacksize, int numobj);
+
+/* src/cuda/cudaanalyse.cu:3344 -- fill the catalogue rows (outobjstruct layout of
+   src/types.h:417-455, mirrored by sxg_outobj below).  The prefs/brain/obj2/wcs
+   pointers are accepted for signature compatibility and not dereferenced; photometry
+   beyond the isophotal columns is left zero. */
+void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *objlist, void *wcs,
+                   int fymin, int fymax, int width, int height, float backsig, float thresh,
+                   double satur_level, double pixscale, double ngamma, double gain,
+                   int numobj);
+
+/* src/cuda/cudadetection.cu:1092 */
+void clear_detection(void);
+
+/* src/cuda/cudainit.cu:160 -- copy the background-subtracted frame into img. */
+void clear_device(float *img);
+
+/* ---- additions (no counterpart in the reference) -------------------------------- */
+
+/* Message of the last failed call, or NULL. */
+const char *sxg_last_error(void);
+
+/* BACK_VALUE for back_type 1 (the reference reads prefs.back_val on the host). */
+void sxg_set_back_value(float value);
+
+/* Segmentation check-image: out[width*height] = catalogue NUMBER (1-based, survivors of
+   CLEAN in object order) on each object's final pixels, 0 elsewhere.  Call after
+   run_clean (before clear_detection). Returns the number of survivors or < 0. */
+int sxg_get_segmap(int32_t *out);
+
+/* Per-object columns (object order, numobj entries).  Returns 0 or < 0 on error. */
+enum sxg_column {
+    SXG_COL_LEVEL = 0,     /* int32  deblending level of the object */
+    SXG_COL_ROOT,          /* int32  raw (level-0) object it comes from */
+    SXG_COL_FIRSTPIX,      /* int32  raster index of its first level pixel */
+    SXG_COL_FLAG,          /* int32  MERGED 2 | SATUR 4 | TRUNC 8 */
+    SXG_COL_FDNPIX,        /* int32  final pixel count */
+    SXG_COL_DNPIX,         /* int32  pixels above the detection threshold */
+    SXG_COL_NPIX,          /* int32  ISOAREA_IMAGE (pixels above the analysis thresh) */
+    SXG_COL_XMIN, SXG_COL_XMAX, SXG_COL_YMIN, SXG_COL_YMAX,   /* int32, 0-based */
+    SXG_COL_PEAKX, SXG_COL_PEAKY,                            /* int32, 1-based */
+    SXG_COL_ISO,           /* int32[8] isophotal areas */
+    SXG_COL_MX, SXG_COL_MY, SXG_COL_MX2, SXG_COL_MY2, SXG_COL_MXY,   /* double */
+    SXG_COL_POSERR_MX2, SXG_COL_POSERR_MY2, SXG_COL_POSERR_MXY,      /* double */
+    SXG_COL_A, SXG_COL_B, SXG_COL_THETA, SXG_COL_CXX, SXG_COL_CYY, SXG_COL_CXY,  /* float */
+    SXG_COL_ABCOR, SXG_COL_FDFLUX, SXG_COL_DFLUX, SXG_COL_FLUX, SXG_COL_FLUXERR,
+    SXG_COL_FDPEAK, SXG_COL_DPEAK, SXG_COL_PEAK, SXG_COL_BKG, SXG_COL_THRESH,
+    SXG_COL_MTHRESH, SXG_COL_FWHM,                                   /* float */
+    SXG_COL_MASTER,        /* uint32 CLEAN master (0xffffffff = kept) */
+    SXG_NCOL
+};
+int sxg_get_column(int column, void *out, size_t bytes);
+
+/* Frame-level values: backmean, backsig, dthresh, thresh, and counts
+   (pixels above threshold, raw objects, raw-object pixels, objects, survivors). */
+typedef struct {
+    float backmean, backsig, dthresh, thresh;
+    int64_t npix_above, npix_raw;
+    int32_t nraw, nobj, nsurv, nbx, nby;
+} sxg_frame_info;
+int sxg_get_frame_info(sxg_frame_info *info);
+
+/* Background maps after filtering (nbx*nby floats each): back, sigma, dback, dsigma. */
+int sxg_get_back_maps(float *back, float *sigma, float *dback, float *dsigma);
+
+/* Filtered (detection) frame, width*height floats. */
+int sxg_get_filtered(float *out);
+
+/* Per-pixel object index (deblended objects, before CLEAN) or -1: width*height int32. */
+int sxg_get_alloc_map(int32_t *out);
+
+/* Device pointers of the resident frame (for callers that keep data on the GPU):
+   background-subtracted frame and filtered frame.  Valid until clear_device. */
+void *sxg_device_frame(int which);
+
+/* Timing of the last frame's stages in milliseconds (HIP events): background, filter,
+   detection, deblending, analysis, clean.  Returns the number of stages written. */
+int sxg_stage_times(float *ms, int n);
+
+/* Release every device buffer the library holds (the workspace is grow-only and kept
+   across frames otherwise). */
+void sxg_release(void);
+
+/* Catalogue row, layout of the reference's outobjstruct (src/types.h:417-455, NAPER 1). */
+typedef struct __attribute__((aligned(16))) {
+    int number;
+    double alpha2000, delta2000;
+    float sposx, sposy;
+    float flux_iso, fluxerr_iso, mag_iso, magerr_iso;
+    float flux_isocor, fluxerr_isocor;
+    float flux_aper[1], fluxerr_aper[1], mag_aper[1], magerr_aper[1];
+    float flux_auto, fluxerr_auto, mag_auto, magerr_auto;
+    float theta;
+    double poserr_mx2, poserr_my2, poserr_mxy;
+    short flag;
+    float elong, ellip, sprob, peak, kronfactor, bkg, fwhm;
+} sxg_outobj;
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,116 @@
+/*
+ * sxo.h -- CPU oracle of the SExtractor detection path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This library is the parity checker for the HIP implementation.  Only the tests, the
+ * smoke check and the benchmark's CPU-baseline leg may load it; the product never
+ * links or calls it.  It restates, in plain sequential C:
+ *   - the CPU background code of the reference (src/back.c, src/misc.c:30 hmedian),
+ *   - the CPU convolution (src/filter.c:39-186),
+ *   - the algorithms of the reference's CUDA detection path for the parts whose CPU
+ *     code was removed (src/cuda/cudadetection.cu, cudadeblend.cu, cudaanalyse.cu,
+ *     cudaclean.cu), each function citing the file:line it follows.
+ * Summation orders that the reference leaves to a library (CUDPP segmented scans) are
+ * fixed here as "sequential over the object's pixels in raster order", the order the
+ * reference's own per-object loops use (cudaanalyse.cu:1770-1835).
+ */
+#ifndef SXO_H
+#define SXO_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SXO_BIG 1e+30
+
+typedef struct {
+    int back_size;           /* BACK_SIZE (square meshes) */
+    int back_fsize;          /* BACK_FILTERSIZE */
+    double back_fthresh;     /* BACK_FILTTHRESH */
+    int back_type;           /* 0 AUTO, 1 MANUAL */
+    double back_value;       /* BACK_VALUE (MANUAL) */
+    int thresh_type;         /* 0 RELATIVE, 1 ABSOLUTE */
+    double detect_thresh;    /* DETECT_THRESH */
+    double analysis_thresh;  /* ANALYSIS_THRESH */
+    int ext_minarea;         /* DETECT_MINAREA */
+    int deblend_nthresh;     /* DEBLEND_NTHRESH */
+    double deblend_mincont;  /* DEBLEND_MINCONT */
+    int clean;               /* CLEAN */
+    double clean_param;      /* CLEAN_PARAM */
+    int clean_stacksize;     /* MEMORY_OBJSTACK */
+    double satur_level;      /* SATUR_LEVEL */
+    double gain;             /* GAIN */
+    const float *mask;       /* filter mask (row-major), NULL = no filtering */
+    int mask_w, mask_h;
+    const float *weight;     /* MAP_WEIGHT image or NULL */
+    double weight_thresh;    /* unused unless weight != NULL */
+} sxo_config;
+
+typedef struct {
+    int width, height, nbx, nby;
+    float backmean, backsig, dthresh, thresh;
+    float *back, *sigma, *dback, *dsigma;   /* filtered mesh maps and y derivatives */
+    float *sub;                             /* background-subtracted frame */
+    float *cd;                              /* filtered frame */
+    float *var;                             /* per-pixel variance (weight maps) or NULL */
+    long npix_above;                        /* P_all */
+    long npix_raw;                          /* P0: pixels of raw objects */
+    int nraw;                               /* R */
+    int *raw_first, *raw_count;             /* raw objects: first raster pixel, size */
+    float *raw_fdflux, *raw_fdpeak;
+    int nobj;                               /* M: deblended objects */
+    int *level, *root, *firstpix, *flag, *fdnpix, *dnpix, *npix;
+    int *xmin, *xmax, *ymin, *ymax, *peakx, *peaky, *iso;   /* iso: 8 per object */
+    double *mx, *my, *mx2, *my2, *mxy, *poserr_mx2, *poserr_my2, *poserr_mxy;
+    float *a, *b, *theta, *cxx, *cyy, *cxy, *abcor, *fdflux, *dflux, *flux, *fluxerr;
+    float *fdpeak, *dpeak, *peak, *bkg, *thresh_obj, *mthresh, *fwhm, *dthresh_obj;
+    uint32_t *master;                       /* 0xffffffff: kept by CLEAN */
+    int *alloc;                             /* per pixel: object index or -1 */
+    int *seg;                               /* per pixel: catalogue NUMBER or 0 */
+    int nsurv;
+    int *objpix_start, *objpix;             /* final pixel lists (raster order) */
+} sxo_result;
+
+/* ---------------- background (src/back.c) ---------------- */
+typedef struct {
+    float mean, sigma, lcut, hcut, qzero, qscale;
+    int nlevels, npix;
+} sxo_mesh;
+
+void sxo_backstat_mesh(const float *img, int W, int x0, int y0, int bw, int bh, sxo_mesh *m);
+void sxo_backhisto_mesh(const float *img, int W, int x0, int y0, int bw, int bh,
+                        const sxo_mesh *m, int *histo);
+float sxo_backguess(const sxo_mesh *m, const int *histo, float *mean, float *sigma);
+float sxo_hmedian(float *ra, int n);
+int sxo_makeback(const float *img, int W, int H, int bs, int nfx, int nfy, double fthresh,
+                 float *back, float *sigma, float *backmean, float *backsig);
+void sxo_filterback(float *back, float *sigma, int nbx, int nby, int nfx, int nfy,
+                    double fthresh, float *backmean, float *backsig);
+void sxo_makebackspline(const float *map, int nbx, int nby, float *dmap);
+void sxo_subbackline(const float *back, const float *dback, int nbx, int nby, int bs,
+                     int W, int y, float *line, float *backline);
+int sxo_background(const float *img, int W, int H, int bs, int nfx, int nfy, double fthresh,
+                   int back_type, double back_value, float *sub, float *back, float *sigma,
+                   float *dback, float *dsigma, float *backmean, float *backsig);
+
+/* ---------------- filtering (src/filter.c) ---------------- */
+int sxo_getconv(const char *filename, float *mask, int maxn, int *mw, int *mh);
+void sxo_convolve(const float *in, int W, int H, const float *mask, int mw, int mh, float *out);
+
+/* ---------------- whole path ---------------- */
+void sxo_config_defaults(sxo_config *c);
+sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg);
+void sxo_free(sxo_result *r);
+/* all-double re-measurement of the catalogue columns over the same final pixel lists */
+int sxo_measure_fp64(const sxo_result *r, double *cols, int ncol);
+
+/* canonical math (shared definition with the HIP kernels) */
+double sxo_log(double x);
+double sxo_exp(double x);
+double sxo_pow(double x, double y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,215 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and by the
+CPU-baseline leg of ``bench.py``, never by the product.  The oracle restates the
+reference's CPU background/filter code and its CUDA detection algorithms (see the
+headers of oracle/sxo_*.c for the file:line each function follows).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.sxo_run.restype = C.POINTER(SxoResult)
+        _lib.sxo_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(SxoConfig)]
+        _lib.sxo_free.argtypes = [C.POINTER(SxoResult)]
+        _lib.sxo_log.restype = C.c_double
+        _lib.sxo_log.argtypes = [C.c_double]
+        _lib.sxo_exp.restype = C.c_double
+        _lib.sxo_exp.argtypes = [C.c_double]
+        _lib.sxo_pow.restype = C.c_double
+        _lib.sxo_pow.argtypes = [C.c_double, C.c_double]
+        _lib.sxo_hmedian.restype = C.c_float
+        _lib.sxo_hmedian.argtypes = [C.c_void_p, C.c_int]
+        _lib.sxo_backguess.restype = C.c_float
+        _lib.sxo_getconv.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        _lib.sxo_convolve.argtypes = [vp, i, i, vp, i, i, vp]
+        _lib.sxo_background.argtypes = [vp, i, i, i, i, i, d, i, d, vp, vp, vp, vp, vp, vp, vp]
+        _lib.sxo_makeback.argtypes = [vp, i, i, i, i, i, d, vp, vp, vp, vp]
+        _lib.sxo_filterback.argtypes = [vp, vp, i, i, i, i, d, vp, vp]
+        _lib.sxo_makebackspline.argtypes = [vp, i, i, vp]
+        _lib.sxo_subbackline.argtypes = [vp, vp, i, i, i, i, i, vp, vp]
+        _lib.sxo_backstat_mesh.argtypes = [vp, i, i, i, i, i, vp]
+        _lib.sxo_backhisto_mesh.argtypes = [vp, i, i, i, i, i, vp, vp]
+        _lib.sxo_backguess.argtypes = [vp, vp, vp, vp]
+    return _lib
+
+
+def ref():
+    """The reference's own back.c/filter.c compiled from the reference tree, or None."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        _ref = C.CDLL(REF_PATH)
+        _ref.ref_hmedian.restype = C.c_float
+        _ref.ref_hmedian.argtypes = [C.c_void_p, C.c_int]
+        _ref.ref_filterback.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+        _ref.ref_getconv.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
+        vp, i = C.c_void_p, C.c_int
+        _ref.ref_makeback.argtypes = [vp, i, i, i, vp, vp, vp]
+        _ref.ref_makebackspline.argtypes = [vp, i, i, vp]
+        _ref.ref_subback.argtypes = [vp, vp, i, i, i, i, i, vp]
+        _ref.ref_convolve.argtypes = [vp, i, i, vp, i, i, vp]
+    return _ref
+
+
+class SxoConfig(C.Structure):
+    _fields_ = [
+        ("back_size", C.c_int), ("back_fsize", C.c_int), ("back_fthresh", C.c_double),
+        ("back_type", C.c_int), ("back_value", C.c_double), ("thresh_type", C.c_int),
+        ("detect_thresh", C.c_double), ("analysis_thresh", C.c_double),
+        ("ext_minarea", C.c_int), ("deblend_nthresh", C.c_int),
+        ("deblend_mincont", C.c_double), ("clean", C.c_int), ("clean_param", C.c_double),
+        ("clean_stacksize", C.c_int), ("satur_level", C.c_double), ("gain", C.c_double),
+        ("mask", C.c_void_p), ("mask_w", C.c_int), ("mask_h", C.c_int),
+        ("weight", C.c_void_p), ("weight_thresh", C.c_double),
+    ]
+
+
+_IP = C.POINTER(C.c_int)
+_FP = C.POINTER(C.c_float)
+_DP = C.POINTER(C.c_double)
+
+
+class SxoResult(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("nbx", C.c_int), ("nby", C.c_int),
+        ("backmean", C.c_float), ("backsig", C.c_float), ("dthresh", C.c_float),
+        ("thresh", C.c_float),
+        ("back", _FP), ("sigma", _FP), ("dback", _FP), ("dsigma", _FP), ("sub", _FP),
+        ("cd", _FP), ("var", _FP),
+        ("npix_above", C.c_long), ("npix_raw", C.c_long), ("nraw", C.c_int),
+        ("raw_first", _IP), ("raw_count", _IP), ("raw_fdflux", _FP), ("raw_fdpeak", _FP),
+        ("nobj", C.c_int),
+        ("level", _IP), ("root", _IP), ("firstpix", _IP), ("flag", _IP), ("fdnpix", _IP),
+        ("dnpix", _IP), ("npix", _IP), ("xmin", _IP), ("xmax", _IP), ("ymin", _IP),
+        ("ymax", _IP), ("peakx", _IP), ("peaky", _IP), ("iso", _IP),
+        ("mx", _DP), ("my", _DP), ("mx2", _DP), ("my2", _DP), ("mxy", _DP),
+        ("poserr_mx2", _DP), ("poserr_my2", _DP), ("poserr_mxy", _DP),
+        ("a", _FP), ("b", _FP), ("theta", _FP), ("cxx", _FP), ("cyy", _FP), ("cxy", _FP),
+        ("abcor", _FP), ("fdflux", _FP), ("dflux", _FP), ("flux", _FP), ("fluxerr", _FP),
+        ("fdpeak", _FP), ("dpeak", _FP), ("peak", _FP), ("bkg", _FP), ("thresh_obj", _FP),
+        ("mthresh", _FP), ("fwhm", _FP), ("dthresh_obj", _FP),
+        ("master", C.POINTER(C.c_uint32)), ("alloc", _IP), ("seg", _IP), ("nsurv", C.c_int),
+        ("objpix_start", _IP), ("objpix", _IP),
+    ]
+
+
+OBJ_INT = ["level", "root", "firstpix", "flag", "fdnpix", "dnpix", "npix", "xmin", "xmax",
+           "ymin", "ymax", "peakx", "peaky"]
+OBJ_DBL = ["mx", "my", "mx2", "my2", "mxy", "poserr_mx2", "poserr_my2", "poserr_mxy"]
+OBJ_FLT = ["a", "b", "theta", "cxx", "cyy", "cxy", "abcor", "fdflux", "dflux", "flux",
+           "fluxerr", "fdpeak", "dpeak", "peak", "bkg", "thresh_obj", "mthresh", "fwhm"]
+
+
+def _arr(ptr, n, dtype):
+    if n <= 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def config(**kw):
+    c = SxoConfig()
+    c.back_size, c.back_fsize, c.back_fthresh = 64, 3, 0.0
+    c.detect_thresh = c.analysis_thresh = 1.5
+    c.ext_minarea, c.deblend_nthresh, c.deblend_mincont = 5, 32, 0.005
+    c.clean, c.clean_param, c.clean_stacksize = 1, 1.0, 3000
+    c.satur_level, c.gain = 50000.0, 0.0
+    for k, v in kw.items():
+        if k != "mask":
+            setattr(c, k, v)
+    return c
+
+
+def run(img, mask=None, **kw):
+    """Run the whole oracle path on a float32 frame; returns a dict of numpy arrays."""
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    H, W = img.shape
+    cfg = config(**kw)
+    keep = None
+    if mask is not None:
+        keep = np.ascontiguousarray(mask, dtype=np.float32)
+        cfg.mask = keep.ctypes.data
+        cfg.mask_h, cfg.mask_w = keep.shape
+    rp = lib().sxo_run(img.ctypes.data, W, H, C.byref(cfg))
+    if not rp:
+        raise RuntimeError("oracle rejected the configuration")
+    r = rp.contents
+    N = W * H
+    out = {k: getattr(r, k) for k in ("backmean", "backsig", "dthresh", "thresh", "npix_above",
+                                      "npix_raw", "nraw", "nobj", "nsurv", "nbx", "nby")}
+    nb = r.nbx * r.nby
+    for k in ("back", "sigma", "dback", "dsigma"):
+        out[k] = _arr(getattr(r, k), nb, np.float32).reshape(r.nby, r.nbx)
+    out["sub"] = _arr(r.sub, N, np.float32).reshape(H, W)
+    out["cd"] = _arr(r.cd, N, np.float32).reshape(H, W)
+    out["alloc"] = _arr(r.alloc, N, np.int32).reshape(H, W)
+    out["seg"] = _arr(r.seg, N, np.int32).reshape(H, W)
+    n = r.nraw
+    out["raw_first"] = _arr(r.raw_first, n, np.int32)
+    out["raw_count"] = _arr(r.raw_count, n, np.int32)
+    out["raw_fdflux"] = _arr(r.raw_fdflux, n, np.float32)
+    out["raw_fdpeak"] = _arr(r.raw_fdpeak, n, np.float32)
+    m = r.nobj
+    for k in OBJ_INT:
+        out[k] = _arr(getattr(r, k), m, np.int32)
+    for k in OBJ_DBL:
+        out[k] = _arr(getattr(r, k), m, np.float64)
+    for k in OBJ_FLT:
+        out[k] = _arr(getattr(r, k), m, np.float32)
+    out["iso"] = _arr(r.iso, 8 * m, np.int32).reshape(m, 8)
+    out["master"] = _arr(r.master, m, np.uint32)
+    lib().sxo_free(rp)
+    return out
+
+
+def getconv(path):
+    buf = np.zeros(1024, dtype=np.float32)
+    mw, mh = C.c_int(), C.c_int()
+    rc = lib().sxo_getconv(path.encode(), buf.ctypes.data, 1024, C.byref(mw), C.byref(mh))
+    if rc:
+        raise RuntimeError("cannot read convolution mask %s (%d)" % (path, rc))
+    return buf[: mw.value * mh.value].reshape(mh.value, mw.value).copy()
+
+
+def convolve(img, mask):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    mask = np.ascontiguousarray(mask, dtype=np.float32)
+    out = np.empty_like(img)
+    H, W = img.shape
+    lib().sxo_convolve(img.ctypes.data, W, H, mask.ctypes.data, mask.shape[1], mask.shape[0],
+                       out.ctypes.data)
+    return out
+
+
+def background(img, bs=64, fsize=3, fthresh=0.0, back_type=0, back_value=0.0):
+    """Background maps and subtracted frame (back.c restatement)."""
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    H, W = img.shape
+    nbx, nby = (W - 1) // bs + 1, (H - 1) // bs + 1
+    sub = np.empty_like(img)
+    maps = [np.empty((nby, nbx), np.float32) for _ in range(4)]
+    bm, bsig = C.c_float(), C.c_float()
+    lib().sxo_background(img.ctypes.data, W, H, bs, fsize, fsize, C.c_double(fthresh),
+                         back_type, C.c_double(back_value), sub.ctypes.data,
+                         maps[0].ctypes.data, maps[1].ctypes.data, maps[2].ctypes.data,
+                         maps[3].ctypes.data, C.byref(bm), C.byref(bsig))
+    return dict(sub=sub, back=maps[0], sigma=maps[1], dback=maps[2], dsigma=maps[3],
+                backmean=bm.value, backsig=bsig.value)

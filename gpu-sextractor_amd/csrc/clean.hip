@@ -1,0 +1,315 @@
+/*
+ * clean.hip -- CLEAN (run_clean, the reference's src/cuda/cudaclean.cu:260), catalogue
+ * rows (endobject_gpu, cudaanalyse.cu:3344, isophotal columns) and the outputs the
+ * reference never wrote: the SEGMENTATION map and per-object columns.
+ *
+ * CLEAN semantics (cudaclean.cu:10-29, :120-174): amp = fdflux/(2 pi a b abcor),
+ * alpha = (pow(amp/dthresh, 1/CLEAN_PARAM) - 1) pi a b / fdnpix; for every pair (i, j)
+ * with 0 < j - i <= 16*ceil(MEMORY_OBJSTACK/16) whose centres are closer than
+ * CLEAN_ZONE (a_i + a_j), the fainter object is a victim of the brighter when the
+ * brighter's profile amp/val, val = 1 + alpha (cxx dx^2 + cyy dy^2 + cxy dx dy) > 1,
+ * exceeds the fainter's mthresh; its master is the smallest such brighter index.
+ * Victims are dropped (the reference disabled merging, :311-368).
+ *
+ * Layout: one lane per object i scans its window of following objects; masters are
+ * combined with atomicMin, so the result does not depend on scheduling.
+ */
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sxg_internal.h"
+#include "sxg_objects.h"
+
+#define CLEAN_ZONE 10.0
+
+__global__ void __launch_bounds__(256) k_clean_prep(ObjCols o, int nobj, double clean_param,
+                                                    double *amp, double *alpha, unsigned *master)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nobj) return;
+    const float unitarea = (float)(SXG_PI * o.a[i] * o.b[i]);
+    const double ampin = amp[i] = o.fdflux[i] / (2 * unitarea * o.abcor[i]);
+    alpha[i] = (sx_pow(ampin / o.dthresh[i], 1.0 / clean_param) - 1) * unitarea / o.fdnpix[i];
+    master[i] = 0xffffffffu;
+}
+
+__global__ void __launch_bounds__(256) k_clean_pairs(ObjCols o, int nobj, int win,
+                                                     const double *__restrict__ amp,
+                                                     const double *__restrict__ alpha,
+                                                     unsigned *master)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nobj) return;
+    const double mxi = o.mx[i], myi = o.my[i];
+    const float ai = o.a[i], fi = o.fdflux[i], mti = o.mthresh[i];
+    const float cxxi = o.cxx[i], cyyi = o.cyy[i], cxyi = o.cxy[i];
+    const double ampi = amp[i], alphai = alpha[i];
+    int jmax = i + win;
+    if (jmax > nobj - 1) jmax = nobj - 1;
+    for (int j = i + 1; j <= jmax; j++) {
+        const float dx = (float)(mxi - o.mx[j]);
+        const float dy = (float)(myi - o.my[j]);
+        float rlim = ai + o.a[j];
+        rlim *= rlim;
+        if (dx * dx + dy * dy < rlim * CLEAN_ZONE * CLEAN_ZONE) {
+            double val;
+            if (fi < o.fdflux[j]) {
+                val = 1 + alpha[j] * (o.cxx[j] * dx * dx + o.cyy[j] * dy * dy + o.cxy[j] * dx * dy);
+                if (val > 1.0 && ((float)(val < 1e10 ? amp[j] * (1.0 / val) : 0.0) > mti))
+                    atomicMin(&master[i], (unsigned)j);
+            } else {
+                val = 1 + alphai * (cxxi * dx * dx + cyyi * dy * dy + cxyi * dx * dy);
+                if (val > 1.0 && ((float)(val < 1e10 ? ampi * (1.0 / val) : 0.0) > o.mthresh[j]))
+                    atomicMin(&master[j], (unsigned)i);
+            }
+        }
+    }
+}
+
+extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacksize, int numobj)
+{
+    SxgState &S = sxg_state();
+    if (!S.have_frame || numobj != S.nobj) {
+        sxg_set_error("run_clean: numobj differs from parcel_out");
+        return;
+    }
+    if (clean_stacksize < 1 || !(clean_param > 0.0)) {
+        sxg_set_error("run_clean: MEMORY_OBJSTACK and CLEAN_PARAM must be positive");
+        return;
+    }
+    ObjCols o;
+    if (!sxg_obj_view(o)) return;
+    const int n = numobj;
+    double *amp = sxg_take<double>("clean.amp", n), *alpha = sxg_take<double>("clean.alpha", n);
+    if (!amp || !alpha) return;
+    sxg_stage_begin(ST_CLEAN);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_clean_prep, dim3(sxg_div_up(n, 256)), dim3(256), 0, S.stream, o, n,
+                           clean_param, amp, alpha, o.master);
+        /* CLEAN N: keep everything (the reference cleans regardless of the flag) */
+        if (S.clean_enabled) {
+            const int win = ((clean_stacksize - 1) / 16 + 1) * 16;
+            hipLaunchKernelGGL(k_clean_pairs, dim3(sxg_div_up(n, 256)), dim3(256), 0, S.stream, o,
+                               n, win, amp, alpha, o.master);
+        }
+    }
+    sxg_stage_end(ST_CLEAN);
+    S.cleaned = true;
+    if (masterIndex && n > 0)
+        SXG_HIP(hipMemcpyAsync(masterIndex, o.master, sizeof(unsigned) * n, hipMemcpyDeviceToHost,
+                               S.stream));
+    SXG_HIP(hipGetLastError());
+    SXG_HIP(hipStreamSynchronize(S.stream));
+}
+
+/* ------------------------------------------------------------- segmentation ---- */
+__global__ void __launch_bounds__(256) k_keepflag(const unsigned *master, int n, int *keep)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) keep[i] = master[i] == 0xffffffffu ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256) k_fill_i32(int *a, long n, int v)
+{
+    const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+
+/* seg[p] = 1-based rank of the surviving object of p (0 for cleaned objects) */
+__global__ void __launch_bounds__(256) k_seg_scatter(const int *fpia, long P0, const int *alloc,
+                                                     const int *keep, const int *rank, int *seg,
+                                                     int mode)
+{
+    const long g = (long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= P0) return;
+    const int p = fpia[g];
+    const int o = alloc[p];
+    seg[p] = mode ? o : (keep[o] ? rank[o] + 1 : 0);
+}
+
+/* 0: segmentation map, 1: object index map; into the device buffer "out.map" */
+static int *build_map(int mode)
+{
+    SxgState &S = sxg_state();
+    int *map = sxg_take<int>("out.map", S.N);
+    if (!map) return nullptr;
+    const long N = (long)S.N;
+    hipLaunchKernelGGL(k_fill_i32, dim3(sxg_div_up(N, 256)), dim3(256), 0, S.stream, map, N,
+                       mode ? -1 : 0);
+    if (S.nobj == 0 || S.npix_raw == 0) return map;
+    ObjCols o;
+    if (!sxg_obj_view(o)) return nullptr;
+    int *keep = sxg_take<int>("out.keep", S.nobj), *rank = sxg_take<int>("out.rank", S.nobj);
+    int *tot = sxg_take<int>("out.total", 4);
+    if (!keep || !rank || !tot) return nullptr;
+    if (mode == 0) {
+        hipLaunchKernelGGL(k_keepflag, dim3(sxg_div_up(S.nobj, 256)), dim3(256), 0, S.stream,
+                           o.master, S.nobj, keep);
+        sxg_scan_excl(keep, rank, S.nobj, tot, S.stream);
+    }
+    hipLaunchKernelGGL(k_seg_scatter, dim3(sxg_div_up(S.npix_raw, 256)), dim3(256), 0, S.stream,
+                       S.fpia, (long)S.npix_raw, sxg_take<int>("deb.alloc", S.N), keep, rank, map,
+                       mode);
+    if (mode == 0) {
+        int ns = 0;
+        if (hipMemcpyAsync(&ns, tot, sizeof(int), hipMemcpyDeviceToHost, S.stream) == hipSuccess &&
+            hipStreamSynchronize(S.stream) == hipSuccess)
+            S.nsurv = ns;
+    }
+    return map;
+}
+
+extern "C" int sxg_get_segmap(int32_t *out)
+{
+    SxgState &S = sxg_state();
+    if (!S.have_frame || !S.cleaned) {
+        sxg_set_error("sxg_get_segmap: call after run_clean");
+        return -1;
+    }
+    int *map = build_map(0);
+    if (!map) return -1;
+    SXG_HIP_RET(hipMemcpyAsync(out, map, sizeof(int) * S.N, hipMemcpyDeviceToHost, S.stream), -1);
+    SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
+    return S.nsurv;
+}
+
+extern "C" int sxg_get_alloc_map(int32_t *out)
+{
+    SxgState &S = sxg_state();
+    if (!S.have_frame) return -1;
+    int *map = build_map(1);
+    if (!map) return -1;
+    SXG_HIP_RET(hipMemcpyAsync(out, map, sizeof(int) * S.N, hipMemcpyDeviceToHost, S.stream), -1);
+    SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ columns ---- */
+bool sxg_obj_alloc(int nobj, ObjCols &o)
+{
+    const size_t n = (size_t)(nobj > 0 ? nobj : 1);
+#define OI(f) o.f = sxg_take<int>("obj." #f, n)
+#define OD(f) o.f = sxg_take<double>("obj." #f, n)
+#define OF(f) o.f = sxg_take<float>("obj." #f, n)
+    OI(level); OI(root); OI(firstpix); OI(flag); OI(fdnpix); OI(dnpix); OI(npix); OI(xmin);
+    OI(xmax); OI(ymin); OI(ymax); OI(peakx); OI(peaky);
+    o.iso = sxg_take<int>("obj.iso", n * SXG_NISO);
+    OD(mx); OD(my); OD(mx2); OD(my2); OD(mxy); OD(poserr_mx2); OD(poserr_my2); OD(poserr_mxy);
+    OF(a); OF(b); OF(theta); OF(cxx); OF(cyy); OF(cxy); OF(abcor); OF(fdflux); OF(dflux);
+    OF(flux); OF(fluxerr); OF(fdpeak); OF(dpeak); OF(peak); OF(bkg); OF(thresh); OF(mthresh);
+    OF(fwhm); OF(dthresh);
+    o.master = sxg_take<unsigned>("obj.master", n);
+#undef OI
+#undef OD
+#undef OF
+    void *all[] = {o.level, o.root, o.firstpix, o.flag, o.fdnpix, o.dnpix, o.npix, o.xmin, o.xmax,
+                   o.ymin, o.ymax, o.peakx, o.peaky, o.iso, o.mx, o.my, o.mx2, o.my2, o.mxy,
+                   o.poserr_mx2, o.poserr_my2, o.poserr_mxy, o.a, o.b, o.theta, o.cxx, o.cyy,
+                   o.cxy, o.abcor, o.fdflux, o.dflux, o.flux, o.fluxerr, o.fdpeak, o.dpeak,
+                   o.peak, o.bkg, o.thresh, o.mthresh, o.fwhm, o.dthresh, o.master};
+    for (void *p : all)
+        if (!p) return false;
+    return true;
+}
+
+bool sxg_obj_view(ObjCols &o) { return sxg_obj_alloc(sxg_state().nobj, o); }
+
+extern "C" int sxg_get_column(int column, void *out, size_t bytes)
+{
+    SxgState &S = sxg_state();
+    ObjCols o;
+    if (column < 0 || column >= SXG_NCOL || !sxg_obj_view(o)) {
+        sxg_set_error("sxg_get_column: bad column");
+        return -1;
+    }
+    const void *src[SXG_NCOL] = {o.level, o.root, o.firstpix, o.flag, o.fdnpix, o.dnpix, o.npix,
+                                 o.xmin, o.xmax, o.ymin, o.ymax, o.peakx, o.peaky, o.iso, o.mx,
+                                 o.my, o.mx2, o.my2, o.mxy, o.poserr_mx2, o.poserr_my2,
+                                 o.poserr_mxy, o.a, o.b, o.theta, o.cxx, o.cyy, o.cxy, o.abcor,
+                                 o.fdflux, o.dflux, o.flux, o.fluxerr, o.fdpeak, o.dpeak, o.peak,
+                                 o.bkg, o.thresh, o.mthresh, o.fwhm, o.master};
+    size_t esize = (column >= SXG_COL_MX && column <= SXG_COL_POSERR_MXY) ? 8 : 4;
+    size_t need = esize * (size_t)S.nobj * (column == SXG_COL_ISO ? SXG_NISO : 1);
+    if (bytes < need) {
+        sxg_set_error("sxg_get_column: output buffer too small");
+        return -1;
+    }
+    if (need == 0) return 0;
+    SXG_HIP_RET(hipMemcpyAsync(out, src[column], need, hipMemcpyDeviceToHost, S.stream), -1);
+    SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
+    return 0;
+}
+
+extern "C" int sxg_get_frame_info(sxg_frame_info *info)
+{
+    SxgState &S = sxg_state();
+    info->backmean = S.backmean;
+    info->backsig = S.backsig;
+    info->dthresh = S.dthresh;
+    info->thresh = S.thresh;
+    info->npix_above = S.npix_above;
+    info->npix_raw = S.npix_raw;
+    info->nraw = S.nraw;
+    info->nobj = S.nobj;
+    info->nsurv = S.nsurv;
+    info->nbx = S.nbx;
+    info->nby = S.nby;
+    return 0;
+}
+
+/* --------------------------------------------------------------- catalogue ---- */
+extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *objlist, void *wcs,
+                              int fymin, int fymax, int width, int height, float backsig,
+                              float thresh, double satur_level, double pixscale, double ngamma,
+                              double gain, int numobj)
+{
+    (void)prefs; (void)brain; (void)flagobj2; (void)wcs; (void)fymin; (void)fymax; (void)height;
+    (void)backsig; (void)thresh; (void)satur_level; (void)pixscale; (void)ngamma; (void)gain;
+    SxgState &S = sxg_state();
+    if (!S.have_frame || numobj != S.nobj || width != S.W) {
+        sxg_set_error("endobject_gpu: numobj/width differ from parcel_out");
+        return;
+    }
+    if (!objlist || numobj == 0) return;
+    const int n = numobj;
+    std::vector<double> mx(n), my(n), pm2(n), pmy(n), pmxy(n);
+    std::vector<float> a(n), b(n), th(n), fl(n), fe(n), pk(n), bk(n), fw(n);
+    std::vector<int> flag(n);
+    if (sxg_get_column(SXG_COL_MX, mx.data(), 8 * n) || sxg_get_column(SXG_COL_MY, my.data(), 8 * n) ||
+        sxg_get_column(SXG_COL_POSERR_MX2, pm2.data(), 8 * n) ||
+        sxg_get_column(SXG_COL_POSERR_MY2, pmy.data(), 8 * n) ||
+        sxg_get_column(SXG_COL_POSERR_MXY, pmxy.data(), 8 * n) ||
+        sxg_get_column(SXG_COL_A, a.data(), 4 * n) || sxg_get_column(SXG_COL_B, b.data(), 4 * n) ||
+        sxg_get_column(SXG_COL_THETA, th.data(), 4 * n) ||
+        sxg_get_column(SXG_COL_FLUX, fl.data(), 4 * n) ||
+        sxg_get_column(SXG_COL_FLUXERR, fe.data(), 4 * n) ||
+        sxg_get_column(SXG_COL_PEAK, pk.data(), 4 * n) || sxg_get_column(SXG_COL_BKG, bk.data(), 4 * n) ||
+        sxg_get_column(SXG_COL_FWHM, fw.data(), 4 * n) ||
+        sxg_get_column(SXG_COL_FLAG, flag.data(), 4 * n))
+        return;
+    sxg_outobj *rows = static_cast<sxg_outobj *>(objlist);
+    for (int i = 0; i < n; i++) {
+        sxg_outobj &r = rows[i];
+        memset(&r, 0, sizeof(r));
+        r.number = i;
+        r.sposx = (float)(mx[i] + 1.0);   /* FITS convention, cudaanalyse.cu:2715-2716 */
+        r.sposy = (float)(my[i] + 1.0);
+        r.flux_iso = fl[i];
+        r.fluxerr_iso = (float)std::sqrt((double)fe[i]);
+        r.mag_iso = r.flux_iso > 0.0 ? (float)(-2.5 * std::log10((double)r.flux_iso) + S.mag_zeropoint) : 99.0f;
+        r.magerr_iso = r.flux_iso > 0.0 ? (float)(1.086 * r.fluxerr_iso / r.flux_iso) : 99.0f;
+        r.theta = th[i];
+        r.poserr_mx2 = pm2[i];
+        r.poserr_my2 = pmy[i];
+        r.poserr_mxy = pmxy[i];
+        r.flag = (short)flag[i];
+        r.elong = a[i] / b[i];
+        r.ellip = 1 - b[i] / a[i];
+        r.peak = pk[i];
+        r.bkg = bk[i];
+        r.fwhm = fw[i];
+    }
+}
+
+extern "C" void sxg_set_mag_zeropoint(double zp) { sxg_state().mag_zeropoint = zp; }

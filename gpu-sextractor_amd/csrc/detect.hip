@@ -1,0 +1,310 @@
+/*
+ * detect.hip -- detection (run_detection, the reference's src/cuda/cudadetection.cu:1107).
+ *
+ * Semantics (cudadetection.cu:36-64, :156-270, :677-1090): a pixel is detected when
+ * cd >= dthresh; detected pixels form 8-connected components labelled by their smallest
+ * raster index; components with at least DETECT_MINAREA pixels are the raw objects,
+ * numbered in label order; the pixels of each raw object are listed in raster order.
+ *
+ * Kernels:
+ *   k_det_init     one lane per pixel: union-find parent = own index or -1
+ *   (compaction)   detected pixel indices in raster order (the foreground list)
+ *   k_det_union    one lane per detected pixel: lock-free union with the left, up-left,
+ *                  up and up-right neighbours.  A root is only ever linked under a
+ *                  smaller index (atomicMin, retried with the value found), so every
+ *                  component ends with its smallest index as root, in one launch, with
+ *                  no host convergence loop (the reference loops with a host sync per
+ *                  iteration, cudadetection.cu:610-638)
+ *   k_det_flatten  label = root; roots remember their foreground position
+ *   k_det_rootstat per root: size (atomics), peak of cd, bounding box
+ *   (compaction)   roots with size >= DETECT_MINAREA, in raster order = raw objects
+ *   (radix sort)   pixels of raw objects grouped by object id, stable, so each object's
+ *                  pixels stay in raster order
+ */
+#include "sxg_internal.h"
+
+__global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, float t, long n,
+                                                  int *par, int *lab)
+{
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    par[p] = cd[p] >= t ? (int)p : -1;
+    lab[p] = -1;
+}
+
+__device__ inline int uf_root(const int *par, int x)
+{
+    int y = par[x];
+    while (y != x) {
+        x = y;
+        y = par[x];
+    }
+    return x;
+}
+
+/* link the two trees; every link goes from a larger root to a smaller index */
+__device__ inline void uf_unite(int *par, int a, int b)
+{
+    while (true) {
+        a = uf_root(par, a);
+        b = uf_root(par, b);
+        if (a == b) return;
+        if (a < b) {
+            int t = a;
+            a = b;
+            b = t;
+        }
+        int old = atomicMin(&par[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, int nfg, int W,
+                                                   int *par)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nfg) return;
+    const int p = fg[k];
+    const int y = p / W, x = p - y * W;
+    if (x > 0 && par[p - 1] >= 0) uf_unite(par, p, p - 1);
+    if (y > 0) {
+        if (x > 0 && par[p - W - 1] >= 0) uf_unite(par, p, p - W - 1);
+        if (par[p - W] >= 0) uf_unite(par, p, p - W);
+        if (x < W - 1 && par[p - W + 1] >= 0) uf_unite(par, p, p - W + 1);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg, int nfg, int *par,
+                                                     int *lab, int *rootpos)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nfg) return;
+    const int p = fg[k];
+    const int r = uf_root(par, p);
+    lab[p] = r;
+    if (r == p) rootpos[p] = k;
+}
+
+struct RootStat {
+    int *cnt, *peak, *xmin, *xmax, *ymin, *ymax;
+};
+
+__global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nfg) return;
+    rs.cnt[k] = 0;
+    rs.peak[k] = 0;
+    rs.xmin[k] = 0x7fffffff;
+    rs.xmax[k] = -1;
+    rs.ymin[k] = 0x7fffffff;
+    rs.ymax[k] = -1;
+}
+
+__global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg, int nfg, int W,
+                                                      const int *__restrict__ lab,
+                                                      const int *__restrict__ rootpos,
+                                                      const float *__restrict__ cd, RootStat rs)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nfg) return;
+    const int p = fg[k];
+    const int rk = rootpos[lab[p]];
+    const int y = p / W, x = p - y * W;
+    atomicAdd(&rs.cnt[rk], 1);
+    atomicMax(&rs.peak[rk], f2ord(cd[p]));   /* detected values are > 0 */
+    atomicMin(&rs.xmin[rk], x);
+    atomicMax(&rs.xmax[rk], x);
+    atomicMin(&rs.ymin[rk], y);
+    atomicMax(&rs.ymax[rk], y);
+}
+
+/* keep[k] = k for kept roots (size >= minarea), -1 otherwise */
+__global__ void __launch_bounds__(256) k_det_rootflag(const int *__restrict__ fg, int nfg,
+                                                      const int *__restrict__ lab,
+                                                      const int *__restrict__ cnt, int minarea,
+                                                      int *keep)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nfg) return;
+    const int p = fg[k];
+    keep[k] = (lab[p] == p && cnt[k] >= minarea) ? k : -1;
+}
+
+__global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ rawpos, int nraw,
+                                                     const int *__restrict__ fg, RootStat rs,
+                                                     int *rawid_of_pos, RawObj ro)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nraw) return;
+    const int k = rawpos[i];
+    rawid_of_pos[k] = i;
+    ro.first[i] = fg[k];
+    ro.count[i] = rs.cnt[k];
+    ro.fdpeak[i] = __int_as_float(rs.peak[k]);
+    ro.xmin[i] = rs.xmin[k];
+    ro.xmax[i] = rs.xmax[k];
+    ro.ymin[i] = rs.ymin[k];
+    ro.ymax[i] = rs.ymax[k];
+}
+
+__global__ void __launch_bounds__(256) k_det_fill(int *a, int n, int v)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+
+/* object id of each detected pixel (or -1 when its component is too small) */
+__global__ void __launch_bounds__(256) k_det_pixkey(const int *__restrict__ fg, int nfg,
+                                                    const int *__restrict__ lab,
+                                                    const int *__restrict__ rootpos,
+                                                    const int *__restrict__ rawid_of_pos,
+                                                    int *key)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nfg) return;
+    key[k] = rawid_of_pos[rootpos[lab[fg[k]]]];
+}
+
+__global__ void __launch_bounds__(256) k_det_gather(const int *__restrict__ idx, int n,
+                                                    const int *__restrict__ key,
+                                                    const int *__restrict__ fg, int *k0, int *v0)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int k = idx[j];
+    k0[j] = key[k];
+    v0[j] = fg[k];
+}
+
+__global__ void __launch_bounds__(256) k_det_copy(const float *a, float *b, long n)
+{
+    const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) b[k] = a[k];
+}
+
+static int read_count(const int *d, hipStream_t s)
+{
+    int h = 0;
+    if (hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        sxg_set_error("run_detection: count readback failed");
+        return -1;
+    }
+    return h;
+}
+
+extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
+                              size_t *numValidObj, size_t *numValidPix)
+{
+    SxgState &S = sxg_state();
+    if (!S.have_frame) {
+        sxg_set_error("run_detection: no frame");
+        return;
+    }
+    if (!(dthreshold > 0.0f)) {
+        /* back.c:195-197 refuses zero or negative thresholds */
+        sxg_set_error("run_detection: the detection threshold must be > 0");
+        return;
+    }
+    const int W = S.W;
+    const long N = (long)S.N;
+    hipStream_t st = S.stream;
+    S.dthresh = dthreshold;
+    S.thresh = thresh1;
+    S.ext_minarea = ext_minarea;
+    if (!S.filtered) {
+        /* FILTER N: detection on the subtracted frame itself */
+        hipLaunchKernelGGL(k_det_copy, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
+                           S.img.as<float>(), S.cd.as<float>(), N);
+    }
+    sxg_stage_begin(ST_DETECT);
+    int *par = S.aux.as<int>(), *lab = S.lab.as<int>();
+    int *rootpos = sxg_take<int>("det.rootpos", S.N);
+    int *cnt_d = sxg_take<int>("det.counts", 8);
+    if (!rootpos || !cnt_d) return;
+    hipLaunchKernelGGL(k_det_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
+                       S.cd.as<float>(), dthreshold, N, par, lab);
+    /* the foreground list has at most N entries; size it after counting */
+    int *fg = sxg_take<int>("det.fg", S.N);
+    if (!fg) return;
+    sxg_compact_nonneg(par, fg, (int)N, cnt_d, st);
+    const int nfg = read_count(cnt_d, st);
+    if (nfg < 0) return;
+    S.npix_above = nfg;
+    const int gfg = sxg_div_up(nfg > 0 ? nfg : 1, 256);
+    RootStat rs;
+    rs.cnt = sxg_take<int>("det.cnt", nfg);
+    rs.peak = sxg_take<int>("det.peak", nfg);
+    rs.xmin = sxg_take<int>("det.xmin", nfg);
+    rs.xmax = sxg_take<int>("det.xmax", nfg);
+    rs.ymin = sxg_take<int>("det.ymin", nfg);
+    rs.ymax = sxg_take<int>("det.ymax", nfg);
+    int *keep = sxg_take<int>("det.keep", nfg);
+    int *rawpos = sxg_take<int>("det.rawpos", nfg);
+    int *rawid_of_pos = sxg_take<int>("det.rawid", nfg);
+    if (!rs.cnt || !rs.peak || !rs.xmin || !rs.xmax || !rs.ymin || !rs.ymax || !keep ||
+        !rawpos || !rawid_of_pos)
+        return;
+    if (nfg > 0) {
+        hipLaunchKernelGGL(k_det_union, dim3(gfg), dim3(256), 0, st, fg, nfg, W, par);
+        hipLaunchKernelGGL(k_det_flatten, dim3(gfg), dim3(256), 0, st, fg, nfg, par, lab, rootpos);
+        hipLaunchKernelGGL(k_det_statinit, dim3(gfg), dim3(256), 0, st, rs, nfg);
+        hipLaunchKernelGGL(k_det_rootstat, dim3(gfg), dim3(256), 0, st, fg, nfg, W, lab, rootpos,
+                           S.cd.as<float>(), rs);
+        hipLaunchKernelGGL(k_det_rootflag, dim3(gfg), dim3(256), 0, st, fg, nfg, lab, rs.cnt,
+                           ext_minarea, keep);
+    }
+    sxg_compact_nonneg(keep, rawpos, nfg, cnt_d, st);
+    const int nraw = read_count(cnt_d, st);
+    if (nraw < 0) return;
+    S.nraw = nraw;
+    RawObj ro;
+    if (!sxg_raw_alloc(nraw, ro)) return;
+    if (nfg > 0) {
+        hipLaunchKernelGGL(k_det_fill, dim3(gfg), dim3(256), 0, st, rawid_of_pos, nfg, -1);
+        if (nraw > 0)
+            hipLaunchKernelGGL(k_det_rawinfo, dim3(sxg_div_up(nraw, 256)), dim3(256), 0, st, rawpos,
+                               nraw, fg, rs, rawid_of_pos, ro);
+        /* pixel keys, kept pixels, stable sort by object */
+        hipLaunchKernelGGL(k_det_pixkey, dim3(gfg), dim3(256), 0, st, fg, nfg, lab, rootpos,
+                           rawid_of_pos, keep);
+    }
+    int *idx = rawpos;   /* reuse: indices of kept pixels in the foreground list */
+    int *k0 = sxg_take<int>("det.k0", nfg), *v0 = sxg_take<int>("det.v0", nfg);
+    int *k1 = sxg_take<int>("det.k1", nfg), *v1 = sxg_take<int>("det.v1", nfg);
+    int *hist = sxg_take<int>("det.hist", 256 * (size_t)(sxg_div_up(nfg + 1, 4096) + 1) + 64);
+    if (!k0 || !v0 || !k1 || !v1 || !hist) return;
+    sxg_compact_nonneg(keep, idx, nfg, cnt_d, st);
+    const int npix = read_count(cnt_d, st);
+    if (npix < 0) return;
+    S.npix_raw = npix;
+    if (npix > 0) {
+        hipLaunchKernelGGL(k_det_gather, dim3(sxg_div_up(npix, 256)), dim3(256), 0, st, idx, npix,
+                           keep, fg, k0, v0);
+        int nbits = 0;
+        while ((1L << nbits) < (long)nraw) nbits++;
+        if (nbits == 0) nbits = 1;
+        int where = sxg_radix_sort(k0, v0, k1, v1, npix, nbits, hist, st);
+        int *ks = where ? k1 : k0, *vs = where ? v1 : v0;
+        S.fpia_keys = ks;
+        S.fpia = vs;
+        /* offsets of each object's pixel run */
+        sxg_scan_excl(ro.count, ro.off, nraw, nullptr, st);
+    } else {
+        S.fpia_keys = k0;
+        S.fpia = v0;
+    }
+    sxg_stage_end(ST_DETECT);
+    SXG_HIP(hipGetLastError());
+    SXG_HIP(hipStreamSynchronize(st));
+    if (numValidObj) *numValidObj = (size_t)nraw;
+    if (numValidPix) *numValidPix = (size_t)npix;
+}
+
+extern "C" void clear_detection(void)
+{
+    SxgState &S = sxg_state();
+    if (S.stream) (void)hipStreamSynchronize(S.stream);
+}

@@ -145,6 +145,7 @@ extern "C" void cudaFilter(float *h_mask, unsigned int mask_width)
     sxg_stage_begin(ST_FILTER);
     sxg_convolve(S.img.as<float>(), S.cd.as<float>(), S.W, S.H, g_dmask, K, S.stream);
     sxg_stage_end(ST_FILTER);
+    S.filtered = true;
     SXG_HIP(hipGetLastError());
     SXG_HIP(hipStreamSynchronize(S.stream));
 }

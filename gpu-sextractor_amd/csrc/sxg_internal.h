@@ -89,9 +89,9 @@ struct SxgState {
     int ext_minarea = 5;
     int64_t npix_above = 0, npix_raw = 0;
     int nraw = 0;
-    DevBuf fg;       /* int[P_all]: foreground pixel indices, raster order */
-    DevBuf rawobj;   /* per raw object: first pixel, count, offset, fdflux, fdpeak, bbox */
-    DevBuf fpia;     /* int[P0]: pixels grouped by raw object, raster order inside */
+    bool filtered = false;  /* cudaFilter ran on this frame */
+    int *fpia = nullptr;      /* int[P0]: pixels grouped by raw object, raster order inside */
+    int *fpia_keys = nullptr; /* int[P0]: raw object id of each fpia entry */
     DevBuf scan_tmp; /* scan scratch */
 
     /* deblending and objects */
@@ -104,8 +104,18 @@ struct SxgState {
     DevBuf rootobj;  /* per root: first object, object count */
     int nsurv = 0;
     bool cleaned = false;
+    bool clean_enabled = true;
+    float deb_base = 0.0f;
+    double mag_zeropoint = 0.0;
     float stage_ms[ST_N] = {};
 };
+
+/* raw (level-0) objects, structure of arrays */
+struct RawObj {
+    int *first, *count, *off, *xmin, *xmax, *ymin, *ymax;
+    float *fdpeak, *fdflux;
+};
+bool sxg_raw_alloc(int nraw, RawObj &ro);   /* (re)binds the named buffers */
 
 SxgState &sxg_state();
 void sxg_stage_begin(StageId s);
@@ -118,5 +128,23 @@ void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s);
 /* Stream compaction of the indices i in [0,n) with flag[i] != 0, in increasing order;
    *count (device) receives the number kept. */
 void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s);
+/* Same, keeping the indices i with v[i] >= 0. */
+void sxg_compact_nonneg(const int *v, int *out, int n, int *count, hipStream_t s);
+/* Stable LSD radix sort of (key, value) pairs on the low nbits of non-negative keys;
+   returns 0 when the result is in (k0, v0), 1 when in (k1, v1). */
+int sxg_radix_sort(int *k0, int *v0, int *k1, int *v1, int n, int nbits, int *hist,
+                   hipStream_t s);
+/* Grow-only device buffers addressed by name (detection/deblending scratch). */
+DevBuf &sxg_buf(const char *name);
+void sxg_release_named();
+/* Typed view of a named buffer holding at least n elements (nullptr on failure). */
+template <class T> inline T *sxg_take(const char *name, size_t n)
+{
+    DevBuf &b = sxg_buf(name);
+    if (!b.ensure(sizeof(T) * (n + 16))) return nullptr;
+    return b.as<T>();
+}
+/* FP32 bits of non-negative floats order like the integers. */
+__device__ inline int f2ord(float v) { return __float_as_int(v); }
 
 static inline int sxg_div_up(long a, long b) { return (int)((a + b - 1) / b); }

@@ -105,9 +105,9 @@ extern "C" void sxg_release(void)
     SxgState &S = sxg_state();
     if (S.stream) (void)hipStreamSynchronize(S.stream);
     DevBuf *bufs[] = {&S.img, &S.cd, &S.lab, &S.aux, &S.mesh, &S.maps, &S.rowspl, &S.small,
-                      &S.fg, &S.rawobj, &S.fpia, &S.scan_tmp, &S.thr, &S.lvl, &S.comps,
-                      &S.objs, &S.rootobj};
+                      &S.scan_tmp, &S.thr, &S.lvl, &S.comps, &S.objs, &S.rootobj};
     for (DevBuf *b : bufs) b->release();
+    sxg_release_named();
     S.have_frame = false;
 }
 
@@ -201,6 +201,10 @@ void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s)
 
 /* ------------------------------------------------------------ compaction ---- */
 /* Contiguous per-thread chunks keep the output in increasing index order. */
+struct PredNonzero { __device__ bool operator()(int v) const { return v != 0; } };
+struct PredNonneg { __device__ bool operator()(int v) const { return v >= 0; } };
+
+template <class P>
 __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_count(const int *flag, int n, int *bsum)
 {
     __shared__ int sm[SCAN_BLOCK];
@@ -208,13 +212,14 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_count(const int *flag, i
     int s = 0;
     for (int i = 0; i < SCAN_ITEMS; i++) {
         long k = base + i;
-        if (k < n && flag[k]) s++;
+        if (k < n && P()(flag[k])) s++;
     }
     int tot;
     block_excl_scan(s, sm, &tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
+template <class P>
 __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_write(const int *flag, int n,
                                                               const int *bsum, int *out)
 {
@@ -223,7 +228,7 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_write(const int *flag, i
     int f[SCAN_ITEMS], s = 0;
     for (int i = 0; i < SCAN_ITEMS; i++) {
         long k = base + i;
-        f[i] = (k < n && flag[k]) ? 1 : 0;
+        f[i] = (k < n && P()(flag[k])) ? 1 : 0;
         s += f[i];
     }
     int tot;
@@ -232,7 +237,8 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_write(const int *flag, i
         if (f[i]) out[pos++] = (int)(base + i);
 }
 
-void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s)
+template <class P>
+static void compact_impl(const int *flag, int *out, int n, int *count, hipStream_t s)
 {
     SxgState &S = sxg_state();
     if (n <= 0) {
@@ -242,9 +248,48 @@ void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t
     const int nb = sxg_div_up(n, SCAN_TILE);
     if (!S.scan_tmp.ensure(sizeof(int) * (nb + 64))) return;
     int *bsum = S.scan_tmp.as<int>();
-    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum);
+    hipLaunchKernelGGL(k_compact_count<P>, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_BLOCK), 0, s, bsum, nb, count);
-    hipLaunchKernelGGL(k_compact_write, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum, out);
+    hipLaunchKernelGGL(k_compact_write<P>, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum, out);
+}
+
+void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s)
+{
+    compact_impl<PredNonzero>(flag, out, n, count, s);
+}
+
+void sxg_compact_nonneg(const int *v, int *out, int n, int *count, hipStream_t s)
+{
+    compact_impl<PredNonneg>(v, out, n, count, s);
+}
+
+/* ------------------------------------------------------- named buffers ---- */
+#include <map>
+static std::map<std::string, DevBuf> &named()
+{
+    static std::map<std::string, DevBuf> bufs;
+    return bufs;
+}
+DevBuf &sxg_buf(const char *name) { return named()[name]; }
+void sxg_release_named()
+{
+    for (auto &kv : named()) kv.second.release();
+}
+
+bool sxg_raw_alloc(int nraw, RawObj &ro)
+{
+    const size_t n = (size_t)(nraw > 0 ? nraw : 1);
+    ro.first = sxg_take<int>("raw.first", n);
+    ro.count = sxg_take<int>("raw.count", n);
+    ro.off = sxg_take<int>("raw.off", n);
+    ro.xmin = sxg_take<int>("raw.xmin", n);
+    ro.xmax = sxg_take<int>("raw.xmax", n);
+    ro.ymin = sxg_take<int>("raw.ymin", n);
+    ro.ymax = sxg_take<int>("raw.ymax", n);
+    ro.fdpeak = sxg_take<float>("raw.fdpeak", n);
+    ro.fdflux = sxg_take<float>("raw.fdflux", n);
+    return ro.first && ro.count && ro.off && ro.xmin && ro.xmax && ro.ymin && ro.ymax &&
+           ro.fdpeak && ro.fdflux;
 }
 
 /* -------------------------------------------------------- frame entry points ---- */
@@ -267,6 +312,7 @@ extern "C" void init_device(int width, int height, float *imgbuf)
     S.N = (size_t)width * height;
     S.nraw = S.nobj = S.nsurv = 0;
     S.cleaned = false;
+    S.filtered = false;
     for (int i = 0; i < ST_N; i++) S.ev_used[i] = false;
     if (!S.img.ensure(S.N * sizeof(float)) || !S.cd.ensure(S.N * sizeof(float)) ||
         !S.lab.ensure(S.N * sizeof(int)) || !S.aux.ensure(S.N * sizeof(int)))

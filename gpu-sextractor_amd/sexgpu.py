@@ -12,6 +12,7 @@ is visible, the calls raise.
 """
 import ctypes as C
 import os
+import struct
 
 import numpy as np
 
@@ -367,3 +368,98 @@ def apply_filter(mask):
         raise ValueError("the C-ABI takes square masks (cudafilter.cu:150)")
     gpu().cudaFilter(mask.ctypes.data, mask.shape[0])
     check()
+
+
+# FLAG() convention of the reference (src/define.h:283): a boolean in the first byte of
+# a double/float/int argument.
+FLAG_TRUE_DOUBLE = struct.unpack("<d", b"\x01" + b"\x00" * 7)[0]
+FLAG_TRUE_FLOAT = struct.unpack("<f", b"\x01" + b"\x00" * 3)[0]
+
+
+def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog")):
+    """Run the whole path in the order of the reference's scan.c:78-146 and collect the
+    requested outputs (numpy arrays)."""
+    L = gpu()
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    H, W = img.shape
+    bm, bsig = run_background(img, p)
+    dthresh, thresh = thresholds(p, bsig)
+    L.init_detection()
+    check()
+    if p.mask is not None:
+        apply_filter(p.mask)
+    nraw, npix = C.c_size_t(), C.c_size_t()
+    L.run_detection(dthresh, thresh, p.ext_minarea, C.byref(nraw), C.byref(npix))
+    check()
+    L.init_deblend(dthresh, p.deblend_nthresh, nraw, npix)
+    check()
+    nobj = L.parcel_out(dthresh, p.deblend_nthresh, min(p.ext_minarea, 3), nraw, npix,
+                        p.deblend_mincont)
+    check()
+    L.clear_deblend(p.deblend_nthresh)
+    bw = min(p.back_size, W)
+    nbx, nby = (W - 1) // bw + 1, (H - 1) // bw + 1
+    L.analyse_gpu(FLAG_TRUE_DOUBLE, 1, 1, FLAG_TRUE_FLOAT, bsig, p.gain, 0.0, thresh, dthresh,
+                  p.satur_level, 0, 0, 0, p.ext_minarea, p.clean, 0, nbx, nby, bw, bw, W, nobj)
+    check()
+    master = np.zeros(max(nobj, 1), np.uint32)
+    L.run_clean(master.ctypes.data, p.clean_param, p.clean_stacksize, nobj)
+    check()
+    out = {"backmean": bm, "backsig": bsig, "dthresh": dthresh, "thresh": thresh,
+           "nraw": nraw.value, "npix_raw": npix.value, "nobj": nobj, "master": master[:nobj]}
+    if "columns" in outputs:
+        for name in COLUMNS:
+            if name != "master":
+                out[name] = get_column(name, nobj)
+    if "seg" in outputs:
+        seg = np.zeros((H, W), np.int32)
+        out["nsurv"] = L.sxg_get_segmap(seg.ctypes.data)
+        check()
+        out["seg"] = seg
+    if "alloc" in outputs:
+        al = np.zeros((H, W), np.int32)
+        L.sxg_get_alloc_map(al.ctypes.data)
+        check()
+        out["alloc"] = al
+    if "catalog" in outputs:
+        rows = (OutObj * max(nobj, 1))()
+        L.endobject_gpu(None, None, None, C.cast(rows, C.c_void_p), None, 0, H, W, H, bsig,
+                        thresh, p.satur_level, 1.0, 0.0, p.gain, nobj)
+        check()
+        out["catalog"] = rows
+    if "cd" in outputs:
+        out["cd"] = filtered((H, W))
+    info = frame_info()
+    out["npix_above"] = info["npix_above"]
+    L.clear_detection()
+    check()
+    sub = np.zeros((H, W), np.float32)
+    L.clear_device(sub.ctypes.data)
+    check()
+    if "sub" in outputs:
+        out["sub"] = sub
+    return out
+
+
+def stage_times():
+    ms = (C.c_float * 6)()
+    gpu().sxg_stage_times(ms, 6)
+    return dict(zip(["background", "filter", "detection", "deblending", "analysis", "clean"],
+                    list(ms)))
+
+
+class OutObj(C.Structure):
+    """Catalogue row, layout of the reference's outobjstruct (src/types.h:417-455)."""
+    _pack_ = 0
+    _fields_ = [("number", C.c_int), ("alpha2000", C.c_double), ("delta2000", C.c_double),
+                ("sposx", C.c_float), ("sposy", C.c_float), ("flux_iso", C.c_float),
+                ("fluxerr_iso", C.c_float), ("mag_iso", C.c_float), ("magerr_iso", C.c_float),
+                ("flux_isocor", C.c_float), ("fluxerr_isocor", C.c_float),
+                ("flux_aper", C.c_float * 1), ("fluxerr_aper", C.c_float * 1),
+                ("mag_aper", C.c_float * 1), ("magerr_aper", C.c_float * 1),
+                ("flux_auto", C.c_float), ("fluxerr_auto", C.c_float), ("mag_auto", C.c_float),
+                ("magerr_auto", C.c_float), ("theta", C.c_float), ("poserr_mx2", C.c_double),
+                ("poserr_my2", C.c_double), ("poserr_mxy", C.c_double), ("flag", C.c_short),
+                ("elong", C.c_float), ("ellip", C.c_float), ("sprob", C.c_float),
+                ("peak", C.c_float), ("kronfactor", C.c_float), ("bkg", C.c_float),
+                ("fwhm", C.c_float), ("_pad", C.c_byte * 4)]

@@ -96,6 +96,9 @@ const char *sxg_last_error(void);
 /* BACK_VALUE for back_type 1 (the reference reads prefs.back_val on the host). */
 void sxg_set_back_value(float value);
 
+/* MAG_ZEROPOINT used by endobject_gpu (the reference reads prefs.mag_zeropoint). */
+void sxg_set_mag_zeropoint(double zp);
+
 /* Segmentation check-image: out[width*height] = catalogue NUMBER (1-based, survivors of
    CLEAN in object order) on each object's final pixels, 0 elsewhere.  Call after
    run_clean (before clear_detection). Returns the number of survivors or < 0. */

@@ -30,9 +30,7 @@
  *   - gatherup falls back to the root's first object when no distance is finite (the
  *     reference falls back to global object 0, cudadeblend.cu:1300) and never reads
  *     past the root's object segment (:1306-1311);
- *   - CLEAN N skips cleaning (the reference cleans regardless of the flag);
- *   - the mthresh heap holds DETECT_MINAREA values (stock SExtractor; the reference
- *     hard-codes 5, cudatypes.h:62, and warns when they differ).
+ *   - CLEAN N skips cleaning (the reference cleans regardless of the flag).
  */
 #include <math.h>
 #include <stdio.h>
@@ -601,7 +599,9 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
     r->master = (uint32_t *)malloc(sizeof(uint32_t) * (nobj + 1));
 
     const double backnoise2 = (double)(r->backsig * r->backsig);
-    const int hsize = cfg->ext_minarea > 0 ? cfg->ext_minarea : 1;
+    /* the mthresh order statistic uses the reference's EXT_MINAREA = 5 (cudatypes.h:62,
+       cudaanalyse.cu:2322-2335), whatever DETECT_MINAREA is */
+    const int hsize = 5;
     float *top = (float *)malloc(sizeof(float) * hsize);
     for (int o = 0; o < nobj; o++) {
         const int *pl = r->objpix + r->objpix_start[o];

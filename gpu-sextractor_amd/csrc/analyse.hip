@@ -280,7 +280,9 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
     if (!sxg_obj_view(A.o)) return;
     S.clean_enabled = prefs_clean_flag != 0;
     sxg_stage_begin(ST_ANALYSE);
+    sxg_kt_begin(SXG_KT_ANALYSE);
     hipLaunchKernelGGL(k_analyse, dim3(S.nraw), dim3(AW), 0, S.stream, A);
+    sxg_kt_end(SXG_KT_ANALYSE);
     sxg_stage_end(ST_ANALYSE);
     SXG_HIP(hipGetLastError());
     SXG_HIP(hipStreamSynchronize(S.stream));

@@ -594,12 +594,18 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
     float *img = S.img.as<float>();
 
     sxg_stage_begin(ST_BACK);
+    sxg_kt_begin(SXG_KT_MESHSTAT);
     hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nmesh, 64)), dim3(64), 0, S.stream, img, W, H,
                        bs, nbx, nmesh, ms);
+    sxg_kt_end(SXG_KT_MESHSTAT);
+    sxg_kt_begin(SXG_KT_MESHMODE);
     hipLaunchKernelGGL(k_meshmode, dim3(nmesh), dim3(MODE_THREADS), 0, S.stream, img, W, H, bs,
                        nbx, ms, back, sigma);
+    sxg_kt_end(SXG_KT_MESHMODE);
+    sxg_kt_begin(SXG_KT_FILTERBACK);
     hipLaunchKernelGGL(k_filterback, dim3(1), dim3(FB_THREADS), 0, S.stream, back, sigma, dback,
                        dsigma, nbx, nby, nfx, nfy, (double)backfthresh, work, n2, outv);
+    sxg_kt_end(SXG_KT_FILTERBACK);
     float hv[2];
     SXG_HIP(hipMemcpyAsync(hv, outv, sizeof(hv), hipMemcpyDeviceToHost, S.stream));
     SXG_HIP(hipStreamSynchronize(S.stream));
@@ -620,10 +626,14 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
         column_tables(W, nbx, bs, seg, dxs);
         SXG_HIP(hipMemcpyAsync(colseg, seg.data(), sizeof(int) * W, hipMemcpyHostToDevice, S.stream));
         SXG_HIP(hipMemcpyAsync(coldx, dxs.data(), sizeof(float) * W, hipMemcpyHostToDevice, S.stream));
+        sxg_kt_begin(SXG_KT_ROWNODES);
         hipLaunchKernelGGL(k_rownodes, dim3(sxg_div_up(H, 256)), dim3(256), 0, S.stream, back,
                            dback, nbx, nby, bs, H, nodes, dnodes, utmp);
+        sxg_kt_end(SXG_KT_ROWNODES);
+        sxg_kt_begin(SXG_KT_SUBBACK);
         hipLaunchKernelGGL(k_subback, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0, S.stream,
                            img, W, H, nbx, nodes, dnodes, colseg, coldx);
+        sxg_kt_end(SXG_KT_SUBBACK);
         SXG_HIP(hipStreamSynchronize(S.stream));   /* host tables must outlive the copies */
     }
     sxg_stage_end(ST_BACK);

@@ -90,8 +90,10 @@ extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacks
         /* CLEAN N: keep everything (the reference cleans regardless of the flag) */
         if (S.clean_enabled) {
             const int win = ((clean_stacksize - 1) / 16 + 1) * 16;
+            sxg_kt_begin(SXG_KT_CLEAN);
             hipLaunchKernelGGL(k_clean_pairs, dim3(sxg_div_up(n, 256)), dim3(256), 0, S.stream, o,
                                n, win, amp, alpha, o.master);
+            sxg_kt_end(SXG_KT_CLEAN);
         }
     }
     sxg_stage_end(ST_CLEAN);
@@ -148,9 +150,11 @@ static int *build_map(int mode)
                            o.master, S.nobj, keep);
         sxg_scan_excl(keep, rank, S.nobj, tot, S.stream);
     }
+    sxg_kt_begin(SXG_KT_SEG);
     hipLaunchKernelGGL(k_seg_scatter, dim3(sxg_div_up(S.npix_raw, 256)), dim3(256), 0, S.stream,
                        S.fpia, (long)S.npix_raw, sxg_take<int>("deb.alloc", S.N), keep, rank, map,
                        mode);
+    sxg_kt_end(SXG_KT_SEG);
     if (mode == 0) {
         int ns = 0;
         if (hipMemcpyAsync(&ns, tot, sizeof(int), hipMemcpyDeviceToHost, S.stream) == hipSuccess &&
@@ -159,6 +163,8 @@ static int *build_map(int mode)
     }
     return map;
 }
+
+bool sxg_build_segmap() { return build_map(0) != nullptr; }
 
 extern "C" int sxg_get_segmap(int32_t *out)
 {

@@ -414,9 +414,12 @@ extern "C" void init_deblend(float basethresh, int nthresh, size_t numValidObj, 
     RawObj ro;
     if (!sxg_raw_alloc(S.nraw, ro)) return;
     const long tot = (long)S.nraw * nthresh;
-    if (tot > 0)
+    if (tot > 0) {
+        sxg_kt_begin(SXG_KT_DEB_THRESH);
         hipLaunchKernelGGL(k_deb_thresh, dim3(sxg_div_up(tot, 256)), dim3(256), 0, S.stream,
                            ro.fdpeak, S.nraw, nthresh, basethresh, thr);
+        sxg_kt_end(SXG_KT_DEB_THRESH);
+    }
     SXG_HIP(hipGetLastError());
     g_deb_ready = true;
 }
@@ -505,7 +508,9 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     a.rootnc = rootnc; a.rootkept = rootkept; a.rawflux = ro.fdflux;
     int nobj = 0;
     if (nraw > 0) {
+        sxg_kt_begin(SXG_KT_DEB_LEVELS);
         hipLaunchKernelGGL(k_deb_levels, dim3(nraw), dim3(DW), 0, st, a);
+        sxg_kt_end(SXG_KT_DEB_LEVELS);
         sxg_scan_excl(rootkept, objbase, nraw, reinterpret_cast<int *>(dtot), st);
         SXG_HIP_RET(hipMemcpyAsync(&nobj, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
         SXG_HIP_RET(hipStreamSynchronize(st), -1);
@@ -516,7 +521,9 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     if (nraw > 0) {
         ObjArgs A;
         A.d = a; A.objbase = objbase; A.alloc = alloc; A.sub = S.img.as<float>(); A.o = oc;
+        sxg_kt_begin(SXG_KT_DEB_OBJECTS);
         hipLaunchKernelGGL(k_deb_objects, dim3(nraw), dim3(DW), 0, st, A);
+        sxg_kt_end(SXG_KT_DEB_OBJECTS);
     }
     sxg_stage_end(ST_DEBLEND);
     SXG_HIP_RET(hipGetLastError(), -1);

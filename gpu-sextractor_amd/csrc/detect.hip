@@ -224,12 +224,16 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     int *rootpos = sxg_take<int>("det.rootpos", S.N);
     int *cnt_d = sxg_take<int>("det.counts", 8);
     if (!rootpos || !cnt_d) return;
+    sxg_kt_begin(SXG_KT_DET_INIT);
     hipLaunchKernelGGL(k_det_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
                        S.cd.as<float>(), dthreshold, N, par, lab);
+    sxg_kt_end(SXG_KT_DET_INIT);
     /* the foreground list has at most N entries; size it after counting */
     int *fg = sxg_take<int>("det.fg", S.N);
     if (!fg) return;
+    sxg_kt_begin(SXG_KT_DET_FG);
     sxg_compact_nonneg(par, fg, (int)N, cnt_d, st);
+    sxg_kt_end(SXG_KT_DET_FG);
     const int nfg = read_count(cnt_d, st);
     if (nfg < 0) return;
     S.npix_above = nfg;
@@ -248,11 +252,17 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         !rawpos || !rawid_of_pos)
         return;
     if (nfg > 0) {
+        sxg_kt_begin(SXG_KT_DET_UNION);
         hipLaunchKernelGGL(k_det_union, dim3(gfg), dim3(256), 0, st, fg, nfg, W, par);
+        sxg_kt_end(SXG_KT_DET_UNION);
+        sxg_kt_begin(SXG_KT_DET_FLATTEN);
         hipLaunchKernelGGL(k_det_flatten, dim3(gfg), dim3(256), 0, st, fg, nfg, par, lab, rootpos);
+        sxg_kt_end(SXG_KT_DET_FLATTEN);
         hipLaunchKernelGGL(k_det_statinit, dim3(gfg), dim3(256), 0, st, rs, nfg);
+        sxg_kt_begin(SXG_KT_DET_STAT);
         hipLaunchKernelGGL(k_det_rootstat, dim3(gfg), dim3(256), 0, st, fg, nfg, W, lab, rootpos,
                            S.cd.as<float>(), rs);
+        sxg_kt_end(SXG_KT_DET_STAT);
         hipLaunchKernelGGL(k_det_rootflag, dim3(gfg), dim3(256), 0, st, fg, nfg, lab, rs.cnt,
                            ext_minarea, keep);
     }
@@ -286,7 +296,9 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         int nbits = 0;
         while ((1L << nbits) < (long)nraw) nbits++;
         if (nbits == 0) nbits = 1;
+        sxg_kt_begin(SXG_KT_DET_SORT);
         int where = sxg_radix_sort(k0, v0, k1, v1, npix, nbits, hist, st);
+        sxg_kt_end(SXG_KT_DET_SORT);
         int *ks = where ? k1 : k0, *vs = where ? v1 : v0;
         S.fpia_keys = ks;
         S.fpia = vs;

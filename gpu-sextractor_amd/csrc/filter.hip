@@ -143,7 +143,9 @@ extern "C" void cudaFilter(float *h_mask, unsigned int mask_width)
     if (!g_dmask) SXG_HIP(hipMalloc(&g_dmask, sizeof(float) * SXG_MAXMASK * SXG_MAXMASK));
     SXG_HIP(hipMemcpyAsync(g_dmask, h_mask, sizeof(float) * K * K, hipMemcpyHostToDevice, S.stream));
     sxg_stage_begin(ST_FILTER);
+    sxg_kt_begin(SXG_KT_CONV);
     sxg_convolve(S.img.as<float>(), S.cd.as<float>(), S.W, S.H, g_dmask, K, S.stream);
+    sxg_kt_end(SXG_KT_CONV);
     sxg_stage_end(ST_FILTER);
     S.filtered = true;
     SXG_HIP(hipGetLastError());

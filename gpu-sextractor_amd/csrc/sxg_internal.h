@@ -117,6 +117,12 @@ struct RawObj {
 };
 bool sxg_raw_alloc(int nraw, RawObj &ro);   /* (re)binds the named buffers */
 
+/* kernel timers (HIP events on the library stream, enabled by sxg_kernel_timing) */
+
+void sxg_kt_begin(int id);
+void sxg_kt_end(int id);
+bool sxg_build_segmap();   /* device segmentation map ("out.map") and survivor count */
+
 SxgState &sxg_state();
 void sxg_stage_begin(StageId s);
 void sxg_stage_end(StageId s);

@@ -1,0 +1,446 @@
+/*
+ * sex_main.cpp -- the `sex` front-end of the MI355X detection path.
+ *
+ *   sex image.fits [-c config.sex] [-KEY value ...]      run
+ *   sex -d                                               print the default configuration
+ *
+ * It reads the keys of the reference's configuration (src/preflist.h) that the detection
+ * path uses, the .param list of catalogue columns and the .conv filter, reads the FITS
+ * frame, calls the C-ABI in the order of the reference's src/scan.c:78-146, and writes
+ * the catalogue (ASCII_HEAD or ASCII, column names, formats and units as in the
+ * reference's key table src/globals.c) and the check-images SEGMENTATION, -BACKGROUND
+ * and FILTERED.  Catalogue NUMBER and the segmentation values are 1-based.
+ */
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "fitsio.h"
+#include "sexgpu.h"
+
+namespace {
+
+struct Config {
+    std::map<std::string, std::string> kv;
+    Config()
+    {
+        /* defaults of the reference's default.sex / preflist.h for the keys used here */
+        kv = {{"CATALOG_NAME", "test.cat"}, {"CATALOG_TYPE", "ASCII_HEAD"},
+              {"PARAMETERS_NAME", "default.param"}, {"DETECT_TYPE", "CCD"},
+              {"DETECT_MINAREA", "5"}, {"THRESH_TYPE", "RELATIVE"}, {"DETECT_THRESH", "1.5"},
+              {"ANALYSIS_THRESH", "1.5"}, {"FILTER", "Y"}, {"FILTER_NAME", "default.conv"},
+              {"DEBLEND_NTHRESH", "32"}, {"DEBLEND_MINCONT", "0.005"}, {"CLEAN", "Y"},
+              {"CLEAN_PARAM", "1.0"}, {"MEMORY_OBJSTACK", "3000"}, {"SATUR_LEVEL", "50000.0"},
+              {"MAG_ZEROPOINT", "0.0"}, {"GAIN", "0.0"}, {"BACK_TYPE", "AUTO"},
+              {"BACK_VALUE", "0.0"}, {"BACK_SIZE", "64"}, {"BACK_FILTERSIZE", "3"},
+              {"BACK_FILTTHRESH", "0.0"}, {"WEIGHT_TYPE", "NONE"}, {"WEIGHT_IMAGE", "weight.fits"},
+              {"CHECKIMAGE_TYPE", "NONE"}, {"CHECKIMAGE_NAME", "check.fits"},
+              {"VERBOSE_TYPE", "NORMAL"}};
+    }
+    std::string get(const std::string &k) const
+    {
+        auto it = kv.find(k);
+        return it == kv.end() ? std::string() : it->second;
+    }
+    double num(const std::string &k) const { return atof(get(k).c_str()); }
+    bool flag(const std::string &k) const
+    {
+        std::string v = get(k);
+        return !v.empty() && (v[0] == 'Y' || v[0] == 'y' || v[0] == 'T' || v[0] == 't');
+    }
+    /* first element of a comma-separated list */
+    std::string first(const std::string &k) const
+    {
+        std::string v = get(k);
+        size_t c = v.find(',');
+        return c == std::string::npos ? v : v.substr(0, c);
+    }
+};
+
+std::string trim(const std::string &s)
+{
+    size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+    return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+bool read_config(const char *path, Config &cfg)
+{
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string line;
+    while (std::getline(f, line)) {
+        size_t h = line.find('#');
+        if (h != std::string::npos) line = line.substr(0, h);
+        line = trim(line);
+        if (line.empty()) continue;
+        std::istringstream ss(line);
+        std::string key, rest;
+        ss >> key;
+        std::getline(ss, rest);
+        cfg.kv[key] = trim(rest);
+    }
+    return true;
+}
+
+struct Column {
+    const char *name, *desc, *fmt, *unit;
+};
+
+/* names, descriptions, formats and units as in the reference's key table (globals.c) */
+const Column kColumns[] = {
+    {"NUMBER", "Running object number", "%10d", ""},
+    {"X_IMAGE", "Object position along x", "%10.3f", "pixel"},
+    {"Y_IMAGE", "Object position along y", "%10.3f", "pixel"},
+    {"FLUX_ISO", "Isophotal flux", "%12.7g", "count"},
+    {"FLUXERR_ISO", "RMS error for isophotal flux", "%12.7g", "count"},
+    {"MAG_ISO", "Isophotal magnitude", "%8.4f", "mag"},
+    {"MAGERR_ISO", "RMS error for isophotal magnitude", "%8.4f", "mag"},
+    {"ISOAREA_IMAGE", "Isophotal area above Analysis threshold", "%9d", "pixel**2"},
+    {"FLUX_MAX", "Peak flux above background", "%12.7g", "count"},
+    {"BACKGROUND", "Background at centroid position", "%12.7g", "count"},
+    {"THRESHOLD", "Detection threshold above background", "%12.7g", "count"},
+    {"X2_IMAGE", "Variance along x", "%15.10e", "pixel**2"},
+    {"Y2_IMAGE", "Variance along y", "%15.10e", "pixel**2"},
+    {"XY_IMAGE", "Covariance between x and y", "%15.10e", "pixel**2"},
+    {"A_IMAGE", "Profile RMS along major axis", "%9.3f", "pixel"},
+    {"B_IMAGE", "Profile RMS along minor axis", "%9.3f", "pixel"},
+    {"THETA_IMAGE", "Position angle (CCW/x)", "%5.1f", "deg"},
+    {"CXX_IMAGE", "Cxx object ellipse parameter", "%12.7e", "pixel**(-2)"},
+    {"CYY_IMAGE", "Cyy object ellipse parameter", "%12.7e", "pixel**(-2)"},
+    {"CXY_IMAGE", "Cxy object ellipse parameter", "%12.7e", "pixel**(-2)"},
+    {"ERRX2_IMAGE", "Variance of position along x", "%15.10e", "pixel**2"},
+    {"ERRY2_IMAGE", "Variance of position along y", "%15.10e", "pixel**2"},
+    {"ERRXY_IMAGE", "Covariance of position between x and y", "%15.10e", "pixel**2"},
+    {"FLAGS", "Extraction flags", "%3d", ""},
+    {"FWHM_IMAGE", "FWHM assuming a gaussian core", "%8.2f", "pixel"},
+    {"ELONGATION", "A_IMAGE/B_IMAGE", "%8.3f", ""},
+    {"ELLIPTICITY", "1 - B_IMAGE/A_IMAGE", "%8.3f", ""},
+    {"XMIN_IMAGE", "Minimum x-coordinate among detected pixels", "%10d", "pixel"},
+    {"YMIN_IMAGE", "Minimum y-coordinate among detected pixels", "%10d", "pixel"},
+    {"XMAX_IMAGE", "Maximum x-coordinate among detected pixels", "%10d", "pixel"},
+    {"YMAX_IMAGE", "Maximum y-coordinate among detected pixels", "%10d", "pixel"},
+    {"XPEAK_IMAGE", "x-coordinate of the brightest pixel", "%10d", "pixel"},
+    {"YPEAK_IMAGE", "y-coordinate of the brightest pixel", "%10d", "pixel"},
+    {"ISO0", "Isophotal area at level 0", "%8d", "pixel**2"},
+    {"ISO1", "Isophotal area at level 1", "%8d", "pixel**2"},
+    {"ISO2", "Isophotal area at level 2", "%8d", "pixel**2"},
+    {"ISO3", "Isophotal area at level 3", "%8d", "pixel**2"},
+    {"ISO4", "Isophotal area at level 4", "%8d", "pixel**2"},
+    {"ISO5", "Isophotal area at level 5", "%8d", "pixel**2"},
+    {"ISO6", "Isophotal area at level 6", "%8d", "pixel**2"},
+    {"ISO7", "Isophotal area at level 7", "%8d", "pixel**2"},
+};
+
+const Column *find_column(const std::string &name)
+{
+    for (const Column &c : kColumns)
+        if (name == c.name) return &c;
+    return nullptr;
+}
+
+bool read_conv(const std::string &path, std::vector<float> &mask, int &mw)
+{
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string line;
+    if (!std::getline(f, line) || line.compare(0, 4, "CONV")) return false;
+    bool norm = line.find("NORM") == std::string::npos || line.find("NONORM") == std::string::npos;
+    int width = 0;
+    while (std::getline(f, line)) {
+        std::istringstream ss(line);
+        std::string tok;
+        int j = 0;
+        while (ss >> tok) {
+            if (j == 0 && tok[0] == '#') break;
+            mask.push_back((float)atof(tok.c_str()));
+            j++;
+        }
+        if (j > width) width = j;
+    }
+    if (width < 1 || mask.size() % width) return false;
+    if (norm) {   /* filter.c:164-181 */
+        double sum = 0.0, var = 0.0;
+        for (float v : mask) {
+            sum += std::fabs((double)v);
+            var += (double)v * v;
+        }
+        if (sum == 0.0) sum = std::sqrt(var);
+        for (float &v : mask) v = (float)(v / sum);
+    }
+    mw = width;
+    return true;
+}
+
+void dump_defaults()
+{
+    Config c;
+    for (auto &kv : c.kv) printf("%-18s %s\n", kv.first.c_str(), kv.second.c_str());
+}
+
+bool fail(const char *what)
+{
+    const char *e = sxg_last_error();
+    if (e) {
+        fprintf(stderr, "sex: %s: %s\n", what, e);
+        return true;
+    }
+    return false;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) {
+        fprintf(stderr, "usage: sex image.fits [-c config.sex] [-KEY value ...] | sex -d\n");
+        return 1;
+    }
+    if (!strcmp(argv[1], "-d") || !strcmp(argv[1], "-dd")) {
+        dump_defaults();
+        return 0;
+    }
+    Config cfg;
+    std::string image;
+    std::vector<std::pair<std::string, std::string>> overrides;
+    for (int i = 1; i < argc; i++) {
+        if (!strcmp(argv[i], "-c") && i + 1 < argc) {
+            if (!read_config(argv[++i], cfg)) {
+                fprintf(stderr, "sex: cannot read configuration %s\n", argv[i]);
+                return 1;
+            }
+        } else if (argv[i][0] == '-' && i + 1 < argc) {
+            overrides.push_back({argv[i] + 1, argv[i + 1]});
+            i++;
+        } else
+            image = argv[i];
+    }
+    for (auto &o : overrides) cfg.kv[o.first] = o.second;   /* main.c:104-107 order */
+    if (image.empty()) {
+        fprintf(stderr, "sex: no image given\n");
+        return 1;
+    }
+    if (cfg.get("WEIGHT_TYPE") != "NONE") {
+        fprintf(stderr, "sex: WEIGHT_TYPE %s is not supported by this build\n",
+                cfg.get("WEIGHT_TYPE").c_str());
+        return 1;
+    }
+    if (cfg.get("DETECT_TYPE") != "CCD") {
+        fprintf(stderr, "sex: only DETECT_TYPE CCD is supported\n");
+        return 1;
+    }
+    /* catalogue columns */
+    std::vector<const Column *> cols;
+    {
+        std::ifstream pf(cfg.get("PARAMETERS_NAME"));
+        if (!pf) {
+            fprintf(stderr, "sex: cannot read %s\n", cfg.get("PARAMETERS_NAME").c_str());
+            return 1;
+        }
+        std::string line;
+        while (std::getline(pf, line)) {
+            line = trim(line);
+            if (line.empty() || line[0] == '#') continue;
+            std::string name = line.substr(0, line.find_first_of(" \t("));
+            const Column *c = find_column(name);
+            if (c)
+                cols.push_back(c);
+            else
+                fprintf(stderr, "sex: warning: catalogue parameter %s is not produced by the "
+                                "detection path; skipped\n", name.c_str());
+        }
+    }
+    /* frame */
+    int W = 0, H = 0;
+    double gain = cfg.num("GAIN"), satur = cfg.num("SATUR_LEVEL");
+    const char *err = nullptr;
+    float *img = sxf_read_image(image.c_str(), &W, &H, nullptr, nullptr, &err);
+    if (!img) {
+        fprintf(stderr, "sex: cannot read %s: %s\n", image.c_str(), err ? err : "?");
+        return 1;
+    }
+    const int bsz = atoi(cfg.first("BACK_SIZE").c_str());
+    const int bw = bsz < W ? bsz : W, bh = bsz < H ? bsz : H;
+    if (bw != bh) {
+        fprintf(stderr, "sex: only square background meshes are supported\n");
+        return 1;
+    }
+    const int nbx = (W - 1) / bw + 1, nby = (H - 1) / bh + 1;
+    const int nfs = atoi(cfg.first("BACK_FILTERSIZE").c_str());
+    const int back_type = cfg.first("BACK_TYPE") == "MANUAL" ? 1 : 0;
+
+    /* makeit.c:189, back.c:104 */
+    init_device(W, H, img);
+    if (fail("init_device")) return 1;
+    sxg_set_back_value((float)atof(cfg.first("BACK_VALUE").c_str()));
+    sxg_set_mag_zeropoint(cfg.num("MAG_ZEROPOINT"));
+    float backmean = 0, backsig = 0;
+    back_estimate(bw, (float)cfg.num("BACK_FILTTHRESH"), &backmean, &backsig,
+                  nbx > 1 ? nfs : 1, nby > 1 ? nfs : 1, back_type);
+    if (fail("back_estimate")) return 1;
+    /* back.c:155-192 */
+    const bool absolute = cfg.first("THRESH_TYPE") == "ABSOLUTE";
+    const double dt0 = atof(cfg.first("DETECT_THRESH").c_str());
+    const double at0 = atof(cfg.first("ANALYSIS_THRESH").c_str());
+    const float dthresh = absolute ? (float)dt0 : (float)(dt0 * backsig);
+    const float thresh = absolute ? (float)at0 : (float)(at0 * backsig);
+    if (!(dthresh > 0.0f) || !(thresh > 0.0f)) {
+        fprintf(stderr, "sex: cannot deal with zero or negative thresholds\n");
+        return 1;
+    }
+    /* scan.c:78-146 */
+    init_detection();
+    if (cfg.flag("FILTER")) {
+        std::vector<float> mask;
+        int mw = 0;
+        if (!read_conv(cfg.get("FILTER_NAME"), mask, mw)) {
+            fprintf(stderr, "sex: cannot read filter %s\n", cfg.get("FILTER_NAME").c_str());
+            return 1;
+        }
+        if ((int)mask.size() != mw * mw) {
+            fprintf(stderr, "sex: the filter mask must be square\n");
+            return 1;
+        }
+        cudaFilter(mask.data(), (unsigned)mw);
+        if (fail("cudaFilter")) return 1;
+    }
+    const int minarea = atoi(cfg.get("DETECT_MINAREA").c_str());
+    const int nthresh = atoi(cfg.get("DEBLEND_NTHRESH").c_str());
+    size_t nraw = 0, npix = 0;
+    run_detection(dthresh, thresh, minarea, &nraw, &npix);
+    if (fail("run_detection")) return 1;
+    init_deblend(dthresh, nthresh, nraw, npix);
+    const int nobj = parcel_out(dthresh, nthresh, minarea < 3 ? minarea : 3, nraw, npix,
+                                cfg.num("DEBLEND_MINCONT"));
+    clear_deblend(nthresh);
+    if (fail("parcel_out") || nobj < 0) return 1;
+    double ftrue;
+    float ftruef;
+    {
+        uint64_t u = 1;
+        uint32_t v = 1;
+        memcpy(&ftrue, &u, 8);
+        memcpy(&ftruef, &v, 4);
+    }
+    analyse_gpu(ftrue, 1, 1, ftruef, backsig, gain, 0.0, thresh, dthresh, satur, 0, 0, 0, minarea,
+                cfg.flag("CLEAN"), 0, nbx, nby, bw, bh, W, nobj);
+    if (fail("analyse_gpu")) return 1;
+    std::vector<uint32_t> master(nobj > 0 ? nobj : 1);
+    run_clean(reinterpret_cast<int *>(master.data()), cfg.num("CLEAN_PARAM"),
+              atoi(cfg.get("MEMORY_OBJSTACK").c_str()), nobj);
+    if (fail("run_clean")) return 1;
+    std::vector<sxg_outobj> rows(nobj > 0 ? nobj : 1);
+    endobject_gpu(nullptr, nullptr, nullptr, rows.data(), nullptr, 0, H, W, H, backsig, thresh,
+                  satur, 1.0, 0.0, gain, nobj);
+    if (fail("endobject_gpu")) return 1;
+    /* extra columns not in the reference's fixed row */
+    std::vector<int> npixc(nobj), xmin(nobj), xmax(nobj), ymin(nobj), ymax(nobj), px(nobj),
+        py(nobj), iso(8 * (size_t)nobj);
+    std::vector<double> mx2(nobj), my2(nobj), mxy(nobj);
+    std::vector<float> a(nobj), b(nobj), cxx(nobj), cyy(nobj), cxy(nobj), thr(nobj);
+    if (nobj > 0) {
+        sxg_get_column(SXG_COL_NPIX, npixc.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_XMIN, xmin.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_XMAX, xmax.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_YMIN, ymin.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_YMAX, ymax.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_PEAKX, px.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_PEAKY, py.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_ISO, iso.data(), 32 * (size_t)nobj);
+        sxg_get_column(SXG_COL_MX2, mx2.data(), 8 * nobj);
+        sxg_get_column(SXG_COL_MY2, my2.data(), 8 * nobj);
+        sxg_get_column(SXG_COL_MXY, mxy.data(), 8 * nobj);
+        sxg_get_column(SXG_COL_A, a.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_B, b.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_CXX, cxx.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_CYY, cyy.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_CXY, cxy.data(), 4 * nobj);
+        sxg_get_column(SXG_COL_THRESH, thr.data(), 4 * nobj);
+        if (fail("sxg_get_column")) return 1;
+    }
+    /* check-images */
+    const std::string ctype = cfg.first("CHECKIMAGE_TYPE"), cname = cfg.first("CHECKIMAGE_NAME");
+    std::vector<int32_t> seg((size_t)W * H);
+    int nsurv = sxg_get_segmap(seg.data());
+    if (fail("sxg_get_segmap")) return 1;
+    if (ctype == "SEGMENTATION" && sxf_write_image(cname.c_str(), seg.data(), W, H, 32)) {
+        fprintf(stderr, "sex: cannot write %s\n", cname.c_str());
+        return 1;
+    }
+    if (ctype == "FILTERED") {
+        std::vector<float> cd((size_t)W * H);
+        sxg_get_filtered(cd.data());
+        sxf_write_image(cname.c_str(), cd.data(), W, H, -32);
+    }
+    clear_detection();
+    clear_device(img);   /* the background-subtracted frame */
+    if (ctype == "-BACKGROUND") sxf_write_image(cname.c_str(), img, W, H, -32);
+    free(img);
+    /* catalogue */
+    FILE *cat = cfg.get("CATALOG_NAME") == "STDOUT" ? stdout : fopen(cfg.get("CATALOG_NAME").c_str(), "w");
+    if (!cat) {
+        fprintf(stderr, "sex: cannot write %s\n", cfg.get("CATALOG_NAME").c_str());
+        return 1;
+    }
+    if (cfg.get("CATALOG_TYPE") == "ASCII_HEAD")
+        for (size_t k = 0; k < cols.size(); k++)
+            fprintf(cat, "#%4zu %-16s %-58s [%s]\n", k + 1, cols[k]->name, cols[k]->desc,
+                    cols[k]->unit);
+    int number = 0;
+    for (int i = 0; i < nobj; i++) {
+        if (master[i] != 0xffffffffu) continue;   /* scan.c:148-157 */
+        const sxg_outobj &r = rows[i];
+        number++;
+        for (size_t k = 0; k < cols.size(); k++) {
+            const std::string n = cols[k]->name;
+            const char *f = cols[k]->fmt;
+            if (k) fputc(' ', cat);
+            if (n == "NUMBER") fprintf(cat, f, number);
+            else if (n == "X_IMAGE") fprintf(cat, f, r.sposx);
+            else if (n == "Y_IMAGE") fprintf(cat, f, r.sposy);
+            else if (n == "FLUX_ISO") fprintf(cat, f, r.flux_iso);
+            else if (n == "FLUXERR_ISO") fprintf(cat, f, r.fluxerr_iso);
+            else if (n == "MAG_ISO") fprintf(cat, f, r.mag_iso);
+            else if (n == "MAGERR_ISO") fprintf(cat, f, r.magerr_iso);
+            else if (n == "ISOAREA_IMAGE") fprintf(cat, f, npixc[i]);
+            else if (n == "FLUX_MAX") fprintf(cat, f, r.peak);
+            else if (n == "BACKGROUND") fprintf(cat, f, r.bkg);
+            else if (n == "THRESHOLD") fprintf(cat, f, thr[i]);
+            else if (n == "X2_IMAGE") fprintf(cat, f, mx2[i]);
+            else if (n == "Y2_IMAGE") fprintf(cat, f, my2[i]);
+            else if (n == "XY_IMAGE") fprintf(cat, f, mxy[i]);
+            else if (n == "A_IMAGE") fprintf(cat, f, a[i]);
+            else if (n == "B_IMAGE") fprintf(cat, f, b[i]);
+            else if (n == "THETA_IMAGE") fprintf(cat, f, r.theta);
+            else if (n == "CXX_IMAGE") fprintf(cat, f, cxx[i]);
+            else if (n == "CYY_IMAGE") fprintf(cat, f, cyy[i]);
+            else if (n == "CXY_IMAGE") fprintf(cat, f, cxy[i]);
+            else if (n == "ERRX2_IMAGE") fprintf(cat, f, r.poserr_mx2);
+            else if (n == "ERRY2_IMAGE") fprintf(cat, f, r.poserr_my2);
+            else if (n == "ERRXY_IMAGE") fprintf(cat, f, r.poserr_mxy);
+            else if (n == "FLAGS") fprintf(cat, f, (int)r.flag);
+            else if (n == "FWHM_IMAGE") fprintf(cat, f, r.fwhm);
+            else if (n == "ELONGATION") fprintf(cat, f, r.elong);
+            else if (n == "ELLIPTICITY") fprintf(cat, f, r.ellip);
+            else if (n == "XMIN_IMAGE") fprintf(cat, f, xmin[i] + 1);
+            else if (n == "YMIN_IMAGE") fprintf(cat, f, ymin[i] + 1);
+            else if (n == "XMAX_IMAGE") fprintf(cat, f, xmax[i] + 1);
+            else if (n == "YMAX_IMAGE") fprintf(cat, f, ymax[i] + 1);
+            else if (n == "XPEAK_IMAGE") fprintf(cat, f, px[i]);
+            else if (n == "YPEAK_IMAGE") fprintf(cat, f, py[i]);
+            else if (n.compare(0, 3, "ISO") == 0 && n.size() == 4)
+                fprintf(cat, f, iso[8 * (size_t)i + (n[3] - '0')]);
+        }
+        fputc('\n', cat);
+    }
+    if (cat != stdout) fclose(cat);
+    if (cfg.get("VERBOSE_TYPE") != "QUIET")
+        fprintf(stderr, "Objects: detected %d  sextracted %d\n", nobj, nsurv);
+    return 0;
+}

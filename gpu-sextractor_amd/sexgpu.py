@@ -19,6 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 GPU_LIB = os.path.join(HERE, "lib", "libsexgpu.so")
 HOST_LIB = os.path.join(HERE, "lib", "libsxhost.so")
+CONFIG_DIR = os.path.join(HERE, "config")
 
 _host = None
 _gpu = None
@@ -232,6 +233,12 @@ def gpu():
         L.sxg_device_frame.restype = vp
         L.sxg_stage_times.argtypes = [vp, i]
         L.sxg_release.argtypes = []
+        L.sxg_set_resident_frame.argtypes = [vp, i, i]
+        L.sxg_process_resident.argtypes = [vp, vp]
+        L.sxg_sync.argtypes = []
+        L.sxg_kernel_timing.argtypes = [i]
+        L.sxg_kernel_times.argtypes = [i, C.POINTER(d), C.POINTER(i)]
+        L.sxg_set_mag_zeropoint.argtypes = [d]
         _gpu = L
     return _gpu
 
@@ -247,7 +254,12 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "endobject_gpu", "clear_detection", "clear_device", "sxg_last_error",
             "sxg_set_back_value", "sxg_get_segmap", "sxg_get_column", "sxg_get_frame_info",
             "sxg_get_back_maps", "sxg_get_filtered", "sxg_get_alloc_map", "sxg_device_frame",
-            "sxg_stage_times", "sxg_release"]
+            "sxg_stage_times", "sxg_release", "sxg_set_mag_zeropoint", "sxg_params",
+            "sxg_set_resident_frame", "sxg_process_resident", "sxg_sync", "sxg_kernel_timing",
+            "sxg_kernel_times"]
+KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
+           "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
+           "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg"]
 
 COLUMNS = ["level", "root", "firstpix", "flag", "fdnpix", "dnpix", "npix", "xmin", "xmax",
            "ymin", "ymax", "peakx", "peaky", "iso", "mx", "my", "mx2", "my2", "mxy",
@@ -463,3 +475,71 @@ class OutObj(C.Structure):
                 ("elong", C.c_float), ("ellip", C.c_float), ("sprob", C.c_float),
                 ("peak", C.c_float), ("kronfactor", C.c_float), ("bkg", C.c_float),
                 ("fwhm", C.c_float), ("_pad", C.c_byte * 4)]
+
+
+class SxgParams(C.Structure):
+    """Mirror of sxg_params (include/sexgpu.h)."""
+    _fields_ = [("back_size", C.c_int), ("back_fsize", C.c_int), ("back_fthresh", C.c_float),
+                ("back_type", C.c_int), ("back_value", C.c_float), ("thresh_type", C.c_int),
+                ("detect_thresh", C.c_double), ("analysis_thresh", C.c_double),
+                ("ext_minarea", C.c_int), ("deblend_nthresh", C.c_int),
+                ("deblend_mincont", C.c_double), ("clean", C.c_int),
+                ("clean_param", C.c_double), ("clean_stacksize", C.c_int),
+                ("satur_level", C.c_double), ("gain", C.c_double),
+                ("mask", C.c_void_p), ("mask_width", C.c_int)]
+
+
+def native_params(p):
+    """sxg_params for a Params object (keeps the mask array alive on the result)."""
+    sp = SxgParams()
+    for name, _ in SxgParams._fields_:
+        if name in ("mask", "mask_width"):
+            continue
+        setattr(sp, name, getattr(p, name))
+    if p.mask is not None:
+        m = np.ascontiguousarray(p.mask, dtype=np.float32)
+        sp._mask = m
+        sp.mask = m.ctypes.data
+        sp.mask_width = m.shape[0]
+    return sp
+
+
+def read_conv(name):
+    """Parse a .conv filter file the way src/filter.c:101-186 does (float elements,
+    NORM divides by the double sum of absolute values)."""
+    path = name if os.path.sep in name else os.path.join(CONFIG_DIR, name)
+    with open(path) as f:
+        lines = f.readlines()
+    if not lines or not lines[0].startswith("CONV"):
+        raise ValueError("not a convolution mask: " + path)
+    norm = ("NORM" in lines[0] and "NONORM" not in lines[0]) or ("NORM" not in lines[0])
+    rows, width = [], 0
+    for ln in lines[1:]:
+        tok = ln.split()
+        if not tok or tok[0].startswith("#"):
+            continue
+        vals = [np.float32(float(t)) for t in tok]
+        width = max(width, len(vals))
+        rows.extend(vals)
+    m = np.array(rows, np.float32)
+    if width < 1 or m.size % width:
+        raise ValueError("bad mask line in " + path)
+    if norm:
+        s = float(np.sum(np.abs(m.astype(np.float64))))
+        if s == 0.0:
+            s = float(np.sqrt(np.sum(m.astype(np.float64) ** 2)))
+        m = (m.astype(np.float64) / s).astype(np.float32)
+    return m.reshape(-1, width)
+
+
+def process_resident(img, p):
+    """Upload img once and run the whole path natively from device memory."""
+    L = gpu()
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    L.sxg_set_resident_frame(img.ctypes.data, img.shape[1], img.shape[0])
+    check()
+    sp = native_params(p)
+    fi = FrameInfo()
+    n = L.sxg_process_resident(C.byref(sp), C.byref(fi))
+    check()
+    return n, {k: getattr(fi, k) for k, _ in FrameInfo._fields_}

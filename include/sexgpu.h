@@ -157,6 +157,44 @@ int sxg_stage_times(float *ms, int n);
    across frames otherwise). */
 void sxg_release(void);
 
+/* Configuration of a whole-frame run (the .sex keys the reference front-end passes). */
+typedef struct {
+    int back_size, back_fsize;
+    float back_fthresh;
+    int back_type;              /* 0 AUTO, 1 MANUAL */
+    float back_value;
+    int thresh_type;            /* 0 RELATIVE, 1 ABSOLUTE */
+    double detect_thresh, analysis_thresh;
+    int ext_minarea, deblend_nthresh;
+    double deblend_mincont;
+    int clean;
+    double clean_param;
+    int clean_stacksize;
+    double satur_level, gain;
+    const float *mask;          /* square filter mask (host memory) or NULL */
+    int mask_width;
+} sxg_params;
+
+/* Keep a copy of a frame in device memory (one H2D transfer). */
+int sxg_set_resident_frame(const float *host, int width, int height);
+/* Run the whole path on the resident frame, in the reference's scan.c order, leaving
+   the catalogue columns and the segmentation map on the device.  Returns the number
+   of deblended objects or < 0. */
+int sxg_process_resident(const sxg_params *p, sxg_frame_info *info);
+/* Wait for the library's stream. */
+void sxg_sync(void);
+
+/* Per-kernel HIP-event timers (off by default; enabling resets them). */
+enum sxg_kernel_id {
+    SXG_KT_COPY = 0, SXG_KT_MESHSTAT, SXG_KT_MESHMODE, SXG_KT_FILTERBACK, SXG_KT_ROWNODES,
+    SXG_KT_SUBBACK, SXG_KT_CONV, SXG_KT_DET_INIT, SXG_KT_DET_FG, SXG_KT_DET_UNION,
+    SXG_KT_DET_FLATTEN, SXG_KT_DET_STAT, SXG_KT_DET_SORT, SXG_KT_DEB_THRESH,
+    SXG_KT_DEB_LEVELS, SXG_KT_DEB_OBJECTS, SXG_KT_ANALYSE, SXG_KT_CLEAN, SXG_KT_SEG,
+    SXG_NKT
+};
+void sxg_kernel_timing(int enable);
+int sxg_kernel_times(int id, double *total_ms, int *launches);
+
 /* Catalogue row, layout of the reference's outobjstruct (src/types.h:417-455, NAPER 1). */
 typedef struct __attribute__((aligned(16))) {
     int number;

@@ -10,21 +10,23 @@
  * the reference's EXT_MINAREA = 5, cudatypes.h:62), the 8 isophotal areas and the FWHM
  * log fit; the SATUR flag of endobject (:2873-2874).
  *
- * Layout: one 64-lane wavefront per raw object; lane j owns object j of that raw object
- * and walks the raw pixel list in raster order, taking the pixels whose object index is
- * its own, so every sum is the sequential raster-order sum of the reference's loops.
+ * Layout: one 64-lane wavefront per object over its contiguous, raster-ordered range of
+ * the object pixel list (opix/objoff, built after deblending).  Lanes compute the
+ * per-pixel terms of a chunk of 64 pixels in parallel; extrema, counts and the
+ * first-maximum position are wavefront reductions; every floating-point sum is folded
+ * in raster order with readlane (sxg_wave.h), so each sum is the reference's sequential
+ * sum bit for bit.
  */
 #include "sxg_internal.h"
 #include "sxg_objects.h"
+#include "sxg_wave.h"
 
-#define AW 64
 #define HEAPN 5
 
 struct AnaArgs {
-    const int *fpia, *off, *cnt;
-    const int *objbase, *rootkept, *alloc;
+    const int *opix, *objoff;
     const float *sub, *cd, *back;
-    int W, H, nbx, nby, backw, backh;
+    int W, H, nbx, nby, backw, backh, nobj;
     float backsig, thresh, dthresh;
     double gain, satur_level;
     int errflag, peakflag, isoflag, fwhmflag, cleanflag;
@@ -55,130 +57,176 @@ __device__ inline float back_at(const float *b, int x, int y, int nx, int ny, in
     return (float)((1 - dy) * (cdx * b0 + dx * b1) + dy * (dx * b2 + cdx * b3));
 }
 
-__device__ void analyse_object(const AnaArgs &A, int off, int n, int obj)
+__global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
 {
+    const int lane = threadIdx.x & 63;
+    const int obj = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (obj >= A.nobj) return;
     const int W = A.W, H = A.H;
     ObjCols o = A.o;
-    /* ---- pass 1: extent, peaks, float flux of the final pixels ---- */
-    float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG, val, cval, rv = 0.0f;
-    int xmin = 2 * SXG_MAXPICSIZE, ymin = 2 * SXG_MAXPICSIZE, xmax = 0, ymax = 0, npx = 0;
-    for (int k = 0; k < n; k++) {
-        const int p = A.fpia[off + k];
-        if (A.alloc[p] != obj) continue;
-        const int y = p / W, x = p - y * W;
-        if (peak < (val = A.sub[p])) peak = val;
-        if (cpeak < (cval = A.cd[p])) cpeak = cval;
-        rv += cval;
-        if (xmin > x) xmin = x;
-        if (xmax < x) xmax = x;
-        if (ymin > y) ymin = y;
-        if (ymax < y) ymax = y;
-        npx++;
-    }
-    o.fdnpix[obj] = npx;
+    const int g0 = A.objoff[obj], npx = A.objoff[obj + 1] - g0;
+    const int *P = A.opix + g0;
     int flag = o.flag[obj];
+    double mxo, myo;
+    float fdflux, oa, ob;
     if (o.level[obj] > 0) {
-        /* ---- preanalyse_robust ---- */
+        /* ---- preanalyse_robust over the final pixels: extent, peaks, float flux ---- */
+        float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG, rv = 0.0f;
+        int xmin = 2 * SXG_MAXPICSIZE, ymin = 2 * SXG_MAXPICSIZE, xmax = 0, ymax = 0;
+        for (int base = 0; base < npx; base += 64) {
+            const int k = base + lane;
+            const bool in = k < npx;
+            float cval = 0.0f;
+            if (in) {
+                const int p = P[k];
+                const int y = p / W, x = p - y * W;
+                peak = fmaxf(peak, A.sub[p]);
+                cval = A.cd[p];
+                cpeak = fmaxf(cpeak, cval);
+                xmin = min(xmin, x);
+                xmax = max(xmax, x);
+                ymin = min(ymin, y);
+                ymax = max(ymax, y);
+            }
+            const int nin = min(64, npx - base);
+            for (int j = 0; j < nin; j++) rv += sxw_rl(cval, j);
+        }
+        peak = sxw_max(peak);
+        cpeak = sxw_max(cpeak);
+        xmin = sxw_min(xmin);
+        xmax = sxw_max(xmax);
+        ymin = sxw_min(ymin);
+        ymax = sxw_max(ymax);
         if (xmin == 0 || xmax == W - 1 || ymin == 0 || ymax == H - 1) flag |= SXG_OBJ_TRUNC;
-        o.fdflux[obj] = rv;
-        o.fdpeak[obj] = cpeak;
-        o.dpeak[obj] = peak;
-        o.xmin[obj] = xmin; o.xmax[obj] = xmax; o.ymin[obj] = ymin; o.ymax[obj] = ymax;
         const double dthresh = (double)A.dthresh;
         const double dthresh2 = (dthresh + peak) / 2.0;
         double mx = 0.0, my = 0.0, tv = 0.0, mx2 = 0.0, my2 = 0.0, mxy = 0.0;
         int area2 = 0, dnpix = 0;
-        for (int k = 0; k < n; k++) {
-            const int p = A.fpia[off + k];
-            if (A.alloc[p] != obj) continue;
-            const int y = p / W - ymin, x = p - (p / W) * W - xmin;
-            cval = A.cd[p];
-            tv += (val = A.sub[p]);
-            if (val > dthresh) dnpix++;
-            if (val > dthresh2) area2++;
-            mx += cval * x;
-            my += cval * y;
-            mx2 += cval * x * x;
-            my2 += cval * y * y;
-            mxy += cval * x * y;
+        for (int base = 0; base < npx; base += 64) {
+            const int k = base + lane;
+            float val = 0.0f, tx = 0.0f, ty = 0.0f, txx = 0.0f, tyy = 0.0f, txy = 0.0f;
+            if (k < npx) {
+                const int p = P[k];
+                const int y = p / W - ymin, x = p - (p / W) * W - xmin;
+                const float cval = A.cd[p];
+                val = A.sub[p];
+                if (val > dthresh) dnpix++;
+                if (val > dthresh2) area2++;
+                tx = cval * x;
+                ty = cval * y;
+                txx = cval * x * x;
+                tyy = cval * y * y;
+                txy = cval * x * y;
+            }
+            const int nin = min(64, npx - base);
+            for (int j = 0; j < nin; j++) {
+                tv += sxw_rl(val, j);
+                mx += sxw_rl(tx, j);
+                my += sxw_rl(ty, j);
+                mx2 += sxw_rl(txx, j);
+                my2 += sxw_rl(tyy, j);
+                mxy += sxw_rl(txy, j);
+            }
         }
+        area2 = sxw_sum(area2);
+        dnpix = sxw_sum(dnpix);
         Moments m;
         sxg_moments(mx, my, mx2, my2, mxy, rv, (flag & SXG_OBJ_MERGED) != 0, o.mx[obj], o.my[obj],
                     xmin, ymin, area2, dnpix, dthresh, dthresh2, m);
-        sxg_store_moments(o, obj, m);
-        o.dnpix[obj] = dnpix;
-        o.dflux[obj] = (float)tv;
+        mxo = m.mx;
+        myo = m.my;
+        fdflux = rv;
+        oa = m.a;
+        ob = m.b;
+        if (lane == 0) {
+            o.fdflux[obj] = rv;
+            o.fdpeak[obj] = cpeak;
+            o.dpeak[obj] = peak;
+            o.xmin[obj] = xmin; o.xmax[obj] = xmax; o.ymin[obj] = ymin; o.ymax[obj] = ymax;
+            sxg_store_moments(o, obj, m);
+            o.dnpix[obj] = dnpix;
+            o.dflux[obj] = (float)tv;
+        }
+    } else {
+        mxo = o.mx[obj];
+        myo = o.my[obj];
+        fdflux = o.fdflux[obj];
+        oa = o.a[obj];
+        ob = o.b[obj];
     }
-    o.dthresh[obj] = A.dthresh;   /* the detection threshold from here on */
-    const double mxo = o.mx[obj], myo = o.my[obj];
-    const float fdflux = o.fdflux[obj];
     /* ---- examineiso ---- */
-    const float bkg = back_at(A.back, (int)(mxo + 0.5), (int)(myo + 0.5), A.nbx, A.nby, A.backw, A.backh);
+    const float bkg = back_at(A.back, (int)(mxo + 0.5), (int)(myo + 0.5), A.nbx, A.nby, A.backw,
+                              A.backh);
     const float dbkg = 0.0f;
-    double tv = 0.0, sigtv = 0.0, esum = 0.0, emx2 = 0.0, emy2 = 0.0, emxy = 0.0, err;
+    double tv = 0.0, sigtv = 0.0, esum = 0.0, emx2 = 0.0, emy2 = 0.0, emxy = 0.0;
     const double var = (double)(A.backsig * A.backsig), dbacknoise2 = var;
     const double xm = A.errflag ? mxo : 0.0, ym = A.errflag ? myo : 0.0;
-    float pk = (float)-SXG_BIG, cdpeak = (float)-SXG_BIG, pix, cdpix;
-    int area = 0, peakx = 0, peaky = 0;
-    int cleanflag = A.cleanflag && npx >= HEAPN;
-    float top[HEAPN];
-    int nfill = 0;
-    for (int k = 0; k < n; k++) {
-        const int p = A.fpia[off + k];
-        if (A.alloc[p] != obj) continue;
-        double x = p % W, y = p / W;
-        pix = A.sub[p];
-        if (pix > pk) pk = pix;
-        cdpix = A.cd[p];
-        if (A.peakflag && cdpix > cdpeak) {
-            cdpeak = cdpix;
-            peakx = (int)x + 1;
-            peaky = (int)y + 1;
-        }
-        sigtv += var;
-        if (pix > A.thresh) area++;
-        tv += pix;
-        if (A.errflag) {
+    float pk = (float)-SXG_BIG, cdpeak = (float)-SXG_BIG;
+    int area = 0, peakp = 0x7fffffff;
+    const bool cleanflag = A.cleanflag && npx >= HEAPN;
+    SxwTop5 top;
+    top.init();
+    for (int base = 0; base < npx; base += 64) {
+        const int k = base + lane;
+        float pix = 0.0f;
+        double err = 0.0, ex2 = 0.0, ey2 = 0.0, exy = 0.0;
+        if (k < npx) {
+            const int p = P[k];
+            double x = p % W, y = p / W;
+            pix = A.sub[p];
+            pk = fmaxf(pk, pix);
+            const float cdpix = A.cd[p];
+            if (A.peakflag && cdpix > cdpeak) {
+                cdpeak = cdpix;
+                peakp = p;
+            }
+            if (pix > A.thresh) area++;
             err = dbacknoise2;
             if (A.gain > 0.0 && cdpix > 0.0) err += cdpix / A.gain;
             x = x - xm;
             y = y - ym;
-            esum += err;
-            emx2 += err * x * x;
-            emy2 += err * y * y;
-            emxy += err * x * y;
+            ex2 = err * x * x;
+            ey2 = err * y * y;
+            exy = err * x * y;
+            if (cleanflag) top.add(cdpix - A.dthresh);
         }
-        if (cleanflag) {
-            const float tpix = cdpix - A.dthresh;
-            if (nfill < HEAPN) {
-                int i = nfill++;
-                while (i > 0 && top[i - 1] > tpix) { top[i] = top[i - 1]; i--; }
-                top[i] = tpix;
-            } else if (tpix > top[0]) {
-                int i = 0;
-                while (i + 1 < HEAPN && top[i + 1] < tpix) { top[i] = top[i + 1]; i++; }
-                top[i] = tpix;
+        const int nin = min(64, npx - base);
+        if (A.errflag) {
+            for (int j = 0; j < nin; j++) {
+                sigtv += var;
+                tv += sxw_rl(pix, j);
+                esum += sxw_rl(err, j);
+                emx2 += sxw_rl(ex2, j);
+                emy2 += sxw_rl(ey2, j);
+                emxy += sxw_rl(exy, j);
+            }
+        } else {
+            for (int j = 0; j < nin; j++) {
+                sigtv += var;
+                tv += sxw_rl(pix, j);
             }
         }
     }
-    o.mthresh[obj] = cleanflag ? top[0] : 0.0f;
+    pk = sxw_max(pk);
+    area = sxw_sum(area);
+    int peakx = 0, peaky = 0;
+    if (A.peakflag) {
+        const float cmax = sxw_max(cdpeak);
+        const int pp = sxw_min(cdpeak == cmax ? peakp : 0x7fffffff);
+        peakx = pp % W + 1;
+        peaky = pp / W + 1;
+    }
+    const float mthresh = cleanflag ? top.fifth() : 0.0f;
+    double pmx2 = 0.0, pmy2 = 0.0, pmxy = 0.0;
     if (A.errflag) {
         const double flux2 = (float)(fdflux * fdflux);
-        o.poserr_mx2[obj] = emx2 / flux2;
-        o.poserr_my2[obj] = emy2 / flux2;
-        o.poserr_mxy[obj] = emxy / flux2;
+        pmx2 = emx2 / flux2;
+        pmy2 = emy2 / flux2;
+        pmxy = emxy / flux2;
     }
     tv -= npx * dbkg;
     if (A.gain > 0.0 && tv > 0.0) sigtv += tv / A.gain;
-    o.npix[obj] = area;
-    o.flux[obj] = (float)tv;
-    o.fluxerr[obj] = (float)sigtv;
     const float thr_o = A.thresh - dbkg, pk_o = pk - dbkg;
-    o.thresh[obj] = thr_o;
-    o.peak[obj] = pk_o;
-    o.peakx[obj] = peakx;
-    o.peaky[obj] = peaky;
-    o.bkg[obj] = bkg;
     /* ---- isophotal areas and FWHM ---- */
     float threshs[SXG_NISO];
     if (pk_o > 0.0 && thr_o > 0.0)
@@ -191,46 +239,78 @@ __device__ void analyse_object(const AnaArgs &A, int off, int n, int obj)
     if (thresh0 < thr_o) thresh0 = thr_o;
     const double sat = (double)(A.satur_level - dbkg);
     double s = 0.0, sx = 0.0, sy = 0.0, sxx = 0.0, sxy = 0.0;
-    for (int k = 0; k < n; k++) {
-        const int p = A.fpia[off + k];
-        if (A.alloc[p] != obj) continue;
-        const float v = A.sub[p];
-        for (int i = 0; i < SXG_NISO && v > threshs[i]; i++) iso[i]++;
-        const double dpix = v - (double)dbkg;
-        if (thresh0 > 0.0 && dpix > thresh0 && dpix < sat) {
-            const double dx = (p % W) - mxo, dy = (p / W) - myo;
-            const double lpix = sx_log(dpix);
-            const double inverr2 = dpix * dpix;
-            s += inverr2;
-            const double d2 = dx * dx + dy * dy;
-            sx += d2 * inverr2;
-            sxx += d2 * d2 * inverr2;
-            sy += lpix * inverr2;
-            sxy += lpix * d2 * inverr2;
+    for (int base = 0; base < npx; base += 64) {
+        const int k = base + lane;
+        bool use = false;
+        double ts = 0.0, tsx = 0.0, tsxx = 0.0, tsy = 0.0, tsxy = 0.0;
+        if (k < npx) {
+            const int p = P[k];
+            const float v = A.sub[p];
+            bool go = true;
+#pragma unroll
+            for (int i = 0; i < SXG_NISO; i++) {
+                go = go && v > threshs[i];
+                iso[i] += go;
+            }
+            const double dpix = v - (double)dbkg;
+            if (thresh0 > 0.0 && dpix > thresh0 && dpix < sat) {
+                use = true;
+                const double dx = (p % W) - mxo, dy = (p / W) - myo;
+                const double lpix = sx_log(dpix);
+                const double inverr2 = dpix * dpix;
+                const double d2 = dx * dx + dy * dy;
+                ts = inverr2;
+                tsx = d2 * inverr2;
+                tsxx = d2 * d2 * inverr2;
+                tsy = lpix * inverr2;
+                tsxy = lpix * d2 * inverr2;
+            }
+        }
+        unsigned long long b = __ballot(use);
+        while (b) {
+            const int j = __builtin_ctzll(b);
+            b &= b - 1;
+            s += sxw_rl(ts, j);
+            sx += sxw_rl(tsx, j);
+            sxx += sxw_rl(tsxx, j);
+            sy += sxw_rl(tsy, j);
+            sxy += sxw_rl(tsxy, j);
         }
     }
-    if (A.isoflag)
-        for (int i = 0; i < SXG_NISO; i++) o.iso[(size_t)obj * SXG_NISO + i] = iso[i];
+#pragma unroll
+    for (int i = 0; i < SXG_NISO; i++) iso[i] = sxw_sum(iso[i]);
     float fwhm = 0.0f;
     if (A.fwhmflag && thresh0 > 0.0) {
         const double d = s * sxx - sx * sx;
         if (fabs(d) > 0.0) {
             double bb = -(s * sxy - sx * sy) / d, bmax;
-            if (bb < (bmax = 1 / (13 * o.a[obj] * o.b[obj]))) bb = bmax;
+            if (bb < (bmax = 1 / (13 * oa * ob))) bb = bmax;
             fwhm = (float)(1.6651 / sqrt(bb));
             if (fwhm > 0.0) fwhm -= 1 / (4 * fwhm);
         }
     }
-    o.fwhm[obj] = fwhm;
     if (pk_o + bkg >= A.satur_level) flag |= SXG_OBJ_SATUR;
+    if (lane != 0) return;
+    o.fdnpix[obj] = npx;
+    o.dthresh[obj] = A.dthresh;   /* the detection threshold from here on */
+    o.mthresh[obj] = mthresh;
+    if (A.errflag) {
+        o.poserr_mx2[obj] = pmx2;
+        o.poserr_my2[obj] = pmy2;
+        o.poserr_mxy[obj] = pmxy;
+    }
+    o.npix[obj] = area;
+    o.flux[obj] = (float)tv;
+    o.fluxerr[obj] = (float)sigtv;
+    o.thresh[obj] = thr_o;
+    o.peak[obj] = pk_o;
+    o.peakx[obj] = peakx;
+    o.peaky[obj] = peaky;
+    o.bkg[obj] = bkg;
+    if (A.isoflag)
+        for (int i = 0; i < SXG_NISO; i++) o.iso[(size_t)obj * SXG_NISO + i] = iso[i];
+    o.fwhm[obj] = fwhm;
     o.flag[obj] = flag;
-}
-
-__global__ void __launch_bounds__(AW) k_analyse(AnaArgs A)
-{
-    const int r = blockIdx.x;
-    const int n = A.cnt[r], off = A.off[r], base = A.objbase[r], nk = A.rootkept[r];
-    for (int j = threadIdx.x; j < nk; j += AW) analyse_object(A, off, n, base + j);
 }
 
 extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int flagobj_iso_0,
@@ -260,13 +340,12 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
         return;
     }
     if (S.nraw == 0 || numobj == 0) return;
-    RawObj ro;
-    if (!sxg_raw_alloc(S.nraw, ro)) return;
+    if (!S.opix || !S.objoff) {
+        sxg_set_error("analyse_gpu: no object pixel list (run parcel_out first)");
+        return;
+    }
     AnaArgs A;
-    A.fpia = S.fpia; A.off = ro.off; A.cnt = ro.count;
-    A.objbase = sxg_take<int>("deb.objbase", S.nraw);
-    A.rootkept = sxg_take<int>("deb.rootkept", S.nraw);
-    A.alloc = sxg_take<int>("deb.alloc", S.N);
+    A.opix = S.opix; A.objoff = S.objoff; A.nobj = numobj;
     A.sub = S.img.as<float>(); A.cd = S.cd.as<float>(); A.back = S.maps.as<float>();
     A.W = S.W; A.H = S.H; A.nbx = nbackx; A.nby = nbacky; A.backw = backw; A.backh = backh;
     A.backsig = backsig; A.thresh = thresh; A.dthresh = dthresh;
@@ -281,7 +360,7 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
     S.clean_enabled = prefs_clean_flag != 0;
     sxg_stage_begin(ST_ANALYSE);
     sxg_kt_begin(SXG_KT_ANALYSE);
-    hipLaunchKernelGGL(k_analyse, dim3(S.nraw), dim3(AW), 0, S.stream, A);
+    hipLaunchKernelGGL(k_analyse, dim3(sxg_div_up(numobj, 4)), dim3(256), 0, S.stream, A);
     sxg_kt_end(SXG_KT_ANALYSE);
     sxg_stage_end(ST_ANALYSE);
     SXG_HIP(hipGetLastError());

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg)
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= nfg) return;
     rs.cnt[k] = 0;
-    rs.peak[k] = 0;
+    rs.peak[k] = (int)0x80000000;   /* below f2ord of every float */
     rs.xmin[k] = 0x7fffffff;
     rs.xmax[k] = -1;
     rs.ymin[k] = 0x7fffffff;
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
     const int rk = rootpos[lab[p]];
     const int y = p / W, x = p - y * W;
     atomicAdd(&rs.cnt[rk], 1);
-    atomicMax(&rs.peak[rk], f2ord(cd[p]));   /* detected values are > 0 */
+    atomicMax(&rs.peak[rk], f2ord(cd[p]));
     atomicMin(&rs.xmin[rk], x);
     atomicMax(&rs.xmax[rk], x);
     atomicMin(&rs.ymin[rk], y);
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ raw
     rawid_of_pos[k] = i;
     ro.first[i] = fg[k];
     ro.count[i] = rs.cnt[k];
-    ro.fdpeak[i] = __int_as_float(rs.peak[k]);
+    ro.fdpeak[i] = ord2f(rs.peak[k]);
     ro.xmin[i] = rs.xmin[k];
     ro.xmax[i] = rs.xmax[k];
     ro.ymin[i] = rs.ymin[k];

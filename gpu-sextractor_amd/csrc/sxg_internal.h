@@ -102,6 +102,8 @@ struct SxgState {
     DevBuf comps;    /* component table */
     DevBuf objs;     /* object columns (SoA) */
     DevBuf rootobj;  /* per root: first object, object count */
+    int *opix = nullptr;      /* int[P0]: raw pixels grouped by object, raster order inside */
+    int *objoff = nullptr;    /* int[nobj + 1]: start of each object in opix */
     int nsurv = 0;
     bool cleaned = false;
     bool clean_enabled = true;
@@ -119,6 +121,7 @@ bool sxg_raw_alloc(int nraw, RawObj &ro);   /* (re)binds the named buffers */
 
 /* kernel timers (HIP events on the library stream, enabled by sxg_kernel_timing) */
 
+long sxg_option(int id);
 void sxg_kt_begin(int id);
 void sxg_kt_end(int id);
 bool sxg_build_segmap();   /* device segmentation map ("out.map") and survivor count */
@@ -151,6 +154,12 @@ template <class T> inline T *sxg_take(const char *name, size_t n)
     return b.as<T>();
 }
 /* FP32 bits of non-negative floats order like the integers. */
-__device__ inline int f2ord(float v) { return __float_as_int(v); }
+/* order-preserving map float -> int (monotone over all non-NaN floats) and its inverse */
+__device__ inline int f2ord(float v)
+{
+    const int i = __float_as_int(v);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ inline float ord2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
 
 static inline int sxg_div_up(long a, long b) { return (int)((a + b - 1) / b); }

@@ -23,6 +23,21 @@ bool sxg_has_error() { return g_has_err; }
 
 extern "C" const char *sxg_last_error(void) { return g_has_err ? g_err.c_str() : nullptr; }
 
+/* ------------------------------------------------------------------ options ---- */
+static long g_opt[SXG_NOPT] = {1024, 6144};
+static const long g_opt_max[SXG_NOPT] = {6400, 6400};
+
+long sxg_option(int id) { return g_opt[id]; }
+
+extern "C" int sxg_set_option(int id, long value)
+{
+    if (id < 0 || id >= SXG_NOPT || value < 0 || value > g_opt_max[id]) return -1;
+    g_opt[id] = value;
+    return 0;
+}
+
+extern "C" long sxg_get_option(int id) { return id >= 0 && id < SXG_NOPT ? g_opt[id] : -1; }
+
 /* ------------------------------------------------------------ workspace ---- */
 bool DevBuf::ensure(size_t want)
 {

@@ -1,0 +1,87 @@
+/*
+ * sxg_wave.h -- 64-lane wavefront helpers shared by the per-object kernels.
+ *
+ * The reference accumulates every per-object sum in one sequential loop over the
+ * object's pixels in raster order.  A wavefront reproduces those sums bit for bit by
+ * letting each lane compute the terms of one pixel (in parallel) and then adding the
+ * terms of the member lanes in lane order (the raster order of a chunk of 64 list
+ * entries) with readlane: the additions are the only sequential part.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+__device__ inline unsigned long long sxw_lanes_below(int lane)
+{
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+/* value of v in lane j (j uniform across the wavefront) */
+__device__ inline float sxw_rl(float v, int j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ inline double sxw_rl(double v, int j)
+{
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)u, j);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), j);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ inline float sxw_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline int sxw_max(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline int sxw_min(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline int sxw_sum(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+/* keep the five largest values seen by this lane, t[0] >= ... >= t[4] (-inf = empty) */
+struct SxwTop5 {
+    float t[5];
+    __device__ void init()
+    {
+#pragma unroll
+        for (int i = 0; i < 5; i++) t[i] = -INFINITY;
+    }
+    __device__ void add(float v)
+    {
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const float hi = fmaxf(t[i], v), lo = fminf(t[i], v);
+            t[i] = hi;
+            v = lo;
+        }
+    }
+    /* the fifth largest value over the wavefront (needs >= 5 values in total) */
+    __device__ float fifth()
+    {
+        const int lane = threadIdx.x & 63;
+        float res = 0.0f;
+#pragma unroll
+        for (int round = 0; round < 5; round++) {
+            const float m = sxw_max(t[0]);
+            const unsigned long long b = __ballot(t[0] == m);
+            if (lane == __builtin_ctzll(b)) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) t[i] = t[i + 1];
+                t[4] = -INFINITY;
+            }
+            res = m;
+        }
+        return res;
+    }
+};

@@ -239,6 +239,9 @@ def gpu():
         L.sxg_kernel_timing.argtypes = [i]
         L.sxg_kernel_times.argtypes = [i, C.POINTER(d), C.POINTER(i)]
         L.sxg_set_mag_zeropoint.argtypes = [d]
+        L.sxg_set_option.argtypes = [i, C.c_long]
+        L.sxg_get_option.argtypes = [i]
+        L.sxg_get_option.restype = C.c_long
         _gpu = L
     return _gpu
 
@@ -254,12 +257,24 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "endobject_gpu", "clear_detection", "clear_device", "sxg_last_error",
             "sxg_set_back_value", "sxg_get_segmap", "sxg_get_column", "sxg_get_frame_info",
             "sxg_get_back_maps", "sxg_get_filtered", "sxg_get_alloc_map", "sxg_device_frame",
-            "sxg_stage_times", "sxg_release", "sxg_set_mag_zeropoint", "sxg_params",
+            "sxg_stage_times", "sxg_release", "sxg_set_mag_zeropoint",
             "sxg_set_resident_frame", "sxg_process_resident", "sxg_sync", "sxg_kernel_timing",
-            "sxg_kernel_times"]
+            "sxg_kernel_times", "sxg_set_option", "sxg_get_option"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
-           "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg"]
+           "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort"]
+
+# tuning options (sxg_option_id): they choose kernel variants, never change results
+OPTIONS = {"deb_lds_small": 0, "deb_lds_large": 1}
+
+
+def set_option(name, value):
+    if gpu().sxg_set_option(OPTIONS[name], int(value)) != 0:
+        raise ValueError("option %s: value %r out of range" % (name, value))
+
+
+def get_option(name):
+    return gpu().sxg_get_option(OPTIONS[name])
 
 COLUMNS = ["level", "root", "firstpix", "flag", "fdnpix", "dnpix", "npix", "xmin", "xmax",
            "ymin", "ymax", "peakx", "peaky", "iso", "mx", "my", "mx2", "my2", "mxy",

@@ -190,10 +190,21 @@ enum sxg_kernel_id {
     SXG_KT_SUBBACK, SXG_KT_CONV, SXG_KT_DET_INIT, SXG_KT_DET_FG, SXG_KT_DET_UNION,
     SXG_KT_DET_FLATTEN, SXG_KT_DET_STAT, SXG_KT_DET_SORT, SXG_KT_DEB_THRESH,
     SXG_KT_DEB_LEVELS, SXG_KT_DEB_OBJECTS, SXG_KT_ANALYSE, SXG_KT_CLEAN, SXG_KT_SEG,
-    SXG_NKT
+    SXG_KT_DEB_LMAX, SXG_KT_OBJ_SORT, SXG_NKT
 };
 void sxg_kernel_timing(int enable);
 int sxg_kernel_times(int id, double *total_ms, int *launches);
+
+/* Tuning options (no counterpart in the reference).  Results never depend on them:
+   they only choose which kernel variant handles which work.
+     SXG_OPT_DEB_LDS_SMALL  raw objects up to this many pixels are deblended by 256-lane
+                            workgroups in LDS (default 1024)
+     SXG_OPT_DEB_LDS_LARGE  up to this many by 1024-lane workgroups in LDS (default 6144,
+                            at most 6400); larger ones use global scratch memory */
+enum sxg_option_id { SXG_OPT_DEB_LDS_SMALL = 0, SXG_OPT_DEB_LDS_LARGE, SXG_NOPT };
+/* Returns 0, or -1 for an unknown option or a value out of range. */
+int sxg_set_option(int id, long value);
+long sxg_get_option(int id);
 
 /* Catalogue row, layout of the reference's outobjstruct (src/types.h:417-455, NAPER 1). */
 typedef struct __attribute__((aligned(16))) {

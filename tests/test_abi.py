@@ -1,0 +1,87 @@
+"""The C-ABI boundary, checked without a GPU: libsexgpu.so loads, exports every function
+include/sexgpu.h declares (the reference's 13 entry points and the additions), and the
+Python binding's list matches the header."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import sexgpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "sexgpu.h")
+LIB = os.path.join(REPO, "gpu-sextractor_amd", "lib", "libsexgpu.so")
+
+# the reference's entry points (src/cuda/*.h), in call order
+REFERENCE_ABI = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run_detection",
+                 "init_deblend", "parcel_out", "clear_deblend", "analyse_gpu", "run_clean",
+                 "endobject_gpu", "clear_detection", "clear_device"]
+
+
+def header_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"//[^\n]*", "", src)
+    src = re.sub(r"^\s*#[^\n]*", "", src, flags=re.M)
+    names = []
+    for stmt in src.split(";"):
+        stmt = stmt.strip()
+        if "(" not in stmt or "typedef" in stmt or "{" in stmt.split("(")[0]:
+            # a declaration may follow the closing brace of a typedef or enum
+            stmt = stmt.split("}")[-1].strip()
+            if "(" not in stmt or "typedef" in stmt:
+                continue
+        m = re.search(r"(\w+)\s*\(", stmt)
+        if m and not stmt.startswith("return"):
+            names.append(m.group(1))
+    return names
+
+
+def exported_symbols(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], check=True,
+                         capture_output=True, text=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_header_declares_reference_abi():
+    names = header_functions()
+    for f in REFERENCE_ABI:
+        assert f in names, f
+    assert len(names) == len(set(names))
+
+
+def test_library_exports_every_declared_function():
+    assert os.path.exists(LIB), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    syms = exported_symbols(LIB)
+    missing = [f for f in header_functions() if f not in syms]
+    assert not missing, missing
+
+
+def test_library_loads_and_binding_matches_header():
+    lib = ctypes.CDLL(LIB)
+    declared = set(header_functions())
+    for f in declared:
+        assert hasattr(lib, f), f
+    assert set(sexgpu.EXPORTED) == declared
+
+
+def test_no_error_before_any_call():
+    lib = ctypes.CDLL(LIB)
+    lib.sxg_last_error.restype = ctypes.c_char_p
+    assert lib.sxg_last_error() is None
+
+
+def test_options_without_device():
+    lib = sexgpu.gpu()
+    assert lib.sxg_get_option(sexgpu.OPTIONS["deb_lds_small"]) >= 0
+    assert lib.sxg_set_option(99, 1) == -1
+    assert lib.sxg_get_option(99) == -1
+    with pytest.raises(ValueError):
+        sexgpu.set_option("deb_lds_large", 100000)
+
+
+def test_outobj_layout():
+    # the reference's outobjstruct with NAPER 1: 160 bytes, 16-byte aligned
+    assert ctypes.sizeof(sexgpu.OutObj) == 160

@@ -106,3 +106,28 @@ def test_catalog_rows_match_columns(frame1024, masks):
     assert np.array_equal(sposx, (g["mx"] + 1.0).astype(np.float32))
     fe = np.array([rows[i].fluxerr_iso for i in range(n)], np.float32)
     assert np.array_equal(fe, np.sqrt(g["fluxerr"].astype(np.float64)).astype(np.float32))
+
+
+@pytest.mark.parametrize("caps", [(0, 0), (64, 256), (6400, 6400), (1024, 6144)])
+def test_deblend_kernel_variants(masks, caps):
+    """Every deblending kernel variant (LDS 256-lane, LDS 1024-lane, global scratch)
+    gives the oracle's result: the size caps move raw objects between them."""
+    img = sexgpu.synth("crowded4096", width=1024, height=1024, nsrc=167000 // 16, seed=1201)
+    m = masks["gauss_3p0_7x7"]
+    old = (sexgpu.get_option("deb_lds_small"), sexgpu.get_option("deb_lds_large"))
+    try:
+        sexgpu.set_option("deb_lds_small", caps[0])
+        sexgpu.set_option("deb_lds_large", caps[1])
+        g = sexgpu.run_frame(img, _params(m))
+    finally:
+        sexgpu.set_option("deb_lds_small", old[0])
+        sexgpu.set_option("deb_lds_large", old[1])
+    compare(g, _oracle(img, m))
+    assert (g["level"] > 0).sum() > 20
+
+
+def test_option_range():
+    with pytest.raises(ValueError):
+        sexgpu.set_option("deb_lds_large", 7000)
+    with pytest.raises(ValueError):
+        sexgpu.set_option("deb_lds_small", -1)

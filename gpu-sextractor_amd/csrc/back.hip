@@ -5,7 +5,8 @@
  * Kernels:
  *   k_meshstat   one lane per mesh: backstat (back.c:214-290).  The double sums of pix
  *                and pix^2 are sequential in raster order inside the mesh, as back.c
- *                does, so each lane runs the chain; float4 loads cut the load count.
+ *                does, so each lane runs the chain; float4 loads cut the load count,
+ *                and 16-lane workgroups put the chains on every CU.
  *   k_meshmode   one workgroup per mesh: histogram in LDS (back.c:297-341), then the
  *                clipped-mode iteration of backguess (back.c:347-426).  Inside the
  *                iteration every sum is an exact integer (< 2^53), so the sequential
@@ -39,10 +40,11 @@ struct MeshStat {
 };
 
 /* ---------------------------------------------------------------- backstat ---- */
-__global__ void __launch_bounds__(64) k_meshstat(const float *__restrict__ img, int W, int H,
-                                                 int bs, int nbx, int nmesh, MeshStat *ms)
+#define MS_LANES 16   /* meshes per workgroup: spreads the chains over every CU */
+__global__ void __launch_bounds__(MS_LANES) k_meshstat(const float *__restrict__ img, int W, int H,
+                                                       int bs, int nbx, int nmesh, MeshStat *ms)
 {
-    const int m = blockIdx.x * 64 + threadIdx.x;
+    const int m = blockIdx.x * MS_LANES + threadIdx.x;
     if (m >= nmesh) return;
     const int mx = m % nbx, my = m / nbx;
     const int x0 = mx * bs, y0 = my * bs;
@@ -595,7 +597,7 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
 
     sxg_stage_begin(ST_BACK);
     sxg_kt_begin(SXG_KT_MESHSTAT);
-    hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nmesh, 64)), dim3(64), 0, S.stream, img, W, H,
+    hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nmesh, MS_LANES)), dim3(MS_LANES), 0, S.stream, img, W, H,
                        bs, nbx, nmesh, ms);
     sxg_kt_end(SXG_KT_MESHSTAT);
     sxg_kt_begin(SXG_KT_MESHMODE);

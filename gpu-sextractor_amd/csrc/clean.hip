@@ -11,8 +11,9 @@
  * exceeds the fainter's mthresh; its master is the smallest such brighter index.
  * Victims are dropped (the reference disabled merging, :311-368).
  *
- * Layout: one lane per object i scans its window of following objects; masters are
- * combined with atomicMin, so the result does not depend on scheduling.
+ * Layout: one lane per object i scans its window of following objects, staged through
+ * LDS per workgroup of 256 objects; masters are combined with atomicMin, so the result
+ * does not depend on scheduling.
  */
 #include <cmath>
 #include <cstring>
@@ -34,37 +35,66 @@ __global__ void __launch_bounds__(256) k_clean_prep(ObjCols o, int nobj, double 
     master[i] = 0xffffffffu;
 }
 
-__global__ void __launch_bounds__(256) k_clean_pairs(ObjCols o, int nobj, int win,
-                                                     const double *__restrict__ amp,
-                                                     const double *__restrict__ alpha,
-                                                     unsigned *master)
+/* one workgroup per 256 consecutive objects i; the window of following objects is
+   staged through LDS 256 at a time and read by broadcast */
+#define CP_T 256
+__global__ void __launch_bounds__(CP_T) k_clean_pairs(ObjCols o, int nobj, int win,
+                                                      const double *__restrict__ amp,
+                                                      const double *__restrict__ alpha,
+                                                      unsigned *master)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nobj) return;
-    const double mxi = o.mx[i], myi = o.my[i];
-    const float ai = o.a[i], fi = o.fdflux[i], mti = o.mthresh[i];
-    const float cxxi = o.cxx[i], cyyi = o.cyy[i], cxyi = o.cxy[i];
-    const double ampi = amp[i], alphai = alpha[i];
-    int jmax = i + win;
-    if (jmax > nobj - 1) jmax = nobj - 1;
-    for (int j = i + 1; j <= jmax; j++) {
-        const float dx = (float)(mxi - o.mx[j]);
-        const float dy = (float)(myi - o.my[j]);
-        float rlim = ai + o.a[j];
-        rlim *= rlim;
-        if (dx * dx + dy * dy < rlim * CLEAN_ZONE * CLEAN_ZONE) {
-            double val;
-            if (fi < o.fdflux[j]) {
-                val = 1 + alpha[j] * (o.cxx[j] * dx * dx + o.cyy[j] * dy * dy + o.cxy[j] * dx * dy);
-                if (val > 1.0 && ((float)(val < 1e10 ? amp[j] * (1.0 / val) : 0.0) > mti))
-                    atomicMin(&master[i], (unsigned)j);
-            } else {
-                val = 1 + alphai * (cxxi * dx * dx + cyyi * dy * dy + cxyi * dx * dy);
-                if (val > 1.0 && ((float)(val < 1e10 ? ampi * (1.0 / val) : 0.0) > o.mthresh[j]))
-                    atomicMin(&master[j], (unsigned)i);
+    __shared__ double smx[CP_T], smy[CP_T], samp[CP_T], salpha[CP_T];
+    __shared__ float sa[CP_T], sf[CP_T], scxx[CP_T], scyy[CP_T], scxy[CP_T], smt[CP_T];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * CP_T;
+    const int i = i0 + t;
+    const bool live = i < nobj;
+    const int ii = live ? i : nobj - 1;
+    const double mxi = o.mx[ii], myi = o.my[ii];
+    const float ai = o.a[ii], fi = o.fdflux[ii], mti = o.mthresh[ii];
+    const float cxxi = o.cxx[ii], cyyi = o.cyy[ii], cxyi = o.cxy[ii];
+    const double ampi = amp[ii], alphai = alpha[ii];
+    const int jmax = live ? min(i + win, nobj - 1) : -1;
+    unsigned mine = 0xffffffffu;
+    const int jlast = min(i0 + CP_T - 1 + win, nobj - 1);
+    for (int j0 = i0 + 1; j0 <= jlast; j0 += CP_T) {
+        __syncthreads();
+        const int jl = j0 + t;
+        if (jl <= jlast) {
+            smx[t] = o.mx[jl];
+            smy[t] = o.my[jl];
+            samp[t] = amp[jl];
+            salpha[t] = alpha[jl];
+            sa[t] = o.a[jl];
+            sf[t] = o.fdflux[jl];
+            scxx[t] = o.cxx[jl];
+            scyy[t] = o.cyy[jl];
+            scxy[t] = o.cxy[jl];
+            smt[t] = o.mthresh[jl];
+        }
+        __syncthreads();
+        const int jb = max(j0, i + 1), je = min(j0 + CP_T - 1, jmax);
+        for (int j = jb; j <= je; j++) {
+            const int q = j - j0;
+            const float dx = (float)(mxi - smx[q]);
+            const float dy = (float)(myi - smy[q]);
+            float rlim = ai + sa[q];
+            rlim *= rlim;
+            if (dx * dx + dy * dy < rlim * CLEAN_ZONE * CLEAN_ZONE) {
+                double val;
+                if (fi < sf[q]) {
+                    val = 1 + salpha[q] * (scxx[q] * dx * dx + scyy[q] * dy * dy + scxy[q] * dx * dy);
+                    if (val > 1.0 && ((float)(val < 1e10 ? samp[q] * (1.0 / val) : 0.0) > mti))
+                        mine = min(mine, (unsigned)j);
+                } else {
+                    val = 1 + alphai * (cxxi * dx * dx + cyyi * dy * dy + cxyi * dx * dy);
+                    if (val > 1.0 && ((float)(val < 1e10 ? ampi * (1.0 / val) : 0.0) > smt[q]))
+                        atomicMin(&master[j], (unsigned)i);
+                }
             }
         }
     }
+    if (live && mine != 0xffffffffu) atomicMin(&master[i], mine);
 }
 
 extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacksize, int numobj)

@@ -79,12 +79,18 @@ template <bool G> __device__ inline int ldw(const int *p)
     return *p;
 }
 
+/* root of x with path halving: every parent pointer points to a smaller-or-equal index
+   (links always go from a larger root to a smaller index), so replacing par[x] by its
+   grandparent with atomicMin keeps every pointer on the path to the same or a merged
+   root and keeps the chains short */
 template <bool G> __device__ inline int uf_find(int *par, int x)
 {
-    int y = ldw<G>(&par[x]);
-    while (y != x) {
-        x = y;
-        y = ldw<G>(&par[x]);
+    int p = ldw<G>(&par[x]);
+    while (p != x) {
+        const int gp = ldw<G>(&par[p]);
+        if (gp != p) atomicMin(&par[x], gp);
+        x = p;
+        p = gp;
     }
     return x;
 }
@@ -346,10 +352,16 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         if (!__syncthreads_or(anyl)) break;
         for (int k = tid; k < n; k += NT) {
             if (m[k] < i) continue;
-#pragma unroll
-            for (int d = 0; d < 4; d++) {
-                const NB j = nb[d * nbs + k];
-                if (j != none && m[j] >= i) uf_unite<G>(par, k, (int)j);
+            /* W, NW, N, NE; NW and NE are already joined through N when N is in the
+               set (they are its W and E neighbours), NW also through W */
+            const NB jw = nb[k], jnw = nb[nbs + k], jn = nb[2 * nbs + k], jne = nb[3 * nbs + k];
+            const bool w = jw != none && m[jw] >= i, nn = jn != none && m[jn] >= i;
+            if (w) uf_unite<G>(par, k, (int)jw);
+            if (nn) {
+                uf_unite<G>(par, k, (int)jn);
+            } else {
+                if (!w && jnw != none && m[jnw] >= i) uf_unite<G>(par, k, (int)jnw);
+                if (jne != none && m[jne] >= i) uf_unite<G>(par, k, (int)jne);
             }
         }
         __syncthreads();

@@ -22,6 +22,7 @@
  *                  pixels stay in raster order
  */
 #include "sxg_internal.h"
+#include "sxg_wave.h"
 
 __global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, float t, long n,
                                                   int *par, int *lab)
@@ -32,12 +33,16 @@ __global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, 
     lab[p] = -1;
 }
 
-__device__ inline int uf_root(const int *par, int x)
+/* root of x; path halving with atomicMin keeps the chains short (every pointer goes
+   to a smaller-or-equal index, so the grandparent is a valid replacement) */
+__device__ inline int uf_root(int *par, int x)
 {
-    int y = par[x];
-    while (y != x) {
-        x = y;
-        y = par[x];
+    int p = par[x];
+    while (p != x) {
+        const int gp = par[p];
+        if (gp != p) atomicMin(&par[x], gp);
+        x = p;
+        p = gp;
     }
     return x;
 }
@@ -60,6 +65,8 @@ __device__ inline void uf_unite(int *par, int a, int b)
     }
 }
 
+/* 8-connected union with the backward neighbours; NW and NE are already joined through
+   N when N is set (they are its W and E neighbours), NW also through W */
 __global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, int nfg, int W,
                                                    int *par)
 {
@@ -67,11 +74,15 @@ __global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, i
     if (k >= nfg) return;
     const int p = fg[k];
     const int y = p / W, x = p - y * W;
-    if (x > 0 && par[p - 1] >= 0) uf_unite(par, p, p - 1);
+    const bool w = x > 0 && par[p - 1] >= 0;
+    if (w) uf_unite(par, p, p - 1);
     if (y > 0) {
-        if (x > 0 && par[p - W - 1] >= 0) uf_unite(par, p, p - W - 1);
-        if (par[p - W] >= 0) uf_unite(par, p, p - W);
-        if (x < W - 1 && par[p - W + 1] >= 0) uf_unite(par, p, p - W + 1);
+        if (par[p - W] >= 0) {
+            uf_unite(par, p, p - W);
+        } else {
+            if (!w && x > 0 && par[p - W - 1] >= 0) uf_unite(par, p, p - W - 1);
+            if (x < W - 1 && par[p - W + 1] >= 0) uf_unite(par, p, p - W + 1);
+        }
     }
 }
 
@@ -102,22 +113,45 @@ __global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg)
     rs.ymax[k] = -1;
 }
 
+/* per-root size, peak and bounding box; lanes of a wavefront that share a root (the
+   pixel list is in raster order, so runs are long) combine their values first and one
+   lane per root issues the atomics */
 __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg, int nfg, int W,
                                                       const int *__restrict__ lab,
                                                       const int *__restrict__ rootpos,
                                                       const float *__restrict__ cd, RootStat rs)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nfg) return;
-    const int p = fg[k];
-    const int rk = rootpos[lab[p]];
-    const int y = p / W, x = p - y * W;
-    atomicAdd(&rs.cnt[rk], 1);
-    atomicMax(&rs.peak[rk], f2ord(cd[p]));
-    atomicMin(&rs.xmin[rk], x);
-    atomicMax(&rs.xmax[rk], x);
-    atomicMin(&rs.ymin[rk], y);
-    atomicMax(&rs.ymax[rk], y);
+    const int lane = threadIdx.x & 63;
+    bool act = k < nfg;
+    int rk = -1, x = 0, y = 0, pv = 0;
+    if (act) {
+        const int p = fg[k];
+        rk = rootpos[lab[p]];
+        y = p / W;
+        x = p - y * W;
+        pv = f2ord(cd[p]);
+    }
+    unsigned long long pending = __ballot(act);
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int r = __shfl(rk, leader);
+        const bool mine = act && rk == r;
+        const unsigned long long grp = __ballot(mine);
+        const int pk = sxw_max(mine ? pv : (int)0x80000000);
+        const int x0 = sxw_min(mine ? x : 0x7fffffff), x1 = sxw_max(mine ? x : -1);
+        const int y0 = sxw_min(mine ? y : 0x7fffffff), y1 = sxw_max(mine ? y : -1);
+        if (lane == leader) {
+            atomicAdd(&rs.cnt[r], __popcll(grp));
+            atomicMax(&rs.peak[r], pk);
+            atomicMin(&rs.xmin[r], x0);
+            atomicMax(&rs.xmax[r], x1);
+            atomicMin(&rs.ymin[r], y0);
+            atomicMax(&rs.ymax[r], y1);
+        }
+        if (mine) act = false;
+        pending &= ~grp;
+    }
 }
 
 /* keep[k] = k for kept roots (size >= minarea), -1 otherwise */

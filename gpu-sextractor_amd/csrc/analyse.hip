@@ -13,9 +13,9 @@
  * Layout: one 64-lane wavefront per object over its contiguous, raster-ordered range of
  * the object pixel list (opix/objoff, built after deblending).  Lanes compute the
  * per-pixel terms of a chunk of 64 pixels in parallel; extrema, counts and the
- * first-maximum position are wavefront reductions; every floating-point sum is folded
- * in raster order with readlane (sxg_wave.h), so each sum is the reference's sequential
- * sum bit for bit.
+ * first-maximum position are wavefront reductions; the floating-point sums are folded in
+ * raster order by transposed folds (sxg_wave.h: lane s adds the terms of sum s), so each
+ * sum is the reference's sequential sum bit for bit.
  */
 #include "sxg_internal.h"
 #include "sxg_objects.h"
@@ -59,9 +59,13 @@ __device__ inline float back_at(const float *b, int x, int y, int nx, int ny, in
 
 __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
 {
-    const int lane = threadIdx.x & 63;
-    const int obj = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ double s_rows[4][6 * SXW_ROW];
+    __shared__ float s_frows[4][SXW_ROW];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int obj = blockIdx.x * 4 + wid;
     if (obj >= A.nobj) return;
+    double *rows = s_rows[wid];
+    float *frows = s_frows[wid];
     const int W = A.W, H = A.H;
     ObjCols o = A.o;
     const int g0 = A.objoff[obj], npx = A.objoff[obj + 1] - g0;
@@ -71,7 +75,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     float fdflux, oa, ob;
     if (o.level[obj] > 0) {
         /* ---- preanalyse_robust over the final pixels: extent, peaks, float flux ---- */
-        float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG, rv = 0.0f;
+        float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG, racc = 0.0f;
         int xmin = 2 * SXG_MAXPICSIZE, ymin = 2 * SXG_MAXPICSIZE, xmax = 0, ymax = 0;
         for (int base = 0; base < npx; base += 64) {
             const int k = base + lane;
@@ -88,9 +92,10 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
                 ymin = min(ymin, y);
                 ymax = max(ymax, y);
             }
-            const int nin = min(64, npx - base);
-            for (int j = 0; j < nin; j++) rv += sxw_rl(cval, j);
+            const float tf[1] = {cval};
+            sxw_fold<1>(frows, tf, in, racc);
         }
+        const float rv = sxw_rl(racc, 0);
         peak = sxw_max(peak);
         cpeak = sxw_max(cpeak);
         xmin = sxw_min(xmin);
@@ -100,7 +105,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
         if (xmin == 0 || xmax == W - 1 || ymin == 0 || ymax == H - 1) flag |= SXG_OBJ_TRUNC;
         const double dthresh = (double)A.dthresh;
         const double dthresh2 = (dthresh + peak) / 2.0;
-        double mx = 0.0, my = 0.0, tv = 0.0, mx2 = 0.0, my2 = 0.0, mxy = 0.0;
+        double acc = 0.0;
         int area2 = 0, dnpix = 0;
         for (int base = 0; base < npx; base += 64) {
             const int k = base + lane;
@@ -118,16 +123,11 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
                 tyy = cval * y * y;
                 txy = cval * x * y;
             }
-            const int nin = min(64, npx - base);
-            for (int j = 0; j < nin; j++) {
-                tv += sxw_rl(val, j);
-                mx += sxw_rl(tx, j);
-                my += sxw_rl(ty, j);
-                mx2 += sxw_rl(txx, j);
-                my2 += sxw_rl(tyy, j);
-                mxy += sxw_rl(txy, j);
-            }
+            const double t6[6] = {val, tx, ty, txx, tyy, txy};
+            sxw_fold<6>(rows, t6, k < npx, acc);
         }
+        const double tv = sxw_rl(acc, 0), mx = sxw_rl(acc, 1), my = sxw_rl(acc, 2);
+        const double mx2 = sxw_rl(acc, 3), my2 = sxw_rl(acc, 4), mxy = sxw_rl(acc, 5);
         area2 = sxw_sum(area2);
         dnpix = sxw_sum(dnpix);
         Moments m;
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     const float bkg = back_at(A.back, (int)(mxo + 0.5), (int)(myo + 0.5), A.nbx, A.nby, A.backw,
                               A.backh);
     const float dbkg = 0.0f;
-    double tv = 0.0, sigtv = 0.0, esum = 0.0, emx2 = 0.0, emy2 = 0.0, emxy = 0.0;
+    double eacc = 0.0;
     const double var = (double)(A.backsig * A.backsig), dbacknoise2 = var;
     const double xm = A.errflag ? mxo : 0.0, ym = A.errflag ? myo : 0.0;
     float pk = (float)-SXG_BIG, cdpeak = (float)-SXG_BIG;
@@ -190,23 +190,12 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
             exy = err * x * y;
             if (cleanflag) top.add(cdpix - A.dthresh);
         }
-        const int nin = min(64, npx - base);
-        if (A.errflag) {
-            for (int j = 0; j < nin; j++) {
-                sigtv += var;
-                tv += sxw_rl(pix, j);
-                esum += sxw_rl(err, j);
-                emx2 += sxw_rl(ex2, j);
-                emy2 += sxw_rl(ey2, j);
-                emxy += sxw_rl(exy, j);
-            }
-        } else {
-            for (int j = 0; j < nin; j++) {
-                sigtv += var;
-                tv += sxw_rl(pix, j);
-            }
-        }
+        /* tv, sigtv, esum, emx2, emy2, emxy (the error sums only with errflag) */
+        const double t6[6] = {pix, var, err, ex2, ey2, exy};
+        sxw_fold<6>(rows, t6, k < npx, eacc);
     }
+    double tv = sxw_rl(eacc, 0), sigtv = sxw_rl(eacc, 1);
+    const double emx2 = sxw_rl(eacc, 3), emy2 = sxw_rl(eacc, 4), emxy = sxw_rl(eacc, 5);
     pk = sxw_max(pk);
     area = sxw_sum(area);
     int peakx = 0, peaky = 0;
@@ -238,7 +227,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     float thresh0 = (float)(pk_o / 5.0);
     if (thresh0 < thr_o) thresh0 = thr_o;
     const double sat = (double)(A.satur_level - dbkg);
-    double s = 0.0, sx = 0.0, sy = 0.0, sxx = 0.0, sxy = 0.0;
+    double facc = 0.0;
     for (int base = 0; base < npx; base += 64) {
         const int k = base + lane;
         bool use = false;
@@ -266,17 +255,11 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
                 tsxy = lpix * d2 * inverr2;
             }
         }
-        unsigned long long b = __ballot(use);
-        while (b) {
-            const int j = __builtin_ctzll(b);
-            b &= b - 1;
-            s += sxw_rl(ts, j);
-            sx += sxw_rl(tsx, j);
-            sxx += sxw_rl(tsxx, j);
-            sy += sxw_rl(tsy, j);
-            sxy += sxw_rl(tsxy, j);
-        }
+        const double t5[5] = {ts, tsx, tsxx, tsy, tsxy};
+        sxw_fold<5>(rows, t5, use, facc);
     }
+    const double s = sxw_rl(facc, 0), sx = sxw_rl(facc, 1), sxx = sxw_rl(facc, 2);
+    const double sy = sxw_rl(facc, 3), sxy = sxw_rl(facc, 4);
 #pragma unroll
     for (int i = 0; i < SXG_NISO; i++) iso[i] = sxw_sum(iso[i]);
     float fwhm = 0.0f;

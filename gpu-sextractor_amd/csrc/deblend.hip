@@ -53,7 +53,7 @@
  *   k_deb_objlist    one wavefront per raw object: its kept components -> objects
  *   k_deb_pre        one wavefront per object: FULL preanalysis over the component's
  *                    level pixels (lanes compute the terms, the raster-order sums are
- *                    folded in lane order with readlane: sxg_wave.h)
+ *                    transposed folds: sxg_wave.h)
  *   k_deb_gather     one lane per raw pixel: gatherup of the pixels no kept component
  *                    holds
  *   then the raw pixels are regrouped by object with the stable radix sort (opix,
@@ -181,7 +181,6 @@ struct DebArgs {
     const long *compoff;    /* region of each raw object in the table */
     int *rootnc;            /* components per raw object */
     int *rootkept;          /* kept objects per raw object */
-    float *rawflux;         /* raw object fdflux (when needed) */
     unsigned long long *top1;   /* highest local maximum: uord(cd) << 32 | raw-list index */
     unsigned *top2;         /* uord(cd) of the second-highest local maximum, 0 if none */
     int *lstar;             /* deepest level that can hold two components */
@@ -271,6 +270,35 @@ __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap_small, 
 }
 
 /*
+ * Significance test fdflux - t*npix > fdflux_root * DEBLEND_MINCONT, where fdflux is the
+ * float sum of cd over the component's pixels in raster order, decided from bounds.
+ * s and sabs are the exact-to-double sums of cd and |cd| (any order); the sequential
+ * float sum differs from s by at most gamma_{n-1} * sabs, gamma_m = m u / (1 - m u),
+ * u = 2^-24 (recursive summation, Higham, Accuracy and Stability of Numerical
+ * Algorithms, 2nd ed., eq. 4.4), plus the double rounding of s itself.  The float
+ * subtraction and the final comparison are bracketed the same way.  Returns 1 or 0 when
+ * the bounds decide, -1 when the exact sequential sum is needed.
+ */
+__device__ inline double deb_sum_err(double sabs, int n)
+{
+    const double m = (double)(n > 1 ? n - 1 : 1) * 0x1p-24;
+    if (m >= 0.5) return INFINITY;
+    return sabs * (m / (1.0 - m) + (double)n * 0x1p-52) * 1.0001;
+}
+
+__device__ inline int deb_sig_bound(double s, double sabs, int n, float b, double v0lo,
+                                    double v0hi)
+{
+    const double e = deb_sum_err(sabs, n);
+    double lo = (s - e) - (double)b, hi = (s + e) - (double)b;
+    lo -= fabs(lo) * 0x1p-23 + 1e-37;
+    hi += fabs(hi) * 0x1p-23 + 1e-37;
+    if (lo > v0hi) return 1;
+    if (hi <= v0lo) return 0;
+    return -1;
+}
+
+/*
  * The component tree of raw object r, levels 1..L*.  cdv, m, par, lab, sz and the four
  * neighbour planes nb[d * nbs + k] are per raw-list index k (LDS or global scratch);
  * pix is the object's raw pixel list (pixw: where to copy it first, LDS only).
@@ -284,6 +312,7 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
     __shared__ int levstart[SXG_MAXNTHRESH + 1];
     __shared__ float tthr[SXG_MAXNTHRESH], ttau[SXG_MAXNTHRESH];
     __shared__ int s_multi, s_nk;
+    __shared__ double s_rsum[NT / 64], s_rabs[NT / 64], s_v0lo, s_v0hi;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = a.cnt[r], off = a.off[r], ls = a.lstar[r];
     const int W = a.W;
@@ -309,9 +338,13 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         cdv[k] = a.cd[p];
     }
     __syncthreads();
-    /* level index and backward neighbours (W, NW, N, NE) by position in the list */
+    /* level index and backward neighbours (W, NW, N, NE) by position in the list; bounds
+       of the root's flux */
+    double rsum = 0.0, rabs = 0.0;
     for (int k = tid; k < n; k += NT) {
         const float v = cdv[k];
+        rsum += v;
+        rabs += fabs(v);
         int mk = 0;
         while (mk < ls && ttau[mk + 1] <= v) mk++;
         m[k] = (unsigned char)mk;
@@ -338,7 +371,25 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         nb[3 * nbs + k] = ne;
         lab[k] = 0;
     }
+    rsum = sxw_sum(rsum);
+    rabs = sxw_sum(rabs);
+    if (lane == 0) {
+        s_rsum[wid] = rsum;
+        s_rabs[wid] = rabs;
+    }
     __syncthreads();
+    if (tid == 0) {
+        double rs = 0.0, ra = 0.0;
+        for (int w = 0; w < NT / 64; w++) {
+            rs += s_rsum[w];
+            ra += s_rabs[w];
+        }
+        /* value0 = (double)fdflux_root * mincont with fdflux_root in [rs - e, rs + e] */
+        const double e = deb_sum_err(ra, n);
+        const double p = (rs - e) * a.mincont, q = (rs + e) * a.mincont;
+        s_v0lo = fmin(p, q) - fabs(fmin(p, q)) * 0x1p-50 - 1e-300;
+        s_v0hi = fmax(p, q) + fabs(fmax(p, q)) * 0x1p-50 + 1e-300;
+    }
     int nc = 1, last = 0;
     for (int i = 1; i <= ls; i++) {
         const float ti = tthr[i];
@@ -410,11 +461,35 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         last = i;
         if (tid == 0) levstart[i + 1] = nc;
         __syncthreads();
-        /* fluxes of the new components whose parent has two or more children */
+        /* significance of each new component whose parent has two or more children
+           (one wavefront per component; only their significance matters) */
         for (int c = nc - nkept + wid; c < nc; c += NT / 64) {
-            if (ldw<true>(&C[C[c].parent].nchild) < 2) continue;
-            const float f = wave_ordered_sum(cdv, n, [&](int k) { return lab[k] == c; });
-            if (lane == 0) C[c].fdflux = f;
+            const int pc = C[c].parent;
+            if (ldw<true>(&C[pc].nchild) < 2) continue;
+            double ps = 0.0, pa = 0.0;
+            for (int k = lane; k < n; k += 64)
+                if (lab[k] == c) {
+                    const double v = cdv[k];
+                    ps += v;
+                    pa += fabs(v);
+                }
+            ps = sxw_sum(ps);
+            pa = sxw_sum(pa);
+            const int cnt = C[c].count;
+            const float b = C[c].dthresh * (float)cnt;
+            int sig = deb_sig_bound(ps, pa, cnt, b, s_v0lo, s_v0hi);
+            if (sig < 0) {
+                /* too close to call: the exact sequential sums */
+                const float f = wave_ordered_sum(cdv, n, [&](int k) { return lab[k] == c; });
+                const float fr = wave_ordered_sum(cdv, n, [](int) { return true; });
+                const float lhs = f - b;
+                sig = (double)lhs > (double)fr * a.mincont;
+                if (lane == 0) C[c].fdflux = f;
+            }
+            if (sig && lane == 0) {
+                atomicOr(&C[c].flags, COMP_SIG);
+                atomicAdd(&C[pc].nson, 1);
+            }
         }
         if (nkept == 0) break;
     }
@@ -427,24 +502,6 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
             a.rootkept[r] = 1;
         }
         return;
-    }
-    if (wid == 0) {
-        const float f = wave_ordered_sum(cdv, n, [](int) { return true; });
-        if (lane == 0) {
-            C[0].fdflux = f;
-            a.rawflux[r] = f;
-        }
-    }
-    __syncthreads();
-    const double value0 = (double)C[0].fdflux * a.mincont;
-    for (int c = 1 + tid; c < nc; c += NT) {
-        const int pc = C[c].parent;
-        if (ldw<true>(&C[pc].nchild) < 2) continue;
-        const float lhs = C[c].fdflux - C[c].dthresh * (float)C[c].count;
-        if ((double)lhs > value0) {
-            atomicOr(&C[c].flags, COMP_SIG);
-            atomicAdd(&C[pc].nson, 1);
-        }
     }
     for (int c = tid; c < nc; c += NT) atomicOr(&C[c].flags, COMP_OKA);
     __syncthreads();
@@ -552,9 +609,13 @@ __global__ void __launch_bounds__(256) k_deb_allocinit(ObjArgs A)
  */
 __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
 {
-    const int lane = threadIdx.x & 63;
-    const int obj = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ double s_rows[4][6 * SXW_ROW];
+    __shared__ float s_frows[4][SXW_ROW];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int obj = blockIdx.x * 4 + wid;
     if (obj >= A.nobj) return;
+    double *rows = s_rows[wid];
+    float *frows = s_frows[wid];
     const DebArgs &d = A.d;
     const int r = A.objraw[obj], c = A.objcomp[obj];
     const Comp *C = d.comps + d.compoff[r];
@@ -562,7 +623,7 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
     const int n = d.cnt[r], off = d.off[r], W = d.W, H = d.H;
     const int *lv = L > 0 ? d.lvl + (size_t)L * d.P0 + off : nullptr;
     /* pass 1: extent, peaks, count; rv is the raster-order float sum of cd */
-    float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG, rv = 0.0f;
+    float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG, racc = 0.0f;
     int xmin = 2 * SXG_MAXPICSIZE, ymin = 2 * SXG_MAXPICSIZE, xmax = 0, ymax = 0, cnt = 0;
     for (int base = 0; base < n; base += 64) {
         const int k = base + lane;
@@ -581,13 +642,10 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
             ymax = max(ymax, y);
             cnt++;
         }
-        unsigned long long b = __ballot(in);
-        while (b) {
-            const int j = __builtin_ctzll(b);
-            b &= b - 1;
-            rv += sxw_rl(cval, j);
-        }
+        const float tf[1] = {cval};
+        sxw_fold<1>(frows, tf, in, racc);
     }
+    const float rv = sxw_rl(racc, 0);
     peak = sxw_max(peak);
     cpeak = sxw_max(cpeak);
     xmin = sxw_min(xmin);
@@ -600,7 +658,7 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
     const double dthresh = (double)d.thr[(size_t)r * d.nthresh + L];
     const double dthresh2 = (dthresh + peak) / 2.0;
     /* pass 2: the moments, double sums of float terms in raster order */
-    double mx = 0.0, my = 0.0, tv = 0.0, mx2 = 0.0, my2 = 0.0, mxy = 0.0;
+    double acc = 0.0;
     int area2 = 0, dnpix = 0;
     for (int base = 0; base < n; base += 64) {
         const int k = base + lane;
@@ -619,18 +677,11 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
             tyy = cval * y * y;
             txy = cval * x * y;
         }
-        unsigned long long b = __ballot(in);
-        while (b) {
-            const int j = __builtin_ctzll(b);
-            b &= b - 1;
-            tv += sxw_rl(val, j);
-            mx += sxw_rl(tx, j);
-            my += sxw_rl(ty, j);
-            mx2 += sxw_rl(txx, j);
-            my2 += sxw_rl(tyy, j);
-            mxy += sxw_rl(txy, j);
-        }
+        const double t6[6] = {val, tx, ty, txx, tyy, txy};
+        sxw_fold<6>(rows, t6, in, acc);
     }
+    const double tv = sxw_rl(acc, 0), mx = sxw_rl(acc, 1), my = sxw_rl(acc, 2);
+    const double mx2 = sxw_rl(acc, 3), my2 = sxw_rl(acc, 4), mxy = sxw_rl(acc, 5);
     area2 = sxw_sum(area2);
     dnpix = sxw_sum(dnpix);
     if (lane != 0) return;
@@ -813,7 +864,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     a.lvl = lvl; a.P0 = P0; a.thr = sxg_take<float>("deb.thr", nr * n_thresh);
     a.nthresh = n_thresh; a.debarea = deb_minarea; a.mincont = deblend_mincont;
     a.W = S.W; a.H = S.H; a.nraw = nraw; a.comps = comps; a.compoff = compoff;
-    a.rootnc = rootnc; a.rootkept = rootkept; a.rawflux = ro.fdflux;
+    a.rootnc = rootnc; a.rootkept = rootkept;
     a.top1 = top1; a.top2 = top2; a.lstar = lstar; a.clist = clist; a.ccount = ccount;
     int nobj = 0;
     if (nraw > 0) {

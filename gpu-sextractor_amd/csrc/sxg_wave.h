@@ -48,6 +48,11 @@ __device__ inline int sxw_sum(int v)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
+__device__ inline double sxw_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
 
 /* keep the five largest values seen by this lane, t[0] >= ... >= t[4] (-inf = empty) */
 struct SxwTop5 {
@@ -85,3 +90,38 @@ struct SxwTop5 {
         return res;
     }
 };
+
+/*
+ * Transposed ordered folds.  The NS terms of each member lane of a chunk are written to
+ * LDS compacted in lane order (row s holds the terms of sum s); then lane s adds row s
+ * in order into its own accumulator.  NS sums advance together at one LDS read and one
+ * add per member, and each sum is still the sequential sum in raster order.
+ * rows points to this wavefront's scratch of NS rows of SXW_ROW elements.
+ */
+#define SXW_ROW 66   /* row stride: rows start on different LDS banks */
+
+template <int NS, class T>
+__device__ inline void sxw_fold(T *rows, const T (&t)[NS], bool member, T &acc)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long b = __ballot(member);
+    const int cnt = __popcll(b);
+    if (member) {
+        const int pos = __popcll(b & sxw_lanes_below(lane));
+#pragma unroll
+        for (int s = 0; s < NS; s++) rows[s * SXW_ROW + pos] = t[s];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < NS) {
+        const T *row = rows + lane * SXW_ROW;
+        T a = acc;
+#pragma unroll 8
+        for (int j = 0; j < cnt; j++) a += row[j];
+        acc = a;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}

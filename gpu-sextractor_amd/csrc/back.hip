@@ -5,8 +5,8 @@
  * Kernels:
  *   k_meshstat   one lane per mesh: backstat (back.c:214-290).  The double sums of pix
  *                and pix^2 are sequential in raster order inside the mesh, as back.c
- *                does, so each lane runs the chain; float4 loads cut the load count,
- *                and 16-lane workgroups put the chains on every CU.
+ *                does, so each lane runs the chain; three wavefronts of the workgroup
+ *                stream the mesh rows through a double-buffered LDS ring meanwhile.
  *   k_meshmode   one workgroup per mesh: histogram in LDS (back.c:297-341), then the
  *                clipped-mode iteration of backguess (back.c:347-426).  Inside the
  *                iteration every sum is an exact integer (< 2^53), so the sequential
@@ -14,9 +14,10 @@
  *                is a merge of two non-decreasing sequences and its end point is found
  *                by a binary search on prefix sums (merge-path), which reproduces the
  *                walk exactly.
- *   k_filterback one workgroup: bad-mesh interpolation and median filtering
- *                (back.c:434-560), global medians, y second derivatives
- *                (back.c:739-780).
+ *   k_fb_*       bad-mesh interpolation and median filtering, one lane per mesh
+ *                (back.c:434-530); global medians by an LDS bitonic sort in one
+ *                workgroup (back.c:532-556); y second derivatives, one lane per mesh
+ *                column with the recurrence in registers (back.c:739-780).
  *   k_rownodes   one lane per frame row: node interpolation along y and the x
  *                tridiagonal solve of subbackline (back.c:790-880).
  *   k_subback    one lane per pixel: bicubic evaluation with the column tables of the
@@ -40,96 +41,239 @@ struct MeshStat {
 };
 
 /* ---------------------------------------------------------------- backstat ---- */
-#define MS_LANES 16   /* meshes per workgroup: spreads the chains over every CU */
-__global__ void __launch_bounds__(MS_LANES) k_meshstat(const float *__restrict__ img, int W, int H,
-                                                       int bs, int nbx, int nmesh, MeshStat *ms)
-{
-    const int m = blockIdx.x * MS_LANES + threadIdx.x;
-    if (m >= nmesh) return;
-    const int mx = m % nbx, my = m / nbx;
-    const int x0 = mx * bs, y0 = my * bs;
-    const int bw = (x0 + bs <= W) ? bs : W - x0;
-    const int bh = (y0 + bs <= H) ? bs : H - y0;
-    const double step = sqrt(2 / SXG_PI) * QUANTIF_NSIGMA / QUANTIF_AMIN;
-    double mean = 0.0, sigma = 0.0, pix, sig;
+/* statistics of one mesh from its two sequential passes */
+struct MeshAcc {
+    double mean = 0.0, sigma = 0.0;
     int npix = 0;
-    const bool vec = ((x0 & 3) == 0) && ((bw & 3) == 0) && ((W & 3) == 0);
-    for (int y = 0; y < bh; y++) {
-        const float *r = img + (size_t)(y0 + y) * W + x0;
-        if (vec) {
-            for (int x = 0; x < bw; x += 4) {
-                float4 v = *reinterpret_cast<const float4 *>(r + x);
-                float vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if ((pix = vv[k]) > -SXG_BIG) {
-                        mean += pix;
-                        sigma += pix * pix;
-                        npix++;
-                    }
-            }
-        } else {
-            for (int x = 0; x < bw; x++)
-                if ((pix = r[x]) > -SXG_BIG) {
-                    mean += pix;
-                    sigma += pix * pix;
-                    npix++;
-                }
-        }
-    }
-    MeshStat o;
+};
+
+__device__ inline void meshstat_pass1_end(const MeshAcc &a, int bw, int bh, MeshStat &o, float &lcut,
+                                          float &hcut, bool &bad)
+{
     o.lcut = o.hcut = o.qzero = o.qscale = 0.0f;
     o.nlevels = o.npix = 0;
-    if ((float)npix < (float)(bw * bh * BACK_MINGOODFRAC)) {
+    bad = (float)a.npix < (float)(bw * bh * BACK_MINGOODFRAC);
+    if (bad) {
         o.mean = o.sigma = (float)-SXG_BIG;
-        ms[m] = o;
         return;
     }
-    mean /= (double)npix;
-    sigma = (sig = sigma / npix - mean * mean) > 0.0 ? sqrt(sig) : 0.0;
-    const float lcut = (float)(mean - 2.0 * sigma);
-    const float hcut = (float)(mean + 2.0 * sigma);
-    o.lcut = lcut;
-    o.hcut = hcut;
-    mean = sigma = 0.0;
-    npix = 0;
-    for (int y = 0; y < bh; y++) {
-        const float *r = img + (size_t)(y0 + y) * W + x0;
-        if (vec) {
-            for (int x = 0; x < bw; x += 4) {
-                float4 v = *reinterpret_cast<const float4 *>(r + x);
-                float vv[4] = {v.x, v.y, v.z, v.w};
+    double sig;
+    const double mean = a.mean / (double)a.npix;
+    const double sigma = (sig = a.sigma / a.npix - mean * mean) > 0.0 ? sqrt(sig) : 0.0;
+    lcut = o.lcut = (float)(mean - 2.0 * sigma);
+    hcut = o.hcut = (float)(mean + 2.0 * sigma);
+}
+
+__device__ inline void meshstat_pass2_end(const MeshAcc &a, MeshStat &o)
+{
+    const double step = sqrt(2 / SXG_PI) * QUANTIF_NSIGMA / QUANTIF_AMIN;
+    double sig;
+    o.npix = a.npix;
+    const double mean = a.mean / (double)a.npix;
+    sig = a.sigma / a.npix - mean * mean;
+    const double sigma = sig > 0.0 ? sqrt(sig) : 0.0;
+    o.mean = (float)mean;
+    o.sigma = (float)sigma;
+    if ((o.nlevels = (int)(step * a.npix + 1)) > SXG_NMAXLEVELS) o.nlevels = SXG_NMAXLEVELS;
+    o.qscale = (float)(sigma > 0.0 ? 2 * QUANTIF_NSIGMA * sigma / o.nlevels : 1.0);
+    o.qzero = (float)(mean - QUANTIF_NSIGMA * sigma);
+}
+
+/*
+ * One workgroup per band of up to MS_MPB meshes along x.  Wavefront 0 keeps one lane
+ * per mesh running the sequential double sums in raster order; the other three
+ * wavefronts stream the band's rows into a double-buffered LDS ring (R rows per batch,
+ * each mesh's row segment padded, ms_seg, so the folding lanes read float4s from
+ * different banks), so the chains never wait on global memory.
+ */
+#define MS_MPB 16
+#define MS_THREADS 256
+
+/* meshes whose rows do not fit the LDS ring: one lane per mesh reading global memory */
+__global__ void __launch_bounds__(64) k_meshstat_direct(const float *__restrict__ img, int W,
+                                                        int H, int bs, int nbx, int nmesh,
+                                                        MeshStat *ms)
+{
+    const int mi = blockIdx.x * 64 + threadIdx.x;
+    if (mi >= nmesh) return;
+    const int x0 = (mi % nbx) * bs, y0 = (mi / nbx) * bs;
+    const int bw = min(bs, W - x0), bh = min(bs, H - y0);
+    MeshAcc acc;
+    MeshStat o;
+    float lcut = 0.0f, hcut = 0.0f;
+    bool bad = false;
+    for (int pass = 0; pass < 2 && !(pass == 1 && bad); pass++) {
+        for (int y = 0; y < bh; y++) {
+            const float *row = img + (size_t)(y0 + y) * W + x0;
+            for (int x = 0; x < bw; x++) {
+                const double pix = row[x];
+                if (pass == 0 ? pix > -SXG_BIG : (pix <= hcut && pix >= lcut)) {
+                    acc.mean += pix;
+                    acc.sigma += pix * pix;
+                    acc.npix++;
+                }
+            }
+        }
+        if (pass == 0) {
+            meshstat_pass1_end(acc, bw, bh, o, lcut, hcut, bad);
+            acc = MeshAcc();
+        } else {
+            meshstat_pass2_end(acc, o);
+        }
+    }
+    ms[mi] = o;
+}
+/* one mesh's rows of a batch, in raster order; PASS 0 takes every pixel > -BIG, PASS 1
+   those in [lcut, hcut].  Branch-free: an excluded pixel adds +0.0, an exact identity
+   here (the sums start at +0.0 and cannot become -0.0 in round-to-nearest) */
+template <int PASS>
+__device__ inline void meshstat_fold(const float *rows, int rstride, int nr, int bw, float lcut,
+                                     float hcut, MeshAcc &acc)
+{
+    double mean = acc.mean, sigma = acc.sigma;
+    int npix = acc.npix;
+    const double lo = lcut, hi = hcut;
+    auto fold = [&](float v) {
+        const double pix = v;
+        const bool in = PASS == 0 ? (pix > -SXG_BIG) : ((pix <= hi) & (pix >= lo));
+        const double t = in ? pix : 0.0;
+        mean += t;
+        sigma += t * t;
+        npix += in;
+    };
+    for (int r = 0; r < nr; r++) {
+        const float *row = rows + r * rstride;
+        int x = 0;
+        /* four LDS reads in flight, then the 16 values in order */
+        for (; x + 16 <= bw; x += 16) {
+            const float4 a = *reinterpret_cast<const float4 *>(row + x);
+            const float4 b = *reinterpret_cast<const float4 *>(row + x + 4);
+            const float4 c = *reinterpret_cast<const float4 *>(row + x + 8);
+            const float4 d = *reinterpret_cast<const float4 *>(row + x + 12);
+            fold(a.x); fold(a.y); fold(a.z); fold(a.w);
+            fold(b.x); fold(b.y); fold(b.z); fold(b.w);
+            fold(c.x); fold(c.y); fold(c.z); fold(c.w);
+            fold(d.x); fold(d.y); fold(d.z); fold(d.w);
+        }
+        for (; x < bw; x++) fold(row[x]);
+    }
+    acc.mean = mean;
+    acc.sigma = sigma;
+    acc.npix = npix;
+}
+
+/* padded LDS segment of one mesh row: a multiple of 4 floats (float4 reads) whose
+   quarter is odd, so the 16 folding lanes start on distinct banks */
+__host__ __device__ inline int ms_seg(int bs)
+{
+    int seg = (bs + 3) & ~3;
+    if (((seg >> 2) & 1) == 0) seg += 4;
+    return seg;
+}
+
+__global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict__ img, int W, int H,
+                                                         int bs, int nbx, int mpb, int R,
+                                                         MeshStat *ms)
+{
+    extern __shared__ __align__(16) float ring[];
+    const int tid = threadIdx.x, wid = tid >> 6;
+    const int mx0 = blockIdx.x * mpb, my = blockIdx.y;
+    const int nm = min(mpb, nbx - mx0);
+    const int x0 = mx0 * bs, y0 = my * bs;
+    const int bandw = min(W, x0 + nm * bs) - x0;
+    const int bh = min(bs, H - y0);
+    const int seg = ms_seg(bs);
+    const int bufsz = R * mpb * seg;
+    const int nbatch = (bh + R - 1) / R;
+    /* the folding lane's mesh */
+    const int m = tid;
+    const bool folder = wid == 0 && m < nm;
+    const int bw = folder ? min(bs, W - (x0 + m * bs)) : 0;
+    MeshAcc acc;
+    MeshStat o;
+    float lcut = 0.0f, hcut = 0.0f;
+    bool bad = false;
+    /* wavefronts 1..3 copy rows [b*R, b*R + nr) of the band into buf; four loads are in
+       flight per thread before any LDS store */
+    const bool vec4 = ((bs | W) & 3) == 0;
+    auto load = [&](int b, float *buf) {
+        const int r0 = b * R, nr = min(R, bh - r0);
+        const int lt = tid - 64, LT = MS_THREADS - 64;
+        if (vec4) {
+            const int q4 = bandw >> 2, tot = nr * q4;
+            for (int e0 = lt; e0 < tot; e0 += 4 * LT) {
+                float4 v[4];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    pix = vv[k];
-                    if (pix <= hcut && pix >= lcut) {
-                        mean += pix;
-                        sigma += pix * pix;
-                        npix++;
+                for (int u = 0; u < 4; u++) {
+                    const int e = e0 + u * LT;
+                    if (e < tot) {
+                        const int r = e / q4, c = 4 * (e - r * q4);
+                        v[u] = *reinterpret_cast<const float4 *>(
+                            img + (size_t)(y0 + r0 + r) * W + x0 + c);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int e = e0 + u * LT;
+                    if (e < tot) {
+                        const int r = e / q4, c = 4 * (e - r * q4);
+                        const int mm = c / bs, off = c - mm * bs;
+                        *reinterpret_cast<float4 *>(buf + (r * mpb + mm) * seg + off) = v[u];
                     }
                 }
             }
         } else {
-            for (int x = 0; x < bw; x++) {
-                pix = r[x];
-                if (pix <= hcut && pix >= lcut) {
-                    mean += pix;
-                    sigma += pix * pix;
-                    npix++;
+            const int tot = nr * bandw;
+            for (int e0 = lt; e0 < tot; e0 += 4 * LT) {
+                float v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int e = e0 + u * LT;
+                    if (e < tot) {
+                        const int r = e / bandw, c = e - r * bandw;
+                        v[u] = img[(size_t)(y0 + r0 + r) * W + x0 + c];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int e = e0 + u * LT;
+                    if (e < tot) {
+                        const int r = e / bandw, c = e - r * bandw;
+                        const int mm = c / bs, off = c - mm * bs;
+                        buf[(r * mpb + mm) * seg + off] = v[u];
+                    }
                 }
             }
         }
+    };
+    for (int pass = 0; pass < 2; pass++) {
+        if (wid != 0) load(0, ring);
+        __syncthreads();
+        for (int b = 0; b < nbatch; b++) {
+            float *cur = ring + (b & 1) * bufsz;
+            if (wid != 0) {
+                if (b + 1 < nbatch) load(b + 1, ring + ((b + 1) & 1) * bufsz);
+            } else if (folder && !(pass == 1 && bad)) {
+                const int nr = min(R, bh - b * R);
+                const float *rows = cur + m * seg;
+                const int rstride = mpb * seg;
+                if (pass == 0)
+                    meshstat_fold<0>(rows, rstride, nr, bw, 0.0f, 0.0f, acc);
+                else
+                    meshstat_fold<1>(rows, rstride, nr, bw, lcut, hcut, acc);
+            }
+            __syncthreads();
+        }
+        if (folder) {
+            if (pass == 0) {
+                meshstat_pass1_end(acc, bw, bh, o, lcut, hcut, bad);
+                acc = MeshAcc();
+            } else if (!bad) {
+                meshstat_pass2_end(acc, o);
+            }
+        }
     }
-    o.npix = npix;
-    mean /= (double)npix;
-    sig = sigma / npix - mean * mean;
-    sigma = sig > 0.0 ? sqrt(sig) : 0.0;
-    o.mean = (float)mean;
-    o.sigma = (float)sigma;
-    if ((o.nlevels = (int)(step * npix + 1)) > SXG_NMAXLEVELS) o.nlevels = SXG_NMAXLEVELS;
-    o.qscale = (float)(sigma > 0.0 ? 2 * QUANTIF_NSIGMA * sigma / o.nlevels : 1.0);
-    o.qzero = (float)(mean - QUANTIF_NSIGMA * sigma);
-    ms[m] = o;
+    if (folder) ms[my * nbx + mx0 + m] = o;
 }
 
 /* --------------------------------------------------------------- backguess ---- */
@@ -309,7 +453,80 @@ __device__ inline float small_median(float *a, int n)
     return (n & 1) ? a[n / 2] : (float)((a[n / 2 - 1] + a[n / 2]) / 2.0);
 }
 
-/* ascending bitonic sort of n2 (power of two) floats in global memory, one workgroup */
+/* median of a sorted array, hmedian's even-n rule */
+__device__ inline float sorted_median(const float *a, int n)
+{
+    if (n < 2) return a[0];
+    return (n & 1) ? a[n / 2] : (float)((a[n / 2 - 1] + a[n / 2]) / 2.0);
+}
+
+/* bad meshes: average of the nearest valid meshes (back.c:461-490); one lane per mesh,
+   into back2/sigma2 (good meshes copied) */
+__global__ void __launch_bounds__(256) k_fb_interp(const float *back, const float *sigma,
+                                                   float *back2, float *sigma2, int nx, int ny)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nx * ny) return;
+    if (back[i] > -SXG_BIG) {
+        back2[i] = back[i];
+        sigma2[i] = sigma[i];
+        return;
+    }
+    const int px = i % nx, py = i / nx;
+    float d2min = (float)SXG_BIG, val = 0.0f, sval = 0.0f;
+    int nmin = 0;
+    for (int j = 0, y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++, j++)
+            if (back[j] > -SXG_BIG) {
+                float d2 = (float)(x - px) * (x - px) + (y - py) * (y - py);
+                if (d2 < d2min) {
+                    val = back[j];
+                    sval = sigma[j];
+                    nmin = 1;
+                    d2min = d2;
+                } else if (d2 == d2min) {
+                    val += back[j];
+                    sval += sigma[j];
+                    nmin++;
+                }
+            }
+    back2[i] = nmin ? val / nmin : 0.0f;
+    sigma2[i] = nmin ? sval / nmin : 1.0f;
+}
+
+/* median filtering with a window clipped symmetrically at the edges (back.c:492-530);
+   one lane per mesh, back2/sigma2 -> back/sigma */
+__global__ void __launch_bounds__(256) k_fb_median(const float *back2, const float *sigma2,
+                                                   float *back, float *sigma, int nx, int ny,
+                                                   int nfx, int nfy, float fthresh)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nx * ny) return;
+    const int npx = nfx / 2, npy = nfy / 2;
+    const int px = i % nx, py = i / nx;
+    int npx2 = nx - px - 1, npy2 = ny - py - 1;
+    if (npx2 > npx) npx2 = npx;
+    if (npx2 > px) npx2 = px;
+    if (npy2 > npy) npy2 = npy;
+    if (npy2 > py) npy2 = py;
+    float bm[SXG_MAXBACKF * SXG_MAXBACKF], sm_[SXG_MAXBACKF * SXG_MAXBACKF];
+    int k = 0;
+    for (int dy = -npy2; dy <= npy2; dy++)
+        for (int dx = -npx2; dx <= npx2; dx++) {
+            bm[k] = back2[(py + dy) * nx + px + dx];
+            sm_[k++] = sigma2[(py + dy) * nx + px + dx];
+        }
+    const float med = small_median(bm, k);
+    if (fabs(med - back2[i]) >= fthresh) {
+        back[i] = med;
+        sigma[i] = small_median(sm_, k);
+    } else {
+        back[i] = back2[i];
+        sigma[i] = sigma2[i];
+    }
+}
+
+/* ascending bitonic sort of n2 (power of two) floats, one workgroup of FB_THREADS */
 __device__ void wg_bitonic(float *a, int n2)
 {
     for (int k = 2; k <= n2; k <<= 1)
@@ -326,97 +543,23 @@ __device__ void wg_bitonic(float *a, int n2)
         }
 }
 
-/* median of a sorted array, hmedian's even-n rule */
-__device__ inline float sorted_median(const float *a, int n)
+/* global medians of the filtered maps (back.c:532-556): sorted copies padded with +inf,
+   in LDS when they fit (dynamic shared memory of 2*n2 floats), else in gbuf */
+__global__ void __launch_bounds__(FB_THREADS) k_fb_global(const float *back, const float *sigma,
+                                                          int np, int n2, int in_lds,
+                                                          float *gbuf, float *out)
 {
-    if (n < 2) return a[0];
-    return (n & 1) ? a[n / 2] : (float)((a[n / 2 - 1] + a[n / 2]) / 2.0);
-}
-
-/* maps: back, sigma (unfiltered in, filtered out), dback, dsigma; work: 4*n2 floats;
-   out[0..1]: backmean, backsig */
-__global__ void __launch_bounds__(FB_THREADS) k_filterback(float *back, float *sigma,
-                                                           float *dback, float *dsigma,
-                                                           int nx, int ny, int nfx, int nfy,
-                                                           double backfthresh, float *work,
-                                                           int n2, float *out)
-{
-    const int np = nx * ny;
-    float *back2 = work, *sigma2 = work + n2, *sbuf = work + 2 * n2, *u = work + 3 * n2;
-    const float fthresh = (float)backfthresh;
-    /* bad meshes: average of the nearest valid meshes (back.c:461-490) */
-    for (int i = threadIdx.x; i < np; i += FB_THREADS) {
-        back2[i] = back[i];
-        sigma2[i] = sigma[i];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < np; i += FB_THREADS) {
-        if (back2[i] > -SXG_BIG) continue;
-        const int px = i % nx, py = i / nx;
-        float d2min = (float)SXG_BIG, val = 0.0f, sval = 0.0f;
-        int nmin = 0;
-        for (int j = 0, y = 0; y < ny; y++)
-            for (int x = 0; x < nx; x++, j++)
-                if (back2[j] > -SXG_BIG) {
-                    float d2 = (float)(x - px) * (x - px) + (y - py) * (y - py);
-                    if (d2 < d2min) {
-                        val = back2[j];
-                        sval = sigma2[j];
-                        nmin = 1;
-                        d2min = d2;
-                    } else if (d2 == d2min) {
-                        val += back2[j];
-                        sval += sigma2[j];
-                        nmin++;
-                    }
-                }
-        back[i] = nmin ? val / nmin : 0.0f;
-        sigma[i] = nmin ? sval / nmin : 1.0f;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < np; i += FB_THREADS)
-        if (back2[i] > -SXG_BIG) { back[i] = back2[i]; sigma[i] = sigma2[i]; }
-    __syncthreads();
-    /* median filtering with a window clipped symmetrically at the edges */
-    const int npx = nfx / 2, npy = nfy / 2;
-    for (int i = threadIdx.x; i < np; i += FB_THREADS) {
-        const int px = i % nx, py = i / nx;
-        int npx2 = nx - px - 1, npy2 = ny - py - 1;
-        if (npx2 > npx) npx2 = npx;
-        if (npx2 > px) npx2 = px;
-        if (npy2 > npy) npy2 = npy;
-        if (npy2 > py) npy2 = py;
-        float bm[SXG_MAXBACKF * SXG_MAXBACKF], sm_[SXG_MAXBACKF * SXG_MAXBACKF];
-        int k = 0;
-        for (int dy = -npy2; dy <= npy2; dy++)
-            for (int dx = -npx2; dx <= npx2; dx++) {
-                bm[k] = back[(py + dy) * nx + px + dx];
-                sm_[k++] = sigma[(py + dy) * nx + px + dx];
-            }
-        float med = small_median(bm, k);
-        if (fabs(med - back[i]) >= fthresh) {
-            back2[i] = med;
-            sigma2[i] = small_median(sm_, k);
-        } else {
-            back2[i] = back[i];
-            sigma2[i] = sigma[i];
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < np; i += FB_THREADS) {
-        back[i] = back2[i];
-        sigma[i] = sigma2[i];
-    }
-    /* global medians: sort copies padded with +inf */
+    extern __shared__ float fbs[];
+    float *sb = in_lds ? fbs : gbuf, *u = sb + n2;
     for (int i = threadIdx.x; i < n2; i += FB_THREADS) {
-        sbuf[i] = i < np ? back2[i] : __builtin_inff();
-        u[i] = i < np ? sigma2[i] : __builtin_inff();
+        sb[i] = i < np ? back[i] : __builtin_inff();
+        u[i] = i < np ? sigma[i] : __builtin_inff();
     }
     __syncthreads();
-    wg_bitonic(sbuf, n2);
+    wg_bitonic(sb, n2);
     wg_bitonic(u, n2);
     if (threadIdx.x == 0) {
-        out[0] = sorted_median(sbuf, np);
+        out[0] = sorted_median(sb, np);
         float bsig = sorted_median(u, np);
         if (bsig <= 0.0) {
             /* median of the strictly positive sigmas (back.c:542-555) */
@@ -429,28 +572,59 @@ __global__ void __launch_bounds__(FB_THREADS) k_filterback(float *back, float *s
         }
         out[1] = bsig;
     }
-    __syncthreads();
-    /* y second derivatives, one lane per mesh column (back.c:739-780); u reused */
-    for (int c = threadIdx.x; c < 2 * nx; c += FB_THREADS) {
-        const int x = c % nx;
-        const float *map = c < nx ? back : sigma;
-        float *dmap = c < nx ? dback : dsigma;
-        float *uu = (c < nx ? back2 : sigma2) + x * (ny > 1 ? ny - 1 : 1);
-        if (ny > 1) {
-            float temp;
-            dmap[x] = uu[0] = 0.0f;
-            for (int y = 1; y < ny - 1; y++) {
-                temp = -1 / (dmap[(y - 1) * nx + x] + 4);
-                dmap[y * nx + x] = temp;
-                temp *= uu[y - 1] - 6 * (map[(y + 1) * nx + x] + map[(y - 1) * nx + x]
-                                         - 2 * map[y * nx + x]);
-                uu[y] = temp;
-            }
-            dmap[(ny - 1) * nx + x] = 0.0f;
-            for (int y = ny - 2; y >= 1; y--)
-                dmap[y * nx + x] = (float)((dmap[y * nx + x] * dmap[(y + 1) * nx + x] + uu[y]) / 6.0);
-        } else
-            dmap[x] = 0.0f;
+}
+
+/* y second derivatives of back and sigma, one lane per mesh column (back.c:739-780);
+   the forward recurrence runs in registers and keeps its per-row values (the
+   tridiagonal factors and uu) for the backward pass in LDS when ny rows of 64 lanes fit
+   (use_lds), else in uu_all (2 * ny floats per column) */
+__global__ void __launch_bounds__(64) k_fb_yderiv(const float *back, const float *sigma,
+                                                  float *dback, float *dsigma, int nx, int ny,
+                                                  int use_lds, float *uu_all)
+{
+    extern __shared__ float yd[];
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    const bool live = c < 2 * nx;
+    const int x = c % nx;
+    const float *map = c < nx ? back : sigma;
+    float *dmap = c < nx ? dback : dsigma;
+    if (!live) return;
+    if (ny < 2) {
+        dmap[x] = 0.0f;
+        return;
+    }
+    /* element y of this column's two scratch rows */
+    float *ta, *tb;
+    size_t st;
+    if (use_lds) {
+        ta = yd + threadIdx.x;
+        tb = yd + (size_t)ny * 64 + threadIdx.x;
+        st = 64;
+    } else {
+        ta = uu_all + (size_t)c * 2 * ny;
+        tb = ta + ny;
+        st = 1;
+    }
+    float dprev = 0.0f, uprev = 0.0f;
+    float m0 = map[x], m1 = map[nx + x];
+    for (int y = 1; y < ny - 1; y++) {
+        const float m2 = map[(y + 1) * nx + x];
+        float temp = -1 / (dprev + 4);
+        ta[y * st] = temp;
+        dprev = temp;
+        temp *= uprev - 6 * (m2 + m0 - 2 * m1);
+        tb[y * st] = temp;
+        uprev = temp;
+        m0 = m1;
+        m1 = m2;
+    }
+    dmap[x] = 0.0f;
+    dmap[(ny - 1) * nx + x] = 0.0f;
+    float dnext = 0.0f;
+    for (int y = ny - 2; y >= 1; y--) {
+        const float d = (float)((ta[y * st] * dnext + tb[y * st]) / 6.0);
+        dmap[y * nx + x] = d;
+        dnext = d;
     }
 }
 
@@ -597,16 +771,55 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
 
     sxg_stage_begin(ST_BACK);
     sxg_kt_begin(SXG_KT_MESHSTAT);
-    hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nmesh, MS_LANES)), dim3(MS_LANES), 0, S.stream, img, W, H,
-                       bs, nbx, nmesh, ms);
+    {
+        /* meshes per workgroup and rows per batch: two buffers of R rows of mpb padded
+           segments within 128 KB of LDS */
+        const size_t segbytes = (size_t)ms_seg(bs) * sizeof(float);
+        const int mpb = (int)std::min<size_t>(MS_MPB, (64 * 1024) / segbytes);
+        if (mpb >= 1) {
+            const size_t rowbytes = mpb * segbytes;
+            const int R = (int)std::max<size_t>(1, std::min<size_t>(8, (64 * 1024) / rowbytes));
+            const size_t lds = 2 * R * rowbytes;
+            SXG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_meshstat),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nbx, mpb), nby), dim3(MS_THREADS), lds,
+                               S.stream, img, W, H, bs, nbx, mpb, R, ms);
+        } else {
+            hipLaunchKernelGGL(k_meshstat_direct, dim3(sxg_div_up(nmesh, 64)), dim3(64), 0,
+                               S.stream, img, W, H, bs, nbx, nmesh, ms);
+        }
+    }
     sxg_kt_end(SXG_KT_MESHSTAT);
     sxg_kt_begin(SXG_KT_MESHMODE);
     hipLaunchKernelGGL(k_meshmode, dim3(nmesh), dim3(MODE_THREADS), 0, S.stream, img, W, H, bs,
                        nbx, ms, back, sigma);
     sxg_kt_end(SXG_KT_MESHMODE);
     sxg_kt_begin(SXG_KT_FILTERBACK);
-    hipLaunchKernelGGL(k_filterback, dim3(1), dim3(FB_THREADS), 0, S.stream, back, sigma, dback,
-                       dsigma, nbx, nby, nfx, nfy, (double)backfthresh, work, n2, outv);
+    {
+        float *back2 = work, *sigma2 = work + n2, *gbuf = work + 2 * n2;
+        const int gm = sxg_div_up(nmesh, 256);
+        hipLaunchKernelGGL(k_fb_interp, dim3(gm), dim3(256), 0, S.stream, back, sigma, back2,
+                           sigma2, nbx, nby);
+        hipLaunchKernelGGL(k_fb_median, dim3(gm), dim3(256), 0, S.stream, back2, sigma2, back,
+                           sigma, nbx, nby, nfx, nfy, (float)backfthresh);
+        const size_t lds = 2 * (size_t)n2 * sizeof(float);
+        const int in_lds = lds <= 128 * 1024;
+        if (in_lds)
+            SXG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fb_global),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_fb_global, dim3(1), dim3(FB_THREADS), in_lds ? lds : 0, S.stream,
+                           back, sigma, nmesh, n2, in_lds, gbuf, outv);
+        /* scratch of 2*nby floats per column: LDS when it fits, else back2/sigma2 and the
+           sort buffer (4*n2 >= 2 * 2*nbx*nby floats) */
+        const size_t ylds = 2 * (size_t)nby * 64 * sizeof(float);
+        const int ylds_ok = ylds <= 128 * 1024;
+        if (ylds_ok)
+            SXG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fb_yderiv),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)ylds));
+        hipLaunchKernelGGL(k_fb_yderiv, dim3(sxg_div_up(2 * nbx, 64)), dim3(64),
+                           ylds_ok ? ylds : 0, S.stream, back, sigma, dback, dsigma, nbx, nby,
+                           ylds_ok, back2);
+    }
     sxg_kt_end(SXG_KT_FILTERBACK);
     float hv[2];
     SXG_HIP(hipMemcpyAsync(hv, outv, sizeof(hv), hipMemcpyDeviceToHost, S.stream));

@@ -16,6 +16,7 @@
  * does not depend on scheduling.
  */
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -57,7 +58,8 @@ __global__ void __launch_bounds__(CP_T) k_clean_pairs(ObjCols o, int nobj, int w
     const int jmax = live ? min(i + win, nobj - 1) : -1;
     unsigned mine = 0xffffffffu;
     const int jlast = min(i0 + CP_T - 1 + win, nobj - 1);
-    for (int j0 = i0 + 1; j0 <= jlast; j0 += CP_T) {
+    /* the window's tiles are shared out over gridDim.y workgroups */
+    for (int j0 = i0 + 1 + blockIdx.y * CP_T; j0 <= jlast; j0 += CP_T * gridDim.y) {
         __syncthreads();
         const int jl = j0 + t;
         if (jl <= jlast) {
@@ -121,7 +123,10 @@ extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacks
         if (S.clean_enabled) {
             const int win = ((clean_stacksize - 1) / 16 + 1) * 16;
             sxg_kt_begin(SXG_KT_CLEAN);
-            hipLaunchKernelGGL(k_clean_pairs, dim3(sxg_div_up(n, 256)), dim3(256), 0, S.stream, o,
+            /* enough workgroups for every CU: split each window into up to 8 slices */
+            const int ntile = (win + CP_T - 1) / CP_T + 1;
+            const int slices = std::max(1, std::min(8, ntile));
+            hipLaunchKernelGGL(k_clean_pairs, dim3(sxg_div_up(n, 256), slices), dim3(256), 0, S.stream, o,
                                n, win, amp, alpha, o.master);
             sxg_kt_end(SXG_KT_CLEAN);
         }

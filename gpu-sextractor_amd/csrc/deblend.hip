@@ -735,10 +735,23 @@ __global__ void __launch_bounds__(256) k_deb_gather(ObjArgs A)
     A.oval[g] = p;
 }
 
+/* pixels per object; lanes with the same object (long runs: the list is grouped by raw
+   object) add their count once */
 __global__ void __launch_bounds__(256) k_deb_objcount(ObjArgs A, int *objcnt)
 {
     const long g = (long)blockIdx.x * 256 + threadIdx.x;
-    if (g < A.d.P0) atomicAdd(&objcnt[A.okey[g]], 1);
+    const int lane = threadIdx.x & 63;
+    bool act = g < A.d.P0;
+    const int key = act ? A.okey[g] : -1;
+    unsigned long long pending = __ballot(act);
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int k = __shfl(key, leader);
+        const unsigned long long grp = __ballot(act && key == k);
+        if (lane == leader) atomicAdd(&objcnt[k], __popcll(grp));
+        if (act && key == k) act = false;
+        pending &= ~grp;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_deb_zero(int *v, long n)

@@ -1,22 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X SExtractor detection path.
 
-One step = one pass of the whole path (background, filtering, detection, deblending,
-analysis, CLEAN, segmentation map) over one synthetic 4096x4096 float32 frame with
-~167k sources (the crowded single-frame workload: 60/40 Moffat stars / Sersic galaxies,
-dN/dF ~ F^-2 up to 1e6 ADU, 8% in close pairs/triples, gauss_3.0_7x7 filter,
-BACK_SIZE 64, BACK_FILTERSIZE 3, DETECT_THRESH 1.5, DETECT_MINAREA 5,
-DEBLEND_NTHRESH 32, DEBLEND_MINCONT 0.005, CLEAN Y with CLEAN_PARAM 1.0).
+One frame = one pass of the whole path (background, filtering, detection, deblending,
+analysis, CLEAN, segmentation map) over a synthetic 4096x4096 float32 frame with ~167k
+sources (the crowded single-frame workload: 60/40 Moffat stars / Sersic galaxies,
+dN/dF ~ F^-2 up to 1e6 ADU, 8% in close pairs/triples, gauss_3.0_7x7 filter, BACK_SIZE 64,
+BACK_FILTERSIZE 3, DETECT_THRESH 1.5, DETECT_MINAREA 5, DEBLEND_NTHRESH 32,
+DEBLEND_MINCONT 0.005, CLEAN Y with CLEAN_PARAM 1.0).
 
-The frame is resident in HBM before the timed region (sxg_set_resident_frame); each step
-runs sxg_process_resident, the C-ABI entry points in the order of the reference's
-scan.c.  value = Mpix/s over the K timed steps; objects/s is reported beside it.
-The roofline entry is for the kernel with the largest share of the timed region,
-measured with HIP events on the library's stream.  The CPU baseline is the oracle (the
-sequential C restatement) timed on one host core on the same frame; its segmentation
-map and object count are also compared with the GPU's (bit-exact expected).
+A step is one batch of B such frames (distinct seeds), resident in HBM before the timed
+region, run by the batched path: W concurrent contexts, one host thread and one HIP
+stream each (sxg_batch_process).  value = Mpix/s of the batched path over the K timed
+steps.  Beside it: the same B frames run frame by frame (one context: the plain
+drop-in sequence), and the single-frame loop on the default context, whose HIP-event
+kernel timers give the roofline entry for the kernel with the largest share.  The CPU
+baseline is the oracle (the sequential C restatement) timed on one host core on frame 0;
+its segmentation map and object count are compared with the GPU's, and every batched
+segmentation map is compared with the frame-by-frame one (bit-exact expected).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--workers W]
+                       [--no-cpu-baseline]
 """
 import argparse
 import ctypes as C
@@ -86,11 +89,36 @@ def cpu_baseline(img, p, gpu_seg, gpu_nobj):
             "objects_per_s": round(o["nsurv"] / dt, 1)}, parity
 
 
+def timed_batches(L, sp, nworkers, steps, B):
+    infos = (sexgpu.FrameInfo * B)()
+    L.sxg_batch_process(C.byref(sp), nworkers, infos, None)   # warm-up (contexts, buffers)
+    sexgpu.check()
+    ms = (C.c_double * B)()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if L.sxg_batch_process(C.byref(sp), nworkers, infos, ms) < 0:
+            sexgpu.check()
+    dt = time.perf_counter() - t0
+    return dt, list(infos), list(ms)
+
+
+def batch_segs(L, frames):
+    out = []
+    for i, f in enumerate(frames):
+        s = np.zeros(f.shape, np.int32)
+        if L.sxg_batch_segmap(i, s.ctypes.data) != 0:
+            sexgpu.check()
+        out.append(s)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=8, help="frames per step (default 8)")
+    ap.add_argument("--workers", type=int, default=4, help="concurrent contexts (default 4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--size", type=int, default=4096, help="frame side (default 4096)")
     args = ap.parse_args()
@@ -101,18 +129,22 @@ def main():
 
     t = time.perf_counter()
     nsrc = int(round(167000 * (args.size / 4096.0) ** 2))
-    img = sexgpu.synth("crowded4096", width=args.size, height=args.size, nsrc=nsrc)
+    B = args.batch
+    frames = [sexgpu.synth("crowded4096", width=args.size, height=args.size, nsrc=nsrc,
+                           seed=1001 + i) for i in range(B)]
+    img = frames[0]
     H, W = img.shape
     N = W * H
-    print("# frame %dx%d, %d sources, fnv64 %s, sha256 %s, generated in %.1f s"
-          % (W, H, nsrc, sexgpu.frame_hash(img),
-             hashlib.sha256(img.tobytes()).hexdigest()[:16], time.perf_counter() - t),
-          file=sys.stderr, flush=True)
+    print("# %d frames %dx%d, %d sources each, frame 0 fnv64 %s sha256 %s, generated in %.1f s"
+          % (B, W, H, nsrc, sexgpu.frame_hash(img), hashlib.sha256(img.tobytes()).hexdigest()[:16],
+             time.perf_counter() - t), file=sys.stderr, flush=True)
     p = workload_params()
+    sp = sexgpu.native_params(p)
     L = sexgpu.gpu()
+
+    # ---- single frame on the default context, with kernel timers
     L.sxg_set_resident_frame(img.ctypes.data, W, H)
     sexgpu.check()
-    sp = sexgpu.native_params(p)
     fi = sexgpu.FrameInfo()
     for _ in range(args.warmup):
         L.sxg_process_resident(C.byref(sp), C.byref(fi))
@@ -125,22 +157,48 @@ def main():
     for _ in range(args.steps):
         nobj = L.sxg_process_resident(C.byref(sp), C.byref(fi))
     L.sxg_sync()
-    t1 = time.perf_counter()
+    dt1 = time.perf_counter() - t0
     sexgpu.check()
-    dt = t1 - t0
     ktab = kernel_table(L)
     L.sxg_kernel_timing(0)
     info = {k: getattr(fi, k) for k, _ in sexgpu.FrameInfo._fields_}
-    ms_step = 1000.0 * dt / args.steps
-    mpix = N * args.steps / dt / 1e6
-    objs = info["nsurv"] * args.steps / dt
-    print("# objects: raw %d, deblended %d, catalogue %d; pixels above threshold %d, in raw "
-          "objects %d" % (info["nraw"], info["nobj"], info["nsurv"], info["npix_above"],
-                          info["npix_raw"]), file=sys.stderr)
-    print("# kernel time per step (ms): " + ", ".join(
+    seg0 = np.zeros((H, W), np.int32)
+    L.sxg_get_segmap(seg0.ctypes.data)
+    sexgpu.check()
+    single_ms = 1000.0 * dt1 / args.steps
+    print("# single frame: %.3f ms; objects: raw %d, deblended %d, catalogue %d; pixels above "
+          "threshold %d, in raw objects %d" % (single_ms, info["nraw"], info["nobj"],
+                                               info["nsurv"], info["npix_above"],
+                                               info["npix_raw"]), file=sys.stderr)
+    print("# kernel time per frame (ms): " + ", ".join(
         "%s %.3f" % (k, v[0] / args.steps) for k, v in sorted(ktab.items(), key=lambda kv: -kv[1][0])),
         file=sys.stderr, flush=True)
-    # dominant kernel with an algorithmic byte model
+
+    # ---- the batch: frame by frame (one context), then W concurrent contexts
+    ptrs = (C.c_void_p * B)(*[f.ctypes.data for f in frames])
+    ws = (C.c_int * B)(*[W] * B)
+    hs = (C.c_int * B)(*[H] * B)
+    if L.sxg_batch_set_frames(B, ptrs, ws, hs) != 0:
+        sexgpu.check()
+    dtp, infp, msp = timed_batches(L, sp, 1, max(1, args.steps // 4), B)
+    plain_steps = max(1, args.steps // 4)
+    segs_plain = batch_segs(L, frames)
+    dtb, infb, msb = timed_batches(L, sp, args.workers, args.steps, B)
+    segs_batch = batch_segs(L, frames)
+    batch_same = all(np.array_equal(a, b) for a, b in zip(segs_plain, segs_batch))
+    batch_same = batch_same and np.array_equal(segs_plain[0], seg0)
+    nsurv = sum(f.nsurv for f in infb)
+    mpix_plain = B * N * plain_steps / dtp / 1e6
+    mpix_batch = B * N * args.steps / dtb / 1e6
+    objs_batch = nsurv * args.steps / dtb
+    objs_plain = nsurv * plain_steps / dtp
+    lat = np.array(msb)
+    print("# batch of %d: frame by frame %.1f Mpix/s, %d workers %.1f Mpix/s (%.2fx); frame "
+          "latency p50 %.2f ms p99 %.2f ms" % (B, mpix_plain, args.workers, mpix_batch,
+                                                 mpix_batch / mpix_plain, np.percentile(lat, 50),
+                                                 np.percentile(lat, 99)), file=sys.stderr, flush=True)
+
+    # ---- roofline of the dominant kernel (single-frame run)
     dom = max((k for k in ktab if k in ALG_BYTES), key=lambda k: ktab[k][0])
     tot, nl = ktab[dom]
     avg_s = tot / nl / 1000.0
@@ -149,28 +207,35 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": dom, "kernel_ms": round(avg_s * 1e3, 4),
-                "share_of_step": round(tot / (dt * 1e3), 3)}
-    seg = np.zeros((H, W), np.int32)
-    # the segmentation map of the last timed step
-    L.sxg_get_segmap(seg.ctypes.data)
-    sexgpu.check()
+                "share_of_frame": round(tot / (dt1 * 1e3), 3)}
+    # whole path against the per-frame algorithmic bytes 12N + 12 P_all
+    balg = 12 * N + 12 * info["npix_above"]
     cpu, parity = (None, None)
     if not args.no_cpu_baseline:
-        cpu, parity = cpu_baseline(img, p, seg, nobj)
+        cpu, parity = cpu_baseline(img, p, seg0, nobj)
     line = {
         "metric": "Mpix/s + objects/s at 4096^2 (~167k obj), batched vs plain; bit-exact seg map",
-        "value": round(mpix, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "crowded %dx%d frame, %d sources, gauss_3.0_7x7, plain path "
-                               "(frame resident in HBM)" % (W, H, nsrc),
-                   "frame": [W, H], "objects_per_s": round(objs, 1),
-                   "objects": info["nsurv"], "deblended": info["nobj"],
-                   "pixels_above_threshold": info["npix_above"], "path": "plain",
-                   "parity_with_oracle": parity},
+        "value": round(mpix_batch, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1000.0 * dtb / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "crowded %dx%d frames, %d sources each (seeds 1001..%d), "
+                               "gauss_3.0_7x7, resident in HBM; step = batch of %d frames on %d "
+                               "concurrent contexts" % (W, H, nsrc, 1000 + B, B, args.workers),
+                   "frame": [W, H], "batch": B, "workers": args.workers,
+                   "objects_per_s": round(objs_batch, 1),
+                   "plain_mpix_s": round(mpix_plain, 2), "plain_objects_per_s": round(objs_plain, 1),
+                   "single_frame_ms": round(single_ms, 3),
+                   "frame_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                   "frame_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                   "objects_per_frame": info["nsurv"], "deblended_per_frame": info["nobj"],
+                   "pixels_above_threshold": info["npix_above"],
+                   "whole_path_hbm_frac": round(balg / (dtb / args.steps / B) / 1e12 / 8.0, 5),
+                   "parity_with_oracle": parity, "batched_equals_plain": bool(batch_same)},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    L.sxg_batch_release()
     return 0
 
 

@@ -761,8 +761,6 @@ __global__ void __launch_bounds__(256) k_deb_zero(int *v, long n)
 }
 
 /* --------------------------------------------------------------- host side ---- */
-static bool g_deb_ready = false;
-
 extern "C" void init_deblend(float basethresh, int nthresh, size_t numValidObj, size_t numValidPix)
 {
     SxgState &S = sxg_state();
@@ -792,7 +790,7 @@ extern "C" void init_deblend(float basethresh, int nthresh, size_t numValidObj, 
         sxg_kt_end(SXG_KT_DEB_THRESH);
     }
     SXG_HIP(hipGetLastError());
-    g_deb_ready = true;
+    S.deb_ready = true;
 }
 
 __global__ void __launch_bounds__(256) k_deb_capacity(const int *cnt, int nraw, int nthresh,
@@ -830,7 +828,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
                           size_t numValidPix, double deblend_mincont)
 {
     SxgState &S = sxg_state();
-    if (!S.have_frame || !g_deb_ready || n_thresh != S.nthresh) {
+    if (!S.have_frame || !S.deb_ready || n_thresh != S.nthresh) {
         sxg_set_error("parcel_out: call init_deblend first with the same DEBLEND_NTHRESH");
         return -1;
     }
@@ -982,5 +980,5 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
 extern "C" void clear_deblend(int nthresh)
 {
     (void)nthresh;
-    g_deb_ready = false;
+    sxg_state().deb_ready = false;
 }

@@ -108,8 +108,6 @@ __global__ void __launch_bounds__(CV_THREADS) k_convolve_any(const float *__rest
         }
 }
 
-static float *g_dmask = nullptr;
-
 /* conv(in -> out) with a square K x K mask already on the device */
 void sxg_convolve(const float *in, float *out, int W, int H, const float *dmask, int K,
                   hipStream_t s)
@@ -140,11 +138,12 @@ extern "C" void cudaFilter(float *h_mask, unsigned int mask_width)
         sxg_set_error("cudaFilter: mask width must be 1..31");
         return;
     }
-    if (!g_dmask) SXG_HIP(hipMalloc(&g_dmask, sizeof(float) * SXG_MAXMASK * SXG_MAXMASK));
-    SXG_HIP(hipMemcpyAsync(g_dmask, h_mask, sizeof(float) * K * K, hipMemcpyHostToDevice, S.stream));
+    if (!S.dmask.ensure(sizeof(float) * SXG_MAXMASK * SXG_MAXMASK)) return;
+    float *dmask = S.dmask.as<float>();
+    SXG_HIP(hipMemcpyAsync(dmask, h_mask, sizeof(float) * K * K, hipMemcpyHostToDevice, S.stream));
     sxg_stage_begin(ST_FILTER);
     sxg_kt_begin(SXG_KT_CONV);
-    sxg_convolve(S.img.as<float>(), S.cd.as<float>(), S.W, S.H, g_dmask, K, S.stream);
+    sxg_convolve(S.img.as<float>(), S.cd.as<float>(), S.W, S.H, dmask, K, S.stream);
     sxg_kt_end(SXG_KT_CONV);
     sxg_stage_end(ST_FILTER);
     S.filtered = true;

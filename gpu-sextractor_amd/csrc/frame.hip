@@ -1,13 +1,22 @@
 /*
- * frame.hip -- whole-frame driver and kernel timers.
+ * frame.hip -- whole-frame driver, the batched path and kernel timers.
  *
- * sxg_process_resident runs the path in the order of the reference's scan.c:78-146
+ * process_frame runs the path in the order of the reference's scan.c:78-146
  * (back_estimate, cudaFilter, run_detection, init_deblend, parcel_out, analyse_gpu,
- * run_clean, segmentation map) on a frame already resident in device memory, so a
- * benchmark or a caller that keeps frames on the GPU pays no PCIe transfer and no host
- * round trip besides the object counts the C-ABI itself returns.
+ * run_clean, segmentation map) on a frame already resident in device memory, on the
+ * calling thread's context, so a caller that keeps frames on the GPU pays no PCIe
+ * transfer and no host round trip besides the object counts the C-ABI itself returns.
+ *
+ * The batched path keeps many frames resident and runs them on several contexts at
+ * once, one host worker thread and one stream per context: the stages of one frame are
+ * mostly latency-bound (sequential per-object sums, per-level barriers, host
+ * synchronisations on counts), so frames in flight together fill the GPU.
  */
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "sxg_internal.h"
@@ -23,7 +32,7 @@ static bool g_kt_on = false;
 
 void sxg_kt_begin(int id)
 {
-    if (!g_kt_on) return;
+    if (!g_kt_on || !sxg_state().is_default) return;
     KTimer &t = g_kt[id];
     if ((int)t.ev.size() < 2 * (t.used + 1)) {
         hipEvent_t a, b;
@@ -36,7 +45,7 @@ void sxg_kt_begin(int id)
 
 void sxg_kt_end(int id)
 {
-    if (!g_kt_on) return;
+    if (!g_kt_on || !sxg_state().is_default) return;
     KTimer &t = g_kt[id];
     if ((int)t.ev.size() < 2 * (t.used + 1)) return;
     (void)hipEventRecord(t.ev[2 * t.used + 1], sxg_state().stream);
@@ -67,21 +76,19 @@ extern "C" int sxg_kernel_times(int id, double *total_ms, int *launches)
 }
 
 /* ---------------------------------------------------------- resident frame ---- */
-static DevBuf g_resident;
-static int g_rw = 0, g_rh = 0;
-
 extern "C" int sxg_set_resident_frame(const float *host, int width, int height)
 {
+    SxgState &S = sxg_state();
     sxg_clear_error();
-    if (width <= 0 || height <= 0 || !host) {
+    if (width <= 0 || height <= 0 || (long)width * height >= 2147483647L || !host) {
         sxg_set_error("sxg_set_resident_frame: bad frame");
         return -1;
     }
     const size_t n = (size_t)width * height;
-    if (!g_resident.ensure(n * sizeof(float))) return -1;
-    SXG_HIP_RET(hipMemcpy(g_resident.ptr, host, n * sizeof(float), hipMemcpyHostToDevice), -1);
-    g_rw = width;
-    g_rh = height;
+    if (!S.resident.ensure(n * sizeof(float))) return -1;
+    SXG_HIP_RET(hipMemcpy(S.resident.ptr, host, n * sizeof(float), hipMemcpyHostToDevice), -1);
+    S.rw = width;
+    S.rh = height;
     return 0;
 }
 
@@ -91,35 +98,27 @@ __global__ void __launch_bounds__(256) k_copy_frame(const float4 *a, float4 *b, 
     if (k < n4) b[k] = a[k];
 }
 
-extern "C" int sxg_process_resident(const sxg_params *p, sxg_frame_info *info)
+/* the whole path on a device frame (not modified), on the current context */
+static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
+                         sxg_frame_info *info)
 {
     SxgState &S = sxg_state();
-    sxg_clear_error();
-    if (!g_resident.ptr || !p) {
-        sxg_set_error("sxg_process_resident: no resident frame");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        sxg_set_error("no HIP device visible");
         return -1;
     }
+    if (!sxg_ensure_stream()) return -1;
     /* init_device from device memory: the working copy is overwritten in place */
-    static std::vector<float> dummy(1);
     {
-        const int W = g_rw, H = g_rh;
         const size_t N = (size_t)W * H;
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-            sxg_set_error("no HIP device visible");
-            return -1;
-        }
-        if (!S.stream) {
-            /* first use: init_device creates the stream; give it a 1-pixel frame */
-            init_device(1, 1, dummy.data());
-            if (sxg_has_error()) return -1;
-        }
         S.W = W;
         S.H = H;
         S.N = N;
         S.nraw = S.nobj = S.nsurv = 0;
         S.cleaned = false;
         S.filtered = false;
+        S.deb_ready = false;
         for (int i = 0; i < ST_N; i++) S.ev_used[i] = false;
         if (!S.img.ensure(N * sizeof(float)) || !S.cd.ensure(N * sizeof(float)) ||
             !S.lab.ensure(N * sizeof(int)) || !S.aux.ensure(N * sizeof(int)))
@@ -127,16 +126,15 @@ extern "C" int sxg_process_resident(const sxg_params *p, sxg_frame_info *info)
         if ((N & 3) == 0) {
             sxg_kt_begin(SXG_KT_COPY);
             hipLaunchKernelGGL(k_copy_frame, dim3(sxg_div_up((long)(N / 4), 256)), dim3(256), 0,
-                               S.stream, (const float4 *)g_resident.ptr, (float4 *)S.img.ptr,
+                               S.stream, (const float4 *)dframe, (float4 *)S.img.ptr,
                                (long)(N / 4));
             sxg_kt_end(SXG_KT_COPY);
         } else {
-            SXG_HIP_RET(hipMemcpyAsync(S.img.ptr, g_resident.ptr, N * sizeof(float),
+            SXG_HIP_RET(hipMemcpyAsync(S.img.ptr, dframe, N * sizeof(float),
                                        hipMemcpyDeviceToDevice, S.stream), -1);
         }
         S.have_frame = true;
     }
-    const int W = S.W, H = S.H;
     const int bw = p->back_size < W ? p->back_size : W;
     const int bh = p->back_size < H ? p->back_size : H;
     if (bw != bh) {
@@ -180,8 +178,145 @@ extern "C" int sxg_process_resident(const sxg_params *p, sxg_frame_info *info)
     return nobj;
 }
 
+extern "C" int sxg_process_resident(const sxg_params *p, sxg_frame_info *info)
+{
+    SxgState &S = sxg_state();
+    sxg_clear_error();
+    if (!S.resident.ptr || !p) {
+        sxg_set_error("sxg_process_resident: no resident frame");
+        return -1;
+    }
+    return process_frame(S.resident.as<float>(), S.rw, S.rh, p, info);
+}
+
 extern "C" void sxg_sync(void)
 {
     SxgState &S = sxg_state();
     if (S.stream) (void)hipStreamSynchronize(S.stream);
+}
+
+/* ---------------------------------------------------------------- batches ---- */
+struct BatchFrame {
+    DevBuf frame;       /* float[W*H] input, resident */
+    DevBuf seg;         /* int32[W*H] segmentation map of the last run */
+    int W = 0, H = 0;
+};
+static std::vector<BatchFrame> g_batch;
+static std::vector<sxg_ctx *> g_workers;
+
+extern "C" int sxg_batch_set_frames(int n, const float *const *host, const int *width,
+                                    const int *height)
+{
+    sxg_clear_error();
+    if (n < 0 || (n > 0 && (!host || !width || !height))) {
+        sxg_set_error("sxg_batch_set_frames: bad arguments");
+        return -1;
+    }
+    for (auto &f : g_batch) {
+        f.frame.release();
+        f.seg.release();
+    }
+    g_batch.assign(n, BatchFrame());
+    for (int i = 0; i < n; i++) {
+        const int W = width[i], H = height[i];
+        if (W <= 0 || H <= 0 || (long)W * H >= 2147483647L || !host[i]) {
+            sxg_set_error("sxg_batch_set_frames: bad frame " + std::to_string(i));
+            return -1;
+        }
+        const size_t N = (size_t)W * H;
+        if (!g_batch[i].frame.ensure(N * sizeof(float)) || !g_batch[i].seg.ensure(N * sizeof(int)))
+            return -1;
+        SXG_HIP_RET(hipMemcpy(g_batch[i].frame.ptr, host[i], N * sizeof(float),
+                              hipMemcpyHostToDevice), -1);
+        g_batch[i].W = W;
+        g_batch[i].H = H;
+    }
+    return 0;
+}
+
+extern "C" int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_info *info,
+                                 double *frame_ms)
+{
+    sxg_clear_error();
+    const int n = (int)g_batch.size();
+    if (!p || nworkers < 1 || nworkers > SXG_MAX_WORKERS) {
+        sxg_set_error("sxg_batch_process: nworkers must be 1.." + std::to_string(SXG_MAX_WORKERS));
+        return -1;
+    }
+    while ((int)g_workers.size() < nworkers) {
+        sxg_ctx *c = sxg_ctx_create();
+        if (!c) {
+            sxg_set_error("sxg_batch_process: context allocation failed");
+            return -1;
+        }
+        g_workers.push_back(c);
+    }
+    std::atomic<int> next(0), failed(0);
+    std::vector<std::string> errs(nworkers);
+    auto work = [&](int w) {
+        sxg_ctx_bind(g_workers[w]);
+        sxg_clear_error();
+        for (int i = next++; i < n && !failed; i = next++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            BatchFrame &f = g_batch[i];
+            sxg_frame_info fi;
+            if (process_frame(f.frame.as<float>(), f.W, f.H, p, &fi) < 0) {
+                failed = 1;
+                break;
+            }
+            SxgState &S = sxg_state();
+            int *map = sxg_take<int>("out.map", S.N);
+            if (!map || hipMemcpyAsync(f.seg.ptr, map, S.N * sizeof(int), hipMemcpyDeviceToDevice,
+                                       S.stream) != hipSuccess ||
+                hipStreamSynchronize(S.stream) != hipSuccess) {
+                sxg_set_error("sxg_batch_process: segmentation copy failed");
+                failed = 1;
+                break;
+            }
+            if (info) info[i] = fi;
+            if (frame_ms)
+                frame_ms[i] = std::chrono::duration<double, std::milli>(
+                                  std::chrono::steady_clock::now() - t0).count();
+        }
+        if (sxg_has_error()) errs[w] = sxg_last_error();
+        sxg_ctx_bind(nullptr);
+    };
+    /* every worker is a thread of its own, so the caller's context binding is untouched */
+    std::vector<std::thread> th;
+    for (int w = 0; w < nworkers; w++) th.emplace_back(work, w);
+    for (auto &t : th) t.join();
+    if (failed) {
+        for (auto &e : errs)
+            if (!e.empty()) {
+                sxg_set_error(e);
+                break;
+            }
+        if (!sxg_has_error()) sxg_set_error("sxg_batch_process: a frame failed");
+        return -1;
+    }
+    return n;
+}
+
+extern "C" int sxg_batch_segmap(int frame, int32_t *out)
+{
+    sxg_clear_error();
+    if (frame < 0 || frame >= (int)g_batch.size() || !out) {
+        sxg_set_error("sxg_batch_segmap: bad frame index");
+        return -1;
+    }
+    const BatchFrame &f = g_batch[frame];
+    SXG_HIP_RET(hipMemcpy(out, f.seg.ptr, (size_t)f.W * f.H * sizeof(int), hipMemcpyDeviceToHost),
+                -1);
+    return 0;
+}
+
+extern "C" void sxg_batch_release(void)
+{
+    for (auto &f : g_batch) {
+        f.frame.release();
+        f.seg.release();
+    }
+    g_batch.clear();
+    for (sxg_ctx *c : g_workers) sxg_ctx_destroy(c);
+    g_workers.clear();
 }

@@ -8,6 +8,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
 
 #include "sexgpu.h"
@@ -110,6 +111,14 @@ struct SxgState {
     float deb_base = 0.0f;
     double mag_zeropoint = 0.0;
     float stage_ms[ST_N] = {};
+
+    /* per-context workspace */
+    std::map<std::string, DevBuf> named;   /* named device buffers (sxg_take) */
+    bool deb_ready = false;                /* init_deblend ran for this frame */
+    DevBuf dmask;                          /* convolution mask */
+    DevBuf resident;                       /* frame kept in HBM (sxg_set_resident_frame) */
+    int rw = 0, rh = 0;
+    bool is_default = false;               /* the process-wide default context */
 };
 
 /* raw (level-0) objects, structure of arrays */
@@ -146,6 +155,7 @@ int sxg_radix_sort(int *k0, int *v0, int *k1, int *v1, int n, int nbits, int *hi
 /* Grow-only device buffers addressed by name (detection/deblending scratch). */
 DevBuf &sxg_buf(const char *name);
 void sxg_release_named();
+bool sxg_ensure_stream();   /* the current context's stream and events */
 /* Typed view of a named buffer holding at least n elements (nullptr on failure). */
 template <class T> inline T *sxg_take(const char *name, size_t n)
 {

@@ -10,8 +10,9 @@
 #include "sxg_internal.h"
 
 /* ---------------------------------------------------------------- errors ---- */
-static std::string g_err;
-static bool g_has_err = false;
+/* per calling thread: contexts may run on several host threads at once */
+static thread_local std::string g_err;
+static thread_local bool g_has_err = false;
 
 void sxg_set_error(const std::string &msg)
 {
@@ -65,13 +66,50 @@ void DevBuf::release()
     bytes = 0;
 }
 
-SxgState &sxg_state()
+/* the context bound to the calling thread (sxg_ctx_bind), else the default one */
+static thread_local SxgState *tl_ctx = nullptr;
+
+static SxgState &default_state()
 {
     static SxgState st;
+    st.is_default = true;
     return st;
 }
 
-static bool ensure_stream()
+SxgState &sxg_state() { return tl_ctx ? *tl_ctx : default_state(); }
+
+struct sxg_ctx {
+    SxgState st;
+};
+
+extern "C" sxg_ctx *sxg_ctx_create(void)
+{
+    sxg_clear_error();
+    return new (std::nothrow) sxg_ctx();
+}
+
+extern "C" void sxg_ctx_destroy(sxg_ctx *c)
+{
+    if (!c) return;
+    SxgState *prev = tl_ctx;
+    tl_ctx = &c->st;
+    sxg_release();
+    if (c->st.stream) {
+        for (int i = 0; i < 2 * ST_N; i++)
+            if (c->st.ev[i]) (void)hipEventDestroy(c->st.ev[i]);
+        (void)hipStreamDestroy(c->st.stream);
+    }
+    tl_ctx = prev == &c->st ? nullptr : prev;
+    delete c;
+}
+
+extern "C" int sxg_ctx_bind(sxg_ctx *c)
+{
+    tl_ctx = c ? &c->st : nullptr;
+    return 0;
+}
+
+bool sxg_ensure_stream()
 {
     SxgState &S = sxg_state();
     if (S.stream) return true;
@@ -123,6 +161,8 @@ extern "C" void sxg_release(void)
                       &S.scan_tmp, &S.thr, &S.lvl, &S.comps, &S.objs, &S.rootobj};
     for (DevBuf *b : bufs) b->release();
     sxg_release_named();
+    S.dmask.release();
+    S.resident.release();
     S.have_frame = false;
 }
 
@@ -279,16 +319,10 @@ void sxg_compact_nonneg(const int *v, int *out, int n, int *count, hipStream_t s
 }
 
 /* ------------------------------------------------------- named buffers ---- */
-#include <map>
-static std::map<std::string, DevBuf> &named()
-{
-    static std::map<std::string, DevBuf> bufs;
-    return bufs;
-}
-DevBuf &sxg_buf(const char *name) { return named()[name]; }
+DevBuf &sxg_buf(const char *name) { return sxg_state().named[name]; }
 void sxg_release_named()
 {
-    for (auto &kv : named()) kv.second.release();
+    for (auto &kv : sxg_state().named) kv.second.release();
 }
 
 bool sxg_raw_alloc(int nraw, RawObj &ro)
@@ -321,7 +355,7 @@ extern "C" void init_device(int width, int height, float *imgbuf)
         sxg_set_error("init_device: no HIP device visible");
         return;
     }
-    if (!ensure_stream()) return;
+    if (!sxg_ensure_stream()) return;
     S.W = width;
     S.H = height;
     S.N = (size_t)width * height;

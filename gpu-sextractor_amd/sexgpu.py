@@ -239,6 +239,14 @@ def gpu():
         L.sxg_kernel_timing.argtypes = [i]
         L.sxg_kernel_times.argtypes = [i, C.POINTER(d), C.POINTER(i)]
         L.sxg_set_mag_zeropoint.argtypes = [d]
+        L.sxg_ctx_create.restype = vp
+        L.sxg_ctx_create.argtypes = []
+        L.sxg_ctx_destroy.argtypes = [vp]
+        L.sxg_ctx_bind.argtypes = [vp]
+        L.sxg_batch_set_frames.argtypes = [i, vp, vp, vp]
+        L.sxg_batch_process.argtypes = [vp, i, vp, vp]
+        L.sxg_batch_segmap.argtypes = [i, vp]
+        L.sxg_batch_release.argtypes = []
         L.sxg_set_option.argtypes = [i, C.c_long]
         L.sxg_get_option.argtypes = [i]
         L.sxg_get_option.restype = C.c_long
@@ -259,7 +267,9 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_get_back_maps", "sxg_get_filtered", "sxg_get_alloc_map", "sxg_device_frame",
             "sxg_stage_times", "sxg_release", "sxg_set_mag_zeropoint",
             "sxg_set_resident_frame", "sxg_process_resident", "sxg_sync", "sxg_kernel_timing",
-            "sxg_kernel_times", "sxg_set_option", "sxg_get_option"]
+            "sxg_kernel_times", "sxg_set_option", "sxg_get_option", "sxg_ctx_create",
+            "sxg_ctx_destroy", "sxg_ctx_bind", "sxg_batch_set_frames", "sxg_batch_process",
+            "sxg_batch_segmap", "sxg_batch_release"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort"]
@@ -558,3 +568,30 @@ def process_resident(img, p):
     n = L.sxg_process_resident(C.byref(sp), C.byref(fi))
     check()
     return n, {k: getattr(fi, k) for k, _ in FrameInfo._fields_}
+
+
+def process_batch(frames, p, nworkers=4, segmaps=True):
+    """Run the whole path on a list of float32 frames (uneven sizes allowed) with
+    nworkers concurrent contexts; returns (infos, frame_ms, seg maps or None)."""
+    L = gpu()
+    frames = [np.ascontiguousarray(f, dtype=np.float32) for f in frames]
+    n = len(frames)
+    ptrs = (C.c_void_p * n)(*[f.ctypes.data for f in frames])
+    ws = (C.c_int * n)(*[f.shape[1] for f in frames])
+    hs = (C.c_int * n)(*[f.shape[0] for f in frames])
+    if L.sxg_batch_set_frames(n, ptrs, ws, hs) != 0:
+        check()
+    sp = native_params(p)
+    infos = (FrameInfo * n)()
+    ms = (C.c_double * n)()
+    if L.sxg_batch_process(C.byref(sp), nworkers, infos, ms) < 0:
+        check()
+    segs = None
+    if segmaps:
+        segs = []
+        for i, f in enumerate(frames):
+            s = np.zeros(f.shape, np.int32)
+            if L.sxg_batch_segmap(i, s.ctypes.data) != 0:
+                check()
+            segs.append(s)
+    return list(infos), list(ms), segs

@@ -184,7 +184,34 @@ int sxg_process_resident(const sxg_params *p, sxg_frame_info *info);
 /* Wait for the library's stream. */
 void sxg_sync(void);
 
-/* Per-kernel HIP-event timers (off by default; enabling resets them). */
+/* ---- contexts and batches (no counterpart in the reference) -------------------
+   A context holds everything the entry points above keep between calls (frame buffers,
+   workspace, stream).  The calling thread uses the context it bound last, or the
+   process-wide default context: several host threads, each bound to its own context,
+   can run the whole path on different frames at the same time.  Errors are per
+   thread. */
+typedef struct sxg_ctx sxg_ctx;
+sxg_ctx *sxg_ctx_create(void);
+void sxg_ctx_destroy(sxg_ctx *ctx);
+/* Bind ctx to the calling thread (NULL: the default context).  Returns 0. */
+int sxg_ctx_bind(sxg_ctx *ctx);
+
+#define SXG_MAX_WORKERS 16
+/* Copy n frames into device memory once (replacing any previous batch). */
+int sxg_batch_set_frames(int n, const float *const *host, const int *width, const int *height);
+/* Run the whole path on every resident frame with nworkers concurrent contexts, one
+   host thread and one stream each.  info[i] and frame_ms[i] (host wall time of frame i
+   from its start to its segmentation map) may be NULL.  The segmentation map of each
+   frame stays on the device.  Returns the number of frames or < 0. */
+int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_info *info,
+                      double *frame_ms);
+/* Segmentation map of frame i of the last sxg_batch_process. */
+int sxg_batch_segmap(int frame, int32_t *out);
+/* Free the batch and its worker contexts. */
+void sxg_batch_release(void);
+
+/* Per-kernel HIP-event timers for the default context (off by default; enabling resets
+   them). */
 enum sxg_kernel_id {
     SXG_KT_COPY = 0, SXG_KT_MESHSTAT, SXG_KT_MESHMODE, SXG_KT_FILTERBACK, SXG_KT_ROWNODES,
     SXG_KT_SUBBACK, SXG_KT_CONV, SXG_KT_DET_INIT, SXG_KT_DET_FG, SXG_KT_DET_UNION,

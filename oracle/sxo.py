@@ -116,7 +116,8 @@ OBJ_INT = ["level", "root", "firstpix", "flag", "fdnpix", "dnpix", "npix", "xmin
            "ymin", "ymax", "peakx", "peaky"]
 OBJ_DBL = ["mx", "my", "mx2", "my2", "mxy", "poserr_mx2", "poserr_my2", "poserr_mxy"]
 OBJ_FLT = ["a", "b", "theta", "cxx", "cyy", "cxy", "abcor", "fdflux", "dflux", "flux",
-           "fluxerr", "fdpeak", "dpeak", "peak", "bkg", "thresh_obj", "mthresh", "fwhm"]
+           "fluxerr", "fdpeak", "dpeak", "peak", "bkg", "thresh_obj", "mthresh", "fwhm",
+           "dthresh_obj"]
 
 
 def _arr(ptr, n, dtype):
@@ -138,8 +139,9 @@ def config(**kw):
     return c
 
 
-def run(img, mask=None, **kw):
-    """Run the whole oracle path on a float32 frame; returns a dict of numpy arrays."""
+def run(img, mask=None, weight=None, **kw):
+    """Run the whole oracle path on a float32 frame (and an optional MAP_WEIGHT map);
+    returns a dict of numpy arrays."""
     img = np.ascontiguousarray(img, dtype=np.float32)
     H, W = img.shape
     cfg = config(**kw)
@@ -148,6 +150,12 @@ def run(img, mask=None, **kw):
         keep = np.ascontiguousarray(mask, dtype=np.float32)
         cfg.mask = keep.ctypes.data
         cfg.mask_h, cfg.mask_w = keep.shape
+    wkeep = None
+    if weight is not None:
+        wkeep = np.ascontiguousarray(weight, dtype=np.float32)
+        if wkeep.shape != img.shape:
+            raise ValueError("weight map and frame differ in shape")
+        cfg.weight = wkeep.ctypes.data
     rp = lib().sxo_run(img.ctypes.data, W, H, C.byref(cfg))
     if not rp:
         raise RuntimeError("oracle rejected the configuration")
@@ -160,6 +168,7 @@ def run(img, mask=None, **kw):
         out[k] = _arr(getattr(r, k), nb, np.float32).reshape(r.nby, r.nbx)
     out["sub"] = _arr(r.sub, N, np.float32).reshape(H, W)
     out["cd"] = _arr(r.cd, N, np.float32).reshape(H, W)
+    out["var"] = _arr(r.var, N, np.float32).reshape(H, W) if r.var else None
     out["alloc"] = _arr(r.alloc, N, np.int32).reshape(H, W)
     out["seg"] = _arr(r.seg, N, np.int32).reshape(H, W)
     n = r.nraw

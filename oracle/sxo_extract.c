@@ -263,10 +263,35 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
     r->alloc = (int *)malloc(sizeof(int) * N);
     r->seg = (int *)calloc(N, sizeof(int));
 
+    /* ---- weight map: variance and mesh mask (weight.c:111-116 with sigfac = 1,
+       weight.c:80); pixels of non-positive weight stay out of the mesh statistics ---- */
+    const float *wmap = cfg->weight;
+    float *masked = NULL;
+    if (wmap) {
+        masked = (float *)malloc(sizeof(float) * N);
+        r->var = (float *)malloc(sizeof(float) * N);
+        for (size_t p = 0; p < N; p++) {
+            const int good = wmap[p] > 0.0f;
+            masked[p] = good ? img[p] : (float)-SXO_BIG;
+            r->var[p] = good ? 1.0f / wmap[p] : (float)SXO_BIG;
+        }
+    }
     /* ---- background and thresholds (back.c:43-208) ---- */
-    sxo_background(img, W, H, bs, cfg->back_fsize, cfg->back_fsize, cfg->back_fthresh,
-                   cfg->back_type, cfg->back_value, r->sub, r->back, r->sigma, r->dback,
-                   r->dsigma, &r->backmean, &r->backsig);
+    sxo_background(masked ? masked : img, W, H, bs, cfg->back_fsize, cfg->back_fsize,
+                   cfg->back_fthresh, cfg->back_type, cfg->back_value, r->sub, r->back,
+                   r->sigma, r->dback, r->dsigma, &r->backmean, &r->backsig);
+    if (masked) {
+        /* the maps come from the masked frame; the subtraction applies to the frame */
+        memcpy(r->sub, img, sizeof(float) * N);
+        if (cfg->back_type == 1) {
+            const float bval = (float)cfg->back_value;
+            for (size_t k = 0; k < N; k++) r->sub[k] -= bval;
+        } else {
+            for (int y = 0; y < H; y++)
+                sxo_subbackline(r->back, r->dback, nbx, nby, bs, W, y, r->sub + (size_t)y * W, NULL);
+        }
+        free(masked);
+    }
     if (cfg->thresh_type == 1) {
         r->dthresh = (float)cfg->detect_thresh;
         r->thresh = (float)cfg->analysis_thresh;
@@ -275,6 +300,11 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         r->thresh = (float)(cfg->analysis_thresh * r->backsig);
     }
     const float dthresh = r->dthresh, thresh = r->thresh;
+    /* per-pixel thresholds: with a weight map, relative thresholds are in units of the
+       local noise sqrt(var) */
+    const int wrel = wmap && cfg->thresh_type != 1;
+#define DTH(p) (wrel ? (float)(cfg->detect_thresh * sqrt((double)r->var[p])) : dthresh)
+#define ATH(p) (wrel ? (float)(cfg->analysis_thresh * sqrt((double)r->var[p])) : thresh)
 
     /* ---- filtering (filter.c:39-94) ---- */
     if (cfg->mask)
@@ -287,7 +317,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
     int *par = (int *)malloc(sizeof(int) * N);
     long nabove = 0;
     for (size_t p = 0; p < N; p++) {
-        if (cd[p] >= dthresh) {
+        if (cd[p] >= DTH(p)) {
             par[p] = (int)p;
             nabove++;
         } else
@@ -352,16 +382,24 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         free(fill);
     }
     free(cnt);
-    /* fdpeak, fdflux per raw object (cudadetection.cu:899-1090; sequential raster sum) */
+    /* fdpeak, fdflux per raw object (cudadetection.cu:899-1090; sequential raster sum);
+       the object's base threshold is the lowest detection threshold among its pixels
+       (the global one without a weight map) */
+    float *rawt0 = (float *)malloc(sizeof(float) * (nraw + 1));
     for (int k = 0; k < nraw; k++) {
-        float pk = (float)-SXO_BIG, fl = 0.0f;
+        float pk = (float)-SXO_BIG, fl = 0.0f, t0 = dthresh;
         for (int i = rawoff[k]; i < rawoff[k + 1]; i++) {
             float v = cd[fpia[i]];
             if (v > pk) pk = v;
             fl += v;
+            if (wrel) {
+                const float t = DTH(fpia[i]);
+                if (i == rawoff[k] || t < t0) t0 = t;
+            }
         }
         r->raw_fdpeak[k] = pk;
         r->raw_fdflux[k] = fl;
+        rawt0[k] = t0;
     }
 
     /* ---- deblending, preanalyse_full and gatherup, one root at a time ---- */
@@ -408,8 +446,8 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         for (int i = 0; i < gw * gh; i++) grid[i] = -1;
 
         /* sub-thresholds (cudadeblend.cu:199-217) */
-        const double t0 = (double)dthresh, pk = (double)r->raw_fdpeak[k];
-        float run = dthresh;
+        const double t0 = (double)rawt0[k], pk = (double)r->raw_fdpeak[k];
+        float run = rawt0[k];
         for (int i = 1; i < nthr; i++) {
             thr[i] = (float)(t0 * sx_pow(pk / t0, (double)i / nthr));
             /* a pixel enters level i only through the kept pixels of level i-1, so the
@@ -421,7 +459,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         int nc = 0;
         if (ccap < 64) { ccap = 64; comps = (comp_t *)realloc(comps, sizeof(comp_t) * ccap); }
         comps[0].level = 0; comps[0].parent = -1; comps[0].first = P[0]; comps[0].count = n;
-        comps[0].fdflux = r->raw_fdflux[k]; comps[0].dthresh = dthresh;
+        comps[0].fdflux = r->raw_fdflux[k]; comps[0].dthresh = rawt0[k];
         nc = 1;
         /* lvl[l*n + i]: component of pixel i at level l (-1 if none) */
         for (int i = 0; i < n; i++) lvl[i] = 0;
@@ -607,10 +645,13 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         const int *pl = r->objpix + r->objpix_start[o];
         const int n = r->objpix_start[o + 1] - r->objpix_start[o];
         pre_t q = pre[o];
+        /* the object's detection threshold: its raw object's base threshold (the global
+           one without a weight map) */
+        const float odth = rawt0[o_root[o]];
         /* preanalyse_robust for level > 0 (cudaanalyse.cu:1906-2126; dthresh reset to the
            detection threshold, cudadeblend.cu:1402-1413) */
         if (o_level[o] > 0)
-            preanalyse(pl, n, sub, cd, W, H, (double)dthresh, pre[o].flag, 1, pre[o].mx,
+            preanalyse(pl, n, sub, cd, W, H, (double)odth, pre[o].flag, 1, pre[o].mx,
                        pre[o].my, &q);
         r->level[o] = o_level[o];
         r->root[o] = o_root[o];
@@ -622,7 +663,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         r->a[o] = q.a; r->b[o] = q.b; r->theta[o] = q.theta;
         r->cxx[o] = q.cxx; r->cyy[o] = q.cyy; r->cxy[o] = q.cxy; r->abcor[o] = q.abcor;
         r->fdflux[o] = q.fdflux; r->dflux[o] = q.dflux; r->fdpeak[o] = q.fdpeak; r->dpeak[o] = q.dpeak;
-        r->dthresh_obj[o] = dthresh;
+        r->dthresh_obj[o] = odth;
 
         /* examineiso (cudaanalyse.cu:2176-2581), CCD mode, GLOBAL background */
         float bkg = back_at(r->back, (int)(q.mx + 0.5), (int)(q.my + 0.5), nbx, nby, bs, bs);
@@ -632,6 +673,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         const double var = backnoise2, dbacknoise2 = backnoise2;
         float peak = (float)-SXO_BIG, cdpeak = (float)-SXO_BIG, pix, cdpix;
         int area = 0, peakx = 0, peaky = 0, nfill = 0;
+        float othr = thresh;   /* the object's analysis threshold: lowest among its pixels */
         int cleanflag = cfg->clean;
         if (n < hsize) cleanflag = 0;
         for (int i = 0; i < n; i++) {
@@ -646,11 +688,14 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
                 peakx = (int)x + 1;
                 peaky = (int)y + 1;
             }
-            var2 = var;
+            /* per-pixel variance with a weight map, else the background's */
+            var2 = wmap ? (double)r->var[p] : var;
             sigtv += var2;
-            if (pix > thresh) area++;
+            const float athr = ATH(p);
+            if (i == 0 || athr < othr) othr = athr;
+            if (pix > athr) area++;
             tv += pix;
-            err = dbacknoise2;
+            err = wmap ? var2 : dbacknoise2;
             if (cfg->gain > 0.0 && cdpix > 0.0) err += cdpix / cfg->gain;
             x = x - xm;
             y = y - ym;
@@ -658,7 +703,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
             emx2 += err * x * x;
             emy2 += err * y * y;
             emxy += err * x * y;
-            if (cleanflag) top_insert(top, hsize, &nfill, (float)(cdpix - dthresh));
+            if (cleanflag) top_insert(top, hsize, &nfill, (float)(cdpix - odth));
         }
         r->mthresh[o] = cleanflag ? top[0] : 0.0f;
         {
@@ -675,7 +720,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         r->npix[o] = area;
         r->flux[o] = (float)tv;
         r->fluxerr[o] = (float)sigtv;
-        r->thresh_obj[o] = thresh - dbkg;
+        r->thresh_obj[o] = othr - dbkg;
         r->peak[o] = peak - dbkg;
         r->peakx[o] = peakx;
         r->peaky[o] = peaky;
@@ -785,7 +830,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         free(num);
     }
 
-    free(pre); free(o_level); free(o_root); free(o_first); free(o_dthresh);
+    free(pre); free(o_level); free(o_root); free(o_first); free(o_dthresh); free(rawt0);
     free(par); free(rawid); free(rawoff); free(fpia); free(pixobj);
     return r;
 }

@@ -184,7 +184,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=8, help="frames per step (default 8)")
-    ap.add_argument("--workers", type=int, default=4, help="concurrent contexts (default 4)")
+    ap.add_argument("--workers", type=int, default=8, help="concurrent contexts (default 8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--size", type=int, default=4096, help="frame side (default 4096)")
     ap.add_argument("--workload", default="crowded", choices=["crowded", "batch64"])

@@ -31,6 +31,8 @@ struct AnaArgs {
     double gain, satur_level;
     int errflag, peakflag, isoflag, fwhmflag, cleanflag;
     ObjCols o;
+    WeightArgs wa;          /* MAP_WEIGHT: per-pixel variance and thresholds */
+    const float *rawt0;     /* detection threshold of each raw object */
 };
 
 /* back.c:685-732 */
@@ -71,6 +73,10 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     const int g0 = A.objoff[obj], npx = A.objoff[obj + 1] - g0;
     const int *P = A.opix + g0;
     int flag = o.flag[obj];
+    /* the object's detection threshold: with per-pixel thresholds, the lowest among the
+       pixels of its raw object */
+    const bool perpix = A.wa.w && A.wa.rel;
+    const float odth = perpix ? A.rawt0[o.root[obj]] : A.dthresh;
     double mxo, myo;
     float fdflux, oa, ob;
     if (o.level[obj] > 0) {
@@ -103,7 +109,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
         ymin = sxw_min(ymin);
         ymax = sxw_max(ymax);
         if (xmin == 0 || xmax == W - 1 || ymin == 0 || ymax == H - 1) flag |= SXG_OBJ_TRUNC;
-        const double dthresh = (double)A.dthresh;
+        const double dthresh = (double)odth;
         const double dthresh2 = (dthresh + peak) / 2.0;
         double acc = 0.0;
         int area2 = 0, dnpix = 0;
@@ -161,7 +167,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     double eacc = 0.0;
     const double var = (double)(A.backsig * A.backsig), dbacknoise2 = var;
     const double xm = A.errflag ? mxo : 0.0, ym = A.errflag ? myo : 0.0;
-    float pk = (float)-SXG_BIG, cdpeak = (float)-SXG_BIG;
+    float pk = (float)-SXG_BIG, cdpeak = (float)-SXG_BIG, othr = (float)SXG_BIG;
     int area = 0, peakp = 0x7fffffff;
     const bool cleanflag = A.cleanflag && npx >= HEAPN;
     SxwTop5 top;
@@ -169,7 +175,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     for (int base = 0; base < npx; base += 64) {
         const int k = base + lane;
         float pix = 0.0f;
-        double err = 0.0, ex2 = 0.0, ey2 = 0.0, exy = 0.0;
+        double err = 0.0, ex2 = 0.0, ey2 = 0.0, exy = 0.0, var2 = 0.0;
         if (k < npx) {
             const int p = P[k];
             double x = p % W, y = p / W;
@@ -180,23 +186,29 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
                 cdpeak = cdpix;
                 peakp = p;
             }
-            if (pix > A.thresh) area++;
-            err = dbacknoise2;
+            /* per-pixel variance and analysis threshold with a weight map */
+            const float varp = A.wa.w ? sxg_var(A.wa.w, p) : 0.0f;
+            var2 = A.wa.w ? (double)varp : var;
+            const float athr = perpix ? sxg_pix_thresh(A.wa.kana, varp) : A.thresh;
+            othr = fminf(othr, athr);
+            if (pix > athr) area++;
+            err = A.wa.w ? var2 : dbacknoise2;
             if (A.gain > 0.0 && cdpix > 0.0) err += cdpix / A.gain;
             x = x - xm;
             y = y - ym;
             ex2 = err * x * x;
             ey2 = err * y * y;
             exy = err * x * y;
-            if (cleanflag) top.add(cdpix - A.dthresh);
+            if (cleanflag) top.add(cdpix - odth);
         }
         /* tv, sigtv, esum, emx2, emy2, emxy (the error sums only with errflag) */
-        const double t6[6] = {pix, var, err, ex2, ey2, exy};
+        const double t6[6] = {pix, var2, err, ex2, ey2, exy};
         sxw_fold<6>(rows, t6, k < npx, eacc);
     }
     double tv = sxw_rl(eacc, 0), sigtv = sxw_rl(eacc, 1);
     const double emx2 = sxw_rl(eacc, 3), emy2 = sxw_rl(eacc, 4), emxy = sxw_rl(eacc, 5);
     pk = sxw_max(pk);
+    othr = sxw_minf(othr);
     area = sxw_sum(area);
     int peakx = 0, peaky = 0;
     if (A.peakflag) {
@@ -215,7 +227,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     }
     tv -= npx * dbkg;
     if (A.gain > 0.0 && tv > 0.0) sigtv += tv / A.gain;
-    const float thr_o = A.thresh - dbkg, pk_o = pk - dbkg;
+    const float thr_o = othr - dbkg, pk_o = pk - dbkg;
     /* ---- isophotal areas and FWHM ---- */
     float threshs[SXG_NISO];
     if (pk_o > 0.0 && thr_o > 0.0)
@@ -275,7 +287,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     if (pk_o + bkg >= A.satur_level) flag |= SXG_OBJ_SATUR;
     if (lane != 0) return;
     o.fdnpix[obj] = npx;
-    o.dthresh[obj] = A.dthresh;   /* the detection threshold from here on */
+    o.dthresh[obj] = odth;   /* the detection threshold from here on */
     o.mthresh[obj] = mthresh;
     if (A.errflag) {
         o.poserr_mx2[obj] = pmx2;
@@ -339,6 +351,10 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
     A.isoflag = *reinterpret_cast<const char *>(&flagobj_iso_0) != 0;
     A.fwhmflag = *reinterpret_cast<const char *>(&flagobj_fwhm) != 0;
     A.cleanflag = prefs_clean_flag != 0;
+    if (!sxg_weight_args(A.wa)) return;
+    RawObj ro;
+    if (!sxg_raw_alloc(S.nraw, ro)) return;
+    A.rawt0 = ro.t0;
     if (!sxg_obj_view(A.o)) return;
     S.clean_enabled = prefs_clean_flag != 0;
     sxg_stage_begin(ST_ANALYSE);

@@ -90,7 +90,8 @@ __device__ inline void meshstat_pass2_end(const MeshAcc &a, MeshStat &o)
 #define MS_THREADS 256
 
 /* meshes whose rows do not fit the LDS ring: one lane per mesh reading global memory */
-__global__ void __launch_bounds__(64) k_meshstat_direct(const float *__restrict__ img, int W,
+__global__ void __launch_bounds__(64) k_meshstat_direct(const float *__restrict__ img,
+                                                        const float *__restrict__ wmap, int W,
                                                         int H, int bs, int nbx, int nmesh,
                                                         MeshStat *ms)
 {
@@ -104,9 +105,9 @@ __global__ void __launch_bounds__(64) k_meshstat_direct(const float *__restrict_
     bool bad = false;
     for (int pass = 0; pass < 2 && !(pass == 1 && bad); pass++) {
         for (int y = 0; y < bh; y++) {
-            const float *row = img + (size_t)(y0 + y) * W + x0;
+            const size_t ro = (size_t)(y0 + y) * W + x0;
             for (int x = 0; x < bw; x++) {
-                const double pix = row[x];
+                const double pix = (wmap && !(wmap[ro + x] > 0.0f)) ? (float)-SXG_BIG : img[ro + x];
                 if (pass == 0 ? pix > -SXG_BIG : (pix <= hcut && pix >= lcut)) {
                     acc.mean += pix;
                     acc.sigma += pix * pix;
@@ -171,8 +172,9 @@ __host__ __device__ inline int ms_seg(int bs)
     return seg;
 }
 
-__global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict__ img, int W, int H,
-                                                         int bs, int nbx, int mpb, int R,
+__global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict__ img,
+                                                         const float *__restrict__ wmap, int W,
+                                                         int H, int bs, int nbx, int mpb, int R,
                                                          MeshStat *ms)
 {
     extern __shared__ __align__(16) float ring[];
@@ -208,8 +210,17 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
                     const int e = e0 + u * LT;
                     if (e < tot) {
                         const int r = e / q4, c = 4 * (e - r * q4);
-                        v[u] = *reinterpret_cast<const float4 *>(
-                            img + (size_t)(y0 + r0 + r) * W + x0 + c);
+                        const size_t o = (size_t)(y0 + r0 + r) * W + x0 + c;
+                        v[u] = *reinterpret_cast<const float4 *>(img + o);
+                        if (wmap) {
+                            /* pixels of non-positive weight are left out: -BIG */
+                            const float4 wv = *reinterpret_cast<const float4 *>(wmap + o);
+                            const float nb = (float)-SXG_BIG;
+                            if (!(wv.x > 0.0f)) v[u].x = nb;
+                            if (!(wv.y > 0.0f)) v[u].y = nb;
+                            if (!(wv.z > 0.0f)) v[u].z = nb;
+                            if (!(wv.w > 0.0f)) v[u].w = nb;
+                        }
                     }
                 }
 #pragma unroll
@@ -231,7 +242,8 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
                     const int e = e0 + u * LT;
                     if (e < tot) {
                         const int r = e / bandw, c = e - r * bandw;
-                        v[u] = img[(size_t)(y0 + r0 + r) * W + x0 + c];
+                        const size_t o = (size_t)(y0 + r0 + r) * W + x0 + c;
+                        v[u] = (wmap && !(wmap[o] > 0.0f)) ? (float)-SXG_BIG : img[o];
                     }
                 }
 #pragma unroll
@@ -302,7 +314,8 @@ __device__ inline long long wg_sum64(long long v, long long *sm)
     return r;
 }
 
-__global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restrict__ img, int W,
+__global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restrict__ img,
+                                                           const float *__restrict__ wmap, int W,
                                                            int H, int bs, int nbx,
                                                            const MeshStat *ms, float *back,
                                                            float *sigma)
@@ -329,7 +342,9 @@ __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restri
     const int npx = bw * bh;
     for (int k = threadIdx.x; k < npx; k += MODE_THREADS) {
         const int y = k / bw, x = k - y * bw;
-        float t = img[(size_t)(y0 + y) * W + x0 + x] / qscale + cste;
+        const size_t o = (size_t)(y0 + y) * W + x0 + x;
+        const float v = (wmap && !(wmap[o] > 0.0f)) ? (float)-SXG_BIG : img[o];
+        float t = v / qscale + cste;
         if (t > -1.0f && t < (float)nlev) atomicAdd(&h[(int)t], 1);
     }
     __syncthreads();
@@ -768,6 +783,9 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
     float *back = S.maps.as<float>(), *sigma = back + nmesh, *dback = sigma + nmesh,
           *dsigma = dback + nmesh;
     float *img = S.img.as<float>();
+    WeightArgs wa;
+    if (!sxg_weight_args(wa)) return;
+    const float *wmap = wa.w;   /* pixels of non-positive weight leave the mesh statistics */
 
     sxg_stage_begin(ST_BACK);
     sxg_kt_begin(SXG_KT_MESHSTAT);
@@ -783,16 +801,16 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
             SXG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_meshstat),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nbx, mpb), nby), dim3(MS_THREADS), lds,
-                               S.stream, img, W, H, bs, nbx, mpb, R, ms);
+                               S.stream, img, wmap, W, H, bs, nbx, mpb, R, ms);
         } else {
             hipLaunchKernelGGL(k_meshstat_direct, dim3(sxg_div_up(nmesh, 64)), dim3(64), 0,
-                               S.stream, img, W, H, bs, nbx, nmesh, ms);
+                               S.stream, img, wmap, W, H, bs, nbx, nmesh, ms);
         }
     }
     sxg_kt_end(SXG_KT_MESHSTAT);
     sxg_kt_begin(SXG_KT_MESHMODE);
-    hipLaunchKernelGGL(k_meshmode, dim3(nmesh), dim3(MODE_THREADS), 0, S.stream, img, W, H, bs,
-                       nbx, ms, back, sigma);
+    hipLaunchKernelGGL(k_meshmode, dim3(nmesh), dim3(MODE_THREADS), 0, S.stream, img, wmap, W, H,
+                       bs, nbx, ms, back, sigma);
     sxg_kt_end(SXG_KT_MESHMODE);
     sxg_kt_begin(SXG_KT_FILTERBACK);
     {

@@ -153,14 +153,19 @@ __device__ inline float wave_ordered_sum(const float *v, int n, Sel sel)
 }
 
 /* ---------------------------------------------------------------- thresholds ---- */
+/* the base of object r: basethresh, or with per-pixel thresholds (weight map) the
+   lowest threshold among the object's pixels, rawt0[r] */
 __global__ void __launch_bounds__(256) k_deb_thresh(const float *__restrict__ fdpeak, int nraw,
-                                                    int nthresh, float base, float *thr)
+                                                    int nthresh, float basethresh,
+                                                    const float *__restrict__ rawt0, int perpix,
+                                                    float *thr)
 {
     const long k = (long)blockIdx.x * 256 + threadIdx.x;
     if (k >= (long)nraw * nthresh) return;
     const int r = (int)(k / nthresh), i = (int)(k - (long)r * nthresh);
-    const double t0 = (double)base, pk = (double)fdpeak[r];
-    thr[k] = i == 0 ? base : (float)(t0 * sx_pow(pk / t0, (double)i / nthresh));
+    const float b = perpix ? rawt0[r] : basethresh;
+    const double t0 = (double)b, pk = (double)fdpeak[r];
+    thr[k] = i == 0 ? b : (float)(t0 * sx_pow(pk / t0, (double)i / nthresh));
 }
 
 
@@ -786,7 +791,8 @@ extern "C" void init_deblend(float basethresh, int nthresh, size_t numValidObj, 
     if (tot > 0) {
         sxg_kt_begin(SXG_KT_DEB_THRESH);
         hipLaunchKernelGGL(k_deb_thresh, dim3(sxg_div_up(tot, 256)), dim3(256), 0, S.stream,
-                           ro.fdpeak, S.nraw, nthresh, basethresh, thr);
+                           ro.fdpeak, S.nraw, nthresh, basethresh, ro.t0,
+                           (int)(S.weighted && S.wargs.rel), thr);
         sxg_kt_end(SXG_KT_DEB_THRESH);
     }
     SXG_HIP(hipGetLastError());

@@ -24,12 +24,19 @@
 #include "sxg_internal.h"
 #include "sxg_wave.h"
 
+/* the detection threshold of pixel p: per pixel with a weight map and relative
+   thresholds, else the global one */
+__device__ inline float det_thresh(const WeightArgs &wa, float t, long p)
+{
+    return (wa.w && wa.rel) ? sxg_pix_thresh(wa.kdet, sxg_var(wa.w, p)) : t;
+}
+
 __global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, float t, long n,
-                                                  int *par, int *lab)
+                                                  int *par, int *lab, WeightArgs wa)
 {
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
     if (p >= n) return;
-    par[p] = cd[p] >= t ? (int)p : -1;
+    par[p] = cd[p] >= det_thresh(wa, t, p) ? (int)p : -1;
     lab[p] = -1;
 }
 
@@ -98,7 +105,7 @@ __global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg,
 }
 
 struct RootStat {
-    int *cnt, *peak, *xmin, *xmax, *ymin, *ymax;
+    int *cnt, *peak, *xmin, *xmax, *ymin, *ymax, *t0;
 };
 
 __global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg)
@@ -111,6 +118,7 @@ __global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg)
     rs.xmax[k] = -1;
     rs.ymin[k] = 0x7fffffff;
     rs.ymax[k] = -1;
+    rs.t0[k] = 0x7fffffff;          /* above f2ord of every float */
 }
 
 /* per-root size, peak and bounding box; lanes of a wavefront that share a root (the
@@ -119,18 +127,21 @@ __global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg)
 __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg, int nfg, int W,
                                                       const int *__restrict__ lab,
                                                       const int *__restrict__ rootpos,
-                                                      const float *__restrict__ cd, RootStat rs)
+                                                      const float *__restrict__ cd, RootStat rs,
+                                                      WeightArgs wa)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    const bool perpix = wa.w && wa.rel;
     bool act = k < nfg;
-    int rk = -1, x = 0, y = 0, pv = 0;
+    int rk = -1, x = 0, y = 0, pv = 0, tv = 0x7fffffff;
     if (act) {
         const int p = fg[k];
         rk = rootpos[lab[p]];
         y = p / W;
         x = p - y * W;
         pv = f2ord(cd[p]);
+        if (perpix) tv = f2ord(det_thresh(wa, 0.0f, p));
     }
     unsigned long long pending = __ballot(act);
     while (pending) {
@@ -148,6 +159,11 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
             atomicMax(&rs.xmax[r], x1);
             atomicMin(&rs.ymin[r], y0);
             atomicMax(&rs.ymax[r], y1);
+        }
+        if (perpix) {
+            /* the lowest pixel threshold of the object */
+            const int t0 = sxw_min(mine ? tv : 0x7fffffff);
+            if (lane == leader) atomicMin(&rs.t0[r], t0);
         }
         if (mine) act = false;
         pending &= ~grp;
@@ -168,7 +184,8 @@ __global__ void __launch_bounds__(256) k_det_rootflag(const int *__restrict__ fg
 
 __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ rawpos, int nraw,
                                                      const int *__restrict__ fg, RootStat rs,
-                                                     int *rawid_of_pos, RawObj ro)
+                                                     int *rawid_of_pos, RawObj ro, float t,
+                                                     int perpix)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nraw) return;
@@ -181,6 +198,7 @@ __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ raw
     ro.xmax[i] = rs.xmax[k];
     ro.ymin[i] = rs.ymin[k];
     ro.ymax[i] = rs.ymax[k];
+    ro.t0[i] = perpix ? ord2f(rs.t0[k]) : t;
 }
 
 __global__ void __launch_bounds__(256) k_det_fill(int *a, int n, int v)
@@ -259,8 +277,10 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     int *cnt_d = sxg_take<int>("det.counts", 8);
     if (!rootpos || !cnt_d) return;
     sxg_kt_begin(SXG_KT_DET_INIT);
+    WeightArgs wa;
+    if (!sxg_weight_args(wa)) return;
     hipLaunchKernelGGL(k_det_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
-                       S.cd.as<float>(), dthreshold, N, par, lab);
+                       S.cd.as<float>(), dthreshold, N, par, lab, wa);
     sxg_kt_end(SXG_KT_DET_INIT);
     /* the foreground list has at most N entries; size it after counting */
     int *fg = sxg_take<int>("det.fg", S.N);
@@ -279,10 +299,11 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     rs.xmax = sxg_take<int>("det.xmax", nfg);
     rs.ymin = sxg_take<int>("det.ymin", nfg);
     rs.ymax = sxg_take<int>("det.ymax", nfg);
+    rs.t0 = sxg_take<int>("det.t0", nfg);
     int *keep = sxg_take<int>("det.keep", nfg);
     int *rawpos = sxg_take<int>("det.rawpos", nfg);
     int *rawid_of_pos = sxg_take<int>("det.rawid", nfg);
-    if (!rs.cnt || !rs.peak || !rs.xmin || !rs.xmax || !rs.ymin || !rs.ymax || !keep ||
+    if (!rs.cnt || !rs.peak || !rs.xmin || !rs.xmax || !rs.ymin || !rs.ymax || !rs.t0 || !keep ||
         !rawpos || !rawid_of_pos)
         return;
     if (nfg > 0) {
@@ -295,7 +316,7 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         hipLaunchKernelGGL(k_det_statinit, dim3(gfg), dim3(256), 0, st, rs, nfg);
         sxg_kt_begin(SXG_KT_DET_STAT);
         hipLaunchKernelGGL(k_det_rootstat, dim3(gfg), dim3(256), 0, st, fg, nfg, W, lab, rootpos,
-                           S.cd.as<float>(), rs);
+                           S.cd.as<float>(), rs, wa);
         sxg_kt_end(SXG_KT_DET_STAT);
         hipLaunchKernelGGL(k_det_rootflag, dim3(gfg), dim3(256), 0, st, fg, nfg, lab, rs.cnt,
                            ext_minarea, keep);
@@ -310,7 +331,7 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         hipLaunchKernelGGL(k_det_fill, dim3(gfg), dim3(256), 0, st, rawid_of_pos, nfg, -1);
         if (nraw > 0)
             hipLaunchKernelGGL(k_det_rawinfo, dim3(sxg_div_up(nraw, 256)), dim3(256), 0, st, rawpos,
-                               nraw, fg, rs, rawid_of_pos, ro);
+                               nraw, fg, rs, rawid_of_pos, ro, dthreshold, (int)(wa.w && wa.rel));
         /* pixel keys, kept pixels, stable sort by object */
         hipLaunchKernelGGL(k_det_pixkey, dim3(gfg), dim3(256), 0, st, fg, nfg, lab, rootpos,
                            rawid_of_pos, keep);

@@ -60,6 +60,21 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+/* MAP_WEIGHT: var = 1/w for w > 0, else BIG (weight.c:111-116, sigfac = 1 as weight.c:80
+   sets it); with relative thresholds (rel) each pixel's thresholds are kdet*sqrt(var)
+   and kana*sqrt(var) */
+struct WeightArgs {
+    const float *w = nullptr;
+    int rel = 0;
+    double kdet = 0.0, kana = 0.0;
+};
+__device__ inline float sxg_var(const float *w, long p)
+{
+    const float v = w[p];
+    return v > 0.0f ? 1.0f / v : (float)SXG_BIG;
+}
+__device__ inline float sxg_pix_thresh(double k, float var) { return (float)(k * sqrt((double)var)); }
+
 enum StageId { ST_BACK = 0, ST_FILTER, ST_DETECT, ST_DEBLEND, ST_ANALYSE, ST_CLEAN, ST_N };
 
 struct SxgState {
@@ -119,13 +134,24 @@ struct SxgState {
     DevBuf resident;                       /* frame kept in HBM (sxg_set_resident_frame) */
     int rw = 0, rh = 0;
     bool is_default = false;               /* the process-wide default context */
+    /* MAP_WEIGHT map (sxg_set_weight) */
+    DevBuf weight;
+    int ww = 0, wh = 0;
+    bool weighted = false;
+    WeightArgs wargs;                      /* w filled in by sxg_weight_args() */
 };
+/* the weight arguments for the current frame (w = nullptr without a weight map); records
+   an error and returns false when the map does not match the frame */
+bool sxg_weight_args(WeightArgs &wa);
 
-/* raw (level-0) objects, structure of arrays */
+/* raw (level-0) objects, structure of arrays; t0: the object's detection threshold
+   (the lowest per-pixel threshold with a weight map, else the global one) */
 struct RawObj {
     int *first, *count, *off, *xmin, *xmax, *ymin, *ymax;
-    float *fdpeak, *fdflux;
+    float *fdpeak, *fdflux, *t0;
 };
+
+
 bool sxg_raw_alloc(int nraw, RawObj &ro);   /* (re)binds the named buffers */
 
 /* kernel timers (HIP events on the library stream, enabled by sxg_kernel_timing) */

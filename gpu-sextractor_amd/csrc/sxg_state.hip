@@ -163,6 +163,8 @@ extern "C" void sxg_release(void)
     sxg_release_named();
     S.dmask.release();
     S.resident.release();
+    S.weight.release();
+    S.weighted = false;
     S.have_frame = false;
 }
 
@@ -337,8 +339,9 @@ bool sxg_raw_alloc(int nraw, RawObj &ro)
     ro.ymax = sxg_take<int>("raw.ymax", n);
     ro.fdpeak = sxg_take<float>("raw.fdpeak", n);
     ro.fdflux = sxg_take<float>("raw.fdflux", n);
+    ro.t0 = sxg_take<float>("raw.t0", n);
     return ro.first && ro.count && ro.off && ro.xmin && ro.xmax && ro.ymin && ro.ymax &&
-           ro.fdpeak && ro.fdflux;
+           ro.fdpeak && ro.fdflux && ro.t0;
 }
 
 /* -------------------------------------------------------- frame entry points ---- */
@@ -410,4 +413,47 @@ extern "C" int sxg_get_filtered(float *out)
                                S.stream), -1);
     SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
     return 0;
+}
+
+/* ------------------------------------------------------------- weight maps ---- */
+extern "C" int sxg_set_weight(const float *w, int width, int height, int thresh_type,
+                              double detect_thresh, double analysis_thresh)
+{
+    SxgState &S = sxg_state();
+    sxg_clear_error();
+    if (!w) {
+        S.weighted = false;
+        S.weight.release();
+        S.ww = S.wh = 0;
+        return 0;
+    }
+    if (width <= 0 || height <= 0 || (long)width * height >= 2147483647L) {
+        sxg_set_error("sxg_set_weight: bad map size");
+        return -1;
+    }
+    if (!sxg_ensure_stream()) return -1;
+    const size_t n = (size_t)width * height;
+    if (!S.weight.ensure(n * sizeof(float))) return -1;
+    SXG_HIP_RET(hipMemcpy(S.weight.ptr, w, n * sizeof(float), hipMemcpyHostToDevice), -1);
+    S.ww = width;
+    S.wh = height;
+    S.weighted = true;
+    S.wargs.rel = thresh_type != 1;
+    S.wargs.kdet = detect_thresh;
+    S.wargs.kana = analysis_thresh;
+    return 0;
+}
+
+bool sxg_weight_args(WeightArgs &wa)
+{
+    SxgState &S = sxg_state();
+    wa = WeightArgs();
+    if (!S.weighted) return true;
+    if (S.ww != S.W || S.wh != S.H) {
+        sxg_set_error("the weight map (sxg_set_weight) does not match the frame size");
+        return false;
+    }
+    wa = S.wargs;
+    wa.w = S.weight.as<float>();
+    return true;
 }

@@ -33,6 +33,11 @@ __device__ inline float sxw_max(float v)
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
     return v;
 }
+__device__ inline float sxw_minf(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
 __device__ inline int sxw_max(int v)
 {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));

@@ -247,6 +247,7 @@ def gpu():
         L.sxg_batch_process.argtypes = [vp, i, vp, vp]
         L.sxg_batch_segmap.argtypes = [i, vp]
         L.sxg_batch_release.argtypes = []
+        L.sxg_set_weight.argtypes = [vp, i, i, i, d, d]
         L.sxg_set_option.argtypes = [i, C.c_long]
         L.sxg_get_option.argtypes = [i]
         L.sxg_get_option.restype = C.c_long
@@ -269,7 +270,7 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_set_resident_frame", "sxg_process_resident", "sxg_sync", "sxg_kernel_timing",
             "sxg_kernel_times", "sxg_set_option", "sxg_get_option", "sxg_ctx_create",
             "sxg_ctx_destroy", "sxg_ctx_bind", "sxg_batch_set_frames", "sxg_batch_process",
-            "sxg_batch_segmap", "sxg_batch_release"]
+            "sxg_batch_segmap", "sxg_batch_release", "sxg_set_weight"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort"]
@@ -413,12 +414,25 @@ FLAG_TRUE_DOUBLE = struct.unpack("<d", b"\x01" + b"\x00" * 7)[0]
 FLAG_TRUE_FLOAT = struct.unpack("<f", b"\x01" + b"\x00" * 3)[0]
 
 
-def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog")):
+def set_weight(w, p):
+    """Install (or with w None remove) the MAP_WEIGHT map for the next frames."""
+    L = gpu()
+    if w is None:
+        L.sxg_set_weight(None, 0, 0, 0, 0.0, 0.0)
+        return
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    if L.sxg_set_weight(w.ctypes.data, w.shape[1], w.shape[0], p.thresh_type, p.detect_thresh,
+                        p.analysis_thresh) != 0:
+        check()
+
+
+def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"), weight=None):
     """Run the whole path in the order of the reference's scan.c:78-146 and collect the
-    requested outputs (numpy arrays)."""
+    requested outputs (numpy arrays).  weight: optional MAP_WEIGHT map."""
     L = gpu()
     img = np.ascontiguousarray(img, dtype=np.float32)
     H, W = img.shape
+    set_weight(weight, p)
     bm, bsig = run_background(img, p)
     dthresh, thresh = thresholds(p, bsig)
     L.init_detection()

@@ -184,6 +184,17 @@ int sxg_process_resident(const sxg_params *p, sxg_frame_info *info);
 /* Wait for the library's stream. */
 void sxg_sync(void);
 
+/* MAP_WEIGHT weight map for the frames processed next on this context (no counterpart
+   in the reference GPU path; the CPU semantics of weight.c:91-126 with sigfac = 1 as
+   weight.c:80 fixes it): var = 1/w for w > 0, else BIG; pixels of non-positive weight
+   leave the mesh statistics; with relative thresholds (thresh_type 0) the detection and
+   analysis thresholds of each pixel are detect_thresh*sqrt(var) and
+   analysis_thresh*sqrt(var), a raw object's deblending base is the lowest threshold among
+   its pixels; FLUXERR_ISO and the position errors use the per-pixel variance.  w == NULL
+   removes the map.  The map must have the size of the frame.  Returns 0 or < 0. */
+int sxg_set_weight(const float *w, int width, int height, int thresh_type,
+                   double detect_thresh, double analysis_thresh);
+
 /* ---- contexts and batches (no counterpart in the reference) -------------------
    A context holds everything the entry points above keep between calls (frame buffers,
    workspace, stream).  The calling thread uses the context it bound last, or the

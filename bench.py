@@ -23,6 +23,12 @@ large-scale background structure, a MAP_WEIGHT inverse-variance map (+-20% vigne
 1% dead columns), mexhat_5.0_11x11, BACK_SIZE 128; frame by frame on the default
 context, with the oracle's segmentation map compared at full size.
 
+--workload hdr4096 runs the precision workload: a 4096x4096 high-dynamic-range frame
+(200 saturated Moffat stars with beta 2.5 wings, 100k faint sources, nested groups of
+5-20 components, SATUR_LEVEL 6e5); the GPU catalogue (the reference's arithmetic, bit-exact
+with the oracle) is compared per column with a long-double evaluation of the same
+quantities on the same pixels, and the maximum relative errors are reported.
+
 --workload batch64 runs the many-frame workload instead: 64 frames whose sizes are drawn
 from {512^2, 1024^2, 2048x1024, 2048^2, 4096x2048} and whose source densities are
 log-uniform in [1e-4, 1e-2] per pixel (the 1024^2 generator settings, default.conv,
@@ -259,6 +265,95 @@ def wide8192(args):
     return 0
 
 
+PRECISION_COLUMNS = ["flux", "mx", "my", "mx2", "my2", "mxy", "a", "b", "theta", "cxx", "cyy",
+                     "cxy"]
+
+
+def precision_table(cols, truth):
+    """Per-column max and median errors of the catalogue against the long-double truth:
+    relative (|d|/|truth|) except XY and CXY (|d| over the scale sqrt(X2*Y2), resp.
+    sqrt(CXX*CYY)) and THETA (absolute, degrees)."""
+    out = {}
+    for k, name in enumerate(PRECISION_COLUMNS):
+        got = np.asarray(cols[name], np.float64)
+        tr = truth[:, k]
+        if name == "theta":
+            err = np.abs(got - tr)
+            err = np.minimum(err, 180.0 - err)
+            out["theta_deg_abs"] = {"max": float(err.max()), "median": float(np.median(err))}
+            continue
+        scale = np.abs(tr)
+        if name == "mxy":
+            scale = np.sqrt(np.abs(truth[:, 3] * truth[:, 4]))
+        if name == "cxy":
+            scale = np.sqrt(np.abs(truth[:, 9] * truth[:, 10]))
+        rel = np.abs(got - tr) / np.maximum(scale, 1e-300)
+        out[name] = {"max": float(rel.max()), "median": float(np.median(rel))}
+    return out
+
+
+def hdr4096(args):
+    """The precision workload."""
+    t = time.perf_counter()
+    img = sexgpu.synth("hdr4096")
+    H, W = img.shape
+    N = W * H
+    print("# frame %dx%d fnv64 %s, generated in %.1f s" % (W, H, sexgpu.frame_hash(img),
+                                                          time.perf_counter() - t),
+          file=sys.stderr, flush=True)
+    p = sexgpu.Params(mask=sexgpu.read_conv("gauss_3.0_7x7.conv"), satur_level=6e5)
+    sp = sexgpu.native_params(p)
+    L = sexgpu.gpu()
+    L.sxg_set_resident_frame(img.ctypes.data, W, H)
+    sexgpu.check()
+    fi = sexgpu.FrameInfo()
+    for _ in range(args.warmup):
+        L.sxg_process_resident(C.byref(sp), C.byref(fi))
+        sexgpu.check()
+    L.sxg_sync()
+    steps = max(1, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nobj = L.sxg_process_resident(C.byref(sp), C.byref(fi))
+    L.sxg_sync()
+    dt = time.perf_counter() - t0
+    sexgpu.check()
+    cols = {name: sexgpu.get_column(name, nobj) for name in PRECISION_COLUMNS}
+    seg = np.zeros((H, W), np.int32)
+    L.sxg_get_segmap(seg.ctypes.data)
+    sexgpu.check()
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import sxo  # the oracle: test infrastructure, the checker and the FP64 reference here
+    t = time.perf_counter()
+    o = sxo.run(img, mask=p.mask, satur_level=6e5, truth=True)
+    dtc = time.perf_counter() - t
+    parity = bool(o["nobj"] == nobj and np.array_equal(o["seg"], seg) and
+                  all(np.array_equal(np.asarray(cols[c]), o[c]) for c in PRECISION_COLUMNS
+                      if c != "theta"))
+    prec = precision_table(cols, o["truth"])
+    for k, v in prec.items():
+        print("# %-14s max %.3e  median %.3e" % (k, v["max"], v["median"]), file=sys.stderr)
+    line = {
+        "metric": "per-column error vs an all-FP64 oracle, 4096^2 HDR frame; bit-exact seg map",
+        "value": round(N * steps / dt / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(1000.0 * dt / steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "HDR 4096x4096: 200 saturated stars, 100k faint sources, nested "
+                               "groups; gauss_3.0_7x7, SATUR_LEVEL 6e5",
+                   "objects": info_n(fi), "precision": prec, "parity_with_oracle": parity},
+        "roofline": None,
+        "cpu_baseline": {"value": round(N / dtc / 1e6, 3), "unit": "Mpix/s", "cores": 1,
+                         "kind": "port", "sample": "one full frame (%.1f s)" % dtc},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def info_n(fi):
+    return {"raw": fi.nraw, "deblended": fi.nobj, "catalogue": fi.nsurv}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,7 +363,8 @@ def main():
     ap.add_argument("--workers", type=int, default=8, help="concurrent contexts (default 8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--size", type=int, default=4096, help="frame side (default 4096)")
-    ap.add_argument("--workload", default="crowded", choices=["crowded", "batch64", "wide8192"])
+    ap.add_argument("--workload", default="crowded",
+                    choices=["crowded", "batch64", "wide8192", "hdr4096"])
     ap.add_argument("--sweep", default="", help="worker counts to time, e.g. 1,2,4,8")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -279,6 +375,8 @@ def main():
         return batch64(args)
     if args.workload == "wide8192":
         return wide8192(args)
+    if args.workload == "hdr4096":
+        return hdr4096(args)
 
     t = time.perf_counter()
     nsrc = int(round(167000 * (args.size / 4096.0) ** 2))

@@ -148,13 +148,21 @@ int sxs_render(const sxs_params *p, float *img, double *src_out, int src_out_cap
     xoshiro rng;
     xo_seed(&rng, p->seed);
 
-    /* catalogue: anchors at uniform positions, companions of close groups at 2-6 px */
+    /* catalogue: anchors at uniform positions, companions of close groups at 2-6 px
+       (from the anchor for pairs and triples; from a random earlier member for the
+       grp_min..grp_max groups, which nests them) */
     int k = 0;
+    const int big = p->grp_min > 0 && p->grp_max >= p->grp_min;
     while (k < nsrc) {
         int group = 1;
-        if (p->group_frac > 0.0 && xo_uniform(&rng) < p->group_frac / 2.5)
+        if (big) {
+            if (p->group_frac > 0.0 && xo_uniform(&rng) < p->group_frac)
+                group = p->grp_min + (int)(xo_uniform(&rng) * (p->grp_max - p->grp_min + 1));
+        } else if (p->group_frac > 0.0 && xo_uniform(&rng) < p->group_frac / 2.5) {
             group = (xo_uniform(&rng) < 0.5) ? 2 : 3;
+        }
         double ax = W * xo_uniform(&rng), ay = H * xo_uniform(&rng);
+        const int k0 = k;
         for (int g = 0; g < group && k < nsrc; g++, k++) {
             source_t *s = &src[k];
             int star = xo_uniform(&rng) < p->star_frac;
@@ -162,10 +170,11 @@ int sxs_render(const sxs_params *p, float *img, double *src_out, int src_out_cap
             if (g == 0) {
                 s->x = ax; s->y = ay;
             } else {
+                const source_t *ref = big ? &src[k0 + (int)(xo_uniform(&rng) * g)] : &src[k0];
                 double sep = 2.0 + 4.0 * xo_uniform(&rng);
                 double ang = 2.0 * SX_PI * xo_uniform(&rng);
-                s->x = ax + sep * cos(ang);
-                s->y = ay + sep * sin(ang);
+                s->x = ref->x + sep * cos(ang);
+                s->y = ref->y + sep * sin(ang);
             }
         }
     }

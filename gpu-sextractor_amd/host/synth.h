@@ -30,6 +30,9 @@ typedef struct {
     int nsat;                 /* extra saturated Moffat stars */
     double sat_beta, sat_peak;
     double clip_level;        /* >0: clip pixel values (detector saturation) */
+    int grp_min, grp_max;     /* >0: groups of grp_min..grp_max members (group_frac of the
+                                 anchors), each companion 2-6 px from an earlier member of
+                                 its group (nested clumps); 0: pairs and triples */
 } sxs_params;
 
 void sxs_defaults(sxs_params *p);

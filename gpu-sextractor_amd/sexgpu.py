@@ -38,7 +38,7 @@ class SxsParams(C.Structure):
         ("blob_x", C.c_double * 8), ("blob_y", C.c_double * 8),
         ("blob_sigma", C.c_double), ("blob_amp", C.c_double),
         ("nsat", C.c_int), ("sat_beta", C.c_double), ("sat_peak", C.c_double),
-        ("clip_level", C.c_double),
+        ("clip_level", C.c_double), ("grp_min", C.c_int), ("grp_max", C.c_int),
     ]
 
 
@@ -98,8 +98,11 @@ def synth_params(preset="plain1024", **kw):
         p.width = p.height = 4096
         p.nsrc = 100000
         p.star_frac = 0.5
-        p.fmin, p.fmax = 30.0, 400.0
-        p.group_frac = 0.5
+        # faint: peaks of about 1-5 sigma for the stars (peak ~ 0.078 F at FWHM 3, beta 3.5)
+        p.fmin, p.fmax = 130.0, 650.0
+        # nested groups of 5-20 components for 10% of the anchors
+        p.group_frac = 0.1
+        p.grp_min, p.grp_max = 5, 20
         p.seed = 1004
         p.nsat = 200
         p.sat_beta = 2.5

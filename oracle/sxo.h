@@ -70,6 +70,8 @@ typedef struct {
     int *seg;                               /* per pixel: catalogue NUMBER or 0 */
     int nsurv;
     int *objpix_start, *objpix;             /* final pixel lists (raster order) */
+    int *levpix_start, *levpix;             /* each object's level pixels (its kept
+                                               component before gatherup), raster order */
 } sxo_result;
 
 /* ---------------- background (src/back.c) ---------------- */
@@ -103,7 +105,13 @@ void sxo_config_defaults(sxo_config *c);
 sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg);
 void sxo_free(sxo_result *r);
 /* all-double re-measurement of the catalogue columns over the same final pixel lists */
-int sxo_measure_fp64(const sxo_result *r, double *cols, int ncol);
+/* All-extended-precision (long double) evaluation of the graded catalogue quantities on
+   the result's own pixel sets: for each object, out[12*o + k] with k = 0 FLUX_ISO, 1 X,
+   2 Y (0-based), 3 X2, 4 Y2, 5 XY, 6 A, 7 B, 8 THETA (degrees), 9 CXX, 10 CYY, 11 CXY.
+   X, Y: the barycentre of the object's level pixels; second moments about it over the
+   final pixels (the reference's FULL and robust/MERGED definitions); the singularity rule
+   of the reference applied.  Returns the number of objects. */
+int sxo_measure_fp64(const sxo_result *r, double *out, int ncol);
 
 /* canonical math (shared definition with the HIP kernels) */
 double sxo_log(double x);

@@ -27,6 +27,7 @@ def lib():
         _lib.sxo_run.restype = C.POINTER(SxoResult)
         _lib.sxo_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(SxoConfig)]
         _lib.sxo_free.argtypes = [C.POINTER(SxoResult)]
+        _lib.sxo_measure_fp64.argtypes = [C.POINTER(SxoResult), C.c_void_p, C.c_int]
         _lib.sxo_log.restype = C.c_double
         _lib.sxo_log.argtypes = [C.c_double]
         _lib.sxo_exp.restype = C.c_double
@@ -108,7 +109,7 @@ class SxoResult(C.Structure):
         ("fdpeak", _FP), ("dpeak", _FP), ("peak", _FP), ("bkg", _FP), ("thresh_obj", _FP),
         ("mthresh", _FP), ("fwhm", _FP), ("dthresh_obj", _FP),
         ("master", C.POINTER(C.c_uint32)), ("alloc", _IP), ("seg", _IP), ("nsurv", C.c_int),
-        ("objpix_start", _IP), ("objpix", _IP),
+        ("objpix_start", _IP), ("objpix", _IP), ("levpix_start", _IP), ("levpix", _IP),
     ]
 
 
@@ -139,9 +140,13 @@ def config(**kw):
     return c
 
 
-def run(img, mask=None, weight=None, **kw):
+TRUTH_COLUMNS = ["flux", "mx", "my", "mx2", "my2", "mxy", "a", "b", "theta", "cxx", "cyy", "cxy"]
+
+
+def run(img, mask=None, weight=None, truth=False, **kw):
     """Run the whole oracle path on a float32 frame (and an optional MAP_WEIGHT map);
-    returns a dict of numpy arrays."""
+    returns a dict of numpy arrays.  truth: also the long-double evaluation of the graded
+    columns on the same pixel sets (out["truth"][:, TRUTH_COLUMNS.index(name)])."""
     img = np.ascontiguousarray(img, dtype=np.float32)
     H, W = img.shape
     cfg = config(**kw)
@@ -185,6 +190,11 @@ def run(img, mask=None, weight=None, **kw):
         out[k] = _arr(getattr(r, k), m, np.float32)
     out["iso"] = _arr(r.iso, 8 * m, np.int32).reshape(m, 8)
     out["master"] = _arr(r.master, m, np.uint32)
+    if truth:
+        t = np.zeros((max(m, 1), 12), np.float64)
+        if lib().sxo_measure_fp64(rp, t.ctypes.data, 12) != m:
+            raise RuntimeError("sxo_measure_fp64 failed")
+        out["truth"] = t[:m]
     lib().sxo_free(rp)
     return out
 

@@ -601,6 +601,9 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         }
     }
     free(grid); free(gid); free(loc); free(lvl); free(tmp); free(comps); free(thr); free(thr_eff);
+    /* level pixels: assigned by a kept component (gathered ones are still negative) */
+    unsigned char *islev = (unsigned char *)calloc(N, 1);
+    for (size_t p = 0; p < N; p++) islev[p] = r->alloc[p] >= 0;
     for (size_t p = 0; p < N; p++)
         if (r->alloc[p] <= -2) r->alloc[p] = -2 - r->alloc[p];
 
@@ -619,6 +622,19 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         }
         free(fill);
     }
+
+    r->levpix_start = (int *)calloc(nobj + 1, sizeof(int));
+    r->levpix = (int *)malloc(sizeof(int) * (r->npix_raw + 1));
+    {
+        int m = 0;
+        for (int o = 0; o < nobj; o++) {
+            r->levpix_start[o] = m;
+            for (int i = r->objpix_start[o]; i < r->objpix_start[o + 1]; i++)
+                if (islev[r->objpix[i]]) r->levpix[m++] = r->objpix[i];
+        }
+        r->levpix_start[nobj] = m;
+    }
+    free(islev);
 
     /* ---- object arrays ---- */
 #define ALLOC_I(f) r->f = (int *)calloc(nobj + 1, sizeof(int))
@@ -838,7 +854,8 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
 void sxo_free(sxo_result *r)
 {
     if (!r) return;
-    void *ptrs[] = {r->back, r->sigma, r->dback, r->dsigma, r->sub, r->cd, r->var, r->raw_first,
+    void *ptrs[] = {r->levpix_start, r->levpix, r->back, r->sigma, r->dback, r->dsigma, r->sub,
+                    r->cd, r->var, r->raw_first,
                     r->raw_count, r->raw_fdflux, r->raw_fdpeak, r->level, r->root, r->firstpix,
                     r->flag, r->fdnpix, r->dnpix, r->npix, r->xmin, r->xmax, r->ymin, r->ymax,
                     r->peakx, r->peaky, r->iso, r->mx, r->my, r->mx2, r->my2, r->mxy,
@@ -850,8 +867,61 @@ void sxo_free(sxo_result *r)
     free(r);
 }
 
-int sxo_measure_fp64(const sxo_result *r, double *cols, int ncol)
+int sxo_measure_fp64(const sxo_result *r, double *out, int ncol)
 {
-    (void)r; (void)cols; (void)ncol;
-    return -1;
+    if (ncol != 12 || !r->levpix) return -1;
+    const int W = r->width;
+    for (int o = 0; o < r->nobj; o++) {
+        typedef long double LD;
+        const int *F = r->objpix + r->objpix_start[o];
+        const int nf = r->objpix_start[o + 1] - r->objpix_start[o];
+        const int *L = r->levpix + r->levpix_start[o];
+        const int nl = r->levpix_start[o + 1] - r->levpix_start[o];
+        /* barycentre of the level pixels, weights cd */
+        LD s = 0, sx = 0, sy = 0;
+        for (int i = 0; i < nl; i++) {
+            const LD c = r->cd[L[i]], x = L[i] % W, y = L[i] / W;
+            s += c;
+            sx += c * x;
+            sy += c * y;
+        }
+        const LD xb = sx / s, yb = sy / s;
+        /* central moments about it over the final pixels */
+        LD t = 0, x2 = 0, y2 = 0, xy = 0, flux = 0;
+        for (int i = 0; i < nf; i++) {
+            const LD c = r->cd[F[i]], dx = (LD)(F[i] % W) - xb, dy = (LD)(F[i] / W) - yb;
+            t += c;
+            x2 += c * dx * dx;
+            y2 += c * dy * dy;
+            xy += c * dx * dy;
+            flux += r->sub[F[i]];
+        }
+        x2 /= t;
+        y2 /= t;
+        xy /= t;
+        LD temp2 = x2 * y2 - xy * xy;
+        if (temp2 < 0.00694L) {
+            x2 += 0.0833333L;
+            y2 += 0.0833333L;
+            temp2 = x2 * y2 - xy * xy;
+        }
+        const LD d = x2 - y2;
+        const LD theta = fabsl(d) > 0 ? atan2l(2 * xy, d) / 2 : 3.14159265358979323846L / 4;
+        const LD temp = sqrtl(0.25L * d * d + xy * xy);
+        const LD pm = 0.5L * (x2 + y2);
+        double *q = out + 12 * (size_t)o;
+        q[0] = (double)flux;
+        q[1] = (double)xb;
+        q[2] = (double)yb;
+        q[3] = (double)x2;
+        q[4] = (double)y2;
+        q[5] = (double)xy;
+        q[6] = (double)sqrtl(pm + temp);
+        q[7] = (double)sqrtl(pm - temp);
+        q[8] = (double)(theta * 180 / 3.14159265358979323846L);
+        q[9] = (double)(y2 / temp2);
+        q[10] = (double)(x2 / temp2);
+        q[11] = (double)(-2 * xy / temp2);
+    }
+    return r->nobj;
 }

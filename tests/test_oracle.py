@@ -253,3 +253,31 @@ def test_clean_two_objects_brute_force():
     # without the wing (CLEAN_PARAM 1) the same pair is not cleaned
     o1 = _run_abs(img, clean=1, **dict(kw, clean_param=1.0))
     assert np.all(o1["master"] == 0xffffffff)
+
+
+def test_fp64_truth_closed_form():
+    """The long-double evaluation of the graded columns on a hand-computed object: a
+    3x4 block with known values (one raw object, no deblending)."""
+    img = np.zeros((20, 24), np.float32)
+    vals = np.array([[10, 20, 30, 5], [40, 80, 60, 10], [15, 25, 35, 8]], np.float32)
+    img[6:9, 9:13] = vals
+    r = sxo.run(img, mask=None, back_type=1, back_value=0.0, thresh_type=1, detect_thresh=1.0,
+                analysis_thresh=1.0, back_size=20, clean=0, truth=True)
+    assert r["nobj"] == 1
+    t = r["truth"][0]
+    ys, xs = np.mgrid[6:9, 9:13]
+    w = vals.astype(np.float64)
+    s = w.sum()
+    xb, yb = (w * xs).sum() / s, (w * ys).sum() / s
+    x2 = (w * (xs - xb) ** 2).sum() / s
+    y2 = (w * (ys - yb) ** 2).sum() / s
+    xy = (w * (xs - xb) * (ys - yb)).sum() / s
+    d = x2 - y2
+    tmp = math.sqrt(0.25 * d * d + xy * xy)
+    det = x2 * y2 - xy * xy
+    want = [s, xb, yb, x2, y2, xy, math.sqrt(0.5 * (x2 + y2) + tmp),
+            math.sqrt(0.5 * (x2 + y2) - tmp), math.degrees(math.atan2(2 * xy, d) / 2),
+            y2 / det, x2 / det, -2 * xy / det]
+    assert np.allclose(t, want, rtol=1e-12, atol=1e-12)
+    # the float catalogue is within the float/double rounding of the truth
+    assert abs(r["mx"][0] - xb) < 1e-6 and abs(r["mx2"][0] - x2) / x2 < 1e-5

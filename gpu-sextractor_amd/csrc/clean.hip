@@ -133,6 +133,9 @@ extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacks
     }
     sxg_stage_end(ST_CLEAN);
     S.cleaned = true;
+    /* the catalogue's graded columns in the chosen precision (after CLEAN: the
+       segmentation map and the CLEAN decisions keep the reference's arithmetic) */
+    if (!sxg_precision_columns((int)sxg_option(SXG_OPT_PRECISION))) return;
     if (masterIndex && n > 0)
         SXG_HIP(hipMemcpyAsync(masterIndex, o.master, sizeof(unsigned) * n, hipMemcpyDeviceToHost,
                                S.stream));

@@ -157,6 +157,7 @@ bool sxg_raw_alloc(int nraw, RawObj &ro);   /* (re)binds the named buffers */
 /* kernel timers (HIP events on the library stream, enabled by sxg_kernel_timing) */
 
 long sxg_option(int id);
+bool sxg_precision_columns(int mode);   /* precision.hip */
 void sxg_kt_begin(int id);
 void sxg_kt_end(int id);
 bool sxg_build_segmap();   /* device segmentation map ("out.map") and survivor count */

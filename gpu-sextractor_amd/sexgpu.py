@@ -278,8 +278,9 @@ KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", 
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort"]
 
-# tuning options (sxg_option_id): they choose kernel variants, never change results
-OPTIONS = {"deb_lds_small": 0, "deb_lds_large": 1}
+# options (sxg_option_id): the size caps choose kernel variants and never change a result;
+# "precision" chooses the arithmetic of the graded catalogue columns
+OPTIONS = {"deb_lds_small": 0, "deb_lds_large": 1, "precision": 2}
 
 
 def set_option(name, value):

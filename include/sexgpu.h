@@ -233,13 +233,19 @@ enum sxg_kernel_id {
 void sxg_kernel_timing(int enable);
 int sxg_kernel_times(int id, double *total_ms, int *launches);
 
-/* Tuning options (no counterpart in the reference).  Results never depend on them:
+/* Options (no counterpart in the reference).  The first two never change a result:
    they only choose which kernel variant handles which work.
      SXG_OPT_DEB_LDS_SMALL  raw objects up to this many pixels are deblended by 256-lane
                             workgroups in LDS (default 1024)
      SXG_OPT_DEB_LDS_LARGE  up to this many by 1024-lane workgroups in LDS (default 6144,
-                            at most 6400); larger ones use global scratch memory */
-enum sxg_option_id { SXG_OPT_DEB_LDS_SMALL = 0, SXG_OPT_DEB_LDS_LARGE, SXG_NOPT };
+                            at most 6400); larger ones use global scratch memory
+   and one option that changes the catalogue (never the segmentation map, the object
+   list or the CLEAN decisions):
+     SXG_OPT_PRECISION      arithmetic of FLUX_ISO, X/Y, X2/Y2/XY, A/B/THETA, CXX/CYY/CXY:
+                            0 the reference's (default, bit-exact), 1 two-pass FP64,
+                            2 compensated FP32 sums with FP64 central moments */
+enum sxg_option_id { SXG_OPT_DEB_LDS_SMALL = 0, SXG_OPT_DEB_LDS_LARGE, SXG_OPT_PRECISION,
+                     SXG_NOPT };
 /* Returns 0, or -1 for an unknown option or a value out of range. */
 int sxg_set_option(int id, long value);
 long sxg_get_option(int id);

@@ -55,24 +55,38 @@ import sexgpu  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per launch of each kernel, in units of the frame size N (4-byte
-# pixels), the number of detected pixels P_all, and raw-object pixels P0:
+# Algorithmic HBM bytes per frame of each kernel family (its launches together), from the
+# frame size N (4-byte pixels) and the frame's counts (sxg_frame_info):
 #   copy: read + write the frame; meshstat/meshmode: read the frame; subback: read +
 #   write the frame; conv: read the subtracted frame + write the filtered frame;
-#   det_init: read cd, write parents and labels; deb_levels/deb_objects/analyse: per raw
-#   pixel read its index, cd, the subtracted value and label/level entries (16 B) and
-#   write its label/allocation (4 B).
+#   det_init: read cd, write parents and labels; det_fg: read the parents, write the
+#   foreground list; deb_levels: per pixel of the multi-peak raw objects read its index
+#   and cd (8 B), plus one 4-byte level label per written label; deb_objects/analyse:
+#   per raw pixel read its index, cd, the subtracted value and its label (16 B) and
+#   write its allocation (4 B).
 ALG_BYTES = {
-    "copy": lambda N, P, P0: 8 * N,
-    "meshstat": lambda N, P, P0: 4 * N,
-    "meshmode": lambda N, P, P0: 4 * N,
-    "subback": lambda N, P, P0: 8 * N,
-    "conv": lambda N, P, P0: 8 * N,
-    "det_init": lambda N, P, P0: 12 * N,
-    "det_fg": lambda N, P, P0: 8 * N + 4 * P,
-    "deb_levels": lambda N, P, P0: 20 * P0,
-    "deb_objects": lambda N, P, P0: 20 * P0,
-    "analyse": lambda N, P, P0: 16 * P0,
+    "copy": lambda N, i: 8 * N,
+    "meshstat": lambda N, i: 4 * N,
+    "meshmode": lambda N, i: 4 * N,
+    "subback": lambda N, i: 8 * N,
+    "conv": lambda N, i: 8 * N,
+    "det_init": lambda N, i: 12 * N,
+    "det_fg": lambda N, i: 4 * N + 4 * i["npix_above"],
+    "deb_levels": lambda N, i: 8 * i["deb_pixels"] + 4 * i["deb_labels"],
+    "deb_objects": lambda N, i: 20 * i["npix_raw"],
+    "analyse": lambda N, i: 20 * i["npix_raw"],
+}
+
+
+# HBM bytes per frame of each kernel family, measured on the crowded 4096x4096 frame
+# (seed 1001) with two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by the
+# same counters on k_copy_frame, which moves a known 64 MiB each way (factors 2.0 and
+# 1.0); see profiles/hbm_traffic_crowded4096_pmc.md.  Used for roofline.traffic on that
+# workload only.
+MEASURED_TRAFFIC_CROWDED4096 = {
+    "deb_levels": 5.312e7, "meshstat": 1.345e8, "meshmode": 6.849e7, "analyse": 6.277e7,
+    "deb_objects": 1.384e8, "det_union": 2.58e8, "conv": 2.003e8, "subback": 1.51e8,
+    "det_init": 2.013e8, "copy": 1.342e8, "clean": 1.08e7, "filterback": 8.89e5,
 }
 
 
@@ -244,7 +258,7 @@ def wide8192(args):
     dom = max((k for k in ktab if k in ALG_BYTES), key=lambda k: ktab[k][0])
     tot, nl = ktab[dom]
     avg_s = tot / nl / 1000.0
-    achieved = ALG_BYTES[dom](N, info["npix_above"], info["npix_raw"]) / avg_s / 1e9
+    achieved = ALG_BYTES[dom](N, info) / avg_s / 1e9
     line = {
         "metric": "Mpix/s + objects/s at 8192^2 with MAP_WEIGHT (wide field); bit-exact seg map",
         "value": round(N * steps / dt / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": steps,
@@ -464,11 +478,15 @@ def main():
     dom = max((k for k in ktab if k in ALG_BYTES), key=lambda k: ktab[k][0])
     tot, nl = ktab[dom]
     avg_s = tot / nl / 1000.0
-    bytes_launch = ALG_BYTES[dom](N, info["npix_above"], info["npix_raw"])
+    bytes_launch = ALG_BYTES[dom](N, info)
     achieved = bytes_launch / avg_s / 1e9
+    traffic = None
+    if args.size == 4096 and dom in MEASURED_TRAFFIC_CROWDED4096:
+        traffic = MEASURED_TRAFFIC_CROWDED4096[dom]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": dom, "kernel_ms": round(avg_s * 1e3, 4),
+                "algorithmic_bytes": bytes_launch,
                 "share_of_frame": round(tot / (dt1 * 1e3), 3)}
     # whole path against the per-frame algorithmic bytes 12N + 12 P_all
     balg = 12 * N + 12 * info["npix_above"]

@@ -299,6 +299,8 @@ extern "C" int sxg_get_frame_info(sxg_frame_info *info)
     info->nsurv = S.nsurv;
     info->nbx = S.nbx;
     info->nby = S.nby;
+    info->deb_pixels = S.deb_pixels;
+    info->deb_labels = S.deb_labels;
     return 0;
 }
 

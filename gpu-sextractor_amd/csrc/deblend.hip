@@ -192,6 +192,7 @@ struct DebArgs {
     int *clist;             /* objects of each size class: clist[cls * nraw + j] */
     int *ccount;            /* [0..2] objects per class, [3] largest object of class 1,
                                [4] consistency error flag */
+    unsigned long long *dwork;  /* [0] pixels, [1] level labels of the multi-peak objects */
     /* global scratch for objects too large for LDS, indexed by raw-list position */
     float *g_cdv;
     unsigned char *g_m;
@@ -231,6 +232,7 @@ __global__ void __launch_bounds__(256) k_deb_prep(DebArgs a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r < 5) a.ccount[r] = 0;
+    if (r < 2) a.dwork[r] = 0ull;
     if (r >= a.nraw) return;
     a.top1[r] = 0ull;
     a.top2[r] = 0u;
@@ -269,6 +271,8 @@ __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap_small, 
         return;
     }
     const int cls = n <= cap_small ? 0 : n <= cap_large ? 1 : 2;
+    atomicAdd(&a.dwork[0], (unsigned long long)n);
+    atomicAdd(&a.dwork[1], (unsigned long long)n * ls);
     const int j = atomicAdd(&a.ccount[cls], 1);
     a.clist[(size_t)cls * a.nraw + j] = r;
     if (cls == 1) atomicMax(&a.ccount[3], n);
@@ -861,8 +865,9 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     int *lstar = sxg_take<int>("deb.lstar", nr);
     int *clist = sxg_take<int>("deb.clist", 3 * nr);
     int *ccount = sxg_take<int>("deb.ccount", 8);
+    auto *dwork = sxg_take<unsigned long long>("deb.dwork", 2);
     if (!compoff || !dtot || !rootnc || !rootkept || !objbase || !alloc || !top1 || !top2 ||
-        !lstar || !clist || !ccount)
+        !lstar || !clist || !ccount || !dwork)
         return -1;
     long ncomp_cap = 0;
     if (nraw > 0) {
@@ -883,6 +888,8 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     a.W = S.W; a.H = S.H; a.nraw = nraw; a.comps = comps; a.compoff = compoff;
     a.rootnc = rootnc; a.rootkept = rootkept;
     a.top1 = top1; a.top2 = top2; a.lstar = lstar; a.clist = clist; a.ccount = ccount;
+    a.dwork = dwork;
+    S.deb_pixels = S.deb_labels = 0;
     int nobj = 0;
     if (nraw > 0) {
         const int cap_small = (int)sxg_option(SXG_OPT_DEB_LDS_SMALL);
@@ -931,7 +938,11 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         sxg_scan_excl(rootkept, objbase, nraw, reinterpret_cast<int *>(dtot), st);
         SXG_HIP_RET(hipMemcpyAsync(&nobj, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
         SXG_HIP_RET(hipMemcpyAsync(cc, ccount + 4, sizeof(int), hipMemcpyDeviceToHost, st), -1);
+        unsigned long long dw[2] = {0, 0};
+        SXG_HIP_RET(hipMemcpyAsync(dw, dwork, sizeof(dw), hipMemcpyDeviceToHost, st), -1);
         SXG_HIP_RET(hipStreamSynchronize(st), -1);
+        S.deb_pixels = (int64_t)dw[0];
+        S.deb_labels = (int64_t)dw[1];
         if (cc[0] != 0) {
             sxg_set_error("parcel_out: inconsistent deblending levels (internal error)");
             return -1;

@@ -104,6 +104,7 @@ struct SxgState {
     float dthresh = 0.0f, thresh = 0.0f;
     int ext_minarea = 5;
     int64_t npix_above = 0, npix_raw = 0;
+    int64_t deb_pixels = 0, deb_labels = 0;   /* level-by-level deblending work */
     int nraw = 0;
     bool filtered = false;  /* cudaFilter ran on this frame */
     int *fpia = nullptr;      /* int[P0]: pixels grouped by raw object, raster order inside */

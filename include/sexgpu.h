@@ -133,6 +133,9 @@ typedef struct {
     float backmean, backsig, dthresh, thresh;
     int64_t npix_above, npix_raw;
     int32_t nraw, nobj, nsurv, nbx, nby;
+    /* deblending work: pixels of the raw objects with two or more local maxima (the only
+       ones processed level by level), and the level labels written for them */
+    int64_t deb_pixels, deb_labels;
 } sxg_frame_info;
 int sxg_get_frame_info(sxg_frame_info *info);
 

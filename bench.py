@@ -72,7 +72,8 @@ ALG_BYTES = {
     "conv": lambda N, i: 8 * N,
     "det_init": lambda N, i: 12 * N,
     "det_fg": lambda N, i: 4 * N + 4 * i["npix_above"],
-    "deb_levels": lambda N, i: 8 * i["deb_pixels"] + 4 * i["deb_labels"],
+    # raw list entry and cd in, the kept-object index out, per pixel of a tree
+    "deb_levels": lambda N, i: 12 * i["deb_pixels"],
     "deb_objects": lambda N, i: 20 * i["npix_raw"],
     "analyse": lambda N, i: 20 * i["npix_raw"],
 }
@@ -84,7 +85,7 @@ ALG_BYTES = {
 # 1.0); see profiles/hbm_traffic_crowded4096_pmc.md.  Used for roofline.traffic on that
 # workload only.
 MEASURED_TRAFFIC_CROWDED4096 = {
-    "deb_levels": 5.312e7, "meshstat": 1.345e8, "meshmode": 6.849e7, "analyse": 6.277e7,
+    "meshstat": 1.345e8, "meshmode": 6.849e7, "analyse": 6.277e7,
     "deb_objects": 1.384e8, "det_union": 2.58e8, "conv": 2.003e8, "subback": 1.51e8,
     "det_init": 2.013e8, "copy": 1.342e8, "clean": 1.08e7, "filterback": 8.89e5,
 }

@@ -36,29 +36,30 @@
  *
  * Kernels:
  *   k_deb_thresh     one lane per (object, level): the sub-thresholds (canonical pow)
- *   k_deb_lmax<0,1>  one lane per raw pixel: local maxima; per object the highest
- *                    (packed value and index, 64-bit atomicMax) then the second-highest
- *   k_deb_classify   one lane per object: L*, the level-0 node, single-peak objects
- *                    finished here; the rest binned by size into LDS classes
- *   k_deb_tree_lds   one workgroup per object whose pixel list fits in LDS: cd, m(p),
- *                    backward-neighbour indices (16-bit), union-find parents, labels
- *                    and sizes all in LDS; per level: union-find with LDS atomicMin
- *                    (smallest list index = first raster pixel becomes the root),
- *                    sizes by LDS atomics, kept components numbered in raster order by
- *                    a workgroup ballot prefix, labels written to lvl[level][pixel];
- *                    fluxes of the children of multi-child nodes by one wavefront each
- *                    (ballot + readlane keeps the raster-order float sum); then the
- *                    tree rules level by level, bottom-up, with atomics on the node table
- *   k_deb_tree_glb   the same code on global scratch for objects too large for LDS
- *   k_deb_objlist    one wavefront per raw object: its kept components -> objects
- *   k_deb_pre        one wavefront per object: FULL preanalysis over the component's
- *                    level pixels (lanes compute the terms, the raster-order sums are
- *                    transposed folds: sxg_wave.h)
- *   k_deb_gather     one lane per raw pixel: gatherup of the pixels no kept component
- *                    holds
+ *   k_deb_lmax<0,1>  one lane per raw pixel: local maxima; per object their number, the
+ *                    highest (packed value and index, 64-bit atomicMax), then the
+ *                    second-highest
+ *   k_deb_lsfirst    one lane per (object, level): L* from the second-highest maximum
+ *   k_deb_nodecap    one lane per object: a bound on the tree's nodes (1 + L* times the
+ *                    smaller of the local maxima and n / deb_minarea), scanned into the
+ *                    node table's regions
+ *   k_deb_classify   one lane per object: single-peak objects are finished; the rest are
+ *                    binned by size (and node bound) into LDS classes
+ *   k_deb_tree_lds   one workgroup per object (64, 256 or 1024 lanes by size; the classes
+ *                    run side by side on their own streams): cd, m(p), neighbours,
+ *                    union-find parents, sizes and the node table in LDS; the tree is
+ *                    built from the top level down with one union-find that only grows
+ *                    (see deb_tree)
+ *   k_deb_tree_glb   the same code on global memory for the largest objects
+ *   k_deb_objlist    one wavefront per raw object: its kept nodes -> objects
+ *   k_deb_pre        one wavefront per object: FULL preanalysis over the node's level
+ *                    pixels (kcomp names them; lanes compute the terms, the raster-order
+ *                    sums are transposed folds: sxg_wave.h)
+ *   k_deb_gather     one lane per raw pixel: gatherup of the pixels no kept node holds
  *   then the raw pixels are regrouped by object with the stable radix sort (opix,
  *   objoff): every later per-object pass reads one contiguous raster-ordered range
  */
+#include <algorithm>
 #include <cmath>
 
 #include "sxg_internal.h"
@@ -74,6 +75,11 @@ __device__ inline float uord_inv(unsigned u) { return ord2f((int)(u ^ 0x80000000
 /* loads of words that other wavefronts of the workgroup update with atomics; for global
    memory they go to the device-coherent level */
 template <bool G> __device__ inline int ldw(const int *p)
+{
+    if (G) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
+}
+template <bool G> __device__ inline double ldw(const double *p)
 {
     if (G) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return *p;
@@ -95,12 +101,25 @@ template <bool G> __device__ inline int uf_find(int *par, int x)
     return x;
 }
 
-/* lock-free union: the larger root is linked to the smaller index */
+/* lock-free union: the larger root is linked to the smaller index; the two paths are
+   walked side by side (path halving on both), so their loads overlap */
 template <bool G> __device__ inline void uf_unite(int *par, int a, int b)
 {
     while (true) {
-        a = uf_find<G>(par, a);
-        b = uf_find<G>(par, b);
+        int pa = ldw<G>(&par[a]), pb = ldw<G>(&par[b]);
+        while (pa != a || pb != b) {
+            const int ga = ldw<G>(&par[pa]), gb = ldw<G>(&par[pb]);
+            if (pa != a) {
+                if (ga != pa) atomicMin(&par[a], ga);
+                a = pa;
+                pa = ga;
+            }
+            if (pb != b) {
+                if (gb != pb) atomicMin(&par[b], gb);
+                b = pb;
+                pb = gb;
+            }
+        }
         if (a == b) return;
         if (a < b) {
             const int t = a;
@@ -118,6 +137,10 @@ template <int NT> __device__ inline int wg_prefix(bool f, int *wsum, int &total)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const unsigned long long b = __ballot(f);
+    if (NT == 64) {
+        total = __popcll(b);
+        return __popcll(b & sxw_lanes_below(lane));
+    }
     if (lane == 0) wsum[wid] = __popcll(b);
     __syncthreads();
     int before = 0, tot = 0;
@@ -170,37 +193,51 @@ __global__ void __launch_bounds__(256) k_deb_thresh(const float *__restrict__ fd
 
 
 /* ------------------------------------------------------------------- levels ---- */
+/* node of the tree while it is built (LDS, or global memory for the largest objects);
+   index = creation order (levels from the top down, raster order inside a level) */
+struct DebNode {
+    int level, parent, count, nchild, nson, flags;
+    int kroot;              /* list index of the first pixel */
+    int kanc;               /* the kept node among the node and its ancestors, -1 if none */
+    double ssum, sabs;      /* sums of cd and |cd| over the pixels (any order) */
+};
+
 struct DebArgs {
     const int *fpia;        /* raw pixel list (raster order within each raw object) */
     const int *rawkey;      /* raw object id of each raw-list entry */
     const int *off, *cnt, *first;   /* raw objects */
     const float *cd;
     const int *lab;         /* detection labels (root pixel of each detected pixel) */
-    int *lvl;               /* lvl[level * P0 + g]: component of the raw pixel g, or -1 */
+    int *kcomp;             /* kcomp[g]: the kept node (of a raw object with a tree) whose
+                               level pixels hold the raw pixel g, or -1 */
     long P0;
     const float *thr;       /* thr[r * nthresh + i] */
     int nthresh, debarea;
     double mincont;
     int W, H, nraw;
-    Comp *comps;            /* component table */
-    const long *compoff;    /* region of each raw object in the table */
-    int *rootnc;            /* components per raw object */
+    Comp *comps;            /* output node table, region compoff[r] of each raw object */
+    DebNode *nodes;         /* build table of the objects built in global memory */
+    int *compoff;           /* node capacity of each raw object, then its region */
+    int *rootnc;            /* nodes per raw object */
     int *rootkept;          /* kept objects per raw object */
     unsigned long long *top1;   /* highest local maximum: uord(cd) << 32 | raw-list index */
     unsigned *top2;         /* uord(cd) of the second-highest local maximum, 0 if none */
-    int *lstar;             /* deepest level that can hold two components */
+    int *nmax;              /* local maxima per raw object */
+    int *lstar;             /* first level above the second-highest maximum, then L* */
     int *clist;             /* objects of each size class: clist[cls * nraw + j] */
-    int *ccount;            /* [0..2] objects per class, [3] largest object of class 1,
-                               [4] consistency error flag */
-    unsigned long long *dwork;  /* [0] pixels, [1] level labels of the multi-peak objects */
-    /* global scratch for objects too large for LDS, indexed by raw-list position */
+    int *ccount;            /* [0..3] objects per class, [4..6] largest object of classes
+                               0..2, [7] consistency error flag */
+    unsigned long long *dwork;  /* [0] pixels, [1] pixel-levels of the multi-peak objects,
+                                   [2] node capacity of the frame */
+    /* global scratch for the objects built in global memory, by raw-list position */
     float *g_cdv;
     unsigned char *g_m;
-    int *g_nb;              /* 4 planes of P0 */
-    int *g_par, *g_lab, *g_sz;
+    int *g_nb;              /* neighbour words: 2 ints per raw-list entry */
+    int *g_par, *g_deep, *g_cnt;
 };
 
-/* PASS 0: highest local maximum of each raw object; PASS 1: the highest of the others */
+/* PASS 0: local maxima (no 8-neighbour of the raw object is higher): the highest of each
+   raw object and their number; PASS 1: the highest of the others */
 template <int PASS> __global__ void __launch_bounds__(256) k_deb_lmax(DebArgs a)
 {
     const long g = (long)blockIdx.x * 256 + threadIdx.x;
@@ -222,60 +259,87 @@ template <int PASS> __global__ void __launch_bounds__(256) k_deb_lmax(DebArgs a)
     }
     const unsigned u = uord(v);
     const unsigned long long key = ((unsigned long long)u << 32) | (unsigned)g;
-    if (PASS == 0)
+    if (PASS == 0) {
         atomicMax(&a.top1[r], key);
-    else if (a.top1[r] != key)
+        atomicAdd(&a.nmax[r], 1);
+    } else if (a.top1[r] != key) {
         atomicMax(&a.top2[r], u);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_deb_prep(DebArgs a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r < 5) a.ccount[r] = 0;
-    if (r < 2) a.dwork[r] = 0ull;
+    if (r < 8) a.ccount[r] = 0;
+    if (r < 3) a.dwork[r] = 0ull;
     if (r >= a.nraw) return;
     a.top1[r] = 0ull;
     a.top2[r] = 0u;
+    a.nmax[r] = 0;
+    a.lstar[r] = a.nthresh;
 }
 
-/* L*, the level-0 node, single-peak objects, size classes */
-__global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap_small, int cap_large)
+/* one lane per (raw object, level >= 1): the first level whose running-maximum
+   sub-threshold tau exceeds the second-highest local maximum (tau_i > v2 iff some
+   t_j > v2, j <= i; a NaN sub-threshold empties its level and ends the descent) */
+__global__ void __launch_bounds__(256) k_deb_lsfirst(DebArgs a)
+{
+    const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    const int N = a.nthresh;
+    if (k >= (long)a.nraw * N) return;
+    const int r = (int)(k / N), i = (int)(k - (long)r * N);
+    const unsigned u2 = a.top2[r];
+    if (i == 0 || u2 == 0u) return;
+    const float t = a.thr[k];
+    if (t != t || t > uord_inv(u2)) atomicMin(&a.lstar[r], i);
+}
+
+/* node capacity of a tree with L* = ls: every node of level i holds a local maximum
+   with m >= i (its highest pixel) and has at least deb_minarea pixels, and the nodes of
+   a level are disjoint, so at most 1 + ls * min(local maxima, n / deb_minarea) */
+__host__ __device__ inline int deb_nodecap(int nmax, int n, int ls, int debarea)
+{
+    if (ls == 0) return 1;
+    const long per = nmax < n / debarea ? nmax : n / debarea;
+    const long c = 1 + (long)ls * per;
+    return c < 0x7fffffff ? (int)c : 0x7fffffff;
+}
+
+/* L* and the node capacity of each raw object (the total in 64 bits) */
+__global__ void __launch_bounds__(256) k_deb_nodecap(DebArgs a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= a.nraw) return;
-    const int n = a.cnt[r], N = a.nthresh;
-    const float *t = a.thr + (size_t)r * N;
-    int ls = 0;
-    const unsigned u2 = a.top2[r];
-    if (u2 != 0u) {
-        const float v2 = uord_inv(u2);
-        float tau = -INFINITY;
-        ls = N - 1;
-        for (int i = 1; i < N; i++) {
-            /* a NaN sub-threshold empties its level and ends the descent */
-            if (t[i] != t[i] || (tau = fmaxf(tau, t[i])) > v2) {
-                ls = i - 1;
-                break;
-            }
-        }
-    }
-    Comp c0;
-    c0.level = 0; c0.parent = -1; c0.first = a.first[r]; c0.count = n;
-    c0.fdflux = 0.0f; c0.dthresh = t[0]; c0.nchild = 0; c0.nson = 0;
-    c0.flags = ls == 0 ? COMP_KEPT : 0;
-    a.comps[a.compoff[r]] = c0;
+    const int ls = a.top2[r] == 0u ? 0 : a.lstar[r] - 1;
     a.lstar[r] = ls;
+    const int c = deb_nodecap(a.nmax[r], a.cnt[r], ls, a.debarea);
+    a.compoff[r] = c;
+    atomicAdd(&a.dwork[2], (unsigned long long)c);
+}
+
+/* single-peak objects are kept as they are (one node); the others go to a size class (0: up to
+   cap0 pixels, 64-lane LDS workgroups; 1: up to cap1, 256 lanes; 2: up to cap2, 1024
+   lanes; 3: global memory), and to class 3 whenever their tree may outgrow the class's
+   LDS node table (ct0..ct2 nodes) */
+__global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int cap1, int cap2,
+                                                      int ct0, int ct1, int ct2)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= a.nraw) return;
+    const int n = a.cnt[r], ls = a.lstar[r];
     if (ls == 0) {
         a.rootnc[r] = 1;
         a.rootkept[r] = 1;
         return;
     }
-    const int cls = n <= cap_small ? 0 : n <= cap_large ? 1 : 2;
+    const int cap = deb_nodecap(a.nmax[r], n, ls, a.debarea);
+    int cls = n <= cap0 ? 0 : n <= cap1 ? 1 : n <= cap2 ? 2 : 3;
+    if ((cls == 0 && cap > ct0) || (cls == 1 && cap > ct1) || (cls == 2 && cap > ct2)) cls = 3;
     atomicAdd(&a.dwork[0], (unsigned long long)n);
     atomicAdd(&a.dwork[1], (unsigned long long)n * ls);
     const int j = atomicAdd(&a.ccount[cls], 1);
     a.clist[(size_t)cls * a.nraw + j] = r;
-    if (cls == 1) atomicMax(&a.ccount[3], n);
+    if (cls < 3) atomicMax(&a.ccount[4 + cls], n);
 }
 
 /*
@@ -307,39 +371,76 @@ __device__ inline int deb_sig_bound(double s, double sabs, int n, float b, doubl
     return -1;
 }
 
+/* backward neighbours of a raw-list entry: the present ones among NW, N, NE are
+   consecutive list entries from j0; W is the previous entry */
+#define NB_W 1
+#define NB_NW 2
+#define NB_N 4
+#define NB_NE 8
+template <typename IT> struct NbP {
+    IT j0, mask;
+};
+
 /*
- * The component tree of raw object r, levels 1..L*.  cdv, m, par, lab, sz and the four
- * neighbour planes nb[d * nbs + k] are per raw-list index k (LDS or global scratch);
- * pix is the object's raw pixel list (pixw: where to copy it first, LDS only).
+ * The component tree of raw object r, built from the top level down.  Per raw-list
+ * index k (LDS, or global scratch when G): cdv (cd), m (level index), nb (backward
+ * neighbours), par (union-find parent), cnt (pixel count of a root, then its node's
+ * index within the level, -1 if dropped), deep (the highest node holding k; 0 = the
+ * root).  pixw: where to copy the pixel list first (LDS only; it shares cnt).  T: the
+ * build table (at least 1 + local maxima * L* nodes).
+ *
+ * V_i = {k : m[k] >= i} only grows as i falls, so one union-find serves every level: at
+ * level i only the pixels that enter (m == i) are joined, to each other and to what is
+ * there; the paths stay short because every pass leaves each pixel pointing at its root.
+ * Entering pixels first point at the head of their run of entering pixels (a ballot
+ * over the 64-entry window); of the vertical links only those that the W neighbour does
+ * not already imply are joined (8-connectivity: the row above is joined at the first
+ * pixel of each overlapping stretch), and of those only the ones with an entering end.
+ * A node's parent is the node of the level below that holds its first pixel; its
+ * significance is decided as soon as it exists, when its level has two or more nodes
+ * (otherwise no parent can have two children there).
  */
-template <int NT, bool G, typename NB>
+template <int NT, bool G, typename IT>
 __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *cdv,
-                                         unsigned char *m, NB *nb, const long nbs, int *par,
-                                         int *lab, int *sz, int *pixw, const int *pix)
+                                         unsigned char *m, NbP<IT> *nb, int *par, int *cnt,
+                                         IT *deep, int *pixw, const int *pix, DebNode *T)
 {
     __shared__ int wsum[NT / 64];
-    __shared__ int levstart[SXG_MAXNTHRESH + 1];
+    __shared__ int levstart[SXG_MAXNTHRESH + 1], levcnt[SXG_MAXNTHRESH + 1];
+    __shared__ int vcount[SXG_MAXNTHRESH + 1];
     __shared__ float tthr[SXG_MAXNTHRESH], ttau[SXG_MAXNTHRESH];
-    __shared__ int s_multi, s_nk;
+    __shared__ int s_multi, s_nk, s_top;
     __shared__ double s_rsum[NT / 64], s_rabs[NT / 64], s_v0lo, s_v0hi;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = a.cnt[r], off = a.off[r], ls = a.lstar[r];
     const int W = a.W;
-    const long P0 = a.P0;
-    Comp *C = a.comps + a.compoff[r];
-    const float *t = a.thr + (size_t)r * a.nthresh;
-    const NB none = (NB)-1;
-    if (tid == 0) {
-        float tau = -INFINITY;
-        for (int i = 1; i <= ls; i++) {
-            tau = fmaxf(tau, t[i]);
-            tthr[i] = t[i];
-            ttau[i] = tau;
+    Comp *const C = a.comps + a.compoff[r];
+    if (wid == 0) {
+        /* sub-thresholds 1..L* and their running maximum tau (L* <= 63: one lane each) */
+        const float v = lane < ls ? a.thr[(size_t)r * a.nthresh + lane + 1] : -INFINITY;
+        float tau = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const float o = __shfl_up(tau, d);
+            if (lane >= d) tau = fmaxf(tau, o);
         }
+        if (lane < ls) {
+            tthr[lane + 1] = v;
+            ttau[lane + 1] = tau;
+        }
+        vcount[lane] = 0;
+        levcnt[lane] = 0;
+        if (lane == 0) {
+            vcount[64] = 0;
+            levcnt[64] = 0;
+        }
+    }
+    if (tid == 0) {
         s_multi = 0;
         s_nk = 0;
-        levstart[0] = 0;
-        levstart[1] = 1;
+        DebNode e;
+        e.level = 0; e.parent = -1; e.count = n; e.nchild = 0; e.nson = 0; e.flags = 0;
+        e.kroot = 0; e.kanc = -1; e.ssum = 0.0; e.sabs = 0.0;
+        T[0] = e;
     }
     for (int k = tid; k < n; k += NT) {
         const int p = a.fpia[off + k];
@@ -347,38 +448,21 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         cdv[k] = a.cd[p];
     }
     __syncthreads();
-    /* level index and backward neighbours (W, NW, N, NE) by position in the list; bounds
-       of the root's flux */
+    /* level index m = max{i <= L* : tau_i <= cd} (tau is non-decreasing), its histogram,
+       bounds of the root's flux */
     double rsum = 0.0, rabs = 0.0;
     for (int k = tid; k < n; k += NT) {
         const float v = cdv[k];
         rsum += v;
         rabs += fabs(v);
-        int mk = 0;
-        while (mk < ls && ttau[mk + 1] <= v) mk++;
-        m[k] = (unsigned char)mk;
-        const int p = pix[k];
-        const int y = p / W, x = p - y * W;
-        NB w = none, nw = none, nn = none, ne = none;
-        if (x > 0 && k > 0 && pix[k - 1] == p - 1) w = (NB)(k - 1);
-        if (y > 0) {
-            const int q0 = p - W - (x > 0 ? 1 : 0);
-            int lo = 0, hi = k;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (pix[mid] < q0) lo = mid + 1;
-                else hi = mid;
-            }
-            int j = lo;
-            if (x > 0 && j < k && pix[j] == p - W - 1) nw = (NB)(j++);
-            if (j < k && pix[j] == p - W) nn = (NB)(j++);
-            if (x < W - 1 && j < k && pix[j] == p - W + 1) ne = (NB)j;
+        int lo = 0, hi = ls;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ttau[mid] <= v) lo = mid;
+            else hi = mid - 1;
         }
-        nb[k] = w;
-        nb[nbs + k] = nw;
-        nb[2 * nbs + k] = nn;
-        nb[3 * nbs + k] = ne;
-        lab[k] = 0;
+        m[k] = (unsigned char)lo;
+        atomicAdd(&vcount[lo], 1);
     }
     rsum = sxw_sum(rsum);
     rabs = sxw_sum(rabs);
@@ -398,177 +482,318 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         const double p = (rs - e) * a.mincont, q = (rs + e) * a.mincont;
         s_v0lo = fmin(p, q) - fabs(fmin(p, q)) * 0x1p-50 - 1e-300;
         s_v0hi = fmax(p, q) + fabs(fmax(p, q)) * 0x1p-50 + 1e-300;
-    }
-    int nc = 1, last = 0;
-    for (int i = 1; i <= ls; i++) {
-        const float ti = tthr[i];
-        int anyl = 0;
-        for (int k = tid; k < n; k += NT) {
-            const bool in = m[k] >= i;
-            par[k] = in ? k : -1;
-            sz[k] = 0;
-            anyl |= in;
+        /* |V_i| = pixels with m >= i; levels where it is below deb_minarea hold no node */
+        int top = 0, acc = 0;
+        for (int i = ls; i >= 1; i--) {
+            acc += vcount[i];
+            if (top == 0 && acc >= a.debarea) top = i;
         }
-        if (!__syncthreads_or(anyl)) break;
+        s_top = top;
+    }
+    /* backward neighbours by position in the list (binary search of the row above) */
+    for (int k = tid; k < n; k += NT) {
+        const int p = pix[k];
+        const int y = p / W, x = p - y * W;
+        int mask = 0, j0 = 0;
+        if (x > 0 && k > 0 && pix[k - 1] == p - 1) mask |= NB_W;
+        if (y > 0) {
+            const int q0 = p - W - (x > 0 ? 1 : 0);
+            int l = 0, h = k;
+            while (l < h) {
+                const int mid = (l + h) >> 1;
+                if (pix[mid] < q0) l = mid + 1;
+                else h = mid;
+            }
+            j0 = l;
+            int j = l;
+            if (x > 0 && j < k && pix[j] == p - W - 1) {
+                mask |= NB_NW;
+                j++;
+            }
+            if (j < k && pix[j] == p - W) {
+                mask |= NB_N;
+                j++;
+            }
+            if (x < W - 1 && j < k && pix[j] == p - W + 1) mask |= NB_NE;
+        }
+        NbP<IT> b;
+        b.j0 = (IT)j0;
+        b.mask = (IT)mask;
+        nb[k] = b;
+        deep[k] = 0;
+    }
+    __syncthreads();
+    const int top = s_top;
+    /* run heads of the pixels entering at the top level (every pixel of V_top) */
+    auto prelink = [&](int lev, int enter_hi) {
+        for (int base = 0; base < n; base += NT) {
+            const int k = base + tid;
+            const int mk = k < n ? (int)m[k] : -1;
+            const bool ent = mk >= lev && mk < enter_hi;
+            bool wen = false;
+            if (ent && (nb[k].mask & NB_W)) {
+                const int mw = m[k - 1];
+                wen = mw >= lev && mw < enter_hi;
+            }
+            const unsigned long long S = __ballot(ent && !wen);
+            if (ent) {
+                const unsigned long long le = S & (~0ull >> (63 - lane));
+                par[k] = k - lane + (le ? 63 - __builtin_clzll(le) : 0);
+            }
+        }
+    };
+    if (top > 0) prelink(top, ls + 1);
+    int nc = 1, last = 0, p0 = 0, p1 = 0, prev = ls + 1;
+    for (int i = top; i >= 1; i--) {
+        __syncthreads();
+        /* join the entering pixels */
         for (int k = tid; k < n; k += NT) {
-            if (m[k] < i) continue;
-            /* W, NW, N, NE; NW and NE are already joined through N when N is in the
-               set (they are its W and E neighbours), NW also through W */
-            const NB jw = nb[k], jnw = nb[nbs + k], jn = nb[2 * nbs + k], jne = nb[3 * nbs + k];
-            const bool w = jw != none && m[jw] >= i, nn = jn != none && m[jn] >= i;
-            if (w) uf_unite<G>(par, k, (int)jw);
+            cnt[k] = 0;
+            const int mk = m[k];
+            if (mk < i) continue;
+            const bool ek = mk < prev;
+            const NbP<IT> b = nb[k];
+            const int msk = (int)b.mask, j0 = (int)b.j0;
+            const int jnw = j0, jn = j0 + ((msk & NB_NW) ? 1 : 0);
+            const int jne = jn + ((msk & NB_N) ? 1 : 0);
+            const int mw = (msk & NB_W) ? (int)m[k - 1] : -1;
+            const int mnw = (msk & NB_NW) ? (int)m[jnw] : -1;
+            const int mn = (msk & NB_N) ? (int)m[jn] : -1;
+            const int mne = (msk & NB_NE) ? (int)m[jne] : -1;
+            const bool wl = mw >= i, nw = mnw >= i, nn = mn >= i, ne = mne >= i;
+            /* runs of entering pixels are linked inside their window already */
+            if (wl && (ek || mw < prev) && !(ek && mw < prev && (k & 63) != 0))
+                uf_unite<G>(par, k, k - 1);
             if (nn) {
-                uf_unite<G>(par, k, (int)jn);
+                /* the W neighbour's N is this pixel's NW */
+                if (!(wl && nw) && (ek || mn < prev)) uf_unite<G>(par, k, jn);
             } else {
-                if (!w && jnw != none && m[jnw] >= i) uf_unite<G>(par, k, (int)jnw);
-                if (jne != none && m[jne] >= i) uf_unite<G>(par, k, (int)jne);
+                if (nw && !wl && (ek || mnw < prev)) uf_unite<G>(par, k, jnw);
+                if (ne && (ek || mne < prev)) uf_unite<G>(par, k, jne);
             }
         }
         __syncthreads();
+        /* roots (the smallest list index = first raster pixel) and sizes; a concurrent
+           path-halving step can only lower par[k] towards the same root */
         for (int k = tid; k < n; k += NT)
-            if (m[k] >= i) par[k] = uf_find<G>(par, k);
+            if (m[k] >= i) {
+                const int rt = uf_find<G>(par, k);
+                par[k] = rt;
+                atomicAdd(&cnt[rt], 1);
+            }
         __syncthreads();
-        for (int k = tid; k < n; k += NT)
-            if (m[k] >= i) atomicAdd(&sz[par[k]], 1);
-        __syncthreads();
-        /* kept components in raster order of their first pixel */
+        /* the nodes of level i in raster order of their first pixel */
         int nkept = 0;
         for (int base = 0; base < n; base += NT) {
             const int k = base + tid;
             const bool isroot = k < n && m[k] >= i && ldw<G>(&par[k]) == k;
-            const int cnt = isroot ? ldw<G>(&sz[k]) : 0;
-            const bool kp = isroot && cnt >= a.debarea;
+            const int ck = isroot ? ldw<G>(&cnt[k]) : 0;
+            const bool kp = isroot && ck >= a.debarea;
             int tot;
             const int pre = wg_prefix<NT>(kp, wsum, tot);
             if (isroot) {
-                int kid = -1;
+                int j = -1;
                 if (kp) {
-                    kid = nc + nkept + pre;
-                    int parent = lab[k];
-                    if (parent < 0) {
-                        /* cannot happen when the level sets nest; reported, never written */
-                        atomicOr(&a.ccount[4], 1);
-                        parent = 0;
-                    }
-                    Comp cc;
-                    cc.level = i; cc.parent = parent; cc.first = a.fpia[off + k]; cc.count = cnt;
-                    cc.fdflux = 0.0f; cc.dthresh = ti; cc.nchild = 0; cc.nson = 0; cc.flags = 0;
-                    C[kid] = cc;
-                    if (atomicAdd(&C[parent].nchild, 1) == 1) s_multi = 1;
+                    j = nkept + pre;
+                    DebNode e;
+                    e.level = i; e.parent = -1; e.count = ck; e.nchild = 0; e.nson = 0;
+                    e.flags = 0; e.kroot = k; e.kanc = -1; e.ssum = 0.0; e.sabs = 0.0;
+                    T[nc + j] = e;
                 }
-                sz[k] = kid;
+                cnt[k] = j;
             }
             nkept += tot;
         }
         __syncthreads();
-        for (int k = tid; k < n; k += NT) {
-            const int l = m[k] >= i ? ldw<G>(&sz[ldw<G>(&par[k])]) : -1;
-            lab[k] = l;
-            a.lvl[(size_t)i * P0 + off + k] = l;
+        const bool sums = nkept >= 2;
+        /* parents of the level i + 1 nodes: the node that holds their first pixel */
+        for (int c = p0 + tid; c < p1; c += NT) {
+            const int j = ldw<G>(&cnt[ldw<G>(&par[T[c].kroot])]);
+            int parent = nc + j;
+            if (j < 0) {
+                /* cannot happen when the level sets nest; reported, never used */
+                atomicOr(&a.ccount[7], 1);
+                parent = 0;
+            }
+            T[c].parent = parent;
+            if (atomicAdd(&T[parent].nchild, 1) == 1) s_multi = 1;
+            if (ldw<G>(&T[c].flags) & COMP_SIG) atomicAdd(&T[parent].nson, 1);
         }
-        nc += nkept;
-        last = i;
-        if (tid == 0) levstart[i + 1] = nc;
-        __syncthreads();
-        /* significance of each new component whose parent has two or more children
-           (one wavefront per component; only their significance matters) */
-        for (int c = nc - nkept + wid; c < nc; c += NT / 64) {
-            const int pc = C[c].parent;
-            if (ldw<true>(&C[pc].nchild) < 2) continue;
-            double ps = 0.0, pa = 0.0;
-            for (int k = lane; k < n; k += 64)
-                if (lab[k] == c) {
-                    const double v = cdv[k];
-                    ps += v;
-                    pa += fabs(v);
+        /* the highest node of each pixel; node sums; run heads of the level i - 1
+           entries */
+        for (int base = 0; base < n; base += NT) {
+            const int k = base + tid;
+            const int mk = k < n ? (int)m[k] : -1;
+            int j = -1;
+            float v = 0.0f;
+            if (mk >= i) {
+                j = ldw<G>(&cnt[ldw<G>(&par[k])]);
+                if (j >= 0) {
+                    if (deep[k] == (IT)0) deep[k] = (IT)(nc + j);
+                    v = cdv[k];
                 }
-            ps = sxw_sum(ps);
-            pa = sxw_sum(pa);
-            const int cnt = C[c].count;
-            const float b = C[c].dthresh * (float)cnt;
-            int sig = deb_sig_bound(ps, pa, cnt, b, s_v0lo, s_v0hi);
-            if (sig < 0) {
-                /* too close to call: the exact sequential sums */
-                const float f = wave_ordered_sum(cdv, n, [&](int k) { return lab[k] == c; });
-                const float fr = wave_ordered_sum(cdv, n, [](int) { return true; });
-                const float lhs = f - b;
-                sig = (double)lhs > (double)fr * a.mincont;
-                if (lane == 0) C[c].fdflux = f;
             }
-            if (sig && lane == 0) {
-                atomicOr(&C[c].flags, COMP_SIG);
-                atomicAdd(&C[pc].nson, 1);
+            if (sums && j >= 0) {
+                /* LDS atomics without a return value: the wavefront does not wait */
+                atomicAdd(&T[nc + j].ssum, (double)v);
+                atomicAdd(&T[nc + j].sabs, fabs((double)v));
+            }
+            if (i > 1) {
+                const bool ent = mk == i - 1;
+                bool wen = false;
+                if (ent && (nb[k].mask & NB_W)) wen = m[k - 1] == i - 1;
+                const unsigned long long S = __ballot(ent && !wen);
+                if (ent) {
+                    const unsigned long long le = S & (~0ull >> (63 - lane));
+                    par[k] = k - lane + (le ? 63 - __builtin_clzll(le) : 0);
+                }
             }
         }
-        if (nkept == 0) break;
-    }
-    __syncthreads();
-    if (!s_multi) {
-        /* no node has two children: the raw object is kept as it is */
         if (tid == 0) {
-            C[0].flags = COMP_KEPT;
-            a.rootnc[r] = nc;
-            a.rootkept[r] = 1;
+            levstart[i] = nc;
+            levcnt[i] = nkept;
         }
-        return;
-    }
-    for (int c = tid; c < nc; c += NT) atomicOr(&C[c].flags, COMP_OKA);
-    __syncthreads();
-    /* A(c) = nson(c) <= 1 and A(children), bottom-up one level at a time */
-    for (int L = last; L >= 0; L--) {
-        for (int c = levstart[L] + tid; c < levstart[L + 1]; c += NT) {
-            int fl = ldw<true>(&C[c].flags);
-            if (ldw<true>(&C[c].nson) > 1) {
-                atomicAnd(&C[c].flags, ~COMP_OKA);
-                fl &= ~COMP_OKA;
+        p0 = nc;
+        p1 = nc + nkept;
+        nc += nkept;
+        if (nkept > 0 && last == 0) last = i;
+        prev = i;
+        if (sums) {
+            __syncthreads();
+            /* significance of the new nodes, one wavefront each; the exact sequential
+               float sums when the bounds cannot decide */
+            const float ti = tthr[i];
+            for (int c = p0 + wid; c < p1; c += NT / 64) {
+                const int cc = T[c].count, kr = T[c].kroot;
+                const float b = ti * (float)cc;
+                int sig = deb_sig_bound(ldw<G>(&T[c].ssum), ldw<G>(&T[c].sabs), cc, b, s_v0lo,
+                                        s_v0hi);
+                if (sig < 0) {
+                    const float f = wave_ordered_sum(
+                        cdv, n, [&](int k) { return m[k] >= i && ldw<G>(&par[k]) == kr; });
+                    const float fr = wave_ordered_sum(cdv, n, [](int) { return true; });
+                    const float lhs = f - b;
+                    sig = (double)lhs > (double)fr * a.mincont;
+                }
+                if (sig && lane == 0) atomicOr(&T[c].flags, COMP_SIG);
             }
-            if (c > 0 && !(fl & COMP_OKA)) atomicAnd(&C[C[c].parent].flags, ~COMP_OKA);
+        }
+    }
+    __syncthreads();
+    /* the level 1 nodes hang from the root */
+    for (int c = p0 + tid; c < p1; c += NT) {
+        T[c].parent = 0;
+        if (atomicAdd(&T[0].nchild, 1) == 1) s_multi = 1;
+        if (ldw<G>(&T[c].flags) & COMP_SIG) atomicAdd(&T[0].nson, 1);
+    }
+    __syncthreads();
+    const bool multi = s_multi != 0;
+    if (multi) {
+        for (int c = tid; c < nc; c += NT) atomicOr(&T[c].flags, COMP_OKA);
+        __syncthreads();
+        /* A(c) = nson(c) <= 1 and A(children), bottom-up one level at a time */
+        for (int L = last; L >= 0; L--) {
+            const int c0 = L > 0 ? levstart[L] : 0, c1 = L > 0 ? c0 + levcnt[L] : 1;
+            for (int c = c0 + tid; c < c1; c += NT) {
+                int fl = ldw<G>(&T[c].flags);
+                if (ldw<G>(&T[c].nson) > 1) {
+                    atomicAnd(&T[c].flags, ~COMP_OKA);
+                    fl &= ~COMP_OKA;
+                }
+                if (c > 0 && !(fl & COMP_OKA)) atomicAnd(&T[T[c].parent].flags, ~COMP_OKA);
+            }
+            __syncthreads();
+        }
+        for (int c = tid; c < nc; c += NT) {
+            const int fl = ldw<G>(&T[c].flags);
+            const bool okA = fl & COMP_OKA;
+            const bool kept =
+                c == 0 ? okA : (ldw<G>(&T[T[c].parent].nson) > 1 && okA && (fl & COMP_SIG));
+            if (kept) {
+                atomicOr(&T[c].flags, COMP_KEPT);
+                atomicAdd(&s_nk, 1);
+            }
         }
         __syncthreads();
+        /* the kept node on each node's path to the root (kept nodes never nest) */
+        for (int L = 0; L <= last; L++) {
+            const int c0 = L > 0 ? levstart[L] : 0, c1 = L > 0 ? c0 + levcnt[L] : 1;
+            for (int c = c0 + tid; c < c1; c += NT) {
+                const int fl = ldw<G>(&T[c].flags);
+                T[c].kanc = (fl & COMP_KEPT) ? c : c == 0 ? -1 : T[T[c].parent].kanc;
+            }
+            __syncthreads();
+        }
     }
-    for (int c = tid; c < nc; c += NT) {
-        const int fl = ldw<true>(&C[c].flags);
-        const bool okA = fl & COMP_OKA;
-        const bool kept = c == 0 ? okA
-                                 : (ldw<true>(&C[C[c].parent].nson) > 1 && okA && (fl & COMP_SIG));
-        if (kept) {
-            atomicOr(&C[c].flags, COMP_KEPT);
-            atomicAdd(&s_nk, 1);
+    /* output order: level, then raster order; levbase[L] = first output index of level L */
+    if (tid == 0) {
+        int acc = 1;
+        for (int L = 1; L <= last; L++) {
+            const int c = levcnt[L];
+            levcnt[L] = acc - levstart[L];   /* output index = creation index + this */
+            acc += c;
         }
     }
     __syncthreads();
+    auto outid = [&](int c) { return c == 0 ? 0 : c + levcnt[T[c].level]; };
+    if (multi)
+        for (int k = tid; k < n; k += NT) {
+            const int ka = T[(int)deep[k]].kanc;
+            a.kcomp[off + k] = ka < 0 ? -1 : outid(ka);
+        }
+    /* what the object kernels read; without a node with two children the raw object is
+       kept as it is */
+    for (int c = tid; c < nc; c += NT) {
+        Comp o;
+        o.level = T[c].level;
+        o.first = a.fpia[off + T[c].kroot];
+        o.flags = multi ? (ldw<G>(&T[c].flags) & COMP_KEPT) : (c == 0 ? COMP_KEPT : 0);
+        C[outid(c)] = o;
+    }
     if (tid == 0) {
         a.rootnc[r] = nc;
-        a.rootkept[r] = s_nk;
+        a.rootkept[r] = multi ? s_nk : 1;
     }
 }
 
-/* LDS bytes per pixel of k_deb_tree_lds: cd, parent, label, size (float/int) + four
-   16-bit neighbours + the level index */
-#define DEB_LDS_BYTES_PER_PIX 25
+/* LDS bytes per pixel of k_deb_tree_lds: cd, parent, count (4 bytes each), the
+   neighbour word (2 x 16 bits), the highest node (16 bits), the level index */
+#define DEB_LDS_BYTES_PER_PIX 19
+/* LDS a workgroup may use (dynamic part; the static part of deb_tree is under 2 KiB) */
+#define DEB_LDS_LIMIT ((size_t)160 * 1024 - 4096)
+
+__host__ __device__ inline size_t deb_lds_pixbytes(int cap)
+{
+    return ((size_t)cap * DEB_LDS_BYTES_PER_PIX + 15) & ~(size_t)15;
+}
 
 template <int NT>
 __global__ void __launch_bounds__(NT) k_deb_tree_lds(DebArgs a, int cls, int cap)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r = a.clist[(size_t)cls * a.nraw + blockIdx.x];
-    float *cdv = reinterpret_cast<float *>(smem);
-    int *par = reinterpret_cast<int *>(smem + 4 * (size_t)cap);
-    int *lab = par + cap;
-    int *sz = lab + cap;
-    unsigned short *nb = reinterpret_cast<unsigned short *>(sz + cap);
-    unsigned char *m = reinterpret_cast<unsigned char *>(nb + 4 * (size_t)cap);
-    /* the pixel list is only needed while the neighbours are found: it shares sz */
-    deb_tree<NT, false, unsigned short>(a, r, cdv, m, nb, cap, par, lab, sz, sz, sz);
+    DebNode *tl = reinterpret_cast<DebNode *>(smem);
+    const int ct = deb_nodecap(a.nmax[r], a.cnt[r], a.lstar[r], a.debarea);
+    float *cdv = reinterpret_cast<float *>(smem + (((size_t)ct * sizeof(DebNode) + 15) & ~15));
+    int *par = reinterpret_cast<int *>(cdv + cap);
+    int *cnt = par + cap;
+    NbP<unsigned short> *nb = reinterpret_cast<NbP<unsigned short> *>(cnt + cap);
+    unsigned short *deep = reinterpret_cast<unsigned short *>(nb + cap);
+    unsigned char *m = reinterpret_cast<unsigned char *>(deep + cap);
+    /* the pixel list is only needed while the neighbours are found: it shares cnt */
+    deb_tree<NT, false, unsigned short>(a, r, cdv, m, nb, par, cnt, deep, cnt, cnt, tl);
 }
 
+/* raw objects too large for LDS or with trees that may outgrow it: all state in global
+   memory */
 __global__ void __launch_bounds__(1024) k_deb_tree_glb(DebArgs a)
 {
-    const int r = a.clist[2 * (size_t)a.nraw + blockIdx.x];
+    const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
     const int off = a.off[r];
-    deb_tree<1024, true, int>(a, r, a.g_cdv + off, a.g_m + off, a.g_nb + off, a.P0,
-                              a.g_par + off, a.g_lab + off, a.g_sz + off, nullptr,
-                              a.fpia + off);
+    deb_tree<1024, true, int>(a, r, a.g_cdv + off, a.g_m + off,
+                              reinterpret_cast<NbP<int> *>(a.g_nb) + off, a.g_par + off,
+                              a.g_cnt + off, a.g_deep + off, nullptr, a.fpia + off,
+                              a.nodes + a.compoff[r]);
 }
 
 /* ------------------------------------------------------------------ objects ---- */
@@ -590,7 +815,20 @@ __global__ void __launch_bounds__(256) k_deb_objlist(ObjArgs A)
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= A.d.nraw) return;
     const int nc = A.d.rootnc[r], base = A.objbase[r];
-    const Comp *C = A.d.comps + A.d.compoff[r];
+    Comp *C = A.d.comps + A.d.compoff[r];
+    if (A.d.lstar[r] == 0) {
+        /* a single local maximum: the raw object is the object */
+        if (lane == 0) {
+            Comp c0;
+            c0.level = 0;
+            c0.first = A.d.first[r];
+            c0.flags = COMP_KEPT;
+            C[0] = c0;
+            A.objraw[base] = r;
+            A.objcomp[base] = 0;
+        }
+        return;
+    }
     const unsigned long long below = sxw_lanes_below(lane);
     int done = 0;
     for (int c0 = 0; c0 < nc; c0 += 64) {
@@ -630,7 +868,7 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
     const Comp *C = d.comps + d.compoff[r];
     const int L = C[c].level;
     const int n = d.cnt[r], off = d.off[r], W = d.W, H = d.H;
-    const int *lv = L > 0 ? d.lvl + (size_t)L * d.P0 + off : nullptr;
+    const int *lv = L > 0 ? d.kcomp + off : nullptr;
     /* pass 1: extent, peaks, count; rv is the raster-order float sum of cd */
     float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG, racc = 0.0f;
     int xmin = 2 * SXG_MAXPICSIZE, ymin = 2 * SXG_MAXPICSIZE, xmax = 0, ymax = 0, cnt = 0;
@@ -803,37 +1041,6 @@ extern "C" void init_deblend(float basethresh, int nthresh, size_t numValidObj, 
     S.deb_ready = true;
 }
 
-__global__ void __launch_bounds__(256) k_deb_capacity(const int *cnt, int nraw, int nthresh,
-                                                      int debarea, long *cap)
-{
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r < nraw) cap[r] = 1 + (long)(nthresh - 1) * (cnt[r] / debarea);
-}
-
-/* exclusive scan of nraw longs (single workgroup; the counts are small) */
-__global__ void __launch_bounds__(1024) k_scan_long(long *v, int n, long *total)
-{
-    __shared__ long sm[1024];
-    long carry = 0;
-    for (int base = 0; base < n; base += 1024) {
-        const int k = base + threadIdx.x;
-        long x = k < n ? v[k] : 0;
-        sm[threadIdx.x] = x;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            long add = threadIdx.x >= o ? sm[threadIdx.x - o] : 0;
-            __syncthreads();
-            sm[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (k < n) v[k] = carry + sm[threadIdx.x] - x;
-        carry += sm[1023];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *total = carry;
-}
-
-
 extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_t numValidObj,
                           size_t numValidPix, double deblend_mincont)
 {
@@ -854,95 +1061,130 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     RawObj ro;
     if (!sxg_raw_alloc(nraw, ro)) return -1;
     const size_t nr = (size_t)(nraw > 0 ? nraw : 1);
-    long *compoff = sxg_take<long>("deb.compoff", nr + 1);
-    long *dtot = sxg_take<long>("deb.total", 4);
+    int *compoff = sxg_take<int>("deb.compoff", nr + 1);
+    int *dtot = sxg_take<int>("deb.total", 4);
     int *rootnc = sxg_take<int>("deb.rootnc", nr);
     int *rootkept = sxg_take<int>("deb.rootkept", nr);
     int *objbase = sxg_take<int>("deb.objbase", nr);
     int *alloc = sxg_take<int>("deb.alloc", S.N);
     auto *top1 = sxg_take<unsigned long long>("deb.top1", nr);
     auto *top2 = sxg_take<unsigned>("deb.top2", nr);
+    int *nmax = sxg_take<int>("deb.nmax", nr);
     int *lstar = sxg_take<int>("deb.lstar", nr);
-    int *clist = sxg_take<int>("deb.clist", 3 * nr);
+    int *clist = sxg_take<int>("deb.clist", 4 * nr);
     int *ccount = sxg_take<int>("deb.ccount", 8);
-    auto *dwork = sxg_take<unsigned long long>("deb.dwork", 2);
+    auto *dwork = sxg_take<unsigned long long>("deb.dwork", 4);
+    int *kcomp = sxg_take<int>("deb.kcomp", (size_t)(P0 > 0 ? P0 : 1));
     if (!compoff || !dtot || !rootnc || !rootkept || !objbase || !alloc || !top1 || !top2 ||
-        !lstar || !clist || !ccount || !dwork)
+        !nmax || !lstar || !clist || !ccount || !dwork || !kcomp)
         return -1;
-    long ncomp_cap = 0;
-    if (nraw > 0) {
-        hipLaunchKernelGGL(k_deb_capacity, dim3(sxg_div_up(nraw, 256)), dim3(256), 0, st, ro.count,
-                           nraw, n_thresh, deb_minarea, compoff);
-        hipLaunchKernelGGL(k_scan_long, dim3(1), dim3(1024), 0, st, compoff, nraw, dtot);
-        SXG_HIP_RET(hipMemcpyAsync(&ncomp_cap, dtot, sizeof(long), hipMemcpyDeviceToHost, st), -1);
-        SXG_HIP_RET(hipStreamSynchronize(st), -1);
-    }
-    Comp *comps = sxg_take<Comp>("deb.comps", (size_t)ncomp_cap + 1);
-    int *lvl = sxg_take<int>("deb.lvl", (size_t)(P0 > 0 ? P0 : 1) * n_thresh);
-    if (!comps || !lvl) return -1;
     DebArgs a{};
     a.fpia = S.fpia; a.rawkey = S.fpia_keys; a.off = ro.off; a.cnt = ro.count; a.first = ro.first;
     a.cd = S.cd.as<float>(); a.lab = S.lab.as<int>();
-    a.lvl = lvl; a.P0 = P0; a.thr = sxg_take<float>("deb.thr", nr * n_thresh);
+    a.kcomp = kcomp; a.P0 = P0; a.thr = sxg_take<float>("deb.thr", nr * n_thresh);
     a.nthresh = n_thresh; a.debarea = deb_minarea; a.mincont = deblend_mincont;
-    a.W = S.W; a.H = S.H; a.nraw = nraw; a.comps = comps; a.compoff = compoff;
+    a.W = S.W; a.H = S.H; a.nraw = nraw; a.compoff = compoff;
     a.rootnc = rootnc; a.rootkept = rootkept;
-    a.top1 = top1; a.top2 = top2; a.lstar = lstar; a.clist = clist; a.ccount = ccount;
-    a.dwork = dwork;
+    a.top1 = top1; a.top2 = top2; a.nmax = nmax; a.lstar = lstar; a.clist = clist;
+    a.ccount = ccount; a.dwork = dwork;
     S.deb_pixels = S.deb_labels = 0;
     int nobj = 0;
     if (nraw > 0) {
-        const int cap_small = (int)sxg_option(SXG_OPT_DEB_LDS_SMALL);
-        const int cap_large = (int)sxg_option(SXG_OPT_DEB_LDS_LARGE);
+        const int cap0 = (int)sxg_option(SXG_OPT_DEB_LDS_TINY);
+        const int cap1 = (int)sxg_option(SXG_OPT_DEB_LDS_SMALL);
+        const int cap2 = (int)sxg_option(SXG_OPT_DEB_LDS_LARGE);
+        /* tree nodes each class keeps in LDS (SXG_OPT_DEB_TABLE, else 192 / 512 / 1024),
+           within what its pixels leave of the workgroup's LDS */
+        const long topt = sxg_option(SXG_OPT_DEB_TABLE);
+        auto nodes_for = [&](int cap, long dflt) -> int {
+            const size_t avail = DEB_LDS_LIMIT - deb_lds_pixbytes((cap + 63) & ~63);
+            const size_t ct = std::min<size_t>(topt > 0 ? (size_t)topt : (size_t)dflt,
+                                               avail / sizeof(DebNode));
+            return (int)std::max<size_t>(ct, 1);
+        };
+        const int ct0 = nodes_for(cap0, 192), ct1 = nodes_for(cap1, 512);
+        const int ct2 = nodes_for(cap2, 1024);
+        const int gr = sxg_div_up(nraw, 256);
         sxg_kt_begin(SXG_KT_DEB_LMAX);
         hipLaunchKernelGGL(k_deb_prep, dim3(sxg_div_up(nraw > 8 ? nraw : 8, 256)), dim3(256), 0,
                            st, a);
         hipLaunchKernelGGL(k_deb_lmax<0>, dim3(sxg_div_up(P0, 256)), dim3(256), 0, st, a);
         hipLaunchKernelGGL(k_deb_lmax<1>, dim3(sxg_div_up(P0, 256)), dim3(256), 0, st, a);
-        hipLaunchKernelGGL(k_deb_classify, dim3(sxg_div_up(nraw, 256)), dim3(256), 0, st, a,
-                           cap_small, cap_large);
+        hipLaunchKernelGGL(k_deb_lsfirst, dim3(sxg_div_up((long)nraw * n_thresh, 256)), dim3(256),
+                           0, st, a);
+        hipLaunchKernelGGL(k_deb_nodecap, dim3(gr), dim3(256), 0, st, a);
+        sxg_scan_excl(compoff, compoff, nraw, dtot, st);
+        hipLaunchKernelGGL(k_deb_classify, dim3(gr), dim3(256), 0, st, a, cap0, cap1, cap2, ct0,
+                           ct1, ct2);
         sxg_kt_end(SXG_KT_DEB_LMAX);
-        int cc[4] = {0, 0, 0, 0};
+        int cc[7] = {0, 0, 0, 0, 0, 0, 0};
+        int ntot = 0;
+        unsigned long long dw[3] = {0, 0, 0};
         SXG_HIP_RET(hipMemcpyAsync(cc, ccount, sizeof(cc), hipMemcpyDeviceToHost, st), -1);
-        SXG_HIP_RET(hipStreamSynchronize(st), -1);
-        if (cc[2] > 0) {
-            a.g_cdv = sxg_take<float>("deb.g_cdv", (size_t)P0);
-            a.g_m = sxg_take<unsigned char>("deb.g_m", (size_t)P0);
-            a.g_nb = sxg_take<int>("deb.g_nb", 4 * (size_t)P0);
-            a.g_par = sxg_take<int>("deb.g_par", (size_t)P0);
-            a.g_lab = sxg_take<int>("deb.g_lab", (size_t)P0);
-            a.g_sz = sxg_take<int>("deb.g_sz", (size_t)P0);
-            if (!a.g_cdv || !a.g_m || !a.g_nb || !a.g_par || !a.g_lab || !a.g_sz) return -1;
-        }
-        sxg_kt_begin(SXG_KT_DEB_LEVELS);
-        if (cc[0] > 0) {
-            const int cap = (cap_small + 63) & ~63;
-            const size_t lds = (size_t)cap * DEB_LDS_BYTES_PER_PIX;
-            SXG_HIP_RET(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_deb_tree_lds<256>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                        -1);
-            hipLaunchKernelGGL(k_deb_tree_lds<256>, dim3(cc[0]), dim3(256), lds, st, a, 0, cap);
-        }
-        if (cc[1] > 0) {
-            const int cap = (cc[3] + 63) & ~63;
-            const size_t lds = (size_t)cap * DEB_LDS_BYTES_PER_PIX;
-            SXG_HIP_RET(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_deb_tree_lds<1024>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                        -1);
-            hipLaunchKernelGGL(k_deb_tree_lds<1024>, dim3(cc[1]), dim3(1024), lds, st, a, 1, cap);
-        }
-        if (cc[2] > 0)
-            hipLaunchKernelGGL(k_deb_tree_glb, dim3(cc[2]), dim3(1024), 0, st, a);
-        sxg_kt_end(SXG_KT_DEB_LEVELS);
-        SXG_HIP_RET(hipGetLastError(), -1);
-        sxg_scan_excl(rootkept, objbase, nraw, reinterpret_cast<int *>(dtot), st);
-        SXG_HIP_RET(hipMemcpyAsync(&nobj, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
-        SXG_HIP_RET(hipMemcpyAsync(cc, ccount + 4, sizeof(int), hipMemcpyDeviceToHost, st), -1);
-        unsigned long long dw[2] = {0, 0};
+        SXG_HIP_RET(hipMemcpyAsync(&ntot, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
         SXG_HIP_RET(hipMemcpyAsync(dw, dwork, sizeof(dw), hipMemcpyDeviceToHost, st), -1);
         SXG_HIP_RET(hipStreamSynchronize(st), -1);
+        if (dw[2] > (unsigned long long)0x7fffffff) {
+            sxg_set_error("parcel_out: deblending trees exceed 2^31 nodes in one frame");
+            return -1;
+        }
         S.deb_pixels = (int64_t)dw[0];
         S.deb_labels = (int64_t)dw[1];
+        a.comps = sxg_take<Comp>("deb.comps", (size_t)ntot + 1);
+        if (!a.comps) return -1;
+        sxg_kt_begin(SXG_KT_DEB_LEVELS);
+        /* one launch per class; LDS: the class's node table, then its largest object's
+           pixels */
+        /* the classes run side by side: each kernel lasts as long as its slowest tree.
+           Everything they read was finished by the synchronisation above. */
+        int nside = 0;
+        auto launch_lds = [&](const void *fn, int cls, int nt, int ct) -> bool {
+            int cap = (cc[4 + cls] + 63) & ~63;
+            const size_t lds = (((size_t)ct * sizeof(DebNode) + 15) & ~(size_t)15) +
+                               deb_lds_pixbytes(cap);
+            SXG_HIP_RET(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)lds),
+                        false);
+            void *args[] = {&a, &cls, &cap};
+            const bool side = cls != 0 && !S.single_stream;
+            hipStream_t ks = side ? S.side[nside] : st;
+            SXG_HIP_RET(hipLaunchKernel(fn, dim3(cc[cls]), dim3(nt), args, lds, ks), false);
+            if (side) SXG_HIP_RET(hipEventRecord(S.side_ev[nside++], ks), false);
+            return true;
+        };
+        if (cc[3] > 0) {
+            a.nodes = sxg_take<DebNode>("deb.nodes", (size_t)ntot + 1);
+            a.g_cdv = sxg_take<float>("deb.g_cdv", (size_t)P0);
+            a.g_m = sxg_take<unsigned char>("deb.g_m", (size_t)P0);
+            a.g_nb = sxg_take<int>("deb.g_nb", 2 * (size_t)P0);
+            a.g_par = sxg_take<int>("deb.g_par", (size_t)P0);
+            a.g_deep = sxg_take<int>("deb.g_deep", (size_t)P0);
+            a.g_cnt = sxg_take<int>("deb.g_cnt", (size_t)P0);
+            if (!a.nodes || !a.g_cdv || !a.g_m || !a.g_nb || !a.g_par || !a.g_deep || !a.g_cnt)
+                return -1;
+            hipStream_t ks = S.single_stream ? st : S.side[nside];
+            hipLaunchKernelGGL(k_deb_tree_glb, dim3(cc[3]), dim3(1024), 0, ks, a);
+            if (!S.single_stream) {
+                SXG_HIP_RET(hipEventRecord(S.side_ev[nside], ks), -1);
+                nside++;
+            }
+        }
+        if (cc[2] > 0 &&
+            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<1024>), 2, 1024, ct2))
+            return -1;
+        if (cc[1] > 0 &&
+            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<256>), 1, 256, ct1))
+            return -1;
+        if (cc[0] > 0 &&
+            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<64>), 0, 64, ct0))
+            return -1;
+        for (int j = 0; j < nside; j++) SXG_HIP_RET(hipStreamWaitEvent(st, S.side_ev[j], 0), -1);
+        sxg_kt_end(SXG_KT_DEB_LEVELS);
+        SXG_HIP_RET(hipGetLastError(), -1);
+        sxg_scan_excl(rootkept, objbase, nraw, dtot, st);
+        SXG_HIP_RET(hipMemcpyAsync(&nobj, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
+        SXG_HIP_RET(hipMemcpyAsync(cc, ccount + 7, sizeof(int), hipMemcpyDeviceToHost, st), -1);
+        SXG_HIP_RET(hipStreamSynchronize(st), -1);
         if (cc[0] != 0) {
             sxg_set_error("parcel_out: inconsistent deblending levels (internal error)");
             return -1;

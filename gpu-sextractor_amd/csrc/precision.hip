@@ -21,8 +21,8 @@
  * used by gatherup and CLEAN, and therefore the segmentation map, are the reference's
  * in every mode: this runs after CLEAN and rewrites catalogue columns only.
  *
- * One wavefront per object; the level pixels of a deblended object are found through
- * the per-level labels of the deblending (lvl[level][raw pixel] == its component).
+ * One wavefront per object; the level pixels of a deblended object are the raw pixels
+ * whose kept component (kcomp, written by the deblending) is the object's.
  */
 #include "sxg_internal.h"
 #include "sxg_objects.h"
@@ -31,8 +31,7 @@
 struct PrecArgs {
     const int *opix, *objoff;        /* final pixels of each object */
     const int *fpia, *rawoff, *rawcnt;   /* raw pixel lists */
-    const int *lvl;                  /* lvl[level * P0 + g] */
-    long P0;
+    const int *kcomp;                /* kept component of each raw pixel, -1 if none */
     const int *objcomp;              /* component of each object in its raw object */
     const float *sub, *cd;
     int W, nobj;
@@ -100,7 +99,7 @@ template <int MODE> __global__ void __launch_bounds__(256) k_precision(PrecArgs 
     const int r = o.root[obj];
     const int c = A.objcomp[obj];
     const int off = A.rawoff[r], n = A.rawcnt[r];
-    const int *lv = L > 0 ? A.lvl + (size_t)L * A.P0 + off : nullptr;
+    const int *lv = L > 0 ? A.kcomp + off : nullptr;
     const int g0 = A.objoff[obj], nf = A.objoff[obj + 1] - g0;
     const int *F = A.opix + g0;
     /* reference point for the offsets: the object's first final pixel */
@@ -202,14 +201,13 @@ bool sxg_precision_columns(int mode)
     A.fpia = S.fpia;
     A.rawoff = ro.off;
     A.rawcnt = ro.count;
-    A.lvl = sxg_take<int>("deb.lvl", (size_t)(S.npix_raw > 0 ? S.npix_raw : 1) * S.nthresh);
-    A.P0 = S.npix_raw;
+    A.kcomp = sxg_take<int>("deb.kcomp", (size_t)(S.npix_raw > 0 ? S.npix_raw : 1));
     A.objcomp = sxg_take<int>("obj.comp", S.nobj);
     A.sub = S.img.as<float>();
     A.cd = S.cd.as<float>();
     A.W = S.W;
     A.nobj = S.nobj;
-    if (!sxg_obj_view(A.o) || !A.lvl || !A.objcomp) return false;
+    if (!sxg_obj_view(A.o) || !A.kcomp || !A.objcomp) return false;
     const dim3 grid(sxg_div_up(S.nobj, 4)), block(256);
     if (mode == 1)
         hipLaunchKernelGGL(k_precision<1>, grid, block, 0, S.stream, A);

@@ -77,12 +77,17 @@ __device__ inline float sxg_pix_thresh(double k, float var) { return (float)(k *
 
 enum StageId { ST_BACK = 0, ST_FILTER, ST_DETECT, ST_DEBLEND, ST_ANALYSE, ST_CLEAN, ST_N };
 
+#define SXG_NSIDE 3
 struct SxgState {
     int W = 0, H = 0;
     size_t N = 0;
     bool have_frame = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2 * ST_N] = {};
+    /* side streams (default priority) for independent launches inside a stage, and the
+       events that join them back into stream */
+    hipStream_t side[SXG_NSIDE] = {};
+    hipEvent_t side_ev[SXG_NSIDE] = {};
     bool ev_used[ST_N] = {};
     float back_value = 0.0f;
 
@@ -135,6 +140,7 @@ struct SxgState {
     DevBuf resident;                       /* frame kept in HBM (sxg_set_resident_frame) */
     int rw = 0, rh = 0;
     bool is_default = false;               /* the process-wide default context */
+    bool single_stream = false;            /* no side streams (batch workers) */
     /* MAP_WEIGHT map (sxg_set_weight) */
     DevBuf weight;
     int ww = 0, wh = 0;
@@ -184,6 +190,8 @@ int sxg_radix_sort(int *k0, int *v0, int *k1, int *v1, int n, int nbits, int *hi
 DevBuf &sxg_buf(const char *name);
 void sxg_release_named();
 bool sxg_ensure_stream();   /* the current context's stream and events */
+/* batch workers: every launch of the context on its one stream */
+void sxg_ctx_single_stream(sxg_ctx *c);
 /* Typed view of a named buffer holding at least n elements (nullptr on failure). */
 template <class T> inline T *sxg_take(const char *name, size_t n)
 {

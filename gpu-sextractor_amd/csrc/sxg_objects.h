@@ -10,11 +10,11 @@
 #define COMP_SIG 2
 #define COMP_OKA 4
 
-/* node of the deblending tree, stored in the raw object's region of the table */
+/* deblending-tree node as the object kernels see it, stored in the raw object's region
+   of the table in the reference's order (level, then raster order of the first pixel):
+   level, first (raster index of its first pixel), flags & COMP_KEPT */
 struct Comp {
-    int level, parent, first, count;
-    float fdflux, dthresh;
-    int nchild, nson, flags;
+    int level, first, flags;
 };
 
 /* per-object columns (structure of arrays, object order) */

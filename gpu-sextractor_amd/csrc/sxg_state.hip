@@ -25,8 +25,8 @@ bool sxg_has_error() { return g_has_err; }
 extern "C" const char *sxg_last_error(void) { return g_has_err ? g_err.c_str() : nullptr; }
 
 /* ------------------------------------------------------------------ options ---- */
-static long g_opt[SXG_NOPT] = {1024, 6144, 0};
-static const long g_opt_max[SXG_NOPT] = {6400, 6400, 2};
+static long g_opt[SXG_NOPT] = {1024, 6144, 0, 256, 0};
+static const long g_opt_max[SXG_NOPT] = {6400, 6400, 2, 6400, 2048};
 
 long sxg_option(int id) { return g_opt[id]; }
 
@@ -97,11 +97,17 @@ extern "C" void sxg_ctx_destroy(sxg_ctx *c)
     if (c->st.stream) {
         for (int i = 0; i < 2 * ST_N; i++)
             if (c->st.ev[i]) (void)hipEventDestroy(c->st.ev[i]);
+        for (int i = 0; i < SXG_NSIDE; i++) {
+            if (c->st.side_ev[i]) (void)hipEventDestroy(c->st.side_ev[i]);
+            if (c->st.side[i]) (void)hipStreamDestroy(c->st.side[i]);
+        }
         (void)hipStreamDestroy(c->st.stream);
     }
     tl_ctx = prev == &c->st ? nullptr : prev;
     delete c;
 }
+
+void sxg_ctx_single_stream(sxg_ctx *c) { c->st.single_stream = true; }
 
 extern "C" int sxg_ctx_bind(sxg_ctx *c)
 {
@@ -121,6 +127,12 @@ bool sxg_ensure_stream()
     for (int i = 0; i < 2 * ST_N; i++)
         if (hipEventCreate(&S.ev[i]) != hipSuccess) {
             sxg_set_error("hipEventCreate failed");
+            return false;
+        }
+    for (int i = 0; i < SXG_NSIDE; i++)
+        if (hipStreamCreateWithFlags(&S.side[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&S.side_ev[i], hipEventDisableTiming) != hipSuccess) {
+            sxg_set_error("side stream creation failed");
             return false;
         }
     return true;

@@ -134,7 +134,8 @@ typedef struct {
     int64_t npix_above, npix_raw;
     int32_t nraw, nobj, nsurv, nbx, nby;
     /* deblending work: pixels of the raw objects with two or more local maxima (the only
-       ones processed level by level), and the level labels written for them */
+       ones that get a component tree), and the sum over them of pixels x L* (the
+       pixel-levels a level-by-level construction would visit) */
     int64_t deb_pixels, deb_labels;
 } sxg_frame_info;
 int sxg_get_frame_info(sxg_frame_info *info);
@@ -236,19 +237,24 @@ enum sxg_kernel_id {
 void sxg_kernel_timing(int enable);
 int sxg_kernel_times(int id, double *total_ms, int *launches);
 
-/* Options (no counterpart in the reference).  The first two never change a result:
-   they only choose which kernel variant handles which work.
-     SXG_OPT_DEB_LDS_SMALL  raw objects up to this many pixels are deblended by 256-lane
-                            workgroups in LDS (default 1024)
+/* Options (no counterpart in the reference).  These never change a result: they only
+   choose which kernel variant handles which work.
+     SXG_OPT_DEB_LDS_TINY   raw objects (with two or more local maxima) up to this many
+                            pixels are deblended by 64-lane workgroups in LDS (default 256)
+     SXG_OPT_DEB_LDS_SMALL  up to this many by 256-lane workgroups in LDS (default 1024)
      SXG_OPT_DEB_LDS_LARGE  up to this many by 1024-lane workgroups in LDS (default 6144,
                             at most 6400); larger ones use global scratch memory
-   and one option that changes the catalogue (never the segmentation map, the object
-   list or the CLEAN decisions):
+     SXG_OPT_DEB_TABLE      deblending-tree nodes an LDS workgroup can hold (0: 192, 512
+                            and 1024 for the three classes; at most 2048, and what the
+                            class's pixels leave of the LDS); a raw object whose tree may
+                            need more is built in global memory
+   This one changes the catalogue (never the segmentation map, the object list or the
+   CLEAN decisions):
      SXG_OPT_PRECISION      arithmetic of FLUX_ISO, X/Y, X2/Y2/XY, A/B/THETA, CXX/CYY/CXY:
                             0 the reference's (default, bit-exact), 1 two-pass FP64,
                             2 compensated FP32 sums with FP64 central moments */
 enum sxg_option_id { SXG_OPT_DEB_LDS_SMALL = 0, SXG_OPT_DEB_LDS_LARGE, SXG_OPT_PRECISION,
-                     SXG_NOPT };
+                     SXG_OPT_DEB_LDS_TINY, SXG_OPT_DEB_TABLE, SXG_NOPT };
 /* Returns 0, or -1 for an unknown option or a value out of range. */
 int sxg_set_option(int id, long value);
 long sxg_get_option(int id);

@@ -108,20 +108,27 @@ def test_catalog_rows_match_columns(frame1024, masks):
     assert np.array_equal(fe, np.sqrt(g["fluxerr"].astype(np.float64)).astype(np.float32))
 
 
-@pytest.mark.parametrize("caps", [(0, 0), (64, 256), (6400, 6400), (1024, 6144)])
+KNOBS = ("deb_lds_tiny", "deb_lds_small", "deb_lds_large", "deb_table")
+
+
+@pytest.mark.parametrize("caps", [(0, 0, 0, 0), (0, 64, 256, 0), (0, 6400, 6400, 0),
+                                  (256, 1024, 6144, 0), (6400, 6400, 6400, 0),
+                                  (256, 1024, 6144, 1), (256, 1024, 6144, 5),
+                                  (0, 0, 0, 3)])
 def test_deblend_kernel_variants(masks, caps):
-    """Every deblending kernel variant (LDS 256-lane, LDS 1024-lane, global scratch)
-    gives the oracle's result: the size caps move raw objects between them."""
+    """Every deblending kernel variant (LDS 64-, 256- and 1024-lane workgroups, global
+    scratch; the tree's node table in LDS or moved to global memory when it outgrows its
+    LDS share) gives the oracle's result: the options move raw objects between them."""
     img = sexgpu.synth("crowded4096", width=1024, height=1024, nsrc=167000 // 16, seed=1201)
     m = masks["gauss_3p0_7x7"]
-    old = (sexgpu.get_option("deb_lds_small"), sexgpu.get_option("deb_lds_large"))
+    old = [sexgpu.get_option(k) for k in KNOBS]
     try:
-        sexgpu.set_option("deb_lds_small", caps[0])
-        sexgpu.set_option("deb_lds_large", caps[1])
+        for k, v in zip(KNOBS, caps):
+            sexgpu.set_option(k, v)
         g = sexgpu.run_frame(img, _params(m))
     finally:
-        sexgpu.set_option("deb_lds_small", old[0])
-        sexgpu.set_option("deb_lds_large", old[1])
+        for k, v in zip(KNOBS, old):
+            sexgpu.set_option(k, v)
     compare(g, _oracle(img, m))
     assert (g["level"] > 0).sum() > 20
 

@@ -468,13 +468,6 @@ __device__ inline float small_median(float *a, int n)
     return (n & 1) ? a[n / 2] : (float)((a[n / 2 - 1] + a[n / 2]) / 2.0);
 }
 
-/* median of a sorted array, hmedian's even-n rule */
-__device__ inline float sorted_median(const float *a, int n)
-{
-    if (n < 2) return a[0];
-    return (n & 1) ? a[n / 2] : (float)((a[n / 2 - 1] + a[n / 2]) / 2.0);
-}
-
 /* bad meshes: average of the nearest valid meshes (back.c:461-490); one lane per mesh,
    into back2/sigma2 (good meshes copied) */
 __global__ void __launch_bounds__(256) k_fb_interp(const float *back, const float *sigma,
@@ -541,52 +534,95 @@ __global__ void __launch_bounds__(256) k_fb_median(const float *back2, const flo
     }
 }
 
-/* ascending bitonic sort of n2 (power of two) floats, one workgroup of FB_THREADS */
-__device__ void wg_bitonic(float *a, int n2)
+/* unsigned order-preserving key of a float */
+__device__ inline unsigned fb_key(float v) { return (unsigned)f2ord(v) ^ 0x80000000u; }
+
+/*
+ * The q-th smallest (0-based) of the values of a[0..n) that pass the filter (all, or the
+ * strictly positive ones), by radix selection on the order-preserving keys: four passes,
+ * each an LDS histogram of the next 8 key bits among the values that match the prefix
+ * found so far, then the bin that holds rank q.  One workgroup of FB_THREADS.
+ */
+__device__ float wg_select(const float *a, int n, int q, bool positive, int *hist, int *sel)
 {
-    for (int k = 2; k <= n2; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n2; i += FB_THREADS) {
-                int l = i ^ j;
-                if (l > i) {
-                    float x = a[i], y = a[l];
-                    bool up = (i & k) == 0;
-                    if ((x > y) == up) { a[i] = y; a[l] = x; }
-                }
-            }
-            __syncthreads();
+    const int tid = threadIdx.x, lane = tid & 63;
+    unsigned prefix = 0, pmask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = tid; i < 256; i += FB_THREADS) hist[i] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += FB_THREADS) {
+            const float v = a[i];
+            if (positive && !(v > 0.0f)) continue;
+            const unsigned u = fb_key(v);
+            if ((u & pmask) == prefix) atomicAdd(&hist[(u >> shift) & 255], 1);
         }
+        __syncthreads();
+        if (tid < 64) {
+            /* four bins per lane; the lane whose running count passes q holds the bin */
+            const int c0 = hist[4 * lane], c1 = hist[4 * lane + 1], c2 = hist[4 * lane + 2],
+                      c3 = hist[4 * lane + 3];
+            const int c = c0 + c1 + c2 + c3;
+            int incl = c;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int o = __shfl_up(incl, d);
+                if (lane >= d) incl += o;
+            }
+            const unsigned long long over = __ballot(incl > q);
+            const int L = over ? __builtin_ctzll(over) : 63;
+            if (lane == L) {
+                int before = incl - c, j = 0;
+                const int cs[4] = {c0, c1, c2, c3};
+                while (j < 3 && before + cs[j] <= q) before += cs[j++];
+                sel[0] = 4 * lane + j;
+                sel[1] = q - before;
+            }
+        }
+        __syncthreads();
+        prefix |= (unsigned)sel[0] << shift;
+        pmask |= 255u << shift;
+        q = sel[1];
+        __syncthreads();
+    }
+    return ord2f((int)(prefix ^ 0x80000000u));
 }
 
-/* global medians of the filtered maps (back.c:532-556): sorted copies padded with +inf,
-   in LDS when they fit (dynamic shared memory of 2*n2 floats), else in gbuf */
-__global__ void __launch_bounds__(FB_THREADS) k_fb_global(const float *back, const float *sigma,
-                                                          int np, int n2, int in_lds,
-                                                          float *gbuf, float *out)
+/* median of the cnt values of a[0..n) that pass the filter, by the even-count rule of
+   hmedian: the mean of the two middle values */
+__device__ float wg_median(const float *a, int n, int cnt, bool positive, int *hist, int *sel)
 {
-    extern __shared__ float fbs[];
-    float *sb = in_lds ? fbs : gbuf, *u = sb + n2;
-    for (int i = threadIdx.x; i < n2; i += FB_THREADS) {
-        sb[i] = i < np ? back[i] : __builtin_inff();
-        u[i] = i < np ? sigma[i] : __builtin_inff();
+    if (cnt < 2) return wg_select(a, n, 0, positive, hist, sel);
+    const float hi = wg_select(a, n, cnt / 2, positive, hist, sel);
+    if (cnt & 1) return hi;
+    const float lo = wg_select(a, n, cnt / 2 - 1, positive, hist, sel);
+    return (float)((lo + hi) / 2.0);
+}
+
+/* global medians of the filtered maps (back.c:532-556): workgroup 0 the background,
+   workgroup 1 the sigma (with the median of the strictly positive sigmas when the median
+   is not positive, 1 when there is none) */
+__global__ void __launch_bounds__(FB_THREADS) k_fb_global(const float *back, const float *sigma,
+                                                          int np, float *out)
+{
+    __shared__ int hist[256];
+    __shared__ int sel[2];
+    __shared__ int s_npos;
+    if (blockIdx.x == 0) {
+        const float m = wg_median(back, np, np, false, hist, sel);
+        if (threadIdx.x == 0) out[0] = m;
+        return;
     }
-    __syncthreads();
-    wg_bitonic(sb, n2);
-    wg_bitonic(u, n2);
-    if (threadIdx.x == 0) {
-        out[0] = sorted_median(sb, np);
-        float bsig = sorted_median(u, np);
-        if (bsig <= 0.0) {
-            /* median of the strictly positive sigmas (back.c:542-555) */
-            int i = np;
-            while (i-- && u[i] > 0.0f) {}
-            if (i >= 0 && i < np - 1)
-                bsig = sorted_median(u + i + 1, np - 1 - i);
-            else
-                bsig = 1.0f;
-        }
-        out[1] = bsig;
+    float bsig = wg_median(sigma, np, np, false, hist, sel);
+    if (bsig <= 0.0f) {
+        if (threadIdx.x == 0) s_npos = 0;
+        __syncthreads();
+        int c = 0;
+        for (int i = threadIdx.x; i < np; i += FB_THREADS) c += sigma[i] > 0.0f;
+        atomicAdd(&s_npos, c);
+        __syncthreads();
+        const int npos = s_npos;
+        bsig = npos > 0 && npos < np ? wg_median(sigma, np, npos, true, hist, sel) : 1.0f;
     }
+    if (threadIdx.x == 0) out[1] = bsig;
 }
 
 /* y second derivatives of back and sigma, one lane per mesh column (back.c:739-780);
@@ -814,19 +850,14 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
     sxg_kt_end(SXG_KT_MESHMODE);
     sxg_kt_begin(SXG_KT_FILTERBACK);
     {
-        float *back2 = work, *sigma2 = work + n2, *gbuf = work + 2 * n2;
+        float *back2 = work, *sigma2 = work + n2;
         const int gm = sxg_div_up(nmesh, 256);
         hipLaunchKernelGGL(k_fb_interp, dim3(gm), dim3(256), 0, S.stream, back, sigma, back2,
                            sigma2, nbx, nby);
         hipLaunchKernelGGL(k_fb_median, dim3(gm), dim3(256), 0, S.stream, back2, sigma2, back,
                            sigma, nbx, nby, nfx, nfy, (float)backfthresh);
-        const size_t lds = 2 * (size_t)n2 * sizeof(float);
-        const int in_lds = lds <= 128 * 1024;
-        if (in_lds)
-            SXG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fb_global),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_fb_global, dim3(1), dim3(FB_THREADS), in_lds ? lds : 0, S.stream,
-                           back, sigma, nmesh, n2, in_lds, gbuf, outv);
+        hipLaunchKernelGGL(k_fb_global, dim3(2), dim3(FB_THREADS), 0, S.stream, back, sigma,
+                           nmesh, outv);
         /* scratch of 2*nby floats per column: LDS when it fits, else back2/sigma2 and the
            sort buffer (4*n2 >= 2 * 2*nbx*nby floats) */
         const size_t ylds = 2 * (size_t)nby * 64 * sizeof(float);

@@ -182,51 +182,82 @@ extern "C" void sxg_release(void)
 
 /* ------------------------------------------------------------------ scan ---- */
 #define SCAN_BLOCK 256
-#define SCAN_ITEMS 4
+#define SCAN_ITEMS 16
 #define SCAN_TILE (SCAN_BLOCK * SCAN_ITEMS)
+/* counts up to this many are scanned by one workgroup in one launch */
+#define SCAN_SMALL 32768
+#define SCAN_SMALL_THREADS 1024
 
-/* workgroup-wide exclusive scan of one value per thread; returns the total */
+/* workgroup-wide exclusive scan of one value per thread (NT threads): an inclusive scan
+   inside each wavefront, then the wavefront totals through LDS; returns the total */
+template <int NT> __device__ inline int block_excl_scan_t(int v, int *sm, int *total)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(x, d);
+        if (lane >= d) x += o;
+    }
+    if (lane == 63) sm[wid] = x;
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int w = 0; w < NT / 64; w++) {
+        const int s = sm[w];
+        before += w < wid ? s : 0;
+        tot += s;
+    }
+    *total = tot;
+    __syncthreads();
+    return before + x - v;
+}
+
 __device__ inline int block_excl_scan(int v, int *sm, int *total)
 {
-    const int t = threadIdx.x;
-    sm[t] = v;
-    __syncthreads();
-    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
-        int add = t >= off ? sm[t - off] : 0;
-        __syncthreads();
-        sm[t] += add;
-        __syncthreads();
+    return block_excl_scan_t<SCAN_BLOCK>(v, sm, total);
+}
+
+/* 16 consecutive ints of a tile row (four 16-byte loads when the row is inside [0, n)) */
+__device__ inline void load16(const int *in, long base, int n, int *v)
+{
+    if (base + SCAN_ITEMS <= n && (((uintptr_t)(in + base)) & 15) == 0) {
+        const int4 *q = reinterpret_cast<const int4 *>(in + base);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int4 t = q[j];
+            v[4 * j] = t.x;
+            v[4 * j + 1] = t.y;
+            v[4 * j + 2] = t.z;
+            v[4 * j + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) v[i] = base + i < n ? in[base + i] : 0;
     }
-    int incl = sm[t];
-    *total = sm[SCAN_BLOCK - 1];
-    __syncthreads();
-    return incl - v;
 }
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const int *in, int n, int *bsum)
 {
-    __shared__ int sm[SCAN_BLOCK];
-    const long base = (long)blockIdx.x * SCAN_TILE;
-    int s = 0;
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        long k = base + (long)i * SCAN_BLOCK + threadIdx.x;
-        if (k < n) s += in[k];
-    }
+    __shared__ int sm[SCAN_BLOCK / 64];
+    const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS], s = 0;
+    load16(in, base, n, v);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
     int tot;
     block_excl_scan(s, sm, &tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
 /* single workgroup: exclusive scan of nb block sums in place, total to *total */
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_blocks(int *bsum, int nb, int *total)
+__global__ void __launch_bounds__(1024) k_scan_blocks(int *bsum, int nb, int *total)
 {
-    __shared__ int sm[SCAN_BLOCK];
+    __shared__ int sm[16];
     int carry = 0;
-    for (int base = 0; base < nb; base += SCAN_BLOCK) {
-        int k = base + threadIdx.x;
-        int v = k < nb ? bsum[k] : 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int k = base + threadIdx.x;
+        const int v = k < nb ? bsum[k] : 0;
         int tot;
-        int ex = block_excl_scan(v, sm, &tot);
+        const int ex = block_excl_scan_t<1024>(v, sm, &tot);
         if (k < nb) bsum[k] = carry + ex;
         carry += tot;
     }
@@ -236,23 +267,50 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_blocks(int *bsum, int nb, i
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const int *in, int *out, int n,
                                                            const int *bsum)
 {
-    __shared__ int sm[SCAN_BLOCK];
+    __shared__ int sm[SCAN_BLOCK / 64];
     const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS], s = 0;
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        long k = base + i;
-        v[i] = k < n ? in[k] : 0;
-        s += v[i];
-    }
+    load16(in, base, n, v);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
     int tot;
     int ex = block_excl_scan(s, sm, &tot) + bsum[blockIdx.x];
+#pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) {
-        long k = base + i;
+        const long k = base + i;
         if (k < n) out[k] = ex;
         ex += v[i];
     }
 }
 
+/* n <= SCAN_SMALL: one workgroup; the values pass through LDS (coalesced loads and
+   stores), each thread scans a contiguous run of at most 32 of them */
+__global__ void __launch_bounds__(SCAN_SMALL_THREADS) k_scan_small(const int *in, int *out, int n,
+                                                                   int *total)
+{
+    extern __shared__ int sv[];
+    __shared__ int sm[SCAN_SMALL_THREADS / 64];
+#pragma unroll 8
+    for (int k = threadIdx.x; k < n; k += SCAN_SMALL_THREADS) sv[k] = in[k];
+    __syncthreads();
+    const int per = (n + SCAN_SMALL_THREADS - 1) / SCAN_SMALL_THREADS;
+    const int b = threadIdx.x * per, e = min(b + per, n);
+    int s = 0;
+    for (int k = b; k < e; k++) s += sv[k];
+    int tot;
+    int ex = block_excl_scan_t<SCAN_SMALL_THREADS>(s, sm, &tot);
+    for (int k = b; k < e; k++) {
+        const int v = sv[k];
+        sv[k] = ex;
+        ex += v;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int k = threadIdx.x; k < n; k += SCAN_SMALL_THREADS) out[k] = sv[k];
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
+/* exclusive prefix sum of n ints (in == out allowed); the total to *total (device) */
 void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s)
 {
     SxgState &S = sxg_state();
@@ -260,11 +318,23 @@ void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s)
         if (total) (void)hipMemsetAsync(total, 0, sizeof(int), s);
         return;
     }
+    if (n <= SCAN_SMALL) {
+        /* the attribute is per function: set once (thread-safe static initialisation) */
+        static const bool attr = [] {
+            return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_scan_small),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       SCAN_SMALL * (int)sizeof(int)) == hipSuccess;
+        }();
+        (void)attr;
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SCAN_SMALL_THREADS),
+                           (size_t)n * sizeof(int), s, in, out, n, total);
+        return;
+    }
     const int nb = sxg_div_up(n, SCAN_TILE);
     if (!S.scan_tmp.ensure(sizeof(int) * (nb + 64))) return;
     int *bsum = S.scan_tmp.as<int>();
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, n, bsum);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_BLOCK), 0, s, bsum, nb, total);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsum, nb, total);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, out, n, bsum);
 }
 
@@ -276,13 +346,12 @@ struct PredNonneg { __device__ bool operator()(int v) const { return v >= 0; } }
 template <class P>
 __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_count(const int *flag, int n, int *bsum)
 {
-    __shared__ int sm[SCAN_BLOCK];
+    __shared__ int sm[SCAN_BLOCK / 64];
     const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
-    int s = 0;
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        long k = base + i;
-        if (k < n && P()(flag[k])) s++;
-    }
+    int v[SCAN_ITEMS], s = 0;
+    load16(flag, base, n, v);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) s += (base + i < n && P()(v[i])) ? 1 : 0;
     int tot;
     block_excl_scan(s, sm, &tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
@@ -292,18 +361,22 @@ template <class P>
 __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_write(const int *flag, int n,
                                                               const int *bsum, int *out)
 {
-    __shared__ int sm[SCAN_BLOCK];
+    __shared__ int sm[SCAN_BLOCK / 64];
     const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
-    int f[SCAN_ITEMS], s = 0;
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        long k = base + i;
-        f[i] = (k < n && P()(flag[k])) ? 1 : 0;
-        s += f[i];
-    }
+    int v[SCAN_ITEMS], s = 0;
+    load16(flag, base, n, v);
+    unsigned f = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++)
+        if (base + i < n && P()(v[i])) {
+            f |= 1u << i;
+            s++;
+        }
     int tot;
     int pos = block_excl_scan(s, sm, &tot) + bsum[blockIdx.x];
+#pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++)
-        if (f[i]) out[pos++] = (int)(base + i);
+        if (f & (1u << i)) out[pos++] = (int)(base + i);
 }
 
 template <class P>
@@ -318,7 +391,7 @@ static void compact_impl(const int *flag, int *out, int n, int *count, hipStream
     if (!S.scan_tmp.ensure(sizeof(int) * (nb + 64))) return;
     int *bsum = S.scan_tmp.as<int>();
     hipLaunchKernelGGL(k_compact_count<P>, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_BLOCK), 0, s, bsum, nb, count);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsum, nb, count);
     hipLaunchKernelGGL(k_compact_write<P>, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum, out);
 }
 

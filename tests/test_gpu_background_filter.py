@@ -101,3 +101,29 @@ def test_odd_geometries_match_oracle(shape, bs):
     assert (g["backmean"], g["backsig"]) == (o["backmean"], o["backsig"])
     sub = sexgpu.subtracted(img.shape)
     assert np.array_equal(_bits(sub), _bits(o["sub"]))
+
+
+def _flat_with_patch(shape, patch, seed=11):
+    img = np.full(shape, 100.0, np.float32)
+    if patch:
+        y0, y1, x0, x1 = patch
+        rng = np.random.default_rng(seed)
+        img[y0:y1, x0:x1] += rng.normal(0, 2, (y1 - y0, x1 - x0)).astype(np.float32)
+    return img
+
+
+@pytest.mark.parametrize("shape,patch", [((512, 512), (0, 192, 0, 512)),
+                                         ((448, 320), (64, 256, 0, 128)),
+                                         ((256, 256), None)])
+def test_zero_sigma_meshes_use_the_positive_median(shape, patch):
+    """Most meshes flat: the global sigma is the median of the positive mesh sigmas, or 1
+    with none (back.c:542-555); odd and even mesh counts."""
+    img = _flat_with_patch(shape, patch)
+    o = sxo.background(img, bs=32, fsize=3)
+    g = _gpu_background(img, 32, 3)
+    assert np.array_equal(_bits(g["sigma"]), _bits(o["sigma"]))
+    assert (g["backmean"], g["backsig"]) == (o["backmean"], o["backsig"])
+    if patch is None:
+        assert g["backsig"] == 1.0
+    else:
+        assert g["backsig"] > 0.0

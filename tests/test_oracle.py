@@ -281,3 +281,25 @@ def test_fp64_truth_closed_form():
     assert np.allclose(t, want, rtol=1e-12, atol=1e-12)
     # the float catalogue is within the float/double rounding of the truth
     assert abs(r["mx"][0] - xb) < 1e-6 and abs(r["mx2"][0] - x2) / x2 < 1e-5
+
+
+def test_global_sigma_rule_for_flat_meshes():
+    """back.c:532-556: the global background and sigma are medians of the filtered maps
+    (mean of the two middle values for an even count); a non-positive sigma median is
+    replaced by the median of the positive mesh sigmas, or 1 when there is none.  Checked
+    with numpy on the oracle's own maps."""
+    def med(v):
+        v = np.sort(v.astype(np.float32))
+        n = len(v)
+        return v[n // 2] if n % 2 else np.float32((np.float64(v[n // 2 - 1]) + v[n // 2]) / 2.0)
+    for shape, patch in (((512, 512), (0, 192, 0, 512)), ((448, 320), (64, 256, 0, 128))):
+        img = np.full(shape, 100.0, np.float32)
+        y0, y1, x0, x1 = patch
+        img[y0:y1, x0:x1] += np.random.default_rng(11).normal(0, 2, (y1 - y0, x1 - x0)).astype(np.float32)
+        o = sxo.background(img, bs=32, fsize=3)
+        sig = o["sigma"].ravel()
+        assert med(sig) <= 0.0
+        assert o["backsig"] == med(sig[sig > 0])
+        assert o["backmean"] == med(o["back"].ravel())
+    o = sxo.background(np.full((256, 256), 100.0, np.float32), bs=32, fsize=3)
+    assert o["backsig"] == 1.0

@@ -80,11 +80,15 @@ __device__ inline void meshstat_pass2_end(const MeshAcc &a, MeshStat &o)
 }
 
 /*
- * One workgroup per band of up to MS_MPB meshes along x.  Wavefront 0 keeps one lane
- * per mesh running the sequential double sums in raster order; the other three
- * wavefronts stream the band's rows into a double-buffered LDS ring (R rows per batch,
- * each mesh's row segment padded, ms_seg, so the folding lanes read float4s from
- * different banks), so the chains never wait on global memory.
+ * One workgroup per band of up to MS_MPB meshes along x.  The sums of back.c are two
+ * sequential double chains per mesh (pix and pix*pix in raster order); they cannot be
+ * reordered (the squares carry up to 48 significant bits, so their double sum rounds),
+ * but nothing else has to sit on the chain: wavefronts 1..3 stream the band's rows
+ * through a double-buffered LDS ring and store, per pixel, t = (included ? pix : +0)
+ * and t2 = (included ? pix*pix : -0) as doubles; in wavefront 0, lane m adds the t of
+ * mesh m and lane MS_MPB + m its t2 (counting the non-negative ones), one double add per
+ * pixel and lane.  Adding +0 to the first sum and -0 to the second is an exact identity
+ * (both start at +0 and stay >= +0 in round-to-nearest).
  */
 #define MS_MPB 16
 #define MS_THREADS 256
@@ -124,97 +128,64 @@ __global__ void __launch_bounds__(64) k_meshstat_direct(const float *__restrict_
     }
     ms[mi] = o;
 }
-/* one mesh's rows of a batch, in raster order; PASS 0 takes every pixel > -BIG, PASS 1
-   those in [lcut, hcut].  Branch-free: an excluded pixel adds +0.0, an exact identity
-   here (the sums start at +0.0 and cannot become -0.0 in round-to-nearest) */
-template <int PASS>
-__device__ inline void meshstat_fold(const float *rows, int rstride, int nr, int bw, float lcut,
-                                     float hcut, MeshAcc &acc)
-{
-    double mean = acc.mean, sigma = acc.sigma;
-    int npix = acc.npix;
-    const double lo = lcut, hi = hcut;
-    auto fold = [&](float v) {
-        const double pix = v;
-        const bool in = PASS == 0 ? (pix > -SXG_BIG) : ((pix <= hi) & (pix >= lo));
-        const double t = in ? pix : 0.0;
-        mean += t;
-        sigma += t * t;
-        npix += in;
-    };
-    for (int r = 0; r < nr; r++) {
-        const float *row = rows + r * rstride;
-        int x = 0;
-        /* four LDS reads in flight, then the 16 values in order */
-        for (; x + 16 <= bw; x += 16) {
-            const float4 a = *reinterpret_cast<const float4 *>(row + x);
-            const float4 b = *reinterpret_cast<const float4 *>(row + x + 4);
-            const float4 c = *reinterpret_cast<const float4 *>(row + x + 8);
-            const float4 d = *reinterpret_cast<const float4 *>(row + x + 12);
-            fold(a.x); fold(a.y); fold(a.z); fold(a.w);
-            fold(b.x); fold(b.y); fold(b.z); fold(b.w);
-            fold(c.x); fold(c.y); fold(c.z); fold(c.w);
-            fold(d.x); fold(d.y); fold(d.z); fold(d.w);
-        }
-        for (; x < bw; x++) fold(row[x]);
-    }
-    acc.mean = mean;
-    acc.sigma = sigma;
-    acc.npix = npix;
-}
-
-/* padded LDS segment of one mesh row: a multiple of 4 floats (float4 reads) whose
-   quarter is odd, so the 16 folding lanes start on distinct banks */
-__host__ __device__ inline int ms_seg(int bs)
-{
-    int seg = (bs + 3) & ~3;
-    if (((seg >> 2) & 1) == 0) seg += 4;
-    return seg;
-}
+/* padded LDS segment of one mesh row in doubles: odd, so the 16 chain lanes of an
+   array start on distinct bank pairs */
+__host__ __device__ inline int ms_seg(int bs) { return bs | 1; }
 
 __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict__ img,
                                                          const float *__restrict__ wmap, int W,
                                                          int H, int bs, int nbx, int mpb, int R,
                                                          MeshStat *ms)
 {
-    extern __shared__ __align__(16) float ring[];
-    const int tid = threadIdx.x, wid = tid >> 6;
+    extern __shared__ __align__(16) double ring[];
+    __shared__ float s_lcut[MS_MPB], s_hcut[MS_MPB];
+    __shared__ int s_cnt[MS_MPB];
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
     const int mx0 = blockIdx.x * mpb, my = blockIdx.y;
     const int nm = min(mpb, nbx - mx0);
     const int x0 = mx0 * bs, y0 = my * bs;
     const int bandw = min(W, x0 + nm * bs) - x0;
     const int bh = min(bs, H - y0);
     const int seg = ms_seg(bs);
-    const int bufsz = R * mpb * seg;
+    const int bufsz = R * mpb * seg;            /* doubles per buffer and array */
     const int nbatch = (bh + R - 1) / R;
-    /* the folding lane's mesh */
-    const int m = tid;
-    const bool folder = wid == 0 && m < nm;
-    const int bw = folder ? min(bs, W - (x0 + m * bs)) : 0;
-    MeshAcc acc;
+    /* mesh of a band column: a shift when the mesh size is a power of two */
+    const int bsh = (bs & (bs - 1)) == 0 ? __builtin_ctz(bs) : -1;
+    /* the chain lane's mesh and array (0: t, 1: t2) */
+    const int m = lane < mpb ? lane : lane - mpb;
+    const int arr = lane < mpb ? 0 : 1;
+    const bool chain = wid == 0 && lane < 2 * mpb && m < nm;
+    const int bw = min(bs, W - (x0 + m * bs));
     MeshStat o;
     float lcut = 0.0f, hcut = 0.0f;
     bool bad = false;
-    /* wavefronts 1..3 copy rows [b*R, b*R + nr) of the band into buf; four loads are in
-       flight per thread before any LDS store */
     const bool vec4 = ((bs | W) & 3) == 0;
-    auto load = [&](int b, float *buf) {
+    /* wavefronts 1..3: rows [b*R, b*R + nr) of the band into buffer half h, and the
+       number of included pixels of each mesh */
+    auto load = [&](int pass, int b, int h) {
+        double *bt = ring + (size_t)(2 * h) * bufsz, *bt2 = bt + bufsz;
         const int r0 = b * R, nr = min(R, bh - r0);
         const int lt = tid - 64, LT = MS_THREADS - 64;
+        auto incl = [&](double pix, int mm) {
+            return pass == 0 ? (pix > -SXG_BIG)
+                             : ((pix <= (double)s_hcut[mm]) & (pix >= (double)s_lcut[mm]));
+        };
         if (vec4) {
+            /* all the batch's loads of a thread in flight before any store */
             const int q4 = bandw >> 2, tot = nr * q4;
+            const int qsh = (q4 & (q4 - 1)) == 0 ? __builtin_ctz(q4) : -1;
             for (int e0 = lt; e0 < tot; e0 += 4 * LT) {
                 float4 v[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int e = e0 + u * LT;
                     if (e < tot) {
-                        const int r = e / q4, c = 4 * (e - r * q4);
-                        const size_t o = (size_t)(y0 + r0 + r) * W + x0 + c;
-                        v[u] = *reinterpret_cast<const float4 *>(img + o);
+                        const int r = qsh >= 0 ? e >> qsh : e / q4, c = 4 * (e - r * q4);
+                        const size_t ofs = (size_t)(y0 + r0 + r) * W + x0 + c;
+                        v[u] = *reinterpret_cast<const float4 *>(img + ofs);
                         if (wmap) {
                             /* pixels of non-positive weight are left out: -BIG */
-                            const float4 wv = *reinterpret_cast<const float4 *>(wmap + o);
+                            const float4 wv = *reinterpret_cast<const float4 *>(wmap + ofs);
                             const float nb = (float)-SXG_BIG;
                             if (!(wv.x > 0.0f)) v[u].x = nb;
                             if (!(wv.y > 0.0f)) v[u].y = nb;
@@ -227,9 +198,23 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
                 for (int u = 0; u < 4; u++) {
                     const int e = e0 + u * LT;
                     if (e < tot) {
-                        const int r = e / q4, c = 4 * (e - r * q4);
-                        const int mm = c / bs, off = c - mm * bs;
-                        *reinterpret_cast<float4 *>(buf + (r * mpb + mm) * seg + off) = v[u];
+                        const int r = qsh >= 0 ? e >> qsh : e / q4, c = 4 * (e - r * q4);
+                        const int mm = bsh >= 0 ? c >> bsh : c / bs, off = c - mm * bs;
+                        const double p0 = v[u].x, p1 = v[u].y, p2 = v[u].z, p3 = v[u].w;
+                        const bool i0 = incl(p0, mm), i1 = incl(p1, mm), i2 = incl(p2, mm),
+                                   i3 = incl(p3, mm);
+                        const int idx = (r * mpb + mm) * seg + off;
+                        double *d = bt + idx, *d2 = bt2 + idx;
+                        d[0] = i0 ? p0 : 0.0;
+                        d[1] = i1 ? p1 : 0.0;
+                        d[2] = i2 ? p2 : 0.0;
+                        d[3] = i3 ? p3 : 0.0;
+                        d2[0] = i0 ? p0 * p0 : -0.0;
+                        d2[1] = i1 ? p1 * p1 : -0.0;
+                        d2[2] = i2 ? p2 * p2 : -0.0;
+                        d2[3] = i3 ? p3 * p3 : -0.0;
+                        const int k = (int)i0 + (int)i1 + (int)i2 + (int)i3;
+                        if (k) atomicAdd(&s_cnt[mm], k);
                     }
                 }
             }
@@ -242,8 +227,8 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
                     const int e = e0 + u * LT;
                     if (e < tot) {
                         const int r = e / bandw, c = e - r * bandw;
-                        const size_t o = (size_t)(y0 + r0 + r) * W + x0 + c;
-                        v[u] = (wmap && !(wmap[o] > 0.0f)) ? (float)-SXG_BIG : img[o];
+                        const size_t ofs = (size_t)(y0 + r0 + r) * W + x0 + c;
+                        v[u] = (wmap && !(wmap[ofs] > 0.0f)) ? (float)-SXG_BIG : img[ofs];
                     }
                 }
 #pragma unroll
@@ -251,41 +236,75 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
                     const int e = e0 + u * LT;
                     if (e < tot) {
                         const int r = e / bandw, c = e - r * bandw;
-                        const int mm = c / bs, off = c - mm * bs;
-                        buf[(r * mpb + mm) * seg + off] = v[u];
+                        const int mm = bsh >= 0 ? c >> bsh : c / bs, off = c - mm * bs;
+                        const double pix = v[u];
+                        const bool in = incl(pix, mm);
+                        const int idx = (r * mpb + mm) * seg + off;
+                        bt[idx] = in ? pix : 0.0;
+                        bt2[idx] = in ? pix * pix : -0.0;
+                        if (in) atomicAdd(&s_cnt[mm], 1);
                     }
                 }
             }
         }
     };
     for (int pass = 0; pass < 2; pass++) {
-        if (wid != 0) load(0, ring);
+        double acc = 0.0;
+        if (tid < MS_MPB) s_cnt[tid] = 0;
+        __syncthreads();
+        if (wid != 0) load(pass, 0, 0);
         __syncthreads();
         for (int b = 0; b < nbatch; b++) {
-            float *cur = ring + (b & 1) * bufsz;
             if (wid != 0) {
-                if (b + 1 < nbatch) load(b + 1, ring + ((b + 1) & 1) * bufsz);
-            } else if (folder && !(pass == 1 && bad)) {
-                const int nr = min(R, bh - b * R);
-                const float *rows = cur + m * seg;
-                const int rstride = mpb * seg;
-                if (pass == 0)
-                    meshstat_fold<0>(rows, rstride, nr, bw, 0.0f, 0.0f, acc);
-                else
-                    meshstat_fold<1>(rows, rstride, nr, bw, lcut, hcut, acc);
+                if (b + 1 < nbatch) load(pass, b + 1, (b + 1) & 1);
+            } else if (chain) {
+                /* the lane's chain: one double add per pixel, the reads of the next
+                   eight in flight meanwhile */
+                const double *src = ring + (size_t)(2 * (b & 1) + arr) * bufsz + m * seg;
+                const int nr = min(R, bh - b * R), rstride = mpb * seg;
+                for (int r = 0; r < nr; r++) {
+                    const double *row = src + r * rstride;
+                    int x = 0;
+                    if (bw >= 16) {
+                        double v[8], w[8];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) v[u] = row[u];
+                        for (x = 8; x + 16 <= bw; x += 16) {
+#pragma unroll
+                            for (int u = 0; u < 8; u++) w[u] = row[x + u];
+#pragma unroll
+                            for (int u = 0; u < 8; u++) acc += v[u];
+#pragma unroll
+                            for (int u = 0; u < 8; u++) v[u] = row[x + 8 + u];
+#pragma unroll
+                            for (int u = 0; u < 8; u++) acc += w[u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; u++) acc += v[u];
+                    }
+                    for (; x < bw; x++) acc += row[x];
+                }
             }
             __syncthreads();
         }
-        if (folder) {
+        /* mesh m: the pix sum from lane m, the pix^2 sum from lane mpb+m */
+        const double sq = __shfl(acc, lane + mpb);
+        if (wid == 0 && lane < nm) {
+            MeshAcc a;
+            a.mean = acc;
+            a.sigma = sq;
+            a.npix = s_cnt[lane];
             if (pass == 0) {
-                meshstat_pass1_end(acc, bw, bh, o, lcut, hcut, bad);
-                acc = MeshAcc();
+                meshstat_pass1_end(a, bw, bh, o, lcut, hcut, bad);
+                s_lcut[lane] = lcut;
+                s_hcut[lane] = hcut;
             } else if (!bad) {
-                meshstat_pass2_end(acc, o);
+                meshstat_pass2_end(a, o);
             }
         }
+        __syncthreads();
     }
-    if (folder) ms[my * nbx + mx0 + m] = o;
+    if (wid == 0 && lane < nm) ms[my * nbx + mx0 + lane] = o;
 }
 
 /* --------------------------------------------------------------- backguess ---- */
@@ -299,21 +318,6 @@ __device__ inline int d2i_sat(double v)
     return (int)v;
 }
 
-/* workgroup sum of 64-bit values, MODE_THREADS threads (every thread gets the total) */
-__device__ inline long long wg_sum64(long long v, long long *sm)
-{
-    const int t = threadIdx.x;
-    sm[t] = v;
-    __syncthreads();
-    for (int off = MODE_THREADS / 2; off > 0; off >>= 1) {
-        if (t < off) sm[t] += sm[t + off];
-        __syncthreads();
-    }
-    long long r = sm[0];
-    __syncthreads();
-    return r;
-}
-
 __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restrict__ img,
                                                            const float *__restrict__ wmap, int W,
                                                            int H, int bs, int nbx,
@@ -323,11 +327,14 @@ __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restri
     /* h[i]: histogram; P0[k] = sum_{j<k} h[j] for k in [0, 4096] */
     __shared__ int h[SXG_NMAXLEVELS];
     __shared__ int P0[SXG_NMAXLEVELS + 1];
-    __shared__ long long sm[MODE_THREADS];
+    __shared__ double sred[2][2][MODE_THREADS / 64];
+    __shared__ int sfirst[2][MODE_THREADS / 64];
+    __shared__ int sscan[MODE_THREADS / 64];
     const int m = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const MeshStat st = ms[m];
     if (st.mean <= -SXG_BIG) {
-        if (threadIdx.x == 0) back[m] = sigma[m] = (float)-SXG_BIG;
+        if (tid == 0) back[m] = sigma[m] = (float)-SXG_BIG;
         return;
     }
     const int mx = m % nbx, my = m / nbx;
@@ -335,12 +342,12 @@ __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restri
     const int bw = (x0 + bs <= W) ? bs : W - x0;
     const int bh = (y0 + bs <= H) ? bs : H - y0;
     const int nlev = st.nlevels;
-    for (int i = threadIdx.x; i < SXG_NMAXLEVELS; i += MODE_THREADS) h[i] = 0;
+    for (int i = tid; i < SXG_NMAXLEVELS; i += MODE_THREADS) h[i] = 0;
     __syncthreads();
     const float qscale = st.qscale;
     const float cste = (float)(0.499999 - st.qzero / qscale);
     const int npx = bw * bh;
-    for (int k = threadIdx.x; k < npx; k += MODE_THREADS) {
+    for (int k = tid; k < npx; k += MODE_THREADS) {
         const int y = k / bw, x = k - y * bw;
         const size_t o = (size_t)(y0 + y) * W + x0 + x;
         const float v = (wmap && !(wmap[o] > 0.0f)) ? (float)-SXG_BIG : img[o];
@@ -348,35 +355,43 @@ __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restri
         if (t > -1.0f && t < (float)nlev) atomicAdd(&h[(int)t], 1);
     }
     __syncthreads();
-    /* prefix counts: thread t owns bins [t*16, t*16+16) */
-    const int per = SXG_NMAXLEVELS / MODE_THREADS;
-    {
-        const int b0 = threadIdx.x * per;
-        int s0 = 0;
-        for (int j = b0; j < b0 + per; j++) s0 += h[j];
-        sm[threadIdx.x] = s0;
-        __syncthreads();
-        for (int off = 1; off < MODE_THREADS; off <<= 1) {
-            long long add = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
-            __syncthreads();
-            sm[threadIdx.x] += add;
-            __syncthreads();
-        }
-        int e0 = (int)(sm[threadIdx.x] - s0);
-        for (int j = b0; j < b0 + per; j++) {
-            P0[j] = e0;
-            e0 += h[j];
-        }
-        if (threadIdx.x == MODE_THREADS - 1) P0[SXG_NMAXLEVELS] = e0;
-        __syncthreads();
+    /* thread t owns bins [16t, 16t + 16): their counts and prefix counts stay in
+       registers; P0 goes to LDS for the lookups at arbitrary bins */
+    constexpr int PER = SXG_NMAXLEVELS / MODE_THREADS;
+    const int b0 = tid * PER;
+    int hc[PER], pc[PER + 1];
+    int csum = 0;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        hc[u] = h[b0 + u];
+        csum += hc[u];
     }
+    {
+        int x = csum;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(x, d);
+            if (lane >= d) x += o;
+        }
+        if (lane == 63) sscan[wid] = x;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wid; w++) before += sscan[w];
+        pc[0] = before + x - csum;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        pc[u + 1] = pc[u] + hc[u];
+        P0[b0 + u] = pc[u];
+    }
+    if (tid == MODE_THREADS - 1) P0[SXG_NMAXLEVELS] = pc[PER];
+    __syncthreads();
 
     /* The iteration of backguess.  Every thread runs the same control flow on the same
-       shared values; the window sums are collective. */
+       shared values; the window sums and the median bin are collective. */
     auto hget = [&](long i) -> long { return (i >= 0 && i < nlev) ? h[i] : 0; };
     unsigned long lowsum, highsum, sum;
     double ftemp, mea, sig, sig1, med;
-    int n, lcut, hcut, nlevelsm1;
+    int n, lcut, hcut, nlevelsm1, par = 0;
     hcut = nlevelsm1 = nlev - 1;
     lcut = 0;
     sig = 10.0 * nlevelsm1;
@@ -394,29 +409,47 @@ __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restri
             hi = hcut;
         } else {
             /* exact integer window sums (each below 2^53, so the sequential double sums
-               of back.c equal them) */
-            long long s1 = 0, s2 = 0;
-            for (int j = lcut + threadIdx.x; j <= hcut; j += MODE_THREADS) {
-                long long c = h[j];
-                s1 += c * j;
-                s2 += c * j * j;
-            }
-            s1 = wg_sum64(s1, sm);
-            s2 = wg_sum64(s2, sm);
-            sum = (unsigned long)(P0[hcut + 1] - P0[lcut]);
-            mea = (double)s1;
-            sig = (double)s2;
-            /* a = #{i in [0,nb): L(i) < H(nb-1-i)},
-               L(i) = P0[lcut+i]-P0[lcut], H(nb-1-i) = P0[hcut+1]-P0[lcut+i+1];
-               the predicate holds for i < a and fails from a on */
+               of back.c equal them, and so do these double sums in any order) */
             const int base = P0[lcut], top = P0[hcut + 1];
-            long a_lo = 0, a_hi = nb;
-            while (a_lo < a_hi) {
-                long mid = (a_lo + a_hi) >> 1;
-                int L = P0[lcut + mid] - base;
-                int Hh = top - P0[lcut + mid + 1];
-                if (L < Hh) a_lo = mid + 1; else a_hi = mid;
+            const int target = base + top;
+            double s1 = 0.0, s2 = 0.0;
+            /* a = #{i in [0,nb): L(i) < H(nb-1-i)} with L(i) = P0[lcut+i]-P0[lcut] and
+               H(nb-1-i) = P0[hcut+1]-P0[lcut+i+1], i.e. the first bin j >= lcut where the
+               non-decreasing P0[j] + P0[j+1] reaches P0[lcut] + P0[hcut+1] */
+            int first = 0x7fffffff;
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const int j = b0 + u;
+                if (j >= lcut && j <= hcut) {
+                    const double c = (double)hc[u];
+                    s1 += c * j;
+                    s2 += c * ((double)j * j);
+                    if (first == 0x7fffffff && pc[u] + pc[u + 1] >= target) first = j;
+                }
             }
+            for (int d = 32; d > 0; d >>= 1) {
+                s1 += __shfl_xor(s1, d);
+                s2 += __shfl_xor(s2, d);
+                first = min(first, __shfl_xor(first, d));
+            }
+            if (lane == 0) {
+                sred[par][0][wid] = s1;
+                sred[par][1][wid] = s2;
+                sfirst[par][wid] = first;
+            }
+            __syncthreads();
+            s1 = s2 = 0.0;
+            first = 0x7fffffff;
+            for (int w = 0; w < MODE_THREADS / 64; w++) {
+                s1 += sred[par][0][w];
+                s2 += sred[par][1][w];
+                first = min(first, sfirst[par][w]);
+            }
+            par ^= 1;
+            sum = (unsigned long)(top - base);
+            mea = s1;
+            sig = s2;
+            const long a_lo = first == 0x7fffffff ? nb : (long)first - lcut;
             lo = lcut + a_lo;
             hi = lo - 1;
             lowsum = (unsigned long)(P0[lo] - base);
@@ -827,8 +860,8 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
     sxg_kt_begin(SXG_KT_MESHSTAT);
     {
         /* meshes per workgroup and rows per batch: two buffers of R rows of mpb padded
-           segments within 128 KB of LDS */
-        const size_t segbytes = (size_t)ms_seg(bs) * sizeof(float);
+           segments of t and t2 (doubles) within 128 KB of LDS */
+        const size_t segbytes = (size_t)ms_seg(bs) * 2 * sizeof(double);
         const int mpb = (int)std::min<size_t>(MS_MPB, (64 * 1024) / segbytes);
         if (mpb >= 1) {
             const size_t rowbytes = mpb * segbytes;

@@ -33,7 +33,23 @@ struct AnaArgs {
     ObjCols o;
     WeightArgs wa;          /* MAP_WEIGHT: per-pixel variance and thresholds */
     const float *rawt0;     /* detection threshold of each raw object */
+    const int *order;       /* objects in launch order: the large ones first */
 };
+
+/* objects of at least ANA_BIG pixels are analysed first: their sequential sums are the
+   longest, so they start at once and the small objects fill the machine around them */
+#define ANA_BIG 256
+
+__global__ void __launch_bounds__(256) k_obj_order(const int *objoff, int nobj, int *cnt,
+                                                   int *order)
+{
+    const int obj = blockIdx.x * 256 + threadIdx.x;
+    if (obj >= nobj) return;
+    if (objoff[obj + 1] - objoff[obj] >= ANA_BIG)
+        order[atomicAdd(&cnt[0], 1)] = obj;
+    else
+        order[nobj - 1 - atomicAdd(&cnt[1], 1)] = obj;
+}
 
 /* back.c:685-732 */
 __device__ inline float back_at(const float *b, int x, int y, int nx, int ny, int backw, int backh)
@@ -61,11 +77,12 @@ __device__ inline float back_at(const float *b, int x, int y, int nx, int ny, in
 
 __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
 {
-    __shared__ double s_rows[4][6 * SXW_ROW];
+    __shared__ double s_rows[4][11 * SXW_ROW];
     __shared__ float s_frows[4][SXW_ROW];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int obj = blockIdx.x * 4 + wid;
-    if (obj >= A.nobj) return;
+    const int slot = blockIdx.x * 4 + wid;
+    if (slot >= A.nobj) return;
+    const int obj = A.order[slot];
     double *rows = s_rows[wid];
     float *frows = s_frows[wid];
     const int W = A.W, H = A.H;
@@ -80,28 +97,22 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     double mxo, myo;
     float fdflux, oa, ob;
     if (o.level[obj] > 0) {
-        /* ---- preanalyse_robust over the final pixels: extent, peaks, float flux ---- */
-        float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG, racc = 0.0f;
+        /* ---- preanalyse_robust over the final pixels ----
+           extent and peaks first (reductions), then one fold pass for the float flux and
+           the six double moment sums (independent chains, one lane each) */
+        float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG;
         int xmin = 2 * SXG_MAXPICSIZE, ymin = 2 * SXG_MAXPICSIZE, xmax = 0, ymax = 0;
-        for (int base = 0; base < npx; base += 64) {
-            const int k = base + lane;
-            const bool in = k < npx;
-            float cval = 0.0f;
-            if (in) {
-                const int p = P[k];
-                const int y = p / W, x = p - y * W;
-                peak = fmaxf(peak, A.sub[p]);
-                cval = A.cd[p];
-                cpeak = fmaxf(cpeak, cval);
-                xmin = min(xmin, x);
-                xmax = max(xmax, x);
-                ymin = min(ymin, y);
-                ymax = max(ymax, y);
-            }
-            const float tf[1] = {cval};
-            sxw_fold<1>(frows, tf, in, racc);
+#pragma unroll 4
+        for (int k = lane; k < npx; k += 64) {
+            const int p = P[k];
+            const int y = p / W, x = p - y * W;
+            peak = fmaxf(peak, A.sub[p]);
+            cpeak = fmaxf(cpeak, A.cd[p]);
+            xmin = min(xmin, x);
+            xmax = max(xmax, x);
+            ymin = min(ymin, y);
+            ymax = max(ymax, y);
         }
-        const float rv = sxw_rl(racc, 0);
         peak = sxw_max(peak);
         cpeak = sxw_max(cpeak);
         xmin = sxw_min(xmin);
@@ -112,15 +123,27 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
         const double dthresh = (double)odth;
         const double dthresh2 = (dthresh + peak) / 2.0;
         double acc = 0.0;
+        float racc = 0.0f;
         int area2 = 0, dnpix = 0;
+        /* the next chunk's pixels are read while this chunk is folded */
+        int np_ = lane < npx ? P[lane] : 0;
+        float nsub = lane < npx ? A.sub[np_] : 0.0f, ncd = lane < npx ? A.cd[np_] : 0.0f;
         for (int base = 0; base < npx; base += 64) {
             const int k = base + lane;
-            float val = 0.0f, tx = 0.0f, ty = 0.0f, txx = 0.0f, tyy = 0.0f, txy = 0.0f;
+            const int pc = np_;
+            const float csub = nsub, ccd = ncd;
+            if (k + 64 < npx) {
+                np_ = P[k + 64];
+                nsub = A.sub[np_];
+                ncd = A.cd[np_];
+            }
+            float val = 0.0f, cval = 0.0f, tx = 0.0f, ty = 0.0f, txx = 0.0f, tyy = 0.0f,
+                  txy = 0.0f;
             if (k < npx) {
-                const int p = P[k];
+                const int p = pc;
                 const int y = p / W - ymin, x = p - (p / W) * W - xmin;
-                const float cval = A.cd[p];
-                val = A.sub[p];
+                cval = ccd;
+                val = csub;
                 if (val > dthresh) dnpix++;
                 if (val > dthresh2) area2++;
                 tx = cval * x;
@@ -130,8 +153,10 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
                 txy = cval * x * y;
             }
             const double t6[6] = {val, tx, ty, txx, tyy, txy};
-            sxw_fold<6>(rows, t6, k < npx, acc);
+            const float tf[1] = {cval};
+            sxw_fold2<6, 1>(rows, frows, t6, tf, k < npx, acc, racc);
         }
+        const float rv = sxw_rl(racc, 6);
         const double tv = sxw_rl(acc, 0), mx = sxw_rl(acc, 1), my = sxw_rl(acc, 2);
         const double mx2 = sxw_rl(acc, 3), my2 = sxw_rl(acc, 4), mxy = sxw_rl(acc, 5);
         area2 = sxw_sum(area2);
@@ -160,11 +185,13 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
         oa = o.a[obj];
         ob = o.b[obj];
     }
-    /* ---- examineiso ---- */
+    /* ---- examineiso ----
+       reductions first (peak, its position, the analysis threshold, ISOAREA, mthresh),
+       then one fold pass for the six isophotal sums and the five FWHM sums (a pixel
+       outside the FWHM selection adds +0, an identity: those sums never hold -0) */
     const float bkg = back_at(A.back, (int)(mxo + 0.5), (int)(myo + 0.5), A.nbx, A.nby, A.backw,
                               A.backh);
     const float dbkg = 0.0f;
-    double eacc = 0.0;
     const double var = (double)(A.backsig * A.backsig), dbacknoise2 = var;
     const double xm = A.errflag ? mxo : 0.0, ym = A.errflag ? myo : 0.0;
     float pk = (float)-SXG_BIG, cdpeak = (float)-SXG_BIG, othr = (float)SXG_BIG;
@@ -172,41 +199,22 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     const bool cleanflag = A.cleanflag && npx >= HEAPN;
     SxwTop5 top;
     top.init();
-    for (int base = 0; base < npx; base += 64) {
-        const int k = base + lane;
-        float pix = 0.0f;
-        double err = 0.0, ex2 = 0.0, ey2 = 0.0, exy = 0.0, var2 = 0.0;
-        if (k < npx) {
-            const int p = P[k];
-            double x = p % W, y = p / W;
-            pix = A.sub[p];
-            pk = fmaxf(pk, pix);
-            const float cdpix = A.cd[p];
-            if (A.peakflag && cdpix > cdpeak) {
-                cdpeak = cdpix;
-                peakp = p;
-            }
-            /* per-pixel variance and analysis threshold with a weight map */
-            const float varp = A.wa.w ? sxg_var(A.wa.w, p) : 0.0f;
-            var2 = A.wa.w ? (double)varp : var;
-            const float athr = perpix ? sxg_pix_thresh(A.wa.kana, varp) : A.thresh;
-            othr = fminf(othr, athr);
-            if (pix > athr) area++;
-            err = A.wa.w ? var2 : dbacknoise2;
-            if (A.gain > 0.0 && cdpix > 0.0) err += cdpix / A.gain;
-            x = x - xm;
-            y = y - ym;
-            ex2 = err * x * x;
-            ey2 = err * y * y;
-            exy = err * x * y;
-            if (cleanflag) top.add(cdpix - odth);
+#pragma unroll 4
+    for (int k = lane; k < npx; k += 64) {
+        const int p = P[k];
+        const float pix = A.sub[p];
+        pk = fmaxf(pk, pix);
+        const float cdpix = A.cd[p];
+        if (A.peakflag && cdpix > cdpeak) {
+            cdpeak = cdpix;
+            peakp = p;
         }
-        /* tv, sigtv, esum, emx2, emy2, emxy (the error sums only with errflag) */
-        const double t6[6] = {pix, var2, err, ex2, ey2, exy};
-        sxw_fold<6>(rows, t6, k < npx, eacc);
+        const float varp = A.wa.w ? sxg_var(A.wa.w, p) : 0.0f;
+        const float athr = perpix ? sxg_pix_thresh(A.wa.kana, varp) : A.thresh;
+        othr = fminf(othr, athr);
+        if (pix > athr) area++;
+        if (cleanflag) top.add(cdpix - odth);
     }
-    double tv = sxw_rl(eacc, 0), sigtv = sxw_rl(eacc, 1);
-    const double emx2 = sxw_rl(eacc, 3), emy2 = sxw_rl(eacc, 4), emxy = sxw_rl(eacc, 5);
     pk = sxw_max(pk);
     othr = sxw_minf(othr);
     area = sxw_sum(area);
@@ -218,17 +226,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
         peaky = pp / W + 1;
     }
     const float mthresh = cleanflag ? top.fifth() : 0.0f;
-    double pmx2 = 0.0, pmy2 = 0.0, pmxy = 0.0;
-    if (A.errflag) {
-        const double flux2 = (float)(fdflux * fdflux);
-        pmx2 = emx2 / flux2;
-        pmy2 = emy2 / flux2;
-        pmxy = emxy / flux2;
-    }
-    tv -= npx * dbkg;
-    if (A.gain > 0.0 && tv > 0.0) sigtv += tv / A.gain;
     const float thr_o = othr - dbkg, pk_o = pk - dbkg;
-    /* ---- isophotal areas and FWHM ---- */
     float threshs[SXG_NISO];
     if (pk_o > 0.0 && thr_o > 0.0)
         for (int i = 0; i < SXG_NISO; i++)
@@ -239,23 +237,46 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     float thresh0 = (float)(pk_o / 5.0);
     if (thresh0 < thr_o) thresh0 = thr_o;
     const double sat = (double)(A.satur_level - dbkg);
-    double facc = 0.0;
+    double eacc = 0.0;
+    float fdummy = 0.0f;
+    /* the next chunk's pixels are read while this chunk is folded */
+    int np_ = lane < npx ? P[lane] : 0;
+    float nsub = lane < npx ? A.sub[np_] : 0.0f, ncd = lane < npx ? A.cd[np_] : 0.0f;
     for (int base = 0; base < npx; base += 64) {
         const int k = base + lane;
-        bool use = false;
+        const int pc = np_;
+        const float csub = nsub, ccd = ncd;
+        if (k + 64 < npx) {
+            np_ = P[k + 64];
+            nsub = A.sub[np_];
+            ncd = A.cd[np_];
+        }
+        float pix = 0.0f;
+        double err = 0.0, ex2 = 0.0, ey2 = 0.0, exy = 0.0, var2 = 0.0;
         double ts = 0.0, tsx = 0.0, tsxx = 0.0, tsy = 0.0, tsxy = 0.0;
         if (k < npx) {
-            const int p = P[k];
-            const float v = A.sub[p];
+            const int p = pc;
+            double x = p % W, y = p / W;
+            pix = csub;
+            const float cdpix = ccd;
+            /* per-pixel variance with a weight map */
+            const float varp = A.wa.w ? sxg_var(A.wa.w, p) : 0.0f;
+            var2 = A.wa.w ? (double)varp : var;
+            err = A.wa.w ? var2 : dbacknoise2;
+            if (A.gain > 0.0 && cdpix > 0.0) err += cdpix / A.gain;
+            x = x - xm;
+            y = y - ym;
+            ex2 = err * x * x;
+            ey2 = err * y * y;
+            exy = err * x * y;
             bool go = true;
 #pragma unroll
             for (int i = 0; i < SXG_NISO; i++) {
-                go = go && v > threshs[i];
+                go = go && pix > threshs[i];
                 iso[i] += go;
             }
-            const double dpix = v - (double)dbkg;
+            const double dpix = pix - (double)dbkg;
             if (thresh0 > 0.0 && dpix > thresh0 && dpix < sat) {
-                use = true;
                 const double dx = (p % W) - mxo, dy = (p / W) - myo;
                 const double lpix = sx_log(dpix);
                 const double inverr2 = dpix * dpix;
@@ -267,11 +288,25 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
                 tsxy = lpix * d2 * inverr2;
             }
         }
-        const double t5[5] = {ts, tsx, tsxx, tsy, tsxy};
-        sxw_fold<5>(rows, t5, use, facc);
+        /* tv, sigtv, esum, emx2, emy2, emxy (the error sums only with errflag), then the
+           FWHM fit sums */
+        const double t11[11] = {pix, var2, err, ex2, ey2, exy, ts, tsx, tsxx, tsy, tsxy};
+        const float tf[1] = {0.0f};
+        sxw_fold2<11, 1>(rows, frows, t11, tf, k < npx, eacc, fdummy);
     }
-    const double s = sxw_rl(facc, 0), sx = sxw_rl(facc, 1), sxx = sxw_rl(facc, 2);
-    const double sy = sxw_rl(facc, 3), sxy = sxw_rl(facc, 4);
+    double tv = sxw_rl(eacc, 0), sigtv = sxw_rl(eacc, 1);
+    const double emx2 = sxw_rl(eacc, 3), emy2 = sxw_rl(eacc, 4), emxy = sxw_rl(eacc, 5);
+    double pmx2 = 0.0, pmy2 = 0.0, pmxy = 0.0;
+    if (A.errflag) {
+        const double flux2 = (float)(fdflux * fdflux);
+        pmx2 = emx2 / flux2;
+        pmy2 = emy2 / flux2;
+        pmxy = emxy / flux2;
+    }
+    tv -= npx * dbkg;
+    if (A.gain > 0.0 && tv > 0.0) sigtv += tv / A.gain;
+    const double s = sxw_rl(eacc, 6), sx = sxw_rl(eacc, 7), sxx = sxw_rl(eacc, 8);
+    const double sy = sxw_rl(eacc, 9), sxy = sxw_rl(eacc, 10);
 #pragma unroll
     for (int i = 0; i < SXG_NISO; i++) iso[i] = sxw_sum(iso[i]);
     float fwhm = 0.0f;
@@ -357,8 +392,14 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
     A.rawt0 = ro.t0;
     if (!sxg_obj_view(A.o)) return;
     S.clean_enabled = prefs_clean_flag != 0;
+    int *order = sxg_take<int>("ana.order", (size_t)numobj + 2);
+    if (!order) return;
+    A.order = order;
     sxg_stage_begin(ST_ANALYSE);
     sxg_kt_begin(SXG_KT_ANALYSE);
+    SXG_HIP(hipMemsetAsync(order + numobj, 0, 2 * sizeof(int), S.stream));
+    hipLaunchKernelGGL(k_obj_order, dim3(sxg_div_up(numobj, 256)), dim3(256), 0, S.stream,
+                       S.objoff, numobj, order + numobj, order);
     hipLaunchKernelGGL(k_analyse, dim3(sxg_div_up(numobj, 4)), dim3(256), 0, S.stream, A);
     sxg_kt_end(SXG_KT_ANALYSE);
     sxg_stage_end(ST_ANALYSE);

@@ -347,12 +347,48 @@ __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restri
     const float qscale = st.qscale;
     const float cste = (float)(0.499999 - st.qzero / qscale);
     const int npx = bw * bh;
-    for (int k = tid; k < npx; k += MODE_THREADS) {
-        const int y = k / bw, x = k - y * bw;
-        const size_t o = (size_t)(y0 + y) * W + x0 + x;
-        const float v = (wmap && !(wmap[o] > 0.0f)) ? (float)-SXG_BIG : img[o];
-        float t = v / qscale + cste;
+    auto bin = [&](float v) {
+        const float t = v / qscale + cste;
         if (t > -1.0f && t < (float)nlev) atomicAdd(&h[(int)t], 1);
+    };
+    if (((bw | W) & 3) == 0) {
+        /* float4 reads, all of a thread's loads in flight before the binning */
+        const int q4 = bw >> 2, nq = q4 * bh;
+        for (int q0 = tid; q0 < nq; q0 += 4 * MODE_THREADS) {
+            float4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int q = q0 + u * MODE_THREADS;
+                if (q < nq) {
+                    const int y = q / q4, x = 4 * (q - y * q4);
+                    const size_t o = (size_t)(y0 + y) * W + x0 + x;
+                    v[u] = *reinterpret_cast<const float4 *>(img + o);
+                    if (wmap) {
+                        const float4 wv = *reinterpret_cast<const float4 *>(wmap + o);
+                        const float nb = (float)-SXG_BIG;
+                        if (!(wv.x > 0.0f)) v[u].x = nb;
+                        if (!(wv.y > 0.0f)) v[u].y = nb;
+                        if (!(wv.z > 0.0f)) v[u].z = nb;
+                        if (!(wv.w > 0.0f)) v[u].w = nb;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (q0 + u * MODE_THREADS < nq) {
+                    bin(v[u].x);
+                    bin(v[u].y);
+                    bin(v[u].z);
+                    bin(v[u].w);
+                }
+        }
+    } else {
+#pragma unroll 4
+        for (int k = tid; k < npx; k += MODE_THREADS) {
+            const int y = k / bw, x = k - y * bw;
+            const size_t o = (size_t)(y0 + y) * W + x0 + x;
+            bin((wmap && !(wmap[o] > 0.0f)) ? (float)-SXG_BIG : img[o]);
+        }
     }
     __syncthreads();
     /* thread t owns bins [16t, 16t + 16): their counts and prefix counts stay in

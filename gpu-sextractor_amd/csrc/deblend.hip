@@ -869,30 +869,22 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
     const int L = C[c].level;
     const int n = d.cnt[r], off = d.off[r], W = d.W, H = d.H;
     const int *lv = L > 0 ? d.kcomp + off : nullptr;
-    /* pass 1: extent, peaks, count; rv is the raster-order float sum of cd */
-    float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG, racc = 0.0f;
+    /* extent, peaks, count (reductions); the node's pixels get the object index */
+    float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG;
     int xmin = 2 * SXG_MAXPICSIZE, ymin = 2 * SXG_MAXPICSIZE, xmax = 0, ymax = 0, cnt = 0;
-    for (int base = 0; base < n; base += 64) {
-        const int k = base + lane;
-        const bool in = k < n && (!lv || lv[k] == c);
-        float cval = 0.0f;
-        if (in) {
-            const int p = d.fpia[off + k];
-            const int y = p / W, x = p - y * W;
-            A.alloc[p] = obj;
-            peak = fmaxf(peak, A.sub[p]);
-            cval = d.cd[p];
-            cpeak = fmaxf(cpeak, cval);
-            xmin = min(xmin, x);
-            xmax = max(xmax, x);
-            ymin = min(ymin, y);
-            ymax = max(ymax, y);
-            cnt++;
-        }
-        const float tf[1] = {cval};
-        sxw_fold<1>(frows, tf, in, racc);
+    for (int k = lane; k < n; k += 64) {
+        if (lv && lv[k] != c) continue;
+        const int p = d.fpia[off + k];
+        const int y = p / W, x = p - y * W;
+        A.alloc[p] = obj;
+        peak = fmaxf(peak, A.sub[p]);
+        cpeak = fmaxf(cpeak, d.cd[p]);
+        xmin = min(xmin, x);
+        xmax = max(xmax, x);
+        ymin = min(ymin, y);
+        ymax = max(ymax, y);
+        cnt++;
     }
-    const float rv = sxw_rl(racc, 0);
     peak = sxw_max(peak);
     cpeak = sxw_max(cpeak);
     xmin = sxw_min(xmin);
@@ -904,17 +896,19 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
     if (xmin == 0 || xmax == W - 1 || ymin == 0 || ymax == H - 1) flag |= SXG_OBJ_TRUNC;
     const double dthresh = (double)d.thr[(size_t)r * d.nthresh + L];
     const double dthresh2 = (dthresh + peak) / 2.0;
-    /* pass 2: the moments, double sums of float terms in raster order */
+    /* one fold pass: rv, the raster-order float sum of cd, and the moments, double sums
+       of float terms in raster order (independent chains, one lane each) */
     double acc = 0.0;
+    float racc = 0.0f;
     int area2 = 0, dnpix = 0;
     for (int base = 0; base < n; base += 64) {
         const int k = base + lane;
         const bool in = k < n && (!lv || lv[k] == c);
-        float val = 0.0f, tx = 0.0f, ty = 0.0f, txx = 0.0f, tyy = 0.0f, txy = 0.0f;
+        float val = 0.0f, cval = 0.0f, tx = 0.0f, ty = 0.0f, txx = 0.0f, tyy = 0.0f, txy = 0.0f;
         if (in) {
             const int p = d.fpia[off + k];
             const int y = p / W - ymin, x = p - (p / W) * W - xmin;
-            const float cval = d.cd[p];
+            cval = d.cd[p];
             val = A.sub[p];
             if (val > dthresh) dnpix++;
             if (val > dthresh2) area2++;
@@ -925,8 +919,10 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
             txy = cval * x * y;
         }
         const double t6[6] = {val, tx, ty, txx, tyy, txy};
-        sxw_fold<6>(rows, t6, in, acc);
+        const float tf[1] = {cval};
+        sxw_fold2<6, 1>(rows, frows, t6, tf, in, acc, racc);
     }
+    const float rv = sxw_rl(racc, 6);
     const double tv = sxw_rl(acc, 0), mx = sxw_rl(acc, 1), my = sxw_rl(acc, 2);
     const double mx2 = sxw_rl(acc, 3), my2 = sxw_rl(acc, 4), mxy = sxw_rl(acc, 5);
     area2 = sxw_sum(area2);

@@ -130,3 +130,47 @@ __device__ inline void sxw_fold(T *rows, const T (&t)[NS], bool member, T &acc)
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+/*
+ * The same with ND double sums and NF float sums advancing together: lane s < ND adds
+ * double row s, lane ND + s adds float row s.  Every chain lane performs one double and
+ * one float add per member (the add it does not own reads a valid row and is ignored),
+ * so the chains of both types interleave in one instruction stream.  drows: ND rows,
+ * frows: NF rows (at least one row each).
+ */
+template <int ND, int NF>
+__device__ inline void sxw_fold2(double *drows, float *frows, const double (&td)[ND],
+                                 const float (&tf)[NF], bool member, double &dacc, float &facc)
+{
+    static_assert(ND + NF <= 64, "one lane per sum");
+    const int lane = threadIdx.x & 63;
+    const unsigned long long b = __ballot(member);
+    const int cnt = __popcll(b);
+    if (member) {
+        const int pos = __popcll(b & sxw_lanes_below(lane));
+#pragma unroll
+        for (int s = 0; s < ND; s++) drows[s * SXW_ROW + pos] = td[s];
+#pragma unroll
+        for (int s = 0; s < NF; s++) frows[s * SXW_ROW + pos] = tf[s];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < ND + NF) {
+        const double *dr = drows + (lane < ND ? lane : 0) * SXW_ROW;
+        const float *fr = frows + (lane >= ND ? lane - ND : 0) * SXW_ROW;
+        double da = dacc;
+        float fa = facc;
+#pragma unroll 8
+        for (int j = 0; j < cnt; j++) {
+            da += dr[j];
+            fa += fr[j];
+        }
+        dacc = da;
+        facc = fa;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+

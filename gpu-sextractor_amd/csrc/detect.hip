@@ -7,10 +7,11 @@
  * numbered in label order; the pixels of each raw object are listed in raster order.
  *
  * Kernels:
- *   k_det_init     one lane per pixel: union-find parent = own index or -1
+ *   k_det_init     one lane per pixel: union-find parent = the first pixel of its run
+ *                  inside its 64-pixel window (a ballot), or -1
  *   (compaction)   detected pixel indices in raster order (the foreground list)
- *   k_det_union    one lane per detected pixel: lock-free union with the left, up-left,
- *                  up and up-right neighbours.  A root is only ever linked under a
+ *   k_det_union    one lane per detected pixel: lock-free union across window edges and
+ *                  with the row above, once per overlapping stretch.  A root is only ever linked under a
  *                  smaller index (atomicMin, retried with the value found), so every
  *                  component ends with its smallest index as root, in one launch, with
  *                  no host convergence loop (the reference loops with a host sync per
@@ -31,13 +32,28 @@ __device__ inline float det_thresh(const WeightArgs &wa, float t, long p)
     return (wa.w && wa.rel) ? sxg_pix_thresh(wa.kdet, sxg_var(wa.w, p)) : t;
 }
 
+/* union-find parent of each pixel: -1 for the background; a detected pixel points at the
+   first pixel of its run (the detected pixels before it on its row, inside its aligned
+   64-pixel window: a ballot), so rows never become chains */
 __global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, float t, long n,
-                                                  int *par, int *lab, WeightArgs wa)
+                                                  int W, int *par, int *lab, WeightArgs wa)
 {
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    par[p] = cd[p] >= det_thresh(wa, t, p) ? (int)p : -1;
+    const int lane = threadIdx.x & 63;
+    const bool live = p < n;
+    const bool fg = live && cd[p] >= det_thresh(wa, t, p);
+    const int x = live ? (int)(p % W) : 0;
+    const unsigned long long F = __ballot(fg);
+    const bool start = fg && (x == 0 || lane == 0 || !((F >> (lane - 1)) & 1ull));
+    const unsigned long long S = __ballot(start);
+    if (!live) return;
     lab[p] = -1;
+    if (!fg) {
+        par[p] = -1;
+        return;
+    }
+    const unsigned long long le = S & (~0ull >> (63 - lane));
+    par[p] = (int)(p - lane + (63 - __builtin_clzll(le)));
 }
 
 /* root of x; path halving with atomicMin keeps the chains short (every pointer goes
@@ -72,8 +88,11 @@ __device__ inline void uf_unite(int *par, int a, int b)
     }
 }
 
-/* 8-connected union with the backward neighbours; NW and NE are already joined through
-   N when N is set (they are its W and E neighbours), NW also through W */
+/* 8-connected union with the backward neighbours.  W is joined only across a window
+   edge (inside a window the run is linked already); NW and NE are already joined through
+   N when N is set (they are its W and E neighbours), NW also through W; and N through W
+   when W and NW are both set (W's N is this pixel's NW), so the row above is joined once
+   per overlapping stretch */
 __global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, int nfg, int W,
                                                    int *par)
 {
@@ -82,12 +101,14 @@ __global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, i
     const int p = fg[k];
     const int y = p / W, x = p - y * W;
     const bool w = x > 0 && par[p - 1] >= 0;
-    if (w) uf_unite(par, p, p - 1);
+    if (w && (p & 63) == 0) uf_unite(par, p, p - 1);
     if (y > 0) {
-        if (par[p - W] >= 0) {
-            uf_unite(par, p, p - W);
+        const bool nn = par[p - W] >= 0;
+        const bool nw = x > 0 && par[p - W - 1] >= 0;
+        if (nn) {
+            if (!(w && nw)) uf_unite(par, p, p - W);
         } else {
-            if (!w && x > 0 && par[p - W - 1] >= 0) uf_unite(par, p, p - W - 1);
+            if (nw && !w) uf_unite(par, p, p - W - 1);
             if (x < W - 1 && par[p - W + 1] >= 0) uf_unite(par, p, p - W + 1);
         }
     }
@@ -280,7 +301,7 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     WeightArgs wa;
     if (!sxg_weight_args(wa)) return;
     hipLaunchKernelGGL(k_det_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
-                       S.cd.as<float>(), dthreshold, N, par, lab, wa);
+                       S.cd.as<float>(), dthreshold, N, W, par, lab, wa);
     sxg_kt_end(SXG_KT_DET_INIT);
     /* the foreground list has at most N entries; size it after counting */
     int *fg = sxg_take<int>("det.fg", S.N);

@@ -19,7 +19,8 @@
  *                workgroup (back.c:532-556); y second derivatives, one lane per mesh
  *                column with the recurrence in registers (back.c:739-780).
  *   k_rownodes   one lane per frame row: node interpolation along y and the x
- *                tridiagonal solve of subbackline (back.c:790-880).
+ *                tridiagonal solve of subbackline (back.c:790-880); 64 rows per
+ *                workgroup with the row arrays in LDS when they fit (k_rownodes_lds).
  *   k_subback    one lane per pixel: bicubic evaluation with the column tables of the
  *                cumulative dx recurrence (back.c:880-908), computed once on the host
  *                with the same float additions, and in-place subtraction.
@@ -797,6 +798,69 @@ __global__ void __launch_bounds__(256) k_rownodes(const float *back, const float
     }
 }
 
+/* the same for RN_ROWS consecutive rows per workgroup (one lane each) with the row
+   arrays in LDS, laid out [x][row] (no bank conflicts), and coalesced stores of the
+   finished rows; used when 3 * nbx * RN_ROWS floats fit in LDS */
+#define RN_ROWS 64
+__global__ void __launch_bounds__(RN_ROWS) k_rownodes_lds(const float *back, const float *dback,
+                                                          int nbx, int nby, int bs, int H,
+                                                          float *nodes, float *dnodes)
+{
+    extern __shared__ float rn[];
+    const int lane = threadIdx.x, y0 = blockIdx.x * RN_ROWS, y = y0 + lane;
+    float *node = rn, *dnode = rn + nbx * RN_ROWS, *u = dnode + nbx * RN_ROWS;
+#define RN(a, x) a[(x) * RN_ROWS + lane]
+    if (y < H) {
+        if (nby > 1) {
+            int yl;
+            float dy = (float)((float)y / bs - 0.5);
+            dy -= (yl = (int)dy);
+            if (yl < 0) {
+                yl = 0;
+                dy = (float)(dy - 1.0);
+            } else if (yl >= nby - 1) {
+                yl = nby < 2 ? 0 : nby - 2;
+                dy = (float)(dy + 1.0);
+            }
+            const float cdy = 1 - dy;
+            const float dy3 = (dy * dy * dy - dy);
+            const float cdy3 = (cdy * cdy * cdy - cdy);
+            const float *blo = back + nbx * yl, *bhi = blo + nbx;
+            const float *dblo = dback + nbx * yl, *dbhi = dblo + nbx;
+            for (int x = 0; x < nbx; x++)
+                RN(node, x) = cdy * blo[x] + dy * bhi[x] + cdy3 * dblo[x] + dy3 * dbhi[x];
+            if (nbx > 1) {
+                float temp, dprev = 0.0f, uprev = 0.0f;
+                RN(dnode, 0) = RN(u, 0) = 0.0f;
+                for (int x = 1; x < nbx - 1; x++) {
+                    temp = -1 / (dprev + 4);
+                    RN(dnode, x) = dprev = temp;
+                    temp *= uprev - 6 * (RN(node, x + 1) + RN(node, x - 1) - 2 * RN(node, x));
+                    RN(u, x) = uprev = temp;
+                }
+                RN(dnode, nbx - 1) = 0.0f;
+                float dnext = 0.0f;
+                for (int x = nbx - 2; x >= 1; x--)
+                    RN(dnode, x) = dnext = (float)((RN(dnode, x) * dnext + RN(u, x)) / 6.0);
+            }
+        } else {
+            for (int x = 0; x < nbx; x++) {
+                RN(node, x) = back[x];
+                RN(dnode, x) = dback[x];
+            }
+        }
+    }
+#undef RN
+    __syncthreads();
+    /* rows y0.. are contiguous in the node tables */
+    const int nr = min(RN_ROWS, H - y0), tot = nr * nbx;
+    for (int e = lane; e < tot; e += RN_ROWS) {
+        const int r = e / nbx, x = e - r * nbx;
+        nodes[(size_t)y0 * nbx + e] = node[x * RN_ROWS + r];
+        dnodes[(size_t)y0 * nbx + e] = dnode[x * RN_ROWS + r];
+    }
+}
+
 /* colseg[x]: index of the left node, coldx[x]: dx of the cumulative recurrence */
 __global__ void __launch_bounds__(256) k_subback(float *img, int W, int H, int nbx,
                                                  const float *nodes, const float *dnodes,
@@ -960,8 +1024,14 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
         SXG_HIP(hipMemcpyAsync(colseg, seg.data(), sizeof(int) * W, hipMemcpyHostToDevice, S.stream));
         SXG_HIP(hipMemcpyAsync(coldx, dxs.data(), sizeof(float) * W, hipMemcpyHostToDevice, S.stream));
         sxg_kt_begin(SXG_KT_ROWNODES);
-        hipLaunchKernelGGL(k_rownodes, dim3(sxg_div_up(H, 256)), dim3(256), 0, S.stream, back,
-                           dback, nbx, nby, bs, H, nodes, dnodes, utmp);
+        const size_t rnlds = 3 * (size_t)nbx * RN_ROWS * sizeof(float);
+        if (rnlds <= 64 * 1024) {
+            hipLaunchKernelGGL(k_rownodes_lds, dim3(sxg_div_up(H, RN_ROWS)), dim3(RN_ROWS), rnlds,
+                               S.stream, back, dback, nbx, nby, bs, H, nodes, dnodes);
+        } else {
+            hipLaunchKernelGGL(k_rownodes, dim3(sxg_div_up(H, 256)), dim3(256), 0, S.stream, back,
+                               dback, nbx, nby, bs, H, nodes, dnodes, utmp);
+        }
         sxg_kt_end(SXG_KT_ROWNODES);
         sxg_kt_begin(SXG_KT_SUBBACK);
         hipLaunchKernelGGL(k_subback, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0, S.stream,

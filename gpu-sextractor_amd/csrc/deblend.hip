@@ -806,7 +806,22 @@ struct ObjArgs {
     int *okey, *oval;       /* object index / pixel of each raw-list entry (sort input) */
     ObjCols o;
     int nobj;
+    const int *order;       /* objects in launch order: those of large raw objects first */
 };
+
+/* objects whose raw object has at least PRE_BIG pixels are preanalysed first (each scans
+   its raw object's list) */
+#define PRE_BIG 256
+
+__global__ void __launch_bounds__(256) k_pre_order(ObjArgs A, int *cnt, int *order)
+{
+    const int obj = blockIdx.x * 256 + threadIdx.x;
+    if (obj >= A.nobj) return;
+    if (A.d.cnt[A.objraw[obj]] >= PRE_BIG)
+        order[atomicAdd(&cnt[0], 1)] = obj;
+    else
+        order[A.nobj - 1 - atomicAdd(&cnt[1], 1)] = obj;
+}
 
 /* objects of each raw object, in component-table order (one wavefront per raw object) */
 __global__ void __launch_bounds__(256) k_deb_objlist(ObjArgs A)
@@ -859,8 +874,9 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
     __shared__ double s_rows[4][6 * SXW_ROW];
     __shared__ float s_frows[4][SXW_ROW];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int obj = blockIdx.x * 4 + wid;
-    if (obj >= A.nobj) return;
+    const int slot = blockIdx.x * 4 + wid;
+    if (slot >= A.nobj) return;
+    const int obj = A.order[slot];
     double *rows = s_rows[wid];
     float *frows = s_frows[wid];
     const DebArgs &d = A.d;
@@ -872,6 +888,7 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
     /* extent, peaks, count (reductions); the node's pixels get the object index */
     float peak = (float)-SXG_BIG, cpeak = (float)-SXG_BIG;
     int xmin = 2 * SXG_MAXPICSIZE, ymin = 2 * SXG_MAXPICSIZE, xmax = 0, ymax = 0, cnt = 0;
+#pragma unroll 4
     for (int k = lane; k < n; k += 64) {
         if (lv && lv[k] != c) continue;
         const int p = d.fpia[off + k];
@@ -901,12 +918,22 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
     double acc = 0.0;
     float racc = 0.0f;
     int area2 = 0, dnpix = 0;
+    /* the next chunk's membership and pixel are read while this chunk is folded */
+    bool nin = lane < n && (!lv || lv[lane] == c);
+    int npix_ = lane < n ? d.fpia[off + lane] : 0;
     for (int base = 0; base < n; base += 64) {
         const int k = base + lane;
-        const bool in = k < n && (!lv || lv[k] == c);
+        const bool in = nin;
+        const int pc = npix_;
+        if (k + 64 < n) {
+            nin = !lv || lv[k + 64] == c;
+            npix_ = d.fpia[off + k + 64];
+        } else {
+            nin = false;
+        }
         float val = 0.0f, cval = 0.0f, tx = 0.0f, ty = 0.0f, txx = 0.0f, tyy = 0.0f, txy = 0.0f;
         if (in) {
-            const int p = d.fpia[off + k];
+            const int p = pc;
             const int y = p / W - ymin, x = p - (p / W) * W - xmin;
             cval = d.cd[p];
             val = A.sub[p];
@@ -1169,10 +1196,10 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<1024>), 2, 1024, ct2))
             return -1;
         if (cc[1] > 0 &&
-            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<256>), 1, 256, ct1))
+            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<1024>), 1, 1024, ct1))
             return -1;
         if (cc[0] > 0 &&
-            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<64>), 0, 64, ct0))
+            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<256>), 0, 256, ct0))
             return -1;
         for (int j = 0; j < nside; j++) SXG_HIP_RET(hipStreamWaitEvent(st, S.side_ev[j], 0), -1);
         sxg_kt_end(SXG_KT_DEB_LEVELS);
@@ -1210,6 +1237,12 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         sxg_kt_begin(SXG_KT_DEB_OBJECTS);
         hipLaunchKernelGGL(k_deb_objlist, dim3(sxg_div_up(nraw, 4)), dim3(256), 0, st, A);
         hipLaunchKernelGGL(k_deb_allocinit, dim3(gp), dim3(256), 0, st, A);
+        int *order = sxg_take<int>("pre.order", (size_t)nobj + 2);
+        if (!order) return -1;
+        SXG_HIP_RET(hipMemsetAsync(order + nobj, 0, 2 * sizeof(int), st), -1);
+        hipLaunchKernelGGL(k_pre_order, dim3(sxg_div_up(nobj, 256)), dim3(256), 0, st, A,
+                           order + nobj, order);
+        A.order = order;
         hipLaunchKernelGGL(k_deb_pre, dim3(sxg_div_up(nobj, 4)), dim3(256), 0, st, A);
         hipLaunchKernelGGL(k_deb_gather, dim3(gp), dim3(256), 0, st, A);
         sxg_kt_end(SXG_KT_DEB_OBJECTS);

@@ -150,10 +150,14 @@ __global__ void __launch_bounds__(256) k_keepflag(const unsigned *master, int n,
     if (i < n) keep[i] = master[i] == 0xffffffffu ? 1 : 0;
 }
 
+/* a[0..n) = v, four per lane with 16-byte stores (a is 16-byte aligned) */
 __global__ void __launch_bounds__(256) k_fill_i32(int *a, long n, int v)
 {
-    const long k = (long)blockIdx.x * 256 + threadIdx.x;
-    if (k < n) a[k] = v;
+    const long q = (long)blockIdx.x * 256 + threadIdx.x, k = 4 * q;
+    if (k + 4 <= n)
+        *reinterpret_cast<int4 *>(a + k) = make_int4(v, v, v, v);
+    else
+        for (long j = k; j < n; j++) a[j] = v;
 }
 
 /* seg[p] = 1-based rank of the surviving object of p (0 for cleaned objects) */
@@ -175,8 +179,8 @@ static int *build_map(int mode)
     int *map = sxg_take<int>("out.map", S.N);
     if (!map) return nullptr;
     const long N = (long)S.N;
-    hipLaunchKernelGGL(k_fill_i32, dim3(sxg_div_up(N, 256)), dim3(256), 0, S.stream, map, N,
-                       mode ? -1 : 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(sxg_div_up(sxg_div_up(N, 4), 256)), dim3(256), 0,
+                       S.stream, map, N, mode ? -1 : 0);
     if (S.nobj == 0 || S.npix_raw == 0) return map;
     ObjCols o;
     if (!sxg_obj_view(o)) return nullptr;

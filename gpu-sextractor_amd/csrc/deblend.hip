@@ -305,41 +305,69 @@ __host__ __device__ inline int deb_nodecap(int nmax, int n, int ls, int debarea)
     return c < 0x7fffffff ? (int)c : 0x7fffffff;
 }
 
-/* L* and the node capacity of each raw object (the total in 64 bits) */
+/* L* and the node capacity of each raw object (the total in 64 bits: one atomic per
+   wavefront) */
 __global__ void __launch_bounds__(256) k_deb_nodecap(DebArgs a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= a.nraw) return;
-    const int ls = a.top2[r] == 0u ? 0 : a.lstar[r] - 1;
-    a.lstar[r] = ls;
-    const int c = deb_nodecap(a.nmax[r], a.cnt[r], ls, a.debarea);
-    a.compoff[r] = c;
-    atomicAdd(&a.dwork[2], (unsigned long long)c);
+    unsigned long long c = 0;
+    if (r < a.nraw) {
+        const int ls = a.top2[r] == 0u ? 0 : a.lstar[r] - 1;
+        a.lstar[r] = ls;
+        const int cap = deb_nodecap(a.nmax[r], a.cnt[r], ls, a.debarea);
+        a.compoff[r] = cap;
+        c = (unsigned long long)cap;
+    }
+    for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&a.dwork[2], c);
 }
 
-/* single-peak objects are kept as they are (one node); the others go to a size class (0: up to
-   cap0 pixels, 64-lane LDS workgroups; 1: up to cap1, 256 lanes; 2: up to cap2, 1024
-   lanes; 3: global memory), and to class 3 whenever their tree may outgrow the class's
-   LDS node table (ct0..ct2 nodes) */
+/* single-peak objects are kept as they are (one node); the others go to a size class (0:
+   up to cap0 pixels, 256-lane LDS workgroups; 1: up to cap1, 1024 lanes; 2: up to cap2,
+   1024 lanes; 3: global memory), and to class 3 whenever their tree may outgrow the
+   class's LDS node table (ct0..ct2 nodes).  The counters take one atomic per wavefront
+   and class. */
 __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int cap1, int cap2,
                                                       int ct0, int ct1, int ct2)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= a.nraw) return;
-    const int n = a.cnt[r], ls = a.lstar[r];
-    if (ls == 0) {
-        a.rootnc[r] = 1;
-        a.rootkept[r] = 1;
-        return;
+    const int lane = threadIdx.x & 63;
+    int cls = -1, n = 0, ls = 0;
+    if (r < a.nraw) {
+        n = a.cnt[r];
+        ls = a.lstar[r];
+        if (ls == 0) {
+            a.rootnc[r] = 1;
+            a.rootkept[r] = 1;
+        } else {
+            const int cap = deb_nodecap(a.nmax[r], n, ls, a.debarea);
+            cls = n <= cap0 ? 0 : n <= cap1 ? 1 : n <= cap2 ? 2 : 3;
+            if ((cls == 0 && cap > ct0) || (cls == 1 && cap > ct1) || (cls == 2 && cap > ct2))
+                cls = 3;
+        }
     }
-    const int cap = deb_nodecap(a.nmax[r], n, ls, a.debarea);
-    int cls = n <= cap0 ? 0 : n <= cap1 ? 1 : n <= cap2 ? 2 : 3;
-    if ((cls == 0 && cap > ct0) || (cls == 1 && cap > ct1) || (cls == 2 && cap > ct2)) cls = 3;
-    atomicAdd(&a.dwork[0], (unsigned long long)n);
-    atomicAdd(&a.dwork[1], (unsigned long long)n * ls);
-    const int j = atomicAdd(&a.ccount[cls], 1);
-    a.clist[(size_t)cls * a.nraw + j] = r;
-    if (cls < 3) atomicMax(&a.ccount[4 + cls], n);
+    unsigned long long w0 = cls >= 0 ? (unsigned long long)n : 0ull;
+    unsigned long long w1 = cls >= 0 ? (unsigned long long)n * ls : 0ull;
+    for (int d = 32; d > 0; d >>= 1) {
+        w0 += __shfl_xor(w0, d);
+        w1 += __shfl_xor(w1, d);
+    }
+    if (lane == 0 && w0) {
+        atomicAdd(&a.dwork[0], w0);
+        atomicAdd(&a.dwork[1], w1);
+    }
+    for (int c = 0; c < 4; c++) {
+        const unsigned long long b = __ballot(cls == c);
+        if (!b) continue;
+        const int lead = __builtin_ctzll(b);
+        int base = 0;
+        if (lane == lead) base = atomicAdd(&a.ccount[c], __popcll(b));
+        base = __shfl(base, lead);
+        int mx = cls == c ? n : 0;
+        for (int d = 32; d > 0; d >>= 1) mx = max(mx, __shfl_xor(mx, d));
+        if (lane == lead && c < 3) atomicMax(&a.ccount[4 + c], mx);
+        if (cls == c) a.clist[(size_t)c * a.nraw + base + __popcll(b & sxw_lanes_below(lane))] = r;
+    }
 }
 
 /*

@@ -132,6 +132,33 @@ template <bool G> __device__ inline void uf_unite(int *par, int a, int b)
     }
 }
 
+/* exclusive prefix of an int over the workgroup in thread order, and the total; all
+   threads call it */
+template <int NT> __device__ inline int wg_scan_excl(int v, int *wsum, int &total)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(x, d);
+        if (lane >= d) x += o;
+    }
+    if (NT == 64) {
+        total = __shfl(x, 63);
+        return x - v;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int w = 0; w < NT / 64; w++) {
+        const int s = wsum[w];
+        before += w < wid ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return before + x - v;
+}
+
 /* exclusive prefix of a flag over the workgroup in thread order; all threads call it */
 template <int NT> __device__ inline int wg_prefix(bool f, int *wsum, int &total)
 {
@@ -202,6 +229,24 @@ struct DebNode {
     double ssum, sabs;      /* sums of cd and |cd| over the pixels (any order) */
 };
 
+/* LDS bytes per pixel of k_deb_tree_lds: cd, parent, count (4 bytes each), the
+   neighbour word (8 bytes), the highest node (16 bits), the level index */
+#define DEB_LDS_BYTES_PER_PIX 23
+/* LDS a workgroup may use (dynamic part; the static part of deb_tree is under 2 KiB) */
+#define DEB_LDS_LIMIT ((size_t)160 * 1024 - 4096)
+
+__host__ __device__ inline size_t deb_lds_pixbytes(int cap)
+{
+    return ((size_t)cap * DEB_LDS_BYTES_PER_PIX + 15) & ~(size_t)15;
+}
+
+/* LDS of k_deb_tree_lds for a tree of ct nodes over n pixels */
+__host__ __device__ inline int deb_lds_bytes(int ct, int n)
+{
+    return (int)((((size_t)ct * sizeof(DebNode) + 15) & ~(size_t)15) +
+                 deb_lds_pixbytes((n + 63) & ~63));
+}
+
 struct DebArgs {
     const int *fpia;        /* raw pixel list (raster order within each raw object) */
     const int *rawkey;      /* raw object id of each raw-list entry */
@@ -225,14 +270,14 @@ struct DebArgs {
     int *nmax;              /* local maxima per raw object */
     int *lstar;             /* first level above the second-highest maximum, then L* */
     int *clist;             /* objects of each size class: clist[cls * nraw + j] */
-    int *ccount;            /* [0..3] objects per class, [4..6] largest object of classes
-                               0..2, [7] consistency error flag */
+    int *ccount;            /* [0..3] objects per class, [4..6] the largest LDS need of
+                               classes 0..2 (bytes), [7] consistency error flag */
     unsigned long long *dwork;  /* [0] pixels, [1] pixel-levels of the multi-peak objects,
                                    [2] node capacity of the frame */
     /* global scratch for the objects built in global memory, by raw-list position */
     float *g_cdv;
     unsigned char *g_m;
-    int *g_nb;              /* neighbour words: 2 ints per raw-list entry */
+    int *g_nb;              /* neighbour words: 4 ints per raw-list entry */
     int *g_par, *g_deep, *g_cnt;
 };
 
@@ -324,15 +369,15 @@ __global__ void __launch_bounds__(256) k_deb_nodecap(DebArgs a)
 
 /* single-peak objects are kept as they are (one node); the others go to a size class (0:
    up to cap0 pixels, 256-lane LDS workgroups; 1: up to cap1, 1024 lanes; 2: up to cap2,
-   1024 lanes; 3: global memory), and to class 3 whenever their tree may outgrow the
-   class's LDS node table (ct0..ct2 nodes).  The counters take one atomic per wavefront
-   and class. */
+   1024 lanes; 3: global memory), and up a class while their pixels and node bound need
+   more LDS than the class's budget (bud0..bud2 bytes).  The counters take one atomic per
+   wavefront and class; [4 + class] records the largest need. */
 __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int cap1, int cap2,
-                                                      int ct0, int ct1, int ct2)
+                                                      int bud0, int bud1, int bud2)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    int cls = -1, n = 0, ls = 0;
+    int cls = -1, n = 0, ls = 0, need = 0;
     if (r < a.nraw) {
         n = a.cnt[r];
         ls = a.lstar[r];
@@ -340,10 +385,13 @@ __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int c
             a.rootnc[r] = 1;
             a.rootkept[r] = 1;
         } else {
-            const int cap = deb_nodecap(a.nmax[r], n, ls, a.debarea);
+            /* LDS of the object's workgroup: its node table, then its pixels */
+            need = deb_lds_bytes(deb_nodecap(a.nmax[r], n, ls, a.debarea), n);
             cls = n <= cap0 ? 0 : n <= cap1 ? 1 : n <= cap2 ? 2 : 3;
-            if ((cls == 0 && cap > ct0) || (cls == 1 && cap > ct1) || (cls == 2 && cap > ct2))
-                cls = 3;
+            /* a tree too large for its class's budget moves up a class */
+            if (cls == 0 && need > bud0) cls = 1;
+            if (cls == 1 && need > bud1) cls = 2;
+            if (cls == 2 && need > bud2) cls = 3;
         }
     }
     unsigned long long w0 = cls >= 0 ? (unsigned long long)n : 0ull;
@@ -363,7 +411,7 @@ __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int c
         int base = 0;
         if (lane == lead) base = atomicAdd(&a.ccount[c], __popcll(b));
         base = __shfl(base, lead);
-        int mx = cls == c ? n : 0;
+        int mx = cls == c ? need : 0;
         for (int d = 32; d > 0; d >>= 1) mx = max(mx, __shfl_xor(mx, d));
         if (lane == lead && c < 3) atomicMax(&a.ccount[4 + c], mx);
         if (cls == c) a.clist[(size_t)c * a.nraw + base + __popcll(b & sxw_lanes_below(lane))] = r;
@@ -405,8 +453,11 @@ __device__ inline int deb_sig_bound(double s, double sabs, int n, float b, doubl
 #define NB_NW 2
 #define NB_N 4
 #define NB_NE 8
-template <typename IT> struct NbP {
-    IT j0, mask;
+template <typename IT> struct __align__(8) NbP {
+    IT j0;
+    /* mask: which neighbours exist; the level indices of the pixel and of its W, NW, N
+       and NE neighbours (0 when absent), so one read decides the pixel's joins */
+    unsigned char mask, mself, mw, mnw, mn, mne;
 };
 
 /*
@@ -546,7 +597,13 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         }
         NbP<IT> b;
         b.j0 = (IT)j0;
-        b.mask = (IT)mask;
+        b.mask = (unsigned char)mask;
+        b.mself = m[k];
+        b.mw = (mask & NB_W) ? m[k - 1] : 0;
+        const int jn = j0 + ((mask & NB_NW) ? 1 : 0), jne = jn + ((mask & NB_N) ? 1 : 0);
+        b.mnw = (mask & NB_NW) ? m[j0] : 0;
+        b.mn = (mask & NB_N) ? m[jn] : 0;
+        b.mne = (mask & NB_NE) ? m[jne] : 0;
         nb[k] = b;
         deep[k] = 0;
     }
@@ -577,17 +634,14 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         /* join the entering pixels */
         for (int k = tid; k < n; k += NT) {
             cnt[k] = 0;
-            const int mk = m[k];
+            const NbP<IT> b = nb[k];
+            const int mk = b.mself;
             if (mk < i) continue;
             const bool ek = mk < prev;
-            const NbP<IT> b = nb[k];
             const int msk = (int)b.mask, j0 = (int)b.j0;
             const int jnw = j0, jn = j0 + ((msk & NB_NW) ? 1 : 0);
             const int jne = jn + ((msk & NB_N) ? 1 : 0);
-            const int mw = (msk & NB_W) ? (int)m[k - 1] : -1;
-            const int mnw = (msk & NB_NW) ? (int)m[jnw] : -1;
-            const int mn = (msk & NB_N) ? (int)m[jn] : -1;
-            const int mne = (msk & NB_NE) ? (int)m[jne] : -1;
+            const int mw = b.mw, mnw = b.mnw, mn = b.mn, mne = b.mne;
             const bool wl = mw >= i, nw = mnw >= i, nn = mn >= i, ne = mne >= i;
             /* runs of entering pixels are linked inside their window already */
             if (wl && (ek || mw < prev) && !(ek && mw < prev && (k & 63) != 0))
@@ -610,27 +664,29 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
                 atomicAdd(&cnt[rt], 1);
             }
         __syncthreads();
-        /* the nodes of level i in raster order of their first pixel */
+        /* the nodes of level i in raster order of their first pixel: each thread numbers
+           a contiguous run of the list after one prefix over the threads' counts */
         int nkept = 0;
-        for (int base = 0; base < n; base += NT) {
-            const int k = base + tid;
-            const bool isroot = k < n && m[k] >= i && ldw<G>(&par[k]) == k;
-            const int ck = isroot ? ldw<G>(&cnt[k]) : 0;
-            const bool kp = isroot && ck >= a.debarea;
-            int tot;
-            const int pre = wg_prefix<NT>(kp, wsum, tot);
-            if (isroot) {
-                int j = -1;
-                if (kp) {
-                    j = nkept + pre;
-                    DebNode e;
-                    e.level = i; e.parent = -1; e.count = ck; e.nchild = 0; e.nson = 0;
-                    e.flags = 0; e.kroot = k; e.kanc = -1; e.ssum = 0.0; e.sabs = 0.0;
-                    T[nc + j] = e;
+        {
+            const int per = (n + NT - 1) / NT;
+            const int kb = tid * per, ke = min(kb + per, n);
+            int mine = 0;
+            for (int k = kb; k < ke; k++)
+                if (m[k] >= i && ldw<G>(&par[k]) == k && ldw<G>(&cnt[k]) >= a.debarea) mine++;
+            int j = wg_scan_excl<NT>(mine, wsum, nkept);
+            for (int k = kb; k < ke; k++)
+                if (m[k] >= i && ldw<G>(&par[k]) == k) {
+                    const int ck = ldw<G>(&cnt[k]);
+                    int jk = -1;
+                    if (ck >= a.debarea) {
+                        jk = j++;
+                        DebNode e;
+                        e.level = i; e.parent = -1; e.count = ck; e.nchild = 0; e.nson = 0;
+                        e.flags = 0; e.kroot = k; e.kanc = -1; e.ssum = 0.0; e.sabs = 0.0;
+                        T[nc + jk] = e;
+                    }
+                    cnt[k] = jk;
                 }
-                cnt[k] = j;
-            }
-            nkept += tot;
         }
         __syncthreads();
         const bool sums = nkept >= 2;
@@ -655,10 +711,14 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
             int j = -1;
             float v = 0.0f;
             if (mk >= i) {
-                j = ldw<G>(&cnt[ldw<G>(&par[k])]);
-                if (j >= 0) {
-                    if (deep[k] == (IT)0) deep[k] = (IT)(nc + j);
-                    v = cdv[k];
+                /* a pixel with its highest node already needs its node only for sums */
+                const IT dk = deep[k];
+                if (dk == (IT)0 || sums) {
+                    j = ldw<G>(&cnt[ldw<G>(&par[k])]);
+                    if (j >= 0) {
+                        if (dk == (IT)0) deep[k] = (IT)(nc + j);
+                        v = cdv[k];
+                    }
                 }
             }
             if (sums && j >= 0) {
@@ -784,24 +844,14 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
     }
 }
 
-/* LDS bytes per pixel of k_deb_tree_lds: cd, parent, count (4 bytes each), the
-   neighbour word (2 x 16 bits), the highest node (16 bits), the level index */
-#define DEB_LDS_BYTES_PER_PIX 19
-/* LDS a workgroup may use (dynamic part; the static part of deb_tree is under 2 KiB) */
-#define DEB_LDS_LIMIT ((size_t)160 * 1024 - 4096)
-
-__host__ __device__ inline size_t deb_lds_pixbytes(int cap)
-{
-    return ((size_t)cap * DEB_LDS_BYTES_PER_PIX + 15) & ~(size_t)15;
-}
-
 template <int NT>
-__global__ void __launch_bounds__(NT) k_deb_tree_lds(DebArgs a, int cls, int cap)
+__global__ void __launch_bounds__(NT) k_deb_tree_lds(DebArgs a, int cls)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r = a.clist[(size_t)cls * a.nraw + blockIdx.x];
     DebNode *tl = reinterpret_cast<DebNode *>(smem);
     const int ct = deb_nodecap(a.nmax[r], a.cnt[r], a.lstar[r], a.debarea);
+    const int cap = (a.cnt[r] + 63) & ~63;
     float *cdv = reinterpret_cast<float *>(smem + (((size_t)ct * sizeof(DebNode) + 15) & ~15));
     int *par = reinterpret_cast<int *>(cdv + cap);
     int *cnt = par + cap;
@@ -1144,17 +1194,15 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         const int cap0 = (int)sxg_option(SXG_OPT_DEB_LDS_TINY);
         const int cap1 = (int)sxg_option(SXG_OPT_DEB_LDS_SMALL);
         const int cap2 = (int)sxg_option(SXG_OPT_DEB_LDS_LARGE);
-        /* tree nodes each class keeps in LDS (SXG_OPT_DEB_TABLE, else 192 / 512 / 1024),
-           within what its pixels leave of the workgroup's LDS */
+        /* LDS budget of each class's workgroups: its pixel capacity and a node table of
+           SXG_OPT_DEB_TABLE nodes (else 192 / 512 / 1024); an object whose own pixels and
+           node bound need more goes to the global kernel */
         const long topt = sxg_option(SXG_OPT_DEB_TABLE);
-        auto nodes_for = [&](int cap, long dflt) -> int {
-            const size_t avail = DEB_LDS_LIMIT - deb_lds_pixbytes((cap + 63) & ~63);
-            const size_t ct = std::min<size_t>(topt > 0 ? (size_t)topt : (size_t)dflt,
-                                               avail / sizeof(DebNode));
-            return (int)std::max<size_t>(ct, 1);
+        auto budget = [&](int cap, long dflt) -> int {
+            const long nodes = topt > 0 ? topt : dflt;
+            return (int)std::min<size_t>(DEB_LDS_LIMIT, (size_t)deb_lds_bytes((int)nodes, cap));
         };
-        const int ct0 = nodes_for(cap0, 192), ct1 = nodes_for(cap1, 512);
-        const int ct2 = nodes_for(cap2, 1024);
+        const int ct0 = budget(cap0, 192), ct1 = budget(cap1, 512), ct2 = budget(cap2, 1024);
         const int gr = sxg_div_up(nraw, 256);
         sxg_kt_begin(SXG_KT_DEB_LMAX);
         hipLaunchKernelGGL(k_deb_prep, dim3(sxg_div_up(nraw > 8 ? nraw : 8, 256)), dim3(256), 0,
@@ -1190,13 +1238,12 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
            Everything they read was finished by the synchronisation above. */
         int nside = 0;
         auto launch_lds = [&](const void *fn, int cls, int nt, int ct) -> bool {
-            int cap = (cc[4 + cls] + 63) & ~63;
-            const size_t lds = (((size_t)ct * sizeof(DebNode) + 15) & ~(size_t)15) +
-                               deb_lds_pixbytes(cap);
+            (void)ct;
+            const size_t lds = (size_t)cc[4 + cls];
             SXG_HIP_RET(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             (int)lds),
                         false);
-            void *args[] = {&a, &cls, &cap};
+            void *args[] = {&a, &cls};
             const bool side = cls != 0 && !S.single_stream;
             hipStream_t ks = side ? S.side[nside] : st;
             SXG_HIP_RET(hipLaunchKernel(fn, dim3(cc[cls]), dim3(nt), args, lds, ks), false);
@@ -1207,7 +1254,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             a.nodes = sxg_take<DebNode>("deb.nodes", (size_t)ntot + 1);
             a.g_cdv = sxg_take<float>("deb.g_cdv", (size_t)P0);
             a.g_m = sxg_take<unsigned char>("deb.g_m", (size_t)P0);
-            a.g_nb = sxg_take<int>("deb.g_nb", 2 * (size_t)P0);
+            a.g_nb = sxg_take<int>("deb.g_nb", 4 * (size_t)P0);
             a.g_par = sxg_take<int>("deb.g_par", (size_t)P0);
             a.g_deep = sxg_take<int>("deb.g_deep", (size_t)P0);
             a.g_cnt = sxg_take<int>("deb.g_cnt", (size_t)P0);

@@ -85,9 +85,9 @@ ALG_BYTES = {
 # 1.0); see profiles/hbm_traffic_crowded4096_pmc.md.  Used for roofline.traffic on that
 # workload only.
 MEASURED_TRAFFIC_CROWDED4096 = {
-    "meshstat": 1.345e8, "meshmode": 6.849e7, "analyse": 6.277e7,
-    "deb_objects": 1.384e8, "det_union": 2.58e8, "conv": 2.003e8, "subback": 1.51e8,
-    "det_init": 2.013e8, "copy": 1.342e8, "clean": 1.08e7, "filterback": 8.89e5,
+    "deb_levels": 1.273e7, "meshstat": 1.345e8, "meshmode": 6.858e7, "analyse": 6.635e7,
+    "conv": 2.003e8, "subback": 1.51e8, "det_init": 2.013e8, "det_union": 7.577e7,
+    "copy": 1.342e8, "clean": 1.08e7, "rownodes": 2.26e6,
 }
 
 

@@ -1237,6 +1237,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         /* the classes run side by side: each kernel lasts as long as its slowest tree.
            Everything they read was finished by the synchronisation above. */
         int nside = 0;
+        const bool use_side = !S.single_stream && sxg_option(SXG_OPT_SIDE_STREAMS) != 0;
         auto launch_lds = [&](const void *fn, int cls, int nt, int ct) -> bool {
             (void)ct;
             const size_t lds = (size_t)cc[4 + cls];
@@ -1244,7 +1245,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
                                             (int)lds),
                         false);
             void *args[] = {&a, &cls};
-            const bool side = cls != 0 && !S.single_stream;
+            const bool side = cls != 0 && use_side;
             hipStream_t ks = side ? S.side[nside] : st;
             SXG_HIP_RET(hipLaunchKernel(fn, dim3(cc[cls]), dim3(nt), args, lds, ks), false);
             if (side) SXG_HIP_RET(hipEventRecord(S.side_ev[nside++], ks), false);
@@ -1260,9 +1261,9 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             a.g_cnt = sxg_take<int>("deb.g_cnt", (size_t)P0);
             if (!a.nodes || !a.g_cdv || !a.g_m || !a.g_nb || !a.g_par || !a.g_deep || !a.g_cnt)
                 return -1;
-            hipStream_t ks = S.single_stream ? st : S.side[nside];
+            hipStream_t ks = use_side ? S.side[nside] : st;
             hipLaunchKernelGGL(k_deb_tree_glb, dim3(cc[3]), dim3(1024), 0, ks, a);
-            if (!S.single_stream) {
+            if (use_side) {
                 SXG_HIP_RET(hipEventRecord(S.side_ev[nside], ks), -1);
                 nside++;
             }

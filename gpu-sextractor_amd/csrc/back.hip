@@ -3,10 +3,10 @@
  * the reference's src/cuda/cudaback.cu:1212), bit-exact with the CPU code src/back.c.
  *
  * Kernels:
- *   k_meshstat   one lane per mesh: backstat (back.c:214-290).  The double sums of pix
- *                and pix^2 are sequential in raster order inside the mesh, as back.c
- *                does, so each lane runs the chain; three wavefronts of the workgroup
- *                stream the mesh rows through a double-buffered LDS ring meanwhile.
+ *   k_meshstat   backstat (back.c:214-290).  The double sums of pix and pix^2 are
+ *                sequential in raster order inside the mesh, as back.c does, so one lane
+ *                runs each sum's chain; three wavefronts of the workgroup stream the mesh
+ *                rows through a double-buffered LDS ring meanwhile.
  *   k_meshmode   one workgroup per mesh: histogram in LDS (back.c:297-341), then the
  *                clipped-mode iteration of backguess (back.c:347-426).  Inside the
  *                iteration every sum is an exact integer (< 2^53), so the sequential
@@ -15,8 +15,8 @@
  *                by a binary search on prefix sums (merge-path), which reproduces the
  *                walk exactly.
  *   k_fb_*       bad-mesh interpolation and median filtering, one lane per mesh
- *                (back.c:434-530); global medians by an LDS bitonic sort in one
- *                workgroup (back.c:532-556); y second derivatives, one lane per mesh
+ *                (back.c:434-530); global medians by radix selection, one workgroup
+ *                per map (back.c:532-556); y second derivatives, one lane per mesh
  *                column with the recurrence in registers (back.c:739-780).
  *   k_rownodes   one lane per frame row: node interpolation along y and the x
  *                tridiagonal solve of subbackline (back.c:790-880); 64 rows per

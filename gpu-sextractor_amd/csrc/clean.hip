@@ -1,6 +1,6 @@
 /*
  * clean.hip -- CLEAN (run_clean, the reference's src/cuda/cudaclean.cu:260), catalogue
- * rows (endobject_gpu, cudaanalyse.cu:3344, isophotal columns) and the outputs the
+ * rows are built by photom.hip (endobject_gpu); here are the outputs the
  * reference never wrote: the SEGMENTATION map and per-object columns.
  *
  * CLEAN semantics (cudaclean.cu:10-29, :120-174): amp = fdflux/(2 pi a b abcor),
@@ -306,60 +306,6 @@ extern "C" int sxg_get_frame_info(sxg_frame_info *info)
     info->deb_pixels = S.deb_pixels;
     info->deb_labels = S.deb_labels;
     return 0;
-}
-
-/* --------------------------------------------------------------- catalogue ---- */
-extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *objlist, void *wcs,
-                              int fymin, int fymax, int width, int height, float backsig,
-                              float thresh, double satur_level, double pixscale, double ngamma,
-                              double gain, int numobj)
-{
-    (void)prefs; (void)brain; (void)flagobj2; (void)wcs; (void)fymin; (void)fymax; (void)height;
-    (void)backsig; (void)thresh; (void)satur_level; (void)pixscale; (void)ngamma; (void)gain;
-    SxgState &S = sxg_state();
-    if (!S.have_frame || numobj != S.nobj || width != S.W) {
-        sxg_set_error("endobject_gpu: numobj/width differ from parcel_out");
-        return;
-    }
-    if (!objlist || numobj == 0) return;
-    const int n = numobj;
-    std::vector<double> mx(n), my(n), pm2(n), pmy(n), pmxy(n);
-    std::vector<float> a(n), b(n), th(n), fl(n), fe(n), pk(n), bk(n), fw(n);
-    std::vector<int> flag(n);
-    if (sxg_get_column(SXG_COL_MX, mx.data(), 8 * n) || sxg_get_column(SXG_COL_MY, my.data(), 8 * n) ||
-        sxg_get_column(SXG_COL_POSERR_MX2, pm2.data(), 8 * n) ||
-        sxg_get_column(SXG_COL_POSERR_MY2, pmy.data(), 8 * n) ||
-        sxg_get_column(SXG_COL_POSERR_MXY, pmxy.data(), 8 * n) ||
-        sxg_get_column(SXG_COL_A, a.data(), 4 * n) || sxg_get_column(SXG_COL_B, b.data(), 4 * n) ||
-        sxg_get_column(SXG_COL_THETA, th.data(), 4 * n) ||
-        sxg_get_column(SXG_COL_FLUX, fl.data(), 4 * n) ||
-        sxg_get_column(SXG_COL_FLUXERR, fe.data(), 4 * n) ||
-        sxg_get_column(SXG_COL_PEAK, pk.data(), 4 * n) || sxg_get_column(SXG_COL_BKG, bk.data(), 4 * n) ||
-        sxg_get_column(SXG_COL_FWHM, fw.data(), 4 * n) ||
-        sxg_get_column(SXG_COL_FLAG, flag.data(), 4 * n))
-        return;
-    sxg_outobj *rows = static_cast<sxg_outobj *>(objlist);
-    for (int i = 0; i < n; i++) {
-        sxg_outobj &r = rows[i];
-        memset(&r, 0, sizeof(r));
-        r.number = i;
-        r.sposx = (float)(mx[i] + 1.0);   /* FITS convention, cudaanalyse.cu:2715-2716 */
-        r.sposy = (float)(my[i] + 1.0);
-        r.flux_iso = fl[i];
-        r.fluxerr_iso = (float)std::sqrt((double)fe[i]);
-        r.mag_iso = r.flux_iso > 0.0 ? (float)(-2.5 * std::log10((double)r.flux_iso) + S.mag_zeropoint) : 99.0f;
-        r.magerr_iso = r.flux_iso > 0.0 ? (float)(1.086 * r.fluxerr_iso / r.flux_iso) : 99.0f;
-        r.theta = th[i];
-        r.poserr_mx2 = pm2[i];
-        r.poserr_my2 = pmy[i];
-        r.poserr_mxy = pmxy[i];
-        r.flag = (short)flag[i];
-        r.elong = a[i] / b[i];
-        r.ellip = 1 - b[i] / a[i];
-        r.peak = pk[i];
-        r.bkg = bk[i];
-        r.fwhm = fw[i];
-    }
 }
 
 extern "C" void sxg_set_mag_zeropoint(double zp) { sxg_state().mag_zeropoint = zp; }

@@ -117,6 +117,7 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
         S.N = N;
         S.nraw = S.nobj = S.nsurv = 0;
         S.cleaned = false;
+        S.phot_ready = false;
         S.filtered = false;
         S.deb_ready = false;
         for (int i = 0; i < ST_N; i++) S.ev_used[i] = false;

@@ -131,6 +131,8 @@ struct SxgState {
     bool clean_enabled = true;
     float deb_base = 0.0f;
     double mag_zeropoint = 0.0;
+    sxg_photom_params phot = {5.0, {2.5, 3.5}, {0.0, 0.0}, 1.2, 0.0, 1, 0, 1, 1, 1};
+    bool phot_ready = false;               /* endobject_gpu ran on this frame */
     float stage_ms[ST_N] = {};
 
     /* per-context workspace */

@@ -449,6 +449,7 @@ extern "C" void init_device(int width, int height, float *imgbuf)
     S.N = (size_t)width * height;
     S.nraw = S.nobj = S.nsurv = 0;
     S.cleaned = false;
+    S.phot_ready = false;
     S.filtered = false;
     for (int i = 0; i < ST_N; i++) S.ev_used[i] = false;
     if (!S.img.ensure(S.N * sizeof(float)) || !S.cd.ensure(S.N * sizeof(float)) ||

@@ -202,6 +202,87 @@ class FrameInfo(C.Structure):
                 ("deb_labels", C.c_int64)]
 
 
+class Brain(C.Structure):
+    """The star/galaxy network (sxg_brain: the reference's brainstruct layout,
+    src/types.h:930-943)."""
+    _fields_ = [("layersnb", C.c_int), ("nn", C.c_int * 3),
+                ("inbias", C.c_double * 10), ("inscale", C.c_double * 10),
+                ("outbias", C.c_double * 10), ("outscale", C.c_double * 10),
+                ("ni", C.c_double * 10), ("no", C.c_double * 10),
+                ("n", C.c_double * 10 * 3), ("w", C.c_double * 10 * 10 * 2),
+                ("b", C.c_double * 10 * 2)]
+
+
+class PhotomParams(C.Structure):
+    _fields_ = [("apert", C.c_double), ("autoparam", C.c_double * 2),
+                ("autoaper", C.c_double * 2), ("seeing_fwhm", C.c_double),
+                ("mag_zeropoint", C.c_double), ("mask_correct", C.c_int), ("photo", C.c_int),
+                ("do_aper", C.c_int), ("do_auto", C.c_int), ("do_class", C.c_int)]
+
+
+def read_nnw(path):
+    """Load a NNW weight table (STARNNW_NAME) the way the reference's getnnw reads it
+    (src/neurro.c:96-180): after the "NNW" line and skipping comment and blank lines, the
+    number of layers and their sizes, the input biases, the input scales, for each layer
+    and neuron its input weights followed by its bias, the output biases and the output
+    scales."""
+    if not os.path.isabs(path) and not os.path.exists(path):
+        path = os.path.join(CONFIG_DIR, path)
+    with open(path) as f:
+        if not f.readline().startswith("NNW"):
+            raise ValueError(path + " is not a NNW table")
+        rows = []
+        for line in f:
+            t = line.lstrip(" \t")
+            if t and t[0] not in "#\n":
+                rows.append([float(v) for v in t.split()])
+    br = Brain()
+    br.layersnb = int(rows[0][0])
+    if not 2 <= br.layersnb <= 3:
+        raise ValueError(path + ": unsupported number of layers")
+    for l in range(br.layersnb):
+        br.nn[l] = int(rows[0][1 + l])
+    nin, nout = br.nn[0], br.nn[br.layersnb - 1]
+    for k in range(nin):
+        br.inbias[k] = rows[1][k]
+        br.inscale[k] = rows[2][k]
+    r = 3
+    for l in range(br.layersnb - 1):
+        for j in range(br.nn[l + 1]):
+            for k in range(br.nn[l]):
+                br.w[l][k][j] = rows[r][k]
+            br.b[l][j] = rows[r][br.nn[l]]
+            r += 1
+    for k in range(nout):
+        br.outbias[k] = rows[r][k]
+        br.outscale[k] = rows[r + 1][k]
+    return br
+
+
+PHOT_COLUMNS = ["flux_iso", "fluxerr_iso", "mag_iso", "magerr_iso", "flux_isocor",
+                "fluxerr_isocor", "flux_aper", "fluxerr_aper", "mag_aper", "magerr_aper",
+                "kron_radius", "flux_auto", "fluxerr_auto", "mag_auto", "magerr_auto",
+                "elongation", "ellipticity", "class_star"]
+
+
+def set_photometry(apert=5.0, autoparam=(2.5, 3.5), autoaper=(0.0, 0.0), seeing_fwhm=1.2,
+                   mag_zeropoint=0.0, mask_correct=1, photo=0, do_aper=1, do_auto=1,
+                   do_class=1):
+    """Photometry settings of endobject_gpu (defaults: the reference's default.sex)."""
+    pp = PhotomParams(apert, (C.c_double * 2)(*autoparam), (C.c_double * 2)(*autoaper),
+                      seeing_fwhm, mag_zeropoint, mask_correct, photo, do_aper, do_auto,
+                      do_class)
+    if gpu().sxg_set_photometry(C.byref(pp)) != 0:
+        check()
+
+
+def get_photometry(name, n):
+    out = np.zeros(max(n, 1), np.float32)
+    if gpu().sxg_get_photometry(PHOT_COLUMNS.index(name), out.ctypes.data, out.size) != 0:
+        check()
+    return out[:n]
+
+
 def gpu():
     """Load lib/libsexgpu.so (raises if it is missing: there is no CPU fallback)."""
     global _gpu
@@ -252,6 +333,8 @@ def gpu():
         L.sxg_batch_segmap.argtypes = [i, vp]
         L.sxg_batch_release.argtypes = []
         L.sxg_set_weight.argtypes = [vp, i, i, i, d, d]
+        L.sxg_set_photometry.argtypes = [C.POINTER(PhotomParams)]
+        L.sxg_get_photometry.argtypes = [i, vp, sz]
         L.sxg_set_option.argtypes = [i, C.c_long]
         L.sxg_get_option.argtypes = [i]
         L.sxg_get_option.restype = C.c_long
@@ -274,10 +357,12 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_set_resident_frame", "sxg_process_resident", "sxg_sync", "sxg_kernel_timing",
             "sxg_kernel_times", "sxg_set_option", "sxg_get_option", "sxg_ctx_create",
             "sxg_ctx_destroy", "sxg_ctx_bind", "sxg_batch_set_frames", "sxg_batch_process",
-            "sxg_batch_segmap", "sxg_batch_release", "sxg_set_weight"]
+            "sxg_batch_segmap", "sxg_batch_release", "sxg_set_weight",
+            "sxg_set_photometry", "sxg_get_photometry"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
-           "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort"]
+           "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort",
+           "phot"]
 
 # options (sxg_option_id): the size caps choose kernel variants and never change a result;
 # "precision" chooses the arithmetic of the graded catalogue columns
@@ -432,9 +517,12 @@ def set_weight(w, p):
         check()
 
 
-def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"), weight=None):
+def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"), weight=None,
+              brain=None, pixscale=1.0, ngamma=0.0):
     """Run the whole path in the order of the reference's scan.c:78-146 and collect the
-    requested outputs (numpy arrays).  weight: optional MAP_WEIGHT map."""
+    requested outputs (numpy arrays).  weight: optional MAP_WEIGHT map; brain: the
+    CLASS_STAR network (read_nnw) for the catalogue; "phot" in outputs: the photometry
+    columns of endobject_gpu."""
     L = gpu()
     img = np.ascontiguousarray(img, dtype=np.float32)
     H, W = img.shape
@@ -478,12 +566,16 @@ def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"
         L.sxg_get_alloc_map(al.ctypes.data)
         check()
         out["alloc"] = al
-    if "catalog" in outputs:
+    if "catalog" in outputs or "phot" in outputs:
         rows = (OutObj * max(nobj, 1))()
-        L.endobject_gpu(None, None, None, C.cast(rows, C.c_void_p), None, 0, H, W, H, bsig,
-                        thresh, p.satur_level, 1.0, 0.0, p.gain, nobj)
+        L.endobject_gpu(None, C.byref(brain) if brain is not None else None, None,
+                        C.cast(rows, C.c_void_p), None, 0, H, W, H, bsig, thresh,
+                        p.satur_level, pixscale, ngamma, p.gain, nobj)
         check()
         out["catalog"] = rows
+        if "phot" in outputs:
+            out["phot"] = {k: get_photometry(k, nobj) for k in PHOT_COLUMNS}
+            out["flag_phot"] = get_column("flag", nobj)
     if "cd" in outputs:
         out["cd"] = filtered((H, W))
     info = frame_info()

@@ -73,10 +73,13 @@ void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int flagobj_iso_0
    return; the definition is void, as here.) */
 void run_clean(int *masterIndex, double clean_param, int clean_stacksize, int numobj);
 
-/* src/cuda/cudaanalyse.cu:3344 -- fill the catalogue rows (outobjstruct layout of
-   src/types.h:417-455, mirrored by sxg_outobj below).  The prefs/brain/obj2/wcs
-   pointers are accepted for signature compatibility and not dereferenced; photometry
-   beyond the isophotal columns is left zero. */
+/* src/cuda/cudaanalyse.cu:3344 -- catalogue photometry (ISOCOR, APER, AUTO, CLASS_STAR,
+   magnitudes, ELONGATION/ELLIPTICITY, the APERT_PB/CROWDED/SATUR flags) and one row per
+   object (outobjstruct layout of src/types.h:417-455, mirrored by sxg_outobj below; objlist
+   may be NULL to keep the results on the device).  brain is read as sxg_brain (NULL: no
+   CLASS_STAR); the photometry settings come from sxg_set_photometry instead of prefs;
+   flagobj2 and wcs are not read (every column is computed; the reference computes no
+   world coordinates).  Whole frames only: fymin 0, fymax height. */
 void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *objlist, void *wcs,
                    int fymin, int fymax, int width, int height, float backsig, float thresh,
                    double satur_level, double pixscale, double ngamma, double gain,
@@ -98,6 +101,49 @@ void sxg_set_back_value(float value);
 
 /* MAG_ZEROPOINT used by endobject_gpu (the reference reads prefs.mag_zeropoint). */
 void sxg_set_mag_zeropoint(double zp);
+
+/* The star/galaxy network, in the layout of the reference's brainstruct
+   (src/types.h:930-943, LAYERS 3, NEURONS 10): endobject_gpu reads its brain argument as
+   this structure, so the reference front-end's own brain pointer can be passed.  ni, no
+   and n are scratch in the reference and are not read. */
+typedef struct __attribute__((aligned(16))) {
+    int layersnb;
+    int nn[3];
+    double inbias[10], inscale[10], outbias[10], outscale[10];
+    double ni[10], no[10];
+    double n[3][10];
+    double w[2][10][10];
+    double b[2][10];
+} sxg_brain;
+
+/* Photometry settings of endobject_gpu.  The reference reads them from its prefs
+   structure (src/prefs.h): PHOT_APERTURES (one aperture, NAPER 1), PHOT_AUTOPARAMS,
+   PHOT_AUTOAPERS, SEEING_FWHM, MAG_ZEROPOINT, MASK_TYPE, DETECT_TYPE; the do_* fields
+   stand for the catalogue parameters that request them (the obj2 flags).  Defaults:
+   the reference's default.sex (5, {2.5, 3.5}, {0, 0}, 1.2, 0, CORRECT, CCD, all on). */
+typedef struct {
+    double apert;           /* aperture diameter in pixels */
+    double autoparam[2];    /* Kron factor, minimum radius */
+    double autoaper[2];     /* minimum diameters: estimation, measurement */
+    double seeing_fwhm;     /* arcsec */
+    double mag_zeropoint;
+    int mask_correct;       /* 1: MASK_TYPE CORRECT (mirror), 0: other objects' pixels = 0 */
+    int photo;              /* 1: DETECT_TYPE PHOTO */
+    int do_aper, do_auto, do_class;
+} sxg_photom_params;
+/* Returns 0, or -1 for invalid settings. */
+int sxg_set_photometry(const sxg_photom_params *p);
+
+/* Photometry columns of the last endobject_gpu call (float, object order). */
+enum sxg_phot_column {
+    SXG_PHOT_FLUX_ISO = 0, SXG_PHOT_FLUXERR_ISO, SXG_PHOT_MAG_ISO, SXG_PHOT_MAGERR_ISO,
+    SXG_PHOT_FLUX_ISOCOR, SXG_PHOT_FLUXERR_ISOCOR, SXG_PHOT_FLUX_APER, SXG_PHOT_FLUXERR_APER,
+    SXG_PHOT_MAG_APER, SXG_PHOT_MAGERR_APER, SXG_PHOT_KRON_RADIUS, SXG_PHOT_FLUX_AUTO,
+    SXG_PHOT_FLUXERR_AUTO, SXG_PHOT_MAG_AUTO, SXG_PHOT_MAGERR_AUTO, SXG_PHOT_ELONGATION,
+    SXG_PHOT_ELLIPTICITY, SXG_PHOT_CLASS_STAR, SXG_NPHOT
+};
+/* out[count >= numobj]; returns 0 or < 0 on error. */
+int sxg_get_photometry(int column, float *out, size_t count);
 
 /* Segmentation check-image: out[width*height] = catalogue NUMBER (1-based, survivors of
    CLEAN in object order) on each object's final pixels, 0 elsewhere.  Call after
@@ -232,7 +278,7 @@ enum sxg_kernel_id {
     SXG_KT_SUBBACK, SXG_KT_CONV, SXG_KT_DET_INIT, SXG_KT_DET_FG, SXG_KT_DET_UNION,
     SXG_KT_DET_FLATTEN, SXG_KT_DET_STAT, SXG_KT_DET_SORT, SXG_KT_DEB_THRESH,
     SXG_KT_DEB_LEVELS, SXG_KT_DEB_OBJECTS, SXG_KT_ANALYSE, SXG_KT_CLEAN, SXG_KT_SEG,
-    SXG_KT_DEB_LMAX, SXG_KT_OBJ_SORT, SXG_NKT
+    SXG_KT_DEB_LMAX, SXG_KT_OBJ_SORT, SXG_KT_PHOT, SXG_NKT
 };
 void sxg_kernel_timing(int enable);
 int sxg_kernel_times(int id, double *total_ms, int *launches);

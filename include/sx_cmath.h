@@ -9,7 +9,9 @@
  *   - the CLEAN amplitude exponent pow(amp/dthresh, 1/CLEAN_PARAM)
  *     (src/cuda/cudaclean.cu:26-27),
  *   - the isophotal area thresholds thresh * pow(peak/thresh, i/8)
- *     (src/cuda/cudaanalyse.cu:2473-2478).
+ *     (src/cuda/cudaanalyse.cu:2473-2478);
+ * and the catalogue photometry uses them for its magnitudes (log10), the PHOTO-mode
+ * exponentials and the star/galaxy network (cudaanalyse.cu:599-700, :2903-2921).
  * libm and the GPU math library round these differently in the last bit, so both the
  * CPU oracle and the HIP kernels use the functions below instead.  They are built only
  * from IEEE-754 +, -, *, / and exact bit manipulation, so any compiler that honours
@@ -103,6 +105,12 @@ SX_HD double sx_exp(double x)
     double s2 = sx_u2d((uint64_t)(int64_t)(k2 + 1023) << 52);
     return p * s1 * s2;
 }
+
+/* log10(x) = log(x) * (1/ln 10), the constant rounded to double; and its float form
+   (the value rounded to float), for the magnitudes and the classifier inputs */
+#define SX_INV_LN10 0.43429448190325182765
+SX_HD double sx_log10(double x) { return sx_log(x) * SX_INV_LN10; }
+SX_HD float sx_log10f(float x) { return (float)sx_log10((double)x); }
 
 SX_HD double sx_pow(double x, double y)
 {

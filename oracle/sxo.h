@@ -113,10 +113,52 @@ void sxo_free(sxo_result *r);
    of the reference applied.  Returns the number of objects. */
 int sxo_measure_fp64(const sxo_result *r, double *out, int ncol);
 
+/* ---------------- catalogue photometry (endobject_gpu) ---------------- */
+/* the classifier network, layout of the reference's brainstruct (src/types.h:930-943,
+   LAYERS 3, NEURONS 10) without its scratch arrays */
+typedef struct {
+    int layersnb;
+    int nn[3];
+    double inbias[10], inscale[10], outbias[10], outscale[10];
+    double w[2][10][10];
+    double b[2][10];
+} sxo_brain;
+
+typedef struct {
+    double apert;            /* PHOT_APERTURES: the one aperture's diameter (NAPER 1) */
+    double autoparam[2];     /* PHOT_AUTOPARAMS: Kron factor, minimum radius */
+    double autoaper[2];      /* PHOT_AUTOAPERS: estimation, measurement minimum diameters */
+    double seeing_fwhm;      /* SEEING_FWHM */
+    double pixscale;         /* field pixel scale */
+    double mag_zeropoint;    /* MAG_ZEROPOINT */
+    double ngamma, gain, satur_level;
+    float backsig, thresh;   /* field backsig and analysis threshold */
+    int mask_correct;        /* MASK_TYPE CORRECT */
+    int photo;               /* DETECT_TYPE PHOTO */
+    int do_aper, do_auto, do_class;
+    const sxo_brain *brain;
+} sxo_photom_config;
+
+typedef struct {             /* caller-allocated arrays of nobj entries */
+    float *flux_iso, *fluxerr_iso, *mag_iso, *magerr_iso;
+    float *flux_isocor, *fluxerr_isocor;
+    float *flux_aper, *fluxerr_aper, *mag_aper, *magerr_aper;
+    float *kronfactor, *flux_auto, *fluxerr_auto, *mag_auto, *magerr_auto;
+    float *elong, *ellip, *sprob;
+    int *flag;
+} sxo_photom;
+
+/* per-pixel photometry labels (object index, -1, or INT_MAX), N entries */
+void sxo_photom_labels(const sxo_result *r, int *lab);
+double sxo_neurresp(const sxo_brain *br, const double *input);
+/* fills out for every object of r; returns nobj or < 0 */
+int sxo_endobject(const sxo_result *r, const sxo_photom_config *pc, sxo_photom *out);
+
 /* canonical math (shared definition with the HIP kernels) */
 double sxo_log(double x);
 double sxo_exp(double x);
 double sxo_pow(double x, double y);
+double sxo_log10(double x);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,8 @@ def lib():
         _lib.sxo_log.argtypes = [C.c_double]
         _lib.sxo_exp.restype = C.c_double
         _lib.sxo_exp.argtypes = [C.c_double]
+        _lib.sxo_log10.restype = C.c_double
+        _lib.sxo_log10.argtypes = [C.c_double]
         _lib.sxo_pow.restype = C.c_double
         _lib.sxo_pow.argtypes = [C.c_double, C.c_double]
         _lib.sxo_hmedian.restype = C.c_float
@@ -49,6 +51,11 @@ def lib():
         _lib.sxo_backstat_mesh.argtypes = [vp, i, i, i, i, i, vp]
         _lib.sxo_backhisto_mesh.argtypes = [vp, i, i, i, i, i, vp, vp]
         _lib.sxo_backguess.argtypes = [vp, vp, vp, vp]
+        _lib.sxo_endobject.argtypes = [C.POINTER(SxoResult), C.POINTER(SxoPhotomConfig),
+                                       C.POINTER(SxoPhotom)]
+        _lib.sxo_photom_labels.argtypes = [C.POINTER(SxoResult), vp]
+        _lib.sxo_neurresp.restype = C.c_double
+        _lib.sxo_neurresp.argtypes = [C.POINTER(SxoBrain), vp]
     return _lib
 
 
@@ -113,6 +120,119 @@ class SxoResult(C.Structure):
     ]
 
 
+class SxoBrain(C.Structure):
+    _fields_ = [("layersnb", C.c_int), ("nn", C.c_int * 3),
+                ("inbias", C.c_double * 10), ("inscale", C.c_double * 10),
+                ("outbias", C.c_double * 10), ("outscale", C.c_double * 10),
+                ("w", C.c_double * 10 * 10 * 2), ("b", C.c_double * 10 * 2)]
+
+
+class SxoPhotomConfig(C.Structure):
+    _fields_ = [("apert", C.c_double), ("autoparam", C.c_double * 2),
+                ("autoaper", C.c_double * 2), ("seeing_fwhm", C.c_double),
+                ("pixscale", C.c_double), ("mag_zeropoint", C.c_double),
+                ("ngamma", C.c_double), ("gain", C.c_double), ("satur_level", C.c_double),
+                ("backsig", C.c_float), ("thresh", C.c_float), ("mask_correct", C.c_int),
+                ("photo", C.c_int), ("do_aper", C.c_int), ("do_auto", C.c_int),
+                ("do_class", C.c_int), ("brain", C.POINTER(SxoBrain))]
+
+
+PHOT_FLT = ["flux_iso", "fluxerr_iso", "mag_iso", "magerr_iso", "flux_isocor",
+            "fluxerr_isocor", "flux_aper", "fluxerr_aper", "mag_aper", "magerr_aper",
+            "kronfactor", "flux_auto", "fluxerr_auto", "mag_auto", "magerr_auto", "elong",
+            "ellip", "sprob"]
+
+
+class SxoPhotom(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in PHOT_FLT] + [("flag", C.c_void_p)]
+
+
+def read_nnw(path):
+    """The classifier weights of a NNW table, parsed as getnnw does (src/neurro.c:96-180):
+    the layer sizes, the input biases and scales, per layer and neuron its input weights
+    followed by its bias, then the output biases and scales."""
+    lines = []
+    with open(path) as f:
+        head = f.readline()
+        if not head.startswith("NNW"):
+            raise ValueError(path + " is not a NNW table")
+        for ln in f:
+            t = ln.lstrip(" \t")
+            if t and t[0] not in "#\n":
+                lines.append([float(v) for v in t.split()])
+    nn = [int(v) for v in lines[0][1:1 + int(lines[0][0])]]
+    L = len(nn)
+    net = {"layersnb": L, "nn": nn, "inbias": lines[1][:nn[0]], "inscale": lines[2][:nn[0]]}
+    w = np.zeros((2, 10, 10))
+    b = np.zeros((2, 10))
+    k = 3
+    for layer in range(L - 1):
+        for j in range(nn[layer + 1]):
+            row = lines[k]
+            w[layer, :nn[layer], j] = row[:nn[layer]]
+            b[layer, j] = row[nn[layer]]
+            k += 1
+    net["w"], net["b"] = w, b
+    net["outbias"] = lines[k][:nn[-1]]
+    net["outscale"] = lines[k + 1][:nn[-1]]
+    return net
+
+
+def brain_struct(net):
+    br = SxoBrain()
+    br.layersnb = net["layersnb"]
+    for i, v in enumerate(net["nn"]):
+        br.nn[i] = v
+    for name in ("inbias", "inscale", "outbias", "outscale"):
+        arr = getattr(br, name)
+        for i, v in enumerate(net[name]):
+            arr[i] = v
+    for l in range(2):
+        for i in range(10):
+            for j in range(10):
+                br.w[l][i][j] = float(net["w"][l, i, j])
+            br.b[l][i] = float(net["b"][l, i])
+    return br
+
+
+PHOTOM_DEFAULTS = {"apert": 5.0, "autoparam": (2.5, 3.5), "autoaper": (0.0, 0.0),
+                   "seeing_fwhm": 1.2, "pixscale": 1.0, "mag_zeropoint": 0.0, "ngamma": 0.0,
+                   "mask_correct": 1, "photo": 0, "do_aper": 1, "do_auto": 1, "do_class": 1,
+                   "nnw": None}
+
+
+def _photom(rp, r, kw, photom):
+    pk = dict(PHOTOM_DEFAULTS)
+    pk.update(photom)
+    pc = SxoPhotomConfig()
+    pc.apert = pk["apert"]
+    pc.autoparam[0], pc.autoparam[1] = pk["autoparam"]
+    pc.autoaper[0], pc.autoaper[1] = pk["autoaper"]
+    for k in ("seeing_fwhm", "pixscale", "mag_zeropoint", "ngamma", "mask_correct", "photo",
+              "do_aper", "do_auto", "do_class"):
+        setattr(pc, k, pk[k])
+    pc.gain = kw.get("gain", 0.0)
+    pc.satur_level = kw.get("satur_level", 50000.0)
+    pc.backsig, pc.thresh = r.backsig, r.thresh
+    br = None
+    if pk["nnw"] is not None:
+        br = brain_struct(pk["nnw"])
+        pc.brain = C.pointer(br)
+    m = r.nobj
+    arrs = {k: np.zeros(max(m, 1), np.float32) for k in PHOT_FLT}
+    arrs["flag"] = np.zeros(max(m, 1), np.int32)
+    ph = SxoPhotom()
+    for k, a in arrs.items():
+        setattr(ph, k, a.ctypes.data)
+    if lib().sxo_endobject(rp, C.byref(pc), C.byref(ph)) != m:
+        raise RuntimeError("sxo_endobject failed")
+    out = {k: a[:m] for k, a in arrs.items()}
+    lab = np.zeros(r.width * r.height, np.int32)
+    lib().sxo_photom_labels(rp, lab.ctypes.data)
+    out["labels"] = lab.reshape(r.height, r.width)
+    return out
+
+
 OBJ_INT = ["level", "root", "firstpix", "flag", "fdnpix", "dnpix", "npix", "xmin", "xmax",
            "ymin", "ymax", "peakx", "peaky"]
 OBJ_DBL = ["mx", "my", "mx2", "my2", "mxy", "poserr_mx2", "poserr_my2", "poserr_mxy"]
@@ -143,10 +263,12 @@ def config(**kw):
 TRUTH_COLUMNS = ["flux", "mx", "my", "mx2", "my2", "mxy", "a", "b", "theta", "cxx", "cyy", "cxy"]
 
 
-def run(img, mask=None, weight=None, truth=False, **kw):
+def run(img, mask=None, weight=None, truth=False, photom=None, **kw):
     """Run the whole oracle path on a float32 frame (and an optional MAP_WEIGHT map);
     returns a dict of numpy arrays.  truth: also the long-double evaluation of the graded
-    columns on the same pixel sets (out["truth"][:, TRUTH_COLUMNS.index(name)])."""
+    columns on the same pixel sets (out["truth"][:, TRUTH_COLUMNS.index(name)]).
+    photom: a dict of photometry settings (keys of PHOTOM_DEFAULTS; nnw = read_nnw(...)):
+    also the catalogue photometry of endobject_gpu, as out["phot"]."""
     img = np.ascontiguousarray(img, dtype=np.float32)
     H, W = img.shape
     cfg = config(**kw)
@@ -195,6 +317,8 @@ def run(img, mask=None, weight=None, truth=False, **kw):
         if lib().sxo_measure_fp64(rp, t.ctypes.data, 12) != m:
             raise RuntimeError("sxo_measure_fp64 failed")
         out["truth"] = t[:m]
+    if photom is not None:
+        out["phot"] = _photom(rp, r, kw, photom)
     lib().sxo_free(rp)
     return out
 

@@ -35,8 +35,14 @@ log-uniform in [1e-4, 1e-2] per pixel (the 1024^2 generator settings, default.co
 seeds 2000+i); it reports aggregate Mpix/s and per-frame p50/p99 latency for the batched
 path against frame-by-frame runs, for each worker count of --sweep.
 
+--workload photom runs the crowded frame through the whole path and the catalogue
+photometry of endobject_gpu (ISOCOR, APER, AUTO/Kron, CLASS_STAR, magnitudes) with the
+reference's default.sex settings; the photometry columns are compared with the oracle's
+at full size.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--workers W]
-                       [--no-cpu-baseline] [--workload crowded|batch64] [--sweep 1,2,4,8]
+                       [--no-cpu-baseline] [--sweep 1,2,4,8]
+                       [--workload crowded|batch64|wide8192|hdr4096|photom]
 """
 import argparse
 import ctypes as C
@@ -280,6 +286,101 @@ def wide8192(args):
     return 0
 
 
+PHOT_ORACLE = {"flux_isocor": "flux_isocor", "flux_aper": "flux_aper",
+               "fluxerr_aper": "fluxerr_aper", "mag_aper": "mag_aper",
+               "kron_radius": "kronfactor", "flux_auto": "flux_auto",
+               "fluxerr_auto": "fluxerr_auto", "mag_auto": "mag_auto",
+               "class_star": "sprob", "mag_iso": "mag_iso"}
+
+
+def photom(args):
+    """The crowded frame through the whole path plus the catalogue photometry of
+    endobject_gpu (ISOCOR, APER, AUTO/Kron, CLASS_STAR, magnitudes), default.sex settings."""
+    img = sexgpu.synth("crowded4096")
+    H, W = img.shape
+    N = W * H
+    p = workload_params()
+    sp = sexgpu.native_params(p)
+    L = sexgpu.gpu()
+    L.sxg_set_resident_frame(img.ctypes.data, W, H)
+    sexgpu.check()
+    br = sexgpu.read_nnw("default.nnw")
+    sexgpu.set_photometry()
+    fi = sexgpu.FrameInfo()
+
+    def step():
+        L.sxg_process_resident(C.byref(sp), C.byref(fi))
+        L.endobject_gpu(None, C.byref(br), None, None, None, 0, H, W, H, fi.backsig, fi.thresh,
+                        p.satur_level, 1.0, 0.0, p.gain, fi.nobj)
+
+    for _ in range(args.warmup):
+        step()
+        sexgpu.check()
+    L.sxg_sync()
+    L.sxg_kernel_timing(1)
+    steps = args.steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    L.sxg_sync()
+    dt = time.perf_counter() - t0
+    sexgpu.check()
+    ktab = kernel_table(L)
+    L.sxg_kernel_timing(0)
+    info = {k: getattr(fi, k) for k, _ in sexgpu.FrameInfo._fields_}
+    L.sxg_get_frame_info(C.byref(fi))
+    apix = fi.phot_pixels
+    ptot, pn = ktab["phot"]
+    phot_s = ptot / pn / 1000.0
+    print("# %.3f ms per frame with photometry; photometry kernels %.3f ms over %d objects, "
+          "%d aperture pixels" % (1000 * dt / steps, phot_s * 1e3, info["nobj"], apix),
+          file=sys.stderr, flush=True)
+    nobj = info["nobj"]
+    cols = {k: sexgpu.get_photometry(k, nobj) for k in sexgpu.PHOT_COLUMNS}
+    seg = np.zeros((H, W), np.int32)
+    L.sxg_get_segmap(seg.ctypes.data)
+    sexgpu.check()
+    cpu = parity = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import sxo  # the oracle: test infrastructure, used here only as the CPU baseline
+        t = time.perf_counter()
+        o = sxo.run(img, mask=p.mask, photom={"nnw": sxo.read_nnw(os.path.join(
+            REPO, "gpu-sextractor_amd", "config", "default.nnw"))})
+        dtc = time.perf_counter() - t
+        parity = bool(o["nobj"] == nobj and np.array_equal(o["seg"], seg) and all(
+            np.array_equal(cols[g], o["phot"][k]) for g, k in PHOT_ORACLE.items()))
+        cpu = {"value": round(N / dtc / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+               "sample": "one full 4096x4096 frame with photometry (%.1f s, single thread)"
+                         % dtc}
+    # photometry B_alg: write and read the label map once (8 B/pixel), and per aperture
+    # pixel read the value and the label (8 B)
+    balg = 8 * N + 8 * apix
+    achieved = balg / phot_s / 1e9
+    line = {
+        "metric": "Mpix/s + objects/s at 4096^2 (crowded), whole path + catalogue photometry "
+                  "(ISOCOR/APER/AUTO/CLASS_STAR); bit-exact columns",
+        "value": round(N * steps / dt / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(1000.0 * dt / steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "crowded 4096x4096, 167000 sources (seed 1001), gauss_3.0_7x7, "
+                               "PHOT_APERTURES 5, PHOT_AUTOPARAMS 2.5,3.5, MASK_TYPE CORRECT, "
+                               "default.nnw",
+                   "objects_per_s": round(nobj * steps / dt, 1), "objects": nobj,
+                   "photometry_ms": round(phot_s * 1e3, 4),
+                   "photometry_objects_per_s": round(nobj / phot_s, 1),
+                   "aperture_pixels": apix, "parity_with_oracle": parity},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "phot", "kernel_ms": round(phot_s * 1e3, 4),
+                     "algorithmic_bytes": balg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 PRECISION_COLUMNS = ["flux", "mx", "my", "mx2", "my2", "mxy", "a", "b", "theta", "cxx", "cyy",
                      "cxy"]
 
@@ -390,7 +491,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--size", type=int, default=4096, help="frame side (default 4096)")
     ap.add_argument("--workload", default="crowded",
-                    choices=["crowded", "batch64", "wide8192", "hdr4096"])
+                    choices=["crowded", "batch64", "wide8192", "hdr4096", "photom"])
     ap.add_argument("--sweep", default="", help="worker counts to time, e.g. 1,2,4,8")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -403,6 +504,8 @@ def main():
         return wide8192(args)
     if args.workload == "hdr4096":
         return hdr4096(args)
+    if args.workload == "photom":
+        return photom(args)
 
     t = time.perf_counter()
     nsrc = int(round(167000 * (args.size / 4096.0) ** 2))

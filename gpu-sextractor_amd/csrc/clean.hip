@@ -305,6 +305,7 @@ extern "C" int sxg_get_frame_info(sxg_frame_info *info)
     info->nby = S.nby;
     info->deb_pixels = S.deb_pixels;
     info->deb_labels = S.deb_labels;
+    info->phot_pixels = S.phot_pixels;
     return 0;
 }
 

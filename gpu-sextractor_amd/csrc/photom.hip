@@ -58,6 +58,7 @@ struct PhotArgs {
         gain, satur;
     int corr, photo, do_aper, do_auto, do_class;
     const sxg_brain *brain;    /* device copy, or nullptr */
+    int *npix;                 /* per object: aperture pixels summed */
 };
 
 /* ------------------------------------------------------------------ labels ---- */
@@ -152,6 +153,7 @@ __global__ void __launch_bounds__(256) k_endobject(PhotArgs A)
     const float a = o.a[obj], b = o.b[obj];
     int flag = o.flag[obj];
     double var = A.backsig * A.backsig;   /* float product, as the reference forms it */
+    int visited = 0;                      /* aperture pixels summed (wave-uniform) */
 
     /* ---- APER (cudaanalyse.cu:104-287) */
     float flux_aper = 0.0f, fluxerr_aper = 0.0f, mag_aper = 0.0f, magerr_aper = 0.0f;
@@ -212,6 +214,7 @@ __global__ void __launch_bounds__(256) k_endobject(PhotArgs A)
                     t[2] = sig;
                 }
             }
+            visited += __popcll(__ballot(member));
             sxw_fold<3, double>(rows, t, member, acc);
         }
         const double area = sxw_rl(acc, 0);
@@ -270,6 +273,7 @@ __global__ void __launch_bounds__(256) k_endobject(PhotArgs A)
             const unsigned long long bi = __ballot(inside), bg = __ballot(good);
             area += __popcll(bg);
             areab += __popcll(bi & ~bg);
+            visited += __popcll(bi);
             sxw_fold<2, double>(rows, t, good, acc);
         }
         const double r1 = sxw_rl(acc, 0), v1 = sxw_rl(acc, 1);
@@ -324,6 +328,7 @@ __global__ void __launch_bounds__(256) k_endobject(PhotArgs A)
                 const unsigned long long bi = __ballot(inside), bm = __ballot(inside && masked);
                 area += __popcll(bi);
                 areab += __popcll(bm);
+                visited += __popcll(bi);
                 sxw_fold<2, double>(rows, t, inside, acc);
             }
             tv = sxw_rl(acc, 0);
@@ -416,6 +421,23 @@ __global__ void __launch_bounds__(256) k_endobject(PhotArgs A)
         p.mag_iso[obj] = fl > 0.0 ? (float)(-2.5 * sx_log10f(fl) + A.zp) : 99.0f;
         p.magerr_iso[obj] = fl > 0.0 ? (float)(1.086 * fei / fl) : 99.0f;
         o.flag[obj] = flag;
+        A.npix[obj] = visited;
+    }
+}
+
+/* the total of the per-object aperture pixel counts (one workgroup) */
+__global__ void __launch_bounds__(1024) k_phot_sum(const int *v, int n, unsigned long long *out)
+{
+    __shared__ unsigned long long part[16];
+    unsigned long long s = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += (unsigned)v[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 16; w++) t += part[w];
+        *out = t;
     }
 }
 
@@ -526,7 +548,8 @@ static bool run_photometry(const sxg_brain *brain, float backsig, float thresh,
     }
     int *lab = sxg_take<int>("phot.lab", S.N);
     int *order = sxg_take<int>("phot.order", (size_t)n + 2);
-    if (!lab || !order) return false;
+    A.npix = sxg_take<int>("phot.npix", (size_t)n);
+    if (!lab || !order || !A.npix) return false;
     A.brain = nullptr;
     if (brain && pp.do_class) {
         if (!brain_ok(brain)) {
@@ -595,6 +618,7 @@ extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *ob
         return;
     }
     S.phot_ready = false;
+    S.phot_pixels = 0;
     if (numobj == 0) {
         S.phot_ready = true;
         return;
@@ -602,6 +626,12 @@ extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *ob
     if (!run_photometry(static_cast<const sxg_brain *>(brain), backsig, thresh, satur_level,
                         pixscale, ngamma, gain))
         return;
+    unsigned long long visited = 0;
+    unsigned long long *vsum = sxg_take<unsigned long long>("phot.npix_sum", 1);
+    if (!vsum) return;
+    hipLaunchKernelGGL(k_phot_sum, dim3(1), dim3(1024), 0, S.stream,
+                       sxg_take<int>("phot.npix", (size_t)numobj), numobj, vsum);
+    SXG_HIP(hipMemcpyAsync(&visited, vsum, sizeof(visited), hipMemcpyDeviceToHost, S.stream));
     if (objlist) {
         PhotArgs A;
         A.nobj = numobj;
@@ -615,6 +645,7 @@ extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *ob
     }
     SXG_HIP(hipGetLastError());
     SXG_HIP(hipStreamSynchronize(S.stream));
+    S.phot_pixels = (int64_t)visited;
     S.phot_ready = true;
 }
 

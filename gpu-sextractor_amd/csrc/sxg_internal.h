@@ -110,6 +110,7 @@ struct SxgState {
     int ext_minarea = 5;
     int64_t npix_above = 0, npix_raw = 0;
     int64_t deb_pixels = 0, deb_labels = 0;   /* level-by-level deblending work */
+    int64_t phot_pixels = 0;                  /* aperture pixels of the photometry */
     int nraw = 0;
     bool filtered = false;  /* cudaFilter ran on this frame */
     int *fpia = nullptr;      /* int[P0]: pixels grouped by raw object, raster order inside */

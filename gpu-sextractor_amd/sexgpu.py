@@ -199,7 +199,7 @@ class FrameInfo(C.Structure):
                 ("thresh", C.c_float), ("npix_above", C.c_int64), ("npix_raw", C.c_int64),
                 ("nraw", C.c_int32), ("nobj", C.c_int32), ("nsurv", C.c_int32),
                 ("nbx", C.c_int32), ("nby", C.c_int32), ("deb_pixels", C.c_int64),
-                ("deb_labels", C.c_int64)]
+                ("deb_labels", C.c_int64), ("phot_pixels", C.c_int64)]
 
 
 class Brain(C.Structure):

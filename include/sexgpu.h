@@ -183,6 +183,9 @@ typedef struct {
        ones that get a component tree), and the sum over them of pixels x L* (the
        pixel-levels a level-by-level construction would visit) */
     int64_t deb_pixels, deb_labels;
+    /* photometry work of the last endobject_gpu call: the aperture pixels summed (inside
+       the APER outer radius, inside both AUTO ellipses) */
+    int64_t phot_pixels;
 } sxg_frame_info;
 int sxg_get_frame_info(sxg_frame_info *info);
 

@@ -286,6 +286,10 @@ def wide8192(args):
     return 0
 
 
+# HBM bytes per frame of the photometry kernels (label map, labels of the raw pixels,
+# order, k_endobject, count), from profiles/photometry_crowded4096_trace_and_traffic.md
+PHOT_TRAFFIC_CROWDED4096 = 3.533e8
+
 PHOT_ORACLE = {"flux_isocor": "flux_isocor", "flux_aper": "flux_aper",
                "fluxerr_aper": "fluxerr_aper", "mag_aper": "mag_aper",
                "kron_radius": "kronfactor", "flux_auto": "flux_auto",
@@ -354,7 +358,8 @@ def photom(args):
                "sample": "one full 4096x4096 frame with photometry (%.1f s, single thread)"
                          % dtc}
     # photometry B_alg: write and read the label map once (8 B/pixel), and per aperture
-    # pixel read the value and the label (8 B)
+    # pixel read the value and the label (8 B); measured traffic of the same kernels
+    # (FETCH_SIZE + WRITE_SIZE, copy-calibrated): PHOT_TRAFFIC_CROWDED4096
     balg = 8 * N + 8 * apix
     achieved = balg / phot_s / 1e9
     line = {
@@ -372,7 +377,8 @@ def photom(args):
                    "photometry_objects_per_s": round(nobj / phot_s, 1),
                    "aperture_pixels": apix, "parity_with_oracle": parity},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": PHOT_TRAFFIC_CROWDED4096,
                      "kernel": "phot", "kernel_ms": round(phot_s * 1e3, 4),
                      "algorithmic_bytes": balg},
         "cpu_baseline": cpu,

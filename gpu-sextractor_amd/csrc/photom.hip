@@ -62,11 +62,25 @@ struct PhotArgs {
 };
 
 /* ------------------------------------------------------------------ labels ---- */
+/* four pixels per lane with 16-byte accesses; the n % 4 tail pixels by the first lanes */
 __global__ void __launch_bounds__(256) k_phot_lab_init(const float *__restrict__ cd, int *lab,
                                                        long n, float dthresh)
 {
-    const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    if (p < n) lab[p] = cd[p] >= dthresh ? INT_MAX : -1;
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    const long n4 = n >> 2;
+    if (q < n4) {
+        const float4 v = reinterpret_cast<const float4 *>(cd)[q];
+        int4 o;
+        o.x = v.x >= dthresh ? INT_MAX : -1;
+        o.y = v.y >= dthresh ? INT_MAX : -1;
+        o.z = v.z >= dthresh ? INT_MAX : -1;
+        o.w = v.w >= dthresh ? INT_MAX : -1;
+        reinterpret_cast<int4 *>(lab)[q] = o;
+    }
+    if (q < (n & 3)) {
+        const long p = (n4 << 2) + q;
+        lab[p] = cd[p] >= dthresh ? INT_MAX : -1;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_phot_lab_scatter(const int *__restrict__ fpia, long P0,
@@ -564,7 +578,7 @@ static bool run_photometry(const sxg_brain *brain, float backsig, float thresh,
     }
     const long N = (long)S.N;
     sxg_kt_begin(SXG_KT_PHOT);
-    hipLaunchKernelGGL(k_phot_lab_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_phot_lab_init, dim3(sxg_div_up((N >> 2) + 4, 256)), dim3(256), 0, st,
                        S.cd.as<float>(), lab, N, S.dthresh);
     if (S.npix_raw > 0)
         hipLaunchKernelGGL(k_phot_lab_scatter, dim3(sxg_div_up((long)S.npix_raw, 256)),

@@ -42,7 +42,10 @@ struct Config {
               {"BACK_VALUE", "0.0"}, {"BACK_SIZE", "64"}, {"BACK_FILTERSIZE", "3"},
               {"BACK_FILTTHRESH", "0.0"}, {"WEIGHT_TYPE", "NONE"}, {"WEIGHT_IMAGE", "weight.fits"},
               {"CHECKIMAGE_TYPE", "NONE"}, {"CHECKIMAGE_NAME", "check.fits"},
-              {"VERBOSE_TYPE", "NORMAL"}};
+              {"VERBOSE_TYPE", "NORMAL"}, {"PHOT_APERTURES", "5"},
+              {"PHOT_AUTOPARAMS", "2.5, 3.5"}, {"PHOT_AUTOAPERS", "0.0,0.0"},
+              {"MASK_TYPE", "CORRECT"}, {"SEEING_FWHM", "1.2"}, {"PIXEL_SCALE", "1.0"},
+              {"STARNNW_NAME", "default.nnw"}};
     }
     std::string get(const std::string &k) const
     {
@@ -56,9 +59,16 @@ struct Config {
         return !v.empty() && (v[0] == 'Y' || v[0] == 'y' || v[0] == 'T' || v[0] == 't');
     }
     /* first element of a comma-separated list */
-    std::string first(const std::string &k) const
+    std::string first(const std::string &k) const { return nth(k, 0); }
+    /* element i of a comma-separated list (the last one when the list is shorter) */
+    std::string nth(const std::string &k, int i) const
     {
         std::string v = get(k);
+        for (; i > 0; i--) {
+            size_t c = v.find(',');
+            if (c == std::string::npos) break;
+            v = v.substr(c + 1);
+        }
         size_t c = v.find(',');
         return c == std::string::npos ? v : v.substr(0, c);
     }
@@ -136,6 +146,18 @@ const Column kColumns[] = {
     {"ISO5", "Isophotal area at level 5", "%8d", "pixel**2"},
     {"ISO6", "Isophotal area at level 6", "%8d", "pixel**2"},
     {"ISO7", "Isophotal area at level 7", "%8d", "pixel**2"},
+    {"FLUX_ISOCOR", "Corrected isophotal flux", "%12.7g", "count"},
+    {"FLUXERR_ISOCOR", "RMS error for corrected isophotal flux", "%12.7g", "count"},
+    {"FLUX_APER", "Flux vector within fixed circular aperture(s)", "%12.7g", "count"},
+    {"FLUXERR_APER", "RMS error vector for aperture flux(es)", "%12.7g", "count"},
+    {"MAG_APER", "Fixed aperture magnitude vector", "%8.4f", "mag"},
+    {"MAGERR_APER", "RMS error vector for fixed aperture mag.", "%8.4f", "mag"},
+    {"FLUX_AUTO", "Flux within a Kron-like elliptical aperture", "%12.7g", "count"},
+    {"FLUXERR_AUTO", "RMS error for AUTO flux", "%12.7g", "count"},
+    {"MAG_AUTO", "Kron-like elliptical aperture magnitude", "%8.4f", "mag"},
+    {"MAGERR_AUTO", "RMS error for AUTO magnitude", "%8.4f", "mag"},
+    {"KRON_RADIUS", "Kron apertures in units of A or B", "%5.2f", ""},
+    {"CLASS_STAR", "S/G classifier output", "%5.2f", ""},
 };
 
 const Column *find_column(const std::string &name)
@@ -175,6 +197,62 @@ bool read_conv(const std::string &path, std::vector<float> &mask, int &mw)
         for (float &v : mask) v = (float)(v / sum);
     }
     mw = width;
+    return true;
+}
+
+/* a NNW weight table as the reference's getnnw reads it (src/neurro.c:96-180): after the
+   "NNW" line and skipping comments and blank lines, the layer count and sizes, the input
+   biases, the input scales, per layer and neuron its input weights then its bias, the
+   output biases and the output scales */
+bool read_nnw(const std::string &path, sxg_brain &br)
+{
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string line;
+    if (!std::getline(f, line) || line.compare(0, 3, "NNW")) return false;
+    std::vector<std::vector<double>> rows;
+    while (std::getline(f, line)) {
+        std::string t = trim(line);
+        if (t.empty() || t[0] == '#') continue;
+        std::istringstream ss(t);
+        std::vector<double> r;
+        double v;
+        while (ss >> v) r.push_back(v);
+        rows.push_back(r);
+    }
+    memset(&br, 0, sizeof(br));
+    if (rows.empty() || rows[0].empty()) return false;
+    br.layersnb = (int)rows[0][0];
+    if (br.layersnb < 2 || br.layersnb > 3 || (int)rows[0].size() < 1 + br.layersnb) return false;
+    for (int l = 0; l < br.layersnb; l++) {
+        br.nn[l] = (int)rows[0][1 + l];
+        if (br.nn[l] < 1 || br.nn[l] > 10) return false;
+    }
+    const int nin = br.nn[0], nout = br.nn[br.layersnb - 1];
+    size_t r = 1;
+    auto row = [&](size_t need) -> const std::vector<double> * {
+        if (r >= rows.size() || rows[r].size() < need) return nullptr;
+        return &rows[r++];
+    };
+    const std::vector<double> *ib = row(nin), *is = row(nin);
+    if (!ib || !is) return false;
+    for (int k = 0; k < nin; k++) {
+        br.inbias[k] = (*ib)[k];
+        br.inscale[k] = (*is)[k];
+    }
+    for (int l = 0; l < br.layersnb - 1; l++)
+        for (int j = 0; j < br.nn[l + 1]; j++) {
+            const std::vector<double> *w = row(br.nn[l] + 1);
+            if (!w) return false;
+            for (int k = 0; k < br.nn[l]; k++) br.w[l][k][j] = (*w)[k];
+            br.b[l][j] = (*w)[br.nn[l]];
+        }
+    const std::vector<double> *ob = row(nout), *os = row(nout);
+    if (!ob || !os) return false;
+    for (int k = 0; k < nout; k++) {
+        br.outbias[k] = (*ob)[k];
+        br.outscale[k] = (*os)[k];
+    }
     return true;
 }
 
@@ -335,9 +413,39 @@ int main(int argc, char **argv)
     run_clean(reinterpret_cast<int *>(master.data()), cfg.num("CLEAN_PARAM"),
               atoi(cfg.get("MEMORY_OBJSTACK").c_str()), nobj);
     if (fail("run_clean")) return 1;
+    /* photometry settings (the reference's prefs) and the columns that request them */
+    auto wants = [&](std::initializer_list<const char *> names) {
+        for (const Column *c : cols)
+            for (const char *n : names)
+                if (!strcmp(c->name, n)) return 1;
+        return 0;
+    };
+    sxg_photom_params pp;
+    pp.apert = atof(cfg.first("PHOT_APERTURES").c_str());
+    pp.autoparam[0] = atof(cfg.nth("PHOT_AUTOPARAMS", 0).c_str());
+    pp.autoparam[1] = atof(cfg.nth("PHOT_AUTOPARAMS", 1).c_str());
+    pp.autoaper[0] = atof(cfg.nth("PHOT_AUTOAPERS", 0).c_str());
+    pp.autoaper[1] = atof(cfg.nth("PHOT_AUTOAPERS", 1).c_str());
+    pp.seeing_fwhm = cfg.num("SEEING_FWHM");
+    pp.mag_zeropoint = cfg.num("MAG_ZEROPOINT");
+    pp.mask_correct = cfg.first("MASK_TYPE") == "CORRECT";
+    pp.photo = cfg.first("DETECT_TYPE") == "PHOTO";
+    pp.do_aper = wants({"FLUX_APER", "FLUXERR_APER", "MAG_APER", "MAGERR_APER"});
+    pp.do_auto = wants({"FLUX_AUTO", "FLUXERR_AUTO", "MAG_AUTO", "MAGERR_AUTO", "KRON_RADIUS"});
+    pp.do_class = wants({"CLASS_STAR"});
+    if (sxg_set_photometry(&pp) != 0) {
+        fail("sxg_set_photometry");
+        return 1;
+    }
+    sxg_brain brain;
+    if (pp.do_class && !read_nnw(cfg.get("STARNNW_NAME"), brain)) {
+        fprintf(stderr, "sex: cannot read the NNW table %s\n", cfg.get("STARNNW_NAME").c_str());
+        return 1;
+    }
+    const double pixscale = cfg.num("PIXEL_SCALE") > 0.0 ? cfg.num("PIXEL_SCALE") : 1.0;
     std::vector<sxg_outobj> rows(nobj > 0 ? nobj : 1);
-    endobject_gpu(nullptr, nullptr, nullptr, rows.data(), nullptr, 0, H, W, H, backsig, thresh,
-                  satur, 1.0, 0.0, gain, nobj);
+    endobject_gpu(nullptr, pp.do_class ? &brain : nullptr, nullptr, rows.data(), nullptr, 0, H,
+                  W, H, backsig, thresh, satur, pixscale, 0.0, gain, nobj);
     if (fail("endobject_gpu")) return 1;
     /* extra columns not in the reference's fixed row */
     std::vector<int> npixc(nobj), xmin(nobj), xmax(nobj), ymin(nobj), ymax(nobj), px(nobj),
@@ -428,6 +536,18 @@ int main(int argc, char **argv)
             else if (n == "FWHM_IMAGE") fprintf(cat, f, r.fwhm);
             else if (n == "ELONGATION") fprintf(cat, f, r.elong);
             else if (n == "ELLIPTICITY") fprintf(cat, f, r.ellip);
+            else if (n == "FLUX_ISOCOR") fprintf(cat, f, r.flux_isocor);
+            else if (n == "FLUXERR_ISOCOR") fprintf(cat, f, r.fluxerr_isocor);
+            else if (n == "FLUX_APER") fprintf(cat, f, r.flux_aper[0]);
+            else if (n == "FLUXERR_APER") fprintf(cat, f, r.fluxerr_aper[0]);
+            else if (n == "MAG_APER") fprintf(cat, f, r.mag_aper[0]);
+            else if (n == "MAGERR_APER") fprintf(cat, f, r.magerr_aper[0]);
+            else if (n == "FLUX_AUTO") fprintf(cat, f, r.flux_auto);
+            else if (n == "FLUXERR_AUTO") fprintf(cat, f, r.fluxerr_auto);
+            else if (n == "MAG_AUTO") fprintf(cat, f, r.mag_auto);
+            else if (n == "MAGERR_AUTO") fprintf(cat, f, r.magerr_auto);
+            else if (n == "KRON_RADIUS") fprintf(cat, f, r.kronfactor);
+            else if (n == "CLASS_STAR") fprintf(cat, f, r.sprob);
             else if (n == "XMIN_IMAGE") fprintf(cat, f, xmin[i] + 1);
             else if (n == "YMIN_IMAGE") fprintf(cat, f, ymin[i] + 1);
             else if (n == "XMAX_IMAGE") fprintf(cat, f, xmax[i] + 1);

@@ -1,0 +1,56 @@
+"""The C++ `sex` front-end (bin/sex) on a FITS frame: it reads the .sex, .param, .conv and
+.nnw files, drives the C-ABI in the reference's scan.c order and writes the ASCII_HEAD
+catalogue and the SEGMENTATION check-image.  Its rows must carry the library's values
+(to the printed precision) for the surviving objects, photometry columns included."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import sexgpu
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(sexgpu.__file__), "bin", "sex")
+CONF = os.path.join(os.path.dirname(sexgpu.__file__), "config")
+PARAMS = ["NUMBER", "X_IMAGE", "Y_IMAGE", "FLUX_ISO", "FLUX_ISOCOR", "FLUX_APER(1)",
+          "MAG_APER(1)", "FLUX_AUTO", "MAG_AUTO", "KRON_RADIUS", "CLASS_STAR", "FLAGS"]
+
+
+def test_sex_binary_catalogue_and_segmentation(tmp_path, frame1024, masks):
+    img_path = str(tmp_path / "frame.fits")
+    sexgpu.write_fits(img_path, frame1024)
+    (tmp_path / "cat.param").write_text("\n".join(PARAMS) + "\n")
+    cfg = {"CATALOG_NAME": str(tmp_path / "out.cat"), "CATALOG_TYPE": "ASCII_HEAD",
+           "PARAMETERS_NAME": str(tmp_path / "cat.param"),
+           "FILTER_NAME": os.path.join(CONF, "default.conv"),
+           "STARNNW_NAME": os.path.join(CONF, "default.nnw"),
+           "CHECKIMAGE_TYPE": "SEGMENTATION", "CHECKIMAGE_NAME": str(tmp_path / "seg.fits"),
+           "VERBOSE_TYPE": "QUIET"}
+    (tmp_path / "run.sex").write_text("".join("%s %s\n" % kv for kv in cfg.items()))
+    r = subprocess.run([BIN, img_path, "-c", str(tmp_path / "run.sex")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = (tmp_path / "out.cat").read_text().splitlines()
+    head = [ln for ln in lines if ln.startswith("#")]
+    assert [h.split()[2] for h in head] == [p.split("(")[0] for p in PARAMS]
+    rows = np.array([[float(v) for v in ln.split()] for ln in lines if not ln.startswith("#")])
+
+    sexgpu.set_photometry()
+    g = sexgpu.run_frame(frame1024, sexgpu.Params(mask=masks["default"]),
+                         outputs=("columns", "phot", "seg"),
+                         brain=sexgpu.read_nnw(os.path.join(CONF, "default.nnw")))
+    keep = g["master"] == 0xFFFFFFFF
+    assert rows.shape[0] == keep.sum() == g["nsurv"]
+    ph = {k: v[keep] for k, v in g["phot"].items()}
+    assert np.array_equal(rows[:, 0], np.arange(1, rows.shape[0] + 1))
+    assert np.allclose(rows[:, 1], (g["mx"][keep] + 1.0), atol=6e-4)
+    for col, key in ((3, "flux_iso"), (4, "flux_isocor"), (5, "flux_aper"), (7, "flux_auto")):
+        assert np.allclose(rows[:, col], ph[key], rtol=1e-6, atol=1e-3), key
+    for col, key, tol in ((6, "mag_aper", 6e-5), (8, "mag_auto", 6e-5), (9, "kron_radius", 6e-3),
+                          (10, "class_star", 6e-3)):
+        assert np.allclose(rows[:, col], ph[key], atol=tol), key
+    assert np.array_equal(rows[:, 11].astype(int), g["flag_phot"][keep])
+    seg = sexgpu.read_fits(str(tmp_path / "seg.fits"))
+    assert np.array_equal(seg.astype(np.int64), g["seg"].astype(np.int64))

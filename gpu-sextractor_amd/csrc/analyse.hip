@@ -30,6 +30,7 @@ struct AnaArgs {
     float backsig, thresh, dthresh;
     double gain, satur_level;
     int errflag, peakflag, isoflag, fwhmflag, cleanflag;
+    int gainflag;           /* WEIGHT_GAIN with a variance map (cudaanalyse.cu:2281) */
     ObjCols o;
     WeightArgs wa;          /* MAP_WEIGHT: per-pixel variance and thresholds */
     const float *rawt0;     /* detection threshold of each raw object */
@@ -209,7 +210,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
             cdpeak = cdpix;
             peakp = p;
         }
-        const float varp = A.wa.w ? sxg_var(A.wa.w, p) : 0.0f;
+        const float varp = A.wa.w ? sxg_var(A.wa, p) : 0.0f;
         const float athr = perpix ? sxg_pix_thresh(A.wa.kana, varp) : A.thresh;
         othr = fminf(othr, athr);
         if (pix > athr) area++;
@@ -260,9 +261,11 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
             pix = csub;
             const float cdpix = ccd;
             /* per-pixel variance with a weight map */
-            const float varp = A.wa.w ? sxg_var(A.wa.w, p) : 0.0f;
+            const float varp = A.wa.w ? sxg_var(A.wa, p) : 0.0f;
             var2 = A.wa.w ? (double)varp : var;
             err = A.wa.w ? var2 : dbacknoise2;
+            /* var is the background noise squared */
+            if (A.gainflag && pix > 0.0 && A.gain > 0.0) var2 += pix / A.gain * varp / var;
             if (A.gain > 0.0 && cdpix > 0.0) err += cdpix / A.gain;
             x = x - xm;
             y = y - ym;
@@ -304,7 +307,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
         pmxy = emxy / flux2;
     }
     tv -= npx * dbkg;
-    if (A.gain > 0.0 && tv > 0.0) sigtv += tv / A.gain;
+    if (!A.gainflag && A.gain > 0.0 && tv > 0.0) sigtv += tv / A.gain;
     const double s = sxw_rl(eacc, 6), sx = sxw_rl(eacc, 7), sxx = sxw_rl(eacc, 8);
     const double sy = sxw_rl(eacc, 9), sxy = sxw_rl(eacc, 10);
 #pragma unroll
@@ -351,7 +354,7 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
                             int backw, int backh, int width, int numobj)
 {
     SxgState &S = sxg_state();
-    (void)ngamma; (void)plistexist_var; (void)plistoff_var; (void)prefs_weightgain_flag;
+    (void)ngamma; (void)plistexist_var; (void)plistoff_var; 
     (void)prefs_ext_minarea;
     if (!S.have_frame) {
         sxg_set_error("analyse_gpu: no frame");
@@ -387,6 +390,7 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
     A.fwhmflag = *reinterpret_cast<const char *>(&flagobj_fwhm) != 0;
     A.cleanflag = prefs_clean_flag != 0;
     if (!sxg_weight_args(A.wa)) return;
+    A.gainflag = A.wa.w && prefs_weightgain_flag;
     RawObj ro;
     if (!sxg_raw_alloc(S.nraw, ro)) return;
     A.rawt0 = ro.t0;

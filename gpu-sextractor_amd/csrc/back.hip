@@ -861,7 +861,10 @@ __global__ void __launch_bounds__(RN_ROWS) k_rownodes_lds(const float *back, con
     }
 }
 
-/* colseg[x]: index of the left node, coldx[x]: dx of the cumulative recurrence */
+/* colseg[x]: index of the left node, coldx[x]: dx of the cumulative recurrence.
+   SUB: subtract the interpolated value from img (subbackline, back.c:790-930); else write
+   it (backrmsline on the sigma maps, back.c:938-1051) */
+template <bool SUB>
 __global__ void __launch_bounds__(256) k_subback(float *img, int W, int H, int nbx,
                                                  const float *nodes, const float *dnodes,
                                                  const int *colseg, const float *coldx)
@@ -878,7 +881,10 @@ __global__ void __launch_bounds__(256) k_subback(float *img, int W, int H, int n
         bval = cdx * (node[s] + (cdx * cdx - 1) * dnode[s]) + dx * (node[s + 1] + (dx * dx - 1) * dnode[s + 1]);
     } else
         bval = node[0];
-    img[k] -= bval;
+    if (SUB)
+        img[k] -= bval;
+    else
+        img[k] = bval;
 }
 
 __global__ void __launch_bounds__(256) k_subconst(float *img, long n, float v)
@@ -914,6 +920,83 @@ static void column_tables(int W, int nbx, int bs, std::vector<int> &seg, std::ve
     }
 }
 
+/* the per-row spline nodes of a mesh map (and its y second derivatives) into the row
+   tables, and the column tables (built on the host once per frame geometry) */
+static bool spline_rows(const float *map, const float *dmap)
+{
+    SxgState &S = sxg_state();
+    const int W = S.W, H = S.H, nbx = S.nbx, nby = S.nby, bs = S.bs;
+    float *nodes = S.rowspl.as<float>(), *dnodes = nodes + (size_t)H * nbx,
+          *utmp = dnodes + (size_t)H * nbx;
+    int *colseg = reinterpret_cast<int *>(utmp + (size_t)H * nbx);
+    float *coldx = reinterpret_cast<float *>(colseg + W);
+    /* the tables sit after the row tables: the key holds everything their place and
+       contents depend on */
+    if (S.col_w != W || S.col_h != H || S.col_nbx != nbx || S.col_bs != bs ||
+        S.col_buf != S.rowspl.ptr || S.col_bytes != S.rowspl.bytes) {
+        std::vector<int> seg;
+        std::vector<float> dxs;
+        column_tables(W, nbx, bs, seg, dxs);
+        SXG_HIP_RET(hipMemcpy(colseg, seg.data(), sizeof(int) * W, hipMemcpyHostToDevice), false);
+        SXG_HIP_RET(hipMemcpy(coldx, dxs.data(), sizeof(float) * W, hipMemcpyHostToDevice), false);
+        S.col_w = W;
+        S.col_h = H;
+        S.col_nbx = nbx;
+        S.col_bs = bs;
+        S.col_buf = S.rowspl.ptr;
+        S.col_bytes = S.rowspl.bytes;
+    }
+    const size_t rnlds = 3 * (size_t)nbx * RN_ROWS * sizeof(float);
+    if (rnlds <= 64 * 1024) {
+        hipLaunchKernelGGL(k_rownodes_lds, dim3(sxg_div_up(H, RN_ROWS)), dim3(RN_ROWS), rnlds,
+                           S.stream, map, dmap, nbx, nby, bs, H, nodes, dnodes);
+    } else {
+        hipLaunchKernelGGL(k_rownodes, dim3(sxg_div_up(H, 256)), dim3(256), 0, S.stream, map, dmap,
+                           nbx, nby, bs, H, nodes, dnodes, utmp);
+    }
+    SXG_HIP_RET(hipGetLastError(), false);
+    return true;
+}
+
+/* subtract the spline of the row tables from img (sub) or write it into img */
+static bool spline_apply(float *img, bool sub)
+{
+    SxgState &S = sxg_state();
+    const int W = S.W, H = S.H, nbx = S.nbx;
+    float *nodes = S.rowspl.as<float>(), *dnodes = nodes + (size_t)H * nbx,
+          *utmp = dnodes + (size_t)H * nbx;
+    int *colseg = reinterpret_cast<int *>(utmp + (size_t)H * nbx);
+    float *coldx = reinterpret_cast<float *>(colseg + W);
+    if (sub)
+        hipLaunchKernelGGL(k_subback<true>, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0,
+                           S.stream, img, W, H, nbx, nodes, dnodes, colseg, coldx);
+    else
+        hipLaunchKernelGGL(k_subback<false>, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0,
+                           S.stream, img, W, H, nbx, nodes, dnodes, colseg, coldx);
+    SXG_HIP_RET(hipGetLastError(), false);
+    return true;
+}
+
+extern "C" int sxg_get_back_rms(float *out)
+{
+    SxgState &S = sxg_state();
+    sxg_clear_error();
+    /* the mesh maps of the last back_estimate stay until the next one (also after
+       clear_device) */
+    const int nm = S.nbx * S.nby;
+    if (nm <= 0 || S.N == 0 || !S.maps.ptr || S.back_W != S.W || S.back_H != S.H) {
+        sxg_set_error("sxg_get_back_rms: call after back_estimate");
+        return -1;
+    }
+    float *sigma = S.maps.as<float>() + nm, *dsigma = sigma + 2 * (size_t)nm;
+    float *rms = sxg_take<float>("back.rms", S.N);
+    if (!rms || !spline_rows(sigma, dsigma) || !spline_apply(rms, false)) return -1;
+    SXG_HIP_RET(hipMemcpyAsync(out, rms, sizeof(float) * S.N, hipMemcpyDeviceToHost, S.stream),
+                -1);
+    SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
+    return 0;
+}
+
 extern "C" void back_estimate(int back_size, float backfthresh, float *backmean, float *backsig,
                               int nbackfx, int nbackfy, int back_type)
 {
@@ -936,16 +1019,22 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
         return;
     }
     S.bs = bs;
+    S.back_W = S.back_H = 0;    /* set once the maps and row tables exist for this size */
     S.nbx = nbx;
     S.nby = nby;
     int n2 = 1;
     while (n2 < nmesh) n2 <<= 1;
     /* mesh scratch: MeshStat[nmesh] + 4*n2 floats of work + 2 outputs */
     const size_t ms_bytes = ((sizeof(MeshStat) * nmesh + 255) / 256) * 256;
+    /* a new row-table buffer loses the column tables kept in it */
+    const size_t rowspl_bytes = S.rowspl.bytes;
     if (!S.mesh.ensure(ms_bytes + sizeof(float) * (4 * (size_t)n2 + 64)) ||
         !S.maps.ensure(sizeof(float) * 4 * (size_t)nmesh) ||
         !S.rowspl.ensure(sizeof(float) * 3 * (size_t)H * nbx + sizeof(float) * 2 * W + 256))
         return;
+    if (S.rowspl.bytes != rowspl_bytes) S.col_w = 0;
+    S.back_W = W;
+    S.back_H = H;
     MeshStat *ms = S.mesh.as<MeshStat>();
     float *work = reinterpret_cast<float *>(reinterpret_cast<char *>(S.mesh.ptr) + ms_bytes);
     float *outv = work + 4 * (size_t)n2;
@@ -954,7 +1043,8 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
     float *img = S.img.as<float>();
     WeightArgs wa;
     if (!sxg_weight_args(wa)) return;
-    const float *wmap = wa.w;   /* pixels of non-positive weight leave the mesh statistics */
+    const float *wmap = wa.mask;   /* pixels with a non-positive mask value leave the mesh
+                                      statistics */
 
     sxg_stage_begin(ST_BACK);
     sxg_kt_begin(SXG_KT_MESHSTAT);
@@ -1014,30 +1104,22 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
         hipLaunchKernelGGL(k_subconst, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0, S.stream,
                            img, (long)S.N, S.back_value);
     } else {
-        float *nodes = S.rowspl.as<float>(), *dnodes = nodes + (size_t)H * nbx,
-              *utmp = dnodes + (size_t)H * nbx;
-        int *colseg = reinterpret_cast<int *>(utmp + (size_t)H * nbx);
-        float *coldx = reinterpret_cast<float *>(colseg + W);
-        std::vector<int> seg;
-        std::vector<float> dxs;
-        column_tables(W, nbx, bs, seg, dxs);
-        SXG_HIP(hipMemcpyAsync(colseg, seg.data(), sizeof(int) * W, hipMemcpyHostToDevice, S.stream));
-        SXG_HIP(hipMemcpyAsync(coldx, dxs.data(), sizeof(float) * W, hipMemcpyHostToDevice, S.stream));
         sxg_kt_begin(SXG_KT_ROWNODES);
-        const size_t rnlds = 3 * (size_t)nbx * RN_ROWS * sizeof(float);
-        if (rnlds <= 64 * 1024) {
-            hipLaunchKernelGGL(k_rownodes_lds, dim3(sxg_div_up(H, RN_ROWS)), dim3(RN_ROWS), rnlds,
-                               S.stream, back, dback, nbx, nby, bs, H, nodes, dnodes);
-        } else {
-            hipLaunchKernelGGL(k_rownodes, dim3(sxg_div_up(H, 256)), dim3(256), 0, S.stream, back,
-                               dback, nbx, nby, bs, H, nodes, dnodes, utmp);
-        }
+        if (!spline_rows(back, dback)) return;
         sxg_kt_end(SXG_KT_ROWNODES);
         sxg_kt_begin(SXG_KT_SUBBACK);
-        hipLaunchKernelGGL(k_subback, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0, S.stream,
-                           img, W, H, nbx, nodes, dnodes, colseg, coldx);
+        if (!spline_apply(img, true)) return;
         sxg_kt_end(SXG_KT_SUBBACK);
-        SXG_HIP(hipStreamSynchronize(S.stream));   /* host tables must outlive the copies */
+    }
+    if (S.weighted && S.wtype == SXG_WEIGHT_BACKGROUND) {
+        /* WEIGHT_TYPE BACKGROUND: the variance map of this frame is the square of the
+           interpolated background rms */
+        if (!S.weight.ensure(sizeof(float) * S.N) || !spline_rows(sigma, dsigma) ||
+            !spline_apply(S.weight.as<float>(), false))
+            return;
+        S.ww = W;
+        S.wh = H;
+        S.wrms_frame = true;
     }
     sxg_stage_end(ST_BACK);
     SXG_HIP(hipGetLastError());

@@ -29,7 +29,7 @@
    thresholds, else the global one */
 __device__ inline float det_thresh(const WeightArgs &wa, float t, long p)
 {
-    return (wa.w && wa.rel) ? sxg_pix_thresh(wa.kdet, sxg_var(wa.w, p)) : t;
+    return (wa.w && wa.rel) ? sxg_pix_thresh(wa.kdet, sxg_var(wa, p)) : t;
 }
 
 /* union-find parent of each pixel: -1 for the background; a detected pixel points at the

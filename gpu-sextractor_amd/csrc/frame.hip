@@ -118,6 +118,7 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
         S.nraw = S.nobj = S.nsurv = 0;
         S.cleaned = false;
         S.phot_ready = false;
+        S.wrms_frame = false;
         S.filtered = false;
         S.deb_ready = false;
         for (int i = 0; i < ST_N; i++) S.ev_used[i] = false;
@@ -170,7 +171,7 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
     const double ftrue = [] { double d; uint64_t u = 1; memcpy(&d, &u, 8); return d; }();
     const float ftruef = [] { float f; uint32_t u = 1; memcpy(&f, &u, 4); return f; }();
     analyse_gpu(ftrue, 1, 1, ftruef, backsig, p->gain, 0.0, thresh, dthresh, p->satur_level, 0, 0,
-                0, p->ext_minarea, p->clean, 0, nbx, nby, bw, bw, W, nobj);
+                S.wgain ? 1 : 0, p->ext_minarea, p->clean, 0, nbx, nby, bw, bw, W, nobj);
     if (sxg_has_error()) return -1;
     run_clean(nullptr, p->clean_param, p->clean_stacksize, nobj);
     if (sxg_has_error()) return -1;

@@ -63,15 +63,23 @@ struct DevBuf {
 /* MAP_WEIGHT: var = 1/w for w > 0, else BIG (weight.c:111-116, sigfac = 1 as weight.c:80
    sets it); with relative thresholds (rel) each pixel's thresholds are kdet*sqrt(var)
    and kana*sqrt(var) */
+/* w: the weight map as given (MAP_WEIGHT weights, MAP_RMS rms, MAP_VAR variances; for
+   WEIGHT_TYPE BACKGROUND the frame's interpolated background rms), or nullptr;
+   mask: the map the mesh statistics read, positive where a pixel takes part */
 struct WeightArgs {
     const float *w = nullptr;
+    const float *mask = nullptr;
+    int type = SXG_WEIGHT_MAP_WEIGHT;
     int rel = 0;
     double kdet = 0.0, kana = 0.0;
 };
-__device__ inline float sxg_var(const float *w, long p)
+/* the calibrated variance of pixel p (weight_to_var, weight.c:93-126, sigfac 1) */
+__device__ inline float sxg_var(const WeightArgs &wa, long p)
 {
-    const float v = w[p];
-    return v > 0.0f ? 1.0f / v : (float)SXG_BIG;
+    const float v = wa.w[p];
+    if (wa.type == SXG_WEIGHT_MAP_WEIGHT) return v > 0.0f ? 1.0f / v : (float)SXG_BIG;
+    if (wa.type == SXG_WEIGHT_MAP_VAR) return v;
+    return v < SXG_BIG ? v * v : v;      /* MAP_RMS and the background rms */
 }
 __device__ inline float sxg_pix_thresh(double k, float var) { return (float)(k * sqrt((double)var)); }
 
@@ -99,6 +107,10 @@ struct SxgState {
 
     /* background */
     int bs = 0, nbx = 0, nby = 0;
+    int back_W = 0, back_H = 0;               /* frame size the mesh maps belong to */
+    int col_w = 0, col_h = 0, col_nbx = 0, col_bs = 0;   /* key of the column tables */
+    const void *col_buf = nullptr;                       /* (and the buffer they are in) */
+    size_t col_bytes = 0;
     float backmean = 0.0f, backsig = 0.0f;
     DevBuf mesh;     /* per-mesh scratch */
     DevBuf maps;     /* back, sigma, dback, dsigma */
@@ -148,6 +160,10 @@ struct SxgState {
     DevBuf weight;
     int ww = 0, wh = 0;
     bool weighted = false;
+    int wtype = SXG_WEIGHT_MAP_WEIGHT;     /* sxg_weight_type */
+    bool wgain = false;                    /* WEIGHT_GAIN */
+    DevBuf wmask;                          /* MAP_RMS / MAP_VAR: mesh statistics mask */
+    bool wrms_frame = false;               /* BACKGROUND: the rms map of this frame is built */
     WeightArgs wargs;                      /* w filled in by sxg_weight_args() */
 };
 /* the weight arguments for the current frame (w = nullptr without a weight map); records

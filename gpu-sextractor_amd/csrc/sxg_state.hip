@@ -450,6 +450,7 @@ extern "C" void init_device(int width, int height, float *imgbuf)
     S.nraw = S.nobj = S.nsurv = 0;
     S.cleaned = false;
     S.phot_ready = false;
+    S.wrms_frame = false;
     S.filtered = false;
     for (int i = 0; i < ST_N; i++) S.ev_used[i] = false;
     if (!S.img.ensure(S.N * sizeof(float)) || !S.cd.ensure(S.N * sizeof(float)) ||
@@ -502,32 +503,72 @@ extern "C" int sxg_get_filtered(float *out)
 }
 
 /* ------------------------------------------------------------- weight maps ---- */
-extern "C" int sxg_set_weight(const float *w, int width, int height, int thresh_type,
-                              double detect_thresh, double analysis_thresh)
+/* MAP_RMS / MAP_VAR: the mesh statistics take the pixels whose variance is below BIG */
+__global__ void __launch_bounds__(256) k_weight_mask(const float *map, long n, int type,
+                                                     float *mask)
+{
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    WeightArgs wa;
+    wa.w = map;
+    wa.type = type;
+    mask[p] = sxg_var(wa, p) < (float)SXG_BIG ? 1.0f : 0.0f;
+}
+
+extern "C" int sxg_set_weight_map(const float *map, int width, int height, int type,
+                                  int thresh_type, double detect_thresh, double analysis_thresh,
+                                  int weight_gain)
 {
     SxgState &S = sxg_state();
     sxg_clear_error();
-    if (!w) {
-        S.weighted = false;
-        S.weight.release();
-        S.ww = S.wh = 0;
-        return 0;
-    }
-    if (width <= 0 || height <= 0 || (long)width * height >= 2147483647L) {
-        sxg_set_error("sxg_set_weight: bad map size");
+    if (type < SXG_WEIGHT_MAP_WEIGHT || type > SXG_WEIGHT_BACKGROUND) {
+        sxg_set_error("sxg_set_weight_map: unknown weight type");
         return -1;
     }
+    S.wrms_frame = false;
+    if (!map && type != SXG_WEIGHT_BACKGROUND) {
+        S.weighted = false;
+        S.weight.release();
+        S.wmask.release();
+        S.ww = S.wh = 0;
+        S.wtype = SXG_WEIGHT_MAP_WEIGHT;
+        S.wgain = false;
+        return 0;
+    }
     if (!sxg_ensure_stream()) return -1;
-    const size_t n = (size_t)width * height;
-    if (!S.weight.ensure(n * sizeof(float))) return -1;
-    SXG_HIP_RET(hipMemcpy(S.weight.ptr, w, n * sizeof(float), hipMemcpyHostToDevice), -1);
-    S.ww = width;
-    S.wh = height;
+    if (map) {
+        if (width <= 0 || height <= 0 || (long)width * height >= 2147483647L) {
+            sxg_set_error("sxg_set_weight_map: bad map size");
+            return -1;
+        }
+        const size_t n = (size_t)width * height;
+        if (!S.weight.ensure(n * sizeof(float))) return -1;
+        SXG_HIP_RET(hipMemcpy(S.weight.ptr, map, n * sizeof(float), hipMemcpyHostToDevice), -1);
+        if (type == SXG_WEIGHT_MAP_RMS || type == SXG_WEIGHT_MAP_VAR) {
+            if (!S.wmask.ensure(n * sizeof(float))) return -1;
+            hipLaunchKernelGGL(k_weight_mask, dim3(sxg_div_up((long)n, 256)), dim3(256), 0,
+                               S.stream, S.weight.as<float>(), (long)n, type, S.wmask.as<float>());
+            SXG_HIP_RET(hipGetLastError(), -1);
+        }
+        S.ww = width;
+        S.wh = height;
+    } else {
+        S.ww = S.wh = 0;    /* BACKGROUND: the map is built per frame */
+    }
     S.weighted = true;
+    S.wtype = type;
+    S.wgain = weight_gain != 0;
     S.wargs.rel = thresh_type != 1;
     S.wargs.kdet = detect_thresh;
     S.wargs.kana = analysis_thresh;
     return 0;
+}
+
+extern "C" int sxg_set_weight(const float *w, int width, int height, int thresh_type,
+                              double detect_thresh, double analysis_thresh)
+{
+    return sxg_set_weight_map(w, width, height, SXG_WEIGHT_MAP_WEIGHT, thresh_type,
+                              detect_thresh, analysis_thresh, 0);
 }
 
 bool sxg_weight_args(WeightArgs &wa)
@@ -535,11 +576,22 @@ bool sxg_weight_args(WeightArgs &wa)
     SxgState &S = sxg_state();
     wa = WeightArgs();
     if (!S.weighted) return true;
+    if (S.wtype == SXG_WEIGHT_BACKGROUND) {
+        /* no mask for the mesh statistics; the rms map once back_estimate has built it */
+        if (!S.wrms_frame) return true;
+        wa = S.wargs;
+        wa.w = S.weight.as<float>();
+        wa.mask = nullptr;
+        wa.type = SXG_WEIGHT_MAP_RMS;
+        return true;
+    }
     if (S.ww != S.W || S.wh != S.H) {
         sxg_set_error("the weight map (sxg_set_weight) does not match the frame size");
         return false;
     }
     wa = S.wargs;
     wa.w = S.weight.as<float>();
+    wa.type = S.wtype;
+    wa.mask = S.wtype == SXG_WEIGHT_MAP_WEIGHT ? wa.w : S.wmask.as<float>();
     return true;
 }

@@ -41,6 +41,7 @@ struct Config {
               {"MAG_ZEROPOINT", "0.0"}, {"GAIN", "0.0"}, {"BACK_TYPE", "AUTO"},
               {"BACK_VALUE", "0.0"}, {"BACK_SIZE", "64"}, {"BACK_FILTERSIZE", "3"},
               {"BACK_FILTTHRESH", "0.0"}, {"WEIGHT_TYPE", "NONE"}, {"WEIGHT_IMAGE", "weight.fits"},
+              {"WEIGHT_GAIN", "Y"},
               {"CHECKIMAGE_TYPE", "NONE"}, {"CHECKIMAGE_NAME", "check.fits"},
               {"VERBOSE_TYPE", "NORMAL"}, {"PHOT_APERTURES", "5"},
               {"PHOT_AUTOPARAMS", "2.5, 3.5"}, {"PHOT_AUTOAPERS", "0.0,0.0"},
@@ -304,9 +305,15 @@ int main(int argc, char **argv)
         fprintf(stderr, "sex: no image given\n");
         return 1;
     }
-    if (cfg.get("WEIGHT_TYPE") != "NONE") {
-        fprintf(stderr, "sex: WEIGHT_TYPE %s is not supported by this build\n",
-                cfg.get("WEIGHT_TYPE").c_str());
+    /* WEIGHT_TYPE (weight.c:35-126): the detection image's weighting, if any */
+    const std::string wtname = cfg.first("WEIGHT_TYPE");
+    int wtype = -1;
+    if (wtname == "MAP_WEIGHT") wtype = SXG_WEIGHT_MAP_WEIGHT;
+    else if (wtname == "MAP_RMS") wtype = SXG_WEIGHT_MAP_RMS;
+    else if (wtname == "MAP_VAR") wtype = SXG_WEIGHT_MAP_VAR;
+    else if (wtname == "BACKGROUND") wtype = SXG_WEIGHT_BACKGROUND;
+    else if (wtname != "NONE") {
+        fprintf(stderr, "sex: WEIGHT_TYPE %s is not supported by this build\n", wtname.c_str());
         return 1;
     }
     if (cfg.get("DETECT_TYPE") != "CCD") {
@@ -353,6 +360,35 @@ int main(int argc, char **argv)
     const int nfs = atoi(cfg.first("BACK_FILTERSIZE").c_str());
     const int back_type = cfg.first("BACK_TYPE") == "MANUAL" ? 1 : 0;
 
+    /* the weight map, as read by newweight (weight.c:35-83) */
+    if (wtype >= 0) {
+        const bool abs_t = cfg.first("THRESH_TYPE") == "ABSOLUTE";
+        float *wmap = nullptr;
+        int ww = 0, wh = 0;
+        if (wtype != SXG_WEIGHT_BACKGROUND) {
+            const char *werr = nullptr;
+            wmap = sxf_read_image(cfg.first("WEIGHT_IMAGE").c_str(), &ww, &wh, nullptr, nullptr,
+                                  &werr);
+            if (!wmap) {
+                fprintf(stderr, "sex: cannot read %s: %s\n", cfg.first("WEIGHT_IMAGE").c_str(),
+                        werr ? werr : "?");
+                return 1;
+            }
+            if (ww != W || wh != H) {
+                fprintf(stderr, "sex: measured frame and weight map have different sizes\n");
+                return 1;
+            }
+        }
+        const int rc = sxg_set_weight_map(wmap, ww, wh, wtype, abs_t ? 1 : 0,
+                                          atof(cfg.first("DETECT_THRESH").c_str()),
+                                          atof(cfg.first("ANALYSIS_THRESH").c_str()),
+                                          cfg.flag("WEIGHT_GAIN") ? 1 : 0);
+        free(wmap);
+        if (rc != 0) {
+            fail("sxg_set_weight_map");
+            return 1;
+        }
+    }
     /* makeit.c:189, back.c:104 */
     init_device(W, H, img);
     if (fail("init_device")) return 1;
@@ -406,8 +442,9 @@ int main(int argc, char **argv)
         memcpy(&ftrue, &u, 8);
         memcpy(&ftruef, &v, 4);
     }
-    analyse_gpu(ftrue, 1, 1, ftruef, backsig, gain, 0.0, thresh, dthresh, satur, 0, 0, 0, minarea,
-                cfg.flag("CLEAN"), 0, nbx, nby, bw, bh, W, nobj);
+    analyse_gpu(ftrue, 1, 1, ftruef, backsig, gain, 0.0, thresh, dthresh, satur, wtype >= 0 ? 1 : 0,
+                0, (wtype >= 0 && cfg.flag("WEIGHT_GAIN")) ? 1 : 0, minarea, cfg.flag("CLEAN"), 0,
+                nbx, nby, bw, bh, W, nobj);
     if (fail("analyse_gpu")) return 1;
     std::vector<uint32_t> master(nobj > 0 ? nobj : 1);
     run_clean(reinterpret_cast<int *>(master.data()), cfg.num("CLEAN_PARAM"),
@@ -480,6 +517,14 @@ int main(int argc, char **argv)
     if (ctype == "SEGMENTATION" && sxf_write_image(cname.c_str(), seg.data(), W, H, 32)) {
         fprintf(stderr, "sex: cannot write %s\n", cname.c_str());
         return 1;
+    }
+    if (ctype == "BACKGROUND_RMS") {
+        std::vector<float> rms((size_t)W * H);
+        if (sxg_get_back_rms(rms.data()) != 0) {
+            fail("sxg_get_back_rms");
+            return 1;
+        }
+        sxf_write_image(cname.c_str(), rms.data(), W, H, -32);
     }
     if (ctype == "FILTERED") {
         std::vector<float> cd((size_t)W * H);

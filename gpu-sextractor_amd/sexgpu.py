@@ -333,6 +333,8 @@ def gpu():
         L.sxg_batch_segmap.argtypes = [i, vp]
         L.sxg_batch_release.argtypes = []
         L.sxg_set_weight.argtypes = [vp, i, i, i, d, d]
+        L.sxg_set_weight_map.argtypes = [vp, i, i, i, i, d, d, i]
+        L.sxg_get_back_rms.argtypes = [vp]
         L.sxg_set_photometry.argtypes = [C.POINTER(PhotomParams)]
         L.sxg_get_photometry.argtypes = [i, vp, sz]
         L.sxg_set_option.argtypes = [i, C.c_long]
@@ -358,7 +360,8 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_kernel_times", "sxg_set_option", "sxg_get_option", "sxg_ctx_create",
             "sxg_ctx_destroy", "sxg_ctx_bind", "sxg_batch_set_frames", "sxg_batch_process",
             "sxg_batch_segmap", "sxg_batch_release", "sxg_set_weight",
-            "sxg_set_photometry", "sxg_get_photometry"]
+            "sxg_set_photometry", "sxg_get_photometry", "sxg_set_weight_map",
+            "sxg_get_back_rms"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort",
@@ -505,20 +508,36 @@ FLAG_TRUE_DOUBLE = struct.unpack("<d", b"\x01" + b"\x00" * 7)[0]
 FLAG_TRUE_FLOAT = struct.unpack("<f", b"\x01" + b"\x00" * 3)[0]
 
 
-def set_weight(w, p):
-    """Install (or with w None remove) the MAP_WEIGHT map for the next frames."""
+WEIGHT_TYPES = {"MAP_WEIGHT": 0, "MAP_RMS": 1, "MAP_VAR": 2, "BACKGROUND": 3}
+
+
+def set_weight(w, p, weight_type=0, weight_gain=0):
+    """Install (or with w None and a map type remove) the weighting for the next frames:
+    weight_type 0 MAP_WEIGHT, 1 MAP_RMS, 2 MAP_VAR (w is the map), 3 BACKGROUND (no map);
+    weight_gain: WEIGHT_GAIN."""
     L = gpu()
+    if isinstance(weight_type, str):
+        weight_type = WEIGHT_TYPES[weight_type]
     if w is None:
-        L.sxg_set_weight(None, 0, 0, 0, 0.0, 0.0)
-        return
-    w = np.ascontiguousarray(w, dtype=np.float32)
-    if L.sxg_set_weight(w.ctypes.data, w.shape[1], w.shape[0], p.thresh_type, p.detect_thresh,
-                        p.analysis_thresh) != 0:
+        rc = L.sxg_set_weight_map(None, 0, 0, weight_type, p.thresh_type, p.detect_thresh,
+                                  p.analysis_thresh, weight_gain)
+    else:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        rc = L.sxg_set_weight_map(w.ctypes.data, w.shape[1], w.shape[0], weight_type,
+                                  p.thresh_type, p.detect_thresh, p.analysis_thresh, weight_gain)
+    if rc != 0:
         check()
 
 
+def back_rms(shape):
+    out = np.zeros(shape, np.float32)
+    if gpu().sxg_get_back_rms(out.ctypes.data) != 0:
+        check()
+    return out
+
+
 def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"), weight=None,
-              brain=None, pixscale=1.0, ngamma=0.0):
+              brain=None, pixscale=1.0, ngamma=0.0, weight_type=0, weight_gain=0):
     """Run the whole path in the order of the reference's scan.c:78-146 and collect the
     requested outputs (numpy arrays).  weight: optional MAP_WEIGHT map; brain: the
     CLASS_STAR network (read_nnw) for the catalogue; "phot" in outputs: the photometry
@@ -526,7 +545,7 @@ def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"
     L = gpu()
     img = np.ascontiguousarray(img, dtype=np.float32)
     H, W = img.shape
-    set_weight(weight, p)
+    set_weight(weight, p, weight_type, weight_gain)
     bm, bsig = run_background(img, p)
     dthresh, thresh = thresholds(p, bsig)
     L.init_detection()
@@ -545,7 +564,8 @@ def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"
     bw = min(p.back_size, W)
     nbx, nby = (W - 1) // bw + 1, (H - 1) // bw + 1
     L.analyse_gpu(FLAG_TRUE_DOUBLE, 1, 1, FLAG_TRUE_FLOAT, bsig, p.gain, 0.0, thresh, dthresh,
-                  p.satur_level, 0, 0, 0, p.ext_minarea, p.clean, 0, nbx, nby, bw, bw, W, nobj)
+                  p.satur_level, 0, 0, 1 if weight_gain else 0, p.ext_minarea, p.clean, 0, nbx,
+                  nby, bw, bw, W, nobj)
     check()
     master = np.zeros(max(nobj, 1), np.uint32)
     L.run_clean(master.ctypes.data, p.clean_param, p.clean_stacksize, nobj)

@@ -173,6 +173,23 @@ enum sxg_column {
 };
 int sxg_get_column(int column, void *out, size_t bytes);
 
+/* Weight maps beyond MAP_WEIGHT (weight.c:35-126): the type of the map given to
+   sxg_set_weight_map.  MAP_RMS: var = rms^2 (rms < BIG); MAP_VAR: var as given;
+   BACKGROUND: no map, the variance is the square of the frame's interpolated background
+   rms (backrmsline, back.c:938-1051).  Pixels whose variance is BIG or more (weight <= 0
+   for MAP_WEIGHT) stay out of the mesh statistics; with RELATIVE thresholds the detection
+   and analysis thresholds are per pixel (DETECT_THRESH * sqrt(var)).  weight_gain:
+   WEIGHT_GAIN Y (the per-pixel flux variance is scaled by var/backsig^2, cudaanalyse.cu:
+   2281, :2354-2355, :2464-2465).  map = NULL with a type other than BACKGROUND removes
+   the weighting. */
+enum sxg_weight_type { SXG_WEIGHT_MAP_WEIGHT = 0, SXG_WEIGHT_MAP_RMS, SXG_WEIGHT_MAP_VAR,
+                       SXG_WEIGHT_BACKGROUND };
+int sxg_set_weight_map(const float *map, int width, int height, int type, int thresh_type,
+                       double detect_thresh, double analysis_thresh, int weight_gain);
+/* The interpolated background rms of the last back_estimate, width*height floats (the
+   BACKGROUND_RMS check-image, backrmsline: back.c:938-1051).  Returns 0 or < 0. */
+int sxg_get_back_rms(float *out);
+
 /* Frame-level values: backmean, backsig, dthresh, thresh, and counts
    (pixels above threshold, raw objects, raw-object pixels, objects, survivors). */
 typedef struct {

@@ -101,6 +101,20 @@ int ref_subback(const float *back, const float *dback, int nbx, int nby, int bs,
     return 0;
 }
 
+/* the interpolated background rms of every row (back.c:938-1051) */
+int ref_backrms(const float *sigma, const float *dsigma, int nbx, int nby, int bs, int W, int H,
+                float *out)
+{
+    picstruct f;
+    setup_field(&f, nbx, nby, 1, 1);
+    f.width = W; f.height = H;
+    f.backw = f.backh = bs;
+    f.sigma = (float *)sigma;
+    f.dsigma = (float *)dsigma;
+    for (int y = 0; y < H; y++) backrmsline(&f, y, out + (size_t)y * W);
+    return 0;
+}
+
 int ref_convolve(const float *img, int W, int H, const float *mask, int mw, int mh, float *out)
 {
     picstruct f;

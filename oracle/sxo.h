@@ -43,8 +43,10 @@ typedef struct {
     double gain;             /* GAIN */
     const float *mask;       /* filter mask (row-major), NULL = no filtering */
     int mask_w, mask_h;
-    const float *weight;     /* MAP_WEIGHT image or NULL */
+    const float *weight;     /* weight map (MAP_WEIGHT, MAP_RMS or MAP_VAR) or NULL */
     double weight_thresh;    /* unused unless weight != NULL */
+    int weight_type;         /* 0 MAP_WEIGHT, 1 MAP_RMS, 2 MAP_VAR, 3 BACKGROUND (no map) */
+    int weight_gain;         /* WEIGHT_GAIN */
 } sxo_config;
 
 typedef struct {
@@ -90,6 +92,10 @@ int sxo_makeback(const float *img, int W, int H, int bs, int nfx, int nfy, doubl
 void sxo_filterback(float *back, float *sigma, int nbx, int nby, int nfx, int nfy,
                     double fthresh, float *backmean, float *backsig);
 void sxo_makebackspline(const float *map, int nbx, int nby, float *dmap);
+/* back.c:938-1051: the same spline as subbackline over the sigma maps, its values written
+   into out (width W) */
+void sxo_backrmsline(const float *sigma, const float *dsigma, int nbx, int nby, int bs, int W,
+                     int y, float *out);
 void sxo_subbackline(const float *back, const float *dback, int nbx, int nby, int bs,
                      int W, int y, float *line, float *backline);
 int sxo_background(const float *img, int W, int H, int bs, int nfx, int nfy, double fthresh,

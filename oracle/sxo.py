@@ -48,6 +48,7 @@ def lib():
         _lib.sxo_filterback.argtypes = [vp, vp, i, i, i, i, d, vp, vp]
         _lib.sxo_makebackspline.argtypes = [vp, i, i, vp]
         _lib.sxo_subbackline.argtypes = [vp, vp, i, i, i, i, i, vp, vp]
+        _lib.sxo_backrmsline.argtypes = [vp, vp, i, i, i, i, i, vp]
         _lib.sxo_backstat_mesh.argtypes = [vp, i, i, i, i, i, vp]
         _lib.sxo_backhisto_mesh.argtypes = [vp, i, i, i, i, i, vp, vp]
         _lib.sxo_backguess.argtypes = [vp, vp, vp, vp]
@@ -74,6 +75,7 @@ def ref():
         _ref.ref_makeback.argtypes = [vp, i, i, i, vp, vp, vp]
         _ref.ref_makebackspline.argtypes = [vp, i, i, vp]
         _ref.ref_subback.argtypes = [vp, vp, i, i, i, i, i, vp]
+        _ref.ref_backrms.argtypes = [vp, vp, i, i, i, i, i, vp]
         _ref.ref_convolve.argtypes = [vp, i, i, vp, i, i, vp]
     return _ref
 
@@ -87,7 +89,8 @@ class SxoConfig(C.Structure):
         ("deblend_mincont", C.c_double), ("clean", C.c_int), ("clean_param", C.c_double),
         ("clean_stacksize", C.c_int), ("satur_level", C.c_double), ("gain", C.c_double),
         ("mask", C.c_void_p), ("mask_w", C.c_int), ("mask_h", C.c_int),
-        ("weight", C.c_void_p), ("weight_thresh", C.c_double),
+        ("weight", C.c_void_p), ("weight_thresh", C.c_double), ("weight_type", C.c_int),
+        ("weight_gain", C.c_int),
     ]
 
 

@@ -340,6 +340,14 @@ void sxo_makebackspline(const float *map, int nbx, int nby, float *dmap)
 }
 
 /* back.c:790-921 for one row (AUTO background); backline may be NULL */
+void sxo_backrmsline(const float *sigma, const float *dsigma, int nbx, int nby, int bs, int W,
+                     int y, float *out)
+{
+    float *scratch = (float *)calloc((size_t)W, sizeof(float));
+    sxo_subbackline(sigma, dsigma, nbx, nby, bs, W, y, scratch, out);
+    free(scratch);
+}
+
 void sxo_subbackline(const float *back, const float *dback, int nbx, int nby, int bs,
                      int W, int y, float *line, float *backline)
 {

@@ -263,17 +263,33 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
     r->alloc = (int *)malloc(sizeof(int) * N);
     r->seg = (int *)calloc(N, sizeof(int));
 
-    /* ---- weight map: variance and mesh mask (weight.c:111-116 with sigfac = 1,
-       weight.c:80); pixels of non-positive weight stay out of the mesh statistics ---- */
-    const float *wmap = cfg->weight;
+    /* ---- weight map: variance and mesh mask (weight_to_var, weight.c:93-126, with
+       sigfac = 1, weight.c:80).  MAP_WEIGHT: pixels of non-positive weight stay out of the
+       mesh statistics; MAP_RMS / MAP_VAR: pixels whose variance is BIG or more.  WEIGHT_TYPE
+       BACKGROUND has no map: the variance is the square of the interpolated background rms
+       (back.c:938-1051), built after the background below. ---- */
+    const int wtype = cfg->weight_type;
+    const float *wmap = wtype == 3 ? NULL : cfg->weight;
     float *masked = NULL;
     if (wmap) {
         masked = (float *)malloc(sizeof(float) * N);
         r->var = (float *)malloc(sizeof(float) * N);
         for (size_t p = 0; p < N; p++) {
-            const int good = wmap[p] > 0.0f;
+            const float v = wmap[p];
+            float var;
+            int good;
+            if (wtype == 1) {
+                var = v < SXO_BIG ? v * v : v;
+                good = var < (float)SXO_BIG;
+            } else if (wtype == 2) {
+                var = v;
+                good = var < (float)SXO_BIG;
+            } else {
+                good = v > 0.0f;
+                var = good ? 1.0f / v : (float)SXO_BIG;
+            }
             masked[p] = good ? img[p] : (float)-SXO_BIG;
-            r->var[p] = good ? 1.0f / wmap[p] : (float)SXO_BIG;
+            r->var[p] = var;
         }
     }
     /* ---- background and thresholds (back.c:43-208) ---- */
@@ -292,6 +308,18 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         }
         free(masked);
     }
+    if (wtype == 3) {
+        r->var = (float *)malloc(sizeof(float) * N);
+        for (int y = 0; y < H; y++)
+            sxo_backrmsline(r->sigma, r->dsigma, nbx, nby, bs, W, y, r->var + (size_t)y * W);
+        for (size_t p = 0; p < N; p++) {
+            const float v = r->var[p];
+            r->var[p] = v < SXO_BIG ? v * v : v;
+        }
+    }
+    const int haswt = r->var != NULL;
+    /* WEIGHT_GAIN with a variance map (cudaanalyse.cu:2281) */
+    const int gainflag = haswt && cfg->weight_gain;
     if (cfg->thresh_type == 1) {
         r->dthresh = (float)cfg->detect_thresh;
         r->thresh = (float)cfg->analysis_thresh;
@@ -302,7 +330,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
     const float dthresh = r->dthresh, thresh = r->thresh;
     /* per-pixel thresholds: with a weight map, relative thresholds are in units of the
        local noise sqrt(var) */
-    const int wrel = wmap && cfg->thresh_type != 1;
+    const int wrel = haswt && cfg->thresh_type != 1;
 #define DTH(p) (wrel ? (float)(cfg->detect_thresh * sqrt((double)r->var[p])) : dthresh)
 #define ATH(p) (wrel ? (float)(cfg->analysis_thresh * sqrt((double)r->var[p])) : thresh)
 
@@ -705,13 +733,15 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
                 peaky = (int)y + 1;
             }
             /* per-pixel variance with a weight map, else the background's */
-            var2 = wmap ? (double)r->var[p] : var;
+            var2 = haswt ? (double)r->var[p] : var;
+            /* cudaanalyse.cu:2354-2355; var is the background noise squared */
+            if (gainflag && pix > 0.0 && cfg->gain > 0.0) var2 += pix / cfg->gain * r->var[p] / var;
             sigtv += var2;
             const float athr = ATH(p);
             if (i == 0 || athr < othr) othr = athr;
             if (pix > athr) area++;
             tv += pix;
-            err = wmap ? var2 : dbacknoise2;
+            err = haswt ? (double)r->var[p] : dbacknoise2;
             if (cfg->gain > 0.0 && cdpix > 0.0) err += cdpix / cfg->gain;
             x = x - xm;
             y = y - ym;
@@ -732,7 +762,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
             r->poserr_mxy[o] = emxy;
         }
         tv -= n * dbkg;
-        if (cfg->gain > 0.0 && tv > 0.0) sigtv += tv / cfg->gain;
+        if (!gainflag && cfg->gain > 0.0 && tv > 0.0) sigtv += tv / cfg->gain;
         r->npix[o] = area;
         r->flux[o] = (float)tv;
         r->fluxerr[o] = (float)sigtv;

@@ -54,3 +54,45 @@ def test_sex_binary_catalogue_and_segmentation(tmp_path, frame1024, masks):
     assert np.array_equal(rows[:, 11].astype(int), g["flag_phot"][keep])
     seg = sexgpu.read_fits(str(tmp_path / "seg.fits"))
     assert np.array_equal(seg.astype(np.int64), g["seg"].astype(np.int64))
+
+
+@pytest.mark.parametrize("wtype", ["MAP_WEIGHT", "BACKGROUND"])
+def test_sex_binary_weighted(tmp_path, masks, wtype):
+    """WEIGHT_TYPE MAP_WEIGHT (a FITS weight map) and BACKGROUND, with WEIGHT_GAIN and a
+    gain: the segmentation check-image equals the library's map for the same weighting;
+    BACKGROUND_RMS writes the interpolated background rms."""
+    img = sexgpu.synth("plain1024", width=640, height=512, nsrc=1500, seed=61)
+    yy, xx = np.mgrid[0:512, 0:640]
+    wt = ((1.0 + 0.3 * np.sin(xx / 50.0)) / 100.0).astype(np.float32)
+    wt[:, 200:204] = 0.0
+    img_path, wt_path = str(tmp_path / "f.fits"), str(tmp_path / "w.fits")
+    sexgpu.write_fits(img_path, img)
+    sexgpu.write_fits(wt_path, wt)
+    (tmp_path / "p.param").write_text("NUMBER\nFLUX_ISO\nFLUXERR_ISO\n")
+    base = {"PARAMETERS_NAME": str(tmp_path / "p.param"), "VERBOSE_TYPE": "QUIET",
+            "FILTER_NAME": os.path.join(CONF, "default.conv"), "WEIGHT_TYPE": wtype,
+            "WEIGHT_IMAGE": wt_path, "WEIGHT_GAIN": "Y", "GAIN": "2.0"}
+    outs = {}
+    for check in ("SEGMENTATION", "BACKGROUND_RMS"):
+        cfg = dict(base, CATALOG_NAME=str(tmp_path / ("c_%s.cat" % check)),
+                   CHECKIMAGE_TYPE=check, CHECKIMAGE_NAME=str(tmp_path / (check + ".fits")))
+        (tmp_path / "w.sex").write_text("".join("%s %s\n" % kv for kv in cfg.items()))
+        r = subprocess.run([BIN, img_path, "-c", str(tmp_path / "w.sex")], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs[check] = sexgpu.read_fits(str(tmp_path / (check + ".fits")))
+    p = sexgpu.Params(mask=masks["default"], gain=2.0)
+    try:
+        g = sexgpu.run_frame(img, p, outputs=("columns", "seg"),
+                             weight=wt if wtype == "MAP_WEIGHT" else None, weight_type=wtype,
+                             weight_gain=1)
+        rms = sexgpu.back_rms(img.shape)
+    finally:
+        sexgpu.set_weight(None, p)
+    assert np.array_equal(outs["SEGMENTATION"].astype(np.int64), g["seg"].astype(np.int64))
+    assert np.array_equal(outs["BACKGROUND_RMS"], rms)
+    rows = np.array([[float(v) for v in ln.split()] for ln in
+                     (tmp_path / "c_SEGMENTATION.cat").read_text().splitlines()
+                     if not ln.startswith("#")])
+    keep = g["master"] == 0xFFFFFFFF
+    assert np.allclose(rows[:, 2], np.sqrt(g["fluxerr"][keep].astype(np.float64)), rtol=1e-6)

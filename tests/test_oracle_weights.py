@@ -95,3 +95,90 @@ def test_masked_pixels_leave_the_mesh_statistics():
     assert np.array_equal(r["back"], bg["back"]) and np.array_equal(r["sigma"], bg["sigma"])
     # the subtraction applies to the real frame: the garbage is subtracted, not -BIG
     assert np.isfinite(r["sub"]).all() and r["sub"][60, 120] > 1e5
+
+
+def test_backrms_equals_the_reference_code():
+    """backrmsline of the oracle equals the reference's own back.c:938-1051 compiled from
+    the reference tree (skipped where that build is absent)."""
+    import pytest
+    R = sxo.ref()
+    if R is None:
+        pytest.skip("reference back.c not built (oracle/_ref)")
+    L = sxo.lib()
+    for (h, w, bs, seed) in ((300, 257, 64, 3), (200, 640, 32, 4), (90, 80, 40, 5), (64, 600, 64, 6)):
+        img = frame(seed=seed, h=h, w=w, nsrc=200)
+        o = sxo.run(img, mask=None, back_size=bs)
+        nbx, nby = o["nbx"], o["nby"]
+        sig, dsig = np.ascontiguousarray(o["sigma"]), np.ascontiguousarray(o["dsigma"])
+        ref = np.zeros((h, w), np.float32)
+        R.ref_backrms(sig.ctypes.data, dsig.ctypes.data, nbx, nby, bs, w, h, ref.ctypes.data)
+        mine = np.zeros((h, w), np.float32)
+        for y in range(h):
+            L.sxo_backrmsline(sig.ctypes.data, dsig.ctypes.data, nbx, nby, bs, w, y,
+                              mine[y].ctypes.data)
+        assert np.array_equal(mine, ref)
+
+
+def test_var_map_equals_the_weight_map_it_comes_from(masks):
+    """weight_to_var: a MAP_VAR map holding 1/w (float) gives the MAP_WEIGHT result."""
+    img = frame(seed=7)
+    wt = np.full(img.shape, 0.01, np.float32)
+    wt[:, ::37] = 0.0
+    wt[40:60, 100:140] = 0.02
+    with np.errstate(divide="ignore"):
+        var = np.where(wt > 0, np.float32(1.0) / wt, BIG).astype(np.float32)
+    a = sxo.run(img, mask=masks["default"], weight=wt)
+    b = sxo.run(img, mask=masks["default"], weight=var, weight_type=2)
+    assert a["nobj"] == b["nobj"] > 10
+    assert np.array_equal(a["seg"], b["seg"])
+    for k in ("flux", "fluxerr", "poserr_mx2", "thresh_obj", "backsig", "dthresh"):
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+
+
+def test_rms_map_squares_below_big_only(masks):
+    """MAP_RMS: var = rms^2 for rms < BIG, else the value itself (weight.c:104-108); a map of
+    exactly representable squares gives the MAP_VAR result of those squares."""
+    img = frame(seed=8)
+    rng = np.random.default_rng(1)
+    rms = (np.float32(2.0) ** rng.integers(1, 4, size=img.shape)).astype(np.float32)
+    rms[:, ::41] = BIG
+    with np.errstate(over="ignore"):
+        var = np.where(rms < BIG, rms * rms, rms).astype(np.float32)
+    a = sxo.run(img, mask=masks["default"], weight=rms, weight_type=1)
+    b = sxo.run(img, mask=masks["default"], weight=var, weight_type=2)
+    assert np.array_equal(np.asarray(a["var"]), var)
+    assert np.array_equal(a["seg"], b["seg"]) and a["nobj"] == b["nobj"]
+    assert not (a["seg"][:, ::41] > 0).any()       # BIG variance: never above threshold
+
+
+def test_background_weighting_is_the_var_map_of_the_rms(masks):
+    """WEIGHT_TYPE BACKGROUND equals MAP_VAR with the square of the interpolated
+    background rms of the same frame."""
+    img = frame(seed=9, h=320, w=384, nsrc=400)
+    a = sxo.run(img, mask=masks["default"], weight_type=3)
+    var = np.asarray(a["var"]).copy()
+    b = sxo.run(img, mask=masks["default"], weight=var, weight_type=2)
+    assert a["nobj"] == b["nobj"] > 10
+    assert np.array_equal(a["seg"], b["seg"])
+    assert np.array_equal(a["fluxerr"], b["fluxerr"])
+    # the rms follows the mesh sigma map: its mean over the frame is close to backsig
+    assert abs(np.sqrt(var).mean() / a["backsig"] - 1) < 0.1
+
+
+def test_weight_gain_counts_positive_pixels_only(masks):
+    """WEIGHT_GAIN: the per-pixel flux variance grows by pix/gain * var/backsig^2 for
+    positive pixels, instead of adding FLUX/gain once (cudaanalyse.cu:2354-2355,
+    :2464-2465).  With var = backsig^2 everywhere, objects whose pixels are all positive
+    get the same error either way; objects with negative pixels get a larger one."""
+    img = frame(seed=10)
+    base = sxo.run(img, mask=masks["default"])
+    v = np.full(img.shape, np.float32(base["backsig"]) ** 2, np.float32)
+    a = sxo.run(img, mask=masks["default"], weight=v, weight_type=2, gain=4.0)
+    b = sxo.run(img, mask=masks["default"], weight=v, weight_type=2, gain=4.0, weight_gain=1)
+    assert np.array_equal(a["seg"], b["seg"])
+    sub = np.asarray(a["sub"])
+    allpos = np.array([(sub[a["alloc"] == o] > 0).all() for o in range(a["nobj"])])
+    assert allpos.any() and (~allpos).any()
+    fa, fb = a["fluxerr"].astype(np.float64), b["fluxerr"].astype(np.float64)
+    assert np.allclose(fa[allpos], fb[allpos], rtol=1e-5)
+    assert (fb[~allpos] > fa[~allpos]).all()

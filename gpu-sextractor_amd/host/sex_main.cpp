@@ -7,9 +7,12 @@
  * It reads the keys of the reference's configuration (src/preflist.h) that the detection
  * path uses, the .param list of catalogue columns and the .conv filter, reads the FITS
  * frame, calls the C-ABI in the order of the reference's src/scan.c:78-146, and writes
- * the catalogue (ASCII_HEAD or ASCII, column names, formats and units as in the
- * reference's key table src/globals.c) and the check-images SEGMENTATION, -BACKGROUND
- * and FILTERED.  Catalogue NUMBER and the segmentation values are 1-based.
+ * the catalogue (ASCII_HEAD with the header lines of catout.c:583-594, ASCII, or
+ * FITS_LDAC as catout.c:720-748 lays it out: an empty primary HDU, LDAC_IMHEAD, and
+ * LDAC_OBJECTS with big-endian columns; column names, formats and units as in the
+ * reference's key table src/globals.c) and the check-images SEGMENTATION, -BACKGROUND,
+ * BACKGROUND_RMS and FILTERED.  Catalogue NUMBER and the segmentation values are
+ * 1-based.
  */
 #include <cmath>
 #include <cstdint>
@@ -103,6 +106,94 @@ bool read_config(const char *path, Config &cfg)
 struct Column {
     const char *name, *desc, *fmt, *unit;
 };
+
+/* the FITS binary-table type of a column (J int32, I int16, E float, D double) */
+char column_tform(const std::string &n)
+{
+    static const char *dbl[] = {"X2_IMAGE", "Y2_IMAGE", "XY_IMAGE", "ERRX2_IMAGE",
+                                "ERRY2_IMAGE", "ERRXY_IMAGE"};
+    static const char *ints[] = {"NUMBER", "ISOAREA_IMAGE", "XMIN_IMAGE", "YMIN_IMAGE",
+                                 "XMAX_IMAGE", "YMAX_IMAGE", "XPEAK_IMAGE", "YPEAK_IMAGE"};
+    for (const char *d : dbl)
+        if (n == d) return 'D';
+    for (const char *d : ints)
+        if (n == d) return 'J';
+    if (n == "FLAGS") return 'I';
+    if (n.compare(0, 3, "ISO") == 0 && n.size() == 4) return 'J';
+    return 'E';
+}
+
+/* TDISP of a printf format: %10.3f -> F10.3, %12.7g -> G12.7, %15.10e -> E15.10, %10d -> I10 */
+std::string tdisp(const char *fmt)
+{
+    std::string f(fmt + 1);
+    const char c = f.back();
+    f.pop_back();
+    const char t = c == 'f' ? 'F' : c == 'g' ? 'G' : c == 'e' ? 'E' : 'I';
+    return std::string(1, t) + f;
+}
+
+/* one catalogue value: integer (J, I) or floating (E, D) */
+struct Cell {
+    char t;
+    long long i;
+    double d;
+};
+
+/* an 80-character FITS header card */
+std::string card(const std::string &key, const std::string &value, const std::string &comment)
+{
+    char buf[81];
+    if (value.empty())
+        snprintf(buf, sizeof buf, "%-80s", key.c_str());
+    else if (comment.empty())
+        snprintf(buf, sizeof buf, "%-8.8s= %-70.70s", key.c_str(), value.c_str());
+    else
+        snprintf(buf, sizeof buf, "%-8.8s= %s / %s", key.c_str(), value.c_str(), comment.c_str());
+    std::string c(buf);
+    c.resize(80, ' ');
+    return c;
+}
+std::string fits_num(long long v)
+{
+    char b[32];
+    snprintf(b, sizeof b, "%20lld", v);
+    return b;
+}
+std::string fits_real(double v)
+{
+    char b[32];
+    snprintf(b, sizeof b, "%20.10G", v);
+    return b;
+}
+std::string fits_str(const std::string &v)
+{
+    std::string q = "'" + v;
+    while (q.size() < 9) q += ' ';
+    return q + "'";
+}
+std::string fits_bool(bool v) { return std::string(19, ' ') + (v ? "T" : "F"); }
+
+/* header and data of an HDU, each padded to whole 2880-byte records */
+void write_hdu(FILE *f, const std::vector<std::string> &cards, const std::vector<char> &data)
+{
+    std::string h;
+    for (const std::string &c : cards) h += c;
+    h += card("END", "", "");
+    h.resize((h.size() + 2879) / 2880 * 2880, ' ');
+    fwrite(h.data(), 1, h.size(), f);
+    if (!data.empty()) {
+        std::vector<char> d(data);
+        d.resize((d.size() + 2879) / 2880 * 2880, 0);
+        fwrite(d.data(), 1, d.size(), f);
+    }
+}
+
+void put_be(std::vector<char> &out, const void *p, int n)
+{
+    const char *c = static_cast<const char *>(p);
+    for (int k = n - 1; k >= 0; k--) out.push_back(c[k]);
+}
 
 /* names, descriptions, formats and units as in the reference's key table (globals.c) */
 const Column kColumns[] = {
@@ -536,73 +627,155 @@ int main(int argc, char **argv)
     if (ctype == "-BACKGROUND") sxf_write_image(cname.c_str(), img, W, H, -32);
     free(img);
     /* catalogue */
-    FILE *cat = cfg.get("CATALOG_NAME") == "STDOUT" ? stdout : fopen(cfg.get("CATALOG_NAME").c_str(), "w");
+    const std::string cattype = cfg.first("CATALOG_TYPE");
+    if (cattype != "ASCII_HEAD" && cattype != "ASCII" && cattype != "FITS_LDAC") {
+        fprintf(stderr, "sex: CATALOG_TYPE %s is not supported by this build\n", cattype.c_str());
+        return 1;
+    }
+    FILE *cat = cfg.get("CATALOG_NAME") == "STDOUT" ? stdout : fopen(cfg.get("CATALOG_NAME").c_str(), "wb");
     if (!cat) {
         fprintf(stderr, "sex: cannot write %s\n", cfg.get("CATALOG_NAME").c_str());
         return 1;
     }
-    if (cfg.get("CATALOG_TYPE") == "ASCII_HEAD")
-        for (size_t k = 0; k < cols.size(); k++)
-            fprintf(cat, "#%4zu %-16s %-58s [%s]\n", k + 1, cols[k]->name, cols[k]->desc,
-                    cols[k]->unit);
+    const std::string ctype_cat = cfg.first("CATALOG_TYPE");
+    /* the value of column n for object i (catalogue NUMBER `number`) */
+    auto cell = [&](const std::string &n, int i, int number) -> Cell {
+        const sxg_outobj &r = rows[i];
+        auto val = [&](auto v) -> Cell {
+            Cell c;
+            c.t = column_tform(n);
+            c.i = (long long)v;
+            c.d = (double)v;
+            return c;
+        };
+        if (n == "NUMBER") return val(number);
+        else if (n == "X_IMAGE") return val(r.sposx);
+        else if (n == "Y_IMAGE") return val(r.sposy);
+        else if (n == "FLUX_ISO") return val(r.flux_iso);
+        else if (n == "FLUXERR_ISO") return val(r.fluxerr_iso);
+        else if (n == "MAG_ISO") return val(r.mag_iso);
+        else if (n == "MAGERR_ISO") return val(r.magerr_iso);
+        else if (n == "ISOAREA_IMAGE") return val(npixc[i]);
+        else if (n == "FLUX_MAX") return val(r.peak);
+        else if (n == "BACKGROUND") return val(r.bkg);
+        else if (n == "THRESHOLD") return val(thr[i]);
+        else if (n == "X2_IMAGE") return val(mx2[i]);
+        else if (n == "Y2_IMAGE") return val(my2[i]);
+        else if (n == "XY_IMAGE") return val(mxy[i]);
+        else if (n == "A_IMAGE") return val(a[i]);
+        else if (n == "B_IMAGE") return val(b[i]);
+        else if (n == "THETA_IMAGE") return val(r.theta);
+        else if (n == "CXX_IMAGE") return val(cxx[i]);
+        else if (n == "CYY_IMAGE") return val(cyy[i]);
+        else if (n == "CXY_IMAGE") return val(cxy[i]);
+        else if (n == "ERRX2_IMAGE") return val(r.poserr_mx2);
+        else if (n == "ERRY2_IMAGE") return val(r.poserr_my2);
+        else if (n == "ERRXY_IMAGE") return val(r.poserr_mxy);
+        else if (n == "FLAGS") return val((int)r.flag);
+        else if (n == "FWHM_IMAGE") return val(r.fwhm);
+        else if (n == "ELONGATION") return val(r.elong);
+        else if (n == "ELLIPTICITY") return val(r.ellip);
+        else if (n == "FLUX_ISOCOR") return val(r.flux_isocor);
+        else if (n == "FLUXERR_ISOCOR") return val(r.fluxerr_isocor);
+        else if (n == "FLUX_APER") return val(r.flux_aper[0]);
+        else if (n == "FLUXERR_APER") return val(r.fluxerr_aper[0]);
+        else if (n == "MAG_APER") return val(r.mag_aper[0]);
+        else if (n == "MAGERR_APER") return val(r.magerr_aper[0]);
+        else if (n == "FLUX_AUTO") return val(r.flux_auto);
+        else if (n == "FLUXERR_AUTO") return val(r.fluxerr_auto);
+        else if (n == "MAG_AUTO") return val(r.mag_auto);
+        else if (n == "MAGERR_AUTO") return val(r.magerr_auto);
+        else if (n == "KRON_RADIUS") return val(r.kronfactor);
+        else if (n == "CLASS_STAR") return val(r.sprob);
+        else if (n == "XMIN_IMAGE") return val(xmin[i] + 1);
+        else if (n == "YMIN_IMAGE") return val(ymin[i] + 1);
+        else if (n == "XMAX_IMAGE") return val(xmax[i] + 1);
+        else if (n == "YMAX_IMAGE") return val(ymax[i] + 1);
+        else if (n == "XPEAK_IMAGE") return val(px[i]);
+        else if (n == "YPEAK_IMAGE") return val(py[i]);
+        else if (n.compare(0, 3, "ISO") == 0 && n.size() == 4)
+            return val(iso[8 * (size_t)i + (n[3] - '0')]);
+        return val(0);
+    };
+    if (ctype_cat == "ASCII_HEAD")   /* catout.c:583-594 */
+        for (size_t k = 0; k < cols.size(); k++) {
+            if (*cols[k]->unit)
+                fprintf(cat, "# %3d %-22.22s %-58.58s [%s]\n", (int)k + 1, cols[k]->name,
+                        cols[k]->desc, cols[k]->unit);
+            else
+                fprintf(cat, "# %3d %-22.22s %-58.58s\n", (int)k + 1, cols[k]->name,
+                        cols[k]->desc);
+        }
     int number = 0;
+    std::vector<char> table;
+    int rowbytes = 0;
+    for (const Column *c : cols) {
+        const char t = column_tform(c->name);
+        rowbytes += t == 'D' ? 8 : t == 'I' ? 2 : 4;
+    }
+    const bool ldac = ctype_cat == "FITS_LDAC";
     for (int i = 0; i < nobj; i++) {
         if (master[i] != 0xffffffffu) continue;   /* scan.c:148-157 */
-        const sxg_outobj &r = rows[i];
         number++;
         for (size_t k = 0; k < cols.size(); k++) {
-            const std::string n = cols[k]->name;
-            const char *f = cols[k]->fmt;
+            const Cell v = cell(cols[k]->name, i, number);
+            if (ldac) {
+                if (v.t == 'J') { const int32_t x = (int32_t)v.i; put_be(table, &x, 4); }
+                else if (v.t == 'I') { const int16_t x = (int16_t)v.i; put_be(table, &x, 2); }
+                else if (v.t == 'E') { const float x = (float)v.d; put_be(table, &x, 4); }
+                else { const double x = v.d; put_be(table, &x, 8); }
+                continue;
+            }
             if (k) fputc(' ', cat);
-            if (n == "NUMBER") fprintf(cat, f, number);
-            else if (n == "X_IMAGE") fprintf(cat, f, r.sposx);
-            else if (n == "Y_IMAGE") fprintf(cat, f, r.sposy);
-            else if (n == "FLUX_ISO") fprintf(cat, f, r.flux_iso);
-            else if (n == "FLUXERR_ISO") fprintf(cat, f, r.fluxerr_iso);
-            else if (n == "MAG_ISO") fprintf(cat, f, r.mag_iso);
-            else if (n == "MAGERR_ISO") fprintf(cat, f, r.magerr_iso);
-            else if (n == "ISOAREA_IMAGE") fprintf(cat, f, npixc[i]);
-            else if (n == "FLUX_MAX") fprintf(cat, f, r.peak);
-            else if (n == "BACKGROUND") fprintf(cat, f, r.bkg);
-            else if (n == "THRESHOLD") fprintf(cat, f, thr[i]);
-            else if (n == "X2_IMAGE") fprintf(cat, f, mx2[i]);
-            else if (n == "Y2_IMAGE") fprintf(cat, f, my2[i]);
-            else if (n == "XY_IMAGE") fprintf(cat, f, mxy[i]);
-            else if (n == "A_IMAGE") fprintf(cat, f, a[i]);
-            else if (n == "B_IMAGE") fprintf(cat, f, b[i]);
-            else if (n == "THETA_IMAGE") fprintf(cat, f, r.theta);
-            else if (n == "CXX_IMAGE") fprintf(cat, f, cxx[i]);
-            else if (n == "CYY_IMAGE") fprintf(cat, f, cyy[i]);
-            else if (n == "CXY_IMAGE") fprintf(cat, f, cxy[i]);
-            else if (n == "ERRX2_IMAGE") fprintf(cat, f, r.poserr_mx2);
-            else if (n == "ERRY2_IMAGE") fprintf(cat, f, r.poserr_my2);
-            else if (n == "ERRXY_IMAGE") fprintf(cat, f, r.poserr_mxy);
-            else if (n == "FLAGS") fprintf(cat, f, (int)r.flag);
-            else if (n == "FWHM_IMAGE") fprintf(cat, f, r.fwhm);
-            else if (n == "ELONGATION") fprintf(cat, f, r.elong);
-            else if (n == "ELLIPTICITY") fprintf(cat, f, r.ellip);
-            else if (n == "FLUX_ISOCOR") fprintf(cat, f, r.flux_isocor);
-            else if (n == "FLUXERR_ISOCOR") fprintf(cat, f, r.fluxerr_isocor);
-            else if (n == "FLUX_APER") fprintf(cat, f, r.flux_aper[0]);
-            else if (n == "FLUXERR_APER") fprintf(cat, f, r.fluxerr_aper[0]);
-            else if (n == "MAG_APER") fprintf(cat, f, r.mag_aper[0]);
-            else if (n == "MAGERR_APER") fprintf(cat, f, r.magerr_aper[0]);
-            else if (n == "FLUX_AUTO") fprintf(cat, f, r.flux_auto);
-            else if (n == "FLUXERR_AUTO") fprintf(cat, f, r.fluxerr_auto);
-            else if (n == "MAG_AUTO") fprintf(cat, f, r.mag_auto);
-            else if (n == "MAGERR_AUTO") fprintf(cat, f, r.magerr_auto);
-            else if (n == "KRON_RADIUS") fprintf(cat, f, r.kronfactor);
-            else if (n == "CLASS_STAR") fprintf(cat, f, r.sprob);
-            else if (n == "XMIN_IMAGE") fprintf(cat, f, xmin[i] + 1);
-            else if (n == "YMIN_IMAGE") fprintf(cat, f, ymin[i] + 1);
-            else if (n == "XMAX_IMAGE") fprintf(cat, f, xmax[i] + 1);
-            else if (n == "YMAX_IMAGE") fprintf(cat, f, ymax[i] + 1);
-            else if (n == "XPEAK_IMAGE") fprintf(cat, f, px[i]);
-            else if (n == "YPEAK_IMAGE") fprintf(cat, f, py[i]);
-            else if (n.compare(0, 3, "ISO") == 0 && n.size() == 4)
-                fprintf(cat, f, iso[8 * (size_t)i + (n[3] - '0')]);
+            if (v.t == 'J' || v.t == 'I')
+                fprintf(cat, cols[k]->fmt, (int)v.i);
+            else
+                fprintf(cat, cols[k]->fmt, v.d);
         }
-        fputc('\n', cat);
+        if (!ldac) fputc('\n', cat);
+    }
+    if (ldac) {
+        /* FITS_LDAC (catout.c:720-748): an empty primary HDU, LDAC_IMHEAD holding the
+           frame's header cards as one string cell, LDAC_OBJECTS the catalogue */
+        std::vector<std::string> im = {
+            card("SIMPLE", fits_bool(true), ""), card("BITPIX", fits_num(-32), ""),
+            card("NAXIS", fits_num(2), ""), card("NAXIS1", fits_num(W), ""),
+            card("NAXIS2", fits_num(H), ""),
+            card("SEXBKGND", fits_real(backmean), "Median background level (ADU)"),
+            card("SEXBKDEV", fits_real(backsig), "Median background RMS (ADU)"),
+            card("SEXTHLD", fits_real(dthresh), "Extraction threshold (ADU)"),
+            card("SEXATHLD", fits_real(thresh), "Analysis threshold (ADU)"),
+            card("SEXNDET", fits_num(nobj), "Number of raw detections"),
+            card("SEXGAIN", fits_real(gain), "Gain used (e-/ADU)"), card("END", "", "")};
+        std::vector<char> imdata;
+        for (const std::string &c : im) imdata.insert(imdata.end(), c.begin(), c.end());
+        const int ncard = (int)im.size();
+        write_hdu(cat, {card("SIMPLE", fits_bool(true), ""), card("BITPIX", fits_num(8), ""),
+                        card("NAXIS", fits_num(0), ""), card("EXTEND", fits_bool(true), "")},
+                  {});
+        write_hdu(cat, {card("XTENSION", fits_str("BINTABLE"), ""), card("BITPIX", fits_num(8), ""),
+                        card("NAXIS", fits_num(2), ""), card("NAXIS1", fits_num(80 * ncard), ""),
+                        card("NAXIS2", fits_num(1), ""), card("PCOUNT", fits_num(0), ""),
+                        card("GCOUNT", fits_num(1), ""), card("TFIELDS", fits_num(1), ""),
+                        card("TTYPE1", fits_str("Field Header Card"), ""),
+                        card("TFORM1", fits_str(std::to_string(80 * ncard) + "A"), ""),
+                        card("TDIM1", fits_str("(80, " + std::to_string(ncard) + ")"), ""),
+                        card("EXTNAME", fits_str("LDAC_IMHEAD"), "")},
+                  imdata);
+        std::vector<std::string> th = {
+            card("XTENSION", fits_str("BINTABLE"), ""), card("BITPIX", fits_num(8), ""),
+            card("NAXIS", fits_num(2), ""), card("NAXIS1", fits_num(rowbytes), ""),
+            card("NAXIS2", fits_num(number), ""), card("PCOUNT", fits_num(0), ""),
+            card("GCOUNT", fits_num(1), ""), card("TFIELDS", fits_num((long long)cols.size()), "")};
+        for (size_t k = 0; k < cols.size(); k++) {
+            const std::string j = std::to_string(k + 1);
+            th.push_back(card("TTYPE" + j, fits_str(cols[k]->name), cols[k]->desc));
+            th.push_back(card("TFORM" + j, fits_str(std::string("1") + column_tform(cols[k]->name)), ""));
+            if (*cols[k]->unit) th.push_back(card("TUNIT" + j, fits_str(cols[k]->unit), ""));
+            th.push_back(card("TDISP" + j, fits_str(tdisp(cols[k]->fmt)), ""));
+        }
+        th.push_back(card("EXTNAME", fits_str("LDAC_OBJECTS"), ""));
+        write_hdu(cat, th, table);
     }
     if (cat != stdout) fclose(cat);
     if (cfg.get("VERBOSE_TYPE") != "QUIET")

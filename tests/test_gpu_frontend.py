@@ -96,3 +96,73 @@ def test_sex_binary_weighted(tmp_path, masks, wtype):
                      if not ln.startswith("#")])
     keep = g["master"] == 0xFFFFFFFF
     assert np.allclose(rows[:, 2], np.sqrt(g["fluxerr"][keep].astype(np.float64)), rtol=1e-6)
+
+
+def _read_fits_tables(path):
+    """A minimal FITS reader for the test: every HDU's header (dict) and raw data bytes."""
+    raw = open(path, "rb").read()
+    pos, hdus = 0, []
+    while pos < len(raw):
+        head = {}
+        while True:
+            block = raw[pos:pos + 2880]
+            pos += 2880
+            cards = [block[i:i + 80].decode("ascii") for i in range(0, 2880, 80)]
+            done = False
+            for c in cards:
+                k = c[:8].strip()
+                if k == "END":
+                    done = True
+                    break
+                if c[8:10] == "= ":
+                    v = c[10:].split(" / ")[0].strip()
+                    head[k] = v.strip("'").strip() if v.startswith("'") else v
+            if done:
+                break
+        n = int(head.get("NAXIS1", 0)) * int(head.get("NAXIS2", 0)) if int(head["NAXIS"]) else 0
+        hdus.append((head, raw[pos:pos + n]))
+        pos += (n + 2879) // 2880 * 2880
+    return hdus
+
+
+def test_sex_binary_fits_ldac(tmp_path, frame1024, masks):
+    """CATALOG_TYPE FITS_LDAC: an empty primary HDU, LDAC_IMHEAD with the frame's header
+    cards, and LDAC_OBJECTS whose big-endian columns hold the library's values exactly."""
+    img_path = str(tmp_path / "frame.fits")
+    sexgpu.write_fits(img_path, frame1024)
+    names = ["NUMBER", "X_IMAGE", "FLUX_AUTO", "X2_IMAGE", "FLAGS", "ISO0", "CLASS_STAR"]
+    (tmp_path / "l.param").write_text("\n".join(names) + "\n")
+    cfg = {"CATALOG_NAME": str(tmp_path / "out.ldac"), "CATALOG_TYPE": "FITS_LDAC",
+           "PARAMETERS_NAME": str(tmp_path / "l.param"),
+           "FILTER_NAME": os.path.join(CONF, "default.conv"),
+           "STARNNW_NAME": os.path.join(CONF, "default.nnw"), "VERBOSE_TYPE": "QUIET"}
+    (tmp_path / "l.sex").write_text("".join("%s %s\n" % kv for kv in cfg.items()))
+    r = subprocess.run([BIN, img_path, "-c", str(tmp_path / "l.sex")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    hdus = _read_fits_tables(str(tmp_path / "out.ldac"))
+    assert len(hdus) == 3 and hdus[0][0]["NAXIS"] == "0"
+    imh, imd = hdus[1]
+    assert imh["EXTNAME"] == "LDAC_IMHEAD" and imd[:6] == b"SIMPLE"
+    assert b"NAXIS1  =                 1024" in imd
+    th, td = hdus[2]
+    assert th["EXTNAME"] == "LDAC_OBJECTS"
+    assert [th["TTYPE%d" % (k + 1)] for k in range(len(names))] == names
+    code = {"J": ">i4", "I": ">i2", "E": ">f4", "D": ">f8"}
+    dt = np.dtype([(n, code[th["TFORM%d" % (k + 1)][-1]]) for k, n in enumerate(names)])
+    assert dt.itemsize == int(th["NAXIS1"])
+    t = np.frombuffer(td, dtype=dt)
+
+    sexgpu.set_photometry()
+    g = sexgpu.run_frame(frame1024, sexgpu.Params(mask=masks["default"]),
+                         outputs=("columns", "phot"),
+                         brain=sexgpu.read_nnw(os.path.join(CONF, "default.nnw")))
+    keep = g["master"] == 0xFFFFFFFF
+    assert len(t) == keep.sum()
+    assert np.array_equal(t["NUMBER"], np.arange(1, len(t) + 1))
+    assert np.array_equal(t["X_IMAGE"], (g["mx"][keep] + 1.0).astype(np.float32))
+    assert np.array_equal(t["FLUX_AUTO"], g["phot"]["flux_auto"][keep])
+    assert np.array_equal(t["X2_IMAGE"], g["mx2"][keep])
+    assert np.array_equal(t["FLAGS"], g["flag_phot"][keep].astype(np.int16))
+    assert np.array_equal(t["ISO0"], g["iso"][keep][:, 0])
+    assert np.array_equal(t["CLASS_STAR"], g["phot"]["class_star"][keep])

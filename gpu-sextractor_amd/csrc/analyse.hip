@@ -407,6 +407,7 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
     hipLaunchKernelGGL(k_analyse, dim3(sxg_div_up(numobj, 4)), dim3(256), 0, S.stream, A);
     sxg_kt_end(SXG_KT_ANALYSE);
     sxg_stage_end(ST_ANALYSE);
+    /* no host output: the next entry points run on the same stream */
     SXG_HIP(hipGetLastError());
-    SXG_HIP(hipStreamSynchronize(S.stream));
+    sxg_stage_wait();
 }

@@ -136,11 +136,15 @@ extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacks
     /* the catalogue's graded columns in the chosen precision (after CLEAN: the
        segmentation map and the CLEAN decisions keep the reference's arithmetic) */
     if (!sxg_precision_columns((int)sxg_option(SXG_OPT_PRECISION))) return;
-    if (masterIndex && n > 0)
+    SXG_HIP(hipGetLastError());
+    /* the only host output is masterIndex: wait only when it is asked for */
+    if (masterIndex && n > 0) {
         SXG_HIP(hipMemcpyAsync(masterIndex, o.master, sizeof(unsigned) * n, hipMemcpyDeviceToHost,
                                S.stream));
-    SXG_HIP(hipGetLastError());
-    SXG_HIP(hipStreamSynchronize(S.stream));
+        SXG_HIP(hipStreamSynchronize(S.stream));
+    } else {
+        sxg_stage_wait();
+    }
 }
 
 /* ------------------------------------------------------------- segmentation ---- */

@@ -1336,8 +1336,10 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         S.objoff = objoff;
     }
     sxg_stage_end(ST_DEBLEND);
+    /* the object count was read above; the kernels queued since need no wait here (the
+       next entry points run on the same stream, the getters wait for it) */
     SXG_HIP_RET(hipGetLastError(), -1);
-    SXG_HIP_RET(hipStreamSynchronize(st), -1);
+    if (!sxg_stage_wait()) return -1;
     return nobj;
 }
 

@@ -385,8 +385,10 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         S.fpia = v0;
     }
     sxg_stage_end(ST_DETECT);
+    /* the counts were read above; the sort and scan queued since run on the stream the
+       next entry points use */
     SXG_HIP(hipGetLastError());
-    SXG_HIP(hipStreamSynchronize(st));
+    if (!sxg_stage_wait()) return;
     if (numValidObj) *numValidObj = (size_t)nraw;
     if (numValidPix) *numValidPix = (size_t)npix;
 }

@@ -15,6 +15,8 @@
  * output still sees its taps in mask row-major order while each LDS row is read once
  * per lane for 8 outputs.  Bound: VALU (2 K^2 FP32 operations per pixel, no FMA).
  */
+#include <cstring>
+
 #include "sxg_internal.h"
 
 #define CV_TX 64
@@ -140,7 +142,16 @@ extern "C" void cudaFilter(float *h_mask, unsigned int mask_width)
     }
     if (!S.dmask.ensure(sizeof(float) * SXG_MAXMASK * SXG_MAXMASK)) return;
     float *dmask = S.dmask.as<float>();
-    SXG_HIP(hipMemcpyAsync(dmask, h_mask, sizeof(float) * K * K, hipMemcpyHostToDevice, S.stream));
+    /* the mask stays on the device while the caller passes the same one; a new one is
+       copied synchronously, so the caller's array is free again on return */
+    const size_t mb = sizeof(float) * K * K;
+    if (S.mask_k != K || S.mask_buf != S.dmask.ptr || memcmp(S.mask_host, h_mask, mb) != 0) {
+        SXG_HIP(hipStreamSynchronize(S.stream));
+        SXG_HIP(hipMemcpy(dmask, h_mask, mb, hipMemcpyHostToDevice));
+        memcpy(S.mask_host, h_mask, mb);
+        S.mask_k = K;
+        S.mask_buf = S.dmask.ptr;
+    }
     sxg_stage_begin(ST_FILTER);
     sxg_kt_begin(SXG_KT_CONV);
     sxg_convolve(S.img.as<float>(), S.cd.as<float>(), S.W, S.H, dmask, K, S.stream);
@@ -148,5 +159,5 @@ extern "C" void cudaFilter(float *h_mask, unsigned int mask_width)
     sxg_stage_end(ST_FILTER);
     S.filtered = true;
     SXG_HIP(hipGetLastError());
-    SXG_HIP(hipStreamSynchronize(S.stream));
+    sxg_stage_wait();
 }

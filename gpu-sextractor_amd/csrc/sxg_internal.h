@@ -152,6 +152,9 @@ struct SxgState {
     std::map<std::string, DevBuf> named;   /* named device buffers (sxg_take) */
     bool deb_ready = false;                /* init_deblend ran for this frame */
     DevBuf dmask;                          /* convolution mask */
+    float mask_host[SXG_MAXMASK * SXG_MAXMASK] = {};   /* the mask dmask holds */
+    int mask_k = 0;
+    const void *mask_buf = nullptr;
     DevBuf resident;                       /* frame kept in HBM (sxg_set_resident_frame) */
     int rw = 0, rh = 0;
     bool is_default = false;               /* the process-wide default context */
@@ -189,6 +192,11 @@ void sxg_kt_end(int id);
 bool sxg_build_segmap();   /* device segmentation map ("out.map") and survivor count */
 
 SxgState &sxg_state();
+/* End of an entry point without a host output: the contexts of the batch workers
+   (several sharing the device) wait for their stream there, which interleaves their
+   kernels on the hardware queues; the default context returns at once and keeps its
+   queue full. */
+bool sxg_stage_wait();
 void sxg_stage_begin(StageId s);
 void sxg_stage_end(StageId s);
 

@@ -43,11 +43,20 @@ extern "C" long sxg_get_option(int id) { return id >= 0 && id < SXG_NOPT ? g_opt
 bool DevBuf::ensure(size_t want)
 {
     if (want <= bytes && ptr) return true;
-    if (ptr) (void)hipFree(ptr);
+    /* grow by half again at least, so that buffers sized by per-frame counts settle
+       after a few frames */
+    const size_t grown = ptr ? bytes + bytes / 2 : 0;
+    size_t alloc = want < 256 ? 256 : want;
+    if (alloc < grown) alloc = grown;
+    alloc = (alloc + 255) & ~(size_t)255;
+    /* queued kernels of any stream may still use the old buffer: the entry points do
+       not wait for their own kernels before returning */
+    if (ptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ptr);
+    }
     ptr = nullptr;
     bytes = 0;
-    size_t alloc = want < 256 ? 256 : want;
-    alloc = (alloc + 255) & ~(size_t)255;
     hipError_t e = hipMalloc(&ptr, alloc);
     if (e != hipSuccess) {
         ptr = nullptr;
@@ -165,6 +174,14 @@ extern "C" int sxg_stage_times(float *ms, int n)
     return k;
 }
 
+bool sxg_stage_wait()
+{
+    SxgState &S = sxg_state();
+    if (S.is_default) return true;
+    SXG_HIP_RET(hipStreamSynchronize(S.stream), false);
+    return true;
+}
+
 extern "C" void sxg_release(void)
 {
     SxgState &S = sxg_state();
@@ -174,6 +191,9 @@ extern "C" void sxg_release(void)
     for (DevBuf *b : bufs) b->release();
     sxg_release_named();
     S.dmask.release();
+    S.mask_k = 0;
+    S.col_w = 0;
+    S.back_W = S.back_H = 0;
     S.resident.release();
     S.weight.release();
     S.weighted = false;

@@ -67,6 +67,9 @@
 #include "sxg_wave.h"
 
 #define DW 64
+/* pixels per lane whose global-memory accesses the tree passes keep in flight together */
+#define DEB_ILP 4
+#define DEB_RUN 8
 
 /* unsigned order-preserving key of a float (0 is below every non-NaN value) */
 __device__ inline unsigned uord(float v) { return (unsigned)f2ord(v) ^ 0x80000000u; }
@@ -99,6 +102,49 @@ template <bool G> __device__ inline int uf_find(int *par, int x)
         p = gp;
     }
     return x;
+}
+
+/* global memory: the lanes that add to the same counter add once, through the lowest
+   of them (same-address atomics serialise in L2); key < 0: nothing to add */
+__device__ inline void wave_add_by_key(int *base, int key, int v)
+{
+    const int lane = threadIdx.x & 63;
+    bool pending = key >= 0;
+    unsigned long long act;
+    while ((act = __ballot(pending)) != 0ull) {
+        const int lead = __builtin_ctzll(act);
+        const int kl = __shfl(key, lead);
+        const bool mine = pending && key == kl;
+        int s = mine ? v : 0;
+        for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d);
+        if (lane == lead) atomicAdd(&base[kl], s);
+        pending = pending && !mine;
+    }
+}
+
+/* the same for the two double sums of a node (their order is free: the significance
+   bound allows for any summation order) */
+template <class Node>
+__device__ inline void wave_add2_by_key(Node *T, int key, double v, double w)
+{
+    const int lane = threadIdx.x & 63;
+    bool pending = key >= 0;
+    unsigned long long act;
+    while ((act = __ballot(pending)) != 0ull) {
+        const int lead = __builtin_ctzll(act);
+        const int kl = __shfl(key, lead);
+        const bool mine = pending && key == kl;
+        double s = mine ? v : 0.0, sa = mine ? w : 0.0;
+        for (int d = 32; d > 0; d >>= 1) {
+            s += __shfl_xor(s, d);
+            sa += __shfl_xor(sa, d);
+        }
+        if (lane == lead) {
+            atomicAdd(&T[kl].ssum, s);
+            atomicAdd(&T[kl].sabs, sa);
+        }
+        pending = pending && !mine;
+    }
 }
 
 /* lock-free union: the larger root is linked to the smaller index; the two paths are
@@ -184,13 +230,13 @@ template <int NT> __device__ inline int wg_prefix(bool f, int *wsum, int &total)
 /* float sum of v[k] over k < n with sel, in increasing k, computed by one wavefront
    (every lane returns the same value); sel(k) is evaluated for k < n only */
 template <class Sel>
-__device__ inline float wave_ordered_sum(const float *v, int n, Sel sel)
+__device__ inline float wave_ordered_sum(const float *v, int start, int n, Sel sel)
 {
     const int lane = threadIdx.x & 63;
     float f = 0.0f;
-    for (int base = 0; base < n; base += 64) {
+    for (int base = start & ~63; base < n; base += 64) {
         const int k = base + lane;
-        const bool in = k < n && sel(k);
+        const bool in = k >= start && k < n && sel(k);
         const int bits = in ? __float_as_int(v[k]) : 0;
         unsigned long long b = __ballot(in);
         while (b) {
@@ -629,14 +675,14 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
     };
     if (top > 0) prelink(top, ls + 1);
     int nc = 1, last = 0, p0 = 0, p1 = 0, prev = ls + 1;
+    float fr_w = 0.0f;          /* the object's float flux when a wavefront needed it */
+    bool fr_ok = false;
     for (int i = top; i >= 1; i--) {
         __syncthreads();
         /* join the entering pixels */
-        for (int k = tid; k < n; k += NT) {
-            cnt[k] = 0;
-            const NbP<IT> b = nb[k];
+        auto join = [&](int k, const NbP<IT> b) {
             const int mk = b.mself;
-            if (mk < i) continue;
+            if (mk < i) return;
             const bool ek = mk < prev;
             const int msk = (int)b.mask, j0 = (int)b.j0;
             const int jnw = j0, jn = j0 + ((msk & NB_NW) ? 1 : 0);
@@ -653,16 +699,75 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
                 if (nw && !wl && (ek || mnw < prev)) uf_unite<G>(par, k, jnw);
                 if (ne && (ek || mne < prev)) uf_unite<G>(par, k, jne);
             }
+        };
+        if constexpr (G) {
+            /* global memory: the neighbour words of DEB_ILP pixels per lane read together */
+            for (int k0 = tid; k0 < n; k0 += DEB_ILP * NT) {
+                NbP<IT> b[DEB_ILP];
+#pragma unroll
+                for (int u = 0; u < DEB_ILP; u++)
+                    if (k0 + u * NT < n) b[u] = nb[k0 + u * NT];
+#pragma unroll
+                for (int u = 0; u < DEB_ILP; u++)
+                    if (k0 + u * NT < n) {
+                        cnt[k0 + u * NT] = 0;
+                        join(k0 + u * NT, b[u]);
+                    }
+            }
+        } else {
+            for (int k = tid; k < n; k += NT) {
+                cnt[k] = 0;
+                join(k, nb[k]);
+            }
         }
         __syncthreads();
         /* roots (the smallest list index = first raster pixel) and sizes; a concurrent
            path-halving step can only lower par[k] towards the same root */
-        for (int k = tid; k < n; k += NT)
-            if (m[k] >= i) {
-                const int rt = uf_find<G>(par, k);
-                par[k] = rt;
-                atomicAdd(&cnt[rt], 1);
+        if constexpr (G) {
+            /* global memory: DEB_ILP pixels per lane walk their paths side by side, so
+               their loads are in flight together */
+            /* a bound the whole wavefront shares: the counts are added wavefront-wide */
+            for (int b0 = 0; b0 < n; b0 += DEB_ILP * NT) {
+                const int k0 = b0 + tid;
+                int x[DEB_ILP], pp[DEB_ILP];
+                bool act[DEB_ILP];
+#pragma unroll
+                for (int u = 0; u < DEB_ILP; u++) {
+                    const int k = k0 + u * NT;
+                    act[u] = k < n && m[k] >= i;
+                    x[u] = k;
+                    pp[u] = act[u] ? ldw<G>(&par[k]) : k;
+                }
+                bool busy = true;
+                while (busy) {
+                    busy = false;
+                    int gp[DEB_ILP];
+#pragma unroll
+                    for (int u = 0; u < DEB_ILP; u++)
+                        gp[u] = pp[u] != x[u] ? ldw<G>(&par[pp[u]]) : pp[u];
+#pragma unroll
+                    for (int u = 0; u < DEB_ILP; u++)
+                        if (pp[u] != x[u]) {
+                            if (gp[u] != pp[u]) atomicMin(&par[x[u]], gp[u]);
+                            x[u] = pp[u];
+                            pp[u] = gp[u];
+                            busy = busy || pp[u] != x[u];
+                        }
+                }
+#pragma unroll
+                for (int u = 0; u < DEB_ILP; u++) {
+                    if (act[u]) par[k0 + u * NT] = x[u];
+                    wave_add_by_key(cnt, act[u] ? x[u] : -1, 1);
+                }
             }
+        } else {
+            for (int k = tid; k < n; k += NT)
+                if (m[k] >= i) {
+                    const int rt = uf_find<G>(par, k);
+                    par[k] = rt;
+                    atomicAdd(&cnt[rt], 1);
+                }
+        }
         __syncthreads();
         /* the nodes of level i in raster order of their first pixel: each thread numbers
            a contiguous run of the list after one prefix over the threads' counts */
@@ -671,22 +776,57 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
             const int per = (n + NT - 1) / NT;
             const int kb = tid * per, ke = min(kb + per, n);
             int mine = 0;
-            for (int k = kb; k < ke; k++)
-                if (m[k] >= i && ldw<G>(&par[k]) == k && ldw<G>(&cnt[k]) >= a.debarea) mine++;
-            int j = wg_scan_excl<NT>(mine, wsum, nkept);
-            for (int k = kb; k < ke; k++)
-                if (m[k] >= i && ldw<G>(&par[k]) == k) {
-                    const int ck = ldw<G>(&cnt[k]);
-                    int jk = -1;
-                    if (ck >= a.debarea) {
-                        jk = j++;
-                        DebNode e;
-                        e.level = i; e.parent = -1; e.count = ck; e.nchild = 0; e.nson = 0;
-                        e.flags = 0; e.kroot = k; e.kanc = -1; e.ssum = 0.0; e.sabs = 0.0;
-                        T[nc + jk] = e;
-                    }
-                    cnt[k] = jk;
+            if constexpr (G) {
+                /* global memory: the run read DEB_RUN pixels at a time, loads in flight
+                   together */
+                for (int k0 = kb; k0 < ke; k0 += DEB_RUN) {
+                    int pk[DEB_RUN], ck[DEB_RUN];
+                    bool in[DEB_RUN];
+#pragma unroll
+                    for (int u = 0; u < DEB_RUN; u++) in[u] = k0 + u < ke && m[k0 + u] >= i;
+#pragma unroll
+                    for (int u = 0; u < DEB_RUN; u++) pk[u] = in[u] ? ldw<G>(&par[k0 + u]) : -1;
+#pragma unroll
+                    for (int u = 0; u < DEB_RUN; u++)
+                        ck[u] = pk[u] == k0 + u ? ldw<G>(&cnt[k0 + u]) : 0;
+#pragma unroll
+                    for (int u = 0; u < DEB_RUN; u++) mine += pk[u] == k0 + u && ck[u] >= a.debarea;
                 }
+            } else {
+                for (int k = kb; k < ke; k++)
+                    if (m[k] >= i && ldw<G>(&par[k]) == k && ldw<G>(&cnt[k]) >= a.debarea) mine++;
+            }
+            int j = wg_scan_excl<NT>(mine, wsum, nkept);
+            auto number = [&](int k, int ck) {
+                int jk = -1;
+                if (ck >= a.debarea) {
+                    jk = j++;
+                    DebNode e;
+                    e.level = i; e.parent = -1; e.count = ck; e.nchild = 0; e.nson = 0;
+                    e.flags = 0; e.kroot = k; e.kanc = -1; e.ssum = 0.0; e.sabs = 0.0;
+                    T[nc + jk] = e;
+                }
+                cnt[k] = jk;
+            };
+            if constexpr (G) {
+                for (int k0 = kb; k0 < ke; k0 += DEB_RUN) {
+                    int pk[DEB_RUN], ck[DEB_RUN];
+                    bool in[DEB_RUN];
+#pragma unroll
+                    for (int u = 0; u < DEB_RUN; u++) in[u] = k0 + u < ke && m[k0 + u] >= i;
+#pragma unroll
+                    for (int u = 0; u < DEB_RUN; u++) pk[u] = in[u] ? ldw<G>(&par[k0 + u]) : -1;
+#pragma unroll
+                    for (int u = 0; u < DEB_RUN; u++)
+                        ck[u] = pk[u] == k0 + u ? ldw<G>(&cnt[k0 + u]) : 0;
+#pragma unroll
+                    for (int u = 0; u < DEB_RUN; u++)
+                        if (pk[u] == k0 + u) number(k0 + u, ck[u]);
+                }
+            } else {
+                for (int k = kb; k < ke; k++)
+                    if (m[k] >= i && ldw<G>(&par[k]) == k) number(k, ldw<G>(&cnt[k]));
+            }
         }
         __syncthreads();
         const bool sums = nkept >= 2;
@@ -705,6 +845,48 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         }
         /* the highest node of each pixel; node sums; run heads of the level i - 1
            entries */
+        if constexpr (G) {
+            /* global memory: the node lookups of DEB_ILP pixels per lane in flight
+               together, then the same steps as below for each of them */
+            for (int b0 = 0; b0 < n; b0 += DEB_ILP * NT) {
+                int mk[DEB_ILP], pk[DEB_ILP], jj[DEB_ILP];
+                IT dk[DEB_ILP];
+                bool need[DEB_ILP];
+#pragma unroll
+                for (int u = 0; u < DEB_ILP; u++) {
+                    const int k = b0 + u * NT + tid;
+                    mk[u] = k < n ? (int)m[k] : -1;
+                    dk[u] = mk[u] >= i ? deep[k] : (IT)0;
+                    need[u] = mk[u] >= i && (dk[u] == (IT)0 || sums);
+                }
+#pragma unroll
+                for (int u = 0; u < DEB_ILP; u++)
+                    pk[u] = need[u] ? ldw<G>(&par[b0 + u * NT + tid]) : 0;
+#pragma unroll
+                for (int u = 0; u < DEB_ILP; u++) jj[u] = need[u] ? ldw<G>(&cnt[pk[u]]) : -1;
+#pragma unroll
+                for (int u = 0; u < DEB_ILP; u++) {
+                    const int k = b0 + u * NT + tid;
+                    const int j = jj[u];
+                    float v = 0.0f;
+                    if (j >= 0) {
+                        if (dk[u] == (IT)0) deep[k] = (IT)(nc + j);
+                        v = cdv[k];
+                    }
+                    if (sums) wave_add2_by_key(T + nc, j, (double)v, fabs((double)v));
+                    if (i > 1) {
+                        const bool ent = mk[u] == i - 1;
+                        bool wen = false;
+                        if (ent && (nb[k].mask & NB_W)) wen = m[k - 1] == i - 1;
+                        const unsigned long long S = __ballot(ent && !wen);
+                        if (ent) {
+                            const unsigned long long le = S & (~0ull >> (63 - lane));
+                            par[k] = k - lane + (le ? 63 - __builtin_clzll(le) : 0);
+                        }
+                    }
+                }
+            }
+        } else
         for (int base = 0; base < n; base += NT) {
             const int k = base + tid;
             const int mk = k < n ? (int)m[k] : -1;
@@ -757,11 +939,16 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
                 int sig = deb_sig_bound(ldw<G>(&T[c].ssum), ldw<G>(&T[c].sabs), cc, b, s_v0lo,
                                         s_v0hi);
                 if (sig < 0) {
+                    /* the node's pixels start at its first pixel kr */
                     const float f = wave_ordered_sum(
-                        cdv, n, [&](int k) { return m[k] >= i && ldw<G>(&par[k]) == kr; });
-                    const float fr = wave_ordered_sum(cdv, n, [](int) { return true; });
+                        cdv, kr, n, [&](int k) { return m[k] >= i && ldw<G>(&par[k]) == kr; });
+                    /* the object's flux, once per wavefront */
+                    if (!fr_ok) {
+                        fr_w = wave_ordered_sum(cdv, 0, n, [](int) { return true; });
+                        fr_ok = true;
+                    }
                     const float lhs = f - b;
-                    sig = (double)lhs > (double)fr * a.mincont;
+                    sig = (double)lhs > (double)fr_w * a.mincont;
                 }
                 if (sig && lane == 0) atomicOr(&T[c].flags, COMP_SIG);
             }

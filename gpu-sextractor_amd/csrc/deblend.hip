@@ -227,23 +227,39 @@ template <int NT> __device__ inline int wg_prefix(bool f, int *wsum, int &total)
     return before + __popcll(b & sxw_lanes_below(lane));
 }
 
-/* float sum of v[k] over k < n with sel, in increasing k, computed by one wavefront
-   (every lane returns the same value); sel(k) is evaluated for k < n only */
+/* float sum of v[k] over start <= k < n with sel, in increasing k, computed by one
+   wavefront (every lane returns the same value); sel(k) is evaluated for k < n only.
+   The scan ends once `need` values are in the sum (need < 0: no limit); four 64-entry
+   chunks are tested and read together, so their loads are in flight at once. */
 template <class Sel>
-__device__ inline float wave_ordered_sum(const float *v, int start, int n, Sel sel)
+__device__ inline float wave_ordered_sum(const float *v, int start, int n, Sel sel,
+                                         int need = -1)
 {
     const int lane = threadIdx.x & 63;
     float f = 0.0f;
-    for (int base = start & ~63; base < n; base += 64) {
-        const int k = base + lane;
-        const bool in = k >= start && k < n && sel(k);
-        const int bits = in ? __float_as_int(v[k]) : 0;
-        unsigned long long b = __ballot(in);
-        while (b) {
-            const int j = __builtin_ctzll(b);
-            b &= b - 1;
-            f += __int_as_float(__builtin_amdgcn_readlane(bits, j));
+    int got = 0;
+    for (int base = start & ~63; base < n; base += 4 * 64) {
+        bool in[4];
+        int bits[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int k = base + u * 64 + lane;
+            in[u] = k >= start && k < n && sel(k);
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            bits[u] = in[u] ? __float_as_int(v[base + u * 64 + lane]) : 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            unsigned long long b = __ballot(in[u]);
+            got += __popcll(b);
+            while (b) {
+                const int j = __builtin_ctzll(b);
+                b &= b - 1;
+                f += __int_as_float(__builtin_amdgcn_readlane(bits[u], j));
+            }
+        }
+        if (need >= 0 && got >= need) break;
     }
     return f;
 }
@@ -525,10 +541,17 @@ template <typename IT> struct __align__(8) NbP {
  * significance is decided as soon as it exists, when its level has two or more nodes
  * (otherwise no parent can have two children there).
  */
-template <int NT, bool G, typename IT>
+/* MODE 0: the whole tree; 1: only the per-pixel setup (cd values, level indices,
+   neighbour words), for the objects whose level components are then built by
+   k_deb_glb_levels; 2: the tree of such an object, with the components of every level
+   read from lvroot / lvcnt (by list index, one P0-long row per level) instead of the
+   growing union-find */
+template <int NT, bool G, typename IT, int MODE = 0>
 __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *cdv,
                                          unsigned char *m, NbP<IT> *nb, int *par, int *cnt,
-                                         IT *deep, int *pixw, const int *pix, DebNode *T)
+                                         IT *deep, int *pixw, const int *pix, DebNode *T,
+                                         int *lvroot = nullptr, int *lvcnt = nullptr,
+                                         const float *frp = nullptr)
 {
     __shared__ int wsum[NT / 64];
     __shared__ int levstart[SXG_MAXNTHRESH + 1], levcnt[SXG_MAXNTHRESH + 1];
@@ -616,6 +639,9 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         s_top = top;
     }
     /* backward neighbours by position in the list (binary search of the row above) */
+    if (MODE == 2)
+        for (int k = tid; k < n; k += NT) deep[k] = 0;
+    else
     for (int k = tid; k < n; k += NT) {
         const int p = pix[k];
         const int y = p / W, x = p - y * W;
@@ -654,6 +680,7 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         deep[k] = 0;
     }
     __syncthreads();
+    if (MODE == 1) return;
     const int top = s_top;
     /* run heads of the pixels entering at the top level (every pixel of V_top) */
     auto prelink = [&](int lev, int enter_hi) {
@@ -673,12 +700,18 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
             }
         }
     };
-    if (top > 0) prelink(top, ls + 1);
+    if (MODE != 2 && top > 0) prelink(top, ls + 1);
     int nc = 1, last = 0, p0 = 0, p1 = 0, prev = ls + 1;
-    float fr_w = 0.0f;          /* the object's float flux when a wavefront needed it */
-    bool fr_ok = false;
+    /* the object's float flux when a wavefront needed it (MODE 2: computed beforehand) */
+    float fr_w = MODE == 2 ? *frp : 0.0f;
+    bool fr_ok = MODE == 2;
     for (int i = top; i >= 1; i--) {
         __syncthreads();
+        if (MODE == 2) {
+            /* the components of level i, built by k_deb_glb_levels */
+            par = lvroot + (size_t)(i - 1) * a.P0;
+            cnt = lvcnt + (size_t)(i - 1) * a.P0;
+        }
         /* join the entering pixels */
         auto join = [&](int k, const NbP<IT> b) {
             const int mk = b.mself;
@@ -700,7 +733,8 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
                 if (ne && (ek || mne < prev)) uf_unite<G>(par, k, jne);
             }
         };
-        if constexpr (G) {
+        if constexpr (MODE == 2) {
+        } else if constexpr (G) {
             /* global memory: the neighbour words of DEB_ILP pixels per lane read together */
             for (int k0 = tid; k0 < n; k0 += DEB_ILP * NT) {
                 NbP<IT> b[DEB_ILP];
@@ -723,7 +757,8 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         __syncthreads();
         /* roots (the smallest list index = first raster pixel) and sizes; a concurrent
            path-halving step can only lower par[k] towards the same root */
-        if constexpr (G) {
+        if constexpr (MODE == 2) {
+        } else if constexpr (G) {
             /* global memory: DEB_ILP pixels per lane walk their paths side by side, so
                their loads are in flight together */
             /* a bound the whole wavefront shares: the counts are added wavefront-wide */
@@ -874,7 +909,7 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
                         v = cdv[k];
                     }
                     if (sums) wave_add2_by_key(T + nc, j, (double)v, fabs((double)v));
-                    if (i > 1) {
+                    if (MODE != 2 && i > 1) {
                         const bool ent = mk[u] == i - 1;
                         bool wen = false;
                         if (ent && (nb[k].mask & NB_W)) wen = m[k - 1] == i - 1;
@@ -939,9 +974,10 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
                 int sig = deb_sig_bound(ldw<G>(&T[c].ssum), ldw<G>(&T[c].sabs), cc, b, s_v0lo,
                                         s_v0hi);
                 if (sig < 0) {
-                    /* the node's pixels start at its first pixel kr */
+                    /* the node's cc pixels start at its first pixel kr */
                     const float f = wave_ordered_sum(
-                        cdv, kr, n, [&](int k) { return m[k] >= i && ldw<G>(&par[k]) == kr; });
+                        cdv, kr, n, [&](int k) { return m[k] >= i && ldw<G>(&par[k]) == kr; },
+                        cc);
                     /* the object's flux, once per wavefront */
                     if (!fr_ok) {
                         fr_w = wave_ordered_sum(cdv, 0, n, [](int) { return true; });
@@ -1059,6 +1095,133 @@ __global__ void __launch_bounds__(1024) k_deb_tree_glb(DebArgs a)
                               reinterpret_cast<NbP<int> *>(a.g_nb) + off, a.g_par + off,
                               a.g_cnt + off, a.g_deep + off, nullptr, a.fpia + off,
                               a.nodes + a.compoff[r]);
+}
+
+/* The same objects with the level components built in parallel: the per-pixel setup
+   (k_deb_glb_prep), then one workgroup per (object, level) builds the 8-connected
+   components of V_i = {m >= i} with their first pixels as roots and their sizes
+   (k_deb_glb_levels), then the tree reads them level by level (k_deb_glb_tree).  The
+   components of V_i do not depend on how they are found, so the tree is the one the
+   growing union-find gives. */
+__global__ void __launch_bounds__(1024) k_deb_glb_prep(DebArgs a)
+{
+    const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
+    const int off = a.off[r];
+    deb_tree<1024, true, int, 1>(a, r, a.g_cdv + off, a.g_m + off,
+                                 reinterpret_cast<NbP<int> *>(a.g_nb) + off, a.g_par + off,
+                                 a.g_cnt + off, a.g_deep + off, nullptr, a.fpia + off,
+                                 a.nodes + a.compoff[r]);
+}
+
+__global__ void __launch_bounds__(1024) k_deb_glb_levels(DebArgs a, int *lvroot, int *lvcnt,
+                                                        float *frbuf)
+{
+    const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
+    const int i = blockIdx.y + 1;
+    const int n = a.cnt[r], off = a.off[r], tid = threadIdx.x;
+    if (i > a.lstar[r]) {
+        /* the last level is never used (L* < nthresh): its workgroup computes the object's
+           float flux, for the significance tests that the bounds leave open */
+        if (i == a.nthresh && tid < 64) {
+            const float f = wave_ordered_sum(a.g_cdv + off, 0, n, [](int) { return true; });
+            if (tid == 0) frbuf[blockIdx.x] = f;
+        }
+        return;
+    }
+    const unsigned char *m = a.g_m + off;
+    const NbP<int> *nb = reinterpret_cast<const NbP<int> *>(a.g_nb) + off;
+    int *par = lvroot + (size_t)(i - 1) * a.P0 + off;
+    int *cnt = lvcnt + (size_t)(i - 1) * a.P0 + off;
+    const int lane = tid & 63;
+    /* every pixel of V_i starts at the head of its run inside its 64-entry window (runs
+       of W neighbours in V_i), so the unions below only join runs */
+    for (int b0 = 0; b0 < n; b0 += 1024) {
+        const int k = b0 + tid;
+        bool in = false, wen = false;
+        if (k < n) {
+            const NbP<int> b = nb[k];
+            in = b.mself >= i;
+            wen = in && b.mw >= i;
+            cnt[k] = 0;
+        }
+        const unsigned long long S = __ballot(in && !wen);
+        if (in) {
+            const unsigned long long le = S & (~0ull >> (63 - lane));
+            par[k] = k - lane + (le ? 63 - __builtin_clzll(le) : 0);
+        }
+    }
+    __syncthreads();
+    /* the backward 8-neighbours in V_i, the redundant ones left out (W: only across a
+       window boundary; N and NW through W: NW is W's N; NE through N: it is N's E) */
+    auto join = [&](int k, const NbP<int> b) {
+        if (b.mself < i) return;
+        const int msk = (int)b.mask, j0 = b.j0;
+        const int jn = j0 + ((msk & NB_NW) ? 1 : 0), jne = jn + ((msk & NB_N) ? 1 : 0);
+        const bool wl = b.mw >= i, nw = b.mnw >= i, nn = b.mn >= i, ne = b.mne >= i;
+        if (wl && (k & 63) == 0) uf_unite<true>(par, k, k - 1);
+        if (nn) {
+            if (!(wl && nw)) uf_unite<true>(par, k, jn);
+        } else {
+            if (nw && !wl) uf_unite<true>(par, k, j0);
+            if (ne) uf_unite<true>(par, k, jne);
+        }
+    };
+    for (int k0 = tid; k0 < n; k0 += DEB_ILP * 1024) {
+        NbP<int> b[DEB_ILP];
+#pragma unroll
+        for (int u = 0; u < DEB_ILP; u++)
+            if (k0 + u * 1024 < n) b[u] = nb[k0 + u * 1024];
+#pragma unroll
+        for (int u = 0; u < DEB_ILP; u++)
+            if (k0 + u * 1024 < n) join(k0 + u * 1024, b[u]);
+    }
+    __syncthreads();
+    /* roots and sizes: DEB_ILP pixels per lane walk their paths side by side; a loop
+       bound the wavefront shares, the sizes added per wavefront */
+    for (int b0 = 0; b0 < n; b0 += DEB_ILP * 1024) {
+        int x[DEB_ILP], pp[DEB_ILP];
+        bool act[DEB_ILP];
+#pragma unroll
+        for (int u = 0; u < DEB_ILP; u++) {
+            const int k = b0 + u * 1024 + tid;
+            act[u] = k < n && m[k] >= i;
+            x[u] = k;
+            pp[u] = act[u] ? ldw<true>(&par[k]) : k;
+        }
+        bool busy = true;
+        while (busy) {
+            busy = false;
+            int gp[DEB_ILP];
+#pragma unroll
+            for (int u = 0; u < DEB_ILP; u++)
+                gp[u] = pp[u] != x[u] ? ldw<true>(&par[pp[u]]) : pp[u];
+#pragma unroll
+            for (int u = 0; u < DEB_ILP; u++)
+                if (pp[u] != x[u]) {
+                    if (gp[u] != pp[u]) atomicMin(&par[x[u]], gp[u]);
+                    x[u] = pp[u];
+                    pp[u] = gp[u];
+                    busy = busy || pp[u] != x[u];
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < DEB_ILP; u++) {
+            if (act[u]) par[b0 + u * 1024 + tid] = x[u];
+            wave_add_by_key(cnt, act[u] ? x[u] : -1, 1);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_deb_glb_tree(DebArgs a, int *lvroot, int *lvcnt,
+                                                      const float *frbuf)
+{
+    const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
+    const int off = a.off[r];
+    deb_tree<1024, true, int, 2>(a, r, a.g_cdv + off, a.g_m + off,
+                                 reinterpret_cast<NbP<int> *>(a.g_nb) + off, a.g_par + off,
+                                 a.g_cnt + off, a.g_deep + off, nullptr, a.fpia + off,
+                                 a.nodes + a.compoff[r], lvroot + off, lvcnt + off,
+                                 frbuf + blockIdx.x);
 }
 
 /* ------------------------------------------------------------------ objects ---- */
@@ -1449,7 +1612,19 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             if (!a.nodes || !a.g_cdv || !a.g_m || !a.g_nb || !a.g_par || !a.g_deep || !a.g_cnt)
                 return -1;
             hipStream_t ks = use_side ? S.side[nside] : st;
-            hipLaunchKernelGGL(k_deb_tree_glb, dim3(cc[3]), dim3(1024), 0, ks, a);
+            if (sxg_option(SXG_OPT_DEB_GLB_LEVELS)) {
+                int *lvroot = sxg_take<int>("deb.lvroot", (size_t)n_thresh * P0);
+                int *lvcnt = sxg_take<int>("deb.lvcnt", (size_t)n_thresh * P0);
+                float *frbuf = sxg_take<float>("deb.glbflux", (size_t)cc[3]);
+                if (!lvroot || !lvcnt || !frbuf) return -1;
+                hipLaunchKernelGGL(k_deb_glb_prep, dim3(cc[3]), dim3(1024), 0, ks, a);
+                hipLaunchKernelGGL(k_deb_glb_levels, dim3(cc[3], n_thresh), dim3(1024), 0, ks, a,
+                                   lvroot, lvcnt, frbuf);
+                hipLaunchKernelGGL(k_deb_glb_tree, dim3(cc[3]), dim3(1024), 0, ks, a, lvroot,
+                                   lvcnt, frbuf);
+            } else {
+                hipLaunchKernelGGL(k_deb_tree_glb, dim3(cc[3]), dim3(1024), 0, ks, a);
+            }
             if (use_side) {
                 SXG_HIP_RET(hipEventRecord(S.side_ev[nside], ks), -1);
                 nside++;

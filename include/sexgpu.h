@@ -316,13 +316,17 @@ int sxg_kernel_times(int id, double *total_ms, int *launches);
                             need more is built in global memory
      SXG_OPT_SIDE_STREAMS   1 (default): the deblending size classes of a frame run side by
                             side on streams of their own; 0: all on the context's stream
+     SXG_OPT_DEB_GLB_LEVELS 1 (default): the trees built in global memory get the components
+                            of all their levels from one workgroup per level, in parallel;
+                            0: one growing union-find per tree, level after level
    This one changes the catalogue (never the segmentation map, the object list or the
    CLEAN decisions):
      SXG_OPT_PRECISION      arithmetic of FLUX_ISO, X/Y, X2/Y2/XY, A/B/THETA, CXX/CYY/CXY:
                             0 the reference's (default, bit-exact), 1 two-pass FP64,
                             2 compensated FP32 sums with FP64 central moments */
 enum sxg_option_id { SXG_OPT_DEB_LDS_SMALL = 0, SXG_OPT_DEB_LDS_LARGE, SXG_OPT_PRECISION,
-                     SXG_OPT_DEB_LDS_TINY, SXG_OPT_DEB_TABLE, SXG_OPT_SIDE_STREAMS, SXG_NOPT };
+                     SXG_OPT_DEB_LDS_TINY, SXG_OPT_DEB_TABLE, SXG_OPT_SIDE_STREAMS,
+                     SXG_OPT_DEB_GLB_LEVELS, SXG_NOPT };
 /* Returns 0, or -1 for an unknown option or a value out of range. */
 int sxg_set_option(int id, long value);
 long sxg_get_option(int id);

@@ -108,17 +108,18 @@ def test_catalog_rows_match_columns(frame1024, masks):
     assert np.array_equal(fe, np.sqrt(g["fluxerr"].astype(np.float64)).astype(np.float32))
 
 
-KNOBS = ("deb_lds_tiny", "deb_lds_small", "deb_lds_large", "deb_table")
+KNOBS = ("deb_lds_tiny", "deb_lds_small", "deb_lds_large", "deb_table", "deb_glb_levels")
 
 
-@pytest.mark.parametrize("caps", [(0, 0, 0, 0), (0, 64, 256, 0), (0, 6400, 6400, 0),
-                                  (256, 1024, 6144, 0), (6400, 6400, 6400, 0),
-                                  (256, 1024, 6144, 1), (256, 1024, 6144, 5),
-                                  (0, 0, 0, 3)])
+@pytest.mark.parametrize("caps", [(0, 0, 0, 0, 1), (0, 0, 0, 0, 0), (0, 64, 256, 0, 1),
+                                  (0, 6400, 6400, 0, 1), (256, 1024, 6144, 0, 1),
+                                  (6400, 6400, 6400, 0, 1), (256, 1024, 6144, 1, 1),
+                                  (256, 1024, 6144, 5, 1), (0, 0, 0, 3, 1), (0, 0, 0, 3, 0)])
 def test_deblend_kernel_variants(masks, caps):
     """Every deblending kernel variant (LDS 64-, 256- and 1024-lane workgroups, global
-    scratch; the tree's node table in LDS or moved to global memory when it outgrows its
-    LDS share) gives the oracle's result: the options move raw objects between them."""
+    scratch with the level components built level by level or all levels in parallel;
+    the tree's node table in LDS or moved to global memory when it outgrows its LDS
+    share) gives the oracle's result: the options move raw objects between them."""
     img = sexgpu.synth("crowded4096", width=1024, height=1024, nsrc=167000 // 16, seed=1201)
     m = masks["gauss_3p0_7x7"]
     old = [sexgpu.get_option(k) for k in KNOBS]

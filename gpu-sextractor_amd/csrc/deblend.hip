@@ -335,7 +335,10 @@ struct DebArgs {
     int *ccount;            /* [0..3] objects per class, [4..6] the largest LDS need of
                                classes 0..2 (bytes), [7] consistency error flag */
     unsigned long long *dwork;  /* [0] pixels, [1] pixel-levels of the multi-peak objects,
-                                   [2] node capacity of the frame */
+                                   [2] node capacity of the frame, [3] pixel-levels and
+                                   [4] node records of the objects built in global memory */
+    unsigned long long *lvbase; /* objects built in global memory: [2r] their first
+                                   pixel-level entry, [2r + 1] their first node record */
     /* global scratch for the objects built in global memory, by raw-list position */
     float *g_cdv;
     unsigned char *g_m;
@@ -378,7 +381,7 @@ __global__ void __launch_bounds__(256) k_deb_prep(DebArgs a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r < 8) a.ccount[r] = 0;
-    if (r < 3) a.dwork[r] = 0ull;
+    if (r < 5) a.dwork[r] = 0ull;
     if (r >= a.nraw) return;
     a.top1[r] = 0ull;
     a.top2[r] = 0u;
@@ -454,6 +457,13 @@ __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int c
             if (cls == 0 && need > bud0) cls = 1;
             if (cls == 1 && need > bud1) cls = 2;
             if (cls == 2 && need > bud2) cls = 3;
+            /* the per-level arrays of the global-memory trees: n entries and
+               n / deb_minarea + 1 node records for each level 1..L* */
+            if (cls == 3) {
+                a.lvbase[2 * r] = atomicAdd(&a.dwork[3], (unsigned long long)n * ls);
+                a.lvbase[2 * r + 1] = atomicAdd(
+                    &a.dwork[4], (unsigned long long)ls * (n / max(a.debarea, 1) + 1));
+            }
         }
     }
     unsigned long long w0 = cls >= 0 ? (unsigned long long)n : 0ull;
@@ -522,6 +532,28 @@ template <typename IT> struct __align__(8) NbP {
     unsigned char mask, mself, mw, mnw, mn, mne;
 };
 
+/* a node of level i found by k_deb_glb_levels: first pixel, pixel count, its double sums
+   while they are added, then the significance bound's answer (1, 0, or -1: decided by
+   the node's float sum f against the object's float flux) */
+struct LvNode {
+    int kroot, count, sig;
+    float f;
+    double ssum, sabs;
+};
+
+/* what the tree of an object built level-parallel reads: node index of each pixel per
+   level (nid[(i - 1) * n + k], -1 outside kept nodes), node records per level (rec[(i -
+   1) * nstride + j]) and their counts (nk[i]), the object's float flux; the per-pixel
+   setup writes the root flux bounds to v0 */
+struct LvArgs {
+    const int *nid;
+    const LvNode *rec;
+    const int *nk;
+    int nstride;
+    float fr;
+    double *v0;
+};
+
 /*
  * The component tree of raw object r, built from the top level down.  Per raw-list
  * index k (LDS, or global scratch when G): cdv (cd), m (level index), nb (backward
@@ -542,16 +574,14 @@ template <typename IT> struct __align__(8) NbP {
  * (otherwise no parent can have two children there).
  */
 /* MODE 0: the whole tree; 1: only the per-pixel setup (cd values, level indices,
-   neighbour words), for the objects whose level components are then built by
-   k_deb_glb_levels; 2: the tree of such an object, with the components of every level
-   read from lvroot / lvcnt (by list index, one P0-long row per level) instead of the
-   growing union-find */
+   neighbour words, root flux bounds), for the objects whose levels are then built by
+   k_deb_glb_levels; 2: the tree of such an object, linking the node records of its
+   levels (lv) instead of growing a union-find */
 template <int NT, bool G, typename IT, int MODE = 0>
 __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *cdv,
                                          unsigned char *m, NbP<IT> *nb, int *par, int *cnt,
                                          IT *deep, int *pixw, const int *pix, DebNode *T,
-                                         int *lvroot = nullptr, int *lvcnt = nullptr,
-                                         const float *frp = nullptr)
+                                         const LvArgs *lv = nullptr)
 {
     __shared__ int wsum[NT / 64];
     __shared__ int levstart[SXG_MAXNTHRESH + 1], levcnt[SXG_MAXNTHRESH + 1];
@@ -680,7 +710,13 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         deep[k] = 0;
     }
     __syncthreads();
-    if (MODE == 1) return;
+    if constexpr (MODE == 1) {
+        if (tid == 0) {
+            lv->v0[0] = s_v0lo;
+            lv->v0[1] = s_v0hi;
+        }
+        return;
+    }
     const int top = s_top;
     /* run heads of the pixels entering at the top level (every pixel of V_top) */
     auto prelink = [&](int lev, int enter_hi) {
@@ -703,14 +739,55 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
     if (MODE != 2 && top > 0) prelink(top, ls + 1);
     int nc = 1, last = 0, p0 = 0, p1 = 0, prev = ls + 1;
     /* the object's float flux when a wavefront needed it (MODE 2: computed beforehand) */
-    float fr_w = MODE == 2 ? *frp : 0.0f;
+    float fr_w = MODE == 2 ? lv->fr : 0.0f;
     bool fr_ok = MODE == 2;
     for (int i = top; i >= 1; i--) {
         __syncthreads();
-        if (MODE == 2) {
-            /* the components of level i, built by k_deb_glb_levels */
-            par = lvroot + (size_t)(i - 1) * a.P0;
-            cnt = lvcnt + (size_t)(i - 1) * a.P0;
+        if constexpr (MODE == 2) {
+            /* the kept nodes of level i from their records, their significance settled */
+            const int nkept = lv->nk[i];
+            const LvNode *rec = lv->rec + (size_t)(i - 1) * lv->nstride;
+            const float ti = tthr[i];
+            for (int c = tid; c < nkept; c += NT) {
+                const LvNode q = rec[c];
+                bool sig = false;
+                if (nkept >= 2) {
+                    sig = q.sig > 0;
+                    if (q.sig < 0) {
+                        const float lhs = q.f - ti * (float)q.count;
+                        sig = (double)lhs > (double)fr_w * a.mincont;
+                    }
+                }
+                DebNode e;
+                e.level = i; e.parent = -1; e.count = q.count; e.nchild = 0; e.nson = 0;
+                e.flags = sig ? COMP_SIG : 0; e.kroot = q.kroot; e.kanc = -1;
+                e.ssum = 0.0; e.sabs = 0.0;
+                T[nc + c] = e;
+            }
+            __syncthreads();
+            /* parents of the level i + 1 nodes: the node that holds their first pixel */
+            const int *nid = lv->nid + (size_t)(i - 1) * n;
+            for (int c = p0 + tid; c < p1; c += NT) {
+                const int j = nid[T[c].kroot];
+                int parent = nc + j;
+                if (j < 0) {
+                    atomicOr(&a.ccount[7], 1);
+                    parent = 0;
+                }
+                T[c].parent = parent;
+                if (atomicAdd(&T[parent].nchild, 1) == 1) s_multi = 1;
+                if (ldw<G>(&T[c].flags) & COMP_SIG) atomicAdd(&T[parent].nson, 1);
+            }
+            if (tid == 0) {
+                levstart[i] = nc;
+                levcnt[i] = nkept;
+            }
+            p0 = nc;
+            p1 = nc + nkept;
+            nc += nkept;
+            if (nkept > 0 && last == 0) last = i;
+            prev = i;
+            continue;
         }
         /* join the entering pixels */
         auto join = [&](int k, const NbP<IT> b) {
@@ -733,8 +810,7 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
                 if (ne && (ek || mne < prev)) uf_unite<G>(par, k, jne);
             }
         };
-        if constexpr (MODE == 2) {
-        } else if constexpr (G) {
+        if constexpr (G) {
             /* global memory: the neighbour words of DEB_ILP pixels per lane read together */
             for (int k0 = tid; k0 < n; k0 += DEB_ILP * NT) {
                 NbP<IT> b[DEB_ILP];
@@ -757,8 +833,7 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         __syncthreads();
         /* roots (the smallest list index = first raster pixel) and sizes; a concurrent
            path-halving step can only lower par[k] towards the same root */
-        if constexpr (MODE == 2) {
-        } else if constexpr (G) {
+        if constexpr (G) {
             /* global memory: DEB_ILP pixels per lane walk their paths side by side, so
                their loads are in flight together */
             /* a bound the whole wavefront shares: the counts are added wavefront-wide */
@@ -909,7 +984,7 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
                         v = cdv[k];
                     }
                     if (sums) wave_add2_by_key(T + nc, j, (double)v, fabs((double)v));
-                    if (MODE != 2 && i > 1) {
+                    if (i > 1) {
                         const bool ent = mk[u] == i - 1;
                         bool wen = false;
                         if (ent && (nb[k].mask & NB_W)) wen = m[k - 1] == i - 1;
@@ -991,6 +1066,21 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
         }
     }
     __syncthreads();
+    if constexpr (MODE == 2) {
+        /* the highest node of each pixel: its node on the highest level that keeps one */
+        for (int k = tid; k < n; k += NT) {
+            int d = 0;
+            for (int i = min((int)m[k], top); i >= 1; i--) {
+                const int j = lv->nid[(size_t)(i - 1) * n + k];
+                if (j >= 0) {
+                    d = levstart[i] + j;
+                    break;
+                }
+            }
+            deep[k] = (IT)d;
+        }
+        __syncthreads();
+    }
     /* the level 1 nodes hang from the root */
     for (int c = p0 + tid; c < p1; c += NT) {
         T[c].parent = 0;
@@ -1097,42 +1187,48 @@ __global__ void __launch_bounds__(1024) k_deb_tree_glb(DebArgs a)
                               a.nodes + a.compoff[r]);
 }
 
-/* The same objects with the level components built in parallel: the per-pixel setup
-   (k_deb_glb_prep), then one workgroup per (object, level) builds the 8-connected
-   components of V_i = {m >= i} with their first pixels as roots and their sizes
-   (k_deb_glb_levels), then the tree reads them level by level (k_deb_glb_tree).  The
-   components of V_i do not depend on how they are found, so the tree is the one the
-   growing union-find gives. */
-__global__ void __launch_bounds__(1024) k_deb_glb_prep(DebArgs a)
+/* The same objects built level-parallel: the per-pixel setup (k_deb_glb_prep), then one
+   workgroup per (object, level i) finds the 8-connected components of V_i = {m >= i},
+   numbers the kept ones (at least deb_minarea pixels) in raster order of their first
+   pixels, adds their sums and settles their significance where the bounds decide
+   (k_deb_glb_levels), then the tree links the node records level by level
+   (k_deb_glb_tree).  The components of V_i do not depend on how they are found, so the
+   tree is the one the growing union-find gives. */
+__global__ void __launch_bounds__(1024) k_deb_glb_prep(DebArgs a, double *v0)
 {
     const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
     const int off = a.off[r];
+    LvArgs lv{};
+    lv.v0 = v0 + 2 * blockIdx.x;
     deb_tree<1024, true, int, 1>(a, r, a.g_cdv + off, a.g_m + off,
                                  reinterpret_cast<NbP<int> *>(a.g_nb) + off, a.g_par + off,
                                  a.g_cnt + off, a.g_deep + off, nullptr, a.fpia + off,
-                                 a.nodes + a.compoff[r]);
+                                 a.nodes + a.compoff[r], &lv);
 }
 
-__global__ void __launch_bounds__(1024) k_deb_glb_levels(DebArgs a, int *lvroot, int *lvcnt,
-                                                        float *frbuf)
+__global__ void __launch_bounds__(1024) k_deb_glb_levels(DebArgs a, int *lvpar, int *lvnid,
+                                                        LvNode *lvrec, int *lvk, float *frbuf,
+                                                        const double *v0)
 {
+    __shared__ int wsum[16];
     const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
     const int i = blockIdx.y + 1;
-    const int n = a.cnt[r], off = a.off[r], tid = threadIdx.x;
+    const int n = a.cnt[r], off = a.off[r], tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const float *cdv = a.g_cdv + off;
     if (i > a.lstar[r]) {
         /* the last level is never used (L* < nthresh): its workgroup computes the object's
            float flux, for the significance tests that the bounds leave open */
         if (i == a.nthresh && tid < 64) {
-            const float f = wave_ordered_sum(a.g_cdv + off, 0, n, [](int) { return true; });
+            const float f = wave_ordered_sum(cdv, 0, n, [](int) { return true; });
             if (tid == 0) frbuf[blockIdx.x] = f;
         }
         return;
     }
     const unsigned char *m = a.g_m + off;
     const NbP<int> *nb = reinterpret_cast<const NbP<int> *>(a.g_nb) + off;
-    int *par = lvroot + (size_t)(i - 1) * a.P0 + off;
-    int *cnt = lvcnt + (size_t)(i - 1) * a.P0 + off;
-    const int lane = tid & 63;
+    int *par = lvpar + a.lvbase[2 * r] + (size_t)(i - 1) * n;
+    int *cnt = lvnid + a.lvbase[2 * r] + (size_t)(i - 1) * n;
+    LvNode *rec = lvrec + a.lvbase[2 * r + 1] + (size_t)(i - 1) * (n / max(a.debarea, 1) + 1);
     /* every pixel of V_i starts at the head of its run inside its 64-entry window (runs
        of W neighbours in V_i), so the unions below only join runs */
     for (int b0 = 0; b0 < n; b0 += 1024) {
@@ -1210,18 +1306,114 @@ __global__ void __launch_bounds__(1024) k_deb_glb_levels(DebArgs a, int *lvroot,
             wave_add_by_key(cnt, act[u] ? x[u] : -1, 1);
         }
     }
+    __syncthreads();
+    /* the kept components in raster order of their first pixels: each thread numbers a
+       contiguous run of the list after one prefix over the threads' counts; a kept root's
+       parent word becomes -2 - its index, another root's -1 */
+    int nkept = 0;
+    {
+        const int per = (n + 1023) / 1024;
+        const int kb = tid * per, ke = min(kb + per, n);
+        auto load = [&](int k0, int *pk, int *ck) {
+#pragma unroll
+            for (int u = 0; u < DEB_RUN; u++)
+                pk[u] = k0 + u < ke && m[k0 + u] >= i ? ldw<true>(&par[k0 + u]) : -1;
+#pragma unroll
+            for (int u = 0; u < DEB_RUN; u++)
+                ck[u] = pk[u] == k0 + u ? ldw<true>(&cnt[k0 + u]) : 0;
+        };
+        int mine = 0;
+        for (int k0 = kb; k0 < ke; k0 += DEB_RUN) {
+            int pk[DEB_RUN], ck[DEB_RUN];
+            load(k0, pk, ck);
+#pragma unroll
+            for (int u = 0; u < DEB_RUN; u++) mine += pk[u] == k0 + u && ck[u] >= a.debarea;
+        }
+        int j = wg_scan_excl<1024>(mine, wsum, nkept);
+        for (int k0 = kb; k0 < ke; k0 += DEB_RUN) {
+            int pk[DEB_RUN], ck[DEB_RUN];
+            load(k0, pk, ck);
+#pragma unroll
+            for (int u = 0; u < DEB_RUN; u++)
+                if (pk[u] == k0 + u) {
+                    if (ck[u] >= a.debarea) {
+                        LvNode q;
+                        q.kroot = k0 + u; q.count = ck[u]; q.sig = 0; q.f = 0.0f;
+                        q.ssum = 0.0; q.sabs = 0.0;
+                        rec[j] = q;
+                        par[k0 + u] = -2 - j;
+                        j++;
+                    } else {
+                        par[k0 + u] = -1;
+                    }
+                }
+        }
+    }
+    if (tid == 0) lvk[(size_t)blockIdx.x * a.nthresh + i] = nkept;
+    __syncthreads();
+    /* the node index of every pixel of V_i (in the count row), the node sums when the
+       level keeps two nodes or more */
+    const bool sums = nkept >= 2;
+    for (int b0 = 0; b0 < n; b0 += DEB_ILP * 1024) {
+        int v[DEB_ILP], w[DEB_ILP];
+        bool in[DEB_ILP];
+#pragma unroll
+        for (int u = 0; u < DEB_ILP; u++) {
+            const int k = b0 + u * 1024 + tid;
+            in[u] = k < n && m[k] >= i;
+            v[u] = in[u] ? ldw<true>(&par[k]) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < DEB_ILP; u++) w[u] = v[u] >= 0 ? ldw<true>(&par[v[u]]) : v[u];
+#pragma unroll
+        for (int u = 0; u < DEB_ILP; u++) {
+            const int k = b0 + u * 1024 + tid;
+            const int nid = in[u] && w[u] <= -2 ? -2 - w[u] : -1;
+            if (in[u]) cnt[k] = nid;
+            if (sums) {
+                const float x = nid >= 0 ? cdv[k] : 0.0f;
+                wave_add2_by_key(rec, nid, (double)x, fabs((double)x));
+            }
+        }
+    }
+    if (!sums) return;
+    __syncthreads();
+    /* significance bounds, one wavefront per node; a node's exact float sum where they
+       cannot decide (its count pixels start at its first pixel) */
+    const float ti = a.thr[(size_t)r * a.nthresh + i];
+    const double v0lo = v0[2 * blockIdx.x], v0hi = v0[2 * blockIdx.x + 1];
+    for (int c = wid; c < nkept; c += 16) {
+        const int cc = ldw<true>(&rec[c].count), kr = ldw<true>(&rec[c].kroot);
+        const float b = ti * (float)cc;
+        const int sig = deb_sig_bound(ldw<true>(&rec[c].ssum), ldw<true>(&rec[c].sabs), cc, b,
+                                      v0lo, v0hi);
+        float f = 0.0f;
+        if (sig < 0)
+            f = wave_ordered_sum(
+                cdv, kr, n, [&](int k) { return m[k] >= i && ldw<true>(&cnt[k]) == c; }, cc);
+        if (lane == 0) {
+            rec[c].sig = sig;
+            rec[c].f = f;
+        }
+    }
 }
 
-__global__ void __launch_bounds__(1024) k_deb_glb_tree(DebArgs a, int *lvroot, int *lvcnt,
+__global__ void __launch_bounds__(1024) k_deb_glb_tree(DebArgs a, const int *lvnid,
+                                                      const LvNode *lvrec, const int *lvk,
                                                       const float *frbuf)
 {
     const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
-    const int off = a.off[r];
+    const int off = a.off[r], n = a.cnt[r];
+    LvArgs lv{};
+    lv.nid = lvnid + a.lvbase[2 * r];
+    lv.rec = lvrec + a.lvbase[2 * r + 1];
+    lv.nk = lvk + (size_t)blockIdx.x * a.nthresh;
+    lv.nstride = n / max(a.debarea, 1) + 1;
+    lv.fr = frbuf[blockIdx.x];
     deb_tree<1024, true, int, 2>(a, r, a.g_cdv + off, a.g_m + off,
                                  reinterpret_cast<NbP<int> *>(a.g_nb) + off, a.g_par + off,
                                  a.g_cnt + off, a.g_deep + off, nullptr, a.fpia + off,
-                                 a.nodes + a.compoff[r], lvroot + off, lvcnt + off,
-                                 frbuf + blockIdx.x);
+                                 a.nodes + a.compoff[r], &lv);
 }
 
 /* ------------------------------------------------------------------ objects ---- */
@@ -1524,10 +1716,11 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     int *lstar = sxg_take<int>("deb.lstar", nr);
     int *clist = sxg_take<int>("deb.clist", 4 * nr);
     int *ccount = sxg_take<int>("deb.ccount", 8);
-    auto *dwork = sxg_take<unsigned long long>("deb.dwork", 4);
+    auto *dwork = sxg_take<unsigned long long>("deb.dwork", 6);
+    auto *lvbase = sxg_take<unsigned long long>("deb.lvbase", 2 * (size_t)nr);
     int *kcomp = sxg_take<int>("deb.kcomp", (size_t)(P0 > 0 ? P0 : 1));
     if (!compoff || !dtot || !rootnc || !rootkept || !objbase || !alloc || !top1 || !top2 ||
-        !nmax || !lstar || !clist || !ccount || !dwork || !kcomp)
+        !nmax || !lstar || !clist || !ccount || !dwork || !kcomp || !lvbase)
         return -1;
     DebArgs a{};
     a.fpia = S.fpia; a.rawkey = S.fpia_keys; a.off = ro.off; a.cnt = ro.count; a.first = ro.first;
@@ -1537,7 +1730,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     a.W = S.W; a.H = S.H; a.nraw = nraw; a.compoff = compoff;
     a.rootnc = rootnc; a.rootkept = rootkept;
     a.top1 = top1; a.top2 = top2; a.nmax = nmax; a.lstar = lstar; a.clist = clist;
-    a.ccount = ccount; a.dwork = dwork;
+    a.ccount = ccount; a.dwork = dwork; a.lvbase = lvbase;
     S.deb_pixels = S.deb_labels = 0;
     int nobj = 0;
     if (nraw > 0) {
@@ -1568,7 +1761,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         sxg_kt_end(SXG_KT_DEB_LMAX);
         int cc[7] = {0, 0, 0, 0, 0, 0, 0};
         int ntot = 0;
-        unsigned long long dw[3] = {0, 0, 0};
+        unsigned long long dw[5] = {0, 0, 0, 0, 0};
         SXG_HIP_RET(hipMemcpyAsync(cc, ccount, sizeof(cc), hipMemcpyDeviceToHost, st), -1);
         SXG_HIP_RET(hipMemcpyAsync(&ntot, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
         SXG_HIP_RET(hipMemcpyAsync(dw, dwork, sizeof(dw), hipMemcpyDeviceToHost, st), -1);
@@ -1613,15 +1806,20 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
                 return -1;
             hipStream_t ks = use_side ? S.side[nside] : st;
             if (sxg_option(SXG_OPT_DEB_GLB_LEVELS)) {
-                int *lvroot = sxg_take<int>("deb.lvroot", (size_t)n_thresh * P0);
-                int *lvcnt = sxg_take<int>("deb.lvcnt", (size_t)n_thresh * P0);
+                const size_t npl = dw[3] > 0 ? (size_t)dw[3] : 1, nrec = dw[4] > 0 ? dw[4] : 1;
+                int *lvpar = sxg_take<int>("deb.lvpar", npl);
+                int *lvnid = sxg_take<int>("deb.lvnid", npl);
+                LvNode *lvrec = sxg_take<LvNode>("deb.lvrec", nrec);
+                int *lvk = sxg_take<int>("deb.lvk", (size_t)cc[3] * n_thresh);
                 float *frbuf = sxg_take<float>("deb.glbflux", (size_t)cc[3]);
-                if (!lvroot || !lvcnt || !frbuf) return -1;
-                hipLaunchKernelGGL(k_deb_glb_prep, dim3(cc[3]), dim3(1024), 0, ks, a);
+                double *v0 = sxg_take<double>("deb.glbv0", 2 * (size_t)cc[3]);
+                if (!lvpar || !lvnid || !lvrec || !lvk || !frbuf || !v0) return -1;
+                hipLaunchKernelGGL(k_deb_glb_prep, dim3(cc[3]), dim3(1024), 0, ks, a, v0);
                 hipLaunchKernelGGL(k_deb_glb_levels, dim3(cc[3], n_thresh), dim3(1024), 0, ks, a,
-                                   lvroot, lvcnt, frbuf);
-                hipLaunchKernelGGL(k_deb_glb_tree, dim3(cc[3]), dim3(1024), 0, ks, a, lvroot,
-                                   lvcnt, frbuf);
+                                   lvpar, lvnid, lvrec, lvk, frbuf, (const double *)v0);
+                hipLaunchKernelGGL(k_deb_glb_tree, dim3(cc[3]), dim3(1024), 0, ks, a,
+                                   (const int *)lvnid, (const LvNode *)lvrec, (const int *)lvk,
+                                   (const float *)frbuf);
             } else {
                 hipLaunchKernelGGL(k_deb_tree_glb, dim3(cc[3]), dim3(1024), 0, ks, a);
             }

@@ -320,6 +320,8 @@ struct DebArgs {
     long P0;
     const float *thr;       /* thr[r * nthresh + i] */
     int nthresh, debarea;
+    int lvmin;              /* global-memory objects of at least this many pixels are built
+                               level-parallel (0: none) */
     double mincont;
     int W, H, nraw;
     Comp *comps;            /* output node table, region compoff[r] of each raw object */
@@ -333,7 +335,8 @@ struct DebArgs {
     int *lstar;             /* first level above the second-highest maximum, then L* */
     int *clist;             /* objects of each size class: clist[cls * nraw + j] */
     int *ccount;            /* [0..3] objects per class, [4..6] the largest LDS need of
-                               classes 0..2 (bytes), [7] consistency error flag */
+                               classes 0..2 (bytes), [7] consistency error flag, [8] class 3
+                               objects built level-parallel */
     unsigned long long *dwork;  /* [0] pixels, [1] pixel-levels of the multi-peak objects,
                                    [2] node capacity of the frame, [3] pixel-levels and
                                    [4] node records of the objects built in global memory */
@@ -380,7 +383,7 @@ template <int PASS> __global__ void __launch_bounds__(256) k_deb_lmax(DebArgs a)
 __global__ void __launch_bounds__(256) k_deb_prep(DebArgs a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r < 8) a.ccount[r] = 0;
+    if (r < 9) a.ccount[r] = 0;
     if (r < 5) a.dwork[r] = 0ull;
     if (r >= a.nraw) return;
     a.top1[r] = 0ull;
@@ -459,7 +462,8 @@ __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int c
             if (cls == 2 && need > bud2) cls = 3;
             /* the per-level arrays of the global-memory trees: n entries and
                n / deb_minarea + 1 node records for each level 1..L* */
-            if (cls == 3) {
+            if (cls == 3 && a.lvmin > 0 && n >= a.lvmin) {
+                atomicAdd(&a.ccount[8], 1);
                 a.lvbase[2 * r] = atomicAdd(&a.dwork[3], (unsigned long long)n * ls);
                 a.lvbase[2 * r + 1] = atomicAdd(
                     &a.dwork[4], (unsigned long long)ls * (n / max(a.debarea, 1) + 1));
@@ -1177,14 +1181,76 @@ __global__ void __launch_bounds__(NT) k_deb_tree_lds(DebArgs a, int cls)
 
 /* raw objects too large for LDS or with trees that may outgrow it: all state in global
    memory */
-__global__ void __launch_bounds__(1024) k_deb_tree_glb(DebArgs a)
+__global__ void __launch_bounds__(1024) k_deb_tree_glb(DebArgs a, int j0)
 {
-    const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
+    const int r = a.clist[3 * (size_t)a.nraw + j0 + blockIdx.x];
     const int off = a.off[r];
     deb_tree<1024, true, int>(a, r, a.g_cdv + off, a.g_m + off,
                               reinterpret_cast<NbP<int> *>(a.g_nb) + off, a.g_par + off,
                               a.g_cnt + off, a.g_deep + off, nullptr, a.fpia + off,
                               a.nodes + a.compoff[r]);
+}
+
+/* the objects of class cls, largest first (ties in list order), into out: a class's
+   kernel then starts its longest trees first and ends sooner; the results do not depend
+   on the order */
+__global__ void __launch_bounds__(256) k_deb_order(const int *__restrict__ clist,
+                                                   const int *__restrict__ cnt, int nraw,
+                                                   int cls, int nc, int *__restrict__ out)
+{
+    __shared__ int sn[256];
+    const int *L = clist + (size_t)cls * nraw;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int nj = j < nc ? cnt[L[j]] : 0;
+    int rank = 0;
+    for (int t0 = 0; t0 < nc; t0 += 256) {
+        __syncthreads();
+        sn[threadIdx.x] = t0 + (int)threadIdx.x < nc ? cnt[L[t0 + threadIdx.x]] : -1;
+        __syncthreads();
+        const int tn = min(256, nc - t0);
+        for (int u = 0; u < tn; u++) {
+            const int x = sn[u];
+            rank += x > nj || (x == nj && t0 + u < j);
+        }
+    }
+    if (j < nc) out[(size_t)cls * nraw + rank] = L[j];
+}
+
+/* float sum of v[0..n) in increasing index order by thread 0, the values staged through
+   LDS in 4096-entry chunks by the other wavefronts while it adds the previous chunk
+   (thread 0's return value is the sum) */
+__device__ inline float block_serial_sum(const float *v, int n, float (*buf)[4096])
+{
+    const int tid = threadIdx.x, NS = blockDim.x - 64;
+    const int nch = (n + 4095) / 4096;
+    auto stage = [&](int c) {
+        if (tid >= 64)
+            for (int t = tid - 64; t < 4096; t += NS) {
+                const int k = c * 4096 + t;
+                buf[c & 1][t] = k < n ? v[k] : 0.0f;
+            }
+    };
+    float f = 0.0f;
+    stage(0);
+    __syncthreads();
+    for (int c = 0; c < nch; c++) {
+        if (c + 1 < nch) stage(c + 1);
+        if (tid == 0) {
+            const int e = min(4096, n - c * 4096);
+            const float4 *q = reinterpret_cast<const float4 *>(buf[c & 1]);
+            int t = 0;
+            for (; t + 4 <= e; t += 4) {
+                const float4 x = q[t >> 2];
+                f += x.x;
+                f += x.y;
+                f += x.z;
+                f += x.w;
+            }
+            for (; t < e; t++) f += buf[c & 1][t];
+        }
+        __syncthreads();
+    }
+    return f;
 }
 
 /* The same objects built level-parallel: the per-pixel setup (k_deb_glb_prep), then one
@@ -1211,19 +1277,20 @@ __global__ void __launch_bounds__(1024) k_deb_glb_levels(DebArgs a, int *lvpar, 
                                                         const double *v0)
 {
     __shared__ int wsum[16];
-    const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
-    const int i = blockIdx.y + 1;
+    __shared__ __align__(16) float sbuf[2][4096];
+    /* blockIdx.y: the object (largest first); blockIdx.x: 0 computes the object's float
+       flux (for the significance tests the bounds leave open), x >= 1 builds level x */
+    const int oj = blockIdx.y;
+    const int r = a.clist[3 * (size_t)a.nraw + oj];
+    const int i = blockIdx.x;
     const int n = a.cnt[r], off = a.off[r], tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const float *cdv = a.g_cdv + off;
-    if (i > a.lstar[r]) {
-        /* the last level is never used (L* < nthresh): its workgroup computes the object's
-           float flux, for the significance tests that the bounds leave open */
-        if (i == a.nthresh && tid < 64) {
-            const float f = wave_ordered_sum(cdv, 0, n, [](int) { return true; });
-            if (tid == 0) frbuf[blockIdx.x] = f;
-        }
+    if (i == 0) {
+        const float f = block_serial_sum(cdv, n, sbuf);
+        if (tid == 0) frbuf[oj] = f;
         return;
     }
+    if (i > a.lstar[r]) return;
     const unsigned char *m = a.g_m + off;
     const NbP<int> *nb = reinterpret_cast<const NbP<int> *>(a.g_nb) + off;
     int *par = lvpar + a.lvbase[2 * r] + (size_t)(i - 1) * n;
@@ -1349,7 +1416,7 @@ __global__ void __launch_bounds__(1024) k_deb_glb_levels(DebArgs a, int *lvpar, 
                 }
         }
     }
-    if (tid == 0) lvk[(size_t)blockIdx.x * a.nthresh + i] = nkept;
+    if (tid == 0) lvk[(size_t)oj * a.nthresh + i] = nkept;
     __syncthreads();
     /* the node index of every pixel of V_i (in the count row), the node sums when the
        level keeps two nodes or more */
@@ -1381,7 +1448,7 @@ __global__ void __launch_bounds__(1024) k_deb_glb_levels(DebArgs a, int *lvpar, 
     /* significance bounds, one wavefront per node; a node's exact float sum where they
        cannot decide (its count pixels start at its first pixel) */
     const float ti = a.thr[(size_t)r * a.nthresh + i];
-    const double v0lo = v0[2 * blockIdx.x], v0hi = v0[2 * blockIdx.x + 1];
+    const double v0lo = v0[2 * oj], v0hi = v0[2 * oj + 1];
     for (int c = wid; c < nkept; c += 16) {
         const int cc = ldw<true>(&rec[c].count), kr = ldw<true>(&rec[c].kroot);
         const float b = ti * (float)cc;
@@ -1715,7 +1782,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     int *nmax = sxg_take<int>("deb.nmax", nr);
     int *lstar = sxg_take<int>("deb.lstar", nr);
     int *clist = sxg_take<int>("deb.clist", 4 * nr);
-    int *ccount = sxg_take<int>("deb.ccount", 8);
+    int *ccount = sxg_take<int>("deb.ccount", 9);
     auto *dwork = sxg_take<unsigned long long>("deb.dwork", 6);
     auto *lvbase = sxg_take<unsigned long long>("deb.lvbase", 2 * (size_t)nr);
     int *kcomp = sxg_take<int>("deb.kcomp", (size_t)(P0 > 0 ? P0 : 1));
@@ -1727,6 +1794,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     a.cd = S.cd.as<float>(); a.lab = S.lab.as<int>();
     a.kcomp = kcomp; a.P0 = P0; a.thr = sxg_take<float>("deb.thr", nr * n_thresh);
     a.nthresh = n_thresh; a.debarea = deb_minarea; a.mincont = deblend_mincont;
+    a.lvmin = (int)std::min<long>(sxg_option(SXG_OPT_DEB_GLB_LEVELS), 0x7fffffffL);
     a.W = S.W; a.H = S.H; a.nraw = nraw; a.compoff = compoff;
     a.rootnc = rootnc; a.rootkept = rootkept;
     a.top1 = top1; a.top2 = top2; a.nmax = nmax; a.lstar = lstar; a.clist = clist;
@@ -1759,7 +1827,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         hipLaunchKernelGGL(k_deb_classify, dim3(gr), dim3(256), 0, st, a, cap0, cap1, cap2, ct0,
                            ct1, ct2);
         sxg_kt_end(SXG_KT_DEB_LMAX);
-        int cc[7] = {0, 0, 0, 0, 0, 0, 0};
+        int cc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         int ntot = 0;
         unsigned long long dw[5] = {0, 0, 0, 0, 0};
         SXG_HIP_RET(hipMemcpyAsync(cc, ccount, sizeof(cc), hipMemcpyDeviceToHost, st), -1);
@@ -1804,28 +1872,46 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             a.g_cnt = sxg_take<int>("deb.g_cnt", (size_t)P0);
             if (!a.nodes || !a.g_cdv || !a.g_m || !a.g_nb || !a.g_par || !a.g_deep || !a.g_cnt)
                 return -1;
+            /* class 3 largest first: the objects of at least lvmin pixels (cc[8]) are
+               built level-parallel, the others one workgroup each, on a stream of their
+               own; both read the ordered list */
+            int *clist3 = sxg_take<int>("deb.clist3", 4 * nr);
+            if (!clist3) return -1;
+            DebArgs a3 = a;
+            a3.clist = clist3;
             hipStream_t ks = use_side ? S.side[nside] : st;
-            if (sxg_option(SXG_OPT_DEB_GLB_LEVELS)) {
+            hipLaunchKernelGGL(k_deb_order, dim3(sxg_div_up(cc[3], 256)), dim3(256), 0, ks,
+                               (const int *)a.clist, (const int *)a.cnt, nraw, 3, cc[3], clist3);
+            const int big = cc[8] <= 65535 ? cc[8] : 0, rest = cc[3] - big;
+            if (rest > 0) {
+                hipStream_t ks2 = ks;
+                if (use_side) {
+                    ks2 = S.side[nside + 1];
+                    SXG_HIP_RET(hipEventRecord(S.side_ev[nside], ks), -1);
+                    SXG_HIP_RET(hipStreamWaitEvent(ks2, S.side_ev[nside], 0), -1);
+                }
+                hipLaunchKernelGGL(k_deb_tree_glb, dim3(rest), dim3(1024), 0, ks2, a3, big);
+                if (use_side) SXG_HIP_RET(hipEventRecord(S.side_ev[nside + 1], ks2), -1);
+            }
+            if (big > 0) {
                 const size_t npl = dw[3] > 0 ? (size_t)dw[3] : 1, nrec = dw[4] > 0 ? dw[4] : 1;
                 int *lvpar = sxg_take<int>("deb.lvpar", npl);
                 int *lvnid = sxg_take<int>("deb.lvnid", npl);
                 LvNode *lvrec = sxg_take<LvNode>("deb.lvrec", nrec);
-                int *lvk = sxg_take<int>("deb.lvk", (size_t)cc[3] * n_thresh);
-                float *frbuf = sxg_take<float>("deb.glbflux", (size_t)cc[3]);
-                double *v0 = sxg_take<double>("deb.glbv0", 2 * (size_t)cc[3]);
+                int *lvk = sxg_take<int>("deb.lvk", (size_t)big * n_thresh);
+                float *frbuf = sxg_take<float>("deb.glbflux", (size_t)big);
+                double *v0 = sxg_take<double>("deb.glbv0", 2 * (size_t)big);
                 if (!lvpar || !lvnid || !lvrec || !lvk || !frbuf || !v0) return -1;
-                hipLaunchKernelGGL(k_deb_glb_prep, dim3(cc[3]), dim3(1024), 0, ks, a, v0);
-                hipLaunchKernelGGL(k_deb_glb_levels, dim3(cc[3], n_thresh), dim3(1024), 0, ks, a,
+                hipLaunchKernelGGL(k_deb_glb_prep, dim3(big), dim3(1024), 0, ks, a3, v0);
+                hipLaunchKernelGGL(k_deb_glb_levels, dim3(n_thresh, big), dim3(1024), 0, ks, a3,
                                    lvpar, lvnid, lvrec, lvk, frbuf, (const double *)v0);
-                hipLaunchKernelGGL(k_deb_glb_tree, dim3(cc[3]), dim3(1024), 0, ks, a,
+                hipLaunchKernelGGL(k_deb_glb_tree, dim3(big), dim3(1024), 0, ks, a3,
                                    (const int *)lvnid, (const LvNode *)lvrec, (const int *)lvk,
                                    (const float *)frbuf);
-            } else {
-                hipLaunchKernelGGL(k_deb_tree_glb, dim3(cc[3]), dim3(1024), 0, ks, a);
             }
             if (use_side) {
                 SXG_HIP_RET(hipEventRecord(S.side_ev[nside], ks), -1);
-                nside++;
+                nside += rest > 0 ? 2 : 1;
             }
         }
         if (cc[2] > 0 &&

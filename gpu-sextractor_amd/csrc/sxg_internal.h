@@ -85,7 +85,7 @@ __device__ inline float sxg_pix_thresh(double k, float var) { return (float)(k *
 
 enum StageId { ST_BACK = 0, ST_FILTER, ST_DETECT, ST_DEBLEND, ST_ANALYSE, ST_CLEAN, ST_N };
 
-#define SXG_NSIDE 3
+#define SXG_NSIDE 4
 struct SxgState {
     int W = 0, H = 0;
     size_t N = 0;

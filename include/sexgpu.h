@@ -316,9 +316,11 @@ int sxg_kernel_times(int id, double *total_ms, int *launches);
                             need more is built in global memory
      SXG_OPT_SIDE_STREAMS   1 (default): the deblending size classes of a frame run side by
                             side on streams of their own; 0: all on the context's stream
-     SXG_OPT_DEB_GLB_LEVELS 1 (default): the trees built in global memory get the components
-                            of all their levels from one workgroup per level, in parallel;
-                            0: one growing union-find per tree, level after level
+     SXG_OPT_DEB_GLB_LEVELS the raw objects built in global memory with at least this many
+                            pixels (default 12288; at most 2^30) get the components of all
+                            their levels from one workgroup per level, in parallel; the
+                            others, and all of them at 0, one growing union-find per tree,
+                            level after level
    This one changes the catalogue (never the segmentation map, the object list or the
    CLEAN decisions):
      SXG_OPT_PRECISION      arithmetic of FLUX_ISO, X/Y, X2/Y2/XY, A/B/THETA, CXX/CYY/CXY:

@@ -114,10 +114,12 @@ KNOBS = ("deb_lds_tiny", "deb_lds_small", "deb_lds_large", "deb_table", "deb_glb
 @pytest.mark.parametrize("caps", [(0, 0, 0, 0, 1), (0, 0, 0, 0, 0), (0, 64, 256, 0, 1),
                                   (0, 6400, 6400, 0, 1), (256, 1024, 6144, 0, 1),
                                   (6400, 6400, 6400, 0, 1), (256, 1024, 6144, 1, 1),
-                                  (256, 1024, 6144, 5, 1), (0, 0, 0, 3, 1), (0, 0, 0, 3, 0)])
+                                  (256, 1024, 6144, 5, 1), (0, 0, 0, 3, 1), (0, 0, 0, 3, 0),
+                                  (0, 64, 256, 0, 2000)])
 def test_deblend_kernel_variants(masks, caps):
     """Every deblending kernel variant (LDS 64-, 256- and 1024-lane workgroups, global
-    scratch with the level components built level by level or all levels in parallel;
+    scratch with the level components built level by level or, from a size on, all
+    levels in parallel;
     the tree's node table in LDS or moved to global memory when it outgrows its LDS
     share) gives the oracle's result: the options move raw objects between them."""
     img = sexgpu.synth("crowded4096", width=1024, height=1024, nsrc=167000 // 16, seed=1201)

@@ -306,8 +306,8 @@ int sxg_kernel_times(int id, double *total_ms, int *launches);
 /* Options (no counterpart in the reference).  These never change a result: they only
    choose which kernel variant handles which work.
      SXG_OPT_DEB_LDS_TINY   raw objects (with two or more local maxima) up to this many
-                            pixels are deblended by 64-lane workgroups in LDS (default 256)
-     SXG_OPT_DEB_LDS_SMALL  up to this many by 256-lane workgroups in LDS (default 1024)
+                            pixels are deblended by 256-lane workgroups in LDS (default 512)
+     SXG_OPT_DEB_LDS_SMALL  up to this many by 1024-lane workgroups in LDS (default 1024)
      SXG_OPT_DEB_LDS_LARGE  up to this many by 1024-lane workgroups in LDS (default 6144,
                             at most 6400); larger ones use global scratch memory
      SXG_OPT_DEB_TABLE      deblending-tree nodes an LDS workgroup can hold (0: 192, 512

@@ -117,7 +117,7 @@ KNOBS = ("deb_lds_tiny", "deb_lds_small", "deb_lds_large", "deb_table", "deb_glb
                                   (256, 1024, 6144, 5, 1), (0, 0, 0, 3, 1), (0, 0, 0, 3, 0),
                                   (0, 64, 256, 0, 2000)])
 def test_deblend_kernel_variants(masks, caps):
-    """Every deblending kernel variant (LDS 64-, 256- and 1024-lane workgroups, global
+    """Every deblending kernel variant (LDS 256- and 1024-lane workgroups, global
     scratch with the level components built level by level or, from a size on, all
     levels in parallel;
     the tree's node table in LDS or moved to global memory when it outgrows its LDS

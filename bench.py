@@ -94,7 +94,7 @@ ALG_BYTES = {
 # 1.0); see profiles/hbm_traffic_crowded4096_pmc.md.  Used for roofline.traffic on that
 # workload only.
 MEASURED_TRAFFIC_CROWDED4096 = {
-    "deb_levels": 1.273e7, "meshstat": 1.345e8, "meshmode": 6.858e7, "analyse": 6.635e7,
+    "deb_levels": 1.271e7, "meshstat": 1.346e8, "meshmode": 6.857e7, "analyse": 6.633e7,
     "conv": 2.003e8, "subback": 1.51e8, "det_init": 2.013e8, "det_union": 7.577e7,
     "copy": 1.342e8, "clean": 1.08e7, "rownodes": 2.26e6,
 }

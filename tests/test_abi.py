@@ -80,6 +80,17 @@ def test_options_without_device():
     assert lib.sxg_get_option(99) == -1
     with pytest.raises(ValueError):
         sexgpu.set_option("deb_lds_large", 100000)
+    # the level-parallel size: 0 (off) up to 2^30 pixels
+    old = sexgpu.get_option("deb_glb_levels")
+    try:
+        assert old == 12288
+        sexgpu.set_option("deb_glb_levels", 0)
+        sexgpu.set_option("deb_glb_levels", 1 << 30)
+        assert sexgpu.get_option("deb_glb_levels") == 1 << 30
+        with pytest.raises(ValueError):
+            sexgpu.set_option("deb_glb_levels", (1 << 30) + 1)
+    finally:
+        sexgpu.set_option("deb_glb_levels", old)
 
 
 def test_outobj_layout():

@@ -336,7 +336,7 @@ struct DebArgs {
     int *clist;             /* objects of each size class: clist[cls * nraw + j] */
     int *ccount;            /* [0..3] objects per class, [4..6] the largest LDS need of
                                classes 0..2 (bytes), [7] consistency error flag, [8] class 3
-                               objects built level-parallel */
+                               objects built level-parallel, [9] order counter */
     unsigned long long *dwork;  /* [0] pixels, [1] pixel-levels of the multi-peak objects,
                                    [2] node capacity of the frame, [3] pixel-levels and
                                    [4] node records of the objects built in global memory */
@@ -383,7 +383,7 @@ template <int PASS> __global__ void __launch_bounds__(256) k_deb_lmax(DebArgs a)
 __global__ void __launch_bounds__(256) k_deb_prep(DebArgs a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r < 9) a.ccount[r] = 0;
+    if (r < 10) a.ccount[r] = 0;
     if (r < 5) a.dwork[r] = 0ull;
     if (r >= a.nraw) return;
     a.top1[r] = 0ull;
@@ -1191,29 +1191,39 @@ __global__ void __launch_bounds__(1024) k_deb_tree_glb(DebArgs a, int j0)
                               a.nodes + a.compoff[r]);
 }
 
-/* the objects of class cls, largest first (ties in list order), into out: a class's
-   kernel then starts its longest trees first and ends sooner; the results do not depend
-   on the order */
+/* the objects of class 3 into out, largest first (ties in list order), so the kernels
+   start their longest trees first: with full, all of them (O(nc^2) comparisons);
+   otherwise those of at least lvmin pixels (the level-parallel ones, nbig of them) in
+   that order, then the others in any order from the counter *ctr (zero on entry),
+   O(nbig * nc).  Either way the level-parallel objects come first.  The results do
+   not depend on the order. */
 __global__ void __launch_bounds__(256) k_deb_order(const int *__restrict__ clist,
                                                    const int *__restrict__ cnt, int nraw,
-                                                   int cls, int nc, int *__restrict__ out)
+                                                   int nc, int lvmin, int nbig, int full,
+                                                   int *ctr, int *__restrict__ out)
 {
     __shared__ int sn[256];
-    const int *L = clist + (size_t)cls * nraw;
+    const int *L = clist + 3 * (size_t)nraw;
+    int *O = out + 3 * (size_t)nraw;
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int nj = j < nc ? cnt[L[j]] : 0;
+    const bool big = j < nc && (full || (lvmin > 0 && nj >= lvmin));
+    if (j < nc && !big) O[nbig + atomicAdd(ctr, 1)] = L[j];
+    if (!__syncthreads_or(big)) return;
     int rank = 0;
     for (int t0 = 0; t0 < nc; t0 += 256) {
         __syncthreads();
         sn[threadIdx.x] = t0 + (int)threadIdx.x < nc ? cnt[L[t0 + threadIdx.x]] : -1;
         __syncthreads();
-        const int tn = min(256, nc - t0);
-        for (int u = 0; u < tn; u++) {
-            const int x = sn[u];
-            rank += x > nj || (x == nj && t0 + u < j);
+        if (big) {
+            const int tn = min(256, nc - t0);
+            for (int u = 0; u < tn; u++) {
+                const int x = sn[u];
+                rank += x > nj || (x == nj && t0 + u < j);
+            }
         }
     }
-    if (j < nc) out[(size_t)cls * nraw + rank] = L[j];
+    if (big) O[rank] = L[j];
 }
 
 /* float sum of v[0..n) in increasing index order by thread 0, the values staged through
@@ -1782,7 +1792,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     int *nmax = sxg_take<int>("deb.nmax", nr);
     int *lstar = sxg_take<int>("deb.lstar", nr);
     int *clist = sxg_take<int>("deb.clist", 4 * nr);
-    int *ccount = sxg_take<int>("deb.ccount", 9);
+    int *ccount = sxg_take<int>("deb.ccount", 10);
     auto *dwork = sxg_take<unsigned long long>("deb.dwork", 6);
     auto *lvbase = sxg_take<unsigned long long>("deb.lvbase", 2 * (size_t)nr);
     int *kcomp = sxg_take<int>("deb.kcomp", (size_t)(P0 > 0 ? P0 : 1));
@@ -1880,9 +1890,12 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             DebArgs a3 = a;
             a3.clist = clist3;
             hipStream_t ks = use_side ? S.side[nside] : st;
-            hipLaunchKernelGGL(k_deb_order, dim3(sxg_div_up(cc[3], 256)), dim3(256), 0, ks,
-                               (const int *)a.clist, (const int *)a.cnt, nraw, 3, cc[3], clist3);
             const int big = cc[8] <= 65535 ? cc[8] : 0, rest = cc[3] - big;
+            hipLaunchKernelGGL(k_deb_order, dim3(sxg_div_up(cc[3], 256)), dim3(256), 0, ks,
+                               (const int *)a.clist, (const int *)a.cnt, nraw, cc[3],
+                               big > 0 ? a.lvmin : 0, big,
+                               cc[3] <= sxg_option(SXG_OPT_DEB_ORDER_FULL) ? 1 : 0, ccount + 9,
+                               clist3);
             if (rest > 0) {
                 hipStream_t ks2 = ks;
                 if (use_side) {

@@ -370,7 +370,8 @@ KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", 
 # options (sxg_option_id): the size caps choose kernel variants and never change a result;
 # "precision" chooses the arithmetic of the graded catalogue columns
 OPTIONS = {"deb_lds_small": 0, "deb_lds_large": 1, "precision": 2, "deb_lds_tiny": 3,
-           "deb_table": 4, "side_streams": 5, "deb_glb_levels": 6}
+           "deb_table": 4, "side_streams": 5, "deb_glb_levels": 6,
+           "deb_order_full": 7}
 
 
 def set_option(name, value):

@@ -321,6 +321,9 @@ int sxg_kernel_times(int id, double *total_ms, int *launches);
                             their levels from one workgroup per level, in parallel; the
                             others, and all of them at 0, one growing union-find per tree,
                             level after level
+     SXG_OPT_DEB_ORDER_FULL the global-memory objects are launched largest first when there
+                            are at most this many of them (default 4096); beyond, only the
+                            level-parallel ones are ordered, the others follow in any order
    This one changes the catalogue (never the segmentation map, the object list or the
    CLEAN decisions):
      SXG_OPT_PRECISION      arithmetic of FLUX_ISO, X/Y, X2/Y2/XY, A/B/THETA, CXX/CYY/CXY:
@@ -328,7 +331,7 @@ int sxg_kernel_times(int id, double *total_ms, int *launches);
                             2 compensated FP32 sums with FP64 central moments */
 enum sxg_option_id { SXG_OPT_DEB_LDS_SMALL = 0, SXG_OPT_DEB_LDS_LARGE, SXG_OPT_PRECISION,
                      SXG_OPT_DEB_LDS_TINY, SXG_OPT_DEB_TABLE, SXG_OPT_SIDE_STREAMS,
-                     SXG_OPT_DEB_GLB_LEVELS, SXG_NOPT };
+                     SXG_OPT_DEB_GLB_LEVELS, SXG_OPT_DEB_ORDER_FULL, SXG_NOPT };
 /* Returns 0, or -1 for an unknown option or a value out of range. */
 int sxg_set_option(int id, long value);
 long sxg_get_option(int id);

@@ -108,18 +108,21 @@ def test_catalog_rows_match_columns(frame1024, masks):
     assert np.array_equal(fe, np.sqrt(g["fluxerr"].astype(np.float64)).astype(np.float32))
 
 
-KNOBS = ("deb_lds_tiny", "deb_lds_small", "deb_lds_large", "deb_table", "deb_glb_levels")
+KNOBS = ("deb_lds_tiny", "deb_lds_small", "deb_lds_large", "deb_table", "deb_glb_levels",
+         "deb_order_full")
 
 
-@pytest.mark.parametrize("caps", [(0, 0, 0, 0, 1), (0, 0, 0, 0, 0), (0, 64, 256, 0, 1),
-                                  (0, 6400, 6400, 0, 1), (256, 1024, 6144, 0, 1),
-                                  (6400, 6400, 6400, 0, 1), (256, 1024, 6144, 1, 1),
-                                  (256, 1024, 6144, 5, 1), (0, 0, 0, 3, 1), (0, 0, 0, 3, 0),
-                                  (0, 64, 256, 0, 2000)])
+@pytest.mark.parametrize("caps", [(0, 0, 0, 0, 1, 4096), (0, 0, 0, 0, 0, 4096),
+                                  (0, 64, 256, 0, 1, 4096), (0, 6400, 6400, 0, 1, 4096),
+                                  (256, 1024, 6144, 0, 1, 4096), (6400, 6400, 6400, 0, 1, 4096),
+                                  (256, 1024, 6144, 1, 1, 4096), (256, 1024, 6144, 5, 1, 4096),
+                                  (0, 0, 0, 3, 1, 4096), (0, 0, 0, 3, 0, 4096),
+                                  (0, 64, 256, 0, 2000, 4096), (0, 64, 256, 0, 2000, 0),
+                                  (0, 0, 0, 0, 1, 0), (0, 0, 0, 0, 0, 0)])
 def test_deblend_kernel_variants(masks, caps):
     """Every deblending kernel variant (LDS 256- and 1024-lane workgroups, global
     scratch with the level components built level by level or, from a size on, all
-    levels in parallel;
+    levels in parallel, launched largest first or only the level-parallel ones ordered;
     the tree's node table in LDS or moved to global memory when it outgrows its LDS
     share) gives the oracle's result: the options move raw objects between them."""
     img = sexgpu.synth("crowded4096", width=1024, height=1024, nsrc=167000 // 16, seed=1201)

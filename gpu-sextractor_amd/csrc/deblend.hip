@@ -45,12 +45,17 @@
  *                    node table's regions
  *   k_deb_classify   one lane per object: single-peak objects are finished; the rest are
  *                    binned by size (and node bound) into LDS classes
- *   k_deb_tree_lds   one workgroup per object (64, 256 or 1024 lanes by size; the classes
+ *   k_deb_tree_lds   one workgroup per object (256 or 1024 lanes by size; the classes
  *                    run side by side on their own streams): cd, m(p), neighbours,
  *                    union-find parents, sizes and the node table in LDS; the tree is
  *                    built from the top level down with one union-find that only grows
  *                    (see deb_tree)
- *   k_deb_tree_glb   the same code on global memory for the largest objects
+ *   k_deb_order      the objects too large for LDS, largest first
+ *   k_deb_tree_glb   the same code as k_deb_tree_lds on global memory, for those of them
+ *                    below SXG_OPT_DEB_GLB_LEVELS pixels
+ *   k_deb_glb_prep,  the larger ones level-parallel: per-pixel setup; one workgroup per
+ *   k_deb_glb_levels (object, level) finding the components of V_i by (1), numbering,
+ *   k_deb_glb_tree   summing and testing the kept ones; the tree linking their records
  *   k_deb_objlist    one wavefront per raw object: its kept nodes -> objects
  *   k_deb_pre        one wavefront per object: FULL preanalysis over the node's level
  *                    pixels (kcomp names them; lanes compute the terms, the raster-order

@@ -178,7 +178,9 @@ def batch64(args):
     hs = (C.c_int * B)(*[f.shape[0] for f in frames])
     if L.sxg_batch_set_frames(B, ptrs, ws, hs) != 0:
         sexgpu.check()
-    sweep = [int(v) for v in args.sweep.split(",")] if args.sweep else [1, args.workers]
+    # small frames are bound by host round trips: twice the workers can pay off here
+    sweep = ([int(v) for v in args.sweep.split(",")] if args.sweep
+             else [1, args.workers, 2 * args.workers])
     res = {}
     ref = None
     for nw in sweep:

@@ -139,6 +139,26 @@ def test_deblend_kernel_variants(masks, caps):
     assert (g["level"] > 0).sum() > 20
 
 
+def test_dense_blends_default_paths(masks):
+    """A 2048^2 frame of the dense regime (fluxes from 3000 ADU; blends of 10k-20k pixels)
+    gives the oracle's result with the default options, where the largest trees are built
+    one workgroup per level and the other global-memory trees one workgroup each, and
+    with every global-memory tree built one workgroup each."""
+    img = sexgpu.synth("crowded4096", width=2048, height=2048, nsrc=167000 // 4, seed=1301,
+                       fmin=3000.0)
+    m = masks["gauss_3p0_7x7"]
+    ref = _oracle(img, m)
+    lvmin = sexgpu.get_option("deb_glb_levels")
+    sizes = np.asarray(ref["raw_count"])
+    assert (sizes >= lvmin).sum() >= 4 and ((sizes > 6144) & (sizes < lvmin)).sum() >= 10
+    try:
+        for v in (lvmin, 0):
+            sexgpu.set_option("deb_glb_levels", v)
+            compare(sexgpu.run_frame(img, _params(m)), ref)
+    finally:
+        sexgpu.set_option("deb_glb_levels", lvmin)
+
+
 def test_option_range():
     with pytest.raises(ValueError):
         sexgpu.set_option("deb_lds_large", 7000)

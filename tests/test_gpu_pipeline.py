@@ -139,6 +139,24 @@ def test_deblend_kernel_variants(masks, caps):
     assert (g["level"] > 0).sum() > 20
 
 
+@pytest.mark.parametrize("kw", [dict(ext_minarea=1), dict(deblend_nthresh=64),
+                                dict(deblend_mincont=0.0)])
+def test_level_parallel_parameter_variants(frame1024, masks, kw):
+    """Every multi-peak object built level-parallel (one workgroup per object and level)
+    under the parameters that stress that build: one-pixel nodes (as many node records
+    as pixels per level), 64 levels, every node significant."""
+    m = masks["default"]
+    old = [sexgpu.get_option(k) for k in KNOBS]
+    try:
+        for k, v in zip(KNOBS, (0, 0, 0, 0, 1, 4096)):
+            sexgpu.set_option(k, v)
+        g = sexgpu.run_frame(frame1024, _params(m, **kw))
+    finally:
+        for k, v in zip(KNOBS, old):
+            sexgpu.set_option(k, v)
+    compare(g, _oracle(frame1024, m, **kw))
+
+
 def test_dense_blends_default_paths(masks):
     """A 2048^2 frame of the dense regime (fluxes from 3000 ADU; blends of 10k-20k pixels)
     gives the oracle's result with the default options, where the largest trees are built

@@ -153,19 +153,10 @@ def batch_segs(L, frames):
     return out
 
 
-BATCH64_SIZES = [(512, 512), (1024, 1024), (1024, 2048), (2048, 2048), (2048, 4096)]
-
-
 def batch64(args):
     """The many-frame workload: aggregate throughput and per-frame latency."""
-    rng = np.random.default_rng(2000)
-    frames = []
     t = time.perf_counter()
-    for i in range(64):
-        h, w = BATCH64_SIZES[int(rng.integers(0, len(BATCH64_SIZES)))]
-        dens = float(10 ** rng.uniform(-4, -2))
-        frames.append(sexgpu.synth("plain1024", width=w, height=h,
-                                   nsrc=max(1, int(round(dens * w * h))), seed=2000 + i))
+    frames = sexgpu.batch64_frames()
     npx = sum(f.size for f in frames)
     print("# 64 frames, %.1f Mpix in all, generated in %.1f s" % (npx / 1e6, time.perf_counter() - t),
           file=sys.stderr, flush=True)

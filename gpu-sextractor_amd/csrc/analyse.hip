@@ -379,7 +379,7 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
     }
     AnaArgs A;
     A.opix = S.opix; A.objoff = S.objoff; A.nobj = numobj;
-    A.sub = S.img.as<float>(); A.cd = S.cd.as<float>(); A.back = S.maps.as<float>();
+    A.sub = S.img.as<float>(); A.cd = S.cd.as<float>(); A.back = sxg_back_map();
     A.W = S.W; A.H = S.H; A.nbx = nbackx; A.nby = nbacky; A.backw = backw; A.backh = backh;
     A.backsig = backsig; A.thresh = thresh; A.dthresh = dthresh;
     A.gain = gain; A.satur_level = satur_level;

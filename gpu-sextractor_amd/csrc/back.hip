@@ -862,118 +862,256 @@ __global__ void __launch_bounds__(RN_ROWS) k_rownodes_lds(const float *back, con
 }
 
 /* colseg[x]: index of the left node, coldx[x]: dx of the cumulative recurrence.
-   SUB: subtract the interpolated value from img (subbackline, back.c:790-930); else write
-   it (backrmsline on the sigma maps, back.c:938-1051) */
+   SUB: dst = src - the interpolated value (subbackline, back.c:790-930); else dst = the
+   value (backrmsline on the sigma maps, back.c:938-1051).  One workgroup per 1024
+   pixels of a row (blockIdx.y), four pixels per lane at a stride of 256. */
 template <bool SUB>
-__global__ void __launch_bounds__(256) k_subback(float *img, int W, int H, int nbx,
+__global__ void __launch_bounds__(256) k_subback(const float *src, float *dst, int W, int nbx,
                                                  const float *nodes, const float *dnodes,
                                                  const int *colseg, const float *coldx)
 {
-    const long k = (long)blockIdx.x * 256 + threadIdx.x;
-    if (k >= (long)W * H) return;
-    const int y = (int)(k / W), x = (int)(k - (long)y * W);
+    const int y = blockIdx.y;
     const float *node = nodes + (size_t)y * nbx, *dnode = dnodes + (size_t)y * nbx;
-    float bval;
-    if (nbx > 1) {
-        const int s = colseg[x];
-        const float dx = coldx[x];
-        const float cdx = 1 - dx;
-        bval = cdx * (node[s] + (cdx * cdx - 1) * dnode[s]) + dx * (node[s + 1] + (dx * dx - 1) * dnode[s + 1]);
-    } else
-        bval = node[0];
-    if (SUB)
-        img[k] -= bval;
-    else
-        img[k] = bval;
+    const size_t row = (size_t)y * W;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int x = blockIdx.x * 1024 + r * 256 + threadIdx.x;
+        if (x >= W) break;
+        float bval;
+        if (nbx > 1) {
+            const int s = colseg[x];
+            const float dx = coldx[x];
+            const float cdx = 1 - dx;
+            bval = cdx * (node[s] + (cdx * cdx - 1) * dnode[s]) + dx * (node[s + 1] + (dx * dx - 1) * dnode[s + 1]);
+        } else
+            bval = node[0];
+        if (SUB)
+            dst[row + x] = src[row + x] - bval;
+        else
+            dst[row + x] = bval;
+    }
 }
 
-__global__ void __launch_bounds__(256) k_subconst(float *img, long n, float v)
+/* dst = src - v, with v = *vdev when vdev is given (the global median on the device) */
+__global__ void __launch_bounds__(256) k_subconst(const float *src, float *dst, long n, float v,
+                                                  const float *vdev)
 {
     const long k = (long)blockIdx.x * 256 + threadIdx.x;
-    if (k < n) img[k] -= v;
+    const float c = vdev ? *vdev : v;
+    if (k < n) dst[k] = src[k] - c;
 }
 
-/* The column tables replay the x loop of subbackline (back.c:880-908) on the host: the
-   float additions dx += xstep and the resets to dx0 are the same IEEE operations. */
-static void column_tables(int W, int nbx, int bs, std::vector<int> &seg, std::vector<float> &dxs)
+__global__ void k_setval(float *p, float v) { *p = v; }
+
+/* The column tables of the x loop of subbackline (back.c:880-908): for each column, the
+   index of its left node and the dx of the cumulative float recurrence dx += xstep,
+   which starts at (xstep - 1)/2 and restarts at dx0 at the half of every mesh t in
+   [1, nbx - 2] (the loop's pixel with i == bs/2, column t*bs + bs/2).  One lane per
+   restart segment performs that segment's additions in the loop's order, so every value
+   is the loop's own IEEE result. */
+__global__ void __launch_bounds__(64) k_coltables(int W, int nbx, int bs, int *colseg,
+                                                  float *coldx)
 {
-    seg.assign(W, 0);
-    dxs.assign(W, 0.0f);
-    if (nbx <= 1) return;
-    const int nx = bs, nbxm1 = nbx - 1;
-    volatile float xstep = (float)(1.0 / nx);   /* volatile: each float operation as written */
-    const int changepoint = nx / 2;
-    volatile float dx = (xstep - 1) / 2;
-    const float dx0 = ((nx + 1) % 2) * xstep / 2;
-    int s = 0;
-    for (int x = 0, i = 0, j = 0; j < W; j++, i++, dx = dx + xstep) {
-        if (i == changepoint && x > 0 && x < nbxm1) {
-            s++;
-            dx = dx0;
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (nbx <= 1) {
+        for (int j = t; j < W; j += gridDim.x * 64) {
+            colseg[j] = 0;
+            coldx[j] = 0.0f;
         }
-        seg[j] = s;
-        dxs[j] = dx;
-        if (i == nx) {
-            x++;
-            i = 0;
-        }
+        return;
     }
+    const int cp = bs / 2;
+    const int nseg = cp >= 1 && nbx > 2 ? nbx - 1 : 1;
+    if (t >= nseg) return;
+    const float xstep = (float)(1.0 / bs);
+    float dx = t == 0 ? (xstep - 1) / 2 : ((bs + 1) % 2) * xstep / 2;
+    const int j0 = t == 0 ? 0 : t * bs + cp;
+    const int j1 = t + 1 < nseg ? min((t + 1) * bs + cp, W) : W;
+    for (int j = j0; j < j1; j++) {
+        colseg[j] = t;
+        coldx[j] = dx;
+        dx = dx + xstep;
+    }
+}
+
+/* Byte layout of the device buffers of one background estimate of a W x H frame with
+   nbx x nby meshes, each part 256-byte aligned: mesh statistics, work (4*n2 floats and
+   the two global medians), the four mesh maps, the per-row spline tables (nodes, dnodes,
+   utmp: H*nbx floats each) and the column tables (W ints, W floats). */
+struct BackLayout {
+    size_t ms, work, maps, rows, cols, total;
+    int n2;
+};
+static inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+static BackLayout back_layout(int W, int H, int nbx, int nby)
+{
+    BackLayout L;
+    const size_t nmesh = (size_t)nbx * nby;
+    L.n2 = 1;
+    while ((size_t)L.n2 < nmesh) L.n2 <<= 1;
+    L.ms = 0;
+    L.work = L.ms + al256(sizeof(MeshStat) * nmesh);
+    L.maps = L.work + al256(sizeof(float) * (4 * (size_t)L.n2 + 64));
+    L.rows = L.maps + al256(sizeof(float) * 4 * nmesh);
+    L.cols = L.rows + al256(sizeof(float) * 3 * (size_t)H * nbx);
+    L.total = L.cols + al256(sizeof(int) * (size_t)W) + al256(sizeof(float) * (size_t)W);
+    return L;
+}
+static BackBufs back_bufs(char *base, const BackLayout &L, int W, int H, int nbx, int nby)
+{
+    BackBufs b;
+    const size_t nmesh = (size_t)nbx * nby;
+    b.ms = reinterpret_cast<MeshStat *>(base + L.ms);
+    b.work = reinterpret_cast<float *>(base + L.work);
+    b.outv = b.work + 4 * (size_t)L.n2;
+    b.back = reinterpret_cast<float *>(base + L.maps);
+    b.sigma = b.back + nmesh;
+    b.dback = b.sigma + nmesh;
+    b.dsigma = b.dback + nmesh;
+    b.nodes = reinterpret_cast<float *>(base + L.rows);
+    b.dnodes = b.nodes + (size_t)H * nbx;
+    b.utmp = b.dnodes + (size_t)H * nbx;
+    b.colseg = reinterpret_cast<int *>(base + L.cols);
+    b.coldx = reinterpret_cast<float *>(base + L.cols + al256(sizeof(int) * (size_t)W));
+    return b;
 }
 
 /* the per-row spline nodes of a mesh map (and its y second derivatives) into the row
-   tables, and the column tables (built on the host once per frame geometry) */
-static bool spline_rows(const float *map, const float *dmap)
+   tables of b */
+static bool spline_rows(const BackGeom &g, const BackBufs &b, const float *map,
+                        const float *dmap, hipStream_t st)
 {
-    SxgState &S = sxg_state();
-    const int W = S.W, H = S.H, nbx = S.nbx, nby = S.nby, bs = S.bs;
-    float *nodes = S.rowspl.as<float>(), *dnodes = nodes + (size_t)H * nbx,
-          *utmp = dnodes + (size_t)H * nbx;
-    int *colseg = reinterpret_cast<int *>(utmp + (size_t)H * nbx);
-    float *coldx = reinterpret_cast<float *>(colseg + W);
-    /* the tables sit after the row tables: the key holds everything their place and
-       contents depend on */
-    if (S.col_w != W || S.col_h != H || S.col_nbx != nbx || S.col_bs != bs ||
-        S.col_buf != S.rowspl.ptr || S.col_bytes != S.rowspl.bytes) {
-        std::vector<int> seg;
-        std::vector<float> dxs;
-        column_tables(W, nbx, bs, seg, dxs);
-        SXG_HIP_RET(hipMemcpy(colseg, seg.data(), sizeof(int) * W, hipMemcpyHostToDevice), false);
-        SXG_HIP_RET(hipMemcpy(coldx, dxs.data(), sizeof(float) * W, hipMemcpyHostToDevice), false);
-        S.col_w = W;
-        S.col_h = H;
-        S.col_nbx = nbx;
-        S.col_bs = bs;
-        S.col_buf = S.rowspl.ptr;
-        S.col_bytes = S.rowspl.bytes;
-    }
-    const size_t rnlds = 3 * (size_t)nbx * RN_ROWS * sizeof(float);
+    const size_t rnlds = 3 * (size_t)g.nbx * RN_ROWS * sizeof(float);
     if (rnlds <= 64 * 1024) {
-        hipLaunchKernelGGL(k_rownodes_lds, dim3(sxg_div_up(H, RN_ROWS)), dim3(RN_ROWS), rnlds,
-                           S.stream, map, dmap, nbx, nby, bs, H, nodes, dnodes);
+        hipLaunchKernelGGL(k_rownodes_lds, dim3(sxg_div_up(g.H, RN_ROWS)), dim3(RN_ROWS), rnlds,
+                           st, map, dmap, g.nbx, g.nby, g.bs, g.H, b.nodes, b.dnodes);
     } else {
-        hipLaunchKernelGGL(k_rownodes, dim3(sxg_div_up(H, 256)), dim3(256), 0, S.stream, map, dmap,
-                           nbx, nby, bs, H, nodes, dnodes, utmp);
+        hipLaunchKernelGGL(k_rownodes, dim3(sxg_div_up(g.H, 256)), dim3(256), 0, st, map, dmap,
+                           g.nbx, g.nby, g.bs, g.H, b.nodes, b.dnodes, b.utmp);
     }
     SXG_HIP_RET(hipGetLastError(), false);
     return true;
 }
 
-/* subtract the spline of the row tables from img (sub) or write it into img */
-static bool spline_apply(float *img, bool sub)
+/* dst = src - the spline of the row tables (sub), or dst = the spline */
+static bool spline_apply(const BackGeom &g, const BackBufs &b, const float *src, float *dst,
+                         bool sub, hipStream_t st)
+{
+    const dim3 grid(sxg_div_up(g.W, 1024), g.H);
+    if (sub)
+        hipLaunchKernelGGL(k_subback<true>, grid, dim3(256), 0, st, src, dst, g.W, g.nbx,
+                           b.nodes, b.dnodes, b.colseg, b.coldx);
+    else
+        hipLaunchKernelGGL(k_subback<false>, grid, dim3(256), 0, st, src, dst, g.W, g.nbx,
+                           b.nodes, b.dnodes, b.colseg, b.coldx);
+    SXG_HIP_RET(hipGetLastError(), false);
+    return true;
+}
+
+/* The background of a W x H frame read from src and subtracted into dst (src == dst is
+   allowed): mesh statistics and modes (pixels whose wmap value is not positive are left
+   out when wmap is given), bad-mesh interpolation and median filtering, the global
+   medians into b.outv[0..1] (device), the y derivatives, then the subtraction of the
+   bicubic spline (back_type 0), of the global median b.outv[0] (1) or of back_value (2,
+   which also becomes b.outv[0]).  Asynchronous on st; allocates nothing. */
+bool sxg_back_run(const BackGeom &g, const BackBufs &b, const float *src, float *dst,
+                  const float *wmap, float backfthresh, int back_type, float back_value,
+                  bool timed, hipStream_t st)
+{
+    const int W = g.W, H = g.H, bs = g.bs, nbx = g.nbx, nby = g.nby, nmesh = nbx * nby;
+    int n2 = 1;
+    while (n2 < nmesh) n2 <<= 1;
+    if (timed) sxg_kt_begin(SXG_KT_MESHSTAT);
+    {
+        /* meshes per workgroup and rows per batch: two buffers of R rows of mpb padded
+           segments of t and t2 (doubles) within 128 KB of LDS */
+        const size_t segbytes = (size_t)ms_seg(bs) * 2 * sizeof(double);
+        const int mpb = (int)std::min<size_t>(MS_MPB, (64 * 1024) / segbytes);
+        if (mpb >= 1) {
+            /* the LDS limit is an attribute of the function: raised once, to the most
+               any launch asks for (128 KB) */
+            static const bool attr = [] {
+                return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_meshstat),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           128 * 1024) == hipSuccess;
+            }();
+            if (!attr) {
+                sxg_set_error("back_estimate: cannot raise the LDS limit of k_meshstat");
+                return false;
+            }
+            const size_t rowbytes = mpb * segbytes;
+            const int R = (int)std::max<size_t>(1, std::min<size_t>(8, (64 * 1024) / rowbytes));
+            const size_t lds = 2 * R * rowbytes;
+            hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nbx, mpb), nby), dim3(MS_THREADS), lds,
+                               st, src, wmap, W, H, bs, nbx, mpb, R, b.ms);
+        } else {
+            hipLaunchKernelGGL(k_meshstat_direct, dim3(sxg_div_up(nmesh, 64)), dim3(64), 0, st,
+                               src, wmap, W, H, bs, nbx, nmesh, b.ms);
+        }
+    }
+    if (timed) sxg_kt_end(SXG_KT_MESHSTAT);
+    if (timed) sxg_kt_begin(SXG_KT_MESHMODE);
+    hipLaunchKernelGGL(k_meshmode, dim3(nmesh), dim3(MODE_THREADS), 0, st, src, wmap, W, H, bs,
+                       nbx, b.ms, b.back, b.sigma);
+    if (timed) sxg_kt_end(SXG_KT_MESHMODE);
+    if (timed) sxg_kt_begin(SXG_KT_FILTERBACK);
+    {
+        float *back2 = b.work, *sigma2 = b.work + n2;
+        const int gm = sxg_div_up(nmesh, 256);
+        hipLaunchKernelGGL(k_fb_interp, dim3(gm), dim3(256), 0, st, b.back, b.sigma, back2, sigma2,
+                           nbx, nby);
+        hipLaunchKernelGGL(k_fb_median, dim3(gm), dim3(256), 0, st, back2, sigma2, b.back, b.sigma,
+                           nbx, nby, g.nfx, g.nfy, backfthresh);
+        hipLaunchKernelGGL(k_fb_global, dim3(2), dim3(FB_THREADS), 0, st, b.back, b.sigma, nmesh,
+                           b.outv);
+        /* scratch of 2*nby floats per column: LDS when it fits, else back2/sigma2 and the
+           sort buffer (4*n2 >= 2 * 2*nbx*nby floats) */
+        const size_t ylds = 2 * (size_t)nby * 64 * sizeof(float);
+        const int ylds_ok = ylds <= 128 * 1024;
+        static const bool yattr = [] {
+            return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fb_yderiv),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       128 * 1024) == hipSuccess;
+        }();
+        if (!yattr) {
+            sxg_set_error("back_estimate: cannot raise the LDS limit of k_fb_yderiv");
+            return false;
+        }
+        hipLaunchKernelGGL(k_fb_yderiv, dim3(sxg_div_up(2 * nbx, 64)), dim3(64),
+                           ylds_ok ? ylds : 0, st, b.back, b.sigma, b.dback, b.dsigma, nbx, nby,
+                           ylds_ok, back2);
+    }
+    if (timed) sxg_kt_end(SXG_KT_FILTERBACK);
+    if (back_type == 1 || back_type == 2) {
+        /* back_type 1 (BACK_TYPE MANUAL): the reference's device code subtracts the global
+           median it has just computed (cudaback.cu:1336-1345), not BACK_VALUE; the caller
+           (back.c:152-153) only overwrites its own backmean with BACK_VALUE afterwards.
+           back_type 2 (an addition): stock SExtractor's MANUAL, the constant BACK_VALUE
+           (back.c:148-149, :807-813), returned as backmean. */
+        if (back_type == 2) hipLaunchKernelGGL(k_setval, dim3(1), dim3(1), 0, st, b.outv, back_value);
+        hipLaunchKernelGGL(k_subconst, dim3(sxg_div_up((long)W * H, 256)), dim3(256), 0, st, src,
+                           dst, (long)W * H, back_value, back_type == 1 ? b.outv : nullptr);
+    } else {
+        if (timed) sxg_kt_begin(SXG_KT_ROWNODES);
+        hipLaunchKernelGGL(k_coltables, dim3(sxg_div_up(nbx, 64)), dim3(64), 0, st, W, nbx, bs,
+                           b.colseg, b.coldx);
+        if (!spline_rows(g, b, b.back, b.dback, st)) return false;
+        if (timed) sxg_kt_end(SXG_KT_ROWNODES);
+        if (timed) sxg_kt_begin(SXG_KT_SUBBACK);
+        if (!spline_apply(g, b, src, dst, true, st)) return false;
+        if (timed) sxg_kt_end(SXG_KT_SUBBACK);
+    }
+    SXG_HIP_RET(hipGetLastError(), false);
+    return true;
+}
+
+/* the context's buffers for a background of geometry g (grow-only) */
+static bool ctx_back_bufs(const BackGeom &g, BackBufs &b)
 {
     SxgState &S = sxg_state();
-    const int W = S.W, H = S.H, nbx = S.nbx;
-    float *nodes = S.rowspl.as<float>(), *dnodes = nodes + (size_t)H * nbx,
-          *utmp = dnodes + (size_t)H * nbx;
-    int *colseg = reinterpret_cast<int *>(utmp + (size_t)H * nbx);
-    float *coldx = reinterpret_cast<float *>(colseg + W);
-    if (sub)
-        hipLaunchKernelGGL(k_subback<true>, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0,
-                           S.stream, img, W, H, nbx, nodes, dnodes, colseg, coldx);
-    else
-        hipLaunchKernelGGL(k_subback<false>, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0,
-                           S.stream, img, W, H, nbx, nodes, dnodes, colseg, coldx);
-    SXG_HIP_RET(hipGetLastError(), false);
+    const BackLayout L = back_layout(g.W, g.H, g.nbx, g.nby);
+    if (!S.backbuf.ensure(L.total)) return false;
+    b = back_bufs(S.backbuf.as<char>(), L, g.W, g.H, g.nbx, g.nby);
     return true;
 }
 
@@ -984,141 +1122,88 @@ extern "C" int sxg_get_back_rms(float *out)
     /* the mesh maps of the last back_estimate stay until the next one (also after
        clear_device) */
     const int nm = S.nbx * S.nby;
-    if (nm <= 0 || S.N == 0 || !S.maps.ptr || S.back_W != S.W || S.back_H != S.H) {
+    if (nm <= 0 || S.N == 0 || !S.backbuf.ptr || S.back_W != S.W || S.back_H != S.H) {
         sxg_set_error("sxg_get_back_rms: call after back_estimate");
         return -1;
     }
-    float *sigma = S.maps.as<float>() + nm, *dsigma = sigma + 2 * (size_t)nm;
+    BackBufs b;
+    if (!ctx_back_bufs(S.bgeom, b)) return -1;
     float *rms = sxg_take<float>("back.rms", S.N);
-    if (!rms || !spline_rows(sigma, dsigma) || !spline_apply(rms, false)) return -1;
+    if (!rms || !spline_rows(S.bgeom, b, b.sigma, b.dsigma, S.stream) ||
+        !spline_apply(S.bgeom, b, nullptr, rms, false, S.stream))
+        return -1;
     SXG_HIP_RET(hipMemcpyAsync(out, rms, sizeof(float) * S.N, hipMemcpyDeviceToHost, S.stream),
                 -1);
     SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
     return 0;
 }
 
-extern "C" void back_estimate(int back_size, float backfthresh, float *backmean, float *backsig,
-                              int nbackfx, int nbackfy, int back_type)
+/* the geometry of a background of a W x H frame; false (error recorded) when invalid */
+bool sxg_back_geom(int W, int H, int back_size, int nbackfx, int nbackfy, BackGeom &g)
+{
+    if (back_size < 1) {
+        sxg_set_error("back_estimate: BACK_SIZE must be >= 1");
+        return false;
+    }
+    g.W = W;
+    g.H = H;
+    g.bs = back_size;
+    g.nbx = (W - 1) / back_size + 1;
+    g.nby = (H - 1) / back_size + 1;
+    /* field.c:139-140: a single mesh along an axis disables filtering along it */
+    g.nfx = g.nbx > 1 ? nbackfx : 1;
+    g.nfy = g.nby > 1 ? nbackfy : 1;
+    if (g.nfx < 1 || g.nfy < 1 || g.nfx > SXG_MAXBACKF || g.nfy > SXG_MAXBACKF) {
+        sxg_set_error("back_estimate: BACK_FILTERSIZE out of range (1..11)");
+        return false;
+    }
+    return true;
+}
+
+/* back_estimate on the context's frame, reading the input frame from src (the context's
+   own frame buffer for the reference entry point; a resident frame for the whole-frame
+   runs, which saves the copy) and writing the subtracted frame into S.img */
+void sxg_back_estimate_from(const float *src, int back_size, float backfthresh, float *backmean,
+                            float *backsig, int nbackfx, int nbackfy, int back_type)
 {
     SxgState &S = sxg_state();
     if (!S.have_frame) {
         sxg_set_error("back_estimate: no frame (call init_device first)");
         return;
     }
-    const int W = S.W, H = S.H;
-    if (back_size < 1) {
-        sxg_set_error("back_estimate: BACK_SIZE must be >= 1");
-        return;
-    }
-    const int bs = back_size;
-    const int nbx = (W - 1) / bs + 1, nby = (H - 1) / bs + 1, nmesh = nbx * nby;
-    /* field.c:139-140: a single mesh along an axis disables filtering along it */
-    const int nfx = nbx > 1 ? nbackfx : 1, nfy = nby > 1 ? nbackfy : 1;
-    if (nfx < 1 || nfy < 1 || nfx > SXG_MAXBACKF || nfy > SXG_MAXBACKF) {
-        sxg_set_error("back_estimate: BACK_FILTERSIZE out of range (1..11)");
-        return;
-    }
-    S.bs = bs;
-    S.back_W = S.back_H = 0;    /* set once the maps and row tables exist for this size */
-    S.nbx = nbx;
-    S.nby = nby;
-    int n2 = 1;
-    while (n2 < nmesh) n2 <<= 1;
-    /* mesh scratch: MeshStat[nmesh] + 4*n2 floats of work + 2 outputs */
-    const size_t ms_bytes = ((sizeof(MeshStat) * nmesh + 255) / 256) * 256;
-    /* a new row-table buffer loses the column tables kept in it */
-    const size_t rowspl_bytes = S.rowspl.bytes;
-    if (!S.mesh.ensure(ms_bytes + sizeof(float) * (4 * (size_t)n2 + 64)) ||
-        !S.maps.ensure(sizeof(float) * 4 * (size_t)nmesh) ||
-        !S.rowspl.ensure(sizeof(float) * 3 * (size_t)H * nbx + sizeof(float) * 2 * W + 256))
-        return;
-    if (S.rowspl.bytes != rowspl_bytes) S.col_w = 0;
-    S.back_W = W;
-    S.back_H = H;
-    MeshStat *ms = S.mesh.as<MeshStat>();
-    float *work = reinterpret_cast<float *>(reinterpret_cast<char *>(S.mesh.ptr) + ms_bytes);
-    float *outv = work + 4 * (size_t)n2;
-    float *back = S.maps.as<float>(), *sigma = back + nmesh, *dback = sigma + nmesh,
-          *dsigma = dback + nmesh;
+    BackGeom g;
+    if (!sxg_back_geom(S.W, S.H, back_size, nbackfx, nbackfy, g)) return;
+    S.bs = g.bs;
+    S.nbx = g.nbx;
+    S.nby = g.nby;
+    S.back_W = S.back_H = 0;    /* set once the maps exist for this size */
+    BackBufs b;
+    if (!ctx_back_bufs(g, b)) return;
+    S.bgeom = g;
+    S.back_W = S.W;
+    S.back_H = S.H;
     float *img = S.img.as<float>();
     WeightArgs wa;
     if (!sxg_weight_args(wa)) return;
-    const float *wmap = wa.mask;   /* pixels with a non-positive mask value leave the mesh
-                                      statistics */
-
     sxg_stage_begin(ST_BACK);
-    sxg_kt_begin(SXG_KT_MESHSTAT);
-    {
-        /* meshes per workgroup and rows per batch: two buffers of R rows of mpb padded
-           segments of t and t2 (doubles) within 128 KB of LDS */
-        const size_t segbytes = (size_t)ms_seg(bs) * 2 * sizeof(double);
-        const int mpb = (int)std::min<size_t>(MS_MPB, (64 * 1024) / segbytes);
-        if (mpb >= 1) {
-            const size_t rowbytes = mpb * segbytes;
-            const int R = (int)std::max<size_t>(1, std::min<size_t>(8, (64 * 1024) / rowbytes));
-            const size_t lds = 2 * R * rowbytes;
-            SXG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_meshstat),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nbx, mpb), nby), dim3(MS_THREADS), lds,
-                               S.stream, img, wmap, W, H, bs, nbx, mpb, R, ms);
-        } else {
-            hipLaunchKernelGGL(k_meshstat_direct, dim3(sxg_div_up(nmesh, 64)), dim3(64), 0,
-                               S.stream, img, wmap, W, H, bs, nbx, nmesh, ms);
-        }
-    }
-    sxg_kt_end(SXG_KT_MESHSTAT);
-    sxg_kt_begin(SXG_KT_MESHMODE);
-    hipLaunchKernelGGL(k_meshmode, dim3(nmesh), dim3(MODE_THREADS), 0, S.stream, img, wmap, W, H,
-                       bs, nbx, ms, back, sigma);
-    sxg_kt_end(SXG_KT_MESHMODE);
-    sxg_kt_begin(SXG_KT_FILTERBACK);
-    {
-        float *back2 = work, *sigma2 = work + n2;
-        const int gm = sxg_div_up(nmesh, 256);
-        hipLaunchKernelGGL(k_fb_interp, dim3(gm), dim3(256), 0, S.stream, back, sigma, back2,
-                           sigma2, nbx, nby);
-        hipLaunchKernelGGL(k_fb_median, dim3(gm), dim3(256), 0, S.stream, back2, sigma2, back,
-                           sigma, nbx, nby, nfx, nfy, (float)backfthresh);
-        hipLaunchKernelGGL(k_fb_global, dim3(2), dim3(FB_THREADS), 0, S.stream, back, sigma,
-                           nmesh, outv);
-        /* scratch of 2*nby floats per column: LDS when it fits, else back2/sigma2 and the
-           sort buffer (4*n2 >= 2 * 2*nbx*nby floats) */
-        const size_t ylds = 2 * (size_t)nby * 64 * sizeof(float);
-        const int ylds_ok = ylds <= 128 * 1024;
-        if (ylds_ok)
-            SXG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fb_yderiv),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)ylds));
-        hipLaunchKernelGGL(k_fb_yderiv, dim3(sxg_div_up(2 * nbx, 64)), dim3(64),
-                           ylds_ok ? ylds : 0, S.stream, back, sigma, dback, dsigma, nbx, nby,
-                           ylds_ok, back2);
-    }
-    sxg_kt_end(SXG_KT_FILTERBACK);
+    /* pixels with a non-positive mask value leave the mesh statistics */
+    if (!sxg_back_run(g, b, src, img, wa.mask, backfthresh, back_type, S.back_value, true,
+                      S.stream))
+        return;
     float hv[2];
-    SXG_HIP(hipMemcpyAsync(hv, outv, sizeof(hv), hipMemcpyDeviceToHost, S.stream));
+    SXG_HIP(hipMemcpyAsync(hv, b.outv, sizeof(hv), hipMemcpyDeviceToHost, S.stream));
     SXG_HIP(hipStreamSynchronize(S.stream));
     S.backmean = hv[0];
     S.backsig = hv[1];
-    if (back_type == 1) {
-        /* BACK_TYPE MANUAL: subtract BACK_VALUE (back.c:148-149, :807-813) */
-        S.backmean = S.back_value;
-        hipLaunchKernelGGL(k_subconst, dim3(sxg_div_up((long)S.N, 256)), dim3(256), 0, S.stream,
-                           img, (long)S.N, S.back_value);
-    } else {
-        sxg_kt_begin(SXG_KT_ROWNODES);
-        if (!spline_rows(back, dback)) return;
-        sxg_kt_end(SXG_KT_ROWNODES);
-        sxg_kt_begin(SXG_KT_SUBBACK);
-        if (!spline_apply(img, true)) return;
-        sxg_kt_end(SXG_KT_SUBBACK);
-    }
     if (S.weighted && S.wtype == SXG_WEIGHT_BACKGROUND) {
         /* WEIGHT_TYPE BACKGROUND: the variance map of this frame is the square of the
            interpolated background rms */
-        if (!S.weight.ensure(sizeof(float) * S.N) || !spline_rows(sigma, dsigma) ||
-            !spline_apply(S.weight.as<float>(), false))
+        if (!S.weight.ensure(sizeof(float) * S.N) ||
+            !spline_rows(g, b, b.sigma, b.dsigma, S.stream) ||
+            !spline_apply(g, b, nullptr, S.weight.as<float>(), false, S.stream))
             return;
-        S.ww = W;
-        S.wh = H;
+        S.ww = S.W;
+        S.wh = S.H;
         S.wrms_frame = true;
     }
     sxg_stage_end(ST_BACK);
@@ -1127,17 +1212,42 @@ extern "C" void back_estimate(int back_size, float backfthresh, float *backmean,
     if (backsig) *backsig = S.backsig;
 }
 
+extern "C" void back_estimate(int back_size, float backfthresh, float *backmean, float *backsig,
+                              int nbackfx, int nbackfy, int back_type)
+{
+    SxgState &S = sxg_state();
+    sxg_back_estimate_from(S.img.as<float>(), back_size, backfthresh, backmean, backsig, nbackfx,
+                           nbackfy, back_type);
+}
+
 extern "C" int sxg_get_back_maps(float *back, float *sigma, float *dback, float *dsigma)
 {
     SxgState &S = sxg_state();
     const int nm = S.nbx * S.nby;
-    if (nm <= 0) return -1;
-    float *d = S.maps.as<float>();
+    if (nm <= 0 || !S.backbuf.ptr) return -1;
+    BackBufs b;
+    if (!ctx_back_bufs(S.bgeom, b)) return -1;
+    const float *src[4] = {b.back, b.sigma, b.dback, b.dsigma};
     float *dst[4] = {back, sigma, dback, dsigma};
     for (int i = 0; i < 4; i++)
         if (dst[i])
-            SXG_HIP_RET(hipMemcpyAsync(dst[i], d + (size_t)i * nm, sizeof(float) * nm,
-                                       hipMemcpyDeviceToHost, S.stream), -1);
+            SXG_HIP_RET(hipMemcpyAsync(dst[i], src[i], sizeof(float) * nm, hipMemcpyDeviceToHost,
+                                       S.stream), -1);
     SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
     return 0;
+}
+
+const float *sxg_back_map()
+{
+    SxgState &S = sxg_state();
+    if (!S.backbuf.ptr) return nullptr;
+    const BackLayout L = back_layout(S.bgeom.W, S.bgeom.H, S.bgeom.nbx, S.bgeom.nby);
+    return reinterpret_cast<const float *>(S.backbuf.as<char>() + L.maps);
+}
+
+/* the BackBufs block of geometry g: its size in bytes, and its parts at base */
+size_t sxg_back_bytes(const BackGeom &g) { return back_layout(g.W, g.H, g.nbx, g.nby).total; }
+BackBufs sxg_back_bufs_at(char *base, const BackGeom &g)
+{
+    return back_bufs(base, back_layout(g.W, g.H, g.nbx, g.nby), g.W, g.H, g.nbx, g.nby);
 }

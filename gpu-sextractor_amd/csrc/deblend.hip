@@ -1864,12 +1864,28 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
            Everything they read was finished by the synchronisation above. */
         int nside = 0;
         const bool use_side = !S.single_stream && sxg_option(SXG_OPT_SIDE_STREAMS) != 0;
+        /* the dynamic-LDS limit is an attribute of the function, shared by every context:
+           it is raised once, to the most any class asks for (DEB_LDS_LIMIT), never per
+           launch (batch workers launch these kernels from several host threads) */
+        static const bool lds_attr = [] {
+            return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_deb_tree_lds<1024>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)DEB_LDS_LIMIT) == hipSuccess &&
+                   hipFuncSetAttribute(reinterpret_cast<const void *>(&k_deb_tree_lds<256>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)DEB_LDS_LIMIT) == hipSuccess;
+        }();
+        if (!lds_attr) {
+            sxg_set_error("parcel_out: cannot raise the LDS limit of the deblending kernels");
+            return -1;
+        }
         auto launch_lds = [&](const void *fn, int cls, int nt, int ct) -> bool {
             (void)ct;
             const size_t lds = (size_t)cc[4 + cls];
-            SXG_HIP_RET(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            (int)lds),
-                        false);
+            if (lds > DEB_LDS_LIMIT) {
+                sxg_set_error("parcel_out: LDS request above the limit (internal error)");
+                return false;
+            }
             void *args[] = {&a, &cls};
             const bool side = cls != 0 && use_side;
             hipStream_t ks = side ? S.side[nside] : st;

@@ -109,7 +109,9 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
         return -1;
     }
     if (!sxg_ensure_stream()) return -1;
-    /* init_device from device memory: the working copy is overwritten in place */
+    /* init_device from device memory: the background stage reads the frame where it is
+       and writes the subtracted frame into the context's buffer, so the input is never
+       copied */
     {
         const size_t N = (size_t)W * H;
         S.W = W;
@@ -125,16 +127,6 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
         if (!S.img.ensure(N * sizeof(float)) || !S.cd.ensure(N * sizeof(float)) ||
             !S.lab.ensure(N * sizeof(int)) || !S.aux.ensure(N * sizeof(int)))
             return -1;
-        if ((N & 3) == 0) {
-            sxg_kt_begin(SXG_KT_COPY);
-            hipLaunchKernelGGL(k_copy_frame, dim3(sxg_div_up((long)(N / 4), 256)), dim3(256), 0,
-                               S.stream, (const float4 *)dframe, (float4 *)S.img.ptr,
-                               (long)(N / 4));
-            sxg_kt_end(SXG_KT_COPY);
-        } else {
-            SXG_HIP_RET(hipMemcpyAsync(S.img.ptr, dframe, N * sizeof(float),
-                                       hipMemcpyDeviceToDevice, S.stream), -1);
-        }
         S.have_frame = true;
     }
     const int bw = p->back_size < W ? p->back_size : W;
@@ -146,9 +138,12 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
     const int nbx = (W - 1) / bw + 1, nby = (H - 1) / bw + 1;
     float backmean = 0.0f, backsig = 0.0f;
     S.back_value = p->back_value;
-    back_estimate(bw, p->back_fthresh, &backmean, &backsig, nbx > 1 ? p->back_fsize : 1,
-                  nby > 1 ? p->back_fsize : 1, p->back_type);
+    sxg_back_estimate_from(dframe, bw, p->back_fthresh, &backmean, &backsig,
+                           nbx > 1 ? p->back_fsize : 1, nby > 1 ? p->back_fsize : 1,
+                           p->back_type);
     if (sxg_has_error()) return -1;
+    /* back.c:152-153: with BACK_TYPE MANUAL the frame's backmean is BACK_VALUE */
+    if (p->back_type == 1) S.backmean = p->back_value;
     float dthresh, thresh;
     if (p->thresh_type == 1) {
         dthresh = (float)p->detect_thresh;
@@ -251,6 +246,8 @@ extern "C" int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_in
             sxg_set_error("sxg_batch_process: context allocation failed");
             return -1;
         }
+        /* one stream per worker: the frames in flight already fill the hardware queues */
+        sxg_ctx_single_stream(c);
         g_workers.push_back(c);
     }
     std::atomic<int> next(0), failed(0);

@@ -85,6 +85,36 @@ __device__ inline float sxg_pix_thresh(double k, float var) { return (float)(k *
 
 enum StageId { ST_BACK = 0, ST_FILTER, ST_DETECT, ST_DEBLEND, ST_ANALYSE, ST_CLEAN, ST_N };
 
+/* ------------------------------------------------------------- background ---- */
+/* geometry of one background estimate: frame, square mesh size, mesh counts and the
+   median-filter size along each axis */
+struct BackGeom {
+    int W = 0, H = 0, bs = 0, nbx = 0, nby = 0, nfx = 1, nfy = 1;
+};
+struct MeshStat;
+/* device buffers of one background estimate (back.hip lays them out in one block) */
+struct BackBufs {
+    MeshStat *ms = nullptr;          /* per-mesh statistics */
+    float *work = nullptr;           /* 4*n2 floats of filtering and selection scratch */
+    float *outv = nullptr;           /* the two global medians: backmean, backsig */
+    float *back = nullptr, *sigma = nullptr, *dback = nullptr, *dsigma = nullptr;   /* maps */
+    float *nodes = nullptr, *dnodes = nullptr, *utmp = nullptr;   /* per-row spline tables */
+    int *colseg = nullptr;           /* per-column left node */
+    float *coldx = nullptr;          /* per-column dx of the cumulative recurrence */
+};
+bool sxg_back_geom(int W, int H, int back_size, int nbackfx, int nbackfy, BackGeom &g);
+bool sxg_back_run(const BackGeom &g, const BackBufs &b, const float *src, float *dst,
+                  const float *wmap, float backfthresh, int back_type, float back_value,
+                  bool timed, hipStream_t st);
+void sxg_back_estimate_from(const float *src, int back_size, float backfthresh, float *backmean,
+                            float *backsig, int nbackfx, int nbackfy, int back_type);
+/* the background map of the last back_estimate on the current context (device) */
+const float *sxg_back_map();
+/* the device block of one background estimate of geometry g: its size, and its parts
+   laid out from base */
+size_t sxg_back_bytes(const BackGeom &g);
+BackBufs sxg_back_bufs_at(char *base, const BackGeom &g);
+
 #define SXG_NSIDE 4
 struct SxgState {
     int W = 0, H = 0;
@@ -108,14 +138,9 @@ struct SxgState {
     /* background */
     int bs = 0, nbx = 0, nby = 0;
     int back_W = 0, back_H = 0;               /* frame size the mesh maps belong to */
-    int col_w = 0, col_h = 0, col_nbx = 0, col_bs = 0;   /* key of the column tables */
-    const void *col_buf = nullptr;                       /* (and the buffer they are in) */
-    size_t col_bytes = 0;
+    BackGeom bgeom;                           /* geometry of the last background */
     float backmean = 0.0f, backsig = 0.0f;
-    DevBuf mesh;     /* per-mesh scratch */
-    DevBuf maps;     /* back, sigma, dback, dsigma */
-    DevBuf rowspl;   /* per-row node / dnode tables */
-    DevBuf small;    /* small host-visible results */
+    DevBuf backbuf;  /* BackBufs of the last background (mesh maps, spline tables) */
 
     /* detection */
     float dthresh = 0.0f, thresh = 0.0f;

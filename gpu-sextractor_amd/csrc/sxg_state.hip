@@ -49,10 +49,18 @@ bool DevBuf::ensure(size_t want)
     size_t alloc = want < 256 ? 256 : want;
     if (alloc < grown) alloc = grown;
     alloc = (alloc + 255) & ~(size_t)255;
-    /* queued kernels of any stream may still use the old buffer: the entry points do
-       not wait for their own kernels before returning */
+    /* queued kernels of the context may still use the old buffer (the entry points do
+       not wait for their own kernels before returning): wait for its streams, not for
+       the whole device, so other contexts keep running */
     if (ptr) {
-        (void)hipDeviceSynchronize();
+        SxgState &S = sxg_state();
+        if (S.stream) {
+            (void)hipStreamSynchronize(S.stream);
+            for (int i = 0; i < SXG_NSIDE; i++)
+                if (S.side[i]) (void)hipStreamSynchronize(S.side[i]);
+        } else {
+            (void)hipDeviceSynchronize();
+        }
         (void)hipFree(ptr);
     }
     ptr = nullptr;
@@ -138,7 +146,9 @@ bool sxg_ensure_stream()
             sxg_set_error("hipEventCreate failed");
             return false;
         }
-    for (int i = 0; i < SXG_NSIDE; i++)
+    /* side streams only where a frame's independent launches may use them: a batch
+       worker's context runs everything on its own stream */
+    for (int i = 0; i < (S.single_stream ? 0 : SXG_NSIDE); i++)
         if (hipStreamCreateWithFlags(&S.side[i], hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&S.side_ev[i], hipEventDisableTiming) != hipSuccess) {
             sxg_set_error("side stream creation failed");
@@ -186,13 +196,12 @@ extern "C" void sxg_release(void)
 {
     SxgState &S = sxg_state();
     if (S.stream) (void)hipStreamSynchronize(S.stream);
-    DevBuf *bufs[] = {&S.img, &S.cd, &S.lab, &S.aux, &S.mesh, &S.maps, &S.rowspl, &S.small,
-                      &S.scan_tmp, &S.thr, &S.lvl, &S.comps, &S.objs, &S.rootobj};
+    DevBuf *bufs[] = {&S.img, &S.cd, &S.lab, &S.aux, &S.backbuf, &S.scan_tmp, &S.thr, &S.lvl,
+                      &S.comps, &S.objs, &S.rootobj};
     for (DevBuf *b : bufs) b->release();
     sxg_release_named();
     S.dmask.release();
     S.mask_k = 0;
-    S.col_w = 0;
     S.back_W = S.back_H = 0;
     S.resident.release();
     S.weight.release();

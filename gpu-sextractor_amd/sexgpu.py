@@ -124,6 +124,23 @@ def synth(preset="plain1024", **kw):
     return img
 
 
+BATCH64_SIZES = [(512, 512), (1024, 1024), (1024, 2048), (2048, 2048), (2048, 4096)]
+
+
+def batch64_frames(n=64, seed=2000):
+    """The many-frame workload: n frames whose (height, width) are drawn from
+    BATCH64_SIZES and whose source densities are log-uniform in [1e-4, 1e-2] per pixel,
+    each rendered with the plain1024 generator (seed 2000 + i)."""
+    rng = np.random.default_rng(seed)
+    frames = []
+    for i in range(n):
+        h, w = BATCH64_SIZES[int(rng.integers(0, len(BATCH64_SIZES)))]
+        dens = float(10 ** rng.uniform(-4, -2))
+        frames.append(synth("plain1024", width=w, height=h,
+                            nsrc=max(1, int(round(dens * w * h))), seed=seed + i))
+    return frames
+
+
 def synth_weight(preset="wide8192", vign=0.2, badcol=0.01, **kw):
     p = synth_params(preset, **kw)
     w = np.empty((p.height, p.width), dtype=np.float32)
@@ -548,6 +565,9 @@ def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"
     H, W = img.shape
     set_weight(weight, p, weight_type, weight_gain)
     bm, bsig = run_background(img, p)
+    if p.back_type == 1:
+        # back.c:152-153: with BACK_TYPE MANUAL the frame's backmean is BACK_VALUE
+        bm = float(np.float32(p.back_value))
     dthresh, thresh = thresholds(p, bsig)
     L.init_detection()
     check()

@@ -32,7 +32,10 @@ void init_device(int width, int height, float *img);
 
 /* src/cuda/cudaback.cu:1212 -- background mesh statistics, mode, median filter,
    splines, global medians (written to *backmean, *backsig) and in-place subtraction.
-   back_type 1 (MANUAL) subtracts the value set by sxg_set_back_value(). */
+   back_type 1 (MANUAL) subtracts the global median *backmean, as the reference's device
+   code does (cudaback.cu:1336-1345; its caller back.c:152-153 then reports BACK_VALUE).
+   back_type 2 (an addition) is stock SExtractor's MANUAL: it subtracts the value set by
+   sxg_set_back_value() and returns it as *backmean. */
 void back_estimate(int back_size, float backfthresh, float *backmean, float *backsig,
                    int nbackfx, int nbackfy, int back_type);
 
@@ -96,7 +99,8 @@ void clear_device(float *img);
 /* Message of the last failed call, or NULL. */
 const char *sxg_last_error(void);
 
-/* BACK_VALUE for back_type 1 (the reference reads prefs.back_val on the host). */
+/* BACK_VALUE: subtracted by back_type 2, reported as backmean by back_type 1 in the
+   whole-frame runs (the reference reads prefs.back_val on the host, back.c:152-153). */
 void sxg_set_back_value(float value);
 
 /* MAG_ZEROPOINT used by endobject_gpu (the reference reads prefs.mag_zeropoint). */
@@ -231,7 +235,7 @@ void sxg_release(void);
 typedef struct {
     int back_size, back_fsize;
     float back_fthresh;
-    int back_type;              /* 0 AUTO, 1 MANUAL */
+    int back_type;              /* 0 AUTO, 1 MANUAL (reference), 2 MANUAL (stock) */
     float back_value;
     int thresh_type;            /* 0 RELATIVE, 1 ABSOLUTE */
     double detect_thresh, analysis_thresh;

@@ -28,8 +28,10 @@ typedef struct {
     int back_size;           /* BACK_SIZE (square meshes) */
     int back_fsize;          /* BACK_FILTERSIZE */
     double back_fthresh;     /* BACK_FILTTHRESH */
-    int back_type;           /* 0 AUTO, 1 MANUAL */
-    double back_value;       /* BACK_VALUE (MANUAL) */
+    int back_type;           /* 0 AUTO; 1 MANUAL as the reference's device code runs it
+                                (subtract the computed global median, report BACK_VALUE);
+                                2 stock MANUAL (subtract BACK_VALUE) */
+    double back_value;       /* BACK_VALUE */
     int thresh_type;         /* 0 RELATIVE, 1 ABSOLUTE */
     double detect_thresh;    /* DETECT_THRESH */
     double analysis_thresh;  /* ANALYSIS_THRESH */

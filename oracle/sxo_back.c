@@ -444,11 +444,16 @@ int sxo_background(const float *img, int W, int H, int bs, int nfx, int nfy, dou
     sxo_makebackspline(back, nbx, nby, dback);
     sxo_makebackspline(sigma, nbx, nby, dsigma);
     memcpy(sub, img, sizeof(float) * (size_t)W * H);
-    if (back_type == 1) {
-        /* BACK_TYPE MANUAL: subtract the constant BACK_VALUE (back.c:148-149, :807-813) */
-        const float bval = (float)back_value;
-        *backmean = bval;
+    if (back_type == 1 || back_type == 2) {
+        /* back_type 1, BACK_TYPE MANUAL as the reference's device code runs it: the
+           constant subtracted is the global median just computed (cudaback.cu:1336-1345);
+           *backmean keeps that median, as back_estimate returns it; the caller reports
+           BACK_VALUE as backmean afterwards (back.c:152-153).
+           back_type 2, stock SExtractor's MANUAL: the constant BACK_VALUE itself
+           (back.c:148-149, :807-813), which becomes backmean. */
+        const float bval = back_type == 1 ? *backmean : (float)back_value;
         for (size_t k = 0; k < (size_t)W * H; k++) sub[k] -= bval;
+        if (back_type == 2) *backmean = bval;
     } else {
         for (int y = 0; y < H; y++)
             sxo_subbackline(back, dback, nbx, nby, bs, W, y, sub + (size_t)y * W, NULL);

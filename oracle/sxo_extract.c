@@ -299,8 +299,10 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
     if (masked) {
         /* the maps come from the masked frame; the subtraction applies to the frame */
         memcpy(r->sub, img, sizeof(float) * N);
-        if (cfg->back_type == 1) {
-            const float bval = (float)cfg->back_value;
+        if (cfg->back_type == 1 || cfg->back_type == 2) {
+            /* the constant sxo_background subtracted (the median of the masked maps for
+               back_type 1, BACK_VALUE for 2) */
+            const float bval = cfg->back_type == 1 ? r->backmean : (float)cfg->back_value;
             for (size_t k = 0; k < N; k++) r->sub[k] -= bval;
         } else {
             for (int y = 0; y < H; y++)
@@ -308,6 +310,8 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         }
         free(masked);
     }
+    /* back.c:152-153: with BACK_TYPE MANUAL the reported backmean is BACK_VALUE */
+    if (cfg->back_type == 1) r->backmean = (float)cfg->back_value;
     if (wtype == 3) {
         r->var = (float *)malloc(sizeof(float) * N);
         for (int y = 0; y < H; y++)

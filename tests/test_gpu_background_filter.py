@@ -76,10 +76,38 @@ def test_synthetic_1024_background_and_filter_match_oracle(frame1024, masks, mas
     assert np.array_equal(_bits(cd), _bits(sxo.convolve(o["sub"], m)))
 
 
-def test_manual_background_subtracts_back_value(frame1024):
+def test_manual_background_subtracts_global_median(frame1024):
+    """BACK_TYPE MANUAL as the reference's device code runs it (cudaback.cu:1336-1345):
+    the constant subtracted is the global median of the filtered mesh map, which
+    back_estimate also returns; BACK_VALUE plays no part in the subtraction."""
     img = frame1024[:300, :257].copy()
+    auto = sxo.background(img, bs=64, fsize=3)
     o = sxo.background(img, bs=64, fsize=3, back_type=1, back_value=997.5)
+    assert o["backmean"] == auto["backmean"]
+    assert np.array_equal(_bits(o["sub"]), _bits(img - np.float32(auto["backmean"])))
     g = _gpu_background(img, 64, 3, back_type=1, back_value=997.5)
+    assert g["backmean"] == o["backmean"] and g["backsig"] == o["backsig"]
+    sub = sexgpu.subtracted(img.shape)
+    assert np.array_equal(_bits(sub), _bits(o["sub"]))
+
+
+def test_manual_background_reports_back_value_for_the_frame(frame1024):
+    """back.c:152-153: the whole-frame run reports BACK_VALUE as the frame's backmean."""
+    img = frame1024[:256, :256].copy()
+    p = sexgpu.Params(back_type=1, back_value=997.5, mask=None)
+    g = sexgpu.run_frame(img, p, outputs=("seg",))
+    o = sxo.run(img, mask=None, back_type=1, back_value=997.5)
+    assert g["backmean"] == np.float32(997.5) and o["backmean"] == np.float32(997.5)
+    assert np.array_equal(g["seg"], o["seg"])
+
+
+def test_stock_manual_background_subtracts_back_value(frame1024):
+    """back_type 2, stock SExtractor's MANUAL (back.c:148-149, :807-813): subtract the
+    constant BACK_VALUE, which becomes backmean."""
+    img = frame1024[:300, :257].copy()
+    o = sxo.background(img, bs=64, fsize=3, back_type=2, back_value=997.5)
+    assert np.array_equal(_bits(o["sub"]), _bits(img - np.float32(997.5)))
+    g = _gpu_background(img, 64, 3, back_type=2, back_value=997.5)
     assert g["backmean"] == np.float32(997.5)
     assert g["backsig"] == o["backsig"]
     sub = sexgpu.subtracted(img.shape)

@@ -126,7 +126,7 @@ def _flat_frame(h, w, blobs, value=0.0):
 
 def _run_abs(img, **kw):
     """Oracle with no filter, MANUAL background of 0 and absolute thresholds."""
-    base = dict(back_type=1, back_value=0.0, thresh_type=1, detect_thresh=1.0,
+    base = dict(back_type=2, back_value=0.0, thresh_type=1, detect_thresh=1.0,
                 analysis_thresh=1.0, back_size=min(64, *img.shape))
     base.update(kw)
     return sxo.run(img, mask=None, **base)
@@ -261,7 +261,7 @@ def test_fp64_truth_closed_form():
     img = np.zeros((20, 24), np.float32)
     vals = np.array([[10, 20, 30, 5], [40, 80, 60, 10], [15, 25, 35, 8]], np.float32)
     img[6:9, 9:13] = vals
-    r = sxo.run(img, mask=None, back_type=1, back_value=0.0, thresh_type=1, detect_thresh=1.0,
+    r = sxo.run(img, mask=None, back_type=2, back_value=0.0, thresh_type=1, detect_thresh=1.0,
                 analysis_thresh=1.0, back_size=20, clean=0, truth=True)
     assert r["nobj"] == 1
     t = r["truth"][0]

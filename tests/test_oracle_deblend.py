@@ -31,7 +31,7 @@ def gaussians(h, w, comps, sky=0.0):
 
 def run_abs(img, **kw):
     """No filter, MANUAL background 0, absolute thresholds."""
-    base = dict(back_type=1, back_value=0.0, thresh_type=1, detect_thresh=5.0,
+    base = dict(back_type=2, back_value=0.0, thresh_type=1, detect_thresh=5.0,
                 analysis_thresh=5.0, back_size=min(64, *img.shape), clean=0)
     base.update(kw)
     return sxo.run(img, mask=None, **base)

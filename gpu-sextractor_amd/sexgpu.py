@@ -357,6 +357,9 @@ def gpu():
         L.sxg_set_option.argtypes = [i, C.c_long]
         L.sxg_get_option.argtypes = [i]
         L.sxg_get_option.restype = C.c_long
+        L.sxg_back_estimate_noalloc.argtypes = [vp, vp, i, i, i, f, i, i, i, f, C.POINTER(f),
+                                                C.POINTER(f), vp]
+        L.sxg_back_estimate_noalloc.restype = i
         _gpu = L
     return _gpu
 
@@ -378,7 +381,7 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_ctx_destroy", "sxg_ctx_bind", "sxg_batch_set_frames", "sxg_batch_process",
             "sxg_batch_segmap", "sxg_batch_release", "sxg_set_weight",
             "sxg_set_photometry", "sxg_get_photometry", "sxg_set_weight_map",
-            "sxg_get_back_rms"]
+            "sxg_get_back_rms", "sxg_back_estimate_noalloc"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort",

@@ -340,6 +340,26 @@ enum sxg_option_id { SXG_OPT_DEB_LDS_SMALL = 0, SXG_OPT_DEB_LDS_LARGE, SXG_OPT_P
 int sxg_set_option(int id, long value);
 long sxg_get_option(int id);
 
+/* ---- allocation-free entry point ----------------------------------------------
+   No counterpart in the reference.  For callers that cannot allocate device memory
+   (code under HIP stream capture, tight serving loops, harnesses that forbid
+   hipMalloc): back_estimate's kernels on a frame the caller holds in device memory.
+   src: width x height floats (device), dst: the background-subtracted frame (device;
+   src == dst allowed), on stream (a hipStream_t; NULL: the null stream).  back_type 0
+   AUTO (spline), 1 MANUAL as the reference's device code (the global median), 2 MANUAL
+   with back_value.  backmean, backsig (host, may be NULL) receive the global medians
+   (back_value for type 2); the call waits for its stream.  No weight map.
+   Every intermediate table lives in one device array of SXG_NOALLOC_SCRATCH_BYTES per
+   GPU, held until the process ends: 8192 x 8192 with BACK_SIZE 64 needs 13.7 MB; a
+   frame whose tables need more is rejected (-1, sxg_last_error says how much).  Calls
+   that overlap on one GPU share that array and race.  Allocates nothing: no hipMalloc,
+   no hipFree, no context workspace.  Returns 0 or -1. */
+#define SXG_NOALLOC_SCRATCH_BYTES (16 * 1024 * 1024)
+int sxg_back_estimate_noalloc(const float *src, float *dst, int width, int height,
+                              int back_size, float backfthresh, int nbackfx, int nbackfy,
+                              int back_type, float back_value, float *backmean,
+                              float *backsig, void *stream);
+
 /* Catalogue row, layout of the reference's outobjstruct (src/types.h:417-455, NAPER 1). */
 typedef struct __attribute__((aligned(16))) {
     int number;

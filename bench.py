@@ -66,38 +66,68 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes per frame of each kernel family (its launches together), from the
 # frame size N (4-byte pixels) and the frame's counts (sxg_frame_info):
-#   copy: read + write the frame; meshstat/meshmode: read the frame; subback: read +
-#   write the frame; conv: read the subtracted frame + write the filtered frame;
-#   det_init: read cd, write parents and labels; det_fg: read the parents, write the
-#   foreground list; deb_levels: per pixel of the multi-peak raw objects read its index
-#   and cd (8 B), plus one 4-byte level label per written label; deb_objects/analyse:
-#   per raw pixel read its index, cd, the subtracted value and its label (16 B) and
-#   write its allocation (4 B).
+#   meshstat/meshmode: read the frame; subback: read + write the frame; conv: read the
+#   subtracted frame + write the filtered frame; det_init: read cd, write parents and
+#   labels; det_fg: read the parents, write the foreground list; deb_levels: per pixel of
+#   the multi-peak raw objects read its index and cd and write its kept node (12 B);
+#   deb_objects: per raw pixel read its index, cd, the subtracted value and its label
+#   (16 B) and write its allocation (4 B).
 ALG_BYTES = {
-    "copy": lambda N, i: 8 * N,
     "meshstat": lambda N, i: 4 * N,
     "meshmode": lambda N, i: 4 * N,
     "subback": lambda N, i: 8 * N,
     "conv": lambda N, i: 8 * N,
     "det_init": lambda N, i: 12 * N,
     "det_fg": lambda N, i: 4 * N + 4 * i["npix_above"],
-    # raw list entry and cd in, the kept-object index out, per pixel of a tree
     "deb_levels": lambda N, i: 12 * i["deb_pixels"],
     "deb_objects": lambda N, i: 20 * i["npix_raw"],
-    "analyse": lambda N, i: 20 * i["npix_raw"],
+}
+
+# The kernel timers that bracket exactly one kernel launch, the kernel's name (as the
+# rocprof kernel trace shows it) and its algorithmic HBM bytes per launch:
+#   k_meshstat, k_meshmode: read the frame (4N); k_subback: read it, write the subtracted
+#   frame (8N); k_convolve: read the subtracted frame, write the filtered one (8N);
+#   k_analyse: per raw pixel its index, value and filtered value (12 B), per object the
+#   ~176 bytes of catalogue columns it writes; k_deb_tree_lds (one launch per size
+#   class): per pixel of the class's objects its index and filtered value in, its kept
+#   node out (12 B).
+SINGLE_KERNELS = {
+    "meshstat": ("k_meshstat", lambda N, i: 4 * N),
+    "meshmode": ("k_meshmode", lambda N, i: 4 * N),
+    "subback": ("k_subback<true>", lambda N, i: 8 * N),
+    "conv": ("k_convolve<K>", lambda N, i: 8 * N),
+    "analyse": ("k_analyse", lambda N, i: 12 * i["npix_raw"] + 176 * i["nobj"]),
+    "deb_tree0": ("k_deb_tree_lds<256>", lambda N, i: 12 * i["deb_class_pixels"][0]),
+    "deb_tree1": ("k_deb_tree_lds<1024> (class 1)", lambda N, i: 12 * i["deb_class_pixels"][1]),
+    "deb_tree2": ("k_deb_tree_lds<1024> (class 2)", lambda N, i: 12 * i["deb_class_pixels"][2]),
 }
 
 
-# HBM bytes per frame of each kernel family, measured on the crowded 4096x4096 frame
-# (seed 1001) with two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by the
-# same counters on k_copy_frame, which moves a known 64 MiB each way (factors 2.0 and
-# 1.0); see profiles/hbm_traffic_crowded4096_pmc.md.  Used for roofline.traffic on that
-# workload only.
+# HBM bytes per launch of single kernels, measured on the crowded 4096x4096 frame (seed
+# 1001) with two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE), each scaled by what the
+# same counter reads for a plain 256 MiB device-to-device copy (bench.py --calibrate-copy);
+# see profiles/hbm_traffic_crowded4096_pmc.md.  Used for roofline.traffic on that workload.
 MEASURED_TRAFFIC_CROWDED4096 = {
-    "deb_levels": 1.271e7, "meshstat": 1.346e8, "meshmode": 6.857e7, "analyse": 6.633e7,
-    "conv": 2.003e8, "subback": 1.51e8, "det_init": 2.013e8, "det_union": 7.577e7,
-    "copy": 1.342e8, "clean": 1.08e7, "rownodes": 2.26e6,
+    "meshstat": 1.346e8, "meshmode": 6.857e7, "analyse": 6.633e7, "conv": 2.003e8,
+    "subback": 1.51e8,
 }
+
+
+def roofline_entry(ktab, N, info, traffic=None):
+    """The roofline object of the dominant kernel: among the timers that bracket one
+    kernel launch, the one with the largest time per frame; achieved = its algorithmic
+    bytes per launch / its mean HIP-event duration (recorded on the stream it runs on)."""
+    cands = [k for k in ktab if k in SINGLE_KERNELS]
+    dom = max(cands, key=lambda k: ktab[k][0])
+    tot, nl = ktab[dom]
+    avg_s = tot / nl / 1000.0
+    name, alg = SINGLE_KERNELS[dom]
+    bytes_launch = int(alg(N, info))
+    achieved = bytes_launch / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": (traffic or {}).get(dom), "kernel": name, "timer": dom,
+            "kernel_ms": round(avg_s * 1e3, 4), "algorithmic_bytes": bytes_launch}
 
 
 def workload_params():
@@ -116,18 +146,61 @@ def kernel_table(L):
     return out
 
 
-def cpu_baseline(img, p, gpu_seg, gpu_nobj):
+def cpu_info():
+    """The host the CPU baseline runs on: logical CPUs, the ones this process may use, and
+    the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
+def oracle_all_core(frames, p, threads):
+    """The oracle on every frame, `threads` frames at a time (one frame per thread, as
+    independent frames shard on a multi-core host); returns (seconds, total pixels,
+    total catalogue objects)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import sxo  # the oracle: test infrastructure, used here only as the CPU baseline
+    from concurrent.futures import ThreadPoolExecutor
+    t = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(lambda f: sxo.run(f, mask=p.mask)["nsurv"], frames))
+    dt = time.perf_counter() - t
+    return dt, sum(f.size for f in frames), sum(res)
+
+
+def cpu_baseline(frames, p, gpu_seg, gpu_nobj):
+    """The oracle (the sequential C restatement, -O3, no FP contraction) on the host: one
+    core on frame 0 with its per-stage times (value), then every frame of the batch with
+    one frame per core."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import sxo  # the oracle: test infrastructure, used here only as the CPU baseline
+    img = frames[0]
     t = time.perf_counter()
     o = sxo.run(img, mask=p.mask)
     dt = time.perf_counter() - t
     H, W = img.shape
     parity = bool(o["nobj"] == gpu_nobj and np.array_equal(o["seg"], gpu_seg))
-    return {"value": round(W * H / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": "one full %dx%d frame of the same workload (%.1f s, single thread)"
-                      % (W, H, dt),
-            "objects_per_s": round(o["nsurv"] / dt, 1)}, parity
+    info = cpu_info()
+    threads = max(1, min(len(frames), info["usable_cpus"], 16))
+    dta, npx, nobj = oracle_all_core(frames, p, threads)
+    out = {"value": round(W * H / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+           "sample": "one full %dx%d frame of the same workload (%.2f s, single thread); "
+                     "all-core: the %d frames of the batch on %d threads (%.2f s)"
+                     % (W, H, dt, len(frames), threads, dta),
+           "objects_per_s": round(o["nsurv"] / dt, 1),
+           "stage_ms": {k: round(v, 1) for k, v in o["stage_ms"].items()},
+           "all_core_mpix_s": round(npx / dta / 1e6, 2),
+           "all_core_objects_per_s": round(nobj / dta, 1), "all_core_threads": threads}
+    out.update(info)
+    return out, parity
 
 
 def timed_batches(L, sp, nworkers, steps, B):
@@ -258,10 +331,6 @@ def wide8192(args):
         parity = bool(o["nobj"] == info["nobj"] and np.array_equal(o["seg"], seg))
         cpu = {"value": round(N / dtc / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
                "sample": "one full %dx%d weighted frame (%.1f s, single thread)" % (W, H, dtc)}
-    dom = max((k for k in ktab if k in ALG_BYTES), key=lambda k: ktab[k][0])
-    tot, nl = ktab[dom]
-    avg_s = tot / nl / 1000.0
-    achieved = ALG_BYTES[dom](N, info) / avg_s / 1e9
     line = {
         "metric": "Mpix/s + objects/s at 8192^2 with MAP_WEIGHT (wide field); bit-exact seg map",
         "value": round(N * steps / dt / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": steps,
@@ -273,9 +342,7 @@ def wide8192(args):
                    "objects_per_s": round(info["nsurv"] * steps / dt, 1),
                    "objects": info["nsurv"], "pixels_above_threshold": info["npix_above"],
                    "parity_with_oracle": parity},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": dom, "kernel_ms": round(avg_s * 1e3, 4)},
+        "roofline": roofline_entry(ktab, N, info),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -430,10 +497,6 @@ def dense4096(args):
         parity = bool(o["nobj"] == info["nobj"] and np.array_equal(o["seg"], seg))
         cpu = {"value": round(N / dtc / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
                "sample": "one full 4096x4096 dense frame (%.1f s, single thread)" % dtc}
-    dom = max((k for k in ktab if k in ALG_BYTES), key=lambda k: ktab[k][0])
-    tot, nl = ktab[dom]
-    avg_s = tot / nl / 1000.0
-    achieved = ALG_BYTES[dom](N, info) / avg_s / 1e9
     line = {
         "metric": "Mpix/s + objects/s at 4096^2 dense (7.6M detected pixels); bit-exact seg map",
         "value": round(N * steps / dt / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": steps,
@@ -445,9 +508,7 @@ def dense4096(args):
                    "objects_per_s": round(info["nsurv"] * steps / dt, 1),
                    "objects": info["nsurv"], "pixels_above_threshold": info["npix_above"],
                    "deblending_pixels": info["deb_pixels"], "parity_with_oracle": parity},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": dom, "kernel_ms": round(avg_s * 1e3, 4)},
+        "roofline": roofline_entry(ktab, N, info),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -569,11 +630,17 @@ def main():
                     choices=["crowded", "batch64", "wide8192", "hdr4096", "photom",
                              "dense4096"])
     ap.add_argument("--sweep", default="", help="worker counts to time, e.g. 1,2,4,8")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 256 MiB device to device (traffic counter calibration)")
+    ap.add_argument("--no-end-to-end", action="store_true",
+                    help="skip the PCIe-inclusive legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (a frame or a batch "
               "of frames runs on one MI355X); run with --gpus 1", file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        calibrate_copy()
     if args.workload == "batch64":
         return batch64(args)
     if args.workload == "wide8192":
@@ -645,6 +712,7 @@ def main():
     segs_batch = batch_segs(L, frames)
     batch_same = all(np.array_equal(a, b) for a, b in zip(segs_plain, segs_batch))
     batch_same = batch_same and np.array_equal(segs_plain[0], seg0)
+    L.sxg_batch_release()
     nsurv = sum(f.nsurv for f in infb)
     mpix_plain = B * N * plain_steps / dtp / 1e6
     mpix_batch = B * N * args.steps / dtb / 1e6
@@ -656,25 +724,18 @@ def main():
                                                  mpix_batch / mpix_plain, np.percentile(lat, 50),
                                                  np.percentile(lat, 99)), file=sys.stderr, flush=True)
 
+    # ---- end to end from host memory
+    e2e = None if args.no_end_to_end else end_to_end(L, frames, p, sp, args, seg0, segs_plain)
+
     # ---- roofline of the dominant kernel (single-frame run)
-    dom = max((k for k in ktab if k in ALG_BYTES), key=lambda k: ktab[k][0])
-    tot, nl = ktab[dom]
-    avg_s = tot / nl / 1000.0
-    bytes_launch = ALG_BYTES[dom](N, info)
-    achieved = bytes_launch / avg_s / 1e9
-    traffic = None
-    if args.size == 4096 and dom in MEASURED_TRAFFIC_CROWDED4096:
-        traffic = MEASURED_TRAFFIC_CROWDED4096[dom]
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": dom, "kernel_ms": round(avg_s * 1e3, 4),
-                "algorithmic_bytes": bytes_launch,
-                "share_of_frame": round(tot / (dt1 * 1e3), 3)}
+    traffic = MEASURED_TRAFFIC_CROWDED4096 if args.size == 4096 else None
+    roofline = roofline_entry(ktab, N, info, traffic)
+    roofline["share_of_frame"] = round(ktab[roofline["timer"]][0] / (dt1 * 1e3), 3)
     # whole path against the per-frame algorithmic bytes 12N + 12 P_all
     balg = 12 * N + 12 * info["npix_above"]
     cpu, parity = (None, None)
     if not args.no_cpu_baseline:
-        cpu, parity = cpu_baseline(img, p, seg0, nobj)
+        cpu, parity = cpu_baseline(frames, p, seg0, nobj)
     line = {
         "metric": "Mpix/s + objects/s at 4096^2 (~167k obj), batched vs plain; bit-exact seg map",
         "value": round(mpix_batch, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
@@ -693,12 +754,99 @@ def main():
                    "objects_per_frame": info["nsurv"], "deblended_per_frame": info["nobj"],
                    "pixels_above_threshold": info["npix_above"],
                    "whole_path_hbm_frac": round(balg / (dtb / args.steps / B) / 1e12 / 8.0, 5),
+                   "end_to_end": e2e,
                    "parity_with_oracle": parity, "batched_equals_plain": bool(batch_same)},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    L.sxg_batch_release()
     return 0
+
+
+def end_to_end(L, frames, p, sp, args, seg0, segs_ref):
+    """End-to-end rates, PCIe included: the frame copied in from host memory, every kernel,
+    the segmentation map and the catalogue rows copied out.
+      plain: the reference's thirteen calls on frame 0 in scan.c order (init_device ...
+             clear_device, which also copies the subtracted frame out), plus sxg_get_segmap
+             and sxg_get_catalog; with pageable host buffers, as the reference front-end
+             passes them, and with pinned ones;
+      batched: sxg_batch_process_host over the batch with W workers, each copying its
+             frame in and its results out on its own stream (pinned buffers)."""
+    H, W = frames[0].shape
+    N = W * H
+    B = len(frames)
+    cap = N // 64
+    res = {}
+    steps = max(2, args.steps // 2)
+    for label, pinned in (("plain_pageable", False), ("plain_pinned", True)):
+        keep = []
+
+        def buf(shape, dtype):
+            if not pinned:
+                return np.zeros(shape, dtype)
+            a = sexgpu.PinnedArray(shape, dtype)
+            keep.append(a)
+            return a.array
+
+        img, seg, sub = buf((H, W), np.float32), buf((H, W), np.int32), buf((H, W), np.float32)
+        img[...] = frames[0]
+        master, rows = buf(cap, np.uint32), buf(cap, sexgpu.ISOROW_DTYPE)
+        sexgpu.run_drop_in(img, p, seg, sub, master, rows)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            nobj, nsurv = sexgpu.run_drop_in(img, p, seg, sub, master, rows)
+        dt = (time.perf_counter() - t0) / steps
+        res[label] = {"ms_per_frame": round(dt * 1e3, 3), "mpix_s": round(N / dt / 1e6, 1),
+                      "objects_per_s": round(nsurv / dt, 1),
+                      "seg_equals_resident": bool(np.array_equal(seg, seg0))}
+        print("# end to end, %s: %.3f ms per frame" % (label, dt * 1e3), file=sys.stderr,
+              flush=True)
+    # the batched path from pinned host memory
+    keep = [sexgpu.PinnedArray(f.shape, np.float32) for f in frames]
+    for a, f in zip(keep, frames):
+        a.array[...] = f
+    psegs = [sexgpu.PinnedArray(f.shape, np.int32) for f in frames]
+    prows = [sexgpu.PinnedArray(cap, sexgpu.ISOROW_DTYPE) for _ in frames]
+    fin = [a.array for a in keep]
+    segs = [a.array for a in psegs]
+    rows = [a.array for a in prows]
+    sexgpu.process_batch_host(fin, p, args.workers, segs, rows)
+    ms_all = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        infos, ms, _, _ = sexgpu.process_batch_host(fin, p, args.workers, segs, rows)
+        ms_all += ms
+    dt = time.perf_counter() - t0
+    same = all(np.array_equal(a, b) for a, b in zip(segs, segs_ref))
+    nsurv = sum(fi.nsurv for fi in infos)
+    res["batched_pinned"] = {"workers": args.workers,
+                             "ms_per_frame": round(dt / args.steps / B * 1e3, 3),
+                             "mpix_s": round(B * N * args.steps / dt / 1e6, 1),
+                             "objects_per_s": round(nsurv * args.steps / dt, 1),
+                             "frame_latency_ms_p50": round(float(np.percentile(ms_all, 50)), 3),
+                             "frame_latency_ms_p99": round(float(np.percentile(ms_all, 99)), 3),
+                             "seg_equals_resident": bool(same)}
+    print("# end to end, batched (%d workers, pinned): %.1f Mpix/s" %
+          (args.workers, res["batched_pinned"]["mpix_s"]), file=sys.stderr, flush=True)
+    L.sxg_batch_release()
+    return res
+
+
+def calibrate_copy():
+    """One device-to-device copy of 256 MiB (the HIP runtime's copy kernel): the dispatch
+    whose FETCH_SIZE and WRITE_SIZE calibrate the traffic counters (known bytes each way)."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    n = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), n) != 0 or hip.hipMalloc(C.byref(b), n) != 0:
+        raise RuntimeError("calibration buffers")
+    hip.hipMemcpy(b, a, n, 3)
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("# calibration copy: 268435456 bytes device to device", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

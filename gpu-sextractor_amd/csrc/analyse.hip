@@ -400,10 +400,10 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
     if (!order) return;
     A.order = order;
     sxg_stage_begin(ST_ANALYSE);
-    sxg_kt_begin(SXG_KT_ANALYSE);
     SXG_HIP(hipMemsetAsync(order + numobj, 0, 2 * sizeof(int), S.stream));
     hipLaunchKernelGGL(k_obj_order, dim3(sxg_div_up(numobj, 256)), dim3(256), 0, S.stream,
                        S.objoff, numobj, order + numobj, order);
+    sxg_kt_begin(SXG_KT_ANALYSE);
     hipLaunchKernelGGL(k_analyse, dim3(sxg_div_up(numobj, 4)), dim3(256), 0, S.stream, A);
     sxg_kt_end(SXG_KT_ANALYSE);
     sxg_stage_end(ST_ANALYSE);

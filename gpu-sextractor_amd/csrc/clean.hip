@@ -176,6 +176,21 @@ __global__ void __launch_bounds__(256) k_seg_scatter(const int *fpia, long P0, c
     seg[p] = mode ? o : (keep[o] ? rank[o] + 1 : 0);
 }
 
+/* the survivors of CLEAN: keep[i], rank[i] (catalogue NUMBER - 1 of a survivor) and their
+   number in tot[0], all on the device (queued on the context's stream) */
+static bool survivor_ranks(const ObjCols &o, int *&keep, int *&rank, int *&tot)
+{
+    SxgState &S = sxg_state();
+    keep = sxg_take<int>("out.keep", S.nobj);
+    rank = sxg_take<int>("out.rank", S.nobj);
+    tot = sxg_take<int>("out.total", 4);
+    if (!keep || !rank || !tot) return false;
+    hipLaunchKernelGGL(k_keepflag, dim3(sxg_div_up(S.nobj, 256)), dim3(256), 0, S.stream,
+                       o.master, S.nobj, keep);
+    sxg_scan_excl(keep, rank, S.nobj, tot, S.stream);
+    return true;
+}
+
 /* 0: segmentation map, 1: object index map; into the device buffer "out.map" */
 static int *build_map(int mode)
 {
@@ -185,16 +200,19 @@ static int *build_map(int mode)
     const long N = (long)S.N;
     hipLaunchKernelGGL(k_fill_i32, dim3(sxg_div_up(sxg_div_up(N, 4), 256)), dim3(256), 0,
                        S.stream, map, N, mode ? -1 : 0);
-    if (S.nobj == 0 || S.npix_raw == 0) return map;
+    if (S.nobj == 0 || S.npix_raw == 0) {
+        if (mode == 0) S.nsurv = 0;
+        return map;
+    }
     ObjCols o;
     if (!sxg_obj_view(o)) return nullptr;
-    int *keep = sxg_take<int>("out.keep", S.nobj), *rank = sxg_take<int>("out.rank", S.nobj);
-    int *tot = sxg_take<int>("out.total", 4);
-    if (!keep || !rank || !tot) return nullptr;
+    int *keep = nullptr, *rank = nullptr, *tot = nullptr;
     if (mode == 0) {
-        hipLaunchKernelGGL(k_keepflag, dim3(sxg_div_up(S.nobj, 256)), dim3(256), 0, S.stream,
-                           o.master, S.nobj, keep);
-        sxg_scan_excl(keep, rank, S.nobj, tot, S.stream);
+        if (!survivor_ranks(o, keep, rank, tot)) return nullptr;
+    } else {
+        keep = sxg_take<int>("out.keep", S.nobj);
+        rank = sxg_take<int>("out.rank", S.nobj);
+        if (!keep || !rank) return nullptr;
     }
     sxg_kt_begin(SXG_KT_SEG);
     hipLaunchKernelGGL(k_seg_scatter, dim3(sxg_div_up(S.npix_raw, 256)), dim3(256), 0, S.stream,
@@ -235,6 +253,84 @@ extern "C" int sxg_get_alloc_map(int32_t *out)
     SXG_HIP_RET(hipMemcpyAsync(out, map, sizeof(int) * S.N, hipMemcpyDeviceToHost, S.stream), -1);
     SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
     return 0;
+}
+
+/* ---------------------------------------------------------- catalogue rows ---- */
+/* one row per survivor of CLEAN, at its catalogue NUMBER - 1 */
+__global__ void __launch_bounds__(256) k_isorows(ObjCols o, int nobj, const int *keep,
+                                                 const int *rank, sxg_isorow *rows)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nobj || !keep[i]) return;
+    sxg_isorow r;
+    r.number = rank[i] + 1;
+    r.flags = o.flag[i];
+    r.isoarea = o.npix[i];
+    r.fdnpix = o.fdnpix[i];
+    /* X_IMAGE, Y_IMAGE are 1-based (cudaanalyse.cu:2715-2716) */
+    r.x = o.mx[i] + 1.0;
+    r.y = o.my[i] + 1.0;
+    r.x2 = o.mx2[i];
+    r.y2 = o.my2[i];
+    r.xy = o.mxy[i];
+    r.a = o.a[i];
+    r.b = o.b[i];
+    r.theta = o.theta[i];
+    r.cxx = o.cxx[i];
+    r.cyy = o.cyy[i];
+    r.cxy = o.cxy[i];
+    r.flux_iso = o.flux[i];
+    r.fluxerr_iso = sqrtf(o.fluxerr[i]);
+    r.peak = o.peak[i];
+    r.bkg = o.bkg[i];
+    rows[rank[i]] = r;
+}
+
+/* Queue the rows of the survivors and their copy to out[0..min(nsurv, cap)) on the
+   context's stream (S.nsurv must be current: after the segmentation map).  Returns the
+   number of rows copied or -1. */
+int sxg_catalog_async(sxg_isorow *out, int cap)
+{
+    SxgState &S = sxg_state();
+    const int n = S.nsurv < cap ? S.nsurv : cap;
+    if (n <= 0 || S.nobj <= 0) return 0;
+    ObjCols o;
+    if (!sxg_obj_view(o)) return -1;
+    int *keep = nullptr, *rank = nullptr, *tot = nullptr;
+    if (!survivor_ranks(o, keep, rank, tot)) return -1;
+    sxg_isorow *rows = sxg_take<sxg_isorow>("out.rows", (size_t)S.nsurv);
+    if (!rows) return -1;
+    hipLaunchKernelGGL(k_isorows, dim3(sxg_div_up(S.nobj, 256)), dim3(256), 0, S.stream, o,
+                       S.nobj, keep, rank, rows);
+    SXG_HIP_RET(hipGetLastError(), -1);
+    SXG_HIP_RET(hipMemcpyAsync(out, rows, sizeof(sxg_isorow) * n, hipMemcpyDeviceToHost,
+                               S.stream), -1);
+    return n;
+}
+
+extern "C" int sxg_get_catalog(sxg_isorow *out, int cap)
+{
+    SxgState &S = sxg_state();
+    sxg_clear_error();
+    if (!S.cleaned || (cap > 0 && !out) || cap < 0) {
+        sxg_set_error("sxg_get_catalog: call after run_clean, with an output buffer");
+        return -1;
+    }
+    if (S.nobj > 0) {
+        /* the number of survivors, read back once */
+        ObjCols o;
+        int *keep = nullptr, *rank = nullptr, *tot = nullptr;
+        if (!sxg_obj_view(o) || !survivor_ranks(o, keep, rank, tot)) return -1;
+        int ns = 0;
+        SXG_HIP_RET(hipMemcpyAsync(&ns, tot, sizeof(int), hipMemcpyDeviceToHost, S.stream), -1);
+        SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
+        S.nsurv = ns;
+    } else {
+        S.nsurv = 0;
+    }
+    if (sxg_catalog_async(out, cap) < 0) return -1;
+    SXG_HIP_RET(hipStreamSynchronize(S.stream), -1);
+    return S.nsurv;
 }
 
 /* ------------------------------------------------------------------ columns ---- */
@@ -309,6 +405,7 @@ extern "C" int sxg_get_frame_info(sxg_frame_info *info)
     info->nby = S.nby;
     info->deb_pixels = S.deb_pixels;
     info->deb_labels = S.deb_labels;
+    for (int c = 0; c < 4; c++) info->deb_class_pixels[c] = S.deb_class_pixels[c];
     info->phot_pixels = S.phot_pixels;
     return 0;
 }

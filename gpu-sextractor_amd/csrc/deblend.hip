@@ -75,6 +75,8 @@
 /* pixels per lane whose global-memory accesses the tree passes keep in flight together */
 #define DEB_ILP 4
 #define DEB_RUN 8
+/* words of the per-frame counter block (DebArgs::ccount) */
+#define DEB_NCOUNT 14
 
 /* unsigned order-preserving key of a float (0 is below every non-NaN value) */
 __device__ inline unsigned uord(float v) { return (unsigned)f2ord(v) ^ 0x80000000u; }
@@ -341,7 +343,8 @@ struct DebArgs {
     int *clist;             /* objects of each size class: clist[cls * nraw + j] */
     int *ccount;            /* [0..3] objects per class, [4..6] the largest LDS need of
                                classes 0..2 (bytes), [7] consistency error flag, [8] class 3
-                               objects built level-parallel, [9] order counter */
+                               objects built level-parallel, [9] order counter, [10..13]
+                               pixels per class */
     unsigned long long *dwork;  /* [0] pixels, [1] pixel-levels of the multi-peak objects,
                                    [2] node capacity of the frame, [3] pixel-levels and
                                    [4] node records of the objects built in global memory */
@@ -388,7 +391,7 @@ template <int PASS> __global__ void __launch_bounds__(256) k_deb_lmax(DebArgs a)
 __global__ void __launch_bounds__(256) k_deb_prep(DebArgs a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r < 10) a.ccount[r] = 0;
+    if (r < DEB_NCOUNT) a.ccount[r] = 0;
     if (r < 5) a.dwork[r] = 0ull;
     if (r >= a.nraw) return;
     a.top1[r] = 0ull;
@@ -492,9 +495,13 @@ __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int c
         int base = 0;
         if (lane == lead) base = atomicAdd(&a.ccount[c], __popcll(b));
         base = __shfl(base, lead);
-        int mx = cls == c ? need : 0;
-        for (int d = 32; d > 0; d >>= 1) mx = max(mx, __shfl_xor(mx, d));
+        int mx = cls == c ? need : 0, px = cls == c ? n : 0;
+        for (int d = 32; d > 0; d >>= 1) {
+            mx = max(mx, __shfl_xor(mx, d));
+            px += __shfl_xor(px, d);
+        }
         if (lane == lead && c < 3) atomicMax(&a.ccount[4 + c], mx);
+        if (lane == lead) atomicAdd(&a.ccount[10 + c], px);
         if (cls == c) a.clist[(size_t)c * a.nraw + base + __popcll(b & sxw_lanes_below(lane))] = r;
     }
 }
@@ -1797,7 +1804,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     int *nmax = sxg_take<int>("deb.nmax", nr);
     int *lstar = sxg_take<int>("deb.lstar", nr);
     int *clist = sxg_take<int>("deb.clist", 4 * nr);
-    int *ccount = sxg_take<int>("deb.ccount", 10);
+    int *ccount = sxg_take<int>("deb.ccount", DEB_NCOUNT);
     auto *dwork = sxg_take<unsigned long long>("deb.dwork", 6);
     auto *lvbase = sxg_take<unsigned long long>("deb.lvbase", 2 * (size_t)nr);
     int *kcomp = sxg_take<int>("deb.kcomp", (size_t)(P0 > 0 ? P0 : 1));
@@ -1815,6 +1822,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     a.top1 = top1; a.top2 = top2; a.nmax = nmax; a.lstar = lstar; a.clist = clist;
     a.ccount = ccount; a.dwork = dwork; a.lvbase = lvbase;
     S.deb_pixels = S.deb_labels = 0;
+    for (int c = 0; c < 4; c++) S.deb_class_pixels[c] = 0;
     int nobj = 0;
     if (nraw > 0) {
         const int cap0 = (int)sxg_option(SXG_OPT_DEB_LDS_TINY);
@@ -1842,7 +1850,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         hipLaunchKernelGGL(k_deb_classify, dim3(gr), dim3(256), 0, st, a, cap0, cap1, cap2, ct0,
                            ct1, ct2);
         sxg_kt_end(SXG_KT_DEB_LMAX);
-        int cc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int cc[DEB_NCOUNT] = {};
         int ntot = 0;
         unsigned long long dw[5] = {0, 0, 0, 0, 0};
         SXG_HIP_RET(hipMemcpyAsync(cc, ccount, sizeof(cc), hipMemcpyDeviceToHost, st), -1);
@@ -1854,6 +1862,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             return -1;
         }
         S.deb_pixels = (int64_t)dw[0];
+        for (int c = 0; c < 4; c++) S.deb_class_pixels[c] = cc[10 + c];
         S.deb_labels = (int64_t)dw[1];
         a.comps = sxg_take<Comp>("deb.comps", (size_t)ntot + 1);
         if (!a.comps) return -1;
@@ -1889,7 +1898,10 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             void *args[] = {&a, &cls};
             const bool side = cls != 0 && use_side;
             hipStream_t ks = side ? S.side[nside] : st;
+            /* each class kernel is timed on the stream it runs on */
+            sxg_kt_begin_on(SXG_KT_DEB_TREE0 + cls, ks);
             SXG_HIP_RET(hipLaunchKernel(fn, dim3(cc[cls]), dim3(nt), args, lds, ks), false);
+            sxg_kt_end_on(SXG_KT_DEB_TREE0 + cls, ks);
             if (side) SXG_HIP_RET(hipEventRecord(S.side_ev[nside++], ks), false);
             return true;
         };
@@ -1924,7 +1936,9 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
                     SXG_HIP_RET(hipEventRecord(S.side_ev[nside], ks), -1);
                     SXG_HIP_RET(hipStreamWaitEvent(ks2, S.side_ev[nside], 0), -1);
                 }
+                sxg_kt_begin_on(SXG_KT_DEB_TREE_GLB, ks2);
                 hipLaunchKernelGGL(k_deb_tree_glb, dim3(rest), dim3(1024), 0, ks2, a3, big);
+                sxg_kt_end_on(SXG_KT_DEB_TREE_GLB, ks2);
                 if (use_side) SXG_HIP_RET(hipEventRecord(S.side_ev[nside + 1], ks2), -1);
             }
             if (big > 0) {
@@ -1936,12 +1950,14 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
                 float *frbuf = sxg_take<float>("deb.glbflux", (size_t)big);
                 double *v0 = sxg_take<double>("deb.glbv0", 2 * (size_t)big);
                 if (!lvpar || !lvnid || !lvrec || !lvk || !frbuf || !v0) return -1;
+                sxg_kt_begin_on(SXG_KT_DEB_LVPAR, ks);
                 hipLaunchKernelGGL(k_deb_glb_prep, dim3(big), dim3(1024), 0, ks, a3, v0);
                 hipLaunchKernelGGL(k_deb_glb_levels, dim3(n_thresh, big), dim3(1024), 0, ks, a3,
                                    lvpar, lvnid, lvrec, lvk, frbuf, (const double *)v0);
                 hipLaunchKernelGGL(k_deb_glb_tree, dim3(big), dim3(1024), 0, ks, a3,
                                    (const int *)lvnid, (const LvNode *)lvrec, (const int *)lvk,
                                    (const float *)frbuf);
+                sxg_kt_end_on(SXG_KT_DEB_LVPAR, ks);
             }
             if (use_side) {
                 SXG_HIP_RET(hipEventRecord(S.side_ev[nside], ks), -1);

@@ -16,6 +16,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -49,6 +50,29 @@ void sxg_kt_end(int id)
     KTimer &t = g_kt[id];
     if ((int)t.ev.size() < 2 * (t.used + 1)) return;
     (void)hipEventRecord(t.ev[2 * t.used + 1], sxg_state().stream);
+    t.used++;
+}
+
+/* the same, recorded on stream st (kernels launched on a side stream) */
+void sxg_kt_begin_on(int id, hipStream_t st)
+{
+    if (!g_kt_on || !sxg_state().is_default) return;
+    KTimer &t = g_kt[id];
+    if ((int)t.ev.size() < 2 * (t.used + 1)) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+        t.ev.push_back(a);
+        t.ev.push_back(b);
+    }
+    (void)hipEventRecord(t.ev[2 * t.used], st);
+}
+
+void sxg_kt_end_on(int id, hipStream_t st)
+{
+    if (!g_kt_on || !sxg_state().is_default) return;
+    KTimer &t = g_kt[id];
+    if ((int)t.ev.size() < 2 * (t.used + 1)) return;
+    (void)hipEventRecord(t.ev[2 * t.used + 1], st);
     t.used++;
 }
 
@@ -90,12 +114,6 @@ extern "C" int sxg_set_resident_frame(const float *host, int width, int height)
     S.rw = width;
     S.rh = height;
     return 0;
-}
-
-__global__ void __launch_bounds__(256) k_copy_frame(const float4 *a, float4 *b, long n4)
-{
-    const long k = (long)blockIdx.x * 256 + threadIdx.x;
-    if (k < n4) b[k] = a[k];
 }
 
 /* the whole path on a device frame (not modified), on the current context */
@@ -231,19 +249,21 @@ extern "C" int sxg_batch_set_frames(int n, const float *const *host, const int *
     return 0;
 }
 
-extern "C" int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_info *info,
-                                 double *frame_ms)
+/* The worker pool of the batched paths: frames 0..n-1 are handed out in order to
+   nworkers contexts, one host thread each (the caller's own binding is untouched);
+   run(i) processes frame i on the calling thread's context and returns false (error
+   recorded) on failure.  frame_ms[i]: wall time of run(i). */
+static int run_pool(int n, int nworkers, const std::function<bool(int)> &run, double *frame_ms,
+                    const char *who)
 {
-    sxg_clear_error();
-    const int n = (int)g_batch.size();
-    if (!p || nworkers < 1 || nworkers > SXG_MAX_WORKERS) {
-        sxg_set_error("sxg_batch_process: nworkers must be 1.." + std::to_string(SXG_MAX_WORKERS));
+    if (nworkers < 1 || nworkers > SXG_MAX_WORKERS) {
+        sxg_set_error(std::string(who) + ": nworkers must be 1.." + std::to_string(SXG_MAX_WORKERS));
         return -1;
     }
     while ((int)g_workers.size() < nworkers) {
         sxg_ctx *c = sxg_ctx_create();
         if (!c) {
-            sxg_set_error("sxg_batch_process: context allocation failed");
+            sxg_set_error(std::string(who) + ": context allocation failed");
             return -1;
         }
         /* one stream per worker: the frames in flight already fill the hardware queues */
@@ -257,22 +277,10 @@ extern "C" int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_in
         sxg_clear_error();
         for (int i = next++; i < n && !failed; i = next++) {
             const auto t0 = std::chrono::steady_clock::now();
-            BatchFrame &f = g_batch[i];
-            sxg_frame_info fi;
-            if (process_frame(f.frame.as<float>(), f.W, f.H, p, &fi) < 0) {
+            if (!run(i)) {
                 failed = 1;
                 break;
             }
-            SxgState &S = sxg_state();
-            int *map = sxg_take<int>("out.map", S.N);
-            if (!map || hipMemcpyAsync(f.seg.ptr, map, S.N * sizeof(int), hipMemcpyDeviceToDevice,
-                                       S.stream) != hipSuccess ||
-                hipStreamSynchronize(S.stream) != hipSuccess) {
-                sxg_set_error("sxg_batch_process: segmentation copy failed");
-                failed = 1;
-                break;
-            }
-            if (info) info[i] = fi;
             if (frame_ms)
                 frame_ms[i] = std::chrono::duration<double, std::milli>(
                                   std::chrono::steady_clock::now() - t0).count();
@@ -280,7 +288,6 @@ extern "C" int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_in
         if (sxg_has_error()) errs[w] = sxg_last_error();
         sxg_ctx_bind(nullptr);
     };
-    /* every worker is a thread of its own, so the caller's context binding is untouched */
     std::vector<std::thread> th;
     for (int w = 0; w < nworkers; w++) th.emplace_back(work, w);
     for (auto &t : th) t.join();
@@ -290,10 +297,87 @@ extern "C" int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_in
                 sxg_set_error(e);
                 break;
             }
-        if (!sxg_has_error()) sxg_set_error("sxg_batch_process: a frame failed");
+        if (!sxg_has_error()) sxg_set_error(std::string(who) + ": a frame failed");
         return -1;
     }
     return n;
+}
+
+extern "C" int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_info *info,
+                                 double *frame_ms)
+{
+    sxg_clear_error();
+    const int n = (int)g_batch.size();
+    if (!p) {
+        sxg_set_error("sxg_batch_process: no parameters");
+        return -1;
+    }
+    auto run = [&](int i) {
+        BatchFrame &f = g_batch[i];
+        sxg_frame_info fi;
+        if (process_frame(f.frame.as<float>(), f.W, f.H, p, &fi) < 0) return false;
+        SxgState &S = sxg_state();
+        int *map = sxg_take<int>("out.map", S.N);
+        if (!map || hipMemcpyAsync(f.seg.ptr, map, S.N * sizeof(int), hipMemcpyDeviceToDevice,
+                                   S.stream) != hipSuccess ||
+            hipStreamSynchronize(S.stream) != hipSuccess) {
+            sxg_set_error("sxg_batch_process: segmentation copy failed");
+            return false;
+        }
+        if (info) info[i] = fi;
+        return true;
+    };
+    return run_pool(n, nworkers, run, frame_ms, "sxg_batch_process");
+}
+
+extern "C" int sxg_batch_process_host(const sxg_params *p, int nworkers, int n,
+                                      const float *const *frames, const int *width,
+                                      const int *height, int32_t *const *segs,
+                                      sxg_isorow *const *rows, const int *row_cap,
+                                      sxg_frame_info *info, double *frame_ms)
+{
+    sxg_clear_error();
+    if (!p || n < 0 || (n > 0 && (!frames || !width || !height || !segs))) {
+        sxg_set_error("sxg_batch_process_host: bad arguments");
+        return -1;
+    }
+    for (int i = 0; i < n; i++)
+        if (width[i] <= 0 || height[i] <= 0 || (long)width[i] * height[i] >= 2147483647L ||
+            !frames[i] || !segs[i] || (rows && rows[i] && (!row_cap || row_cap[i] < 0))) {
+            sxg_set_error("sxg_batch_process_host: bad frame " + std::to_string(i));
+            return -1;
+        }
+    auto run = [&](int i) {
+        const int W = width[i], H = height[i];
+        const size_t N = (size_t)W * H;
+        if (!sxg_ensure_stream()) return false;
+        SxgState &S = sxg_state();
+        /* the worker's own input buffer: the frame is copied in on the worker's stream,
+           ahead of the kernels that read it */
+        float *in = sxg_take<float>("batch.in", N);
+        if (!in) return false;
+        if (hipMemcpyAsync(in, frames[i], N * sizeof(float), hipMemcpyHostToDevice, S.stream) !=
+            hipSuccess) {
+            sxg_set_error("sxg_batch_process_host: frame copy failed");
+            return false;
+        }
+        sxg_frame_info fi;
+        if (process_frame(in, W, H, p, &fi) < 0) return false;
+        int *map = sxg_take<int>("out.map", S.N);
+        if (!map || hipMemcpyAsync(segs[i], map, N * sizeof(int), hipMemcpyDeviceToHost,
+                                   S.stream) != hipSuccess) {
+            sxg_set_error("sxg_batch_process_host: segmentation copy failed");
+            return false;
+        }
+        if (rows && rows[i] && sxg_catalog_async(rows[i], row_cap[i]) < 0) return false;
+        if (hipStreamSynchronize(S.stream) != hipSuccess) {
+            sxg_set_error("sxg_batch_process_host: stream wait failed");
+            return false;
+        }
+        if (info) info[i] = fi;
+        return true;
+    };
+    return run_pool(n, nworkers, run, frame_ms, "sxg_batch_process_host");
 }
 
 extern "C" int sxg_batch_segmap(int frame, int32_t *out)

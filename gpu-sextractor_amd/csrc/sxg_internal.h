@@ -147,6 +147,7 @@ struct SxgState {
     int ext_minarea = 5;
     int64_t npix_above = 0, npix_raw = 0;
     int64_t deb_pixels = 0, deb_labels = 0;   /* level-by-level deblending work */
+    int64_t deb_class_pixels[4] = {};         /* pixels per deblending size class */
     int64_t phot_pixels = 0;                  /* aperture pixels of the photometry */
     int nraw = 0;
     bool filtered = false;  /* cudaFilter ran on this frame */
@@ -214,7 +215,12 @@ long sxg_option(int id);
 bool sxg_precision_columns(int mode);   /* precision.hip */
 void sxg_kt_begin(int id);
 void sxg_kt_end(int id);
+void sxg_kt_begin_on(int id, hipStream_t st);   /* the same, on a given stream */
+void sxg_kt_end_on(int id, hipStream_t st);
 bool sxg_build_segmap();   /* device segmentation map ("out.map") and survivor count */
+/* queue the catalogue rows of the survivors and their copy to host memory (after the
+   segmentation map); the number of rows copied, or -1 */
+int sxg_catalog_async(sxg_isorow *out, int cap);
 
 SxgState &sxg_state();
 /* End of an entry point without a host output: the contexts of the batch workers

@@ -216,7 +216,8 @@ class FrameInfo(C.Structure):
                 ("thresh", C.c_float), ("npix_above", C.c_int64), ("npix_raw", C.c_int64),
                 ("nraw", C.c_int32), ("nobj", C.c_int32), ("nsurv", C.c_int32),
                 ("nbx", C.c_int32), ("nby", C.c_int32), ("deb_pixels", C.c_int64),
-                ("deb_labels", C.c_int64), ("phot_pixels", C.c_int64)]
+                ("deb_labels", C.c_int64), ("phot_pixels", C.c_int64),
+                ("deb_class_pixels", C.c_int64 * 4)]
 
 
 class Brain(C.Structure):
@@ -360,6 +361,8 @@ def gpu():
         L.sxg_back_estimate_noalloc.argtypes = [vp, vp, i, i, i, f, i, i, i, f, C.POINTER(f),
                                                 C.POINTER(f), vp]
         L.sxg_back_estimate_noalloc.restype = i
+        L.sxg_get_catalog.argtypes = [vp, i]
+        L.sxg_batch_process_host.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp]
         _gpu = L
     return _gpu
 
@@ -381,11 +384,12 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_ctx_destroy", "sxg_ctx_bind", "sxg_batch_set_frames", "sxg_batch_process",
             "sxg_batch_segmap", "sxg_batch_release", "sxg_set_weight",
             "sxg_set_photometry", "sxg_get_photometry", "sxg_set_weight_map",
-            "sxg_get_back_rms", "sxg_back_estimate_noalloc"]
+            "sxg_get_back_rms", "sxg_back_estimate_noalloc", "sxg_get_catalog",
+            "sxg_batch_process_host"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort",
-           "phot"]
+           "phot", "deb_tree0", "deb_tree1", "deb_tree2", "deb_tree_glb", "deb_lvpar"]
 
 # options (sxg_option_id): the size caps choose kernel variants and never change a result;
 # "precision" chooses the arithmetic of the graded catalogue columns
@@ -634,6 +638,47 @@ def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"
     return out
 
 
+def run_drop_in(img, p, seg, sub, master, rows):
+    """One frame through the reference's thirteen calls in scan.c order, from host memory
+    (init_device ... run_clean, clear_detection, clear_device), plus the segmentation map
+    and the catalogue rows: the end-to-end sequence a front-end linked against the library
+    runs.  img: the float32 frame; seg (int32, frame shape), sub (float32, frame shape),
+    master (uint32) and rows (ISOROW_DTYPE) are preallocated outputs (pageable or pinned).
+    Returns (objects, survivors)."""
+    L = gpu()
+    H, W = img.shape
+    L.init_device(W, H, img.ctypes.data)
+    bw = min(p.back_size, W)
+    nbx, nby = (W - 1) // bw + 1, (H - 1) // bw + 1
+    L.sxg_set_back_value(p.back_value)
+    bm, bsig = C.c_float(), C.c_float()
+    L.back_estimate(bw, p.back_fthresh, C.byref(bm), C.byref(bsig),
+                    p.back_fsize if nbx > 1 else 1, p.back_fsize if nby > 1 else 1, p.back_type)
+    dthresh, thresh = thresholds(p, bsig.value)
+    L.init_detection()
+    if p.mask is not None:
+        L.cudaFilter(p.mask.ctypes.data, p.mask.shape[0])
+    nraw, npix = C.c_size_t(), C.c_size_t()
+    L.run_detection(dthresh, thresh, p.ext_minarea, C.byref(nraw), C.byref(npix))
+    L.init_deblend(dthresh, p.deblend_nthresh, nraw, npix)
+    nobj = L.parcel_out(dthresh, p.deblend_nthresh, min(p.ext_minarea, 3), nraw, npix,
+                        p.deblend_mincont)
+    L.clear_deblend(p.deblend_nthresh)
+    if nobj < 0 or nobj > master.size:
+        check()
+        raise RuntimeError("parcel_out: %d objects, master buffer of %d" % (nobj, master.size))
+    L.analyse_gpu(FLAG_TRUE_DOUBLE, 1, 1, FLAG_TRUE_FLOAT, bsig.value, p.gain, 0.0, thresh,
+                  dthresh, p.satur_level, 0, 0, 0, p.ext_minarea, p.clean, 0, nbx, nby, bw, bw, W,
+                  nobj)
+    L.run_clean(master.ctypes.data, p.clean_param, p.clean_stacksize, nobj)
+    nsurv = L.sxg_get_segmap(seg.ctypes.data)
+    L.sxg_get_catalog(rows.ctypes.data, len(rows))
+    L.clear_detection()
+    L.clear_device(sub.ctypes.data)
+    check()
+    return nobj, nsurv
+
+
 def stage_times():
     ms = (C.c_float * 6)()
     gpu().sxg_stage_times(ms, 6)
@@ -656,6 +701,95 @@ class OutObj(C.Structure):
                 ("elong", C.c_float), ("ellip", C.c_float), ("sprob", C.c_float),
                 ("peak", C.c_float), ("kronfactor", C.c_float), ("bkg", C.c_float),
                 ("fwhm", C.c_float), ("_pad", C.c_byte * 4)]
+
+
+class IsoRow(C.Structure):
+    """Mirror of sxg_isorow (include/sexgpu.h): one catalogue row per survivor."""
+    _fields_ = [("number", C.c_int32), ("flags", C.c_int32), ("isoarea", C.c_int32),
+                ("fdnpix", C.c_int32), ("x", C.c_double), ("y", C.c_double),
+                ("x2", C.c_double), ("y2", C.c_double), ("xy", C.c_double),
+                ("a", C.c_float), ("b", C.c_float), ("theta", C.c_float),
+                ("cxx", C.c_float), ("cyy", C.c_float), ("cxy", C.c_float),
+                ("flux_iso", C.c_float), ("fluxerr_iso", C.c_float), ("peak", C.c_float),
+                ("bkg", C.c_float)]
+
+
+ISOROW_DTYPE = np.dtype([(name, np.dtype(t)) for name, t in
+                         [("number", "<i4"), ("flags", "<i4"), ("isoarea", "<i4"),
+                          ("fdnpix", "<i4"), ("x", "<f8"), ("y", "<f8"), ("x2", "<f8"),
+                          ("y2", "<f8"), ("xy", "<f8"), ("a", "<f4"), ("b", "<f4"),
+                          ("theta", "<f4"), ("cxx", "<f4"), ("cyy", "<f4"), ("cxy", "<f4"),
+                          ("flux_iso", "<f4"), ("fluxerr_iso", "<f4"), ("peak", "<f4"),
+                          ("bkg", "<f4")]])
+
+
+def get_catalog(cap):
+    """The catalogue rows of the last frame (after run_clean) as a numpy record array."""
+    out = np.zeros(max(cap, 1), ISOROW_DTYPE)
+    n = gpu().sxg_get_catalog(out.ctypes.data, cap)
+    if n < 0:
+        check()
+    return out[:min(n, cap)]
+
+
+_hip = None
+
+
+def _hiprt():
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        _hip.hipHostFree.argtypes = [C.c_void_p]
+    return _hip
+
+
+class PinnedArray:
+    """A numpy array over page-locked host memory (hipHostMalloc), freed with the object:
+    copies between it and the device run asynchronously."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        ptr = C.c_void_p()
+        if _hiprt().hipHostMalloc(C.byref(ptr), max(n, 1), 0) != 0:
+            raise MemoryError("hipHostMalloc of %d bytes failed" % n)
+        self.ptr = ptr.value
+        buf = (C.c_byte * max(n, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _hiprt().hipHostFree(self.ptr)
+            self.ptr = None
+
+
+def process_batch_host(frames, p, nworkers=4, segs=None, rows=None, row_cap=None):
+    """The batched path end to end from host memory (sxg_batch_process_host): frames are
+    float32 arrays (pinned ones overlap the copies with other workers' kernels); segs and
+    rows, when given, are the output arrays (int32 of the frame's shape, ISOROW_DTYPE of
+    row_cap[i] rows).  Returns (infos, frame_ms, segs, rows)."""
+    L = gpu()
+    n = len(frames)
+    if segs is None:
+        segs = [np.zeros(f.shape, np.int32) for f in frames]
+    if rows is None:
+        row_cap = [max(1, f.size // 64) for f in frames]
+        rows = [np.zeros(c, ISOROW_DTYPE) for c in row_cap]
+    ptrs = (C.c_void_p * n)(*[f.ctypes.data for f in frames])
+    ws = (C.c_int * n)(*[f.shape[1] for f in frames])
+    hs = (C.c_int * n)(*[f.shape[0] for f in frames])
+    sp = (C.c_void_p * n)(*[s.ctypes.data for s in segs])
+    rp = (C.c_void_p * n)(*[r.ctypes.data for r in rows])
+    caps = (C.c_int * n)(*[len(r) for r in rows])
+    params = native_params(p)
+    infos = (FrameInfo * n)()
+    ms = (C.c_double * n)()
+    if L.sxg_batch_process_host(C.byref(params), nworkers, n, ptrs, ws, hs, sp, rp, caps, infos,
+                                ms) < 0:
+        check()
+    rows = [r[:min(fi.nsurv, len(r))] for r, fi in zip(rows, infos)]
+    return list(infos), list(ms), segs, rows
 
 
 class SxgParams(C.Structure):

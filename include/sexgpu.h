@@ -154,6 +154,24 @@ int sxg_get_photometry(int column, float *out, size_t count);
    run_clean (before clear_detection). Returns the number of survivors or < 0. */
 int sxg_get_segmap(int32_t *out);
 
+/* Catalogue row of the isophotal analysis, one per survivor of CLEAN in NUMBER order:
+   the columns of the reference's analysis (cudaanalyse.cu:1906-2581, :2715-2716). */
+typedef struct {
+    int32_t number;             /* NUMBER, 1-based: the object's segmentation value */
+    int32_t flags;              /* FLAGS: MERGED 2 | SATUR 4 | TRUNC 8 */
+    int32_t isoarea;            /* ISOAREA_IMAGE: pixels above the analysis threshold */
+    int32_t fdnpix;             /* pixels of the object */
+    double x, y;                /* X_IMAGE, Y_IMAGE (1-based) */
+    double x2, y2, xy;          /* X2_IMAGE, Y2_IMAGE, XY_IMAGE */
+    float a, b, theta;          /* A_IMAGE, B_IMAGE, THETA_IMAGE (degrees) */
+    float cxx, cyy, cxy;        /* CXX_IMAGE, CYY_IMAGE, CXY_IMAGE */
+    float flux_iso, fluxerr_iso;   /* FLUX_ISO, FLUXERR_ISO */
+    float peak, bkg;            /* FLUX_MAX, BACKGROUND */
+} sxg_isorow;
+/* The rows of the last frame into out[0..min(survivors, cap)), with one device-to-host
+   copy: after run_clean, until the next frame.  Returns the number of survivors or < 0. */
+int sxg_get_catalog(sxg_isorow *out, int cap);
+
 /* Per-object columns (object order, numobj entries).  Returns 0 or < 0 on error. */
 enum sxg_column {
     SXG_COL_LEVEL = 0,     /* int32  deblending level of the object */
@@ -207,6 +225,9 @@ typedef struct {
     /* photometry work of the last endobject_gpu call: the aperture pixels summed (inside
        the APER outer radius, inside both AUTO ellipses) */
     int64_t phot_pixels;
+    /* pixels of the multi-peak raw objects in each deblending size class (0: 256-lane LDS
+       trees, 1 and 2: 1024-lane LDS trees, 3: global-memory trees) */
+    int64_t deb_class_pixels[4];
 } sxg_frame_info;
 int sxg_get_frame_info(sxg_frame_info *info);
 
@@ -290,6 +311,17 @@ int sxg_batch_set_frames(int n, const float *const *host, const int *width, cons
    frame stays on the device.  Returns the number of frames or < 0. */
 int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_info *info,
                       double *frame_ms);
+/* The batched path end to end, from host memory: frame i (width[i] x height[i] floats at
+   frames[i]) is copied in by the worker that takes it, on that worker's stream, the whole
+   path runs, and the segmentation map is copied to segs[i] (width*height int32) and the
+   catalogue rows to rows[i] (up to row_cap[i] rows; rows or rows[i] NULL: none).  With
+   pinned host buffers (hipHostMalloc) the copies of one worker overlap the kernels of the
+   others.  info[i] and frame_ms[i] (host wall time of frame i from its copy in to its
+   last copy out) may be NULL.  Returns the number of frames or < 0. */
+int sxg_batch_process_host(const sxg_params *p, int nworkers, int n, const float *const *frames,
+                           const int *width, const int *height, int32_t *const *segs,
+                           sxg_isorow *const *rows, const int *row_cap, sxg_frame_info *info,
+                           double *frame_ms);
 /* Segmentation map of frame i of the last sxg_batch_process. */
 int sxg_batch_segmap(int frame, int32_t *out);
 /* Free the batch and its worker contexts. */
@@ -302,7 +334,12 @@ enum sxg_kernel_id {
     SXG_KT_SUBBACK, SXG_KT_CONV, SXG_KT_DET_INIT, SXG_KT_DET_FG, SXG_KT_DET_UNION,
     SXG_KT_DET_FLATTEN, SXG_KT_DET_STAT, SXG_KT_DET_SORT, SXG_KT_DEB_THRESH,
     SXG_KT_DEB_LEVELS, SXG_KT_DEB_OBJECTS, SXG_KT_ANALYSE, SXG_KT_CLEAN, SXG_KT_SEG,
-    SXG_KT_DEB_LMAX, SXG_KT_OBJ_SORT, SXG_KT_PHOT, SXG_NKT
+    SXG_KT_DEB_LMAX, SXG_KT_OBJ_SORT, SXG_KT_PHOT,
+    /* the deblending tree kernels one by one, each timed on the stream it runs on: size
+       classes 0, 1, 2 (k_deb_tree_lds), the one-workgroup global-memory trees
+       (k_deb_tree_glb) and the level-parallel chain (k_deb_glb_prep/levels/tree) */
+    SXG_KT_DEB_TREE0, SXG_KT_DEB_TREE1, SXG_KT_DEB_TREE2, SXG_KT_DEB_TREE_GLB, SXG_KT_DEB_LVPAR,
+    SXG_NKT
 };
 void sxg_kernel_timing(int enable);
 int sxg_kernel_times(int id, double *total_ms, int *launches);

@@ -51,6 +51,9 @@ typedef struct {
     int weight_gain;         /* WEIGHT_GAIN */
 } sxo_config;
 
+/* stages timed by sxo_run: background, filtering, detection, deblending (with gatherup
+   and the final pixel lists), analysis, CLEAN, segmentation map */
+#define SXO_NSTAGE 7
 typedef struct {
     int width, height, nbx, nby;
     float backmean, backsig, dthresh, thresh;
@@ -76,6 +79,7 @@ typedef struct {
     int *objpix_start, *objpix;             /* final pixel lists (raster order) */
     int *levpix_start, *levpix;             /* each object's level pixels (its kept
                                                component before gatherup), raster order */
+    double stage_ms[SXO_NSTAGE];            /* wall time of each stage of sxo_run */
 } sxo_result;
 
 /* ---------------- background (src/back.c) ---------------- */

@@ -120,6 +120,7 @@ class SxoResult(C.Structure):
         ("mthresh", _FP), ("fwhm", _FP), ("dthresh_obj", _FP),
         ("master", C.POINTER(C.c_uint32)), ("alloc", _IP), ("seg", _IP), ("nsurv", C.c_int),
         ("objpix_start", _IP), ("objpix", _IP), ("levpix_start", _IP), ("levpix", _IP),
+        ("stage_ms", C.c_double * 7),
     ]
 
 
@@ -266,6 +267,10 @@ def config(**kw):
 TRUTH_COLUMNS = ["flux", "mx", "my", "mx2", "my2", "mxy", "a", "b", "theta", "cxx", "cyy", "cxy"]
 
 
+# the stages sxo_run times (wall clock, milliseconds)
+STAGES = ("background", "filter", "detection", "deblending", "analysis", "clean", "segmap")
+
+
 def run(img, mask=None, weight=None, truth=False, photom=None, **kw):
     """Run the whole oracle path on a float32 frame (and an optional MAP_WEIGHT map);
     returns a dict of numpy arrays.  truth: also the long-double evaluation of the graded
@@ -315,6 +320,7 @@ def run(img, mask=None, weight=None, truth=False, photom=None, **kw):
         out[k] = _arr(getattr(r, k), m, np.float32)
     out["iso"] = _arr(r.iso, 8 * m, np.int32).reshape(m, 8)
     out["master"] = _arr(r.master, m, np.uint32)
+    out["stage_ms"] = dict(zip(STAGES, r.stage_ms))
     if truth:
         t = np.zeros((max(m, 1), 12), np.float64)
         if lib().sxo_measure_fp64(rp, t.ctypes.data, 12) != m:

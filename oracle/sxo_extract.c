@@ -36,6 +36,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "sxo.h"
 #include "../include/sx_cmath.h"
@@ -243,10 +244,25 @@ static void top_insert(float *top, int h, int *nfill, float v)
 }
 
 /* -------------------------------------------------------------- the run ---- */
+/* monotonic wall clock in milliseconds (the CPU baseline's per-stage times) */
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
 sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
 {
     sxo_result *r = (sxo_result *)calloc(1, sizeof(sxo_result));
     if (!r) return NULL;
+    double t_mark = now_ms();
+#define STAGE_END(k)                                                                       \
+    do {                                                                                   \
+        const double t_now = now_ms();                                                     \
+        r->stage_ms[k] = t_now - t_mark;                                                   \
+        t_mark = t_now;                                                                    \
+    } while (0)
     const size_t N = (size_t)W * H;
     const int bw = cfg->back_size < W ? cfg->back_size : W;
     const int bh = cfg->back_size < H ? cfg->back_size : H;
@@ -338,6 +354,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
 #define DTH(p) (wrel ? (float)(cfg->detect_thresh * sqrt((double)r->var[p])) : dthresh)
 #define ATH(p) (wrel ? (float)(cfg->analysis_thresh * sqrt((double)r->var[p])) : thresh)
 
+    STAGE_END(0);
     /* ---- filtering (filter.c:39-94) ---- */
     if (cfg->mask)
         sxo_convolve(r->sub, W, H, cfg->mask, cfg->mask_w, cfg->mask_h, r->cd);
@@ -345,6 +362,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         memcpy(r->cd, r->sub, sizeof(float) * N);
     const float *cd = r->cd, *sub = r->sub;
 
+    STAGE_END(1);
     /* ---- detection: 8-connected labels, root = smallest raster index ---- */
     int *par = (int *)malloc(sizeof(int) * N);
     long nabove = 0;
@@ -434,6 +452,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         rawt0[k] = t0;
     }
 
+    STAGE_END(2);
     /* ---- deblending, preanalyse_full and gatherup, one root at a time ---- */
     const int nthr = cfg->deblend_nthresh;
     const int debarea = cfg->ext_minarea < MAXDEBAREA ? cfg->ext_minarea : MAXDEBAREA;
@@ -668,6 +687,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
     }
     free(islev);
 
+    STAGE_END(3);
     /* ---- object arrays ---- */
 #define ALLOC_I(f) r->f = (int *)calloc(nobj + 1, sizeof(int))
 #define ALLOC_F(f) r->f = (float *)calloc(nobj + 1, sizeof(float))
@@ -831,6 +851,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
     }
     free(top);
 
+    STAGE_END(4);
     /* ---- CLEAN (cudaclean.cu:10-29, :120-174) ---- */
     for (int o = 0; o < nobj; o++) r->master[o] = 0xffffffffu;
     if (cfg->clean && nobj > 1) {
@@ -867,6 +888,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         free(alpha);
     }
 
+    STAGE_END(5);
     /* ---- segmentation map: survivors numbered 1.. in object order ---- */
     {
         int *num = (int *)malloc(sizeof(int) * (nobj + 1));
@@ -882,6 +904,8 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
 
     free(pre); free(o_level); free(o_root); free(o_first); free(o_dthresh); free(rawt0);
     free(par); free(rawid); free(rawoff); free(fpia); free(pixobj);
+    STAGE_END(6);
+#undef STAGE_END
     return r;
 }
 

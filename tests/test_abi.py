@@ -96,3 +96,21 @@ def test_options_without_device():
 def test_outobj_layout():
     # the reference's outobjstruct with NAPER 1: 160 bytes, 16-byte aligned
     assert ctypes.sizeof(sexgpu.OutObj) == 160
+
+
+def test_isorow_layout_matches_header(tmp_path):
+    """The ctypes and numpy mirrors of sxg_isorow have the C layout: size and every
+    field's offset, from a program compiled against include/sexgpu.h."""
+    fields = [name for name, _ in sexgpu.IsoRow._fields_]
+    body = "".join('printf(" %%zu", offsetof(sxg_isorow, %s));' % f for f in fields)
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "sexgpu.h"\n'
+                   'int main(void) { printf("%%zu", sizeof(sxg_isorow)); %s return 0; }\n' % body)
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(sexgpu.IsoRow) == sexgpu.ISOROW_DTYPE.itemsize
+    assert got[1:] == [getattr(sexgpu.IsoRow, f).offset for f in fields]
+    assert got[1:] == [sexgpu.ISOROW_DTYPE.fields[f][1] for f in fields]

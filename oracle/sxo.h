@@ -115,6 +115,9 @@ void sxo_convolve(const float *in, int W, int H, const float *mask, int mw, int 
 /* ---------------- whole path ---------------- */
 void sxo_config_defaults(sxo_config *c);
 sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg);
+/* the bilinear value of a nx x ny mesh map b at pixel (x, y), meshes of backw x backh
+   (back.c:685-732, cudaanalyse.cu:2128-2172): the BACKGROUND of an object's centroid */
+float sxo_back_at(const float *b, int x, int y, int nx, int ny, int backw, int backh);
 void sxo_free(sxo_result *r);
 /* all-double re-measurement of the catalogue columns over the same final pixel lists */
 /* All-extended-precision (long double) evaluation of the graded catalogue quantities on

@@ -38,6 +38,9 @@ def lib():
         _lib.sxo_pow.argtypes = [C.c_double, C.c_double]
         _lib.sxo_hmedian.restype = C.c_float
         _lib.sxo_hmedian.argtypes = [C.c_void_p, C.c_int]
+        _lib.sxo_back_at.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int]
+        _lib.sxo_back_at.restype = C.c_float
         _lib.sxo_backguess.restype = C.c_float
         _lib.sxo_getconv.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]
@@ -321,6 +324,12 @@ def run(img, mask=None, weight=None, truth=False, photom=None, **kw):
     out["iso"] = _arr(r.iso, 8 * m, np.int32).reshape(m, 8)
     out["master"] = _arr(r.master, m, np.uint32)
     out["stage_ms"] = dict(zip(STAGES, r.stage_ms))
+    # each object's final pixels and its level pixels (the kept component before
+    # gatherup), raster order: object o owns [start[o], start[o + 1])
+    out["objpix_start"] = _arr(r.objpix_start, m + 1, np.int32)
+    out["objpix"] = _arr(r.objpix, int(out["objpix_start"][-1]) if m else 0, np.int32)
+    out["levpix_start"] = _arr(r.levpix_start, m + 1, np.int32)
+    out["levpix"] = _arr(r.levpix, int(out["levpix_start"][-1]) if m else 0, np.int32)
     if truth:
         t = np.zeros((max(m, 1), 12), np.float64)
         if lib().sxo_measure_fp64(rp, t.ctypes.data, 12) != m:

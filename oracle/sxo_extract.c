@@ -206,7 +206,7 @@ static void preanalyse(const int *pix, int n, const float *sub, const float *cd,
 }
 
 /* back.c:685-732 / cudaanalyse.cu:2128-2172: bilinear mesh value */
-static float back_at(const float *b, int x, int y, int nx, int ny, int backw, int backh)
+float sxo_back_at(const float *b, int x, int y, int nx, int ny, int backw, int backh)
 {
     int xl, yl, pos;
     double dx, dy, cdx;
@@ -734,7 +734,7 @@ sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg)
         r->dthresh_obj[o] = odth;
 
         /* examineiso (cudaanalyse.cu:2176-2581), CCD mode, GLOBAL background */
-        float bkg = back_at(r->back, (int)(q.mx + 0.5), (int)(q.my + 0.5), nbx, nby, bs, bs);
+        float bkg = sxo_back_at(r->back, (int)(q.mx + 0.5), (int)(q.my + 0.5), nbx, nby, bs, bs);
         float dbkg = 0.0f;
         double tv = 0.0, sigtv = 0.0, esum = 0.0, emx2 = 0.0, emy2 = 0.0, emxy = 0.0, err, var2;
         double xm = q.mx, ym = q.my, x, y;

@@ -28,6 +28,7 @@
  * neuron per lane.
  */
 #include <climits>
+#include <cstring>
 
 #include "sxg_internal.h"
 #include "sxg_objects.h"
@@ -538,14 +539,73 @@ extern "C" int sxg_set_photometry(const sxg_photom_params *p)
     return 0;
 }
 
+/* Byte offsets of the fields endobject_gpu reads from the caller's prefstruct and
+   obj2struct, as the reference's build lays them out on x86-64 (src/prefs.h,
+   src/types.h, NAPER 1): the ones its endobject kernels read (cudaanalyse.cu:145-151,
+   :349-584, :2905, :3363, :3491-3507), and the enum values they are compared with
+   (DETECT_TYPE PHOTO, MASK_TYPE CORRECT).  sxg_reference_layout() returns them, so a test
+   compiled against the reference's headers can check them. */
+enum RefLayout : long {
+    RL_PREFS_NAPER = 2096, RL_PREFS_APERT = 1840, RL_PREFS_AUTOPARAM = 2120,
+    RL_PREFS_AUTOAPER = 2168, RL_PREFS_SEEING = 2232, RL_PREFS_MAGZP = 2192,
+    RL_PREFS_MASKTYPE = 7152, RL_PREFS_DETECTTYPE = 1696, RL_OBJ2_FLUX_APER = 88,
+    RL_OBJ2_FLUX_AUTO = 36, RL_OBJ2_SPROB = 528, RL_DETECT_PHOTO = 1, RL_MASK_CORRECT = 2,
+    RL_PREFS_SIZE = 7696, RL_OBJ2_SIZE = 576
+};
+
+extern "C" int sxg_reference_layout(long *out, int n)
+{
+    static const long v[] = {RL_PREFS_SIZE, RL_PREFS_NAPER, RL_PREFS_APERT, RL_PREFS_AUTOPARAM,
+                             RL_PREFS_AUTOAPER, RL_PREFS_SEEING, RL_PREFS_MAGZP,
+                             RL_PREFS_MASKTYPE, RL_PREFS_DETECTTYPE, RL_OBJ2_SIZE,
+                             RL_OBJ2_FLUX_APER, RL_OBJ2_FLUX_AUTO, RL_OBJ2_SPROB,
+                             RL_DETECT_PHOTO, RL_MASK_CORRECT};
+    const int nv = (int)(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; out && i < n && i < nv; i++) out[i] = v[i];
+    return nv;
+}
+
+/* the photometry settings of the reference's prefs (and the groups its obj2 flags ask for:
+   the first byte of each flag field, define.h:283); false (error recorded) when its NAPER
+   is not this build's (the reference aborts, cudaanalyse.cu:3363-3367) */
+static bool prefs_settings(const void *prefs, const void *flagobj2, sxg_photom_params &pp)
+{
+    if (prefs) {
+        const char *pb = static_cast<const char *>(prefs);
+        int naper = 0, mask = 0, detect = 0;
+        memcpy(&naper, pb + RL_PREFS_NAPER, sizeof(int));
+        if (naper != 1) {
+            sxg_set_error("endobject_gpu: prefs.naper is " + std::to_string(naper) +
+                          ", this library measures one aperture (NAPER 1)");
+            return false;
+        }
+        memcpy(&pp.apert, pb + RL_PREFS_APERT, sizeof(double));
+        memcpy(pp.autoparam, pb + RL_PREFS_AUTOPARAM, 2 * sizeof(double));
+        memcpy(pp.autoaper, pb + RL_PREFS_AUTOAPER, 2 * sizeof(double));
+        memcpy(&pp.seeing_fwhm, pb + RL_PREFS_SEEING, sizeof(double));
+        memcpy(&pp.mag_zeropoint, pb + RL_PREFS_MAGZP, sizeof(double));
+        memcpy(&mask, pb + RL_PREFS_MASKTYPE, sizeof(int));
+        memcpy(&detect, pb + RL_PREFS_DETECTTYPE, sizeof(int));
+        pp.mask_correct = mask == RL_MASK_CORRECT;
+        pp.photo = detect == RL_DETECT_PHOTO;
+    }
+    if (flagobj2) {
+        const char *ob = static_cast<const char *>(flagobj2);
+        pp.do_aper = ob[RL_OBJ2_FLUX_APER] != 0;
+        pp.do_auto = ob[RL_OBJ2_FLUX_AUTO] != 0;
+        pp.do_class = ob[RL_OBJ2_SPROB] != 0;
+    }
+    return true;
+}
+
 /* the photometry of the current frame's objects into the "phot.*" columns and the flags;
    brain may be nullptr (CLASS_STAR 0) */
-static bool run_photometry(const sxg_brain *brain, float backsig, float thresh,
-                           double satur_level, double pixscale, double ngamma, double gain)
+static bool run_photometry(const sxg_photom_params &pp, const sxg_brain *brain, float backsig,
+                           float thresh, double satur_level, double pixscale, double ngamma,
+                           double gain)
 {
     SxgState &S = sxg_state();
     hipStream_t st = S.stream;
-    const sxg_photom_params &pp = S.phot;
     const int n = S.nobj;
     PhotArgs A;
     A.W = S.W;
@@ -599,7 +659,7 @@ static bool run_photometry(const sxg_brain *brain, float backsig, float thresh,
     A.autoaper1 = pp.autoaper[1];
     A.seeing = pp.seeing_fwhm;
     A.pixscale = pixscale;
-    A.zp = S.mag_zeropoint;
+    A.zp = pp.mag_zeropoint;
     A.ngamma = ngamma;
     A.gain = gain;
     A.satur = satur_level;
@@ -619,7 +679,7 @@ extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *ob
                               float thresh, double satur_level, double pixscale, double ngamma,
                               double gain, int numobj)
 {
-    (void)prefs; (void)flagobj2; (void)wcs;
+    (void)wcs;
     SxgState &S = sxg_state();
     sxg_clear_error();
     if (!S.have_frame || !S.cleaned || numobj != S.nobj || width != S.W || height != S.H) {
@@ -637,7 +697,12 @@ extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *ob
         S.phot_ready = true;
         return;
     }
-    if (!run_photometry(static_cast<const sxg_brain *>(brain), backsig, thresh, satur_level,
+    /* the reference front-end's prefs and obj2 flags when given, else the settings of
+       sxg_set_photometry (MAG_ZEROPOINT: sxg_set_mag_zeropoint) */
+    sxg_photom_params pp = S.phot;
+    pp.mag_zeropoint = S.mag_zeropoint;
+    if (!prefs_settings(prefs, flagobj2, pp)) return;
+    if (!run_photometry(pp, static_cast<const sxg_brain *>(brain), backsig, thresh, satur_level,
                         pixscale, ngamma, gain))
         return;
     unsigned long long visited = 0;

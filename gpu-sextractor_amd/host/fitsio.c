@@ -1,8 +1,9 @@
 /*
  * fitsio.c -- minimal FITS image I/O (primary HDU, 2880-byte blocks, big-endian data).
  * Reading follows the conversions of the reference's FITS library
- * (src/fits/fitsbody.c:195-466): integer data are scaled by BSCALE/BZERO, floating-point
- * NaN/Inf pixels are replaced by -BIG so that they never pass a threshold.
+ * (src/fits/fitsbody.c:195-345, src/fits/fitshead.c:148-160): integer data are scaled by
+ * BSCALE/BZERO in float, BLANK and NaN/Inf pixels are replaced by -BIG so that they never
+ * pass a threshold.
  */
 #include <math.h>
 #include <stdint.h>
@@ -27,13 +28,19 @@ static int card_value(const char *card, const char *key, double *val)
     return 1;
 }
 
+/* The conversions of read_body (fitsbody.c:195-345): BSCALE and BZERO are rounded to
+   float (PIXTYPE) and integer pixels are scaled in float arithmetic (BITPIX 8 bytes are
+   signed unless BITSGN = 0, fitshead.c:159-160); pixels equal to BLANK, and IEEE NaN or
+   Inf pixels, become -BIG; BITPIX -64 pixels are scaled in double, then rounded. */
 float *sxf_read_image(const char *path, int *width, int *height, double *gain, double *satur,
                       const char **err)
 {
     FILE *f = fopen(path, "rb");
     if (!f) { if (err) *err = "cannot open file"; return NULL; }
     char card[80];
-    double bitpix = 0, naxis = 0, n1 = 0, n2 = 0, bscale = 1.0, bzero = 0.0, v;
+    double bitpix = 0, naxis = 0, n1 = 0, n2 = 0, bscale = 1.0, bzero = 0.0, blank = 0.0,
+           bitsgn = 1.0, v;
+    int blankflag = 0;
     long ncards = 0;
     int end = 0;
     while (!end && fread(card, 1, 80, f) == 80) {
@@ -45,6 +52,8 @@ float *sxf_read_image(const char *path, int *width, int *height, double *gain, d
         else if (card_value(card, "NAXIS", &v)) naxis = v;
         else if (card_value(card, "BSCALE", &v)) bscale = v;
         else if (card_value(card, "BZERO", &v)) bzero = v;
+        else if (card_value(card, "BLANK", &v)) { blank = v; blankflag = 1; }
+        else if (card_value(card, "BITSGN", &v)) bitsgn = v;
         else if (gain && card_value(card, "GAIN", &v)) *gain = v;
         else if (satur && card_value(card, "SATURATE", &v)) *satur = v;
     }
@@ -52,6 +61,11 @@ float *sxf_read_image(const char *path, int *width, int *height, double *gain, d
     long hdrbytes = ((ncards * 80 + FBLOCK - 1) / FBLOCK) * FBLOCK;
     fseek(f, hdrbytes, SEEK_SET);
     int W = (int)n1, H = (int)n2, bp = (int)bitpix, nb = abs(bp) / 8;
+    if (bp != 8 && bp != 16 && bp != 32 && bp != -32 && bp != -64) {
+        fclose(f);
+        if (err) *err = "unsupported BITPIX";
+        return NULL;
+    }
     size_t n = (size_t)W * H;
     unsigned char *raw = (unsigned char *)malloc(n * nb);
     float *img = (float *)malloc(n * sizeof(float));
@@ -61,33 +75,54 @@ float *sxf_read_image(const char *path, int *width, int *height, double *gain, d
         return NULL;
     }
     fclose(f);
+    const float bs = (float)bscale, bz = (float)bzero, mbig = (float)-1e30;
+    const long lblank = (long)blank;
     for (size_t i = 0; i < n; i++) {
-        unsigned char *b = raw + i * nb;
-        double x = 0.0;
+        const unsigned char *b = raw + i * nb;
         switch (bp) {
-        case 8: x = b[0] * bscale + bzero; break;
-        case 16: x = (int16_t)((b[0] << 8) | b[1]) * bscale + bzero; break;
-        case 32: x = (int32_t)(((uint32_t)b[0] << 24) | ((uint32_t)b[1] << 16) | ((uint32_t)b[2] << 8) | b[3]) * bscale + bzero; break;
+        case 8: {
+            const int cv = bitsgn != 0.0 ? (int)(signed char)b[0] : (int)b[0];
+            const int bl = bitsgn != 0.0 ? (int)(signed char)lblank : (int)(unsigned char)lblank;
+            img[i] = blankflag && cv == bl ? mbig : (float)cv * bs + bz;
+            break;
+        }
+        case 16: {
+            const uint16_t u = (uint16_t)((b[0] << 8) | b[1]);
+            const int sv = bitsgn != 0.0 ? (int)(int16_t)u : (int)u;
+            const int bl = bitsgn != 0.0 ? (int)(int16_t)lblank : (int)(uint16_t)lblank;
+            img[i] = blankflag && sv == bl ? mbig : (float)sv * bs + bz;
+            break;
+        }
+        case 32: {
+            const uint32_t u = ((uint32_t)b[0] << 24) | ((uint32_t)b[1] << 16) |
+                               ((uint32_t)b[2] << 8) | b[3];
+            if (bitsgn != 0.0) {
+                const int32_t iv = (int32_t)u;
+                img[i] = blankflag && iv == (int32_t)lblank ? mbig : (float)iv * bs + bz;
+            } else {
+                img[i] = blankflag && u == (uint32_t)lblank ? mbig : (float)u * bs + bz;
+            }
+            break;
+        }
         case -32: {
-            uint32_t u = ((uint32_t)b[0] << 24) | ((uint32_t)b[1] << 16) | ((uint32_t)b[2] << 8) | b[3];
+            const uint32_t u = ((uint32_t)b[0] << 24) | ((uint32_t)b[1] << 16) |
+                               ((uint32_t)b[2] << 8) | b[3];
             float fv;
             memcpy(&fv, &u, 4);
-            if ((u & 0x7f800000u) == 0x7f800000u) { img[i] = -1e30f; continue; }
-            img[i] = (bscale == 1.0 && bzero == 0.0) ? fv : (float)(fv * bscale + bzero);
-            continue;
+            img[i] = (u & 0x7f800000u) == 0x7f800000u ? mbig : fv * bs + bz;
+            break;
         }
-        case -64: {
+        default: {   /* -64 */
             uint64_t u = 0;
             for (int k = 0; k < 8; k++) u = (u << 8) | b[k];
             double dv;
             memcpy(&dv, &u, 8);
-            if (!isfinite(dv)) { img[i] = -1e30f; continue; }
-            x = dv * bscale + bzero;
+            img[i] = (u & 0x7ff0000000000000ull) == 0x7ff0000000000000ull
+                         ? mbig
+                         : (float)(dv * (double)bs + (double)bz);
             break;
         }
-        default: free(raw); free(img); if (err) *err = "unsupported BITPIX"; return NULL;
         }
-        img[i] = (float)x;
     }
     free(raw);
     *width = W;

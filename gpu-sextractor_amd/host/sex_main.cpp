@@ -49,7 +49,7 @@ struct Config {
               {"VERBOSE_TYPE", "NORMAL"}, {"PHOT_APERTURES", "5"},
               {"PHOT_AUTOPARAMS", "2.5, 3.5"}, {"PHOT_AUTOAPERS", "0.0,0.0"},
               {"MASK_TYPE", "CORRECT"}, {"SEEING_FWHM", "1.2"}, {"PIXEL_SCALE", "1.0"},
-              {"STARNNW_NAME", "default.nnw"}};
+              {"STARNNW_NAME", "default.nnw"}, {"CATALOG_ROWS", "PARAM"}};
     }
     std::string get(const std::string &k) const
     {
@@ -366,6 +366,21 @@ bool fail(const char *what)
 
 }  // namespace
 
+/* One catalogue row as the reference writes it (printobj, scan.c:331-406): the fields of
+   its outobjstruct in a fixed format, whatever the .param list holds. */
+static void print_reference_row(FILE *s, const sxg_outobj &o)
+{
+    fprintf(s,
+            "%10d %11.7f %+11.7f %10.3f %10.3f %12.7g %12.7g %8.4f %8.4f %12.7g %12.7g %12.7g "
+            "%12.7g %8.4f %8.4f %12.7g %12.7g %8.4f %8.4f %5.1f %15.10e %15.10e %15.10e %3d "
+            "%8.3f %8.3f %5.2f %12.7g %5.2f %12.7g %8.2f\n",
+            o.number, o.alpha2000, o.delta2000, o.sposx, o.sposy, o.flux_iso, o.fluxerr_iso,
+            o.mag_iso, o.magerr_iso, o.flux_isocor, o.fluxerr_isocor, o.flux_aper[0],
+            o.fluxerr_aper[0], o.mag_aper[0], o.magerr_aper[0], o.flux_auto, o.fluxerr_auto,
+            o.mag_auto, o.magerr_auto, o.theta, o.poserr_mx2, o.poserr_my2, o.poserr_mxy,
+            (int)o.flag, o.elong, o.ellip, o.sprob, o.peak, o.kronfactor, o.bkg, o.fwhm);
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) {
@@ -558,9 +573,18 @@ int main(int argc, char **argv)
     pp.mag_zeropoint = cfg.num("MAG_ZEROPOINT");
     pp.mask_correct = cfg.first("MASK_TYPE") == "CORRECT";
     pp.photo = cfg.first("DETECT_TYPE") == "PHOTO";
-    pp.do_aper = wants({"FLUX_APER", "FLUXERR_APER", "MAG_APER", "MAGERR_APER"});
-    pp.do_auto = wants({"FLUX_AUTO", "FLUXERR_AUTO", "MAG_AUTO", "MAGERR_AUTO", "KRON_RADIUS"});
-    pp.do_class = wants({"CLASS_STAR"});
+    /* CATALOG_ROWS REFERENCE: the rows of the reference's printobj, which hold every
+       photometry group */
+    const std::string rowfmt = cfg.first("CATALOG_ROWS");
+    if (rowfmt != "PARAM" && rowfmt != "REFERENCE") {
+        fprintf(stderr, "sex: CATALOG_ROWS must be PARAM or REFERENCE\n");
+        return 1;
+    }
+    const bool refrows = rowfmt == "REFERENCE";
+    pp.do_aper = refrows || wants({"FLUX_APER", "FLUXERR_APER", "MAG_APER", "MAGERR_APER"});
+    pp.do_auto = refrows ||
+                 wants({"FLUX_AUTO", "FLUXERR_AUTO", "MAG_AUTO", "MAGERR_AUTO", "KRON_RADIUS"});
+    pp.do_class = refrows || wants({"CLASS_STAR"});
     if (sxg_set_photometry(&pp) != 0) {
         fail("sxg_set_photometry");
         return 1;
@@ -714,8 +738,18 @@ int main(int argc, char **argv)
         rowbytes += t == 'D' ? 8 : t == 'I' ? 2 : 4;
     }
     const bool ldac = ctype_cat == "FITS_LDAC";
+    if (refrows && ldac) {
+        fprintf(stderr, "sex: CATALOG_ROWS REFERENCE writes ASCII rows only\n");
+        return 1;
+    }
     for (int i = 0; i < nobj; i++) {
         if (master[i] != 0xffffffffu) continue;   /* scan.c:148-157 */
+        if (refrows) {
+            /* scan.c:150-156: the survivors numbered from 0, one printobj row each */
+            rows[i].number = number++;
+            print_reference_row(cat, rows[i]);
+            continue;
+        }
         number++;
         for (size_t k = 0; k < cols.size(); k++) {
             const Cell v = cell(cols[k]->name, i, number);

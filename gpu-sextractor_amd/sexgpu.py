@@ -362,6 +362,7 @@ def gpu():
                                                 C.POINTER(f), vp]
         L.sxg_back_estimate_noalloc.restype = i
         L.sxg_get_catalog.argtypes = [vp, i]
+        L.sxg_reference_layout.argtypes = [vp, i]
         L.sxg_batch_process_host.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp]
         _gpu = L
     return _gpu
@@ -385,7 +386,7 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_batch_segmap", "sxg_batch_release", "sxg_set_weight",
             "sxg_set_photometry", "sxg_get_photometry", "sxg_set_weight_map",
             "sxg_get_back_rms", "sxg_back_estimate_noalloc", "sxg_get_catalog",
-            "sxg_batch_process_host"]
+            "sxg_batch_process_host", "sxg_reference_layout"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort",
@@ -562,11 +563,13 @@ def back_rms(shape):
 
 
 def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"), weight=None,
-              brain=None, pixscale=1.0, ngamma=0.0, weight_type=0, weight_gain=0):
+              brain=None, pixscale=1.0, ngamma=0.0, weight_type=0, weight_gain=0, prefs=None,
+              flagobj2=None):
     """Run the whole path in the order of the reference's scan.c:78-146 and collect the
     requested outputs (numpy arrays).  weight: optional MAP_WEIGHT map; brain: the
     CLASS_STAR network (read_nnw) for the catalogue; "phot" in outputs: the photometry
-    columns of endobject_gpu."""
+    columns of endobject_gpu; prefs, flagobj2: bytearrays laid out as the reference's
+    prefstruct and obj2struct (reference_layout()), passed to endobject_gpu."""
     L = gpu()
     img = np.ascontiguousarray(img, dtype=np.float32)
     H, W = img.shape
@@ -616,9 +619,11 @@ def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"
         out["alloc"] = al
     if "catalog" in outputs or "phot" in outputs:
         rows = (OutObj * max(nobj, 1))()
-        L.endobject_gpu(None, C.byref(brain) if brain is not None else None, None,
-                        C.cast(rows, C.c_void_p), None, 0, H, W, H, bsig, thresh,
-                        p.satur_level, pixscale, ngamma, p.gain, nobj)
+        def buf(b):
+            return None if b is None else C.cast((C.c_char * len(b)).from_buffer(b), C.c_void_p)
+        L.endobject_gpu(buf(prefs), C.byref(brain) if brain is not None else None,
+                        buf(flagobj2), C.cast(rows, C.c_void_p), None, 0, H, W, H, bsig,
+                        thresh, p.satur_level, pixscale, ngamma, p.gain, nobj)
         check()
         out["catalog"] = rows
         if "phot" in outputs:
@@ -677,6 +682,18 @@ def run_drop_in(img, p, seg, sub, master, rows):
     L.clear_device(sub.ctypes.data)
     check()
     return nobj, nsurv
+
+
+REFERENCE_LAYOUT = ["prefs_size", "naper", "apert", "autoparam", "autoaper", "seeing_fwhm",
+                    "mag_zeropoint", "mask_type", "detect_type", "obj2_size", "flux_aper",
+                    "flux_auto", "sprob", "PHOTO", "MASK_CORRECT"]
+
+
+def reference_layout():
+    """The offsets endobject_gpu assumes in the reference's prefstruct and obj2struct."""
+    out = (C.c_long * 32)()
+    n = gpu().sxg_reference_layout(out, 32)
+    return dict(zip(REFERENCE_LAYOUT, list(out[:n])))
 
 
 def stage_times():

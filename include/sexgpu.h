@@ -79,10 +79,15 @@ void run_clean(int *masterIndex, double clean_param, int clean_stacksize, int nu
 /* src/cuda/cudaanalyse.cu:3344 -- catalogue photometry (ISOCOR, APER, AUTO, CLASS_STAR,
    magnitudes, ELONGATION/ELLIPTICITY, the APERT_PB/CROWDED/SATUR flags) and one row per
    object (outobjstruct layout of src/types.h:417-455, mirrored by sxg_outobj below; objlist
-   may be NULL to keep the results on the device).  brain is read as sxg_brain (NULL: no
-   CLASS_STAR); the photometry settings come from sxg_set_photometry instead of prefs;
-   flagobj2 and wcs are not read (every column is computed; the reference computes no
-   world coordinates).  Whole frames only: fymin 0, fymax height. */
+   may be NULL to keep the results on the device).  prefs is the reference's prefstruct:
+   PHOT_APERTURES, PHOT_AUTOPARAMS, PHOT_AUTOAPERS, SEEING_FWHM, MAG_ZEROPOINT, MASK_TYPE
+   and DETECT_TYPE are read from it at the offsets of the reference's layout
+   (sxg_reference_layout; a naper other than 1 is an error, as the reference aborts);
+   flagobj2 (the reference's obj2struct of flags) switches the APER, AUTO and CLASS_STAR
+   groups by the first byte of its flux_aper, flux_auto and sprob fields.  With prefs NULL
+   the settings of sxg_set_photometry apply, with flagobj2 NULL every group is computed.
+   brain is read as sxg_brain (NULL: no CLASS_STAR); wcs is not read (the reference
+   computes no world coordinates).  Whole frames only: fymin 0, fymax height. */
 void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *objlist, void *wcs,
                    int fymin, int fymax, int width, int height, float backsig, float thresh,
                    double satur_level, double pixscale, double ngamma, double gain,
@@ -137,6 +142,13 @@ typedef struct {
 } sxg_photom_params;
 /* Returns 0, or -1 for invalid settings. */
 int sxg_set_photometry(const sxg_photom_params *p);
+
+/* The byte offsets endobject_gpu assumes in the reference's structures (NAPER 1, x86-64):
+   sizeof(prefstruct), prefs.naper, .apert, .autoparam, .autoaper, .seeing_fwhm,
+   .mag_zeropoint, .mask_type, .detect_type, sizeof(obj2struct), obj2.flux_aper, .flux_auto,
+   .sprob, then the values of PHOTO and MASK_CORRECT.  Writes min(n, count) values to out
+   and returns the count. */
+int sxg_reference_layout(long *out, int n);
 
 /* Photometry columns of the last endobject_gpu call (float, object order). */
 enum sxg_phot_column {

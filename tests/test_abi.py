@@ -114,3 +114,32 @@ def test_isorow_layout_matches_header(tmp_path):
     assert got[0] == ctypes.sizeof(sexgpu.IsoRow) == sexgpu.ISOROW_DTYPE.itemsize
     assert got[1:] == [getattr(sexgpu.IsoRow, f).offset for f in fields]
     assert got[1:] == [sexgpu.ISOROW_DTYPE.fields[f][1] for f in fields]
+
+
+REFSRC = "/root/reference/src"
+
+
+@pytest.mark.skipif(not os.path.isdir(REFSRC), reason="the reference's headers are not here")
+def test_reference_struct_offsets(tmp_path):
+    """endobject_gpu reads the reference front-end's prefstruct and obj2struct at fixed
+    offsets: a program compiled against the reference's own headers (src/prefs.h,
+    src/types.h, NAPER 1) prints the same offsets, sizes and enum values."""
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "define.h"\n#include "globals.h"\n'
+        '#include "prefs.h"\n#define O(s, f) printf(" %zu", offsetof(s, f))\n'
+        'int main(void) { printf("%zu", sizeof(prefstruct)); O(prefstruct, naper);\n'
+        'O(prefstruct, apert); O(prefstruct, autoparam); O(prefstruct, autoaper);\n'
+        'O(prefstruct, seeing_fwhm); O(prefstruct, mag_zeropoint); O(prefstruct, mask_type);\n'
+        'O(prefstruct, detect_type); printf(" %zu", sizeof(obj2struct));\n'
+        'O(obj2struct, flux_aper); O(obj2struct, flux_auto); O(obj2struct, sprob);\n'
+        'printf(" %d %d\\n", (int)PHOTO, (int)MASK_CORRECT); return 0; }\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-DHAVE_CONFIG_H", "-I", os.path.dirname(REFSRC), "-I", REFSRC,
+                    "-I", os.path.join(REFSRC, "fits"), "-I", os.path.join(REFSRC, "wcs"), "-w",
+                    "-o", str(exe), str(src)], check=True)
+    want = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                           text=True).stdout.split()]
+    out = (ctypes.c_long * 32)()
+    n = sexgpu.gpu().sxg_reference_layout(out, 32)
+    assert list(out[:n]) == want

@@ -166,3 +166,70 @@ def test_sex_binary_fits_ldac(tmp_path, frame1024, masks):
     assert np.array_equal(t["FLAGS"], g["flag_phot"][keep].astype(np.int16))
     assert np.array_equal(t["ISO0"], g["iso"][keep][:, 0])
     assert np.array_equal(t["CLASS_STAR"], g["phot"]["class_star"][keep])
+
+
+# the settings of the reference's tests/default.sex that the detection path reads
+GALAXIES_SEX = {"DETECT_MINAREA": "2", "DETECT_THRESH": "1.5", "ANALYSIS_THRESH": "2",
+                "FILTER": "Y", "DEBLEND_NTHRESH": "32", "DEBLEND_MINCONT": "0.005",
+                "CLEAN": "Y", "CLEAN_PARAM": "1.0", "MASK_TYPE": "CORRECT",
+                "PHOT_APERTURES": "5", "PHOT_AUTOPARAMS": "2.5, 3.5", "SATUR_LEVEL": "50000.0",
+                "MAG_ZEROPOINT": "0.0", "GAIN": "0.0", "PIXEL_SCALE": "1.0",
+                "SEEING_FWHM": "1.2", "BACK_SIZE": "64", "BACK_FILTERSIZE": "3",
+                "MEMORY_OBJSTACK": "3000", "WEIGHT_TYPE": "NONE"}
+
+
+def test_reference_test_frame_against_oracle(tmp_path):
+    """The reference's own test run (tests/efigi1.test: tests/galaxies.fits, BITPIX 16
+    with BSCALE/BZERO, with its tests/default.sex) through bin/sex, writing the rows of
+    the reference's printobj (CATALOG_ROWS REFERENCE): the segmentation check-image equals
+    the oracle's on the same decoded frame, and every row field equals the oracle's value
+    to its printed precision (survivors numbered from 0, as scan.c:150-156 does)."""
+    from conftest import GOLDEN
+    import sxo
+    path = os.path.join(GOLDEN, "galaxies.fits")
+    (tmp_path / "p.param").write_text("NUMBER\nX_IMAGE\nY_IMAGE\n")
+    cfg = dict(GALAXIES_SEX, CATALOG_NAME=str(tmp_path / "g.cat"), CATALOG_TYPE="ASCII",
+               CATALOG_ROWS="REFERENCE", PARAMETERS_NAME=str(tmp_path / "p.param"),
+               FILTER_NAME=os.path.join(CONF, "default.conv"),
+               STARNNW_NAME=os.path.join(CONF, "default.nnw"), CHECKIMAGE_TYPE="SEGMENTATION",
+               CHECKIMAGE_NAME=str(tmp_path / "seg.fits"), VERBOSE_TYPE="QUIET")
+    (tmp_path / "g.sex").write_text("".join("%s %s\n" % kv for kv in cfg.items()))
+    r = subprocess.run([BIN, path, "-c", str(tmp_path / "g.sex")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = np.array([[float(v) for v in ln.split()]
+                     for ln in (tmp_path / "g.cat").read_text().splitlines()])
+    img = sexgpu.read_fits(path)
+    o = sxo.run(img, mask=sexgpu.read_conv("default.conv"), ext_minarea=2, analysis_thresh=2.0,
+                photom={"nnw": sxo.read_nnw(os.path.join(CONF, "default.nnw")), "apert": 5.0,
+                        "autoparam": (2.5, 3.5)})
+    seg = sexgpu.read_fits(str(tmp_path / "seg.fits"))
+    assert np.array_equal(seg.astype(np.int64), o["seg"].astype(np.int64))
+    keep = o["master"] == 0xFFFFFFFF
+    ph = {k: v[keep] for k, v in o["phot"].items() if k != "labels"}
+    n = int(keep.sum())
+    assert rows.shape == (n, 31) and n == o["nsurv"] > 100
+    f4 = np.float32
+    want = {0: np.arange(n), 3: (o["mx"][keep] + 1.0).astype(f4), 4: (o["my"][keep] + 1.0).astype(f4),
+            5: ph["flux_iso"], 6: ph["fluxerr_iso"], 7: ph["mag_iso"], 8: ph["magerr_iso"],
+            9: ph["flux_isocor"], 10: ph["fluxerr_isocor"], 11: ph["flux_aper"],
+            12: ph["fluxerr_aper"], 13: ph["mag_aper"], 14: ph["magerr_aper"],
+            15: ph["flux_auto"], 16: ph["fluxerr_auto"], 17: ph["mag_auto"],
+            18: ph["magerr_auto"], 19: o["theta"][keep], 20: o["poserr_mx2"][keep],
+            21: o["poserr_my2"][keep], 22: o["poserr_mxy"][keep], 23: ph["flag"],
+            24: ph["elong"], 25: ph["ellip"], 26: ph["sprob"], 27: o["peak"][keep],
+            28: ph["kronfactor"], 29: o["bkg"][keep], 30: o["fwhm"][keep]}
+    # the printed precision of each field (scan.c:331-406)
+    fixed = {3: 6e-4, 4: 6e-4, 7: 6e-5, 8: 6e-5, 13: 6e-5, 14: 6e-5, 17: 6e-5, 18: 6e-5,
+             19: 0.06, 24: 6e-4, 25: 6e-4, 26: 6e-3, 28: 6e-3, 30: 6e-3}
+    for c, v in want.items():
+        v = np.asarray(v, np.float64)
+        if c in (0, 23):
+            assert np.array_equal(rows[:, c], v), c
+        elif c in fixed:
+            assert np.allclose(rows[:, c], v, rtol=0, atol=fixed[c]), c
+        elif c in (20, 21, 22):
+            assert np.allclose(rows[:, c], v, rtol=1e-9, atol=0), c
+        else:
+            assert np.allclose(rows[:, c], v, rtol=1e-6, atol=1e-30), c
+    assert np.all(rows[:, 1] == 0.0) and np.all(rows[:, 2] == 0.0)

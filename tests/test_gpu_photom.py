@@ -111,3 +111,48 @@ def test_empty_frame():
                                             analysis_thresh=1e6),
                          outputs=("columns", "phot"), brain=sexgpu.read_nnw(NNW))
     assert g["nobj"] == 0 and all(len(v) == 0 for v in g["phot"].values())
+
+
+def _reference_structs(apert, autoparam, seeing, zp, mask_correct, groups, naper=1):
+    """A prefstruct and an obj2struct of flags in the reference's layout, as its front-end
+    fills them (prefs.c, and FLAG(obj2.x) set for the .param groups)."""
+    import struct
+    lay = sexgpu.reference_layout()
+    prefs = bytearray(lay["prefs_size"])
+    struct.pack_into("<i", prefs, lay["naper"], naper)
+    struct.pack_into("<d", prefs, lay["apert"], apert)
+    struct.pack_into("<2d", prefs, lay["autoparam"], *autoparam)
+    struct.pack_into("<2d", prefs, lay["autoaper"], 0.0, 0.0)
+    struct.pack_into("<d", prefs, lay["seeing_fwhm"], seeing)
+    struct.pack_into("<d", prefs, lay["mag_zeropoint"], zp)
+    struct.pack_into("<i", prefs, lay["mask_type"], lay["MASK_CORRECT"] if mask_correct else 0)
+    struct.pack_into("<i", prefs, lay["detect_type"], 0)
+    flags = bytearray(lay["obj2_size"])
+    for g in groups:
+        flags[lay[g]] = 1
+    return prefs, flags
+
+
+@pytest.mark.parametrize("groups", [("flux_aper", "flux_auto", "sprob"), ("flux_auto",)])
+def test_reference_prefs_and_flags(frame1024, masks, groups):
+    """endobject_gpu as scan.c:124-143 calls it, with the reference's prefs and obj2 flags
+    (no sxg_set_photometry): the settings read from them give the oracle's columns for the
+    same settings, and the groups the flags leave off are not measured."""
+    prefs, flags = _reference_structs(8.0, (3.0, 2.0), 1.0, 25.0, False, groups)
+    g = sexgpu.run_frame(frame1024, sexgpu.Params(mask=masks["default"]),
+                         outputs=("columns", "phot"), brain=sexgpu.read_nnw(NNW), prefs=prefs,
+                         flagobj2=flags)
+    o = sxo.run(frame1024, mask=masks["default"], photom={
+        "nnw": sxo.read_nnw(NNW), "apert": 8.0, "autoparam": (3.0, 2.0), "seeing_fwhm": 1.0,
+        "mag_zeropoint": 25.0, "mask_correct": 0, "do_aper": int("flux_aper" in groups),
+        "do_auto": int("flux_auto" in groups), "do_class": int("sprob" in groups)})
+    compare(g, o)
+    if "flux_aper" not in groups:
+        assert not np.any(g["phot"]["flux_aper"]) and not np.any(g["phot"]["class_star"])
+
+
+def test_reference_prefs_with_other_naper_is_an_error(frame1024, masks):
+    prefs, flags = _reference_structs(5.0, (2.5, 3.5), 1.2, 0.0, True, ("flux_aper",), naper=3)
+    with pytest.raises(RuntimeError, match="naper"):
+        sexgpu.run_frame(frame1024, sexgpu.Params(mask=masks["default"]),
+                         outputs=("columns", "phot"), prefs=prefs, flagobj2=flags)

@@ -457,7 +457,8 @@ __global__ void __launch_bounds__(1024) k_phot_sum(const int *v, int n, unsigned
 }
 
 /* catalogue rows, the outobjstruct fields endobject_kernel fills (cudaanalyse.cu:3049-3095) */
-__global__ void __launch_bounds__(256) k_phot_rows(PhotArgs A, sxg_outobj *rows)
+__global__ void __launch_bounds__(256) k_phot_rows(PhotArgs A, const double *alpha,
+                                                   const double *delta, sxg_outobj *rows)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= A.nobj) return;
@@ -465,8 +466,8 @@ __global__ void __launch_bounds__(256) k_phot_rows(PhotArgs A, sxg_outobj *rows)
     const PhotCols &p = A.p;
     sxg_outobj r = {};
     r.number = i + 1;
-    r.alpha2000 = 0.0;
-    r.delta2000 = 0.0;
+    r.alpha2000 = alpha[i];
+    r.delta2000 = delta[i];
     r.sposx = (float)(o.mx[i] + 1.0);
     r.sposy = (float)(o.my[i] + 1.0);
     r.flux_iso = p.flux_iso[i];
@@ -562,7 +563,8 @@ extern "C" int sxg_reference_layout(long *out, int n)
                              RL_DETECT_PHOTO, RL_MASK_CORRECT};
     const int nv = (int)(sizeof(v) / sizeof(v[0]));
     for (int i = 0; out && i < n && i < nv; i++) out[i] = v[i];
-    return nv;
+    /* then the wcsstruct offsets (wcs.hip) */
+    return nv + sxg_wcs_layout(out && n > nv ? out + nv : nullptr, n > nv ? n - nv : 0);
 }
 
 /* the photometry settings of the reference's prefs (and the groups its obj2 flags ask for:
@@ -679,7 +681,6 @@ extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *ob
                               float thresh, double satur_level, double pixscale, double ngamma,
                               double gain, int numobj)
 {
-    (void)wcs;
     SxgState &S = sxg_state();
     sxg_clear_error();
     if (!S.have_frame || !S.cleaned || numobj != S.nobj || width != S.W || height != S.H) {
@@ -703,7 +704,8 @@ extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *ob
     pp.mag_zeropoint = S.mag_zeropoint;
     if (!prefs_settings(prefs, flagobj2, pp)) return;
     if (!run_photometry(pp, static_cast<const sxg_brain *>(brain), backsig, thresh, satur_level,
-                        pixscale, ngamma, gain))
+                        pixscale, ngamma, gain) ||
+        !sxg_astrometry(wcs))
         return;
     unsigned long long visited = 0;
     unsigned long long *vsum = sxg_take<unsigned long long>("phot.npix_sum", 1);
@@ -718,6 +720,8 @@ extern "C" void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *ob
         sxg_outobj *drows = sxg_take<sxg_outobj>("phot.rows", (size_t)numobj);
         if (!drows) return;
         hipLaunchKernelGGL(k_phot_rows, dim3(sxg_div_up(numobj, 256)), dim3(256), 0, S.stream, A,
+                           (const double *)sxg_take<double>("wcs.alpha", (size_t)numobj),
+                           (const double *)sxg_take<double>("wcs.delta", (size_t)numobj),
                            drows);
         SXG_HIP(hipMemcpyAsync(objlist, drows, sizeof(sxg_outobj) * numobj,
                                hipMemcpyDeviceToHost, S.stream));

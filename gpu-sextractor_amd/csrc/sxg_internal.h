@@ -172,6 +172,10 @@ struct SxgState {
     double mag_zeropoint = 0.0;
     sxg_photom_params phot = {5.0, {2.5, 3.5}, {0.0, 0.0}, 1.2, 0.0, 1, 0, 1, 1, 1};
     bool phot_ready = false;               /* endobject_gpu ran on this frame */
+    /* world coordinates: the WCS of sxg_set_wcs (a WcsDev, wcs.hip) and whether the
+       "wcs.*" columns hold the current frame's */
+    bool wcs_set = false, wcs_ready = false;
+    alignas(8) unsigned char wcs_dev[128] = {};
     float stage_ms[ST_N] = {};
 
     /* per-context workspace */
@@ -221,6 +225,9 @@ bool sxg_build_segmap();   /* device segmentation map ("out.map") and survivor c
 /* queue the catalogue rows of the survivors and their copy to host memory (after the
    segmentation map); the number of rows copied, or -1 */
 int sxg_catalog_async(sxg_isorow *out, int cap);
+/* world coordinates of the current frame's objects into the "wcs.*" columns (wcs.hip) */
+bool sxg_astrometry(const void *wcs);
+int sxg_wcs_layout(long *out, int n);
 
 SxgState &sxg_state();
 /* End of an entry point without a host output: the contexts of the batch workers

@@ -130,6 +130,79 @@ float *sxf_read_image(const char *path, int *width, int *height, double *gain, d
     return img;
 }
 
+/* the string value of a card (quotes and trailing blanks removed) */
+static int card_string(const char *card, const char *key, char *out, int n)
+{
+    size_t kl = strlen(key);
+    if (strncmp(card, key, kl) || (card[kl] != ' ' && card[kl] != '=')) return 0;
+    const char *q = memchr(card, '\'', 80);
+    if (!q) return 0;
+    int k = 0;
+    for (const char *c = q + 1; c < card + 80 && *c != '\'' && k < n - 1; c++) out[k++] = *c;
+    while (k > 0 && out[k - 1] == ' ') k--;
+    out[k] = 0;
+    return 1;
+}
+
+int sxf_read_wcs(const char *path, char ctype[2][9], double crval[2], double crpix[2],
+                 double cd[4], double *longpole, double *latpole, double *equinox)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    char card[80], s[80];
+    int have_ct[2] = {0, 0}, have_cd = 0, have_pc = 0, have_eq = 0;
+    double cdelt[2] = {1.0, 1.0}, cdm[4] = {1.0, 0.0, 0.0, 1.0}, pc[4] = {1.0, 0.0, 0.0, 1.0};
+    double crota = 0.0, epoch = 2000.0, v;
+    crval[0] = crval[1] = 0.0;
+    crpix[0] = crpix[1] = 1.0;
+    *longpole = *latpole = 999.0;
+    *equinox = 2000.0;
+    while (fread(card, 1, 80, f) == 80) {
+        if (!strncmp(card, "END ", 4)) break;
+        for (int l = 0; l < 2; l++) {
+            char key[16];
+            snprintf(key, sizeof(key), "CTYPE%d", l + 1);
+            if (card_string(card, key, s, sizeof(s))) {
+                const size_t m = strlen(s) < 8 ? strlen(s) : 8;
+                memset(ctype[l], 0, 9);
+                memcpy(ctype[l], s, m);
+                have_ct[l] = 1;
+            }
+            snprintf(key, sizeof(key), "CRVAL%d", l + 1);
+            if (card_value(card, key, &v)) crval[l] = v;
+            snprintf(key, sizeof(key), "CRPIX%d", l + 1);
+            if (card_value(card, key, &v)) crpix[l] = v;
+            snprintf(key, sizeof(key), "CDELT%d", l + 1);
+            if (card_value(card, key, &v)) cdelt[l] = v;
+            for (int j = 0; j < 2; j++) {
+                snprintf(key, sizeof(key), "CD%d_%d", l + 1, j + 1);
+                if (card_value(card, key, &v)) { cdm[2 * l + j] = v; have_cd = 1; }
+                snprintf(key, sizeof(key), "PC%03d%03d", l + 1, j + 1);
+                if (card_value(card, key, &v)) { pc[2 * l + j] = v; have_pc = 1; }
+            }
+        }
+        if (card_value(card, "CROTA2", &v)) crota = v;
+        if (card_value(card, "LONPOLE", &v) || card_value(card, "LONGPOLE", &v)) *longpole = v;
+        if (card_value(card, "LATPOLE", &v)) *latpole = v;
+        if (card_value(card, "EPOCH", &v)) epoch = v;
+        if (card_value(card, "EQUINOX", &v)) { *equinox = v; have_eq = 1; }
+    }
+    fclose(f);
+    if (!have_eq) *equinox = epoch;
+    if (have_cd) {
+        for (int k = 0; k < 4; k++) cd[k] = cdm[k];
+    } else if (have_pc) {
+        for (int k = 0; k < 4; k++) cd[k] = pc[k] * cdelt[k / 2];
+    } else {
+        const double r = crota * 3.14159265358979323846 / 180.0;
+        cd[0] = cos(r) * cdelt[0];
+        cd[2] = sin(r) * cdelt[0];
+        cd[1] = -sin(r) * cdelt[1];
+        cd[3] = cos(r) * cdelt[1];
+    }
+    return have_ct[0] && have_ct[1];
+}
+
 static void put_card(FILE *f, const char *text)
 {
     char card[80];

@@ -111,7 +111,8 @@ struct Column {
 char column_tform(const std::string &n)
 {
     static const char *dbl[] = {"X2_IMAGE", "Y2_IMAGE", "XY_IMAGE", "ERRX2_IMAGE",
-                                "ERRY2_IMAGE", "ERRXY_IMAGE"};
+                                "ERRY2_IMAGE", "ERRXY_IMAGE", "ALPHA_J2000", "DELTA_J2000",
+                                "X_WORLD", "Y_WORLD"};
     static const char *ints[] = {"NUMBER", "ISOAREA_IMAGE", "XMIN_IMAGE", "YMIN_IMAGE",
                                  "XMAX_IMAGE", "YMAX_IMAGE", "XPEAK_IMAGE", "YPEAK_IMAGE"};
     for (const char *d : dbl)
@@ -198,6 +199,10 @@ void put_be(std::vector<char> &out, const void *p, int n)
 /* names, descriptions, formats and units as in the reference's key table (globals.c) */
 const Column kColumns[] = {
     {"NUMBER", "Running object number", "%10d", ""},
+    {"ALPHA_J2000", "Right ascension of barycenter (J2000)", "%11.7f", "deg"},
+    {"DELTA_J2000", "Declination of barycenter (J2000)", "%+11.7f", "deg"},
+    {"X_WORLD", "Barycenter position along world x axis", "%15e", "deg"},
+    {"Y_WORLD", "Barycenter position along world y axis", "%15e", "deg"},
     {"X_IMAGE", "Object position along x", "%10.3f", "pixel"},
     {"Y_IMAGE", "Object position along y", "%10.3f", "pixel"},
     {"FLUX_ISO", "Isophotal flux", "%12.7g", "count"},
@@ -595,10 +600,28 @@ int main(int argc, char **argv)
         return 1;
     }
     const double pixscale = cfg.num("PIXEL_SCALE") > 0.0 ? cfg.num("PIXEL_SCALE") : 1.0;
+    /* the frame's celestial WCS, for ALPHA_J2000/DELTA_J2000 and X_WORLD/Y_WORLD */
+    {
+        sxg_wcs wcs;
+        memset(&wcs, 0, sizeof(wcs));
+        const int hw = sxf_read_wcs(image.c_str(), wcs.ctype, wcs.crval, wcs.crpix, wcs.cd,
+                                    &wcs.longpole, &wcs.latpole, &wcs.equinox);
+        if (hw == 1 && sxg_set_wcs(&wcs) != 0) {
+            fprintf(stderr, "sex: warning: %s; world coordinates are 0\n", sxg_last_error());
+            sxg_set_wcs(nullptr);
+        } else if (hw != 1) {
+            sxg_set_wcs(nullptr);
+        }
+    }
     std::vector<sxg_outobj> rows(nobj > 0 ? nobj : 1);
     endobject_gpu(nullptr, pp.do_class ? &brain : nullptr, nullptr, rows.data(), nullptr, 0, H,
                   W, H, backsig, thresh, satur, pixscale, 0.0, gain, nobj);
     if (fail("endobject_gpu")) return 1;
+    std::vector<double> xw(nobj), yw(nobj), ra(nobj), dec(nobj);
+    if (nobj > 0 && sxg_get_world(xw.data(), yw.data(), ra.data(), dec.data(), nobj) != 0) {
+        fail("sxg_get_world");
+        return 1;
+    }
     /* extra columns not in the reference's fixed row */
     std::vector<int> npixc(nobj), xmin(nobj), xmax(nobj), ymin(nobj), ymax(nobj), px(nobj),
         py(nobj), iso(8 * (size_t)nobj);
@@ -673,6 +696,10 @@ int main(int argc, char **argv)
             return c;
         };
         if (n == "NUMBER") return val(number);
+        else if (n == "ALPHA_J2000") return val(ra[i]);
+        else if (n == "DELTA_J2000") return val(dec[i]);
+        else if (n == "X_WORLD") return val(xw[i]);
+        else if (n == "Y_WORLD") return val(yw[i]);
         else if (n == "X_IMAGE") return val(r.sposx);
         else if (n == "Y_IMAGE") return val(r.sposy);
         else if (n == "FLUX_ISO") return val(r.flux_iso);

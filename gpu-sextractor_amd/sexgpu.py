@@ -58,7 +58,27 @@ def host():
                                          C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_char_p)]
         _host.sxf_write_image.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
+        _host.sxf_read_wcs.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
     return _host
+
+
+def read_fits_wcs(path):
+    """The celestial WCS keywords of a FITS header (host/fitsio.c), or None without CTYPEs:
+    dict of ctype, crval, crpix, cd, longpole, latpole, equinox."""
+    ct = ((C.c_char * 9) * 2)()
+    crval, crpix, cd = (C.c_double * 2)(), (C.c_double * 2)(), (C.c_double * 4)()
+    lp, la, eq = C.c_double(), C.c_double(), C.c_double()
+    rc = host().sxf_read_wcs(path.encode(), ct, crval, crpix, cd, C.byref(lp), C.byref(la),
+                             C.byref(eq))
+    if rc < 0:
+        raise OSError("cannot read " + path)
+    if rc == 0:
+        return None
+    return {"ctype": (ct[0].value.decode(), ct[1].value.decode()), "crval": list(crval),
+            "crpix": list(crpix), "cd": list(cd), "longpole": lp.value, "latpole": la.value,
+            "equinox": eq.value}
 
 
 def synth_params(preset="plain1024", **kw):
@@ -363,6 +383,8 @@ def gpu():
         L.sxg_back_estimate_noalloc.restype = i
         L.sxg_get_catalog.argtypes = [vp, i]
         L.sxg_reference_layout.argtypes = [vp, i]
+        L.sxg_set_wcs.argtypes = [vp]
+        L.sxg_get_world.argtypes = [vp, vp, vp, vp, sz]
         L.sxg_batch_process_host.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp]
         _gpu = L
     return _gpu
@@ -386,7 +408,8 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_batch_segmap", "sxg_batch_release", "sxg_set_weight",
             "sxg_set_photometry", "sxg_get_photometry", "sxg_set_weight_map",
             "sxg_get_back_rms", "sxg_back_estimate_noalloc", "sxg_get_catalog",
-            "sxg_batch_process_host", "sxg_reference_layout"]
+            "sxg_batch_process_host", "sxg_reference_layout", "sxg_set_wcs",
+            "sxg_get_world"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort",
@@ -564,12 +587,13 @@ def back_rms(shape):
 
 def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"), weight=None,
               brain=None, pixscale=1.0, ngamma=0.0, weight_type=0, weight_gain=0, prefs=None,
-              flagobj2=None):
+              flagobj2=None, wcs=None):
     """Run the whole path in the order of the reference's scan.c:78-146 and collect the
     requested outputs (numpy arrays).  weight: optional MAP_WEIGHT map; brain: the
     CLASS_STAR network (read_nnw) for the catalogue; "phot" in outputs: the photometry
     columns of endobject_gpu; prefs, flagobj2: bytearrays laid out as the reference's
-    prefstruct and obj2struct (reference_layout()), passed to endobject_gpu."""
+    prefstruct and obj2struct (reference_layout()), passed to endobject_gpu; wcs: the
+    address of a wcsstruct of the reference's layout, passed to endobject_gpu."""
     L = gpu()
     img = np.ascontiguousarray(img, dtype=np.float32)
     H, W = img.shape
@@ -622,10 +646,11 @@ def run_frame(img, p, outputs=("columns", "seg", "alloc", "sub", "cd", "catalog"
         def buf(b):
             return None if b is None else C.cast((C.c_char * len(b)).from_buffer(b), C.c_void_p)
         L.endobject_gpu(buf(prefs), C.byref(brain) if brain is not None else None,
-                        buf(flagobj2), C.cast(rows, C.c_void_p), None, 0, H, W, H, bsig,
+                        buf(flagobj2), C.cast(rows, C.c_void_p), wcs, 0, H, W, H, bsig,
                         thresh, p.satur_level, pixscale, ngamma, p.gain, nobj)
         check()
         out["catalog"] = rows
+        out["world"] = get_world(nobj)
         if "phot" in outputs:
             out["phot"] = {k: get_photometry(k, nobj) for k in PHOT_COLUMNS}
             out["flag_phot"] = get_column("flag", nobj)
@@ -686,7 +711,41 @@ def run_drop_in(img, p, seg, sub, master, rows):
 
 REFERENCE_LAYOUT = ["prefs_size", "naper", "apert", "autoparam", "autoaper", "seeing_fwhm",
                     "mag_zeropoint", "mask_type", "detect_type", "obj2_size", "flux_aper",
-                    "flux_auto", "sprob", "PHOTO", "MASK_CORRECT"]
+                    "flux_auto", "sprob", "PHOTO", "MASK_CORRECT", "wcs_size", "wcs_naxis",
+                    "wcs_ctype", "wcs_crval", "wcs_crpix", "wcs_cd", "wcs_longpole",
+                    "wcs_latpole", "wcs_lat", "wcs_lng", "wcs_equinox"]
+
+
+class SxgWcs(C.Structure):
+    """Mirror of sxg_wcs (include/sexgpu.h)."""
+    _fields_ = [("ctype", (C.c_char * 9) * 2), ("crval", C.c_double * 2),
+                ("crpix", C.c_double * 2), ("cd", C.c_double * 4), ("longpole", C.c_double),
+                ("latpole", C.c_double), ("equinox", C.c_double)]
+
+
+def set_wcs(ctype=None, crval=(0.0, 0.0), crpix=(0.0, 0.0), cd=(1.0, 0.0, 0.0, 1.0),
+            longpole=999.0, latpole=999.0, equinox=2000.0):
+    """sxg_set_wcs (ctype None removes the WCS)."""
+    if ctype is None:
+        gpu().sxg_set_wcs(None)
+        return
+    w = SxgWcs()
+    for k in range(2):
+        w.ctype[k].value = ctype[k].encode()
+        w.crval[k], w.crpix[k] = crval[k], crpix[k]
+    for k in range(4):
+        w.cd[k] = cd[k]
+    w.longpole, w.latpole, w.equinox = longpole, latpole, equinox
+    if gpu().sxg_set_wcs(C.byref(w)) != 0:
+        check()
+
+
+def get_world(n):
+    """(X_WORLD, Y_WORLD, ALPHA_J2000, DELTA_J2000) of the last endobject_gpu call."""
+    out = [np.zeros(max(n, 1)) for _ in range(4)]
+    if gpu().sxg_get_world(*[a.ctypes.data for a in out], max(n, 1)) != 0:
+        check()
+    return [a[:n] for a in out]
 
 
 def reference_layout():

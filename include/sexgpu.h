@@ -86,8 +86,11 @@ void run_clean(int *masterIndex, double clean_param, int clean_stacksize, int nu
    flagobj2 (the reference's obj2struct of flags) switches the APER, AUTO and CLASS_STAR
    groups by the first byte of its flux_aper, flux_auto and sprob fields.  With prefs NULL
    the settings of sxg_set_photometry apply, with flagobj2 NULL every group is computed.
-   brain is read as sxg_brain (NULL: no CLASS_STAR); wcs is not read (the reference
-   computes no world coordinates).  Whole frames only: fymin 0, fymax height. */
+   brain is read as sxg_brain (NULL: no CLASS_STAR).  wcs is the reference's wcsstruct:
+   with a celestial pair of a zenithal projection (TAN, SIN, ARC, STG, ZEA) the rows get
+   ALPHA_J2000/DELTA_J2000 (see sxg_set_wcs, which takes precedence; the reference's GPU
+   code writes zeros, cudaanalyse.cu:3033-3047).  Whole frames only: fymin 0, fymax
+   height. */
 void endobject_gpu(void *prefs, void *brain, void *flagobj2, void *objlist, void *wcs,
                    int fymin, int fymax, int width, int height, float backsig, float thresh,
                    double satur_level, double pixscale, double ngamma, double gain,
@@ -143,12 +146,31 @@ typedef struct {
 /* Returns 0, or -1 for invalid settings. */
 int sxg_set_photometry(const sxg_photom_params *p);
 
-/* The byte offsets endobject_gpu assumes in the reference's structures (NAPER 1, x86-64):
-   sizeof(prefstruct), prefs.naper, .apert, .autoparam, .autoaper, .seeing_fwhm,
+/* The byte offsets endobject_gpu assumes in the reference's structures (NAPER 1, NAXIS 2,
+   x86-64): sizeof(prefstruct), prefs.naper, .apert, .autoparam, .autoaper, .seeing_fwhm,
    .mag_zeropoint, .mask_type, .detect_type, sizeof(obj2struct), obj2.flux_aper, .flux_auto,
-   .sprob, then the values of PHOTO and MASK_CORRECT.  Writes min(n, count) values to out
-   and returns the count. */
+   .sprob, the values of PHOTO and MASK_CORRECT, then sizeof(wcsstruct), wcs.naxis, .ctype,
+   .crval, .crpix, .cd, .longpole, .latpole, .lat, .lng, .equinox.  Writes min(n, count)
+   values to out and returns the count. */
 int sxg_reference_layout(long *out, int n);
+
+/* World coordinates (no counterpart in the reference's GPU path, which writes zeros;
+   its CPU semantics: astrom.c:86-140, raw_to_wcs fitswcs.c:1251, precess fitswcs.c:1718):
+   the WCS of the frames processed next, from the FITS header keywords.  ctype: CTYPE1/2,
+   a right ascension / declination pair in either order with a zenithal projection (TAN,
+   SIN, ARC, STG, ZEA); cd: CD1_1, CD1_2, CD2_1, CD2_2; longpole 999: the default (180);
+   positions are precessed to J2000 from an equinox other than 2000 (IAU 1976).  NULL
+   removes it.  Returns 0, or -1 for a WCS that is not covered. */
+typedef struct {
+    char ctype[2][9];
+    double crval[2], crpix[2], cd[4];
+    double longpole, latpole, equinox;
+} sxg_wcs;
+int sxg_set_wcs(const sxg_wcs *wcs);
+/* After endobject_gpu: X_WORLD, Y_WORLD (the axes' order, the WCS's equinox) and
+   ALPHA_J2000, DELTA_J2000 in degrees, object order (any pointer may be NULL); zeros
+   without a WCS, 1e31 outside the projection's domain.  Returns 0 or < 0. */
+int sxg_get_world(double *xw, double *yw, double *alpha2000, double *delta2000, size_t count);
 
 /* Photometry columns of the last endobject_gpu call (float, object order). */
 enum sxg_phot_column {

@@ -10,9 +10,13 @@
  * unchanged: backstat/backhisto/backguess (back.c:214-426) once per mesh row, exactly as
  * stock makeback does when a mesh row fits BACK_BUFSIZE; filterback (back.c:434-560);
  * makebackspline (back.c:739-780); subbackline (back.c:790-921); convolve (filter.c:39)
- * and getconv (filter.c:101); hmedian (misc.c:30).
+ * and getconv (filter.c:101); hmedian (misc.c:30); and the world coordinates of
+ * computeastrom (astrom.c:97-120): create_wcs/init_wcs (fitswcs.c:99-252), raw_to_wcs
+ * (fitswcs.c:1251) with the reference's WCSLIB (src/wcs/) and precess (fitswcs.c:1718).
  */
 #include "config.h"
+#include <math.h>
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -23,6 +27,7 @@
 #include "back.h"
 #include "field.h"
 #include "filter.h"
+#include "fitswcs.h"
 
 /* per-mesh statistics: mean sigma lcut hcut qzero qscale nlevels npix, then mode/sigma */
 int ref_makeback(const float *img, int W, int H, int bs, float *stats, float *back, float *sigma)
@@ -152,3 +157,60 @@ int ref_getconv(const char *path, float *mask, int maxn, int *mw, int *mh)
 }
 
 float ref_hmedian(float *ra, int n) { return hmedian(ra, n); }
+
+/* a wcsstruct built by the reference's create_wcs from header values, with the CD matrix,
+   LONPOLE/LATPOLE and EQUINOX set and init_wcs run again (as read_wcs leaves it) */
+void *ref_wcs_new(const char *ctype1, const char *ctype2, const double *crval,
+                  const double *crpix, const double *cd, double longpole, double latpole,
+                  double equinox)
+{
+    char c1[9] = {0}, c2[9] = {0};
+    char *ct[2] = {c1, c2};
+    int naxisn[2] = {4096, 4096};
+    strncpy(c1, ctype1, 8);
+    strncpy(c2, ctype2, 8);
+    wcsstruct *w = create_wcs(ct, (double *)crval, (double *)crpix, NULL, naxisn, 2);
+    if (!w) return NULL;
+    for (int k = 0; k < 4; k++) w->cd[k] = cd[k];
+    w->longpole = longpole;
+    w->latpole = latpole;
+    w->epoch = w->equinox = equinox;
+    init_wcs(w);
+    return w;
+}
+
+/* astrom.c:97-120: world coordinates (mxw, myw) and J2000 (alpha2000, delta2000) of n
+   1-based pixel positions; -1 when a position has no mapping */
+int ref_wcs_world(void *wv, int n, const double *pix, double *world, double *radec)
+{
+    wcsstruct *wcs = (wcsstruct *)wv;
+    const int lng = wcs->lng, lat = wcs->lat;
+    for (int i = 0; i < n; i++) {
+        double raw[2] = {pix[2 * i], pix[2 * i + 1]}, wp[2];
+        if (raw_to_wcs(wcs, raw, wp)) return -1;
+        world[2 * i] = wp[0];
+        world[2 * i + 1] = wp[1];
+        if (fabs(wcs->equinox - 2000.0) > 0.003)
+            precess(wcs->equinox, wp[lng < lat ? 0 : 1], wp[lng < lat ? 1 : 0], 2000.0,
+                    &radec[2 * i], &radec[2 * i + 1]);
+        else {
+            radec[2 * i] = lng < lat ? wp[0] : wp[1];
+            radec[2 * i + 1] = lng < lat ? wp[1] : wp[0];
+        }
+    }
+    return 0;
+}
+
+/* offsets of the wcsstruct fields a GPU astrometry reads (types.h:547-592) */
+int ref_wcs_layout(long *out, int n)
+{
+    const long v[] = {(long)sizeof(wcsstruct), (long)offsetof(wcsstruct, naxis),
+                      (long)offsetof(wcsstruct, ctype), (long)offsetof(wcsstruct, crval),
+                      (long)offsetof(wcsstruct, crpix), (long)offsetof(wcsstruct, cd),
+                      (long)offsetof(wcsstruct, longpole), (long)offsetof(wcsstruct, latpole),
+                      (long)offsetof(wcsstruct, lat), (long)offsetof(wcsstruct, lng),
+                      (long)offsetof(wcsstruct, equinox)};
+    const int nv = (int)(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < n && i < nv; i++) out[i] = v[i];
+    return nv;
+}

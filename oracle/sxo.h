@@ -115,6 +115,22 @@ void sxo_convolve(const float *in, int W, int H, const float *mask, int mw, int 
 /* ---------------- whole path ---------------- */
 void sxo_config_defaults(sxo_config *c);
 sxo_result *sxo_run(const float *img, int W, int H, const sxo_config *cfg);
+
+/* world coordinates (sxo_wcs.c): a celestial WCS with a zenithal projection */
+typedef struct {
+    int lng, lat;             /* the axes of right ascension and declination */
+    int proj;                 /* 0 TAN, 1 SIN, 2 ARC, 3 STG, 4 ZEA */
+    double crval[2], crpix[2], cd[4];
+    double phip;              /* native longitude of the celestial pole (LONPOLE) */
+    double equinox;
+} sxo_wcs;
+/* 0, or -1 for a WCS this restatement does not cover; longpole 999: the default */
+int sxo_wcs_setup(sxo_wcs *w, const char *ctype1, const char *ctype2, const double *crval,
+                  const double *crpix, const double *cd, double longpole, double equinox);
+/* pixel (px, py), 1-based, to world[lng, lat] (degrees, the equinox of the WCS) and
+   radec[2] (J2000); 0, or -1 outside the projection's domain */
+int sxo_wcs_world(const sxo_wcs *w, double px, double py, double *world, double *radec);
+void sxo_precess(double yin, double ain, double din, double yout, double *aout, double *dout);
 /* the bilinear value of a nx x ny mesh map b at pixel (x, y), meshes of backw x backh
    (back.c:685-732, cudaanalyse.cu:2128-2172): the BACKGROUND of an object's centroid */
 float sxo_back_at(const float *b, int x, int y, int nx, int ny, int backw, int backh);

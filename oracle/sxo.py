@@ -41,6 +41,12 @@ def lib():
         _lib.sxo_back_at.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_int]
         _lib.sxo_back_at.restype = C.c_float
+        _lib.sxo_wcs_setup.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+        _lib.sxo_wcs_world.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p,
+                                       C.c_void_p]
+        _lib.sxo_precess.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib.sxo_backguess.restype = C.c_float
         _lib.sxo_getconv.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]
@@ -80,6 +86,11 @@ def ref():
         _ref.ref_subback.argtypes = [vp, vp, i, i, i, i, i, vp]
         _ref.ref_backrms.argtypes = [vp, vp, i, i, i, i, i, vp]
         _ref.ref_convolve.argtypes = [vp, i, i, vp, i, i, vp]
+        _ref.ref_wcs_new.argtypes = [C.c_char_p, C.c_char_p, vp, vp, vp, C.c_double,
+                                     C.c_double, C.c_double]
+        _ref.ref_wcs_new.restype = vp
+        _ref.ref_wcs_world.argtypes = [vp, i, vp, vp, vp]
+        _ref.ref_wcs_layout.argtypes = [vp, i]
     return _ref
 
 
@@ -339,6 +350,49 @@ def run(img, mask=None, weight=None, truth=False, photom=None, **kw):
         out["phot"] = _photom(rp, r, kw, photom)
     lib().sxo_free(rp)
     return out
+
+
+class SxoWcs(C.Structure):
+    """Mirror of sxo_wcs (sxo.h)."""
+    _fields_ = [("lng", C.c_int), ("lat", C.c_int), ("proj", C.c_int),
+                ("crval", C.c_double * 2), ("crpix", C.c_double * 2), ("cd", C.c_double * 4),
+                ("phip", C.c_double), ("equinox", C.c_double)]
+
+
+def wcs_world(ctype, crval, crpix, cd, pix, longpole=999.0, equinox=2000.0):
+    """World coordinates (n x 2, the axes' order) and J2000 right ascension and
+    declination (n x 2) of 1-based pixel positions pix (n x 2) by the oracle's zenithal
+    WCS; rows outside the projection's domain are NaN."""
+    w = SxoWcs()
+    arr = lambda v: np.ascontiguousarray(v, np.float64)
+    cv, cp, m = arr(crval), arr(crpix), arr(cd).ravel()
+    if lib().sxo_wcs_setup(C.byref(w), ctype[0].encode(), ctype[1].encode(), cv.ctypes.data,
+                           cp.ctypes.data, m.ctypes.data, longpole, equinox) != 0:
+        raise ValueError("WCS not covered by the oracle: %r" % (ctype,))
+    pix = arr(pix).reshape(-1, 2)
+    world = np.full_like(pix, np.nan)
+    radec = np.full_like(pix, np.nan)
+    wo, ro = np.zeros(2), np.zeros(2)
+    for k, (x, y) in enumerate(pix):
+        if lib().sxo_wcs_world(C.byref(w), x, y, wo.ctypes.data, ro.ctypes.data) == 0:
+            world[k], radec[k] = wo, ro
+    return world, radec
+
+
+def ref_wcs_world(ctype, crval, crpix, cd, pix, longpole=999.0, latpole=999.0,
+                  equinox=2000.0):
+    """The same through the reference's own create_wcs/raw_to_wcs/precess (oracle/_ref);
+    also returns the reference's wcsstruct pointer (kept alive for the process)."""
+    arr = lambda v: np.ascontiguousarray(v, np.float64)
+    cv, cp, m = arr(crval), arr(crpix), arr(cd).ravel()
+    wp = ref().ref_wcs_new(ctype[0].encode(), ctype[1].encode(), cv.ctypes.data,
+                           cp.ctypes.data, m.ctypes.data, longpole, latpole, equinox)
+    pix = arr(pix).reshape(-1, 2)
+    world, radec = np.zeros_like(pix), np.zeros_like(pix)
+    if ref().ref_wcs_world(wp, len(pix), pix.ctypes.data, world.ctypes.data,
+                           radec.ctypes.data) != 0:
+        raise ValueError("the reference found no mapping for a position")
+    return world, radec, wp
 
 
 def getconv(path):

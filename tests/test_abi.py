@@ -121,9 +121,9 @@ REFSRC = "/root/reference/src"
 
 @pytest.mark.skipif(not os.path.isdir(REFSRC), reason="the reference's headers are not here")
 def test_reference_struct_offsets(tmp_path):
-    """endobject_gpu reads the reference front-end's prefstruct and obj2struct at fixed
-    offsets: a program compiled against the reference's own headers (src/prefs.h,
-    src/types.h, NAPER 1) prints the same offsets, sizes and enum values."""
+    """endobject_gpu reads the reference front-end's prefstruct, obj2struct and wcsstruct
+    at fixed offsets: a program compiled against the reference's own headers (src/prefs.h,
+    src/types.h, NAPER 1, NAXIS 2) prints the same offsets, sizes and enum values."""
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "define.h"\n#include "globals.h"\n'
@@ -133,7 +133,10 @@ def test_reference_struct_offsets(tmp_path):
         'O(prefstruct, seeing_fwhm); O(prefstruct, mag_zeropoint); O(prefstruct, mask_type);\n'
         'O(prefstruct, detect_type); printf(" %zu", sizeof(obj2struct));\n'
         'O(obj2struct, flux_aper); O(obj2struct, flux_auto); O(obj2struct, sprob);\n'
-        'printf(" %d %d\\n", (int)PHOTO, (int)MASK_CORRECT); return 0; }\n')
+        'printf(" %d %d", (int)PHOTO, (int)MASK_CORRECT); printf(" %zu", sizeof(wcsstruct));\n'
+        'O(wcsstruct, naxis); O(wcsstruct, ctype); O(wcsstruct, crval); O(wcsstruct, crpix);\n'
+        'O(wcsstruct, cd); O(wcsstruct, longpole); O(wcsstruct, latpole); O(wcsstruct, lat);\n'
+        'O(wcsstruct, lng); O(wcsstruct, equinox); printf("\\n"); return 0; }\n')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-DHAVE_CONFIG_H", "-I", os.path.dirname(REFSRC), "-I", REFSRC,
                     "-I", os.path.join(REFSRC, "fits"), "-I", os.path.join(REFSRC, "wcs"), "-w",

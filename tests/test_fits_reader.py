@@ -112,3 +112,31 @@ def test_reference_test_frame():
     got = sexgpu.read_fits(path)
     assert np.array_equal(got.view(np.int32), want.view(np.int32))
     assert 3700.0 < float(np.median(got)) < 10000.0 and float(got.max()) < 98529.0
+
+
+def test_header_wcs_keywords(tmp_path):
+    """The WCS keywords as read_wcs reads them (fitswcs.c:320-470): the CD matrix from
+    CDi_j; else PC00i00j times CDELTi; else CDELTi with CROTA2 (cd = R(CROTA2) diag(CDELT));
+    EQUINOX, else EPOCH; CRPIX defaults to 1, LONPOLE/LATPOLE to 999."""
+    import math
+    data = np.zeros((4, 5))
+    p = str(tmp_path / "w.fits")
+    base = dict(CTYPE1="'RA---TAN'", CTYPE2="'DEC--TAN'", CRVAL1=12.5, CRVAL2=-33.0)
+    _write(p, data, -32, CD1_1=-1e-4, CD1_2=2e-5, CD2_1=3e-5, CD2_2=1.5e-4, EQUINOX=1950.0,
+           LONPOLE=170.0, **base)
+    w = sexgpu.read_fits_wcs(p)
+    assert w["ctype"] == ("RA---TAN", "DEC--TAN") and w["crval"] == [12.5, -33.0]
+    assert w["cd"] == [-1e-4, 2e-5, 3e-5, 1.5e-4] and w["crpix"] == [1.0, 1.0]
+    assert (w["equinox"], w["longpole"], w["latpole"]) == (1950.0, 170.0, 999.0)
+    _write(p, data, -32, CDELT1=-2e-4, CDELT2=3e-4, PC001001=0.8, PC001002=-0.6,
+           PC002001=0.6, PC002002=0.8, EPOCH=1985.0, CRPIX1=7.0, **base)
+    w = sexgpu.read_fits_wcs(p)
+    assert np.allclose(w["cd"], [-2e-4 * 0.8, -2e-4 * -0.6, 3e-4 * 0.6, 3e-4 * 0.8], rtol=1e-15)
+    assert w["equinox"] == 1985.0 and w["crpix"][0] == 7.0
+    _write(p, data, -32, CDELT1=-2e-4, CDELT2=3e-4, CROTA2=30.0, **base)
+    w = sexgpu.read_fits_wcs(p)
+    c, s = math.cos(math.radians(30.0)), math.sin(math.radians(30.0))
+    assert np.allclose(w["cd"], [c * -2e-4, -s * 3e-4, s * -2e-4, c * 3e-4], rtol=1e-15)
+    assert w["equinox"] == 2000.0
+    _write(p, data, -32)
+    assert sexgpu.read_fits_wcs(p) is None

@@ -35,6 +35,7 @@ struct AnaArgs {
     WeightArgs wa;          /* MAP_WEIGHT: per-pixel variance and thresholds */
     const float *rawt0;     /* detection threshold of each raw object */
     const int *order;       /* objects in launch order: the large ones first */
+    const PlanDev *pd;      /* planned run: nobj, backsig, thresh and dthresh on the device */
 };
 
 /* objects of at least ANA_BIG pixels are analysed first: their sequential sums are the
@@ -42,9 +43,10 @@ struct AnaArgs {
 #define ANA_BIG 256
 
 __global__ void __launch_bounds__(256) k_obj_order(const int *objoff, int nobj, int *cnt,
-                                                   int *order)
+                                                   int *order, const PlanDev *pd)
 {
     const int obj = blockIdx.x * 256 + threadIdx.x;
+    if (pd) nobj = pd->nobj;
     if (obj >= nobj) return;
     if (objoff[obj + 1] - objoff[obj] >= ANA_BIG)
         order[atomicAdd(&cnt[0], 1)] = obj;
@@ -82,8 +84,12 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     __shared__ float s_frows[4][SXW_ROW];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int slot = blockIdx.x * 4 + wid;
-    if (slot >= A.nobj) return;
+    if (slot >= (A.pd ? A.pd->nobj : A.nobj)) return;
     const int obj = A.order[slot];
+    /* the frame's noise and thresholds (from the device in a planned run) */
+    const float backsig = A.pd ? A.pd->backsig : A.backsig;
+    const float athresh = A.pd ? A.pd->thresh : A.thresh;
+    const float adthresh = A.pd ? A.pd->dthresh : A.dthresh;
     double *rows = s_rows[wid];
     float *frows = s_frows[wid];
     const int W = A.W, H = A.H;
@@ -94,7 +100,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     /* the object's detection threshold: with per-pixel thresholds, the lowest among the
        pixels of its raw object */
     const bool perpix = A.wa.w && A.wa.rel;
-    const float odth = perpix ? A.rawt0[o.root[obj]] : A.dthresh;
+    const float odth = perpix ? A.rawt0[o.root[obj]] : adthresh;
     double mxo, myo;
     float fdflux, oa, ob;
     if (o.level[obj] > 0) {
@@ -193,7 +199,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     const float bkg = back_at(A.back, (int)(mxo + 0.5), (int)(myo + 0.5), A.nbx, A.nby, A.backw,
                               A.backh);
     const float dbkg = 0.0f;
-    const double var = (double)(A.backsig * A.backsig), dbacknoise2 = var;
+    const double var = (double)(backsig * backsig), dbacknoise2 = var;
     const double xm = A.errflag ? mxo : 0.0, ym = A.errflag ? myo : 0.0;
     float pk = (float)-SXG_BIG, cdpeak = (float)-SXG_BIG, othr = (float)SXG_BIG;
     int area = 0, peakp = 0x7fffffff;
@@ -211,7 +217,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
             peakp = p;
         }
         const float varp = A.wa.w ? sxg_var(A.wa, p) : 0.0f;
-        const float athr = perpix ? sxg_pix_thresh(A.wa.kana, varp) : A.thresh;
+        const float athr = perpix ? sxg_pix_thresh(A.wa.kana, varp) : athresh;
         othr = fminf(othr, athr);
         if (pix > athr) area++;
         if (cleanflag) top.add(cdpix - odth);
@@ -346,6 +352,11 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     o.flag[obj] = flag;
 }
 
+__global__ void k_ana_zero2(int *p)
+{
+    if (threadIdx.x < 2) p[threadIdx.x] = 0;
+}
+
 extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int flagobj_iso_0,
                             float flagobj_fwhm, float backsig, double gain, double ngamma,
                             float thresh, float dthresh, double satur_level, int plistexist_var,
@@ -399,10 +410,13 @@ extern "C" void analyse_gpu(double flagobj_poserr_mx2, int flagobj_peakx, int fl
     int *order = sxg_take<int>("ana.order", (size_t)numobj + 2);
     if (!order) return;
     A.order = order;
+    /* a planned run: numobj is a capacity, the count and thresholds are on the device */
+    A.pd = S.plan ? S.pdev : nullptr;
     sxg_stage_begin(ST_ANALYSE);
-    SXG_HIP(hipMemsetAsync(order + numobj, 0, 2 * sizeof(int), S.stream));
+    /* the two counters of k_obj_order (a fill kernel: no memset node in a graph) */
+    hipLaunchKernelGGL(k_ana_zero2, dim3(1), dim3(64), 0, S.stream, order + numobj);
     hipLaunchKernelGGL(k_obj_order, dim3(sxg_div_up(numobj, 256)), dim3(256), 0, S.stream,
-                       S.objoff, numobj, order + numobj, order);
+                       S.objoff, numobj, order + numobj, order, A.pd);
     sxg_kt_begin(SXG_KT_ANALYSE);
     hipLaunchKernelGGL(k_analyse, dim3(sxg_div_up(numobj, 4)), dim3(256), 0, S.stream, A);
     sxg_kt_end(SXG_KT_ANALYSE);

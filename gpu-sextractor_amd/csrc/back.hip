@@ -1190,11 +1190,15 @@ void sxg_back_estimate_from(const float *src, int back_size, float backfthresh, 
     if (!sxg_back_run(g, b, src, img, wa.mask, backfthresh, back_type, S.back_value, true,
                       S.stream))
         return;
-    float hv[2];
-    SXG_HIP(hipMemcpyAsync(hv, b.outv, sizeof(hv), hipMemcpyDeviceToHost, S.stream));
-    SXG_HIP(hipStreamSynchronize(S.stream));
-    S.backmean = hv[0];
-    S.backsig = hv[1];
+    if (!S.plan) {
+        float hv[2];
+        SXG_HIP(hipMemcpyAsync(hv, b.outv, sizeof(hv), hipMemcpyDeviceToHost, S.stream));
+        SXG_HIP(hipStreamSynchronize(S.stream));
+        S.backmean = hv[0];
+        S.backsig = hv[1];
+    }
+    /* a planned run leaves the medians on the device (b.outv): the thresholds are
+       derived from them there */
     if (S.weighted && S.wtype == SXG_WEIGHT_BACKGROUND) {
         /* WEIGHT_TYPE BACKGROUND: the variance map of this frame is the square of the
            interpolated background rms */

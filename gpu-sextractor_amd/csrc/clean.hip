@@ -26,10 +26,11 @@
 #define CLEAN_ZONE 10.0
 
 __global__ void __launch_bounds__(256) k_clean_prep(ObjCols o, int nobj, double clean_param,
-                                                    double *amp, double *alpha, unsigned *master)
+                                                    double *amp, double *alpha, unsigned *master,
+                                                    const int *nd)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nobj) return;
+    if (i >= sxg_dn(nd, nobj)) return;
     const float unitarea = (float)(SXG_PI * o.a[i] * o.b[i]);
     const double ampin = amp[i] = o.fdflux[i] / (2 * unitarea * o.abcor[i]);
     alpha[i] = (sx_pow(ampin / o.dthresh[i], 1.0 / clean_param) - 1) * unitarea / o.fdnpix[i];
@@ -42,12 +43,14 @@ __global__ void __launch_bounds__(256) k_clean_prep(ObjCols o, int nobj, double 
 __global__ void __launch_bounds__(CP_T) k_clean_pairs(ObjCols o, int nobj, int win,
                                                       const double *__restrict__ amp,
                                                       const double *__restrict__ alpha,
-                                                      unsigned *master)
+                                                      unsigned *master, const int *nd)
 {
     __shared__ double smx[CP_T], smy[CP_T], samp[CP_T], salpha[CP_T];
     __shared__ float sa[CP_T], sf[CP_T], scxx[CP_T], scyy[CP_T], scxy[CP_T], smt[CP_T];
     const int t = threadIdx.x;
     const int i0 = blockIdx.x * CP_T;
+    nobj = sxg_dn(nd, nobj);
+    if (i0 >= nobj) return;
     const int i = i0 + t;
     const bool live = i < nobj;
     const int ii = live ? i : nobj - 1;
@@ -112,13 +115,14 @@ extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacks
     }
     ObjCols o;
     if (!sxg_obj_view(o)) return;
-    const int n = numobj;
+    const int n = numobj;   /* a capacity in a planned run (the count is on the device) */
+    const int *nd = S.plan ? &S.pdev->nobj : nullptr;
     double *amp = sxg_take<double>("clean.amp", n), *alpha = sxg_take<double>("clean.alpha", n);
     if (!amp || !alpha) return;
     sxg_stage_begin(ST_CLEAN);
     if (n > 0) {
         hipLaunchKernelGGL(k_clean_prep, dim3(sxg_div_up(n, 256)), dim3(256), 0, S.stream, o, n,
-                           clean_param, amp, alpha, o.master);
+                           clean_param, amp, alpha, o.master, nd);
         /* CLEAN N: keep everything (the reference cleans regardless of the flag) */
         if (S.clean_enabled) {
             const int win = ((clean_stacksize - 1) / 16 + 1) * 16;
@@ -127,7 +131,7 @@ extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacks
             const int ntile = (win + CP_T - 1) / CP_T + 1;
             const int slices = std::max(1, std::min(8, ntile));
             hipLaunchKernelGGL(k_clean_pairs, dim3(sxg_div_up(n, 256), slices), dim3(256), 0, S.stream, o,
-                               n, win, amp, alpha, o.master);
+                               n, win, amp, alpha, o.master, nd);
             sxg_kt_end(SXG_KT_CLEAN);
         }
     }
@@ -148,10 +152,11 @@ extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacks
 }
 
 /* ------------------------------------------------------------- segmentation ---- */
-__global__ void __launch_bounds__(256) k_keepflag(const unsigned *master, int n, int *keep)
+__global__ void __launch_bounds__(256) k_keepflag(const unsigned *master, int n, int *keep,
+                                                  const int *nd)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) keep[i] = master[i] == 0xffffffffu ? 1 : 0;
+    if (i < sxg_dn(nd, n)) keep[i] = master[i] == 0xffffffffu ? 1 : 0;
 }
 
 /* a[0..n) = v, four per lane with 16-byte stores (a is 16-byte aligned) */
@@ -167,9 +172,10 @@ __global__ void __launch_bounds__(256) k_fill_i32(int *a, long n, int v)
 /* seg[p] = 1-based rank of the surviving object of p (0 for cleaned objects) */
 __global__ void __launch_bounds__(256) k_seg_scatter(const int *fpia, long P0, const int *alloc,
                                                      const int *keep, const int *rank, int *seg,
-                                                     int mode)
+                                                     int mode, const int *nd)
 {
     const long g = (long)blockIdx.x * 256 + threadIdx.x;
+    if (nd) P0 = *nd;
     if (g >= P0) return;
     const int p = fpia[g];
     const int o = alloc[p];
@@ -177,17 +183,19 @@ __global__ void __launch_bounds__(256) k_seg_scatter(const int *fpia, long P0, c
 }
 
 /* the survivors of CLEAN: keep[i], rank[i] (catalogue NUMBER - 1 of a survivor) and their
-   number in tot[0], all on the device (queued on the context's stream) */
+   number in tot[0] (in a planned run the device's nsurv), all on the device (queued on
+   the context's stream) */
 static bool survivor_ranks(const ObjCols &o, int *&keep, int *&rank, int *&tot)
 {
     SxgState &S = sxg_state();
     keep = sxg_take<int>("out.keep", S.nobj);
     rank = sxg_take<int>("out.rank", S.nobj);
-    tot = sxg_take<int>("out.total", 4);
+    tot = S.plan ? &S.pdev->nsurv : sxg_take<int>("out.total", 4);
     if (!keep || !rank || !tot) return false;
+    const int *nd = S.plan ? &S.pdev->nobj : nullptr;
     hipLaunchKernelGGL(k_keepflag, dim3(sxg_div_up(S.nobj, 256)), dim3(256), 0, S.stream,
-                       o.master, S.nobj, keep);
-    sxg_scan_excl(keep, rank, S.nobj, tot, S.stream);
+                       o.master, S.nobj, keep, nd);
+    sxg_scan_excl(keep, rank, S.nobj, tot, S.stream, nd);
     return true;
 }
 
@@ -202,6 +210,9 @@ static int *build_map(int mode)
                        S.stream, map, N, mode ? -1 : 0);
     if (S.nobj == 0 || S.npix_raw == 0) {
         if (mode == 0) S.nsurv = 0;
+        if (mode == 0 && S.plan)
+            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, S.stream, &S.pdev->nsurv, 1L,
+                               0);
         return map;
     }
     ObjCols o;
@@ -217,9 +228,10 @@ static int *build_map(int mode)
     sxg_kt_begin(SXG_KT_SEG);
     hipLaunchKernelGGL(k_seg_scatter, dim3(sxg_div_up(S.npix_raw, 256)), dim3(256), 0, S.stream,
                        S.fpia, (long)S.npix_raw, sxg_take<int>("deb.alloc", S.N), keep, rank, map,
-                       mode);
+                       mode, (const int *)(S.plan ? &S.pdev->npix : nullptr));
     sxg_kt_end(SXG_KT_SEG);
-    if (mode == 0) {
+    /* a planned run leaves the survivor count on the device */
+    if (mode == 0 && !S.plan) {
         int ns = 0;
         if (hipMemcpyAsync(&ns, tot, sizeof(int), hipMemcpyDeviceToHost, S.stream) == hipSuccess &&
             hipStreamSynchronize(S.stream) == hipSuccess)

@@ -277,9 +277,13 @@ __device__ inline float wave_ordered_sum(const float *v, int start, int n, Sel s
 __global__ void __launch_bounds__(256) k_deb_thresh(const float *__restrict__ fdpeak, int nraw,
                                                     int nthresh, float basethresh,
                                                     const float *__restrict__ rawt0, int perpix,
-                                                    float *thr)
+                                                    float *thr, const PlanDev *pd)
 {
     const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    if (pd) {
+        nraw = pd->nraw;
+        basethresh = pd->dthresh;
+    }
     if (k >= (long)nraw * nthresh) return;
     const int r = (int)(k / nthresh), i = (int)(k - (long)r * nthresh);
     const float b = perpix ? rawt0[r] : basethresh;
@@ -331,6 +335,8 @@ struct DebArgs {
                                level-parallel (0: none) */
     double mincont;
     int W, H, nraw;
+    int cstride;            /* entries per class in clist (nraw, or its capacity) */
+    const PlanDev *pd;      /* planned run: the device counts (nraw, npix), else nullptr */
     Comp *comps;            /* output node table, region compoff[r] of each raw object */
     DebNode *nodes;         /* build table of the objects built in global memory */
     int *compoff;           /* node capacity of each raw object, then its region */
@@ -357,12 +363,22 @@ struct DebArgs {
     int *g_par, *g_deep, *g_cnt;
 };
 
+/* the raw objects and their pixels: the device counts of a planned run */
+__device__ inline int deb_nraw(const DebArgs &a) { return a.pd ? a.pd->nraw : a.nraw; }
+__device__ inline long deb_P0(const DebArgs &a) { return a.pd ? (long)a.pd->npix : a.P0; }
+/* class-3 objects built level-parallel (none when there are more than a grid holds) */
+__device__ inline int deb_nbig(const DebArgs &a)
+{
+    const int c8 = a.ccount[8];
+    return c8 <= 65535 ? c8 : 0;
+}
+
 /* PASS 0: local maxima (no 8-neighbour of the raw object is higher): the highest of each
    raw object and their number; PASS 1: the highest of the others */
 template <int PASS> __global__ void __launch_bounds__(256) k_deb_lmax(DebArgs a)
 {
     const long g = (long)blockIdx.x * 256 + threadIdx.x;
-    if (g >= a.P0) return;
+    if (g >= deb_P0(a)) return;
     const int p = a.fpia[g], r = a.rawkey[g];
     const int root = a.lab[p];
     const float v = a.cd[p];
@@ -393,7 +409,7 @@ __global__ void __launch_bounds__(256) k_deb_prep(DebArgs a)
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r < DEB_NCOUNT) a.ccount[r] = 0;
     if (r < 5) a.dwork[r] = 0ull;
-    if (r >= a.nraw) return;
+    if (r >= deb_nraw(a)) return;
     a.top1[r] = 0ull;
     a.top2[r] = 0u;
     a.nmax[r] = 0;
@@ -407,7 +423,7 @@ __global__ void __launch_bounds__(256) k_deb_lsfirst(DebArgs a)
 {
     const long k = (long)blockIdx.x * 256 + threadIdx.x;
     const int N = a.nthresh;
-    if (k >= (long)a.nraw * N) return;
+    if (k >= (long)deb_nraw(a) * N) return;
     const int r = (int)(k / N), i = (int)(k - (long)r * N);
     const unsigned u2 = a.top2[r];
     if (i == 0 || u2 == 0u) return;
@@ -432,7 +448,7 @@ __global__ void __launch_bounds__(256) k_deb_nodecap(DebArgs a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
     unsigned long long c = 0;
-    if (r < a.nraw) {
+    if (r < deb_nraw(a)) {
         const int ls = a.top2[r] == 0u ? 0 : a.lstar[r] - 1;
         a.lstar[r] = ls;
         const int cap = deb_nodecap(a.nmax[r], a.cnt[r], ls, a.debarea);
@@ -454,7 +470,7 @@ __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int c
     const int r = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
     int cls = -1, n = 0, ls = 0, need = 0;
-    if (r < a.nraw) {
+    if (r < deb_nraw(a)) {
         n = a.cnt[r];
         ls = a.lstar[r];
         if (ls == 0) {
@@ -502,7 +518,7 @@ __global__ void __launch_bounds__(256) k_deb_classify(DebArgs a, int cap0, int c
         }
         if (lane == lead && c < 3) atomicMax(&a.ccount[4 + c], mx);
         if (lane == lead) atomicAdd(&a.ccount[10 + c], px);
-        if (cls == c) a.clist[(size_t)c * a.nraw + base + __popcll(b & sxw_lanes_below(lane))] = r;
+        if (cls == c) a.clist[(size_t)c * a.cstride + base + __popcll(b & sxw_lanes_below(lane))] = r;
     }
 }
 
@@ -1177,7 +1193,9 @@ template <int NT>
 __global__ void __launch_bounds__(NT) k_deb_tree_lds(DebArgs a, int cls)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int r = a.clist[(size_t)cls * a.nraw + blockIdx.x];
+    /* a planned run's grid is the class's capacity */
+    if (a.pd && (int)blockIdx.x >= a.ccount[cls]) return;
+    const int r = a.clist[(size_t)cls * a.cstride + blockIdx.x];
     DebNode *tl = reinterpret_cast<DebNode *>(smem);
     const int ct = deb_nodecap(a.nmax[r], a.cnt[r], a.lstar[r], a.debarea);
     const int cap = (a.cnt[r] + 63) & ~63;
@@ -1195,7 +1213,13 @@ __global__ void __launch_bounds__(NT) k_deb_tree_lds(DebArgs a, int cls)
    memory */
 __global__ void __launch_bounds__(1024) k_deb_tree_glb(DebArgs a, int j0)
 {
-    const int r = a.clist[3 * (size_t)a.nraw + j0 + blockIdx.x];
+    if (a.pd) {
+        /* planned run: the level-parallel objects come first, the grid is the class's
+           capacity */
+        j0 = deb_nbig(a);
+        if ((int)blockIdx.x >= a.ccount[3] - j0) return;
+    }
+    const int r = a.clist[3 * (size_t)a.cstride + j0 + blockIdx.x];
     const int off = a.off[r];
     deb_tree<1024, true, int>(a, r, a.g_cdv + off, a.g_m + off,
                               reinterpret_cast<NbP<int> *>(a.g_nb) + off, a.g_par + off,
@@ -1208,13 +1232,21 @@ __global__ void __launch_bounds__(1024) k_deb_tree_glb(DebArgs a, int j0)
    otherwise those of at least lvmin pixels (the level-parallel ones, nbig of them) in
    that order, then the others in any order from the counter *ctr (zero on entry),
    O(nbig * nc).  Either way the level-parallel objects come first.  The results do
-   not depend on the order. */
+   not depend on the order.  cc (planned runs): nc, nbig and full from the class counts
+   on the device (full when nc <= fullmax). */
 __global__ void __launch_bounds__(256) k_deb_order(const int *__restrict__ clist,
                                                    const int *__restrict__ cnt, int nraw,
                                                    int nc, int lvmin, int nbig, int full,
-                                                   int *ctr, int *__restrict__ out)
+                                                   int *ctr, int *__restrict__ out,
+                                                   const int *cc, int fullmax)
 {
     __shared__ int sn[256];
+    if (cc) {
+        nc = cc[3];
+        nbig = cc[8] <= 65535 ? cc[8] : 0;
+        if (nbig == 0) lvmin = 0;
+        full = nc <= fullmax ? 1 : 0;
+    }
     const int *L = clist + 3 * (size_t)nraw;
     int *O = out + 3 * (size_t)nraw;
     const int j = blockIdx.x * 256 + threadIdx.x;
@@ -1284,7 +1316,8 @@ __device__ inline float block_serial_sum(const float *v, int n, float (*buf)[409
    tree is the one the growing union-find gives. */
 __global__ void __launch_bounds__(1024) k_deb_glb_prep(DebArgs a, double *v0)
 {
-    const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
+    if (a.pd && (int)blockIdx.x >= deb_nbig(a)) return;
+    const int r = a.clist[3 * (size_t)a.cstride + blockIdx.x];
     const int off = a.off[r];
     LvArgs lv{};
     lv.v0 = v0 + 2 * blockIdx.x;
@@ -1303,7 +1336,8 @@ __global__ void __launch_bounds__(1024) k_deb_glb_levels(DebArgs a, int *lvpar, 
     /* blockIdx.y: the object (largest first); blockIdx.x: 0 computes the object's float
        flux (for the significance tests the bounds leave open), x >= 1 builds level x */
     const int oj = blockIdx.y;
-    const int r = a.clist[3 * (size_t)a.nraw + oj];
+    if (a.pd && oj >= deb_nbig(a)) return;
+    const int r = a.clist[3 * (size_t)a.cstride + oj];
     const int i = blockIdx.x;
     const int n = a.cnt[r], off = a.off[r], tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const float *cdv = a.g_cdv + off;
@@ -1491,7 +1525,8 @@ __global__ void __launch_bounds__(1024) k_deb_glb_tree(DebArgs a, const int *lvn
                                                       const LvNode *lvrec, const int *lvk,
                                                       const float *frbuf)
 {
-    const int r = a.clist[3 * (size_t)a.nraw + blockIdx.x];
+    if (a.pd && (int)blockIdx.x >= deb_nbig(a)) return;
+    const int r = a.clist[3 * (size_t)a.cstride + blockIdx.x];
     const int off = a.off[r], n = a.cnt[r];
     LvArgs lv{};
     lv.nid = lvnid + a.lvbase[2 * r];
@@ -1522,14 +1557,18 @@ struct ObjArgs {
    its raw object's list) */
 #define PRE_BIG 256
 
+/* the objects: the device count of a planned run */
+__device__ inline int obj_n(const ObjArgs &A) { return A.d.pd ? A.d.pd->nobj : A.nobj; }
+
 __global__ void __launch_bounds__(256) k_pre_order(ObjArgs A, int *cnt, int *order)
 {
     const int obj = blockIdx.x * 256 + threadIdx.x;
-    if (obj >= A.nobj) return;
+    const int nobj = obj_n(A);
+    if (obj >= nobj) return;
     if (A.d.cnt[A.objraw[obj]] >= PRE_BIG)
         order[atomicAdd(&cnt[0], 1)] = obj;
     else
-        order[A.nobj - 1 - atomicAdd(&cnt[1], 1)] = obj;
+        order[nobj - 1 - atomicAdd(&cnt[1], 1)] = obj;
 }
 
 /* objects of each raw object, in component-table order (one wavefront per raw object) */
@@ -1537,7 +1576,7 @@ __global__ void __launch_bounds__(256) k_deb_objlist(ObjArgs A)
 {
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= A.d.nraw) return;
+    if (r >= deb_nraw(A.d)) return;
     const int nc = A.d.rootnc[r], base = A.objbase[r];
     Comp *C = A.d.comps + A.d.compoff[r];
     if (A.d.lstar[r] == 0) {
@@ -1571,7 +1610,7 @@ __global__ void __launch_bounds__(256) k_deb_objlist(ObjArgs A)
 __global__ void __launch_bounds__(256) k_deb_allocinit(ObjArgs A)
 {
     const long g = (long)blockIdx.x * 256 + threadIdx.x;
-    if (g < A.d.P0) A.alloc[A.d.fpia[g]] = -1;
+    if (g < deb_P0(A.d)) A.alloc[A.d.fpia[g]] = -1;
 }
 
 /*
@@ -1584,7 +1623,7 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
     __shared__ float s_frows[4][SXW_ROW];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int slot = blockIdx.x * 4 + wid;
-    if (slot >= A.nobj) return;
+    if (slot >= obj_n(A)) return;
     const int obj = A.order[slot];
     double *rows = s_rows[wid];
     float *frows = s_frows[wid];
@@ -1688,7 +1727,7 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
 __global__ void __launch_bounds__(256) k_deb_gather(ObjArgs A)
 {
     const long g = (long)blockIdx.x * 256 + threadIdx.x;
-    if (g >= A.d.P0) return;
+    if (g >= deb_P0(A.d)) return;
     const int p = A.d.fpia[g];
     int a = A.alloc[p];
     if (a < 0) {
@@ -1720,7 +1759,7 @@ __global__ void __launch_bounds__(256) k_deb_objcount(ObjArgs A, int *objcnt)
 {
     const long g = (long)blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    bool act = g < A.d.P0;
+    bool act = g < deb_P0(A.d);
     const int key = act ? A.okey[g] : -1;
     unsigned long long pending = __ballot(act);
     while (pending) {
@@ -1737,6 +1776,36 @@ __global__ void __launch_bounds__(256) k_deb_zero(int *v, long n)
 {
     const long k = (long)blockIdx.x * 256 + threadIdx.x;
     if (k < n) v[k] = 0;
+}
+
+/* planned runs: the node total and the level-parallel arrays against their capacities
+   (the 64-bit node sum too: a total past 2^31 is an overflow); on any overflow of the
+   frame the class counts are zeroed so that no tree kernel has work */
+__global__ void k_deb_plan_guard(const unsigned long long *dw, PlanDev *pd, int *cc,
+                                 const int *ntot, long long cap_tot, long long cap_npl,
+                                 long long cap_nrec)
+{
+    if (threadIdx.x != 0) return;
+    if (dw[2] > (unsigned long long)cap_tot || (long long)*ntot > cap_tot || *ntot < 0)
+        pd->ovf |= 1 << PL_NTOT;
+    if (dw[3] > (unsigned long long)cap_npl) pd->ovf |= 1 << PL_NPL;
+    if (dw[4] > (unsigned long long)cap_nrec) pd->ovf |= 1 << PL_NREC;
+    if (pd->ovf) {
+        pd->nfg = pd->nraw = pd->npix = pd->nobj = pd->nsurv = 0;
+        for (int c = 0; c < 4; c++) cc[c] = 0;
+        cc[8] = 0;
+    }
+}
+
+/* planned runs: the frame's deblending work into its device block, and the level check
+   (the eager run's "inconsistent deblending levels") as an overflow bit */
+__global__ void k_deb_plan_info(const unsigned long long *dw, const int *cc, PlanDev *pd)
+{
+    if (threadIdx.x != 0) return;
+    pd->deb_pixels = (long long)dw[0];
+    pd->deb_labels = (long long)dw[1];
+    for (int c = 0; c < 4; c++) pd->class_pixels[c] = cc[10 + c];
+    if (cc[7] != 0) pd->ovf |= 1 << SXG_PLAN_DEBLEVELS;
 }
 
 /* --------------------------------------------------------------- host side ---- */
@@ -1766,7 +1835,8 @@ extern "C" void init_deblend(float basethresh, int nthresh, size_t numValidObj, 
         sxg_kt_begin(SXG_KT_DEB_THRESH);
         hipLaunchKernelGGL(k_deb_thresh, dim3(sxg_div_up(tot, 256)), dim3(256), 0, S.stream,
                            ro.fdpeak, S.nraw, nthresh, basethresh, ro.t0,
-                           (int)(S.weighted && S.wargs.rel), thr);
+                           (int)(S.weighted && S.wargs.rel), thr,
+                           (const PlanDev *)(S.plan ? S.pdev : nullptr));
         sxg_kt_end(SXG_KT_DEB_THRESH);
     }
     SXG_HIP(hipGetLastError());
@@ -1787,6 +1857,9 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     }
     (void)basethresh; (void)numValidObj; (void)numValidPix;
     hipStream_t st = S.stream;
+    /* a planned run: S.nraw and S.npix_raw are capacities, the counts are on the device */
+    const FramePlan *pl = S.plan;
+    PlanDev *pd = pl ? S.pdev : nullptr;
     const int nraw = S.nraw;
     const long P0 = S.npix_raw;
     sxg_stage_begin(ST_DEBLEND);
@@ -1818,6 +1891,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     a.nthresh = n_thresh; a.debarea = deb_minarea; a.mincont = deblend_mincont;
     a.lvmin = (int)std::min<long>(sxg_option(SXG_OPT_DEB_GLB_LEVELS), 0x7fffffffL);
     a.W = S.W; a.H = S.H; a.nraw = nraw; a.compoff = compoff;
+    a.cstride = nraw; a.pd = pd;
     a.rootnc = rootnc; a.rootkept = rootkept;
     a.top1 = top1; a.top2 = top2; a.nmax = nmax; a.lstar = lstar; a.clist = clist;
     a.ccount = ccount; a.dwork = dwork; a.lvbase = lvbase;
@@ -1846,24 +1920,50 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         hipLaunchKernelGGL(k_deb_lsfirst, dim3(sxg_div_up((long)nraw * n_thresh, 256)), dim3(256),
                            0, st, a);
         hipLaunchKernelGGL(k_deb_nodecap, dim3(gr), dim3(256), 0, st, a);
-        sxg_scan_excl(compoff, compoff, nraw, dtot, st);
+        sxg_scan_excl(compoff, compoff, nraw, dtot, st, pd ? &pd->nraw : nullptr);
         hipLaunchKernelGGL(k_deb_classify, dim3(gr), dim3(256), 0, st, a, cap0, cap1, cap2, ct0,
                            ct1, ct2);
         sxg_kt_end(SXG_KT_DEB_LMAX);
         int cc[DEB_NCOUNT] = {};
         int ntot = 0;
         unsigned long long dw[5] = {0, 0, 0, 0, 0};
-        SXG_HIP_RET(hipMemcpyAsync(cc, ccount, sizeof(cc), hipMemcpyDeviceToHost, st), -1);
-        SXG_HIP_RET(hipMemcpyAsync(&ntot, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
-        SXG_HIP_RET(hipMemcpyAsync(dw, dwork, sizeof(dw), hipMemcpyDeviceToHost, st), -1);
-        SXG_HIP_RET(hipStreamSynchronize(st), -1);
-        if (dw[2] > (unsigned long long)0x7fffffff) {
-            sxg_set_error("parcel_out: deblending trees exceed 2^31 nodes in one frame");
-            return -1;
+        if (pl) {
+            /* the capacities stand for the counts; the guards check the device counts
+               against them (the node total first, then every class) */
+            ntot = (int)pl->cap[PL_NTOT];
+            for (int c = 0; c < 4; c++) cc[c] = (int)pl->cap[PL_C0 + c];
+            /* the class's LDS: its budget, or less when the plan asks less */
+            const int ctc[3] = {ct0, ct1, ct2};
+            for (int c = 0; c < 3; c++)
+                cc[4 + c] = (int)std::min<int64_t>(pl->cap[PL_LDS0 + c], ctc[c]);
+            cc[8] = (int)pl->cap[PL_BIG];
+            dw[3] = (unsigned long long)pl->cap[PL_NPL];
+            dw[4] = (unsigned long long)pl->cap[PL_NREC];
+            hipLaunchKernelGGL(k_deb_plan_guard, dim3(1), dim3(64), 0, st, dwork, pd, ccount,
+                               dtot, (long long)pl->cap[PL_NTOT], (long long)pl->cap[PL_NPL],
+                               (long long)pl->cap[PL_NREC]);
+            for (int c = 0; c < 3; c++) sxg_plan_guard(ccount + 4 + c, cc[4 + c], PL_LDS0 + c, ccount);
+            sxg_plan_guard(ccount + 8, cc[8], PL_BIG, ccount);
+            for (int c = 0; c < 4; c++) sxg_plan_guard(ccount + c, cc[c], PL_C0 + c, ccount);
+        } else {
+            SXG_HIP_RET(hipMemcpyAsync(cc, ccount, sizeof(cc), hipMemcpyDeviceToHost, st), -1);
+            SXG_HIP_RET(hipMemcpyAsync(&ntot, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
+            SXG_HIP_RET(hipMemcpyAsync(dw, dwork, sizeof(dw), hipMemcpyDeviceToHost, st), -1);
+            SXG_HIP_RET(hipStreamSynchronize(st), -1);
+            if (dw[2] > (unsigned long long)0x7fffffff) {
+                sxg_set_error("parcel_out: deblending trees exceed 2^31 nodes in one frame");
+                return -1;
+            }
+            S.deb_pixels = (int64_t)dw[0];
+            for (int c = 0; c < 4; c++) S.deb_class_pixels[c] = cc[10 + c];
+            S.deb_labels = (int64_t)dw[1];
+            S.seen.cap[PL_NTOT] = ntot;
+            for (int c = 0; c < 4; c++) S.seen.cap[PL_C0 + c] = cc[c];
+            for (int c = 0; c < 3; c++) S.seen.cap[PL_LDS0 + c] = cc[4 + c];
+            S.seen.cap[PL_BIG] = cc[8];
+            S.seen.cap[PL_NPL] = (int64_t)dw[3];
+            S.seen.cap[PL_NREC] = (int64_t)dw[4];
         }
-        S.deb_pixels = (int64_t)dw[0];
-        for (int c = 0; c < 4; c++) S.deb_class_pixels[c] = cc[10 + c];
-        S.deb_labels = (int64_t)dw[1];
         a.comps = sxg_take<Comp>("deb.comps", (size_t)ntot + 1);
         if (!a.comps) return -1;
         sxg_kt_begin(SXG_KT_DEB_LEVELS);
@@ -1872,7 +1972,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         /* the classes run side by side: each kernel lasts as long as its slowest tree.
            Everything they read was finished by the synchronisation above. */
         int nside = 0;
-        const bool use_side = !S.single_stream && sxg_option(SXG_OPT_SIDE_STREAMS) != 0;
+        const bool use_side = !S.single_stream && !pl && sxg_option(SXG_OPT_SIDE_STREAMS) != 0;
         /* the dynamic-LDS limit is an attribute of the function, shared by every context:
            it is raised once, to the most any class asks for (DEB_LDS_LIMIT), never per
            launch (batch workers launch these kernels from several host threads) */
@@ -1923,12 +2023,14 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             DebArgs a3 = a;
             a3.clist = clist3;
             hipStream_t ks = use_side ? S.side[nside] : st;
-            const int big = cc[8] <= 65535 ? cc[8] : 0, rest = cc[3] - big;
+            /* planned: the capacities bound both parts (the split is on the device) */
+            const int big = cc[8] <= 65535 ? cc[8] : 0, rest = pl ? cc[3] : cc[3] - big;
+            const long fullmax = sxg_option(SXG_OPT_DEB_ORDER_FULL);
             hipLaunchKernelGGL(k_deb_order, dim3(sxg_div_up(cc[3], 256)), dim3(256), 0, ks,
                                (const int *)a.clist, (const int *)a.cnt, nraw, cc[3],
-                               big > 0 ? a.lvmin : 0, big,
-                               cc[3] <= sxg_option(SXG_OPT_DEB_ORDER_FULL) ? 1 : 0, ccount + 9,
-                               clist3);
+                               big > 0 ? a.lvmin : 0, big, cc[3] <= fullmax ? 1 : 0, ccount + 9,
+                               clist3, (const int *)(pl ? ccount : nullptr),
+                               (int)std::min<long>(fullmax, 0x7fffffffL));
             if (rest > 0) {
                 hipStream_t ks2 = ks;
                 if (use_side) {
@@ -1976,14 +2078,29 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         for (int j = 0; j < nside; j++) SXG_HIP_RET(hipStreamWaitEvent(st, S.side_ev[j], 0), -1);
         sxg_kt_end(SXG_KT_DEB_LEVELS);
         SXG_HIP_RET(hipGetLastError(), -1);
-        sxg_scan_excl(rootkept, objbase, nraw, dtot, st);
-        SXG_HIP_RET(hipMemcpyAsync(&nobj, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
-        SXG_HIP_RET(hipMemcpyAsync(cc, ccount + 7, sizeof(int), hipMemcpyDeviceToHost, st), -1);
-        SXG_HIP_RET(hipStreamSynchronize(st), -1);
-        if (cc[0] != 0) {
-            sxg_set_error("parcel_out: inconsistent deblending levels (internal error)");
-            return -1;
+        if (pl) {
+            sxg_scan_excl(rootkept, objbase, nraw, &pd->nobj, st, &pd->nraw);
+            nobj = (int)pl->cap[PL_NOBJ];
+            sxg_plan_guard(&pd->nobj, nobj, PL_NOBJ);
+            /* the frame's deblending work and level check, for its record */
+            hipLaunchKernelGGL(k_deb_plan_info, dim3(1), dim3(64), 0, st,
+                               (const unsigned long long *)dwork, (const int *)ccount, pd);
+        } else {
+            sxg_scan_excl(rootkept, objbase, nraw, dtot, st);
+            SXG_HIP_RET(hipMemcpyAsync(&nobj, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);
+            SXG_HIP_RET(hipMemcpyAsync(cc, ccount + 7, sizeof(int), hipMemcpyDeviceToHost, st),
+                        -1);
+            SXG_HIP_RET(hipStreamSynchronize(st), -1);
+            if (cc[0] != 0) {
+                sxg_set_error("parcel_out: inconsistent deblending levels (internal error)");
+                return -1;
+            }
+            S.seen.cap[PL_NOBJ] = nobj;
         }
+    } else if (pl) {
+        hipLaunchKernelGGL(k_deb_zero, dim3(1), dim3(256), 0, st, &pd->nobj, 1L);
+    } else {
+        for (int k = PL_NTOT; k <= PL_NOBJ; k++) S.seen.cap[k] = 0;
     }
     S.nobj = nobj;
     S.opix = nullptr;
@@ -2011,7 +2128,8 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         hipLaunchKernelGGL(k_deb_allocinit, dim3(gp), dim3(256), 0, st, A);
         int *order = sxg_take<int>("pre.order", (size_t)nobj + 2);
         if (!order) return -1;
-        SXG_HIP_RET(hipMemsetAsync(order + nobj, 0, 2 * sizeof(int), st), -1);
+        /* the two counters of k_pre_order (a fill kernel: no memset node in a graph) */
+        hipLaunchKernelGGL(k_deb_zero, dim3(1), dim3(256), 0, st, order + nobj, 2L);
         hipLaunchKernelGGL(k_pre_order, dim3(sxg_div_up(nobj, 256)), dim3(256), 0, st, A,
                            order + nobj, order);
         A.order = order;
@@ -2020,13 +2138,19 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         sxg_kt_end(SXG_KT_DEB_OBJECTS);
         /* raw pixels regrouped by object (stable: raster order inside each object) */
         sxg_kt_begin(SXG_KT_OBJ_SORT);
-        hipLaunchKernelGGL(k_deb_zero, dim3(sxg_div_up(nobj, 256)), dim3(256), 0, st, objcnt,
-                           (long)nobj);
+        /* planned: the counts past the device's nobj stay zero, so the scan over the
+           capacity puts the total at objoff[nobj] as well */
+        const long nz = pl ? (long)nobj + 1 : (long)nobj;
+        hipLaunchKernelGGL(k_deb_zero, dim3(sxg_div_up(nz, 256)), dim3(256), 0, st, objcnt, nz);
         hipLaunchKernelGGL(k_deb_objcount, dim3(gp), dim3(256), 0, st, A, objcnt);
-        sxg_scan_excl(objcnt, objoff, nobj, objoff + nobj, st);
+        if (pl)
+            sxg_scan_excl(objcnt, objoff, nobj + 1, nullptr, st);
+        else
+            sxg_scan_excl(objcnt, objoff, nobj, objoff + nobj, st);
         int nbits = 1;
         while (nbits < 31 && ((nobj - 1) >> nbits) != 0) nbits++;
-        const int where = sxg_radix_sort(k0, v0, k1, v1, (int)P0, nbits, hist, st);
+        const int where = sxg_radix_sort(k0, v0, k1, v1, (int)P0, nbits, hist, st,
+                                         pl ? &pd->npix : nullptr);
         sxg_kt_end(SXG_KT_OBJ_SORT);
         S.opix = where ? v1 : v0;
         S.objoff = objoff;

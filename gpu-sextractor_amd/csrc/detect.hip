@@ -36,8 +36,10 @@ __device__ inline float det_thresh(const WeightArgs &wa, float t, long p)
    first pixel of its run (the detected pixels before it on its row, inside its aligned
    64-pixel window: a ballot), so rows never become chains */
 __global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, float t, long n,
-                                                  int W, int *par, int *lab, WeightArgs wa)
+                                                  int W, int *par, int *lab, WeightArgs wa,
+                                                  const float *td)
 {
+    if (td) t = *td;
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool live = p < n;
@@ -94,10 +96,10 @@ __device__ inline void uf_unite(int *par, int a, int b)
    when W and NW are both set (W's N is this pixel's NW), so the row above is joined once
    per overlapping stretch */
 __global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, int nfg, int W,
-                                                   int *par)
+                                                   int *par, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nfg) return;
+    if (k >= sxg_dn(nd, nfg)) return;
     const int p = fg[k];
     const int y = p / W, x = p - y * W;
     const bool w = x > 0 && par[p - 1] >= 0;
@@ -115,10 +117,10 @@ __global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, i
 }
 
 __global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg, int nfg, int *par,
-                                                     int *lab, int *rootpos)
+                                                     int *lab, int *rootpos, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nfg) return;
+    if (k >= sxg_dn(nd, nfg)) return;
     const int p = fg[k];
     const int r = uf_root(par, p);
     lab[p] = r;
@@ -129,10 +131,10 @@ struct RootStat {
     int *cnt, *peak, *xmin, *xmax, *ymin, *ymax, *t0;
 };
 
-__global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg)
+__global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nfg) return;
+    if (k >= sxg_dn(nd, nfg)) return;
     rs.cnt[k] = 0;
     rs.peak[k] = (int)0x80000000;   /* below f2ord of every float */
     rs.xmin[k] = 0x7fffffff;
@@ -149,12 +151,12 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
                                                       const int *__restrict__ lab,
                                                       const int *__restrict__ rootpos,
                                                       const float *__restrict__ cd, RootStat rs,
-                                                      WeightArgs wa)
+                                                      WeightArgs wa, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool perpix = wa.w && wa.rel;
-    bool act = k < nfg;
+    bool act = k < sxg_dn(nd, nfg);
     int rk = -1, x = 0, y = 0, pv = 0, tv = 0x7fffffff;
     if (act) {
         const int p = fg[k];
@@ -195,10 +197,10 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
 __global__ void __launch_bounds__(256) k_det_rootflag(const int *__restrict__ fg, int nfg,
                                                       const int *__restrict__ lab,
                                                       const int *__restrict__ cnt, int minarea,
-                                                      int *keep)
+                                                      int *keep, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nfg) return;
+    if (k >= sxg_dn(nd, nfg)) return;
     const int p = fg[k];
     keep[k] = (lab[p] == p && cnt[k] >= minarea) ? k : -1;
 }
@@ -206,10 +208,11 @@ __global__ void __launch_bounds__(256) k_det_rootflag(const int *__restrict__ fg
 __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ rawpos, int nraw,
                                                      const int *__restrict__ fg, RootStat rs,
                                                      int *rawid_of_pos, RawObj ro, float t,
-                                                     int perpix)
+                                                     int perpix, const int *nd, const float *td)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nraw) return;
+    if (i >= sxg_dn(nd, nraw)) return;
+    if (td) t = *td;
     const int k = rawpos[i];
     rawid_of_pos[k] = i;
     ro.first[i] = fg[k];
@@ -222,10 +225,10 @@ __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ raw
     ro.t0[i] = perpix ? ord2f(rs.t0[k]) : t;
 }
 
-__global__ void __launch_bounds__(256) k_det_fill(int *a, int n, int v)
+__global__ void __launch_bounds__(256) k_det_fill(int *a, int n, int v, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k < n) a[k] = v;
+    if (k < sxg_dn(nd, n)) a[k] = v;
 }
 
 /* object id of each detected pixel (or -1 when its component is too small) */
@@ -233,19 +236,20 @@ __global__ void __launch_bounds__(256) k_det_pixkey(const int *__restrict__ fg, 
                                                     const int *__restrict__ lab,
                                                     const int *__restrict__ rootpos,
                                                     const int *__restrict__ rawid_of_pos,
-                                                    int *key)
+                                                    int *key, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nfg) return;
+    if (k >= sxg_dn(nd, nfg)) return;
     key[k] = rawid_of_pos[rootpos[lab[fg[k]]]];
 }
 
 __global__ void __launch_bounds__(256) k_det_gather(const int *__restrict__ idx, int n,
                                                     const int *__restrict__ key,
-                                                    const int *__restrict__ fg, int *k0, int *v0)
+                                                    const int *__restrict__ fg, int *k0, int *v0,
+                                                    const int *nd)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= n) return;
+    if (j >= sxg_dn(nd, n)) return;
     const int k = idx[j];
     k0[j] = key[k];
     v0[j] = fg[k];
@@ -276,7 +280,10 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         sxg_set_error("run_detection: no frame");
         return;
     }
-    if (!(dthreshold > 0.0f)) {
+    /* a planned run reads the thresholds on the device (PlanDev) and its counts stay
+       there: every size below is then the plan's capacity */
+    PlanDev *pd = S.plan ? S.pdev : nullptr;
+    if (!pd && !(dthreshold > 0.0f)) {
         /* back.c:195-197 refuses zero or negative thresholds */
         sxg_set_error("run_detection: the detection threshold must be > 0");
         return;
@@ -297,20 +304,32 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     int *rootpos = sxg_take<int>("det.rootpos", S.N);
     int *cnt_d = sxg_take<int>("det.counts", 8);
     if (!rootpos || !cnt_d) return;
+    const float *td = pd ? &pd->dthresh : nullptr;
+    int *nfg_d = pd ? &pd->nfg : cnt_d, *nraw_d = pd ? &pd->nraw : cnt_d;
+    int *npix_d = pd ? &pd->npix : cnt_d;
+    /* the device counts the kernels read (planned runs) */
+    const int *nfg_k = pd ? nfg_d : nullptr, *nraw_k = pd ? nraw_d : nullptr;
+    const int *npix_k = pd ? npix_d : nullptr;
     sxg_kt_begin(SXG_KT_DET_INIT);
     WeightArgs wa;
     if (!sxg_weight_args(wa)) return;
     hipLaunchKernelGGL(k_det_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
-                       S.cd.as<float>(), dthreshold, N, W, par, lab, wa);
+                       S.cd.as<float>(), dthreshold, N, W, par, lab, wa, td);
     sxg_kt_end(SXG_KT_DET_INIT);
     /* the foreground list has at most N entries; size it after counting */
     int *fg = sxg_take<int>("det.fg", S.N);
     if (!fg) return;
     sxg_kt_begin(SXG_KT_DET_FG);
-    sxg_compact_nonneg(par, fg, (int)N, cnt_d, st);
+    sxg_compact_nonneg(par, fg, (int)N, nfg_d, st);
     sxg_kt_end(SXG_KT_DET_FG);
-    const int nfg = read_count(cnt_d, st);
-    if (nfg < 0) return;
+    int nfg;
+    if (pd) {
+        nfg = (int)S.plan->cap[PL_NFG];
+        sxg_plan_guard(nfg_d, nfg, PL_NFG);
+    } else {
+        nfg = read_count(cnt_d, st);
+        if (nfg < 0) return;
+    }
     S.npix_above = nfg;
     const int gfg = sxg_div_up(nfg > 0 ? nfg : 1, 256);
     RootStat rs;
@@ -329,57 +348,78 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         return;
     if (nfg > 0) {
         sxg_kt_begin(SXG_KT_DET_UNION);
-        hipLaunchKernelGGL(k_det_union, dim3(gfg), dim3(256), 0, st, fg, nfg, W, par);
+        hipLaunchKernelGGL(k_det_union, dim3(gfg), dim3(256), 0, st, fg, nfg, W, par, nfg_k);
         sxg_kt_end(SXG_KT_DET_UNION);
         sxg_kt_begin(SXG_KT_DET_FLATTEN);
-        hipLaunchKernelGGL(k_det_flatten, dim3(gfg), dim3(256), 0, st, fg, nfg, par, lab, rootpos);
+        hipLaunchKernelGGL(k_det_flatten, dim3(gfg), dim3(256), 0, st, fg, nfg, par, lab, rootpos,
+                           nfg_k);
         sxg_kt_end(SXG_KT_DET_FLATTEN);
-        hipLaunchKernelGGL(k_det_statinit, dim3(gfg), dim3(256), 0, st, rs, nfg);
+        hipLaunchKernelGGL(k_det_statinit, dim3(gfg), dim3(256), 0, st, rs, nfg, nfg_k);
         sxg_kt_begin(SXG_KT_DET_STAT);
         hipLaunchKernelGGL(k_det_rootstat, dim3(gfg), dim3(256), 0, st, fg, nfg, W, lab, rootpos,
-                           S.cd.as<float>(), rs, wa);
+                           S.cd.as<float>(), rs, wa, nfg_k);
         sxg_kt_end(SXG_KT_DET_STAT);
         hipLaunchKernelGGL(k_det_rootflag, dim3(gfg), dim3(256), 0, st, fg, nfg, lab, rs.cnt,
-                           ext_minarea, keep);
+                           ext_minarea, keep, nfg_k);
     }
-    sxg_compact_nonneg(keep, rawpos, nfg, cnt_d, st);
-    const int nraw = read_count(cnt_d, st);
-    if (nraw < 0) return;
+    sxg_compact_nonneg(keep, rawpos, nfg, nraw_d, st, nfg_k);
+    int nraw;
+    if (pd) {
+        nraw = (int)S.plan->cap[PL_NRAW];
+        sxg_plan_guard(nraw_d, nraw, PL_NRAW);
+    } else {
+        nraw = read_count(cnt_d, st);
+        if (nraw < 0) return;
+    }
     S.nraw = nraw;
     RawObj ro;
     if (!sxg_raw_alloc(nraw, ro)) return;
     if (nfg > 0) {
-        hipLaunchKernelGGL(k_det_fill, dim3(gfg), dim3(256), 0, st, rawid_of_pos, nfg, -1);
+        hipLaunchKernelGGL(k_det_fill, dim3(gfg), dim3(256), 0, st, rawid_of_pos, nfg, -1, nfg_k);
         if (nraw > 0)
             hipLaunchKernelGGL(k_det_rawinfo, dim3(sxg_div_up(nraw, 256)), dim3(256), 0, st, rawpos,
-                               nraw, fg, rs, rawid_of_pos, ro, dthreshold, (int)(wa.w && wa.rel));
+                               nraw, fg, rs, rawid_of_pos, ro, dthreshold, (int)(wa.w && wa.rel),
+                               nraw_k, td);
         /* pixel keys, kept pixels, stable sort by object */
         hipLaunchKernelGGL(k_det_pixkey, dim3(gfg), dim3(256), 0, st, fg, nfg, lab, rootpos,
-                           rawid_of_pos, keep);
+                           rawid_of_pos, keep, nfg_k);
     }
     int *idx = rawpos;   /* reuse: indices of kept pixels in the foreground list */
     int *k0 = sxg_take<int>("det.k0", nfg), *v0 = sxg_take<int>("det.v0", nfg);
     int *k1 = sxg_take<int>("det.k1", nfg), *v1 = sxg_take<int>("det.v1", nfg);
     int *hist = sxg_take<int>("det.hist", 256 * (size_t)(sxg_div_up(nfg + 1, 4096) + 1) + 64);
     if (!k0 || !v0 || !k1 || !v1 || !hist) return;
-    sxg_compact_nonneg(keep, idx, nfg, cnt_d, st);
-    const int npix = read_count(cnt_d, st);
-    if (npix < 0) return;
+    sxg_compact_nonneg(keep, idx, nfg, npix_d, st, nfg_k);
+    int npix;
+    if (pd) {
+        npix = (int)S.plan->cap[PL_NPIX];
+        sxg_plan_guard(npix_d, npix, PL_NPIX);
+    } else {
+        npix = read_count(cnt_d, st);
+        if (npix < 0) return;
+    }
     S.npix_raw = npix;
+    if (!pd) {
+        S.seen.cap[PL_NFG] = nfg;
+        S.seen.cap[PL_NRAW] = nraw;
+        S.seen.cap[PL_NPIX] = npix;
+    }
     if (npix > 0) {
         hipLaunchKernelGGL(k_det_gather, dim3(sxg_div_up(npix, 256)), dim3(256), 0, st, idx, npix,
-                           keep, fg, k0, v0);
+                           keep, fg, k0, v0, npix_k);
+        /* enough key bits for every object id (for the capacity in a planned run: more
+           passes over zero digits leave a stable sort's order as it is) */
         int nbits = 0;
         while ((1L << nbits) < (long)nraw) nbits++;
         if (nbits == 0) nbits = 1;
         sxg_kt_begin(SXG_KT_DET_SORT);
-        int where = sxg_radix_sort(k0, v0, k1, v1, npix, nbits, hist, st);
+        int where = sxg_radix_sort(k0, v0, k1, v1, npix, nbits, hist, st, npix_k);
         sxg_kt_end(SXG_KT_DET_SORT);
         int *ks = where ? k1 : k0, *vs = where ? v1 : v0;
         S.fpia_keys = ks;
         S.fpia = vs;
         /* offsets of each object's pixel run */
-        sxg_scan_excl(ro.count, ro.off, nraw, nullptr, st);
+        sxg_scan_excl(ro.count, ro.off, nraw, nullptr, st, nraw_k);
     } else {
         S.fpia_keys = k0;
         S.fpia = v0;

@@ -146,6 +146,10 @@ extern "C" void cudaFilter(float *h_mask, unsigned int mask_width)
        copied synchronously, so the caller's array is free again on return */
     const size_t mb = sizeof(float) * K * K;
     if (S.mask_k != K || S.mask_buf != S.dmask.ptr || memcmp(S.mask_host, h_mask, mb) != 0) {
+        if (S.capturing) {
+            sxg_set_error("cudaFilter: a new mask while a graph is captured");
+            return;
+        }
         SXG_HIP(hipStreamSynchronize(S.stream));
         SXG_HIP(hipMemcpy(dmask, h_mask, mb, hipMemcpyHostToDevice));
         memcpy(S.mask_host, h_mask, mb);

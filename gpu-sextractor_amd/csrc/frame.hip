@@ -33,7 +33,7 @@ static bool g_kt_on = false;
 
 void sxg_kt_begin(int id)
 {
-    if (!g_kt_on || !sxg_state().is_default) return;
+    if (!g_kt_on || !sxg_state().is_default || sxg_state().plan) return;
     KTimer &t = g_kt[id];
     if ((int)t.ev.size() < 2 * (t.used + 1)) {
         hipEvent_t a, b;
@@ -46,7 +46,7 @@ void sxg_kt_begin(int id)
 
 void sxg_kt_end(int id)
 {
-    if (!g_kt_on || !sxg_state().is_default) return;
+    if (!g_kt_on || !sxg_state().is_default || sxg_state().plan) return;
     KTimer &t = g_kt[id];
     if ((int)t.ev.size() < 2 * (t.used + 1)) return;
     (void)hipEventRecord(t.ev[2 * t.used + 1], sxg_state().stream);
@@ -56,7 +56,7 @@ void sxg_kt_end(int id)
 /* the same, recorded on stream st (kernels launched on a side stream) */
 void sxg_kt_begin_on(int id, hipStream_t st)
 {
-    if (!g_kt_on || !sxg_state().is_default) return;
+    if (!g_kt_on || !sxg_state().is_default || sxg_state().plan) return;
     KTimer &t = g_kt[id];
     if ((int)t.ev.size() < 2 * (t.used + 1)) {
         hipEvent_t a, b;
@@ -69,7 +69,7 @@ void sxg_kt_begin_on(int id, hipStream_t st)
 
 void sxg_kt_end_on(int id, hipStream_t st)
 {
-    if (!g_kt_on || !sxg_state().is_default) return;
+    if (!g_kt_on || !sxg_state().is_default || sxg_state().plan) return;
     KTimer &t = g_kt[id];
     if ((int)t.ev.size() < 2 * (t.used + 1)) return;
     (void)hipEventRecord(t.ev[2 * t.used + 1], st);
@@ -116,9 +116,65 @@ extern "C" int sxg_set_resident_frame(const float *host, int width, int height)
     return 0;
 }
 
-/* the whole path on a device frame (not modified), on the current context */
+/* ------------------------------------------------------------ planned runs ---- */
+/* the host copy of one planned frame: its device block and the device clock (wall_clock64)
+   at its first and last kernel */
+struct PlanRecord {
+    PlanDev pd;
+    unsigned long long t0, t1;
+};
+
+__global__ void k_plan_begin(PlanDev *pd, unsigned long long *t0)
+{
+    int *w = reinterpret_cast<int *>(pd);
+    for (int k = threadIdx.x; k < (int)(sizeof(PlanDev) / sizeof(int)); k += blockDim.x) w[k] = 0;
+    if (threadIdx.x == 0 && t0) *t0 = wall_clock64();
+}
+
+/* the thresholds from the background's medians (outv: backmean, backsig), as the host
+   derives them in an eager run; manual: BACK_TYPE MANUAL reports BACK_VALUE as backmean */
+__global__ void k_plan_thresh(PlanDev *pd, const float *outv, int thresh_type, double dt,
+                              double at, int manual, float back_value)
+{
+    if (threadIdx.x != 0) return;
+    const float backsig = outv[1];
+    pd->backmean = manual ? back_value : outv[0];
+    pd->backsig = backsig;
+    float d, t;
+    if (thresh_type == 1) {
+        d = (float)dt;
+        t = (float)at;
+    } else {
+        d = (float)(dt * backsig);
+        t = (float)(at * backsig);
+    }
+    pd->dthresh = d;
+    pd->thresh = t;
+    /* back.c:195-197 refuses zero or negative thresholds: the eager rerun reports it */
+    if (!(d > 0.0f)) pd->ovf |= 1 << SXG_PLAN_BADTHRESH;
+}
+
+__global__ void k_plan_end(const PlanDev *pd, PlanRecord *rec)
+{
+    if (threadIdx.x != 0) return;
+    rec->pd = *pd;
+    rec->t1 = wall_clock64();
+}
+
+/* resets the context's planned-run state when the frame's queueing ends, however it ends */
+struct PlanScope {
+    SxgState &S;
+    explicit PlanScope(SxgState &s, const FramePlan *plan) : S(s) { S.plan = plan; }
+    ~PlanScope() { S.plan = nullptr; }
+};
+
+/* The whole path on a device frame (not modified), on the current context.  plan: a
+   planned run (counts stay on the device, sized by the plan's capacities; nothing waits
+   or allocates once the context has run a frame of that plan), whose device block is
+   copied to *rec (device) at the end; info is then not written. */
 static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
-                         sxg_frame_info *info)
+                         sxg_frame_info *info, const FramePlan *plan = nullptr,
+                         PlanRecord *rec = nullptr)
 {
     SxgState &S = sxg_state();
     int ndev = 0;
@@ -127,6 +183,13 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
         return -1;
     }
     if (!sxg_ensure_stream()) return -1;
+    PlanScope scope(S, plan);
+    if (plan) {
+        S.pdev = sxg_take<PlanDev>("plan.dev", 1);
+        if (!S.pdev) return -1;
+        hipLaunchKernelGGL(k_plan_begin, dim3(1), dim3(64), 0, S.stream, S.pdev,
+                           rec ? &rec->t0 : nullptr);
+    }
     /* init_device from device memory: the background stage reads the frame where it is
        and writes the subtracted frame into the context's buffer, so the input is never
        copied */
@@ -162,8 +225,14 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
     if (sxg_has_error()) return -1;
     /* back.c:152-153: with BACK_TYPE MANUAL the frame's backmean is BACK_VALUE */
     if (p->back_type == 1) S.backmean = p->back_value;
-    float dthresh, thresh;
-    if (p->thresh_type == 1) {
+    float dthresh = 0.0f, thresh = 0.0f;
+    if (plan) {
+        /* the medians stay on the device: so do the thresholds */
+        BackBufs b = sxg_back_bufs_at(S.backbuf.as<char>(), S.bgeom);
+        hipLaunchKernelGGL(k_plan_thresh, dim3(1), dim3(64), 0, S.stream, S.pdev,
+                           (const float *)b.outv, p->thresh_type, p->detect_thresh,
+                           p->analysis_thresh, p->back_type == 1 ? 1 : 0, p->back_value);
+    } else if (p->thresh_type == 1) {
         dthresh = (float)p->detect_thresh;
         thresh = (float)p->analysis_thresh;
     } else {
@@ -189,8 +258,33 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
     run_clean(nullptr, p->clean_param, p->clean_stacksize, nobj);
     if (sxg_has_error()) return -1;
     if (!sxg_build_segmap()) return -1;
+    if (plan) {
+        if (rec) hipLaunchKernelGGL(k_plan_end, dim3(1), dim3(64), 0, S.stream, S.pdev, rec);
+        SXG_HIP_RET(hipGetLastError(), -1);
+        return 0;
+    }
     if (info) sxg_get_frame_info(info);
     return nobj;
+}
+
+/* the frame's sxg_frame_info from the record of a planned run */
+static void info_from_record(const PlanRecord &r, int nbx, int nby, sxg_frame_info *fi)
+{
+    memset(fi, 0, sizeof(*fi));
+    fi->backmean = r.pd.backmean;
+    fi->backsig = r.pd.backsig;
+    fi->dthresh = r.pd.dthresh;
+    fi->thresh = r.pd.thresh;
+    fi->npix_above = r.pd.nfg;
+    fi->npix_raw = r.pd.npix;
+    fi->nraw = r.pd.nraw;
+    fi->nobj = r.pd.nobj;
+    fi->nsurv = r.pd.nsurv;
+    fi->nbx = nbx;
+    fi->nby = nby;
+    fi->deb_pixels = r.pd.deb_pixels;
+    fi->deb_labels = r.pd.deb_labels;
+    for (int c = 0; c < 4; c++) fi->deb_class_pixels[c] = r.pd.class_pixels[c];
 }
 
 extern "C" int sxg_process_resident(const sxg_params *p, sxg_frame_info *info)
@@ -217,6 +311,7 @@ struct BatchFrame {
     int W = 0, H = 0;
 };
 static std::vector<BatchFrame> g_batch;
+static void graph_drop();   /* forgets the batch's graph (its frames change) */
 static std::vector<sxg_ctx *> g_workers;
 
 extern "C" int sxg_batch_set_frames(int n, const float *const *host, const int *width,
@@ -227,6 +322,7 @@ extern "C" int sxg_batch_set_frames(int n, const float *const *host, const int *
         sxg_set_error("sxg_batch_set_frames: bad arguments");
         return -1;
     }
+    graph_drop();
     for (auto &f : g_batch) {
         f.frame.release();
         f.seg.release();
@@ -380,6 +476,347 @@ extern "C" int sxg_batch_process_host(const sxg_params *p, int nworkers, int n,
     return run_pool(n, nworkers, run, frame_ms, "sxg_batch_process_host");
 }
 
+/* ---------------------------------------------------------- graph batches ---- */
+/* The resident batch as one HIP graph: a branch per worker context, each running its
+   frames one after the other as planned runs (no host wait, no allocation), the
+   segmentation maps copied into the frames' slots and every frame's record into pinned
+   host memory at the end.  Built in four steps:
+     1. calibration: every frame eagerly (the worker pool), recording its counts and time;
+     2. plans: each count with a margin (margin_pct percent plus a few units) as its
+        capacity; branches by longest-first greedy assignment of the calibrated times;
+     3. warm-up: every frame once as a planned run on its branch's context, uncaptured,
+        which sizes the contexts' buffers for the plans (the graph records addresses);
+     4. capture: the branches recorded from an origin stream (fork and join events) and
+        the graph instantiated.
+   A launch replays all of it; a frame whose record shows an overflow (counts above the
+   plan, e.g. after sxg_batch_update_frame loaded a denser frame) runs again eagerly on
+   its context, which may grow its buffers and so marks the graph for a rebuild. */
+struct GraphBatch {
+    bool built = false;
+    sxg_params p{};
+    std::vector<float> mask;
+    int nworkers = 0, margin = 0;
+    std::vector<FramePlan> plans;
+    std::vector<int> branch;        /* worker context of each frame */
+    std::vector<double> calib_ms;   /* calibrated time of each frame */
+    std::vector<uint64_t> gen;      /* each worker's allocation generation at capture */
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf rec;                     /* PlanRecord of each frame (device) */
+    PlanRecord *hrec = nullptr;     /* their pinned host copy */
+    int nrec = 0;
+    long builds = 0, reruns = 0, launches = 0;
+    double clock_khz = 0.0;         /* wall_clock64 rate */
+    long opts[SXG_NOPT] = {};       /* the options the graph was built with */
+};
+static GraphBatch g_gb;
+static int g_plan_margin = 25;
+
+static void graph_drop()
+{
+    GraphBatch &G = g_gb;
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    if (G.graph) (void)hipGraphDestroy(G.graph);
+    G.exec = nullptr;
+    G.graph = nullptr;
+    G.built = false;
+}
+
+static void graph_release()
+{
+    GraphBatch &G = g_gb;
+    graph_drop();
+    if (G.origin) (void)hipStreamSynchronize(G.origin);
+    for (hipEvent_t e : G.join) (void)hipEventDestroy(e);
+    G.join.clear();
+    if (G.fork) (void)hipEventDestroy(G.fork);
+    G.fork = nullptr;
+    if (G.origin) (void)hipStreamDestroy(G.origin);
+    G.origin = nullptr;
+    if (G.hrec) (void)hipHostFree(G.hrec);
+    G.hrec = nullptr;
+    G.nrec = 0;
+    G.rec.release();
+}
+
+/* capacity of a count seen in calibration: the margin, and a few units of slack */
+static FramePlan plan_from(const FramePlan &seen, int margin)
+{
+    static const int64_t slack[PL_N] = {4096, 64, 1024, 256, 16, 8, 4, 4, 2, 65536, 1024,
+                                        0, 0, 0, 128};
+    FramePlan pl;
+    for (int k = 0; k < PL_N; k++) {
+        if (k >= PL_LDS0 && k <= PL_LDS2) continue;
+        pl.cap[k] = seen.cap[k] + seen.cap[k] * margin / 100 + slack[k];
+    }
+    /* the level-parallel objects are one grid row each (at most 65535) */
+    pl.cap[PL_BIG] = std::min<int64_t>(pl.cap[PL_BIG], 65535);
+    /* LDS of a class's workgroups: the class's budget (the kernel is launched with the
+       smaller of the two), except that class 2 asks for 76 KiB when its largest tree
+       seen, with the margin, fits there: two of its 1024-lane workgroups then share a
+       CU */
+    for (int c = 0; c < 3; c++) pl.cap[PL_LDS0 + c] = 0x7fffffff;
+    const int64_t need2 = seen.cap[PL_LDS2] + seen.cap[PL_LDS2] * margin / 100;
+    if (need2 <= 76 * 1024) pl.cap[PL_LDS2] = 76 * 1024;
+    for (int k = 0; k < PL_N; k++) pl.cap[k] = std::min<int64_t>(pl.cap[k], 0x7ffffff0);
+    return pl;
+}
+
+static bool same_params(const sxg_params &a, const sxg_params *b, const std::vector<float> &mask)
+{
+    /* field by field (the struct has padding) */
+    if (a.back_size != b->back_size || a.back_fsize != b->back_fsize ||
+        a.back_fthresh != b->back_fthresh || a.back_type != b->back_type ||
+        a.back_value != b->back_value || a.thresh_type != b->thresh_type ||
+        a.detect_thresh != b->detect_thresh || a.analysis_thresh != b->analysis_thresh ||
+        a.ext_minarea != b->ext_minarea || a.deblend_nthresh != b->deblend_nthresh ||
+        a.deblend_mincont != b->deblend_mincont || a.clean != b->clean ||
+        a.clean_param != b->clean_param || a.clean_stacksize != b->clean_stacksize ||
+        a.satur_level != b->satur_level || a.gain != b->gain || a.mask_width != b->mask_width)
+        return false;
+    const size_t nm = b->mask && b->mask_width > 0 ? (size_t)b->mask_width * b->mask_width : 0;
+    if (nm != mask.size()) return false;
+    return nm == 0 || memcmp(mask.data(), b->mask, nm * sizeof(float)) == 0;
+}
+
+/* the seg map of frame i from the context's "out.map", on the context's stream */
+static bool copy_seg(int i)
+{
+    SxgState &S = sxg_state();
+    BatchFrame &f = g_batch[i];
+    int *map = sxg_take<int>("out.map", S.N);
+    if (!map || hipMemcpyAsync(f.seg.ptr, map, S.N * sizeof(int), hipMemcpyDeviceToDevice,
+                               S.stream) != hipSuccess) {
+        sxg_set_error("graph batch: segmentation copy failed");
+        return false;
+    }
+    return true;
+}
+
+static int graph_build(const sxg_params *p, int nworkers, int margin)
+{
+    GraphBatch &G = g_gb;
+    const int n = (int)g_batch.size();
+    graph_drop();
+    G.builds++;
+    /* 1. calibration */
+    G.plans.assign(n, FramePlan());
+    G.calib_ms.assign(n, 0.0);
+    std::vector<FramePlan> seen(n);
+    auto calib = [&](int i) {
+        BatchFrame &f = g_batch[i];
+        if (process_frame(f.frame.as<float>(), f.W, f.H, p, nullptr) < 0) return false;
+        seen[i] = sxg_state().seen;
+        return hipStreamSynchronize(sxg_state().stream) == hipSuccess;
+    };
+    if (run_pool(n, nworkers, calib, G.calib_ms.data(), "graph batch calibration") < 0) return -1;
+    /* 2. plans and branches (longest calibrated frame first, to the least loaded branch) */
+    for (int i = 0; i < n; i++) G.plans[i] = plan_from(seen[i], margin);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return G.calib_ms[a] > G.calib_ms[b]; });
+    std::vector<double> load(nworkers, 0.0);
+    G.branch.assign(n, 0);
+    for (int i : order) {
+        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        G.branch[i] = w;
+        load[w] += G.calib_ms[i];
+    }
+    /* the records and the streams and events of the fork and join */
+    if (G.nrec < n) {
+        if (G.hrec) (void)hipHostFree(G.hrec);
+        G.hrec = nullptr;
+        G.nrec = 0;
+        SXG_HIP_RET(hipHostMalloc((void **)&G.hrec, sizeof(PlanRecord) * (n > 0 ? n : 1)), -1);
+        G.nrec = n;
+    }
+    if (!G.rec.ensure(sizeof(PlanRecord) * (n > 0 ? n : 1))) return -1;
+    if (!G.origin) {
+        SXG_HIP_RET(hipStreamCreateWithFlags(&G.origin, hipStreamNonBlocking), -1);
+        SXG_HIP_RET(hipEventCreateWithFlags(&G.fork, hipEventDisableTiming), -1);
+        int rate = 0;
+        SXG_HIP_RET(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0), -1);
+        G.clock_khz = rate;
+    }
+    while ((int)G.join.size() < nworkers) {
+        hipEvent_t e;
+        SXG_HIP_RET(hipEventCreateWithFlags(&e, hipEventDisableTiming), -1);
+        G.join.push_back(e);
+    }
+    PlanRecord *drec = G.rec.as<PlanRecord>();
+    SxgState *prev = sxg_bound_state();
+    auto queue_branch = [&](int w) -> bool {
+        sxg_ctx_bind(g_workers[w]);
+        for (int i = 0; i < n; i++) {
+            if (G.branch[i] != w) continue;
+            BatchFrame &f = g_batch[i];
+            if (process_frame(f.frame.as<float>(), f.W, f.H, p, nullptr, &G.plans[i], drec + i) < 0 ||
+                !copy_seg(i))
+                return false;
+        }
+        return true;
+    };
+    /* 3. warm-up, uncaptured: sizes every buffer; the planned frames must not overflow
+       here (they are the calibrated frames) */
+    for (int w = 0; w < nworkers; w++) {
+        if (!queue_branch(w) || hipStreamSynchronize(sxg_state().stream) != hipSuccess) {
+            if (!sxg_has_error()) sxg_set_error("graph batch: warm-up failed");
+            sxg_bind_state(prev);
+            return -1;
+        }
+    }
+    sxg_bind_state(prev);
+    SXG_HIP_RET(hipMemcpy(G.hrec, drec, sizeof(PlanRecord) * n, hipMemcpyDeviceToHost), -1);
+    for (int i = 0; i < n; i++)
+        if (G.hrec[i].pd.ovf != 0) {
+            const PlanDev &d = G.hrec[i].pd;
+            std::string caps;
+            for (int k = 0; k < PL_N; k++)
+                caps += " " + std::to_string(seen[i].cap[k]) + "/" +
+                        std::to_string(G.plans[i].cap[k]);
+            sxg_set_error("graph batch: calibrated frame " + std::to_string(i) +
+                          " overflowed its plan (internal error): flags " +
+                          std::to_string(d.ovf) + ", counts nfg " + std::to_string(d.nfg) +
+                          " nraw " + std::to_string(d.nraw) + " npix " + std::to_string(d.npix) +
+                          " nobj " + std::to_string(d.nobj) + "; seen/capacity" + caps);
+            return -1;
+        }
+    /* 4. capture */
+    SXG_HIP_RET(hipStreamBeginCapture(G.origin, hipStreamCaptureModeRelaxed), -1);
+    bool ok = hipEventRecord(G.fork, G.origin) == hipSuccess;
+    for (int w = 0; ok && w < nworkers; w++) {
+        sxg_ctx_bind(g_workers[w]);
+        SxgState &S = sxg_state();
+        ok = hipStreamWaitEvent(S.stream, G.fork, 0) == hipSuccess;
+        S.capturing = true;
+        ok = ok && queue_branch(w);
+        S.capturing = false;
+        ok = ok && hipEventRecord(G.join[w], S.stream) == hipSuccess &&
+             hipStreamWaitEvent(G.origin, G.join[w], 0) == hipSuccess;
+    }
+    sxg_bind_state(prev);
+    ok = ok && hipMemcpyAsync(G.hrec, drec, sizeof(PlanRecord) * n, hipMemcpyDeviceToHost,
+                              G.origin) == hipSuccess;
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(G.origin, &graph);
+    if (!ok || ec != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!sxg_has_error())
+            sxg_set_error(std::string("graph batch: capture failed: ") + hipGetErrorString(ec));
+        return -1;
+    }
+    G.graph = graph;
+    SXG_HIP_RET(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0), -1);
+    G.gen.assign(nworkers, 0);
+    for (int w = 0; w < nworkers; w++) G.gen[w] = sxg_ctx_state(g_workers[w]).alloc_gen;
+    G.p = *p;
+    G.mask.assign(p->mask && p->mask_width > 0 ? p->mask : nullptr,
+                  p->mask && p->mask_width > 0 ? p->mask + (size_t)p->mask_width * p->mask_width
+                                               : nullptr);
+    G.nworkers = nworkers;
+    G.margin = margin;
+    for (int k = 0; k < SXG_NOPT; k++) G.opts[k] = sxg_option(k);
+    G.built = true;
+    return 0;
+}
+
+extern "C" int sxg_batch_process_graph(const sxg_params *p, int nworkers, sxg_frame_info *info,
+                                       double *frame_ms)
+{
+    sxg_clear_error();
+    GraphBatch &G = g_gb;
+    const int n = (int)g_batch.size();
+    if (!p) {
+        sxg_set_error("sxg_batch_process_graph: no parameters");
+        return -1;
+    }
+    if (nworkers < 1 || nworkers > SXG_MAX_WORKERS) {
+        sxg_set_error("sxg_batch_process_graph: nworkers must be 1.." +
+                      std::to_string(SXG_MAX_WORKERS));
+        return -1;
+    }
+    if (n == 0) return 0;
+    bool fresh = G.built && G.nworkers == nworkers && G.margin == g_plan_margin &&
+                 same_params(G.p, p, G.mask) && (int)g_workers.size() >= nworkers;
+    for (int w = 0; fresh && w < nworkers; w++)
+        fresh = sxg_ctx_state(g_workers[w]).alloc_gen == G.gen[w];
+    /* the options choose launch shapes the graph has recorded */
+    for (int k = 0; fresh && k < SXG_NOPT; k++) fresh = G.opts[k] == sxg_option(k);
+    if (!fresh && graph_build(p, nworkers, g_plan_margin) < 0) return -1;
+    SXG_HIP_RET(hipGraphLaunch(G.exec, G.origin), -1);
+    SXG_HIP_RET(hipStreamSynchronize(G.origin), -1);
+    G.launches++;
+    /* frames past their plan run again eagerly on their own context */
+    SxgState *prev = sxg_bound_state();
+    int rerun = 0;
+    for (int i = 0; i < n; i++) {
+        const PlanRecord &r = G.hrec[i];
+        BatchFrame &f = g_batch[i];
+        const int nbx = (f.W - 1) / std::min(p->back_size, f.W) + 1;
+        const int nby = (f.H - 1) / std::min(p->back_size, f.W) + 1;
+        if (r.pd.ovf == 0) {
+            if (info) info_from_record(r, nbx, nby, &info[i]);
+            if (frame_ms) frame_ms[i] = (double)(r.t1 - r.t0) / G.clock_khz;
+            continue;
+        }
+        sxg_ctx_bind(g_workers[G.branch[i]]);
+        const auto t0 = std::chrono::steady_clock::now();
+        sxg_frame_info fi;
+        const bool ok = process_frame(f.frame.as<float>(), f.W, f.H, p, &fi) >= 0 && copy_seg(i) &&
+                        hipStreamSynchronize(sxg_state().stream) == hipSuccess;
+        sxg_bind_state(prev);
+        if (!ok) {
+            if (!sxg_has_error()) sxg_set_error("sxg_batch_process_graph: eager rerun failed");
+            return -1;
+        }
+        if (info) info[i] = fi;
+        if (frame_ms)
+            frame_ms[i] = std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - t0).count();
+        rerun++;
+    }
+    G.reruns += rerun;
+    /* frames that outgrew their plans are calibrated again at the next call */
+    if (rerun > 0) graph_drop();
+    return rerun;
+}
+
+extern "C" int sxg_batch_graph_stats(long *builds, long *launches, long *reruns)
+{
+    if (builds) *builds = g_gb.builds;
+    if (launches) *launches = g_gb.launches;
+    if (reruns) *reruns = g_gb.reruns;
+    return g_gb.built ? 1 : 0;
+}
+
+extern "C" int sxg_batch_set_plan_margin(int percent)
+{
+    if (percent < 0 || percent > 1000) {
+        sxg_set_error("sxg_batch_set_plan_margin: 0..1000 percent");
+        return -1;
+    }
+    g_plan_margin = percent;
+    return 0;
+}
+
+extern "C" int sxg_batch_update_frame(int frame, const float *host)
+{
+    sxg_clear_error();
+    if (frame < 0 || frame >= (int)g_batch.size() || !host) {
+        sxg_set_error("sxg_batch_update_frame: bad frame index");
+        return -1;
+    }
+    BatchFrame &f = g_batch[frame];
+    /* the slot keeps its address: a built graph reads the new pixels at its next launch */
+    SXG_HIP_RET(hipMemcpy(f.frame.ptr, host, (size_t)f.W * f.H * sizeof(float),
+                          hipMemcpyHostToDevice), -1);
+    return 0;
+}
+
 extern "C" int sxg_batch_segmap(int frame, int32_t *out)
 {
     sxg_clear_error();
@@ -395,6 +832,7 @@ extern "C" int sxg_batch_segmap(int frame, int32_t *out)
 
 extern "C" void sxg_batch_release(void)
 {
+    graph_release();
     for (auto &f : g_batch) {
         f.frame.release();
         f.seg.release();

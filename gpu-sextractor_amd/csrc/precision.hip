@@ -35,6 +35,7 @@ struct PrecArgs {
     const int *objcomp;              /* component of each object in its raw object */
     const float *sub, *cd;
     int W, nobj;
+    const int *nd;                   /* planned run: the device's object count */
     ObjCols o;
 };
 
@@ -92,7 +93,7 @@ template <int MODE> __global__ void __launch_bounds__(256) k_precision(PrecArgs 
 {
     const int lane = threadIdx.x & 63;
     const int obj = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (obj >= A.nobj) return;
+    if (obj >= sxg_dn(A.nd, A.nobj)) return;
     ObjCols o = A.o;
     const int W = A.W;
     const int L = o.level[obj];
@@ -207,6 +208,7 @@ bool sxg_precision_columns(int mode)
     A.cd = S.cd.as<float>();
     A.W = S.W;
     A.nobj = S.nobj;
+    A.nd = S.plan ? &S.pdev->nobj : nullptr;
     if (!sxg_obj_view(A.o) || !A.kcomp || !A.objcomp) return false;
     const dim3 grid(sxg_div_up(S.nobj, 4)), block(256);
     if (mode == 1)

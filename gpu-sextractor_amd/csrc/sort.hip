@@ -20,9 +20,11 @@
 #define RS_BINS 256
 
 __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const int *__restrict__ keys, int n,
-                                                        int shift, int ntiles, int *hist)
+                                                        int shift, int ntiles, int *hist,
+                                                        const int *nd)
 {
     __shared__ int h[RS_BINS];
+    n = sxg_dn(nd, n);
     h[threadIdx.x] = 0;
     __syncthreads();
     const long base = (long)blockIdx.x * RS_TILE;
@@ -38,9 +40,10 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const int *__restrict
                                                            const int *__restrict__ vin, int n,
                                                            int shift, int ntiles,
                                                            const int *__restrict__ offs,
-                                                           int *kout, int *vout)
+                                                           int *kout, int *vout, const int *nd)
 {
     __shared__ int cnt[RS_BINS];
+    n = sxg_dn(nd, n);
     __shared__ int wc[RS_THREADS / 64][RS_BINS];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     cnt[t] = offs[t * ntiles + blockIdx.x];
@@ -81,9 +84,11 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const int *__restrict
 
 /* Sorts n pairs by the low `nbits` bits of the keys (keys must be >= 0).  The result is
    in (k0, v0) when the number of passes is even, otherwise in (k1, v1); the function
-   returns which (0 or 1).  hist must hold 256 * ceil(n/4096) + 64 ints. */
+   returns which (0 or 1).  hist must hold 256 * ceil(n/4096) + 64 ints.  With nd the
+   pairs are the first *nd (<= n): the tiles past them count nothing, which leaves every
+   offset as it is. */
 int sxg_radix_sort(int *k0, int *v0, int *k1, int *v1, int n, int nbits, int *hist,
-                   hipStream_t s)
+                   hipStream_t s, const int *nd)
 {
     if (n <= 0) return 0;
     const int ntiles = sxg_div_up(n, RS_TILE);
@@ -92,10 +97,10 @@ int sxg_radix_sort(int *k0, int *v0, int *k1, int *v1, int n, int nbits, int *hi
         int *ki = cur ? k1 : k0, *vi = cur ? v1 : v0;
         int *ko = cur ? k0 : k1, *vo = cur ? v0 : v1;
         hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, ki, n, shift, ntiles,
-                           hist);
+                           hist, nd);
         sxg_scan_excl(hist, hist, RS_BINS * ntiles, nullptr, s);
         hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, s, ki, vi, n, shift,
-                           ntiles, hist, ko, vo);
+                           ntiles, hist, ko, vo, nd);
         cur ^= 1;
     }
     return cur;

@@ -115,11 +115,59 @@ const float *sxg_back_map();
 size_t sxg_back_bytes(const BackGeom &g);
 BackBufs sxg_back_bufs_at(char *base, const BackGeom &g);
 
+/* ------------------------------------------------------ count-planned runs ---- */
+/* A planned run queues a whole frame without reading a count back to the host: every
+   count that sizes a launch or a buffer is replaced by a capacity fixed beforehand (the
+   counts of an eager run of a frame of the same kind, with a margin), the kernels read
+   the actual count on the device and stop there, and a guard after each count flags one
+   above its capacity.  A flagged frame's counts are zeroed at once, so every later kernel
+   of the frame finds no work and stays inside its buffers; the caller runs that frame
+   again eagerly.  Such a run neither waits for the device nor allocates, so a stream
+   capture can record it into a HIP graph. */
+enum PlanCap {
+    PL_NFG = 0,   /* detected pixels */
+    PL_NRAW,      /* raw objects */
+    PL_NPIX,      /* pixels of raw objects */
+    PL_NTOT,      /* node capacity of the deblending trees */
+    PL_C0, PL_C1, PL_C2, PL_C3,     /* raw objects per deblending size class */
+    PL_BIG,       /* class-3 objects built level-parallel */
+    PL_NPL,       /* pixel-levels of those */
+    PL_NREC,      /* node records of those */
+    PL_LDS0, PL_LDS1, PL_LDS2,      /* LDS bytes of the tree workgroups of classes 0..2 */
+    PL_NOBJ,      /* deblended objects */
+    PL_N
+};
+struct FramePlan {
+    int64_t cap[PL_N] = {};
+};
+/* the device block of a planned run (one per context) */
+struct PlanDev {
+    int nfg, nraw, npix, nobj, nsurv;
+    int ovf;          /* bit k: the count of PlanCap k exceeded its capacity; bit 30: the
+                         detection threshold is not positive; bit 29: deblending levels
+                         inconsistent */
+    int pad[2];
+    float dthresh, thresh, backmean, backsig;
+    /* deblending work (sxg_frame_info's deb_pixels, deb_labels, deb_class_pixels) */
+    long long deb_pixels, deb_labels, class_pixels[4];
+};
+#define SXG_PLAN_BADTHRESH 30
+#define SXG_PLAN_DEBLEVELS 29
+/* n, or the device count *nd when a planned run passes one */
+__device__ inline int sxg_dn(const int *nd, int n) { return nd ? *nd : n; }
+
 #define SXG_NSIDE 4
 struct SxgState {
     int W = 0, H = 0;
     size_t N = 0;
     bool have_frame = false;
+    /* planned run (nullptr: eager) and its device block; capturing: a stream capture is
+       recording this context's launches (allocation is an error then) */
+    const FramePlan *plan = nullptr;
+    PlanDev *pdev = nullptr;
+    bool capturing = false;
+    uint64_t alloc_gen = 0;     /* bumped by every reallocation of a device buffer */
+    FramePlan seen;             /* the counts of the last eager run (capacities of a plan) */
     hipStream_t stream = nullptr;
     hipEvent_t ev[2 * ST_N] = {};
     /* side streams (default priority) for independent launches inside a stage, and the
@@ -230,6 +278,9 @@ bool sxg_astrometry(const void *wcs);
 int sxg_wcs_layout(long *out, int n);
 
 SxgState &sxg_state();
+/* the state bound to the calling thread (nullptr: the default context), and rebinding it */
+SxgState *sxg_bound_state();
+void sxg_bind_state(SxgState *s);
 /* End of an entry point without a host output: the contexts of the batch workers
    (several sharing the device) wait for their stream there, which interleaves their
    kernels on the hardware queues; the default context returns at once and keeps its
@@ -241,22 +292,37 @@ void sxg_stage_end(StageId s);
 /* ---------------------------------------------------------- primitives ---- */
 /* Exclusive prefix sum of n int32 values (out may alias in); *total (device) receives
    the sum when total != nullptr. */
-void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s);
+/* With nd (planned runs) n is a capacity and the number of values is *nd (<= n), read
+   on the device; the launches are sized for n. */
+void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s,
+                   const int *nd = nullptr);
 /* Stream compaction of the indices i in [0,n) with flag[i] != 0, in increasing order;
    *count (device) receives the number kept. */
-void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s);
+void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s,
+                       const int *nd = nullptr);
 /* Same, keeping the indices i with v[i] >= 0. */
-void sxg_compact_nonneg(const int *v, int *out, int n, int *count, hipStream_t s);
+void sxg_compact_nonneg(const int *v, int *out, int n, int *count, hipStream_t s,
+                        const int *nd = nullptr);
 /* Stable LSD radix sort of (key, value) pairs on the low nbits of non-negative keys;
    returns 0 when the result is in (k0, v0), 1 when in (k1, v1). */
 int sxg_radix_sort(int *k0, int *v0, int *k1, int *v1, int n, int nbits, int *hist,
-                   hipStream_t s);
+                   hipStream_t s, const int *nd = nullptr);
+/* Planned runs: queue the check of the device count *v against cap (bit: its PlanCap);
+   above it, or after an earlier overflow of the frame, the frame's counts are zeroed
+   (also the deblending class counts ccount[0..3] and [8] when ccount is given). */
+void sxg_plan_guard(int *v, int64_t cap, int bit, int *ccount = nullptr);
+/* the capacity of count k in a planned run, else n itself */
+inline int64_t sxg_plan_cap(const SxgState &S, int k, int64_t n)
+{
+    return S.plan ? S.plan->cap[k] : n;
+}
 /* Grow-only device buffers addressed by name (detection/deblending scratch). */
 DevBuf &sxg_buf(const char *name);
 void sxg_release_named();
 bool sxg_ensure_stream();   /* the current context's stream and events */
 /* batch workers: every launch of the context on its one stream */
 void sxg_ctx_single_stream(sxg_ctx *c);
+SxgState &sxg_ctx_state(sxg_ctx *c);   /* the state of a context */
 /* Typed view of a named buffer holding at least n elements (nullptr on failure). */
 template <class T> inline T *sxg_take(const char *name, size_t n)
 {

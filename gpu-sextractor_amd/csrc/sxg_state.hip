@@ -43,6 +43,17 @@ extern "C" long sxg_get_option(int id) { return id >= 0 && id < SXG_NOPT ? g_opt
 bool DevBuf::ensure(size_t want)
 {
     if (want <= bytes && ptr) return true;
+    {
+        SxgState &S = sxg_state();
+        /* a graph records the buffers' addresses: a planned run that is being captured
+           must find every buffer at its size already */
+        if (S.capturing) {
+            sxg_set_error("device buffer of " + std::to_string(want) +
+                          " bytes requested while a graph is captured (plan not warmed up)");
+            return false;
+        }
+        S.alloc_gen++;
+    }
     /* grow by half again at least, so that buffers sized by per-frame counts settle
        after a few frames */
     const size_t grown = ptr ? bytes + bytes / 2 : 0;
@@ -94,6 +105,8 @@ static SxgState &default_state()
 }
 
 SxgState &sxg_state() { return tl_ctx ? *tl_ctx : default_state(); }
+SxgState *sxg_bound_state() { return tl_ctx; }
+void sxg_bind_state(SxgState *s) { tl_ctx = s; }
 
 struct sxg_ctx {
     SxgState st;
@@ -125,6 +138,7 @@ extern "C" void sxg_ctx_destroy(sxg_ctx *c)
 }
 
 void sxg_ctx_single_stream(sxg_ctx *c) { c->st.single_stream = true; }
+SxgState &sxg_ctx_state(sxg_ctx *c) { return c->st; }
 
 extern "C" int sxg_ctx_bind(sxg_ctx *c)
 {
@@ -157,15 +171,19 @@ bool sxg_ensure_stream()
     return true;
 }
 
+/* the stage events and waits are left out of planned runs (a capture records no timing
+   and no host wait) */
 void sxg_stage_begin(StageId s)
 {
     SxgState &S = sxg_state();
+    if (S.plan) return;
     (void)hipEventRecord(S.ev[2 * s], S.stream);
 }
 
 void sxg_stage_end(StageId s)
 {
     SxgState &S = sxg_state();
+    if (S.plan) return;
     (void)hipEventRecord(S.ev[2 * s + 1], S.stream);
     S.ev_used[s] = true;
 }
@@ -187,7 +205,7 @@ extern "C" int sxg_stage_times(float *ms, int n)
 bool sxg_stage_wait()
 {
     SxgState &S = sxg_state();
-    if (S.is_default) return true;
+    if (S.is_default || S.plan) return true;
     SXG_HIP_RET(hipStreamSynchronize(S.stream), false);
     return true;
 }
@@ -264,9 +282,11 @@ __device__ inline void load16(const int *in, long base, int n, int *v)
     }
 }
 
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const int *in, int n, int *bsum)
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const int *in, int n, int *bsum,
+                                                            const int *nd)
 {
     __shared__ int sm[SCAN_BLOCK / 64];
+    n = sxg_dn(nd, n);
     const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS], s = 0;
     load16(in, base, n, v);
@@ -294,9 +314,10 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(int *bsum, int nb, int *to
 }
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const int *in, int *out, int n,
-                                                           const int *bsum)
+                                                           const int *bsum, const int *nd)
 {
     __shared__ int sm[SCAN_BLOCK / 64];
+    n = sxg_dn(nd, n);
     const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS], s = 0;
     load16(in, base, n, v);
@@ -315,10 +336,11 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const int *in, int *o
 /* n <= SCAN_SMALL: one workgroup; the values pass through LDS (coalesced loads and
    stores), each thread scans a contiguous run of at most 32 of them */
 __global__ void __launch_bounds__(SCAN_SMALL_THREADS) k_scan_small(const int *in, int *out, int n,
-                                                                   int *total)
+                                                                   int *total, const int *nd)
 {
     extern __shared__ int sv[];
     __shared__ int sm[SCAN_SMALL_THREADS / 64];
+    n = sxg_dn(nd, n);
 #pragma unroll 8
     for (int k = threadIdx.x; k < n; k += SCAN_SMALL_THREADS) sv[k] = in[k];
     __syncthreads();
@@ -339,12 +361,17 @@ __global__ void __launch_bounds__(SCAN_SMALL_THREADS) k_scan_small(const int *in
     if (threadIdx.x == 0 && total) *total = tot;
 }
 
+__global__ void k_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
 /* exclusive prefix sum of n ints (in == out allowed); the total to *total (device) */
-void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s)
+void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s, const int *nd)
 {
     SxgState &S = sxg_state();
     if (n <= 0) {
-        if (total) (void)hipMemsetAsync(total, 0, sizeof(int), s);
+        if (total) hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, total, 0);
         return;
     }
     if (n <= SCAN_SMALL) {
@@ -356,15 +383,16 @@ void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s)
         }();
         (void)attr;
         hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SCAN_SMALL_THREADS),
-                           (size_t)n * sizeof(int), s, in, out, n, total);
+                           (size_t)n * sizeof(int), s, in, out, n, total, nd);
         return;
     }
     const int nb = sxg_div_up(n, SCAN_TILE);
     if (!S.scan_tmp.ensure(sizeof(int) * (nb + 64))) return;
     int *bsum = S.scan_tmp.as<int>();
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, n, bsum);
+    /* with a device count the blocks past it sum to zero: the same offsets and total */
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, n, bsum, nd);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsum, nb, total);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, out, n, bsum);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, out, n, bsum, nd);
 }
 
 /* ------------------------------------------------------------ compaction ---- */
@@ -373,9 +401,11 @@ struct PredNonzero { __device__ bool operator()(int v) const { return v != 0; } 
 struct PredNonneg { __device__ bool operator()(int v) const { return v >= 0; } };
 
 template <class P>
-__global__ void __launch_bounds__(SCAN_BLOCK) k_compact_count(const int *flag, int n, int *bsum)
+__global__ void __launch_bounds__(SCAN_BLOCK) k_compact_count(const int *flag, int n, int *bsum,
+                                                              const int *nd)
 {
     __shared__ int sm[SCAN_BLOCK / 64];
+    n = sxg_dn(nd, n);
     const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS], s = 0;
     load16(flag, base, n, v);
@@ -388,9 +418,11 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_count(const int *flag, i
 
 template <class P>
 __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_write(const int *flag, int n,
-                                                              const int *bsum, int *out)
+                                                              const int *bsum, int *out,
+                                                              const int *nd)
 {
     __shared__ int sm[SCAN_BLOCK / 64];
+    n = sxg_dn(nd, n);
     const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS], s = 0;
     load16(flag, base, n, v);
@@ -409,29 +441,57 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_compact_write(const int *flag, i
 }
 
 template <class P>
-static void compact_impl(const int *flag, int *out, int n, int *count, hipStream_t s)
+static void compact_impl(const int *flag, int *out, int n, int *count, hipStream_t s,
+                         const int *nd)
 {
     SxgState &S = sxg_state();
     if (n <= 0) {
-        (void)hipMemsetAsync(count, 0, sizeof(int), s);
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, count, 0);
         return;
     }
     const int nb = sxg_div_up(n, SCAN_TILE);
     if (!S.scan_tmp.ensure(sizeof(int) * (nb + 64))) return;
     int *bsum = S.scan_tmp.as<int>();
-    hipLaunchKernelGGL(k_compact_count<P>, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum);
+    hipLaunchKernelGGL(k_compact_count<P>, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum, nd);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsum, nb, count);
-    hipLaunchKernelGGL(k_compact_write<P>, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum, out);
+    hipLaunchKernelGGL(k_compact_write<P>, dim3(nb), dim3(SCAN_BLOCK), 0, s, flag, n, bsum, out,
+                       nd);
 }
 
-void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s)
+void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s,
+                       const int *nd)
 {
-    compact_impl<PredNonzero>(flag, out, n, count, s);
+    compact_impl<PredNonzero>(flag, out, n, count, s, nd);
 }
 
-void sxg_compact_nonneg(const int *v, int *out, int n, int *count, hipStream_t s)
+void sxg_compact_nonneg(const int *v, int *out, int n, int *count, hipStream_t s, const int *nd)
 {
-    compact_impl<PredNonneg>(v, out, n, count, s);
+    compact_impl<PredNonneg>(v, out, n, count, s, nd);
+}
+
+/* ------------------------------------------------------------ plan guards ---- */
+__global__ void k_plan_guard(int *v, long long cap, int bit, PlanDev *pd, int *cc)
+{
+    if (threadIdx.x != 0) return;
+    const int x = *v;
+    if (x < 0 || (long long)x > cap) pd->ovf |= 1 << bit;
+    if (pd->ovf) {
+        /* the frame is run again eagerly: nothing after this point has work to do */
+        *v = 0;
+        pd->nfg = pd->nraw = pd->npix = pd->nobj = pd->nsurv = 0;
+        if (cc) {
+            for (int c = 0; c < 4; c++) cc[c] = 0;
+            cc[8] = 0;
+        }
+    }
+}
+
+void sxg_plan_guard(int *v, int64_t cap, int bit, int *ccount)
+{
+    SxgState &S = sxg_state();
+    if (!S.plan) return;
+    hipLaunchKernelGGL(k_plan_guard, dim3(1), dim3(64), 0, S.stream, v, (long long)cap, bit,
+                       S.pdev, ccount);
 }
 
 /* ------------------------------------------------------- named buffers ---- */

@@ -386,6 +386,10 @@ def gpu():
         L.sxg_set_wcs.argtypes = [vp]
         L.sxg_get_world.argtypes = [vp, vp, vp, vp, sz]
         L.sxg_batch_process_host.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.sxg_batch_process_graph.argtypes = [vp, i, vp, vp]
+        L.sxg_batch_set_plan_margin.argtypes = [i]
+        L.sxg_batch_graph_stats.argtypes = [vp, vp, vp]
+        L.sxg_batch_update_frame.argtypes = [i, vp]
         _gpu = L
     return _gpu
 
@@ -409,7 +413,8 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_set_photometry", "sxg_get_photometry", "sxg_set_weight_map",
             "sxg_get_back_rms", "sxg_back_estimate_noalloc", "sxg_get_catalog",
             "sxg_batch_process_host", "sxg_reference_layout", "sxg_set_wcs",
-            "sxg_get_world"]
+            "sxg_get_world", "sxg_batch_process_graph", "sxg_batch_set_plan_margin",
+            "sxg_batch_graph_stats", "sxg_batch_update_frame"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort",
@@ -936,9 +941,9 @@ def process_resident(img, p):
     return n, {k: getattr(fi, k) for k, _ in FrameInfo._fields_}
 
 
-def process_batch(frames, p, nworkers=4, segmaps=True):
-    """Run the whole path on a list of float32 frames (uneven sizes allowed) with
-    nworkers concurrent contexts; returns (infos, frame_ms, seg maps or None)."""
+def set_batch_frames(frames):
+    """Copy float32 frames (uneven sizes allowed) into device memory as the resident
+    batch (sxg_batch_set_frames)."""
     L = gpu()
     frames = [np.ascontiguousarray(f, dtype=np.float32) for f in frames]
     n = len(frames)
@@ -947,17 +952,53 @@ def process_batch(frames, p, nworkers=4, segmaps=True):
     hs = (C.c_int * n)(*[f.shape[0] for f in frames])
     if L.sxg_batch_set_frames(n, ptrs, ws, hs) != 0:
         check()
+    return frames
+
+
+def batch_segmaps(shapes):
+    """The segmentation maps of the resident batch's frames (their shapes given)."""
+    L = gpu()
+    segs = []
+    for i, shp in enumerate(shapes):
+        s = np.zeros(shp, np.int32)
+        if L.sxg_batch_segmap(i, s.ctypes.data) != 0:
+            check()
+        segs.append(s)
+    return segs
+
+
+def process_batch(frames, p, nworkers=4, segmaps=True):
+    """Run the whole path on a list of float32 frames (uneven sizes allowed) with
+    nworkers concurrent contexts; returns (infos, frame_ms, seg maps or None)."""
+    L = gpu()
+    frames = set_batch_frames(frames)
+    n = len(frames)
     sp = native_params(p)
     infos = (FrameInfo * n)()
     ms = (C.c_double * n)()
     if L.sxg_batch_process(C.byref(sp), nworkers, infos, ms) < 0:
         check()
-    segs = None
-    if segmaps:
-        segs = []
-        for i, f in enumerate(frames):
-            s = np.zeros(f.shape, np.int32)
-            if L.sxg_batch_segmap(i, s.ctypes.data) != 0:
-                check()
-            segs.append(s)
+    segs = batch_segmaps([f.shape for f in frames]) if segmaps else None
     return list(infos), list(ms), segs
+
+
+def process_batch_graph(n, shapes, p, nworkers=4, segmaps=True):
+    """One replay of the resident batch's graph (sxg_batch_process_graph; the frames set
+    with set_batch_frames); returns (infos, frame_ms, seg maps or None, frames run again
+    eagerly)."""
+    L = gpu()
+    sp = native_params(p)
+    infos = (FrameInfo * n)()
+    ms = (C.c_double * n)()
+    rerun = L.sxg_batch_process_graph(C.byref(sp), nworkers, infos, ms)
+    if rerun < 0:
+        check()
+    segs = batch_segmaps(shapes) if segmaps else None
+    return list(infos), list(ms), segs, rerun
+
+
+def graph_stats():
+    """(builds, launches, eager reruns) of the graph batch."""
+    b, l, r = C.c_long(), C.c_long(), C.c_long()
+    gpu().sxg_batch_graph_stats(C.byref(b), C.byref(l), C.byref(r))
+    return b.value, l.value, r.value

@@ -356,6 +356,25 @@ int sxg_batch_process_host(const sxg_params *p, int nworkers, int n, const float
                            const int *width, const int *height, int32_t *const *segs,
                            sxg_isorow *const *rows, const int *row_cap, sxg_frame_info *info,
                            double *frame_ms);
+/* The resident batch as one HIP graph: a branch per worker context (frames assigned
+   longest first to the least loaded branch), every frame a count-planned run whose
+   launches are sized by capacities from an eager calibration run of that frame (counts
+   plus the plan margin, sxg_batch_set_plan_margin), with no host synchronisation and no
+   allocation inside.  The graph is built on the first call, and again when the
+   parameters, nworkers or the frames change or a worker's buffers were reallocated; each
+   call replays it once.  A frame whose counts exceed its plan (e.g. new pixels loaded
+   with sxg_batch_update_frame) is run again eagerly on its context.  The results equal
+   sxg_batch_process's.  info[i] as there; frame_ms[i]: device time from frame i's first
+   kernel to its last (its branch's clock, wall_clock64).  Returns the number of frames
+   run again eagerly, or < 0. */
+int sxg_batch_process_graph(const sxg_params *p, int nworkers, sxg_frame_info *info,
+                            double *frame_ms);
+/* Plan margin of the graph batch in percent (default 25; 0..1000). */
+int sxg_batch_set_plan_margin(int percent);
+/* Graph builds, launches and eager reruns so far; returns 1 when a graph is built. */
+int sxg_batch_graph_stats(long *builds, long *launches, long *reruns);
+/* Replace the pixels of resident frame i (same size; the graph stays valid). */
+int sxg_batch_update_frame(int frame, const float *host);
 /* Segmentation map of frame i of the last sxg_batch_process. */
 int sxg_batch_segmap(int frame, int32_t *out);
 /* Free the batch and its worker contexts. */

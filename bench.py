@@ -9,9 +9,11 @@ BACK_FILTERSIZE 3, DETECT_THRESH 1.5, DETECT_MINAREA 5, DEBLEND_NTHRESH 32,
 DEBLEND_MINCONT 0.005, CLEAN Y with CLEAN_PARAM 1.0).
 
 A step is one batch of B such frames (distinct seeds), resident in HBM before the timed
-region, run by the batched path: W concurrent contexts (default 4), one host thread and
-one HIP stream each (sxg_batch_process).  value = Mpix/s of the batched path over the K timed
-steps.  Beside it: the same B frames run frame by frame (one context: the plain
+region, run by the batched path in its two forms: W concurrent contexts (default 4), one
+host thread and one HIP stream each (sxg_batch_process), and the same batch as one HIP
+graph with a branch per context (sxg_batch_process_graph: count-planned runs, built once
+before the timed region, replayed once per step).  value = Mpix/s of the faster of the two
+over the K timed steps (config.batched_path names it).  Beside it: the same B frames run frame by frame (one context: the plain
 drop-in sequence), and the single-frame loop on the default context, whose HIP-event
 kernel timers give the roofline entry for the kernel with the largest share.  The CPU
 baseline is the oracle (the sequential C restatement) timed on one host core on frame 0;
@@ -28,6 +30,9 @@ context, with the oracle's segmentation map compared at full size.
 5-20 components, SATUR_LEVEL 6e5); the GPU catalogue (the reference's arithmetic, bit-exact
 with the oracle) is compared per column with a long-double evaluation of the same
 quantities on the same pixels, and the maximum relative errors are reported.
+
+The default run also times the 64-frame set below briefly (frame by frame, the worker
+pool and the graph; config.batch64).
 
 --workload batch64 runs the many-frame workload instead: 64 frames whose sizes are drawn
 from {512^2, 1024^2, 2048x1024, 2048^2, 4096x2048} and whose source densities are
@@ -218,6 +223,29 @@ def timed_batches(L, sp, nworkers, steps, B):
     return dt, list(infos), list(ms)
 
 
+def timed_graph(L, sp, nworkers, steps, B):
+    """The graph batch (sxg_batch_process_graph): the first call builds the graph
+    (calibration, plans, warm-up, capture) and replays it once, untimed; then `steps`
+    timed replays.  Returns (build s, timed s, infos, per-frame device ms, eager reruns)."""
+    infos = (sexgpu.FrameInfo * B)()
+    ms = (C.c_double * B)()
+    t0 = time.perf_counter()
+    if L.sxg_batch_process_graph(C.byref(sp), nworkers, infos, ms) < 0:
+        sexgpu.check()
+    tb = time.perf_counter() - t0
+    reruns = 0
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = L.sxg_batch_process_graph(C.byref(sp), nworkers, infos, ms)
+        if r < 0:
+            sexgpu.check()
+        reruns += r
+        lat += list(ms)
+    dt = time.perf_counter() - t0
+    return tb, dt, list(infos), lat, reruns
+
+
 def batch_segs(L, frames):
     out = []
     for i, f in enumerate(frames):
@@ -263,22 +291,84 @@ def batch64(args):
                    "frame_ms_p99": round(float(np.percentile(lat, 99)), 3),
                    "ms_per_batch": round(1000.0 * dt / steps, 2), "same_as_first": bool(same)}
         print("# %d workers: %s" % (nw, res[nw]), file=sys.stderr, flush=True)
+    graph = graph_legs(L, sp, frames, ref, sweep, max(1, args.steps // 2))
     best = max(res, key=lambda k: res[k]["mpix_s"])
+    gbest = max(graph, key=lambda k: graph[k]["mpix_s"])
+    use_graph = graph[gbest]["mpix_s"] > res[best]["mpix_s"]
+    top = graph[gbest] if use_graph else res[best]
     plain = res.get(1)
     line = {
         "metric": "Mpix/s + objects/s, 64 uneven frames, batched vs plain; bit-exact seg maps",
-        "value": res[best]["mpix_s"], "unit": "Mpix/s", "n_gpus": 1,
-        "steps": max(1, args.steps // 2), "warmup": 1, "ms_per_step": res[best]["ms_per_batch"],
+        "value": top["mpix_s"], "unit": "Mpix/s", "n_gpus": 1,
+        "steps": max(1, args.steps // 2), "warmup": 1, "ms_per_step": top["ms_per_batch"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "64 frames of uneven sizes (512^2 .. 4096x2048), densities "
                                "log-U[1e-4,1e-2], default.conv; step = the 64-frame batch",
-                   "workers": best, "by_workers": res,
-                   "plain": plain},
+                   "path": "graph" if use_graph else "workers",
+                   "workers": gbest if use_graph else best, "by_workers": res,
+                   "graph_by_branches": graph, "plain": plain},
     }
     print(json.dumps(line), flush=True)
     L.sxg_batch_release()
     return 0
+
+
+def batch64_summary(L, args):
+    """The 64 uneven frames (the --workload batch64 set) in the default run, briefly: frame
+    by frame on one context, the worker pool and the graph with --workers contexts; Mpix/s,
+    per-frame p50/p99 and whether every map equals the frame-by-frame one."""
+    t = time.perf_counter()
+    frames = sexgpu.batch64_frames()
+    npx = sum(f.size for f in frames)
+    print("# batch64: %d frames, %.1f Mpix, generated in %.1f s" % (
+        len(frames), npx / 1e6, time.perf_counter() - t), file=sys.stderr, flush=True)
+    p = sexgpu.Params(mask=sexgpu.read_conv("default.conv"))
+    sp = sexgpu.native_params(p)
+    sexgpu.set_batch_frames(frames)
+    B = len(frames)
+    out = {}
+    steps = 2
+    for name, nw in (("frame_by_frame", 1), ("workers", args.workers)):
+        dt, infos, ms = timed_batches(L, sp, nw, steps, B)
+        segs = batch_segs(L, frames)
+        if name == "frame_by_frame":
+            ref = segs
+        lat = np.array(ms)
+        out[name] = {"contexts": nw, "mpix_s": round(npx * steps / dt / 1e6, 1),
+                     "frame_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                     "frame_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                     "same_as_frame_by_frame": all(np.array_equal(a, b) for a, b in zip(ref, segs))}
+    g = graph_legs(L, sp, frames, ref, [args.workers], 3)[args.workers]
+    g["contexts"] = args.workers
+    out["graph"] = g
+    out["graph_vs_frame_by_frame"] = round(g["mpix_s"] / out["frame_by_frame"]["mpix_s"], 2)
+    L.sxg_batch_release()
+    print("# batch64: %s" % out, file=sys.stderr, flush=True)
+    return out
+
+
+def graph_legs(L, sp, frames, ref, sweep, steps):
+    """The resident batch as one HIP graph, for each branch count of sweep: throughput,
+    per-frame device latency, build time, eager reruns, and the segmentation maps
+    compared with ref (the eager batch's)."""
+    npx = sum(f.size for f in frames)
+    B = len(frames)
+    out = {}
+    for nw in sweep:
+        tb, dt, infos, lat, reruns = timed_graph(L, sp, nw, steps, B)
+        segs = batch_segs(L, frames)
+        same = all(np.array_equal(a, b) for a, b in zip(ref, segs))
+        lat = np.array(lat)
+        out[nw] = {"mpix_s": round(npx * steps / dt / 1e6, 1),
+                   "objects_per_s": round(sum(f.nsurv for f in infos) * steps / dt, 1),
+                   "frame_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                   "frame_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                   "ms_per_batch": round(1000.0 * dt / steps, 2),
+                   "build_ms": round(1000.0 * tb, 1), "eager_reruns": int(reruns),
+                   "same_as_eager": bool(same)}
+        print("# graph, %d branches: %s" % (nw, out[nw]), file=sys.stderr, flush=True)
+    return out
 
 
 def wide8192(args):
@@ -636,6 +726,8 @@ def main():
                     help="first copy 256 MiB device to device (traffic counter calibration)")
     ap.add_argument("--no-end-to-end", action="store_true",
                     help="skip the PCIe-inclusive legs")
+    ap.add_argument("--no-batch64", action="store_true",
+                    help="skip the 64-frame legs of the default run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (a frame or a batch "
@@ -714,13 +806,26 @@ def main():
     segs_batch = batch_segs(L, frames)
     batch_same = all(np.array_equal(a, b) for a, b in zip(segs_plain, segs_batch))
     batch_same = batch_same and np.array_equal(segs_plain[0], seg0)
+    # the same batch as one HIP graph (a branch per worker context)
+    tgb, dtg, infg, latg, reruns = timed_graph(L, sp, args.workers, args.steps, B)
+    segs_graph = batch_segs(L, frames)
+    graph_same = all(np.array_equal(a, b) for a, b in zip(segs_plain, segs_graph))
     L.sxg_batch_release()
+    mpix_graph = B * N * args.steps / dtg / 1e6
+    print("# graph batch: %.1f Mpix/s, build %.0f ms, eager reruns %d, frame device time p50 "
+          "%.2f ms p99 %.2f ms, maps equal %s" % (mpix_graph, 1e3 * tgb, reruns,
+                                                   np.percentile(latg, 50), np.percentile(latg, 99),
+                                                   graph_same), file=sys.stderr, flush=True)
     nsurv = sum(f.nsurv for f in infb)
     mpix_plain = B * N * plain_steps / dtp / 1e6
+    mpix_workers = B * N * args.steps / dtb / 1e6
+    # the batched path's rate: the faster of the worker pool and the graph
+    use_graph = mpix_graph > mpix_workers
+    dtb = dtg if use_graph else dtb
     mpix_batch = B * N * args.steps / dtb / 1e6
     objs_batch = nsurv * args.steps / dtb
     objs_plain = nsurv * plain_steps / dtp
-    lat = np.array(msb)
+    lat = np.array(latg if use_graph else msb)
     print("# batch of %d: frame by frame %.1f Mpix/s, %d workers %.1f Mpix/s (%.2fx); frame "
           "latency p50 %.2f ms p99 %.2f ms" % (B, mpix_plain, args.workers, mpix_batch,
                                                  mpix_batch / mpix_plain, np.percentile(lat, 50),
@@ -728,6 +833,9 @@ def main():
 
     # ---- end to end from host memory
     e2e = None if args.no_end_to_end else end_to_end(L, frames, p, sp, args, seg0, segs_plain)
+
+    # ---- the 64 uneven frames: frame by frame, worker pool and graph
+    b64 = None if args.no_batch64 else batch64_summary(L, args)
 
     # ---- roofline of the dominant kernel (single-frame run)
     traffic = MEASURED_TRAFFIC_CROWDED4096 if args.size == 4096 else None
@@ -748,6 +856,10 @@ def main():
                                "gauss_3.0_7x7, resident in HBM; step = batch of %d frames on %d "
                                "concurrent contexts" % (W, H, nsrc, 1000 + B, B, args.workers),
                    "frame": [W, H], "batch": B, "workers": args.workers,
+                   "batched_path": "graph" if use_graph else "workers",
+                   "workers_mpix_s": round(mpix_workers, 2), "graph_mpix_s": round(mpix_graph, 2),
+                   "graph_build_ms": round(1e3 * tgb, 1), "graph_eager_reruns": int(reruns),
+                   "graph_equals_plain": bool(graph_same),
                    "objects_per_s": round(objs_batch, 1),
                    "plain_mpix_s": round(mpix_plain, 2), "plain_objects_per_s": round(objs_plain, 1),
                    "single_frame_ms": round(single_ms, 3),
@@ -756,7 +868,7 @@ def main():
                    "objects_per_frame": info["nsurv"], "deblended_per_frame": info["nobj"],
                    "pixels_above_threshold": info["npix_above"],
                    "whole_path_hbm_frac": round(balg / (dtb / args.steps / B) / 1e12 / 8.0, 5),
-                   "end_to_end": e2e,
+                   "end_to_end": e2e, "batch64": b64,
                    "parity_with_oracle": parity, "batched_equals_plain": bool(batch_same)},
         "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -292,6 +292,11 @@ def batch64(args):
                    "ms_per_batch": round(1000.0 * dt / steps, 2), "same_as_first": bool(same)}
         print("# %d workers: %s" % (nw, res[nw]), file=sys.stderr, flush=True)
     graph = graph_legs(L, sp, frames, ref, sweep, max(1, args.steps // 2))
+    # and with the autotuner's branches and deblending size classes
+    tune = sexgpu.autotune(p, max_branches=16, replays=2)
+    tg = graph_legs(L, sp, frames, ref, [tune["branches"]], max(1, args.steps // 2))
+    tg[tune["branches"]]["autotune"] = tune
+    graph["tuned"] = tg[tune["branches"]]
     best = max(res, key=lambda k: res[k]["mpix_s"])
     gbest = max(graph, key=lambda k: graph[k]["mpix_s"])
     use_graph = graph[gbest]["mpix_s"] > res[best]["mpix_s"]
@@ -306,7 +311,8 @@ def batch64(args):
         "config": {"workload": "64 frames of uneven sizes (512^2 .. 4096x2048), densities "
                                "log-U[1e-4,1e-2], default.conv; step = the 64-frame batch",
                    "path": "graph" if use_graph else "workers",
-                   "workers": gbest if use_graph else best, "by_workers": res,
+                   "workers": (graph[gbest].get("autotune", {}).get("branches", gbest)
+                               if use_graph else best), "by_workers": res,
                    "graph_by_branches": graph, "plain": plain},
     }
     print(json.dumps(line), flush=True)
@@ -339,8 +345,10 @@ def batch64_summary(L, args):
                      "frame_ms_p50": round(float(np.percentile(lat, 50)), 3),
                      "frame_ms_p99": round(float(np.percentile(lat, 99)), 3),
                      "same_as_frame_by_frame": all(np.array_equal(a, b) for a, b in zip(ref, segs))}
-    g = graph_legs(L, sp, frames, ref, [args.workers], 3)[args.workers]
-    g["contexts"] = args.workers
+    tune = sexgpu.autotune(p, max_branches=8, replays=2)
+    g = graph_legs(L, sp, frames, ref, [tune["branches"]], 3)[tune["branches"]]
+    g["contexts"] = tune["branches"]
+    g["autotune"] = tune
     out["graph"] = g
     out["graph_vs_frame_by_frame"] = round(g["mpix_s"] / out["frame_by_frame"]["mpix_s"], 2)
     L.sxg_batch_release()
@@ -806,8 +814,13 @@ def main():
     segs_batch = batch_segs(L, frames)
     batch_same = all(np.array_equal(a, b) for a, b in zip(segs_plain, segs_batch))
     batch_same = batch_same and np.array_equal(segs_plain[0], seg0)
-    # the same batch as one HIP graph (a branch per worker context)
-    tgb, dtg, infg, latg, reruns = timed_graph(L, sp, args.workers, args.steps, B)
+    # the same batch as one HIP graph, its branches and deblending size classes chosen by
+    # the autotuner (results checked equal across its candidates)
+    t = time.perf_counter()
+    tune = sexgpu.autotune(p, max_branches=min(8, B), replays=2)
+    tune["tune_s"] = round(time.perf_counter() - t, 2)
+    print("# autotune: %s" % tune, file=sys.stderr, flush=True)
+    tgb, dtg, infg, latg, reruns = timed_graph(L, sp, tune["branches"], args.steps, B)
     segs_graph = batch_segs(L, frames)
     graph_same = all(np.array_equal(a, b) for a, b in zip(segs_plain, segs_graph))
     L.sxg_batch_release()
@@ -859,7 +872,7 @@ def main():
                    "batched_path": "graph" if use_graph else "workers",
                    "workers_mpix_s": round(mpix_workers, 2), "graph_mpix_s": round(mpix_graph, 2),
                    "graph_build_ms": round(1e3 * tgb, 1), "graph_eager_reruns": int(reruns),
-                   "graph_equals_plain": bool(graph_same),
+                   "graph_equals_plain": bool(graph_same), "autotune": tune,
                    "objects_per_s": round(objs_batch, 1),
                    "plain_mpix_s": round(mpix_plain, 2), "plain_objects_per_s": round(objs_plain, 1),
                    "single_frame_ms": round(single_ms, 3),

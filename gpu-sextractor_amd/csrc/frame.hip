@@ -15,9 +15,12 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "sxg_internal.h"
@@ -816,6 +819,178 @@ extern "C" int sxg_batch_update_frame(int frame, const float *host)
                           hipMemcpyHostToDevice), -1);
     return 0;
 }
+
+/* -------------------------------------------------------------- autotuner ---- */
+/* For the resident batch, the number of graph branches and the deblending size classes
+   (the options SXG_OPT_DEB_LDS_TINY/SMALL/LARGE, which route the multi-peak raw objects to
+   256-lane LDS, 1024-lane LDS or global-memory trees) that replay fastest, keyed by the
+   batch's mean frame size, detected-pixel density and frame count.  Coordinate descent:
+   the branch counts with the current classes, then the class candidates with the best
+   branch count; each candidate is a graph build and `replays` timed replays.  Every
+   candidate's results are compared with the first one's (each frame's counts and a hash
+   of its segmentation map): a difference is an error, since the options only choose
+   kernels. */
+__global__ void k_seg_hash(const int *seg, long n, unsigned long long *out)
+{
+    unsigned long long h = 0;
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (long)gridDim.x * blockDim.x) {
+        /* an order-free sum of per-pixel mixes of (position, label) */
+        unsigned long long x = ((unsigned long long)p << 32) ^ (unsigned)seg[p];
+        x ^= x >> 33;
+        x *= 0xff51afd7ed558ccdull;
+        x ^= x >> 33;
+        x *= 0xc4ceb9fe1a85ec53ull;
+        x ^= x >> 33;
+        h += x;
+    }
+    for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, h);
+}
+
+struct TuneKey {
+    int size_bucket, density_bucket, count_bucket;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(size_bucket, density_bucket, count_bucket) <
+               std::tie(o.size_bucket, o.density_bucket, o.count_bucket);
+    }
+};
+static std::map<TuneKey, sxg_tune_choice> g_tune_table;
+
+static int ilog2(double v) { return v >= 1.0 ? (int)std::floor(std::log2(v)) : 0; }
+
+/* the results a candidate must reproduce: per frame its counts and its map's hash */
+static bool batch_signature(const sxg_frame_info *info, int n, std::vector<uint64_t> &sig)
+{
+    DevBuf h;
+    if (!h.ensure(sizeof(unsigned long long) * (n > 0 ? n : 1))) return false;
+    SXG_HIP_RET(hipMemset(h.ptr, 0, sizeof(unsigned long long) * (n > 0 ? n : 1)), false);
+    for (int i = 0; i < n; i++) {
+        const BatchFrame &f = g_batch[i];
+        hipLaunchKernelGGL(k_seg_hash, dim3(256), dim3(256), 0, 0, f.seg.as<int>(),
+                           (long)f.W * f.H, h.as<unsigned long long>() + i);
+    }
+    std::vector<unsigned long long> hh(n);
+    const hipError_t e = hipMemcpy(hh.data(), h.ptr, sizeof(unsigned long long) * n,
+                                   hipMemcpyDeviceToHost);
+    h.release();
+    SXG_HIP_RET(e, false);
+    sig.clear();
+    for (int i = 0; i < n; i++) {
+        sig.push_back(hh[i]);
+        sig.push_back((uint64_t)info[i].nraw << 32 | (uint32_t)info[i].nobj);
+        sig.push_back((uint64_t)info[i].nsurv << 32 | (uint32_t)info[i].npix_raw);
+        sig.push_back((uint64_t)info[i].deb_labels);
+    }
+    return true;
+}
+
+extern "C" int sxg_batch_autotune(const sxg_params *p, int max_branches, int replays,
+                                  sxg_tune_choice *out)
+{
+    sxg_clear_error();
+    const int n = (int)g_batch.size();
+    if (!p || !out || n == 0 || max_branches < 1 || max_branches > SXG_MAX_WORKERS ||
+        replays < 1) {
+        sxg_set_error("sxg_batch_autotune: parameters, an output, a resident batch, "
+                      "1..16 branches and at least one replay are needed");
+        return -1;
+    }
+    /* the key: mean frame size, detected pixels per pixel (one eager pass), frame count */
+    std::vector<int64_t> nfg(n, 0);
+    auto probe = [&](int i) {
+        BatchFrame &f = g_batch[i];
+        if (process_frame(f.frame.as<float>(), f.W, f.H, p, nullptr) < 0) return false;
+        nfg[i] = sxg_state().seen.cap[PL_NFG];
+        return hipStreamSynchronize(sxg_state().stream) == hipSuccess;
+    };
+    if (run_pool(n, std::min(max_branches, n), probe, nullptr, "sxg_batch_autotune") < 0)
+        return -1;
+    double pix = 0.0, det = 0.0;
+    for (int i = 0; i < n; i++) {
+        pix += (double)g_batch[i].W * g_batch[i].H;
+        det += (double)nfg[i];
+    }
+    const TuneKey key{ilog2(pix / n), ilog2(1e4 * det / pix), ilog2((double)n)};
+    auto hit = g_tune_table.find(key);
+    if (hit != g_tune_table.end()) {
+        *out = hit->second;
+        out->candidates = 0;
+        sxg_set_option(SXG_OPT_DEB_LDS_TINY, out->deb_lds_tiny);
+        sxg_set_option(SXG_OPT_DEB_LDS_SMALL, out->deb_lds_small);
+        sxg_set_option(SXG_OPT_DEB_LDS_LARGE, out->deb_lds_large);
+        return 0;
+    }
+    std::vector<sxg_frame_info> info(n);
+    std::vector<uint64_t> ref, sig;
+    int tried = 0;
+    /* the replay time of one candidate (ms per batch), its results checked */
+    auto time_candidate = [&](int branches, const long cls[3], double &ms) -> bool {
+        sxg_set_option(SXG_OPT_DEB_LDS_TINY, cls[0]);
+        sxg_set_option(SXG_OPT_DEB_LDS_SMALL, cls[1]);
+        sxg_set_option(SXG_OPT_DEB_LDS_LARGE, cls[2]);
+        /* the first call builds the graph for this candidate */
+        if (sxg_batch_process_graph(p, branches, info.data(), nullptr) < 0) return false;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < replays; r++)
+            if (sxg_batch_process_graph(p, branches, info.data(), nullptr) < 0) return false;
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                 .count() / replays;
+        if (!batch_signature(info.data(), n, sig)) return false;
+        if (ref.empty()) {
+            ref = sig;
+        } else if (sig != ref) {
+            sxg_set_error("sxg_batch_autotune: a candidate setting changed a result "
+                          "(internal error)");
+            return false;
+        }
+        tried++;
+        return true;
+    };
+    const long cur[3] = {sxg_option(SXG_OPT_DEB_LDS_TINY), sxg_option(SXG_OPT_DEB_LDS_SMALL),
+                         sxg_option(SXG_OPT_DEB_LDS_LARGE)};
+    sxg_tune_choice best{};
+    best.ms_per_batch = 1e300;
+    const int bc[] = {1, 2, 4, 8, 16};
+    for (int b : bc) {
+        if (b > max_branches || b > n) continue;
+        double ms;
+        if (!time_candidate(b, cur, ms)) return -1;
+        if (ms < best.ms_per_batch) {
+            best.ms_per_batch = ms;
+            best.branches = b;
+        }
+    }
+    /* size classes: tiny <= small <= large <= 6400 pixels */
+    static const long cls_cand[][3] = {{512, 1024, 6144}, {256, 1024, 6144}, {1024, 2048, 6400},
+                                       {512, 2048, 4096}, {256, 512, 6400}};
+    long bestc[3] = {cur[0], cur[1], cur[2]};
+    for (const auto &c : cls_cand) {
+        if (c[0] == cur[0] && c[1] == cur[1] && c[2] == cur[2]) continue;
+        double ms;
+        if (!time_candidate(best.branches, c, ms)) return -1;
+        if (ms < best.ms_per_batch) {
+            best.ms_per_batch = ms;
+            for (int k = 0; k < 3; k++) bestc[k] = c[k];
+        }
+    }
+    best.deb_lds_tiny = bestc[0];
+    best.deb_lds_small = bestc[1];
+    best.deb_lds_large = bestc[2];
+    best.candidates = tried;
+    best.size_bucket = key.size_bucket;
+    best.density_bucket = key.density_bucket;
+    best.count_bucket = key.count_bucket;
+    sxg_set_option(SXG_OPT_DEB_LDS_TINY, bestc[0]);
+    sxg_set_option(SXG_OPT_DEB_LDS_SMALL, bestc[1]);
+    sxg_set_option(SXG_OPT_DEB_LDS_LARGE, bestc[2]);
+    g_tune_table[key] = best;
+    *out = best;
+    return 0;
+}
+
+extern "C" void sxg_autotune_forget(void) { g_tune_table.clear(); }
 
 extern "C" int sxg_batch_segmap(int frame, int32_t *out)
 {

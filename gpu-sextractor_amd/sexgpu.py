@@ -390,6 +390,8 @@ def gpu():
         L.sxg_batch_set_plan_margin.argtypes = [i]
         L.sxg_batch_graph_stats.argtypes = [vp, vp, vp]
         L.sxg_batch_update_frame.argtypes = [i, vp]
+        L.sxg_batch_autotune.argtypes = [vp, i, i, C.POINTER(TuneChoice)]
+        L.sxg_autotune_forget.argtypes = []
         _gpu = L
     return _gpu
 
@@ -414,7 +416,8 @@ EXPORTED = ["init_device", "back_estimate", "init_detection", "cudaFilter", "run
             "sxg_get_back_rms", "sxg_back_estimate_noalloc", "sxg_get_catalog",
             "sxg_batch_process_host", "sxg_reference_layout", "sxg_set_wcs",
             "sxg_get_world", "sxg_batch_process_graph", "sxg_batch_set_plan_margin",
-            "sxg_batch_graph_stats", "sxg_batch_update_frame"]
+            "sxg_batch_graph_stats", "sxg_batch_update_frame", "sxg_batch_autotune",
+            "sxg_autotune_forget"]
 KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", "conv",
            "det_init", "det_fg", "det_union", "det_flatten", "det_stat", "det_sort",
            "deb_thresh", "deb_levels", "deb_objects", "analyse", "clean", "seg", "deb_lmax", "obj_sort",
@@ -995,6 +998,24 @@ def process_batch_graph(n, shapes, p, nworkers=4, segmaps=True):
         check()
     segs = batch_segmaps(shapes) if segmaps else None
     return list(infos), list(ms), segs, rerun
+
+
+class TuneChoice(C.Structure):
+    """sxg_tune_choice"""
+    _fields_ = [("branches", C.c_int), ("deb_lds_tiny", C.c_long), ("deb_lds_small", C.c_long),
+                ("deb_lds_large", C.c_long), ("ms_per_batch", C.c_double),
+                ("candidates", C.c_int), ("size_bucket", C.c_int), ("density_bucket", C.c_int),
+                ("count_bucket", C.c_int)]
+
+
+def autotune(p, max_branches=8, replays=2):
+    """sxg_batch_autotune on the resident batch: the chosen graph branches and deblending
+    size classes (set as the options), as a dict."""
+    ch = TuneChoice()
+    if gpu().sxg_batch_autotune(C.byref(native_params(p)), max_branches, replays,
+                                C.byref(ch)) != 0:
+        check()
+    return {k: getattr(ch, k) for k, _ in TuneChoice._fields_}
 
 
 def graph_stats():

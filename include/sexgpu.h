@@ -375,6 +375,26 @@ int sxg_batch_set_plan_margin(int percent);
 int sxg_batch_graph_stats(long *builds, long *launches, long *reruns);
 /* Replace the pixels of resident frame i (same size; the graph stays valid). */
 int sxg_batch_update_frame(int frame, const float *host);
+/* Autotuner of the graph batch (no counterpart in the reference): the number of graph
+   branches (1, 2, 4, 8, 16 up to max_branches) and the deblending size classes
+   (SXG_OPT_DEB_LDS_TINY/SMALL/LARGE) that replay the resident batch fastest, by
+   coordinate descent, each candidate timed over `replays` graph replays.  Choices are
+   kept in a table keyed by the batch's mean frame size, detected-pixel density and frame
+   count (log2 buckets): a batch with a known key takes the stored choice at once
+   (candidates = 0).  The chosen classes are set as the options; pass `branches` to
+   sxg_batch_process_graph.  Every candidate's results (each frame's counts and a hash of
+   its segmentation map) must equal the first one's, else an error is returned.
+   Returns 0 or < 0. */
+typedef struct {
+    int branches;
+    long deb_lds_tiny, deb_lds_small, deb_lds_large;
+    double ms_per_batch;    /* the chosen setting's replay time when it was tuned */
+    int candidates;         /* settings timed by this call */
+    int size_bucket, density_bucket, count_bucket;   /* the table key */
+} sxg_tune_choice;
+int sxg_batch_autotune(const sxg_params *p, int max_branches, int replays, sxg_tune_choice *out);
+/* Empty the autotuner's table. */
+void sxg_autotune_forget(void);
 /* Segmentation map of frame i of the last sxg_batch_process. */
 int sxg_batch_segmap(int frame, int32_t *out);
 /* Free the batch and its worker contexts. */

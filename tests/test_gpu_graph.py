@@ -2,7 +2,8 @@
 graph of count-planned runs give, frame by frame, the eager batch's frame values and
 segmentation maps and the oracle's; new pixels within the plans replay the same graph,
 and a frame past its plan runs again eagerly with the same results, after which the
-graph is planned again."""
+graph is planned again; the autotuner's choice of branches and deblending size classes
+leaves every result as it is."""
 import ctypes as C
 
 import numpy as np
@@ -119,3 +120,40 @@ def test_graph_rejects_bad_arguments():
     assert L.sxg_batch_process_graph(None, 2, None, None) < 0
     assert L.sxg_batch_set_plan_margin(-1) < 0
     assert L.sxg_batch_update_frame(10 ** 6, None) < 0
+
+
+def test_autotune_choice_never_changes_results(masks):
+    """The autotuner times graph branches and deblending size classes on the resident
+    batch (checking that every candidate gives the same counts and maps); replaying with
+    its choice gives the eager batch's results, and the same batch again hits its table."""
+    m = masks["default"]
+    p = sexgpu.Params(mask=m)
+    L = sexgpu.gpu()
+    frames = _frames(seed=3)[:4] + [sexgpu.synth("crowded4096", width=1024, height=1024,
+                                                 nsrc=167000 // 16, seed=7100)]
+    shapes = [f.shape for f in frames]
+    keep = {k: sexgpu.get_option(k) for k in ("deb_lds_tiny", "deb_lds_small", "deb_lds_large")}
+    L.sxg_autotune_forget()
+    try:
+        infos_e, _, segs_e = sexgpu.process_batch(frames, p, nworkers=2)
+        ch = sexgpu.autotune(p, max_branches=4, replays=1)
+        print("autotune:", ch, flush=True)
+        assert ch["candidates"] >= 3 and ch["branches"] in (1, 2, 4)
+        infos_g, _, segs_g, rerun = sexgpu.process_batch_graph(len(frames), shapes, p,
+                                                               nworkers=ch["branches"])
+        assert rerun == 0
+        for i in range(len(frames)):
+            # the pixels per size class are the work split the classes choose, not a result
+            a, b = _info(infos_g[i]), _info(infos_e[i])
+            assert sum(a.pop("deb_class_pixels")) == sum(b.pop("deb_class_pixels")), i
+            assert a == b, i
+            assert np.array_equal(segs_g[i], segs_e[i]), i
+        again = sexgpu.autotune(p, max_branches=4, replays=1)
+        assert again["candidates"] == 0
+        assert {k: again[k] for k in ("branches", "deb_lds_tiny", "deb_lds_small",
+                                      "deb_lds_large")} == \
+            {k: ch[k] for k in ("branches", "deb_lds_tiny", "deb_lds_small", "deb_lds_large")}
+    finally:
+        for k, v in keep.items():
+            sexgpu.set_option(k, v)
+        L.sxg_autotune_forget()

@@ -1886,7 +1886,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         return -1;
     DebArgs a{};
     a.fpia = S.fpia; a.rawkey = S.fpia_keys; a.off = ro.off; a.cnt = ro.count; a.first = ro.first;
-    a.cd = S.cd.as<float>(); a.lab = S.lab.as<int>();
+    a.cd = S.cd.as<float>(); a.lab = S.aux.as<int>();   /* the detection labels */
     a.kcomp = kcomp; a.P0 = P0; a.thr = sxg_take<float>("deb.thr", nr * n_thresh);
     a.nthresh = n_thresh; a.debarea = deb_minarea; a.mincont = deblend_mincont;
     a.lvmin = (int)std::min<long>(sxg_option(SXG_OPT_DEB_GLB_LEVELS), 0x7fffffffL);

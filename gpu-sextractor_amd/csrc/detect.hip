@@ -34,9 +34,11 @@ __device__ inline float det_thresh(const WeightArgs &wa, float t, long p)
 
 /* union-find parent of each pixel: -1 for the background; a detected pixel points at the
    first pixel of its run (the detected pixels before it on its row, inside its aligned
-   64-pixel window: a ballot), so rows never become chains */
+   64-pixel window: a ballot), so rows never become chains.  After k_det_flatten the same
+   plane is the label map: the root of each detected pixel, -1 elsewhere (one plane
+   written per pixel, not two) */
 __global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, float t, long n,
-                                                  int W, int *par, int *lab, WeightArgs wa,
+                                                  int W, int *par, WeightArgs wa,
                                                   const float *td)
 {
     if (td) t = *td;
@@ -49,7 +51,6 @@ __global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, 
     const bool start = fg && (x == 0 || lane == 0 || !((F >> (lane - 1)) & 1ull));
     const unsigned long long S = __ballot(start);
     if (!live) return;
-    lab[p] = -1;
     if (!fg) {
         par[p] = -1;
         return;
@@ -116,14 +117,18 @@ __global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, i
     }
 }
 
+/* full path compression: every detected pixel's parent becomes its root (a valid parent
+   while other lanes still walk: the root is an ancestor with the smallest index, so the
+   path halving's atomicMin never moves a pointer past it), and roots remember their
+   foreground position */
 __global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg, int nfg, int *par,
-                                                     int *lab, int *rootpos, const int *nd)
+                                                     int *rootpos, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= sxg_dn(nd, nfg)) return;
     const int p = fg[k];
     const int r = uf_root(par, p);
-    lab[p] = r;
+    par[p] = r;
     if (r == p) rootpos[p] = k;
 }
 
@@ -300,7 +305,8 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
                            S.img.as<float>(), S.cd.as<float>(), N);
     }
     sxg_stage_begin(ST_DETECT);
-    int *par = S.aux.as<int>(), *lab = S.lab.as<int>();
+    /* par: the union-find parents, then (after k_det_flatten) the labels */
+    int *par = S.aux.as<int>(), *lab = par;
     int *rootpos = sxg_take<int>("det.rootpos", S.N);
     int *cnt_d = sxg_take<int>("det.counts", 8);
     if (!rootpos || !cnt_d) return;
@@ -314,7 +320,7 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     WeightArgs wa;
     if (!sxg_weight_args(wa)) return;
     hipLaunchKernelGGL(k_det_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
-                       S.cd.as<float>(), dthreshold, N, W, par, lab, wa, td);
+                       S.cd.as<float>(), dthreshold, N, W, par, wa, td);
     sxg_kt_end(SXG_KT_DET_INIT);
     /* the foreground list has at most N entries; size it after counting */
     int *fg = sxg_take<int>("det.fg", S.N);
@@ -351,7 +357,7 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         hipLaunchKernelGGL(k_det_union, dim3(gfg), dim3(256), 0, st, fg, nfg, W, par, nfg_k);
         sxg_kt_end(SXG_KT_DET_UNION);
         sxg_kt_begin(SXG_KT_DET_FLATTEN);
-        hipLaunchKernelGGL(k_det_flatten, dim3(gfg), dim3(256), 0, st, fg, nfg, par, lab, rootpos,
+        hipLaunchKernelGGL(k_det_flatten, dim3(gfg), dim3(256), 0, st, fg, nfg, par, rootpos,
                            nfg_k);
         sxg_kt_end(SXG_KT_DET_FLATTEN);
         hipLaunchKernelGGL(k_det_statinit, dim3(gfg), dim3(256), 0, st, rs, nfg, nfg_k);

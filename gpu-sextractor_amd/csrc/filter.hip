@@ -24,6 +24,21 @@
 #define CV_R 8                       /* output rows per lane */
 #define CV_THREADS (CV_TX * CV_TY / CV_R)
 
+/* The tile of this workgroup.  Workgroups go to the eight XCDs round robin by linear
+   index, so the linear index is mapped to tiles such that each XCD takes one run of
+   consecutive tiles in raster order: neighbouring tiles, which read each other's halo
+   columns and rows, then share them in that XCD's L2 instead of each XCD fetching them
+   from HBM.  A bijection on the tiles (the first n % 8 XCDs take one tile more). */
+__device__ inline void cv_tile(int &X0, int &Y0)
+{
+    const int gx = gridDim.x, n = gx * gridDim.y;
+    const int b = blockIdx.y * gx + blockIdx.x;
+    const int q = n >> 3, r = n & 7, xcd = b & 7, slot = b >> 3;
+    const int t = xcd * q + min(xcd, r) + slot;
+    X0 = (t % gx) * CV_TX;
+    Y0 = (t / gx) * CV_TY;
+}
+
 template <int K>
 __global__ void __launch_bounds__(CV_THREADS) k_convolve(const float *__restrict__ in,
                                                          float *__restrict__ out, int W, int H,
@@ -32,7 +47,8 @@ __global__ void __launch_bounds__(CV_THREADS) k_convolve(const float *__restrict
     constexpr int HW = K / 2;
     constexpr int LW = CV_TX + K - 1, LH = CV_TY + K - 1;
     __shared__ float tile[LH][LW + 1];
-    const int X0 = blockIdx.x * CV_TX, Y0 = blockIdx.y * CV_TY;
+    int X0, Y0;
+    cv_tile(X0, Y0);
     for (int k = threadIdx.x; k < LW * LH; k += CV_THREADS) {
         const int ly = k / LW, lx = k - ly * LW;
         const int gx = X0 + lx - HW, gy = Y0 + ly - HW;
@@ -79,7 +95,8 @@ __global__ void __launch_bounds__(CV_THREADS) k_convolve_any(const float *__rest
 {
     extern __shared__ float tile[];
     const int HW = K / 2, LW = CV_TX + K - 1, LH = CV_TY + K - 1;
-    const int X0 = blockIdx.x * CV_TX, Y0 = blockIdx.y * CV_TY;
+    int X0, Y0;
+    cv_tile(X0, Y0);
     for (int k = threadIdx.x; k < LW * LH; k += CV_THREADS) {
         const int ly = k / LW, lx = k - ly * LW;
         const int gx = X0 + lx - HW, gy = Y0 + ly - HW;

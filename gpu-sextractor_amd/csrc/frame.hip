@@ -209,7 +209,7 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
         S.deb_ready = false;
         for (int i = 0; i < ST_N; i++) S.ev_used[i] = false;
         if (!S.img.ensure(N * sizeof(float)) || !S.cd.ensure(N * sizeof(float)) ||
-            !S.lab.ensure(N * sizeof(int)) || !S.aux.ensure(N * sizeof(int)))
+            !S.aux.ensure(N * sizeof(int)))
             return -1;
         S.have_frame = true;
     }

@@ -180,8 +180,7 @@ struct SxgState {
     /* frame buffers */
     DevBuf img;      /* float[N]: input, background-subtracted in place */
     DevBuf cd;       /* float[N]: filtered (detection) frame */
-    DevBuf lab;      /* int[N]: labels, then the per-pixel object index */
-    DevBuf aux;      /* int[N]: union-find parents / counters */
+    DevBuf aux;      /* int[N]: union-find parents, then the detection labels */
 
     /* background */
     int bs = 0, nbx = 0, nby = 0;

@@ -214,7 +214,7 @@ extern "C" void sxg_release(void)
 {
     SxgState &S = sxg_state();
     if (S.stream) (void)hipStreamSynchronize(S.stream);
-    DevBuf *bufs[] = {&S.img, &S.cd, &S.lab, &S.aux, &S.backbuf, &S.scan_tmp, &S.thr, &S.lvl,
+    DevBuf *bufs[] = {&S.img, &S.cd, &S.aux, &S.backbuf, &S.scan_tmp, &S.thr, &S.lvl,
                       &S.comps, &S.objs, &S.rootobj};
     for (DevBuf *b : bufs) b->release();
     sxg_release_named();
@@ -543,7 +543,7 @@ extern "C" void init_device(int width, int height, float *imgbuf)
     S.filtered = false;
     for (int i = 0; i < ST_N; i++) S.ev_used[i] = false;
     if (!S.img.ensure(S.N * sizeof(float)) || !S.cd.ensure(S.N * sizeof(float)) ||
-        !S.lab.ensure(S.N * sizeof(int)) || !S.aux.ensure(S.N * sizeof(int)))
+        !S.aux.ensure(S.N * sizeof(int)))
         return;
     SXG_HIP(hipMemcpyAsync(S.img.ptr, imgbuf, S.N * sizeof(float), hipMemcpyHostToDevice,
                            S.stream));

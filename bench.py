@@ -32,7 +32,8 @@ with the oracle) is compared per column with a long-double evaluation of the sam
 quantities on the same pixels, and the maximum relative errors are reported.
 
 The default run also times the 64-frame set below briefly (frame by frame, the worker
-pool and the graph; config.batch64).
+pool and the graph; config.batch64), and four dense frames (--workload dense4096's
+generator: one frame, frame by frame and the graph; config.dense4096).
 
 --workload batch64 runs the many-frame workload instead: 64 frames whose sizes are drawn
 from {512^2, 1024^2, 2048x1024, 2048^2, 4096x2048} and whose source densities are
@@ -353,6 +354,50 @@ def batch64_summary(L, args):
     out["graph_vs_frame_by_frame"] = round(g["mpix_s"] / out["frame_by_frame"]["mpix_s"], 2)
     L.sxg_batch_release()
     print("# batch64: %s" % out, file=sys.stderr, flush=True)
+    return out
+
+
+def dense_summary(L, args, nframes=4):
+    """The dense regime in the default run, briefly: the crowded generator with fluxes from
+    3000 ADU (7.6 M pixels above the threshold, as the paper's 4096^2 image had 7.8 M),
+    nframes frames (seeds 1001..): one frame on the default context, then the batch frame
+    by frame and as a graph (autotuned branches), every map compared with the eager
+    batch's."""
+    t = time.perf_counter()
+    frames = [sexgpu.synth("crowded4096", fmin=3000.0, seed=1001 + i) for i in range(nframes)]
+    print("# dense: %d frames generated in %.1f s" % (nframes, time.perf_counter() - t),
+          file=sys.stderr, flush=True)
+    H, W = frames[0].shape
+    N = W * H
+    p = workload_params()
+    sp = sexgpu.native_params(p)
+    L.sxg_set_resident_frame(frames[0].ctypes.data, W, H)
+    fi = sexgpu.FrameInfo()
+    L.sxg_process_resident(C.byref(sp), C.byref(fi))
+    sexgpu.check()
+    L.sxg_sync()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        L.sxg_process_resident(C.byref(sp), C.byref(fi))
+    L.sxg_sync()
+    single_ms = 1000.0 * (time.perf_counter() - t0) / 2
+    sexgpu.check()
+    sexgpu.set_batch_frames(frames)
+    dtp, infp, _ = timed_batches(L, sp, 1, 1, nframes)
+    ref = batch_segs(L, frames)
+    tune = sexgpu.autotune(p, max_branches=nframes, replays=1)
+    g = graph_legs(L, sp, frames, ref, [tune["branches"]], 2)[tune["branches"]]
+    L.sxg_batch_release()
+    nsurv = sum(f.nsurv for f in infp)
+    out = {"single_frame_ms": round(single_ms, 3),
+           "objects_per_frame": fi.nsurv, "deblended_per_frame": fi.nobj,
+           "pixels_above_threshold": fi.npix_above,
+           "frame_by_frame_mpix_s": round(nframes * N / dtp / 1e6, 1),
+           "frame_by_frame_objects_per_s": round(nsurv / dtp, 1),
+           "graph_mpix_s": g["mpix_s"], "graph_objects_per_s": g["objects_per_s"],
+           "graph_branches": tune["branches"], "graph_frame_ms_p50": g["frame_ms_p50"],
+           "graph_frame_ms_p99": g["frame_ms_p99"], "graph_equals_eager": g["same_as_eager"]}
+    print("# dense: %s" % out, file=sys.stderr, flush=True)
     return out
 
 
@@ -736,6 +781,8 @@ def main():
                     help="skip the PCIe-inclusive legs")
     ap.add_argument("--no-batch64", action="store_true",
                     help="skip the 64-frame legs of the default run")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the dense-frame legs of the default run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (a frame or a batch "
@@ -849,6 +896,8 @@ def main():
 
     # ---- the 64 uneven frames: frame by frame, worker pool and graph
     b64 = None if args.no_batch64 else batch64_summary(L, args)
+    # ---- the dense regime
+    dense = None if args.no_dense else dense_summary(L, args)
 
     # ---- roofline of the dominant kernel (single-frame run)
     traffic = MEASURED_TRAFFIC_CROWDED4096 if args.size == 4096 else None
@@ -881,7 +930,7 @@ def main():
                    "objects_per_frame": info["nsurv"], "deblended_per_frame": info["nobj"],
                    "pixels_above_threshold": info["npix_above"],
                    "whole_path_hbm_frac": round(balg / (dtb / args.steps / B) / 1e12 / 8.0, 5),
-                   "end_to_end": e2e, "batch64": b64,
+                   "end_to_end": e2e, "batch64": b64, "dense4096": dense,
                    "parity_with_oracle": parity, "batched_equals_plain": bool(batch_same)},
         "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -877,14 +877,7 @@ __global__ void __launch_bounds__(256) k_subback(const float *src, float *dst, i
     for (int r = 0; r < 4; r++) {
         const int x = blockIdx.x * 1024 + r * 256 + threadIdx.x;
         if (x >= W) break;
-        float bval;
-        if (nbx > 1) {
-            const int s = colseg[x];
-            const float dx = coldx[x];
-            const float cdx = 1 - dx;
-            bval = cdx * (node[s] + (cdx * cdx - 1) * dnode[s]) + dx * (node[s + 1] + (dx * dx - 1) * dnode[s + 1]);
-        } else
-            bval = node[0];
+        const float bval = sxg_spline_at(node, dnode, colseg, coldx, nbx, x);
         if (SUB)
             dst[row + x] = src[row + x] - bval;
         else
@@ -1040,7 +1033,9 @@ bool sxg_back_run(const BackGeom &g, const BackBufs &b, const float *src, float 
                 return false;
             }
             const size_t rowbytes = mpb * segbytes;
-            const int R = (int)std::max<size_t>(1, std::min<size_t>(8, (64 * 1024) / rowbytes));
+            const long rows = sxg_option(SXG_OPT_MESH_ROWS);
+            int R = (int)std::max<size_t>(1, std::min<size_t>(8, (64 * 1024) / rowbytes));
+            if (rows > 0 && rows < R) R = (int)rows;
             const size_t lds = 2 * R * rowbytes;
             hipLaunchKernelGGL(k_meshstat, dim3(sxg_div_up(nbx, mpb), nby), dim3(MS_THREADS), lds,
                                st, src, wmap, W, H, bs, nbx, mpb, R, b.ms);
@@ -1097,9 +1092,13 @@ bool sxg_back_run(const BackGeom &g, const BackBufs &b, const float *src, float 
                            b.colseg, b.coldx);
         if (!spline_rows(g, b, b.back, b.dback, st)) return false;
         if (timed) sxg_kt_end(SXG_KT_ROWNODES);
-        if (timed) sxg_kt_begin(SXG_KT_SUBBACK);
-        if (!spline_apply(g, b, src, dst, true, st)) return false;
-        if (timed) sxg_kt_end(SXG_KT_SUBBACK);
+        /* dst nullptr: the caller subtracts the spline itself (from the row and column
+           tables just built) */
+        if (dst) {
+            if (timed) sxg_kt_begin(SXG_KT_SUBBACK);
+            if (!spline_apply(g, b, src, dst, true, st)) return false;
+            if (timed) sxg_kt_end(SXG_KT_SUBBACK);
+        }
     }
     SXG_HIP_RET(hipGetLastError(), false);
     return true;
@@ -1186,9 +1185,13 @@ void sxg_back_estimate_from(const float *src, int back_size, float backfthresh, 
     WeightArgs wa;
     if (!sxg_weight_args(wa)) return;
     sxg_stage_begin(ST_BACK);
-    /* pixels with a non-positive mask value leave the mesh statistics */
-    if (!sxg_back_run(g, b, src, img, wa.mask, backfthresh, back_type, S.back_value, true,
-                      S.stream))
+    /* pixels with a non-positive mask value leave the mesh statistics; a whole-frame run
+       may leave the spline's subtraction to its fused filtering kernel */
+    const bool defer = S.defer_sub && back_type == 0;
+    S.defer_sub = false;
+    S.sub_src = defer ? src : nullptr;
+    if (!sxg_back_run(g, b, src, defer ? nullptr : img, wa.mask, backfthresh, back_type,
+                      S.back_value, true, S.stream))
         return;
     if (!S.plan) {
         float hv[2];

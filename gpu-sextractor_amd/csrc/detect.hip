@@ -25,11 +25,9 @@
 #include "sxg_internal.h"
 #include "sxg_wave.h"
 
-/* the detection threshold of pixel p: per pixel with a weight map and relative
-   thresholds, else the global one */
 __device__ inline float det_thresh(const WeightArgs &wa, float t, long p)
 {
-    return (wa.w && wa.rel) ? sxg_pix_thresh(wa.kdet, sxg_var(wa, p)) : t;
+    return sxg_det_thresh(wa, t, p);
 }
 
 /* union-find parent of each pixel: -1 for the background; a detected pixel points at the
@@ -316,12 +314,16 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     /* the device counts the kernels read (planned runs) */
     const int *nfg_k = pd ? nfg_d : nullptr, *nraw_k = pd ? nraw_d : nullptr;
     const int *npix_k = pd ? npix_d : nullptr;
-    sxg_kt_begin(SXG_KT_DET_INIT);
     WeightArgs wa;
     if (!sxg_weight_args(wa)) return;
-    hipLaunchKernelGGL(k_det_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
-                       S.cd.as<float>(), dthreshold, N, W, par, wa, td);
-    sxg_kt_end(SXG_KT_DET_INIT);
+    /* the parents are written already when the frame's convolution was fused with them */
+    if (!S.det_init_done) {
+        sxg_kt_begin(SXG_KT_DET_INIT);
+        hipLaunchKernelGGL(k_det_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
+                           S.cd.as<float>(), dthreshold, N, W, par, wa, td);
+        sxg_kt_end(SXG_KT_DET_INIT);
+    }
+    S.det_init_done = false;
     /* the foreground list has at most N entries; size it after counting */
     int *fg = sxg_take<int>("det.fg", S.N);
     if (!fg) return;

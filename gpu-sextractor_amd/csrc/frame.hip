@@ -222,6 +222,13 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
     const int nbx = (W - 1) / bw + 1, nby = (H - 1) / bw + 1;
     float backmean = 0.0f, backsig = 0.0f;
     S.back_value = p->back_value;
+    /* the spline's subtraction, the convolution and the detection parents in one pass
+       (frames of a width that is a multiple of 64, masks of 3 to 11) */
+    const int K = p->mask ? p->mask_width : 0;
+    const bool fuse = p->back_type == 0 && (K == 3 || K == 5 || K == 7 || K == 9 || K == 11) &&
+                      (W & 63) == 0 && sxg_option(SXG_OPT_FUSED_FILTER) != 0;
+    S.defer_sub = fuse;
+    S.det_init_done = false;
     sxg_back_estimate_from(dframe, bw, p->back_fthresh, &backmean, &backsig,
                            nbx > 1 ? p->back_fsize : 1, nby > 1 ? p->back_fsize : 1,
                            p->back_type);
@@ -243,7 +250,14 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
         thresh = (float)(p->analysis_thresh * backsig);
     }
     init_detection();
-    if (p->mask && p->mask_width > 0) cudaFilter(const_cast<float *>(p->mask), p->mask_width);
+    if (fuse) {
+        if (!sxg_subconv_detect(p->mask, K, dthresh)) {
+            if (!sxg_has_error()) sxg_set_error("process_frame: the fused filter did not run");
+            return -1;
+        }
+    } else if (p->mask && p->mask_width > 0) {
+        cudaFilter(const_cast<float *>(p->mask), p->mask_width);
+    }
     if (sxg_has_error()) return -1;
     size_t nraw = 0, npix = 0;
     run_detection(dthresh, thresh, p->ext_minarea, &nraw, &npix);

@@ -82,6 +82,27 @@ __device__ inline float sxg_var(const WeightArgs &wa, long p)
     return v < SXG_BIG ? v * v : v;      /* MAP_RMS and the background rms */
 }
 __device__ inline float sxg_pix_thresh(double k, float var) { return (float)(k * sqrt((double)var)); }
+/* the detection threshold of pixel p: per pixel with a weight map and relative
+   thresholds, else the global one */
+__device__ inline float sxg_det_thresh(const WeightArgs &wa, float t, long p)
+{
+    return (wa.w && wa.rel) ? sxg_pix_thresh(wa.kdet, sxg_var(wa, p)) : t;
+}
+/* the background spline at column x of a frame row: the row's nodes and their second
+   derivatives (node, dnode; nbx of each) and the column tables of the cumulative dx
+   recurrence (subbackline, back.c:880-908) */
+__device__ inline float sxg_spline_at(const float *node, const float *dnode, const int *colseg,
+                                      const float *coldx, int nbx, int x)
+{
+    if (nbx > 1) {
+        const int s = colseg[x];
+        const float dx = coldx[x];
+        const float cdx = 1 - dx;
+        return cdx * (node[s] + (cdx * cdx - 1) * dnode[s]) +
+               dx * (node[s + 1] + (dx * dx - 1) * dnode[s + 1]);
+    }
+    return node[0];
+}
 
 enum StageId { ST_BACK = 0, ST_FILTER, ST_DETECT, ST_DEBLEND, ST_ANALYSE, ST_CLEAN, ST_N };
 
@@ -108,6 +129,11 @@ bool sxg_back_run(const BackGeom &g, const BackBufs &b, const float *src, float 
                   bool timed, hipStream_t st);
 void sxg_back_estimate_from(const float *src, int back_size, float backfthresh, float *backmean,
                             float *backsig, int nbackfx, int nbackfy, int back_type);
+/* whole-frame runs: subtraction of the background spline (tables of the last
+   back_estimate), the K x K convolution and the detection parents of k_det_init in one
+   pass over the input frame (filter.hip); false when the frame or mask does not suit it
+   (nothing queued) */
+bool sxg_subconv_detect(const float *h_mask, int K, float dthresh);
 /* the background map of the last back_estimate on the current context (device) */
 const float *sxg_back_map();
 /* the device block of one background estimate of geometry g: its size, and its parts
@@ -198,6 +224,11 @@ struct SxgState {
     int64_t phot_pixels = 0;                  /* aperture pixels of the photometry */
     int nraw = 0;
     bool filtered = false;  /* cudaFilter ran on this frame */
+    /* whole-frame runs: the background stage leaves the subtraction to the fused
+       subtraction, convolution and thresholding kernel (sub_src: the input frame), which
+       also writes the union-find parents (det_init_done) */
+    bool defer_sub = false, det_init_done = false;
+    const float *sub_src = nullptr;
     int *fpia = nullptr;      /* int[P0]: pixels grouped by raw object, raster order inside */
     int *fpia_keys = nullptr; /* int[P0]: raw object id of each fpia entry */
     DevBuf scan_tmp; /* scan scratch */

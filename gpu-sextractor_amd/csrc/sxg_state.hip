@@ -25,8 +25,8 @@ bool sxg_has_error() { return g_has_err; }
 extern "C" const char *sxg_last_error(void) { return g_has_err ? g_err.c_str() : nullptr; }
 
 /* ------------------------------------------------------------------ options ---- */
-static long g_opt[SXG_NOPT] = {1024, 6144, 0, 512, 0, 1, 12288, 4096};
-static const long g_opt_max[SXG_NOPT] = {6400, 6400, 2, 6400, 2048, 1, 1L << 30, 1L << 30};
+static long g_opt[SXG_NOPT] = {1024, 6144, 0, 512, 0, 1, 12288, 4096, 0, 1};
+static const long g_opt_max[SXG_NOPT] = {6400, 6400, 2, 6400, 2048, 1, 1L << 30, 1L << 30, 8, 1};
 
 long sxg_option(int id) { return g_opt[id]; }
 

@@ -427,7 +427,7 @@ KERNELS = ["copy", "meshstat", "meshmode", "filterback", "rownodes", "subback", 
 # "precision" chooses the arithmetic of the graded catalogue columns
 OPTIONS = {"deb_lds_small": 0, "deb_lds_large": 1, "precision": 2, "deb_lds_tiny": 3,
            "deb_table": 4, "side_streams": 5, "deb_glb_levels": 6,
-           "deb_order_full": 7}
+           "deb_order_full": 7, "mesh_rows": 8, "fused_filter": 9}
 
 
 def set_option(name, value):

@@ -438,6 +438,15 @@ int sxg_kernel_times(int id, double *total_ms, int *launches);
      SXG_OPT_DEB_ORDER_FULL the global-memory objects are launched largest first when there
                             are at most this many of them (default 4096); beyond, only the
                             level-parallel ones are ordered, the others follow in any order
+     SXG_OPT_MESH_ROWS      frame rows per batch of the mesh-statistics kernel's LDS ring
+                            (0, default: as many as fit in 64 KB per buffer, at most 8; else
+                            1..8): fewer rows hold less LDS while its chains run, which
+                            leaves room on each CU for other frames' kernels
+     SXG_OPT_FUSED_FILTER   1 (default): the whole-frame runs (sxg_process_resident and the
+                            batches) subtract the background spline, convolve and write the
+                            detection parents in one kernel for frames of a width that is a
+                            multiple of 64 and masks of 3 to 11; 0: three kernels, as the
+                            reference's entry points run them
    This one changes the catalogue (never the segmentation map, the object list or the
    CLEAN decisions):
      SXG_OPT_PRECISION      arithmetic of FLUX_ISO, X/Y, X2/Y2/XY, A/B/THETA, CXX/CYY/CXY:
@@ -445,7 +454,8 @@ int sxg_kernel_times(int id, double *total_ms, int *launches);
                             2 compensated FP32 sums with FP64 central moments */
 enum sxg_option_id { SXG_OPT_DEB_LDS_SMALL = 0, SXG_OPT_DEB_LDS_LARGE, SXG_OPT_PRECISION,
                      SXG_OPT_DEB_LDS_TINY, SXG_OPT_DEB_TABLE, SXG_OPT_SIDE_STREAMS,
-                     SXG_OPT_DEB_GLB_LEVELS, SXG_OPT_DEB_ORDER_FULL, SXG_NOPT };
+                     SXG_OPT_DEB_GLB_LEVELS, SXG_OPT_DEB_ORDER_FULL, SXG_OPT_MESH_ROWS,
+                     SXG_OPT_FUSED_FILTER, SXG_NOPT };
 /* Returns 0, or -1 for an unknown option or a value out of range. */
 int sxg_set_option(int id, long value);
 long sxg_get_option(int id);

@@ -92,7 +92,9 @@ ALG_BYTES = {
 # The kernel timers that bracket exactly one kernel launch, the kernel's name (as the
 # rocprof kernel trace shows it) and its algorithmic HBM bytes per launch:
 #   k_meshstat, k_meshmode: read the frame (4N); k_subback: read it, write the subtracted
-#   frame (8N); k_convolve: read the subtracted frame, write the filtered one (8N);
+#   frame (8N); k_convolve: read the subtracted frame, write the filtered one (8N); the
+#   whole-frame runs' k_subconv_det (timed as conv, with no subback launch): read the
+#   frame, write the subtracted and filtered frames and the detection parents (16N);
 #   k_analyse: per raw pixel its index, value and filtered value (12 B), per object the
 #   ~176 bytes of catalogue columns it writes; k_deb_tree_lds (one launch per size
 #   class): per pixel of the class's objects its index and filtered value in, its kept
@@ -130,6 +132,9 @@ def roofline_entry(ktab, N, info, traffic=None):
     tot, nl = ktab[dom]
     avg_s = tot / nl / 1000.0
     name, alg = SINGLE_KERNELS[dom]
+    if dom == "conv" and "subback" not in ktab:
+        # the fused subtraction, convolution and detection parents
+        name, alg = "k_subconv_det<K>", (lambda N, i: 16 * N)
     bytes_launch = int(alg(N, info))
     achieved = bytes_launch / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

@@ -69,6 +69,7 @@ sys.path.insert(0, os.path.join(REPO, "gpu-sextractor_amd"))
 import sexgpu  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+DEFAULT_OPTIONS = {}
 
 # Algorithmic HBM bytes per frame of each kernel family (its launches together), from the
 # frame size N (4-byte pixels) and the frame's counts (sxg_frame_info):
@@ -337,6 +338,7 @@ def batch64_summary(L, args):
         len(frames), npx / 1e6, time.perf_counter() - t), file=sys.stderr, flush=True)
     p = sexgpu.Params(mask=sexgpu.read_conv("default.conv"))
     sp = sexgpu.native_params(p)
+    restore_options(DEFAULT_OPTIONS)
     sexgpu.set_batch_frames(frames)
     B = len(frames)
     out = {}
@@ -362,6 +364,17 @@ def batch64_summary(L, args):
     return out
 
 
+def default_options():
+    """The option values as the library starts (the autotuner sets the size classes it
+    picks; each leg of the default run starts from these)."""
+    return {k: sexgpu.get_option(k) for k in sexgpu.OPTIONS}
+
+
+def restore_options(saved):
+    for k, v in saved.items():
+        sexgpu.set_option(k, v)
+
+
 def dense_summary(L, args, nframes=4):
     """The dense regime in the default run, briefly: the crowded generator with fluxes from
     3000 ADU (7.6 M pixels above the threshold, as the paper's 4096^2 image had 7.8 M),
@@ -376,6 +389,7 @@ def dense_summary(L, args, nframes=4):
     N = W * H
     p = workload_params()
     sp = sexgpu.native_params(p)
+    restore_options(DEFAULT_OPTIONS)
     L.sxg_set_resident_frame(frames[0].ctypes.data, W, H)
     fi = sexgpu.FrameInfo()
     L.sxg_process_resident(C.byref(sp), C.byref(fi))
@@ -789,6 +803,8 @@ def main():
     ap.add_argument("--no-dense", action="store_true",
                     help="skip the dense-frame legs of the default run")
     args = ap.parse_args()
+    global DEFAULT_OPTIONS
+    DEFAULT_OPTIONS = default_options()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (a frame or a batch "
               "of frames runs on one MI355X); run with --gpus 1", file=sys.stderr)
@@ -896,7 +912,8 @@ def main():
                                                  mpix_batch / mpix_plain, np.percentile(lat, 50),
                                                  np.percentile(lat, 99)), file=sys.stderr, flush=True)
 
-    # ---- end to end from host memory
+    # ---- end to end from host memory (the options as the library starts)
+    restore_options(DEFAULT_OPTIONS)
     e2e = None if args.no_end_to_end else end_to_end(L, frames, p, sp, args, seg0, segs_plain)
 
     # ---- the 64 uneven frames: frame by frame, worker pool and graph

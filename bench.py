@@ -117,10 +117,11 @@ SINGLE_KERNELS = {
 # same counter reads for a plain 256 MiB device-to-device copy (bench.py --calibrate-copy);
 # see profiles/hbm_traffic_crowded4096_pmc.md.  Used for roofline.traffic on that workload.
 # (k_deb_tree_lds<1024> runs twice per frame, classes 1 and 2; the counters see one kernel
-# name, so both timers carry the mean of its two launches)
+# name, so both timers carry the mean of its two launches).  "conv_fused" is the whole-frame
+# runs' k_subconv_det<7>, the kernel the conv timer brackets when no subback runs.
 MEASURED_TRAFFIC_CROWDED4096 = {
-    "meshstat": 1.346e8, "meshmode": 6.859e7, "analyse": 6.613e7, "conv": 2.003e8,
-    "subback": 1.406e8, "deb_tree0": 1.070e7, "deb_tree1": 1.04e6, "deb_tree2": 1.04e6,
+    "meshstat": 1.346e8, "meshmode": 6.859e7, "analyse": 6.612e7, "conv_fused": 2.719e8,
+    "deb_tree0": 1.070e7, "deb_tree1": 1.04e6, "deb_tree2": 1.04e6,
 }
 
 
@@ -133,14 +134,15 @@ def roofline_entry(ktab, N, info, traffic=None):
     tot, nl = ktab[dom]
     avg_s = tot / nl / 1000.0
     name, alg = SINGLE_KERNELS[dom]
+    tkey = dom
     if dom == "conv" and "subback" not in ktab:
         # the fused subtraction, convolution and detection parents
-        name, alg = "k_subconv_det<K>", (lambda N, i: 16 * N)
+        name, alg, tkey = "k_subconv_det<K>", (lambda N, i: 16 * N), "conv_fused"
     bytes_launch = int(alg(N, info))
     achieved = bytes_launch / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": (traffic or {}).get(dom), "kernel": name, "timer": dom,
+            "traffic": (traffic or {}).get(tkey), "kernel": name, "timer": dom,
             "kernel_ms": round(avg_s * 1e3, 4), "algorithmic_bytes": bytes_launch}
 
 
@@ -802,6 +804,9 @@ def main():
                     help="skip the 64-frame legs of the default run")
     ap.add_argument("--no-dense", action="store_true",
                     help="skip the dense-frame legs of the default run")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the graph legs (for kernel-trace runs: rocprofv3's tracer does "
+                         "not follow graph launches here)")
     args = ap.parse_args()
     global DEFAULT_OPTIONS
     DEFAULT_OPTIONS = default_options()
@@ -884,19 +889,22 @@ def main():
     batch_same = batch_same and np.array_equal(segs_plain[0], seg0)
     # the same batch as one HIP graph, its branches and deblending size classes chosen by
     # the autotuner (results checked equal across its candidates)
-    t = time.perf_counter()
-    tune = sexgpu.autotune(p, max_branches=min(8, B), replays=2)
-    tune["tune_s"] = round(time.perf_counter() - t, 2)
-    print("# autotune: %s" % tune, file=sys.stderr, flush=True)
-    tgb, dtg, infg, latg, reruns = timed_graph(L, sp, tune["branches"], args.steps, B)
-    segs_graph = batch_segs(L, frames)
-    graph_same = all(np.array_equal(a, b) for a, b in zip(segs_plain, segs_graph))
+    tune, tgb, reruns, graph_same, mpix_graph, latg = None, 0.0, 0, None, 0.0, [0.0]
+    if not args.no_graph:
+        t = time.perf_counter()
+        tune = sexgpu.autotune(p, max_branches=min(8, B), replays=2)
+        tune["tune_s"] = round(time.perf_counter() - t, 2)
+        print("# autotune: %s" % tune, file=sys.stderr, flush=True)
+        tgb, dtg, infg, latg, reruns = timed_graph(L, sp, tune["branches"], args.steps, B)
+        segs_graph = batch_segs(L, frames)
+        graph_same = all(np.array_equal(a, b) for a, b in zip(segs_plain, segs_graph))
+        mpix_graph = B * N * args.steps / dtg / 1e6
+        print("# graph batch: %.1f Mpix/s, build %.0f ms, eager reruns %d, frame device time "
+              "p50 %.2f ms p99 %.2f ms, maps equal %s" % (mpix_graph, 1e3 * tgb, reruns,
+                                                          np.percentile(latg, 50),
+                                                          np.percentile(latg, 99), graph_same),
+              file=sys.stderr, flush=True)
     L.sxg_batch_release()
-    mpix_graph = B * N * args.steps / dtg / 1e6
-    print("# graph batch: %.1f Mpix/s, build %.0f ms, eager reruns %d, frame device time p50 "
-          "%.2f ms p99 %.2f ms, maps equal %s" % (mpix_graph, 1e3 * tgb, reruns,
-                                                   np.percentile(latg, 50), np.percentile(latg, 99),
-                                                   graph_same), file=sys.stderr, flush=True)
     nsurv = sum(f.nsurv for f in infb)
     mpix_plain = B * N * plain_steps / dtp / 1e6
     mpix_workers = B * N * args.steps / dtb / 1e6
@@ -943,7 +951,7 @@ def main():
                    "batched_path": "graph" if use_graph else "workers",
                    "workers_mpix_s": round(mpix_workers, 2), "graph_mpix_s": round(mpix_graph, 2),
                    "graph_build_ms": round(1e3 * tgb, 1), "graph_eager_reruns": int(reruns),
-                   "graph_equals_plain": bool(graph_same), "autotune": tune,
+                   "graph_equals_plain": graph_same, "autotune": tune,
                    "objects_per_s": round(objs_batch, 1),
                    "plain_mpix_s": round(mpix_plain, 2), "plain_objects_per_s": round(objs_plain, 1),
                    "single_frame_ms": round(single_ms, 3),

@@ -10,7 +10,8 @@
  *
  * Layout: a workgroup of 256 lanes produces a 64 x 32 output tile.  The input tile with
  * its halo is staged in LDS once (coalesced row loads, one global read per input pixel
- * plus halo); each lane owns one column and 8 consecutive rows.  Sweeping the input
+ * plus halo, all of a lane's loads in flight together: cv_load); each lane owns one
+ * column and 8 consecutive rows.  Sweeping the input
  * rows once, input row t feeds output row r through mask row i = t - r, so every
  * output still sees its taps in mask row-major order while each LDS row is read once
  * per lane for 8 outputs.  Bound: VALU (2 K^2 FP32 operations per pixel, no FMA).
@@ -40,6 +41,134 @@ __device__ inline void cv_tile(int &X0, int &Y0)
     Y0 = (t / gx) * CV_TY;
 }
 
+/* the background spline of a frame: per frame row, nbx nodes and their second
+   derivatives; per column, its left node and dx (the cumulative recurrence of
+   subbackline, back.c:880-908) */
+struct SplineArgs {
+    const float *nodes, *dnodes;   /* per frame row: nbx nodes and second derivatives */
+    const int *colseg;
+    const float *coldx;
+    int nbx;
+};
+
+/* what a staged tile pixel holds: the frame value (TS_COPY), or the frame value minus the
+   background spline at that pixel (TS_SPLINE: rows of nbx > 1 nodes; TS_NODE0: one node
+   per row), the expression of sxg_spline_at */
+enum TileSrc { TS_COPY = 0, TS_SPLINE, TS_NODE0 };
+
+/*
+ * Stage the (CV_TY + K - 1) x (CV_TX + K - 1) input tile of the workgroup at (X0, Y0) in
+ * LDS, 0 outside the frame.  Every global load of a lane is issued before the first is
+ * used, so a workgroup waits for about two memory round trips instead of one or more per
+ * staged pixel: the columns 0..63 of the tile go by rows, one row per wavefront and step,
+ * with a lane's column tables read once for all its rows; the right strip (columns
+ * 64..63 + K - 1) is a flat list over the workgroup.  The addresses are clamped into the
+ * frame, so that the loads need no branch, and the values outside it are replaced by 0
+ * afterwards.
+ */
+template <int K, int MODE>
+__device__ __forceinline__ void cv_load(float (*tile)[CV_TX + K], const float *__restrict__ src,
+                                        const SplineArgs &sp, int W, int H, int X0, int Y0)
+{
+    constexpr int HW = K / 2;
+    constexpr int LH = CV_TY + K - 1;
+    constexpr int NW = CV_THREADS / 64;
+    constexpr int NRW = (LH + NW - 1) / NW;              /* rows per wavefront */
+    constexpr int NS = (K - 1) * LH;                     /* right-strip pixels */
+    constexpr int NSU = (NS + CV_THREADS - 1) / CV_THREADS;
+    constexpr bool SPL = MODE == TS_SPLINE;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nbx = sp.nbx;
+    /* the lane's column of the main part and its spline segment */
+    const int gxm = X0 + lane - HW;
+    const bool cm = gxm >= 0 && gxm < W;
+    const int gxmc = min(max(gxm, 0), W - 1);
+    int sm = 0;
+    float dxm = 0.0f;
+    if (SPL) {
+        sm = sp.colseg[gxmc];
+        dxm = sp.coldx[gxmc];
+    }
+    /* the lane's right-strip pixels */
+    int sly[NSU > 0 ? NSU : 1], slx[NSU > 0 ? NSU : 1], sgx[NSU > 0 ? NSU : 1],
+        ss[NSU > 0 ? NSU : 1];
+    float sdx[NSU > 0 ? NSU : 1];
+    bool sin_[NSU > 0 ? NSU : 1];
+#pragma unroll
+    for (int u = 0; u < NSU; u++) {
+        const int e = threadIdx.x + u * CV_THREADS;
+        const int ly = e / (K - 1), lx = CV_TX + (e - ly * (K - 1));
+        const int gx = X0 + lx - HW, gy = Y0 + ly - HW;
+        sly[u] = ly;
+        slx[u] = lx;
+        sin_[u] = e < NS && gx >= 0 && gx < W && gy >= 0 && gy < H;
+        sgx[u] = min(max(gx, 0), W - 1);
+        ss[u] = 0;
+        sdx[u] = 0.0f;
+        if (SPL) {
+            ss[u] = sp.colseg[sgx[u]];
+            sdx[u] = sp.coldx[sgx[u]];
+        }
+    }
+    /* every load: frame values, then the spline nodes of their rows */
+    float mv[NRW], mn0[NRW], mn1[NRW], md0[NRW], md1[NRW];
+#pragma unroll
+    for (int u = 0; u < NRW; u++) {
+        const int gy = min(max(Y0 + wid + NW * u - HW, 0), H - 1);
+        mv[u] = src[(size_t)gy * W + gxmc];
+        const float *nr = sp.nodes + (size_t)gy * nbx, *dr = sp.dnodes + (size_t)gy * nbx;
+        if (SPL) {
+            mn0[u] = nr[sm];
+            mn1[u] = nr[sm + 1];
+            md0[u] = dr[sm];
+            md1[u] = dr[sm + 1];
+        } else if (MODE == TS_NODE0) {
+            mn0[u] = nr[0];
+        }
+    }
+    float sv[NSU > 0 ? NSU : 1], sn0[NSU > 0 ? NSU : 1], sn1[NSU > 0 ? NSU : 1],
+        sd0[NSU > 0 ? NSU : 1], sd1[NSU > 0 ? NSU : 1];
+#pragma unroll
+    for (int u = 0; u < NSU; u++) {
+        const int gy = min(max(Y0 + sly[u] - HW, 0), H - 1);
+        sv[u] = src[(size_t)gy * W + sgx[u]];
+        const float *nr = sp.nodes + (size_t)gy * nbx, *dr = sp.dnodes + (size_t)gy * nbx;
+        if (SPL) {
+            sn0[u] = nr[ss[u]];
+            sn1[u] = nr[ss[u] + 1];
+            sd0[u] = dr[ss[u]];
+            sd1[u] = dr[ss[u] + 1];
+        } else if (MODE == TS_NODE0) {
+            sn0[u] = nr[0];
+        }
+    }
+    /* the staged values (sxg_spline_at's expression, term for term) */
+    auto back = [](float dx, float n0, float n1, float d0, float d1) {
+        const float cdx = 1 - dx;
+        return cdx * (n0 + (cdx * cdx - 1) * d0) + dx * (n1 + (dx * dx - 1) * d1);
+    };
+#pragma unroll
+    for (int u = 0; u < NRW; u++) {
+        const int ly = wid + NW * u;
+        if (ly < LH) {
+            const int gy = Y0 + ly - HW;
+            float v = mv[u];
+            if (SPL) v = v - back(dxm, mn0[u], mn1[u], md0[u], md1[u]);
+            else if (MODE == TS_NODE0) v = v - mn0[u];
+            tile[ly][lane] = cm && gy >= 0 && gy < H ? v : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NSU; u++) {
+        if (threadIdx.x + u * CV_THREADS < NS) {
+            float v = sv[u];
+            if (SPL) v = v - back(sdx[u], sn0[u], sn1[u], sd0[u], sd1[u]);
+            else if (MODE == TS_NODE0) v = v - sn0[u];
+            tile[sly[u]][slx[u]] = sin_[u] ? v : 0.0f;
+        }
+    }
+}
+
 template <int K>
 __global__ void __launch_bounds__(CV_THREADS) k_convolve(const float *__restrict__ in,
                                                          float *__restrict__ out, int W, int H,
@@ -50,11 +179,7 @@ __global__ void __launch_bounds__(CV_THREADS) k_convolve(const float *__restrict
     __shared__ float tile[LH][LW + 1];
     int X0, Y0;
     cv_tile(X0, Y0);
-    for (int k = threadIdx.x; k < LW * LH; k += CV_THREADS) {
-        const int ly = k / LW, lx = k - ly * LW;
-        const int gx = X0 + lx - HW, gy = Y0 + ly - HW;
-        tile[ly][lx] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? in[(size_t)gy * W + gx] : 0.0f;
-    }
+    cv_load<K, TS_COPY>(tile, in, SplineArgs{}, W, H, X0, Y0);
     __syncthreads();
     const int tx = threadIdx.x % CV_TX, ty = threadIdx.x / CV_TX;
     const int r0 = ty * CV_R;
@@ -198,17 +323,11 @@ extern "C" void cudaFilter(float *h_mask, unsigned int mask_width)
    and tap order) and the detection parents (k_det_init's ballot over aligned 64-pixel
    windows: with W a multiple of 64 these are the tile rows a wavefront holds).  The
    tile loader subtracts the spline from every input pixel it stages (the halo too, with
-   the same value the neighbouring tile computes) and writes the tile's own pixels to
-   sub; the lanes then convolve, write cd, compare with the detection threshold and link
-   each detected pixel to the first pixel of its run inside its window, or write -1.
+   the same value the neighbouring tile computes); the lanes then convolve, write the
+   tile's own subtracted pixels to sub and the filtered ones to cd, compare with the
+   detection threshold and link each detected pixel to the first pixel of its run inside
+   its window, or write -1.
    One read of the frame and three plane writes instead of three kernels' six passes. */
-struct SplineArgs {
-    const float *nodes, *dnodes;   /* per frame row: nbx nodes and second derivatives */
-    const int *colseg;
-    const float *coldx;
-    int nbx;
-};
-
 template <int K>
 __global__ void __launch_bounds__(CV_THREADS) k_subconv_det(const float *__restrict__ src,
                                                             float *__restrict__ sub,
@@ -224,20 +343,8 @@ __global__ void __launch_bounds__(CV_THREADS) k_subconv_det(const float *__restr
     if (td) t = *td;
     int X0, Y0;
     cv_tile(X0, Y0);
-    for (int k = threadIdx.x; k < LW * LH; k += CV_THREADS) {
-        const int ly = k / LW, lx = k - ly * LW;
-        const int gx = X0 + lx - HW, gy = Y0 + ly - HW;
-        float v = 0.0f;
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const size_t o = (size_t)gy * W + gx;
-            const float bval = sxg_spline_at(sp.nodes + (size_t)gy * sp.nbx,
-                                             sp.dnodes + (size_t)gy * sp.nbx, sp.colseg,
-                                             sp.coldx, sp.nbx, gx);
-            v = src[o] - bval;
-            if (lx >= HW && lx < HW + CV_TX && ly >= HW && ly < HW + CV_TY) sub[o] = v;
-        }
-        tile[ly][lx] = v;
-    }
+    if (sp.nbx > 1) cv_load<K, TS_SPLINE>(tile, src, sp, W, H, X0, Y0);
+    else cv_load<K, TS_NODE0>(tile, src, sp, W, H, X0, Y0);
     __syncthreads();
     const int tx = threadIdx.x % CV_TX, ty = threadIdx.x / CV_TX;
     const int r0 = ty * CV_R;
@@ -273,6 +380,8 @@ __global__ void __launch_bounds__(CV_THREADS) k_subconv_det(const float *__restr
         const bool start = fg && (lane == 0 || !((F >> (lane - 1)) & 1ull));
         const unsigned long long S = __ballot(start);
         if (live) {
+            /* the subtracted pixel, as staged */
+            sub[p] = tile[HW + r0 + r][HW + tx];
             out[p] = acc[r];
             par[p] = fg ? (int)(p - lane + (63 - __builtin_clzll(S & below))) : -1;
         }

@@ -8,12 +8,13 @@
  *                runs each sum's chain; three wavefronts of the workgroup stream the mesh
  *                rows through a double-buffered LDS ring meanwhile.
  *   k_meshmode   one workgroup per mesh: histogram in LDS (back.c:297-341), then the
- *                clipped-mode iteration of backguess (back.c:347-426).  Inside the
- *                iteration every sum is an exact integer (< 2^53), so the sequential
- *                double sums equal prefix-sum differences; the two-pointer median walk
- *                is a merge of two non-decreasing sequences and its end point is found
- *                by a binary search on prefix sums (merge-path), which reproduces the
- *                walk exactly.
+ *                clipped-mode iteration of backguess (back.c:347-426) on one wavefront.
+ *                Inside the iteration every sum is an exact integer (< 2^53), so the
+ *                sequential double sums equal prefix-sum differences (64-bit integer
+ *                prefixes per 16 bins plus the bins of the two partial blocks); the
+ *                two-pointer median walk is a merge of two non-decreasing sequences and
+ *                its end point is found by a 64-way search on the prefix counts
+ *                (merge-path), which reproduces the walk exactly.
  *   k_fb_*       bad-mesh interpolation and median filtering, one lane per mesh
  *                (back.c:434-530); global medians by radix selection, one workgroup
  *                per map (back.c:532-556); y second derivatives, one lane per mesh
@@ -325,12 +326,14 @@ __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restri
                                                            const MeshStat *ms, float *back,
                                                            float *sigma)
 {
-    /* h[i]: histogram; P0[k] = sum_{j<k} h[j] for k in [0, 4096] */
-    __shared__ int h[SXG_NMAXLEVELS];
+    /* the histogram h, then in place its prefix counts P0[k] = sum_{j<k} h[j] for k in
+       [0, 4096] (h[j] = P0[j + 1] - P0[j]); C1[t], C2[t]: the sums of h[j] j and h[j] j^2
+       over the bins j < 16 t, exact in 64-bit integers */
     __shared__ int P0[SXG_NMAXLEVELS + 1];
-    __shared__ double sred[2][2][MODE_THREADS / 64];
-    __shared__ int sfirst[2][MODE_THREADS / 64];
+    __shared__ long long C1[MODE_THREADS + 1], C2[MODE_THREADS + 1];
     __shared__ int sscan[MODE_THREADS / 64];
+    __shared__ long long sscan1[MODE_THREADS / 64], sscan2[MODE_THREADS / 64];
+    int *const h = P0;
     const int m = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const MeshStat st = ms[m];
@@ -392,43 +395,87 @@ __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restri
         }
     }
     __syncthreads();
-    /* thread t owns bins [16t, 16t + 16): their counts and prefix counts stay in
-       registers; P0 goes to LDS for the lookups at arbitrary bins */
+    /* thread t owns bins [16t, 16t + 16): the exclusive prefixes of their counts and of
+       their two moment sums over the workgroup */
     constexpr int PER = SXG_NMAXLEVELS / MODE_THREADS;
     const int b0 = tid * PER;
-    int hc[PER], pc[PER + 1];
+    int hc[PER];
     int csum = 0;
+    long long q1 = 0, q2 = 0;
 #pragma unroll
     for (int u = 0; u < PER; u++) {
-        hc[u] = h[b0 + u];
+        const int j = b0 + u;
+        hc[u] = h[j];
         csum += hc[u];
+        q1 += (long long)hc[u] * j;
+        q2 += (long long)hc[u] * ((long long)j * j);
     }
-    {
-        int x = csum;
-        for (int d = 1; d < 64; d <<= 1) {
-            const int o = __shfl_up(x, d);
-            if (lane >= d) x += o;
+    int x = csum;
+    long long x1 = q1, x2 = q2;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(x, d);
+        const long long o1 = __shfl_up(x1, d), o2 = __shfl_up(x2, d);
+        if (lane >= d) {
+            x += o;
+            x1 += o1;
+            x2 += o2;
         }
-        if (lane == 63) sscan[wid] = x;
-        __syncthreads();
-        int before = 0;
-        for (int w = 0; w < wid; w++) before += sscan[w];
-        pc[0] = before + x - csum;
     }
+    if (lane == 63) {
+        sscan[wid] = x;
+        sscan1[wid] = x1;
+        sscan2[wid] = x2;
+    }
+    __syncthreads();   /* every thread has read its bins of h and the wave totals are in */
+    int before = 0;
+    long long before1 = 0, before2 = 0;
+    for (int w = 0; w < wid; w++) {
+        before += sscan[w];
+        before1 += sscan1[w];
+        before2 += sscan2[w];
+    }
+    int pc = before + x - csum;
+    C1[tid] = before1 + x1 - q1;
+    C2[tid] = before2 + x2 - q2;
 #pragma unroll
     for (int u = 0; u < PER; u++) {
-        pc[u + 1] = pc[u] + hc[u];
-        P0[b0 + u] = pc[u];
+        P0[b0 + u] = pc;
+        pc += hc[u];
     }
-    if (tid == MODE_THREADS - 1) P0[SXG_NMAXLEVELS] = pc[PER];
+    if (tid == MODE_THREADS - 1) {
+        P0[SXG_NMAXLEVELS] = pc;
+        C1[MODE_THREADS] = before1 + x1;
+        C2[MODE_THREADS] = before2 + x2;
+    }
     __syncthreads();
-
-    /* The iteration of backguess.  Every thread runs the same control flow on the same
-       shared values; the window sums and the median bin are collective. */
-    auto hget = [&](long i) -> long { return (i >= 0 && i < nlev) ? h[i] : 0; };
+    /* The iteration of backguess runs on one wavefront (the others are done): each of
+       its window sums is two coarse entries and at most 15 bins on each side, and the
+       median bin a 64-way search on P0. */
+    if (wid != 0) return;
+    auto hget = [&](long i) -> long { return (i >= 0 && i < nlev) ? P0[i + 1] - P0[i] : 0; };
+    /* exact sums of h[j] j and h[j] j^2 over j < k for the two ends of a window: lanes
+       0..15 the bins of kA's 16-bin block below kA, lanes 16..31 those of kB's */
+    auto wsums = [&](int kA, int kB, long long &s1, long long &s2) {
+        const int k = lane < 16 ? kA : kB;
+        const int j = (k & ~15) + (lane & 15);
+        long long a1 = 0, a2 = 0;
+        if (lane < 32 && j < k) {
+            const long long c = P0[j + 1] - P0[j];
+            a1 = c * j;
+            a2 = c * ((long long)j * j);
+        }
+        for (int d = 8; d > 0; d >>= 1) {
+            a1 += __shfl_xor(a1, d);
+            a2 += __shfl_xor(a2, d);
+        }
+        const long long pA1 = C1[kA >> 4] + __shfl(a1, 0), pB1 = C1[kB >> 4] + __shfl(a1, 16);
+        const long long pA2 = C2[kA >> 4] + __shfl(a2, 0), pB2 = C2[kB >> 4] + __shfl(a2, 16);
+        s1 = pB1 - pA1;
+        s2 = pB2 - pA2;
+    };
     unsigned long lowsum, highsum, sum;
     double ftemp, mea, sig, sig1, med;
-    int n, lcut, hcut, nlevelsm1, par = 0;
+    int n, lcut, hcut, nlevelsm1;
     hcut = nlevelsm1 = nlev - 1;
     lcut = 0;
     sig = 10.0 * nlevelsm1;
@@ -446,46 +493,35 @@ __global__ void __launch_bounds__(MODE_THREADS) k_meshmode(const float *__restri
             hi = hcut;
         } else {
             /* exact integer window sums (each below 2^53, so the sequential double sums
-               of back.c equal them, and so do these double sums in any order) */
+               of back.c equal them) */
             const int base = P0[lcut], top = P0[hcut + 1];
             const int target = base + top;
-            double s1 = 0.0, s2 = 0.0;
+            long long s1, s2;
+            wsums(lcut, hcut + 1, s1, s2);
             /* a = #{i in [0,nb): L(i) < H(nb-1-i)} with L(i) = P0[lcut+i]-P0[lcut] and
                H(nb-1-i) = P0[hcut+1]-P0[lcut+i+1], i.e. the first bin j >= lcut where the
-               non-decreasing P0[j] + P0[j+1] reaches P0[lcut] + P0[hcut+1] */
+               non-decreasing P0[j] + P0[j+1] reaches P0[lcut] + P0[hcut+1]: the range
+               is cut into 64 chunks, the first chunk whose last bin reaches it holds
+               that bin, and the search goes on inside it (two rounds for 4096 bins) */
             int first = 0x7fffffff;
-#pragma unroll
-            for (int u = 0; u < PER; u++) {
-                const int j = b0 + u;
-                if (j >= lcut && j <= hcut) {
-                    const double c = (double)hc[u];
-                    s1 += c * j;
-                    s2 += c * ((double)j * j);
-                    if (first == 0x7fffffff && pc[u] + pc[u + 1] >= target) first = j;
+            int rlo = lcut, rhi = hcut;
+            while (true) {
+                const int step = (rhi - rlo + 1 + 63) >> 6;
+                const int jend = min(rlo + (lane + 1) * step - 1, rhi);
+                const bool c = rlo + lane * step <= rhi && P0[jend] + P0[jend + 1] >= target;
+                const unsigned long long bal = __ballot(c);
+                if (!bal) break;
+                const int ci = __builtin_ctzll(bal);
+                if (step == 1) {
+                    first = rlo + ci;
+                    break;
                 }
+                rhi = min(rlo + (ci + 1) * step - 1, rhi);
+                rlo = rlo + ci * step;
             }
-            for (int d = 32; d > 0; d >>= 1) {
-                s1 += __shfl_xor(s1, d);
-                s2 += __shfl_xor(s2, d);
-                first = min(first, __shfl_xor(first, d));
-            }
-            if (lane == 0) {
-                sred[par][0][wid] = s1;
-                sred[par][1][wid] = s2;
-                sfirst[par][wid] = first;
-            }
-            __syncthreads();
-            s1 = s2 = 0.0;
-            first = 0x7fffffff;
-            for (int w = 0; w < MODE_THREADS / 64; w++) {
-                s1 += sred[par][0][w];
-                s2 += sred[par][1][w];
-                first = min(first, sfirst[par][w]);
-            }
-            par ^= 1;
             sum = (unsigned long)(top - base);
-            mea = s1;
-            sig = s2;
+            mea = (double)s1;
+            sig = (double)s2;
             const long a_lo = first == 0x7fffffff ? nb : (long)first - lcut;
             lo = lcut + a_lo;
             hi = lo - 1;

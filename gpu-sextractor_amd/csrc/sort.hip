@@ -7,10 +7,11 @@
  * One pass per 8-bit digit:
  *   k_rs_hist     per tile of 4096 elements, digit histogram in LDS;
  *   scan          exclusive scan of the digit-major (digit, tile) counts;
- *   k_rs_scatter  per tile, 16 rounds of 256 elements in index order: lanes with equal
- *                 digits find each other with 8 ballots (one per digit bit), their rank
- *                 is the popcount of the lower lanes, and wave totals are combined in
- *                 LDS; a running per-digit counter keeps the scatter stable.
+ *   k_rs_scatter  per tile (its pairs read into registers first), 16 rounds of 256
+ *                 elements in index order: lanes with equal digits find each other with
+ *                 8 ballots (one per digit bit), their rank is the popcount of the lower
+ *                 lanes, and wave totals are combined in LDS; a running per-digit
+ *                 counter keeps the scatter stable.
  */
 #include "sxg_internal.h"
 
@@ -28,10 +29,17 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const int *__restrict__ 
     h[threadIdx.x] = 0;
     __syncthreads();
     const long base = (long)blockIdx.x * RS_TILE;
+    /* every key of the thread read before the first is counted */
+    int kr[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
-        long e = base + (long)r * RS_THREADS + threadIdx.x;
-        if (e < n) atomicAdd(&h[((unsigned)keys[e] >> shift) & 255], 1);
+        const long e = base + (long)r * RS_THREADS + threadIdx.x;
+        kr[r] = e < n ? keys[e] : -1;
     }
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++)
+        if (base + (long)r * RS_THREADS + threadIdx.x < n)
+            atomicAdd(&h[((unsigned)kr[r] >> shift) & 255], 1);
     __syncthreads();
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
@@ -51,10 +59,20 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const int *__restrict
     __syncthreads();
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const long base = (long)blockIdx.x * RS_TILE;
+    /* the thread's pairs of all rounds read before the first round: one memory round
+       trip per tile instead of one per round */
+    int kr[RS_ROUNDS], vr[RS_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const long e = base + (long)r * RS_THREADS + t;
+        kr[r] = e < n ? kin[e] : 0;
+        vr[r] = e < n ? vin[e] : 0;
+    }
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         const long e = base + (long)r * RS_THREADS + t;
         const bool valid = e < n;
-        int key = valid ? kin[e] : 0, val = valid ? vin[e] : 0;
+        const int key = kr[r], val = vr[r];
         const int d = ((unsigned)key >> shift) & 255;
         unsigned long long m = __ballot(valid);
 #pragma unroll

@@ -59,6 +59,26 @@ __device__ inline double sxw_sum(double v)
     return v;
 }
 
+/* op over the 64 lanes of an int, the result in every lane, without LDS: DPP moves
+   inside each row of 16 lanes (quad permutations for the partners at distance 1 and 2,
+   then the half-row and row mirrors, each of which pairs a lane with one of the group it
+   has not seen yet) and one readlane per row.  Every lane of the wavefront must be
+   active (the readlanes take rows 0..3 from lanes 0, 16, 32 and 48). */
+template <class Op> __device__ inline int sxw_dpp_all(int v, Op op)
+{
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));    /* 1,0,3,2 */
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));    /* 2,3,0,1 */
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));   /* half mirror */
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));   /* mirror */
+    const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return op(op(a, b), op(c, d));
+}
+struct SxwMax { __device__ int operator()(int a, int b) const { return max(a, b); } };
+struct SxwMin { __device__ int operator()(int a, int b) const { return min(a, b); } };
+__device__ inline int sxw_dpp_max(int v) { return sxw_dpp_all(v, SxwMax{}); }
+__device__ inline int sxw_dpp_min(int v) { return sxw_dpp_all(v, SxwMin{}); }
+
 /* keep the five largest values seen by this lane, t[0] >= ... >= t[4] (-inf = empty) */
 struct SxwTop5 {
     float t[5];

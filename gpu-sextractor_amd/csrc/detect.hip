@@ -175,9 +175,9 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
         const int r = __shfl(rk, leader);
         const bool mine = act && rk == r;
         const unsigned long long grp = __ballot(mine);
-        const int pk = sxw_dpp_max(mine ? pv : (int)0x80000000);
-        const int x0 = sxw_dpp_min(mine ? x : 0x7fffffff), x1 = sxw_dpp_max(mine ? x : -1);
-        const int y0 = sxw_dpp_min(mine ? y : 0x7fffffff), y1 = sxw_dpp_max(mine ? y : -1);
+        const int pk = sxw_max(mine ? pv : (int)0x80000000);
+        const int x0 = sxw_min(mine ? x : 0x7fffffff), x1 = sxw_max(mine ? x : -1);
+        const int y0 = sxw_min(mine ? y : 0x7fffffff), y1 = sxw_max(mine ? y : -1);
         if (lane == leader) {
             atomicAdd(&rs.cnt[r], __popcll(grp));
             atomicMax(&rs.peak[r], pk);
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
         }
         if (perpix) {
             /* the lowest pixel threshold of the object */
-            const int t0 = sxw_dpp_min(mine ? tv : 0x7fffffff);
+            const int t0 = sxw_min(mine ? tv : 0x7fffffff);
             if (lane == leader) atomicMin(&rs.t0[r], t0);
         }
         if (mine) act = false;

@@ -38,21 +38,14 @@ __device__ inline float sxw_minf(float v)
     for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
     return v;
 }
-__device__ inline int sxw_max(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ inline int sxw_min(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ inline int sxw_sum(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
+template <class Op> __device__ inline int sxw_dpp_all(int v, Op op);
+struct SxwMax { __device__ int operator()(int a, int b) const { return max(a, b); } };
+struct SxwMin { __device__ int operator()(int a, int b) const { return min(a, b); } };
+struct SxwAdd { __device__ int operator()(int a, int b) const { return a + b; } };
+/* int reductions over the full wavefront (every lane active), without LDS */
+__device__ inline int sxw_max(int v) { return sxw_dpp_all(v, SxwMax{}); }
+__device__ inline int sxw_min(int v) { return sxw_dpp_all(v, SxwMin{}); }
+__device__ inline int sxw_sum(int v) { return sxw_dpp_all(v, SxwAdd{}); }
 __device__ inline double sxw_sum(double v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -74,10 +67,6 @@ template <class Op> __device__ inline int sxw_dpp_all(int v, Op op)
     const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
     return op(op(a, b), op(c, d));
 }
-struct SxwMax { __device__ int operator()(int a, int b) const { return max(a, b); } };
-struct SxwMin { __device__ int operator()(int a, int b) const { return min(a, b); } };
-__device__ inline int sxw_dpp_max(int v) { return sxw_dpp_all(v, SxwMax{}); }
-__device__ inline int sxw_dpp_min(int v) { return sxw_dpp_all(v, SxwMin{}); }
 
 /* keep the five largest values seen by this lane, t[0] >= ... >= t[4] (-inf = empty) */
 struct SxwTop5 {

@@ -13,7 +13,12 @@
  *
  * Layout: one lane per object i scans its window of following objects, staged through
  * LDS per workgroup of 256 objects; masters are combined with atomicMin, so the result
- * does not depend on scheduling.
+ * does not depend on scheduling.  Every 64 consecutive objects have a bounding box of
+ * their centres and their largest a (k_clean_prep); a wavefront skips the 64 staged
+ * objects whose box is farther from the box of its own 64 than CLEAN_ZONE times the sum
+ * of the two largest a, with a margin above the rounding of the pair test, since no
+ * pair of them can pass it (the objects follow the raster order of their roots, so most
+ * of a window is far away).
  */
 #include <cmath>
 #include <algorithm>
@@ -25,16 +30,57 @@
 
 #define CLEAN_ZONE 10.0
 
+/* the box of each group of 64 objects: min and max of mx and my, the largest a */
+struct CleanBox {
+    double x0, x1, y0, y1, amax;
+};
+
+__device__ inline double wave_dmax(double v)
+{
+    for (int d = 32; d > 0; d >>= 1) v = fmax(v, __shfl_xor(v, d));
+    return v;
+}
+
 __global__ void __launch_bounds__(256) k_clean_prep(ObjCols o, int nobj, double clean_param,
                                                     double *amp, double *alpha, unsigned *master,
-                                                    const int *nd)
+                                                    CleanBox *box, const int *nd)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= sxg_dn(nd, nobj)) return;
-    const float unitarea = (float)(SXG_PI * o.a[i] * o.b[i]);
-    const double ampin = amp[i] = o.fdflux[i] / (2 * unitarea * o.abcor[i]);
-    alpha[i] = (sx_pow(ampin / o.dthresh[i], 1.0 / clean_param) - 1) * unitarea / o.fdnpix[i];
-    master[i] = 0xffffffffu;
+    const bool live = i < sxg_dn(nd, nobj);
+    double x = -INFINITY, y = -INFINITY, nx = -INFINITY, ny = -INFINITY, am = -INFINITY;
+    if (live) {
+        const float unitarea = (float)(SXG_PI * o.a[i] * o.b[i]);
+        const double ampin = amp[i] = o.fdflux[i] / (2 * unitarea * o.abcor[i]);
+        alpha[i] = (sx_pow(ampin / o.dthresh[i], 1.0 / clean_param) - 1) * unitarea / o.fdnpix[i];
+        master[i] = 0xffffffffu;
+        x = o.mx[i];
+        y = o.my[i];
+        nx = -x;
+        ny = -y;
+        am = o.a[i];
+    }
+    /* the group is the wavefront: 64 consecutive objects */
+    x = wave_dmax(x);
+    y = wave_dmax(y);
+    nx = wave_dmax(nx);
+    ny = wave_dmax(ny);
+    am = wave_dmax(am);
+    if ((threadIdx.x & 63) == 0 && i < sxg_dn(nd, nobj)) {
+        CleanBox b;
+        b.x0 = -nx; b.x1 = x; b.y0 = -ny; b.y1 = y; b.amax = am;
+        box[i >> 6] = b;
+    }
+}
+
+/* no pair of the two groups of objects can pass the CLEAN_ZONE test: the squared
+   distance of their boxes exceeds (a_i + a_j)^2 CLEAN_ZONE^2 for the largest a of each by
+   more than the pair test's float rounding (a few 1e-7 relative) */
+__device__ inline bool clean_far(const CleanBox &p, const CleanBox &q)
+{
+    const double dx = fmax(0.0, fmax(q.x0 - p.x1, p.x0 - q.x1));
+    const double dy = fmax(0.0, fmax(q.y0 - p.y1, p.y0 - q.y1));
+    const double r = (p.amax + q.amax) * CLEAN_ZONE;
+    return dx * dx + dy * dy > r * r * 1.0001 + 1e-9;
 }
 
 /* one workgroup per 256 consecutive objects i; the window of following objects is
@@ -43,7 +89,9 @@ __global__ void __launch_bounds__(256) k_clean_prep(ObjCols o, int nobj, double 
 __global__ void __launch_bounds__(CP_T) k_clean_pairs(ObjCols o, int nobj, int win,
                                                       const double *__restrict__ amp,
                                                       const double *__restrict__ alpha,
-                                                      unsigned *master, const int *nd)
+                                                      unsigned *master,
+                                                      const CleanBox *__restrict__ box,
+                                                      const int *nd)
 {
     __shared__ double smx[CP_T], smy[CP_T], samp[CP_T], salpha[CP_T];
     __shared__ float sa[CP_T], sf[CP_T], scxx[CP_T], scyy[CP_T], scxy[CP_T], smt[CP_T];
@@ -61,6 +109,9 @@ __global__ void __launch_bounds__(CP_T) k_clean_pairs(ObjCols o, int nobj, int w
     const int jmax = live ? min(i + win, nobj - 1) : -1;
     unsigned mine = 0xffffffffu;
     const int jlast = min(i0 + CP_T - 1 + win, nobj - 1);
+    /* the box of this wavefront's 64 objects (a wavefront past the last object reads
+       the last group's box and tests nothing) */
+    const CleanBox bi = box[min((i0 >> 6) + (t >> 6), (nobj - 1) >> 6)];
     /* the window's tiles are shared out over gridDim.y workgroups */
     for (int j0 = i0 + 1 + blockIdx.y * CP_T; j0 <= jlast; j0 += CP_T * gridDim.y) {
         __syncthreads();
@@ -78,23 +129,35 @@ __global__ void __launch_bounds__(CP_T) k_clean_pairs(ObjCols o, int nobj, int w
             smt[t] = o.mthresh[jl];
         }
         __syncthreads();
-        const int jb = max(j0, i + 1), je = min(j0 + CP_T - 1, jmax);
-        for (int j = jb; j <= je; j++) {
-            const int q = j - j0;
-            const float dx = (float)(mxi - smx[q]);
-            const float dy = (float)(myi - smy[q]);
-            float rlim = ai + sa[q];
-            rlim *= rlim;
-            if (dx * dx + dy * dy < rlim * CLEAN_ZONE * CLEAN_ZONE) {
-                double val;
-                if (fi < sf[q]) {
-                    val = 1 + salpha[q] * (scxx[q] * dx * dx + scyy[q] * dy * dy + scxy[q] * dx * dy);
-                    if (val > 1.0 && ((float)(val < 1e10 ? samp[q] * (1.0 / val) : 0.0) > mti))
-                        mine = min(mine, (unsigned)j);
-                } else {
-                    val = 1 + alphai * (cxxi * dx * dx + cyyi * dy * dy + cxyi * dx * dy);
-                    if (val > 1.0 && ((float)(val < 1e10 ? ampi * (1.0 / val) : 0.0) > smt[q]))
-                        atomicMin(&master[j], (unsigned)i);
+        for (int s0 = j0; s0 < j0 + CP_T && s0 <= jlast; s0 += 64) {
+            /* the 64 staged objects from s0: the union of the two boxes covering them */
+            const int s1 = min(s0 + 63, jlast);
+            CleanBox bj = box[s0 >> 6];
+            if ((s1 >> 6) != (s0 >> 6)) {
+                const CleanBox b2 = box[s1 >> 6];
+                bj.x0 = fmin(bj.x0, b2.x0); bj.x1 = fmax(bj.x1, b2.x1);
+                bj.y0 = fmin(bj.y0, b2.y0); bj.y1 = fmax(bj.y1, b2.y1);
+                bj.amax = fmax(bj.amax, b2.amax);
+            }
+            if (clean_far(bi, bj)) continue;
+            const int jb = max(s0, i + 1), je = min(s1, jmax);
+            for (int j = jb; j <= je; j++) {
+                const int q = j - j0;
+                const float dx = (float)(mxi - smx[q]);
+                const float dy = (float)(myi - smy[q]);
+                float rlim = ai + sa[q];
+                rlim *= rlim;
+                if (dx * dx + dy * dy < rlim * CLEAN_ZONE * CLEAN_ZONE) {
+                    double val;
+                    if (fi < sf[q]) {
+                        val = 1 + salpha[q] * (scxx[q] * dx * dx + scyy[q] * dy * dy + scxy[q] * dx * dy);
+                        if (val > 1.0 && ((float)(val < 1e10 ? samp[q] * (1.0 / val) : 0.0) > mti))
+                            mine = min(mine, (unsigned)j);
+                    } else {
+                        val = 1 + alphai * (cxxi * dx * dx + cyyi * dy * dy + cxyi * dx * dy);
+                        if (val > 1.0 && ((float)(val < 1e10 ? ampi * (1.0 / val) : 0.0) > smt[q]))
+                            atomicMin(&master[j], (unsigned)i);
+                    }
                 }
             }
         }
@@ -118,11 +181,12 @@ extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacks
     const int n = numobj;   /* a capacity in a planned run (the count is on the device) */
     const int *nd = S.plan ? &S.pdev->nobj : nullptr;
     double *amp = sxg_take<double>("clean.amp", n), *alpha = sxg_take<double>("clean.alpha", n);
-    if (!amp || !alpha) return;
+    CleanBox *box = sxg_take<CleanBox>("clean.box", (size_t)sxg_div_up(n > 0 ? n : 1, 64));
+    if (!amp || !alpha || !box) return;
     sxg_stage_begin(ST_CLEAN);
     if (n > 0) {
         hipLaunchKernelGGL(k_clean_prep, dim3(sxg_div_up(n, 256)), dim3(256), 0, S.stream, o, n,
-                           clean_param, amp, alpha, o.master, nd);
+                           clean_param, amp, alpha, o.master, box, nd);
         /* CLEAN N: keep everything (the reference cleans regardless of the flag) */
         if (S.clean_enabled) {
             const int win = ((clean_stacksize - 1) / 16 + 1) * 16;
@@ -131,7 +195,7 @@ extern "C" void run_clean(int *masterIndex, double clean_param, int clean_stacks
             const int ntile = (win + CP_T - 1) / CP_T + 1;
             const int slices = std::max(1, std::min(8, ntile));
             hipLaunchKernelGGL(k_clean_pairs, dim3(sxg_div_up(n, 256), slices), dim3(256), 0, S.stream, o,
-                               n, win, amp, alpha, o.master, nd);
+                               n, win, amp, alpha, o.master, box, nd);
             sxg_kt_end(SXG_KT_CLEAN);
         }
     }

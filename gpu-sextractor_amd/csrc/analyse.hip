@@ -245,7 +245,6 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
     if (thresh0 < thr_o) thresh0 = thr_o;
     const double sat = (double)(A.satur_level - dbkg);
     double eacc = 0.0;
-    float fdummy = 0.0f;
     /* the next chunk's pixels are read while this chunk is folded */
     int np_ = lane < npx ? P[lane] : 0;
     float nsub = lane < npx ? A.sub[np_] : 0.0f, ncd = lane < npx ? A.cd[np_] : 0.0f;
@@ -300,8 +299,7 @@ __global__ void __launch_bounds__(256) k_analyse(AnaArgs A)
         /* tv, sigtv, esum, emx2, emy2, emxy (the error sums only with errflag), then the
            FWHM fit sums */
         const double t11[11] = {pix, var2, err, ex2, ey2, exy, ts, tsx, tsxx, tsy, tsxy};
-        const float tf[1] = {0.0f};
-        sxw_fold2<11, 1>(rows, frows, t11, tf, k < npx, eacc, fdummy);
+        sxw_fold<11, double>(rows, t11, k < npx, eacc);
     }
     double tv = sxw_rl(eacc, 0), sigtv = sxw_rl(eacc, 1);
     const double emx2 = sxw_rl(eacc, 3), emy2 = sxw_rl(eacc, 4), emxy = sxw_rl(eacc, 5);

@@ -28,16 +28,12 @@ __device__ inline double sxw_rl(double v, int j)
     return __hiloint2double(hi, lo);
 }
 
-__device__ inline float sxw_max(float v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ inline float sxw_minf(float v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    return v;
-}
+template <class Op> __device__ inline float sxw_dpp_allf(float v, Op op);
+struct SxwMaxF { __device__ float operator()(float a, float b) const { return fmaxf(a, b); } };
+struct SxwMinF { __device__ float operator()(float a, float b) const { return fminf(a, b); } };
+/* float extrema over the full wavefront (every lane active), without LDS */
+__device__ inline float sxw_max(float v) { return sxw_dpp_allf(v, SxwMaxF{}); }
+__device__ inline float sxw_minf(float v) { return sxw_dpp_allf(v, SxwMinF{}); }
 template <class Op> __device__ inline int sxw_dpp_all(int v, Op op);
 struct SxwMax { __device__ int operator()(int a, int b) const { return max(a, b); } };
 struct SxwMin { __device__ int operator()(int a, int b) const { return min(a, b); } };
@@ -65,6 +61,26 @@ template <class Op> __device__ inline int sxw_dpp_all(int v, Op op)
     v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));   /* mirror */
     const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
     const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return op(op(a, b), op(c, d));
+}
+template <class Op> __device__ inline float sxw_dpp_allf(float v, Op op)
+{
+    auto mv = [](float x, int ctrl) {
+        const int i = __float_as_int(x);
+        int r;
+        switch (ctrl) {
+        case 0: r = __builtin_amdgcn_update_dpp(i, i, 0xB1, 0xF, 0xF, false); break;
+        case 1: r = __builtin_amdgcn_update_dpp(i, i, 0x4E, 0xF, 0xF, false); break;
+        case 2: r = __builtin_amdgcn_update_dpp(i, i, 0x141, 0xF, 0xF, false); break;
+        default: r = __builtin_amdgcn_update_dpp(i, i, 0x140, 0xF, 0xF, false); break;
+        }
+        return __int_as_float(r);
+    };
+    v = op(v, mv(v, 0));
+    v = op(v, mv(v, 1));
+    v = op(v, mv(v, 2));
+    v = op(v, mv(v, 3));
+    const float a = sxw_rl(v, 0), b = sxw_rl(v, 16), c = sxw_rl(v, 32), d = sxw_rl(v, 48);
     return op(op(a, b), op(c, d));
 }
 

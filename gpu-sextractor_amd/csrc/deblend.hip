@@ -36,9 +36,9 @@
  *
  * Kernels:
  *   k_deb_thresh     one lane per (object, level): the sub-thresholds (canonical pow)
- *   k_deb_lmax<0,1>  one lane per raw pixel: local maxima; per object their number, the
- *                    highest (packed value and index, 64-bit atomicMax), then the
- *                    second-highest
+ *   k_deb_lmax<0,1>  one lane per raw pixel: local maxima (flagged per pixel); per object
+ *                    their number, the highest (packed value and index, 64-bit
+ *                    atomicMax), then, over the flagged pixels only, the second-highest
  *   k_deb_lsfirst    one lane per (object, level): L* from the second-highest maximum
  *   k_deb_nodecap    one lane per object: a bound on the tree's nodes (1 + L* times the
  *                    smaller of the local maxima and n / deb_minarea), scanned into the
@@ -345,6 +345,7 @@ struct DebArgs {
     unsigned long long *top1;   /* highest local maximum: uord(cd) << 32 | raw-list index */
     unsigned *top2;         /* uord(cd) of the second-highest local maximum, 0 if none */
     int *nmax;              /* local maxima per raw object */
+    unsigned char *lmflag;  /* per raw-list entry: 1 for a local maximum (first pass) */
     int *lstar;             /* first level above the second-highest maximum, then L* */
     int *clist;             /* objects of each size class: clist[cls * nraw + j] */
     int *ccount;            /* [0..3] objects per class, [4..6] the largest LDS need of
@@ -379,20 +380,27 @@ template <int PASS> __global__ void __launch_bounds__(256) k_deb_lmax(DebArgs a)
 {
     const long g = (long)blockIdx.x * 256 + threadIdx.x;
     if (g >= deb_P0(a)) return;
+    /* the second pass visits the local maxima the first one flagged */
+    if (PASS == 1 && !a.lmflag[g]) return;
     const int p = a.fpia[g], r = a.rawkey[g];
-    const int root = a.lab[p];
     const float v = a.cd[p];
-    const int W = a.W, H = a.H;
-    const int y = p / W, x = p - y * W;
-    for (int dy = -1; dy <= 1; dy++) {
-        const int yy = y + dy;
-        if (yy < 0 || yy >= H) continue;
-        for (int dx = -1; dx <= 1; dx++) {
-            const int xx = x + dx;
-            if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
-            const int q = yy * W + xx;
-            if (a.lab[q] == root && a.cd[q] > v) return;
+    if (PASS == 0) {
+        const int root = a.lab[p];
+        const int W = a.W, H = a.H;
+        const int y = p / W, x = p - y * W;
+        bool top = true;
+        for (int dy = -1; dy <= 1; dy++) {
+            const int yy = y + dy;
+            if (yy < 0 || yy >= H) continue;
+            for (int dx = -1; dx <= 1; dx++) {
+                const int xx = x + dx;
+                if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
+                const int q = yy * W + xx;
+                if (a.lab[q] == root && a.cd[q] > v) top = false;
+            }
         }
+        a.lmflag[g] = top ? 1 : 0;
+        if (!top) return;
     }
     const unsigned u = uord(v);
     const unsigned long long key = ((unsigned long long)u << 32) | (unsigned)g;
@@ -1881,8 +1889,9 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     auto *dwork = sxg_take<unsigned long long>("deb.dwork", 6);
     auto *lvbase = sxg_take<unsigned long long>("deb.lvbase", 2 * (size_t)nr);
     int *kcomp = sxg_take<int>("deb.kcomp", (size_t)(P0 > 0 ? P0 : 1));
+    auto *lmflag = sxg_take<unsigned char>("deb.lmflag", (size_t)(P0 > 0 ? P0 : 1));
     if (!compoff || !dtot || !rootnc || !rootkept || !objbase || !alloc || !top1 || !top2 ||
-        !nmax || !lstar || !clist || !ccount || !dwork || !kcomp || !lvbase)
+        !nmax || !lstar || !clist || !ccount || !dwork || !kcomp || !lvbase || !lmflag)
         return -1;
     DebArgs a{};
     a.fpia = S.fpia; a.rawkey = S.fpia_keys; a.off = ro.off; a.cnt = ro.count; a.first = ro.first;
@@ -1894,6 +1903,7 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
     a.cstride = nraw; a.pd = pd;
     a.rootnc = rootnc; a.rootkept = rootkept;
     a.top1 = top1; a.top2 = top2; a.nmax = nmax; a.lstar = lstar; a.clist = clist;
+    a.lmflag = lmflag;
     a.ccount = ccount; a.dwork = dwork; a.lvbase = lvbase;
     S.deb_pixels = S.deb_labels = 0;
     for (int c = 0; c < 4; c++) S.deb_class_pixels[c] = 0;

@@ -115,17 +115,36 @@ __global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, i
     }
 }
 
+/* Runs inside a wavefront: the foreground list is in raster order, so a lane whose
+   previous lane holds its W neighbour (p - 1 on the same row) is in that lane's
+   component.  run_head() gives each active lane the lane of the first pixel of its run
+   within the wavefront; a value found by that lane serves the whole run. */
+__device__ inline int run_head(bool act, int p, int W)
+{
+    const int lane = threadIdx.x & 63;
+    const int pl = __shfl_up(p, 1);
+    const bool cont = act && lane > 0 && pl == p - 1 && p % W != 0;
+    const unsigned long long S = __ballot(act && !cont);
+    const unsigned long long le = S & (~0ull >> (63 - lane));
+    return le ? 63 - __builtin_clzll(le) : lane;
+}
+
 /* full path compression: every detected pixel's parent becomes its root (a valid parent
    while other lanes still walk: the root is an ancestor with the smallest index, so the
    path halving's atomicMin never moves a pointer past it), and roots remember their
-   foreground position */
-__global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg, int nfg, int *par,
-                                                     int *rootpos, const int *nd)
+   foreground position.  Only the first pixel of each run in a wavefront walks; the
+   others take its root. */
+__global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg, int nfg, int W,
+                                                     int *par, int *rootpos, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= sxg_dn(nd, nfg)) return;
-    const int p = fg[k];
-    const int r = uf_root(par, p);
+    const bool act = k < sxg_dn(nd, nfg);
+    const int p = act ? fg[k] : -1;
+    const int h = run_head(act, p, W);
+    int r = -1;
+    if (act && h == (int)(threadIdx.x & 63)) r = uf_root(par, p);
+    r = __shfl(r, h);
+    if (!act) return;
     par[p] = r;
     if (r == p) rootpos[p] = k;
 }
@@ -161,9 +180,13 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
     const bool perpix = wa.w && wa.rel;
     bool act = k < sxg_dn(nd, nfg);
     int rk = -1, x = 0, y = 0, pv = 0, tv = 0x7fffffff;
+    const int pp = act ? fg[k] : -1;
+    /* the root's position, looked up by the first pixel of each run */
+    const int h = run_head(act, pp, W);
+    if (act && h == lane) rk = rootpos[lab[pp]];
+    rk = __shfl(rk, h);
     if (act) {
-        const int p = fg[k];
-        rk = rootpos[lab[p]];
+        const int p = pp;
         y = p / W;
         x = p - y * W;
         pv = f2ord(cd[p]);
@@ -235,15 +258,21 @@ __global__ void __launch_bounds__(256) k_det_fill(int *a, int n, int v, const in
 }
 
 /* object id of each detected pixel (or -1 when its component is too small) */
-__global__ void __launch_bounds__(256) k_det_pixkey(const int *__restrict__ fg, int nfg,
+__global__ void __launch_bounds__(256) k_det_pixkey(const int *__restrict__ fg, int nfg, int W,
                                                     const int *__restrict__ lab,
                                                     const int *__restrict__ rootpos,
                                                     const int *__restrict__ rawid_of_pos,
                                                     int *key, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= sxg_dn(nd, nfg)) return;
-    key[k] = rawid_of_pos[rootpos[lab[fg[k]]]];
+    const bool act = k < sxg_dn(nd, nfg);
+    const int p = act ? fg[k] : -1;
+    /* one lookup chain per run of the wavefront */
+    const int h = run_head(act, p, W);
+    int v = -1;
+    if (act && h == (int)(threadIdx.x & 63)) v = rawid_of_pos[rootpos[lab[p]]];
+    v = __shfl(v, h);
+    if (act) key[k] = v;
 }
 
 __global__ void __launch_bounds__(256) k_det_gather(const int *__restrict__ idx, int n,
@@ -359,8 +388,8 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         hipLaunchKernelGGL(k_det_union, dim3(gfg), dim3(256), 0, st, fg, nfg, W, par, nfg_k);
         sxg_kt_end(SXG_KT_DET_UNION);
         sxg_kt_begin(SXG_KT_DET_FLATTEN);
-        hipLaunchKernelGGL(k_det_flatten, dim3(gfg), dim3(256), 0, st, fg, nfg, par, rootpos,
-                           nfg_k);
+        hipLaunchKernelGGL(k_det_flatten, dim3(gfg), dim3(256), 0, st, fg, nfg, W, par,
+                           rootpos, nfg_k);
         sxg_kt_end(SXG_KT_DET_FLATTEN);
         hipLaunchKernelGGL(k_det_statinit, dim3(gfg), dim3(256), 0, st, rs, nfg, nfg_k);
         sxg_kt_begin(SXG_KT_DET_STAT);
@@ -389,8 +418,8 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
                                nraw, fg, rs, rawid_of_pos, ro, dthreshold, (int)(wa.w && wa.rel),
                                nraw_k, td);
         /* pixel keys, kept pixels, stable sort by object */
-        hipLaunchKernelGGL(k_det_pixkey, dim3(gfg), dim3(256), 0, st, fg, nfg, lab, rootpos,
-                           rawid_of_pos, keep, nfg_k);
+        hipLaunchKernelGGL(k_det_pixkey, dim3(gfg), dim3(256), 0, st, fg, nfg, W, lab,
+                           rootpos, rawid_of_pos, keep, nfg_k);
     }
     int *idx = rawpos;   /* reuse: indices of kept pixels in the foreground list */
     int *k0 = sxg_take<int>("det.k0", nfg), *v0 = sxg_take<int>("det.v0", nfg);

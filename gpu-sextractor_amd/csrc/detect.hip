@@ -200,7 +200,10 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
         const unsigned long long grp = __ballot(mine);
         const int pk = sxw_max(mine ? pv : (int)0x80000000);
         const int x0 = sxw_min(mine ? x : 0x7fffffff), x1 = sxw_max(mine ? x : -1);
-        const int y0 = sxw_min(mine ? y : 0x7fffffff), y1 = sxw_max(mine ? y : -1);
+        /* the list is in raster order: y grows with the lane, so the group's first and
+           last lanes hold its extreme rows */
+        const int y0 = __builtin_amdgcn_readlane(y, leader);
+        const int y1 = __builtin_amdgcn_readlane(y, 63 - __builtin_clzll(grp));
         if (lane == leader) {
             atomicAdd(&rs.cnt[r], __popcll(grp));
             atomicMax(&rs.peak[r], pk);

@@ -147,7 +147,7 @@ __device__ inline void sxw_fold(T *rows, const T (&t)[NS], bool member, T &acc)
     if (lane < NS) {
         const T *row = rows + lane * SXW_ROW;
         T a = acc;
-#pragma unroll 8
+#pragma unroll 16
         for (int j = 0; j < cnt; j++) a += row[j];
         acc = a;
     }
@@ -186,7 +186,7 @@ __device__ inline void sxw_fold2(double *drows, float *frows, const double (&td)
         const float *fr = frows + (lane >= ND ? lane - ND : 0) * SXW_ROW;
         double da = dacc;
         float fa = facc;
-#pragma unroll 8
+#pragma unroll 16
         for (int j = 0; j < cnt; j++) {
             da += dr[j];
             fa += fr[j];

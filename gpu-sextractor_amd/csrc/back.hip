@@ -623,6 +623,60 @@ __global__ void __launch_bounds__(256) k_fb_median(const float *back2, const flo
     if (npx2 > px) npx2 = px;
     if (npy2 > npy) npy2 = npy;
     if (npy2 > py) npy2 = py;
+    if (npx <= 1 && npy <= 1) {
+        /* windows up to 3 x 3 in registers: the clipped window's values, +inf for the
+           cells outside it, sorted by an odd-even transposition network (nine rounds
+           sort nine values); its element k/2 is hmedian's for an odd count.  Equal
+           values are then bitwise equal unless a zero or a NaN is present (+0 and -0,
+           or the NaN orderings, where insertion order matters): those windows take
+           the general path below. */
+        float b9[9], s9[9];
+        bool plain = true;
+        int k = 0;
+#pragma unroll
+        for (int u = 0; u < 9; u++) {
+            const int dy = u / 3 - 1, dx = u % 3 - 1;
+            const bool in = dy >= -npy2 && dy <= npy2 && dx >= -npx2 && dx <= npx2;
+            const int q = (py + (in ? dy : 0)) * nx + px + (in ? dx : 0);
+            const float bv = back2[q], sv = sigma2[q];
+            b9[u] = in ? bv : INFINITY;
+            s9[u] = in ? sv : INFINITY;
+            plain = plain && (!in || (bv != 0.0f && bv == bv && sv != 0.0f && sv == sv));
+            k += in ? 1 : 0;
+        }
+        if (plain && (k & 1)) {
+            auto sort9 = [](float (&a)[9]) {
+#pragma unroll
+                for (int r = 0; r < 9; r++)
+#pragma unroll
+                    for (int j = r & 1; j + 1 < 9; j += 2) {
+                        const float lo = a[j + 1] < a[j] ? a[j + 1] : a[j];
+                        const float hi = a[j + 1] < a[j] ? a[j] : a[j + 1];
+                        a[j] = lo;
+                        a[j + 1] = hi;
+                    }
+            };
+            auto pick = [k](const float (&a)[9]) {
+                float m = a[0];
+#pragma unroll
+                for (int u = 1; u < 9; u++)
+                    if (u == k / 2) m = a[u];
+                return m;
+            };
+            sort9(b9);
+            const float med = pick(b9);
+            const float b2i = back2[i], s2i = sigma2[i];
+            if (fabs(med - b2i) >= fthresh) {
+                sort9(s9);
+                back[i] = med;
+                sigma[i] = pick(s9);
+            } else {
+                back[i] = b2i;
+                sigma[i] = s2i;
+            }
+            return;
+        }
+    }
     float bm[SXG_MAXBACKF * SXG_MAXBACKF], sm_[SXG_MAXBACKF * SXG_MAXBACKF];
     int k = 0;
     for (int dy = -npy2; dy <= npy2; dy++)

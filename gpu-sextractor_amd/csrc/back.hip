@@ -267,19 +267,20 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
                 for (int r = 0; r < nr; r++) {
                     const double *row = src + r * rstride;
                     int x = 0;
-                    if (bw >= 16) {
+                    if (bw >= 8) {
+                        /* groups of eight, the next group read while this one is added;
+                           the last full group ends the row (bw = 64: no single-value
+                           tail, whose reads would each wait for the LDS) */
                         double v[8], w[8];
 #pragma unroll
                         for (int u = 0; u < 8; u++) v[u] = row[u];
-                        for (x = 8; x + 16 <= bw; x += 16) {
+                        for (x = 8; x + 8 <= bw; x += 8) {
 #pragma unroll
                             for (int u = 0; u < 8; u++) w[u] = row[x + u];
 #pragma unroll
                             for (int u = 0; u < 8; u++) acc += v[u];
 #pragma unroll
-                            for (int u = 0; u < 8; u++) v[u] = row[x + 8 + u];
-#pragma unroll
-                            for (int u = 0; u < 8; u++) acc += w[u];
+                            for (int u = 0; u < 8; u++) v[u] = w[u];
                         }
 #pragma unroll
                         for (int u = 0; u < 8; u++) acc += v[u];

@@ -106,9 +106,9 @@ SINGLE_KERNELS = {
     "subback": ("k_subback<true>", lambda N, i: 8 * N),
     "conv": ("k_convolve<K>", lambda N, i: 8 * N),
     "analyse": ("k_analyse", lambda N, i: 12 * i["npix_raw"] + 176 * i["nobj"]),
-    "deb_tree0": ("k_deb_tree_lds<256>", lambda N, i: 12 * i["deb_class_pixels"][0]),
-    "deb_tree1": ("k_deb_tree_lds<1024> (class 1)", lambda N, i: 12 * i["deb_class_pixels"][1]),
-    "deb_tree2": ("k_deb_tree_lds<1024> (class 2)", lambda N, i: 12 * i["deb_class_pixels"][2]),
+    "deb_tree0": ("k_deb_tree_lds<256, 0>", lambda N, i: 12 * i["deb_class_pixels"][0]),
+    "deb_tree1": ("k_deb_tree_lds<1024, 1>", lambda N, i: 12 * i["deb_class_pixels"][1]),
+    "deb_tree2": ("k_deb_tree_lds<1024, 2>", lambda N, i: 12 * i["deb_class_pixels"][2]),
 }
 
 
@@ -116,12 +116,12 @@ SINGLE_KERNELS = {
 # 1001) with two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE), each scaled by what the
 # same counter reads for a plain 256 MiB device-to-device copy (bench.py --calibrate-copy);
 # see profiles/hbm_traffic_crowded4096_pmc.md.  Used for roofline.traffic on that workload.
-# (k_deb_tree_lds<1024> runs twice per frame, classes 1 and 2; the counters see one kernel
-# name, so both timers carry the mean of its two launches).  "conv_fused" is the whole-frame
-# runs' k_subconv_det<7>, the kernel the conv timer brackets when no subback runs.
+# (k_deb_tree_lds<NT, C> carries its size class C in its name, so each class has its own
+# figure).  "conv_fused" is the whole-frame runs' k_subconv_det<7>, the kernel the conv
+# timer brackets when no subback runs.
 MEASURED_TRAFFIC_CROWDED4096 = {
     "meshstat": 1.346e8, "meshmode": 6.859e7, "analyse": 6.599e7, "conv_fused": 2.724e8,
-    "deb_tree0": 1.070e7, "deb_tree1": 1.04e6, "deb_tree2": 1.04e6,
+    "deb_tree0": 1.069e7, "deb_tree1": 1.12e6, "deb_tree2": 9.4e5,
 }
 
 

@@ -1197,7 +1197,9 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
     }
 }
 
-template <int NT>
+/* CLS names the size class in the kernel's symbol only (classes 1 and 2 share the code
+   and the workgroup size; a kernel trace then times each class on its own) */
+template <int NT, int CLS>
 __global__ void __launch_bounds__(NT) k_deb_tree_lds(DebArgs a, int cls)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1987,10 +1989,13 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
            it is raised once, to the most any class asks for (DEB_LDS_LIMIT), never per
            launch (batch workers launch these kernels from several host threads) */
         static const bool lds_attr = [] {
-            return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_deb_tree_lds<1024>),
+            return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_deb_tree_lds<1024, 2>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)DEB_LDS_LIMIT) == hipSuccess &&
-                   hipFuncSetAttribute(reinterpret_cast<const void *>(&k_deb_tree_lds<256>),
+                   hipFuncSetAttribute(reinterpret_cast<const void *>(&k_deb_tree_lds<1024, 1>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)DEB_LDS_LIMIT) == hipSuccess &&
+                   hipFuncSetAttribute(reinterpret_cast<const void *>(&k_deb_tree_lds<256, 0>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)DEB_LDS_LIMIT) == hipSuccess;
         }();
@@ -2077,13 +2082,13 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             }
         }
         if (cc[2] > 0 &&
-            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<1024>), 2, 1024, ct2))
+            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<1024, 2>), 2, 1024, ct2))
             return -1;
         if (cc[1] > 0 &&
-            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<1024>), 1, 1024, ct1))
+            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<1024, 1>), 1, 1024, ct1))
             return -1;
         if (cc[0] > 0 &&
-            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<256>), 0, 256, ct0))
+            !launch_lds(reinterpret_cast<const void *>(&k_deb_tree_lds<256, 0>), 0, 256, ct0))
             return -1;
         for (int j = 0; j < nside; j++) SXG_HIP_RET(hipStreamWaitEvent(st, S.side_ev[j], 0), -1);
         sxg_kt_end(SXG_KT_DEB_LEVELS);

@@ -1782,6 +1782,91 @@ __global__ void __launch_bounds__(256) k_deb_objcount(ObjArgs A, int *objcnt)
     }
 }
 
+/*
+ * The raw pixels regrouped by object, stably (each object's pixels in the raster order
+ * of its raw object's list), from the object offsets: a raw object kept as one object
+ * moves as one block (k_obj_place1, a lane per pixel); a raw object split into several,
+ * one wavefront each (k_obj_place_multi), walks its list 64 entries at a time with the
+ * next free position of each of its objects in LDS: the lanes of one object take
+ * consecutive positions in lane order, and the chunks follow the list's order.
+ */
+__global__ void __launch_bounds__(256) k_obj_place1(ObjArgs A, const int *__restrict__ objoff,
+                                                    int *opix)
+{
+    const long g = (long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= deb_P0(A.d)) return;
+    const int r = A.d.rawkey[g];
+    if (A.d.rootkept[r] != 1) return;
+    opix[objoff[A.okey[g]] + (int)(g - A.d.off[r])] = A.d.fpia[g];
+}
+
+/* positions kept in LDS per wavefront: objects of one raw object per window */
+#define OP_WIN 1024
+/* list chunks of 64 entries read ahead of the one being placed */
+#define OP_AHEAD 4
+__global__ void __launch_bounds__(256) k_obj_place_multi(ObjArgs A,
+                                                         const int *__restrict__ objoff,
+                                                         int *opix)
+{
+    __shared__ int s_next[4][OP_WIN];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + wid;
+    if (r >= deb_nraw(A.d)) return;
+    const int nk = A.d.rootkept[r];
+    if (nk < 2) return;
+    const int base = A.objbase[r], off = A.d.off[r], n = A.d.cnt[r];
+    const unsigned long long below = sxw_lanes_below(lane);
+    int *nxt = s_next[wid];
+    for (int w0 = 0; w0 < nk; w0 += OP_WIN) {
+        const int wn = min(OP_WIN, nk - w0);
+        for (int j = lane; j < wn; j += 64) nxt[j] = objoff[base + w0 + j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        /* chunk c's object ids and pixels, OP_AHEAD chunks in flight */
+        int qk[OP_AHEAD], qp[OP_AHEAD];
+#pragma unroll
+        for (int u = 0; u < OP_AHEAD; u++) {
+            const int k = u * 64 + lane;
+            qk[u] = k < n ? A.okey[off + k] : -1;
+            qp[u] = k < n ? A.d.fpia[off + k] : 0;
+        }
+        for (int c0 = 0; c0 < n; c0 += 64 * OP_AHEAD) {
+#pragma unroll
+            for (int u = 0; u < OP_AHEAD; u++) {
+                const int c = c0 + u * 64;
+                if (c >= n) break;
+                const int key = qk[u], pix = qp[u];
+                /* the same slot's chunk of the next round */
+                const int kn = c + 64 * OP_AHEAD + lane;
+                qk[u] = kn < n ? A.okey[off + kn] : -1;
+                qp[u] = kn < n ? A.d.fpia[off + kn] : 0;
+                const int j = key - base - w0;
+                bool act = c + lane < n && j >= 0 && j < wn;
+                unsigned long long pending = __ballot(act);
+                while (pending) {
+                    const int lead = __builtin_ctzll(pending);
+                    const int jl = __shfl(j, lead);
+                    const unsigned long long grp = __ballot(act && j == jl);
+                    const int start = nxt[jl];
+                    if (act && j == jl) {
+                        opix[start + __popcll(grp & below)] = pix;
+                        act = false;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (lane == lead) nxt[jl] = start + __popcll(grp);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    pending &= ~grp;
+                }
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_deb_zero(int *v, long n)
 {
     const long k = (long)blockIdx.x * 256 + threadIdx.x;
@@ -2162,12 +2247,11 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             sxg_scan_excl(objcnt, objoff, nobj + 1, nullptr, st);
         else
             sxg_scan_excl(objcnt, objoff, nobj, objoff + nobj, st);
-        int nbits = 1;
-        while (nbits < 31 && ((nobj - 1) >> nbits) != 0) nbits++;
-        const int where = sxg_radix_sort(k0, v0, k1, v1, (int)P0, nbits, hist, st,
-                                         pl ? &pd->npix : nullptr);
+        hipLaunchKernelGGL(k_obj_place1, dim3(gp), dim3(256), 0, st, A, (const int *)objoff, v1);
+        hipLaunchKernelGGL(k_obj_place_multi, dim3(sxg_div_up(nraw, 4)), dim3(256), 0, st, A,
+                           (const int *)objoff, v1);
         sxg_kt_end(SXG_KT_OBJ_SORT);
-        S.opix = where ? v1 : v0;
+        S.opix = v1;
         S.objoff = objoff;
     }
     sxg_stage_end(ST_DEBLEND);

@@ -1876,18 +1876,30 @@ __global__ void __launch_bounds__(256) k_deb_zero(int *v, long n)
 /* planned runs: the node total and the level-parallel arrays against their capacities
    (the 64-bit node sum too: a total past 2^31 is an overflow); on any overflow of the
    frame the class counts are zeroed so that no tree kernel has work */
+/* the capacities of the class words of the counter block: LDS bytes of classes 0..2,
+   level-parallel objects, objects per class 0..3 */
+struct DebGuardCaps {
+    long long lds[3], big, cls[4];
+};
+
 __global__ void k_deb_plan_guard(const unsigned long long *dw, PlanDev *pd, int *cc,
                                  const int *ntot, long long cap_tot, long long cap_npl,
-                                 long long cap_nrec)
+                                 long long cap_nrec, DebGuardCaps gc)
 {
     if (threadIdx.x != 0) return;
     if (dw[2] > (unsigned long long)cap_tot || (long long)*ntot > cap_tot || *ntot < 0)
         pd->ovf |= 1 << PL_NTOT;
     if (dw[3] > (unsigned long long)cap_npl) pd->ovf |= 1 << PL_NPL;
     if (dw[4] > (unsigned long long)cap_nrec) pd->ovf |= 1 << PL_NREC;
+    /* the class words against their capacities (one launch for all of them) */
+    for (int c = 0; c < 3; c++)
+        if (cc[4 + c] < 0 || (long long)cc[4 + c] > gc.lds[c]) pd->ovf |= 1 << (PL_LDS0 + c);
+    if (cc[8] < 0 || (long long)cc[8] > gc.big) pd->ovf |= 1 << PL_BIG;
+    for (int c = 0; c < 4; c++)
+        if (cc[c] < 0 || (long long)cc[c] > gc.cls[c]) pd->ovf |= 1 << (PL_C0 + c);
     if (pd->ovf) {
         pd->nfg = pd->nraw = pd->npix = pd->nobj = pd->nsurv = 0;
-        for (int c = 0; c < 4; c++) cc[c] = 0;
+        for (int c = 0; c < 7; c++) cc[c] = 0;
         cc[8] = 0;
     }
 }
@@ -2036,12 +2048,13 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
             cc[8] = (int)pl->cap[PL_BIG];
             dw[3] = (unsigned long long)pl->cap[PL_NPL];
             dw[4] = (unsigned long long)pl->cap[PL_NREC];
+            DebGuardCaps gc;
+            for (int c = 0; c < 3; c++) gc.lds[c] = cc[4 + c];
+            gc.big = cc[8];
+            for (int c = 0; c < 4; c++) gc.cls[c] = cc[c];
             hipLaunchKernelGGL(k_deb_plan_guard, dim3(1), dim3(64), 0, st, dwork, pd, ccount,
                                dtot, (long long)pl->cap[PL_NTOT], (long long)pl->cap[PL_NPL],
-                               (long long)pl->cap[PL_NREC]);
-            for (int c = 0; c < 3; c++) sxg_plan_guard(ccount + 4 + c, cc[4 + c], PL_LDS0 + c, ccount);
-            sxg_plan_guard(ccount + 8, cc[8], PL_BIG, ccount);
-            for (int c = 0; c < 4; c++) sxg_plan_guard(ccount + c, cc[c], PL_C0 + c, ccount);
+                               (long long)pl->cap[PL_NREC], gc);
         } else {
             SXG_HIP_RET(hipMemcpyAsync(cc, ccount, sizeof(cc), hipMemcpyDeviceToHost, st), -1);
             SXG_HIP_RET(hipMemcpyAsync(&ntot, dtot, sizeof(int), hipMemcpyDeviceToHost, st), -1);

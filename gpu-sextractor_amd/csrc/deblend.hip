@@ -1582,8 +1582,10 @@ __global__ void __launch_bounds__(256) k_pre_order(ObjArgs A, int *cnt, int *ord
 }
 
 /* objects of each raw object, in component-table order (one wavefront per raw object) */
-__global__ void __launch_bounds__(256) k_deb_objlist(ObjArgs A)
+__global__ void __launch_bounds__(256) k_deb_objlist(ObjArgs A, int *zero2)
 {
+    /* the two counters of k_pre_order start at zero (no fill launch) */
+    if (blockIdx.x == 0 && threadIdx.x < 2) zero2[threadIdx.x] = 0;
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= deb_nraw(A.d)) return;
@@ -1617,11 +1619,6 @@ __global__ void __launch_bounds__(256) k_deb_objlist(ObjArgs A)
     }
 }
 
-__global__ void __launch_bounds__(256) k_deb_allocinit(ObjArgs A)
-{
-    const long g = (long)blockIdx.x * 256 + threadIdx.x;
-    if (g < deb_P0(A.d)) A.alloc[A.d.fpia[g]] = -1;
-}
 
 /*
  * FULL preanalysis of each kept component over its level pixels (one wavefront per
@@ -1734,14 +1731,19 @@ __global__ void __launch_bounds__(256) k_deb_pre(ObjArgs A)
 
 /* gatherup: every unassigned pixel goes to the nearest kept object of its raw object
    in the ellipse metric (first minimum in object order); then the sort keys */
-__global__ void __launch_bounds__(256) k_deb_gather(ObjArgs A)
+__global__ void __launch_bounds__(256) k_deb_gather(ObjArgs A, int *objcnt, long nz)
 {
     const long g = (long)blockIdx.x * 256 + threadIdx.x;
+    /* the per-object pixel counts of k_deb_objcount start at zero (no fill launch) */
+    if (g < nz) objcnt[g] = 0;
     if (g >= deb_P0(A.d)) return;
     const int p = A.d.fpia[g];
-    int a = A.alloc[p];
-    if (a < 0) {
-        const int r = A.d.rawkey[g];
+    const int r = A.d.rawkey[g];
+    /* k_deb_pre gave every pixel of a raw object kept whole, and every pixel of a kept
+       node, its object; the rest are the pixels of split raw objects outside all kept
+       nodes */
+    int a;
+    if (A.d.rootkept[r] >= 2 && A.d.kcomp[g] < 0) {
         const int base = A.objbase[r], nk = A.d.rootkept[r];
         const int y = p / A.d.W, x = p - y * A.d.W;
         const ObjCols &o = A.o;
@@ -1758,6 +1760,8 @@ __global__ void __launch_bounds__(256) k_deb_gather(ObjArgs A)
             }
         }
         A.alloc[p] = a;
+    } else {
+        a = A.alloc[p];
     }
     A.okey[g] = a;
     A.oval[g] = p;
@@ -2237,24 +2241,22 @@ extern "C" int parcel_out(float basethresh, int n_thresh, int deb_minarea, size_
         A.oval = v0;
         const int gp = sxg_div_up(P0, 256);
         sxg_kt_begin(SXG_KT_DEB_OBJECTS);
-        hipLaunchKernelGGL(k_deb_objlist, dim3(sxg_div_up(nraw, 4)), dim3(256), 0, st, A);
-        hipLaunchKernelGGL(k_deb_allocinit, dim3(gp), dim3(256), 0, st, A);
         int *order = sxg_take<int>("pre.order", (size_t)nobj + 2);
         if (!order) return -1;
-        /* the two counters of k_pre_order (a fill kernel: no memset node in a graph) */
-        hipLaunchKernelGGL(k_deb_zero, dim3(1), dim3(256), 0, st, order + nobj, 2L);
+        hipLaunchKernelGGL(k_deb_objlist, dim3(sxg_div_up(nraw, 4)), dim3(256), 0, st, A,
+                           order + nobj);
         hipLaunchKernelGGL(k_pre_order, dim3(sxg_div_up(nobj, 256)), dim3(256), 0, st, A,
                            order + nobj, order);
         A.order = order;
         hipLaunchKernelGGL(k_deb_pre, dim3(sxg_div_up(nobj, 4)), dim3(256), 0, st, A);
-        hipLaunchKernelGGL(k_deb_gather, dim3(gp), dim3(256), 0, st, A);
+        /* planned: the counts past the device's nobj stay zero, so the scan over the
+           capacity puts the total at objoff[nobj] as well; the gather kernel clears them */
+        const long nz = pl ? (long)nobj + 1 : (long)nobj;
+        hipLaunchKernelGGL(k_deb_gather, dim3(sxg_div_up(std::max<long>(P0, nz), 256)), dim3(256),
+                           0, st, A, objcnt, nz);
         sxg_kt_end(SXG_KT_DEB_OBJECTS);
         /* raw pixels regrouped by object (stable: raster order inside each object) */
         sxg_kt_begin(SXG_KT_OBJ_SORT);
-        /* planned: the counts past the device's nobj stay zero, so the scan over the
-           capacity puts the total at objoff[nobj] as well */
-        const long nz = pl ? (long)nobj + 1 : (long)nobj;
-        hipLaunchKernelGGL(k_deb_zero, dim3(sxg_div_up(nz, 256)), dim3(256), 0, st, objcnt, nz);
         hipLaunchKernelGGL(k_deb_objcount, dim3(gp), dim3(256), 0, st, A, objcnt);
         if (pl)
             sxg_scan_excl(objcnt, objoff, nobj + 1, nullptr, st);

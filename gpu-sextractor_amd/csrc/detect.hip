@@ -129,13 +129,19 @@ __device__ inline int run_head(bool act, int p, int W)
     return le ? 63 - __builtin_clzll(le) : lane;
 }
 
+struct RootStat {
+    int *cnt, *peak, *xmin, *xmax, *ymin, *ymax, *t0;
+};
+
 /* full path compression: every detected pixel's parent becomes its root (a valid parent
    while other lanes still walk: the root is an ancestor with the smallest index, so the
    path halving's atomicMin never moves a pointer past it), and roots remember their
    foreground position.  Only the first pixel of each run in a wavefront walks; the
-   others take its root. */
+   others take its root.  The root statistics and the raw-object index of each
+   foreground position start here (no fill launches). */
 __global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg, int nfg, int W,
-                                                     int *par, int *rootpos, const int *nd)
+                                                     int *par, int *rootpos, RootStat rs,
+                                                     int *rawid_of_pos, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const bool act = k < sxg_dn(nd, nfg);
@@ -147,16 +153,6 @@ __global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg,
     if (!act) return;
     par[p] = r;
     if (r == p) rootpos[p] = k;
-}
-
-struct RootStat {
-    int *cnt, *peak, *xmin, *xmax, *ymin, *ymax, *t0;
-};
-
-__global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg, const int *nd)
-{
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= sxg_dn(nd, nfg)) return;
     rs.cnt[k] = 0;
     rs.peak[k] = (int)0x80000000;   /* below f2ord of every float */
     rs.xmin[k] = 0x7fffffff;
@@ -164,7 +160,9 @@ __global__ void __launch_bounds__(256) k_det_statinit(RootStat rs, int nfg, cons
     rs.ymin[k] = 0x7fffffff;
     rs.ymax[k] = -1;
     rs.t0[k] = 0x7fffffff;          /* above f2ord of every float */
+    rawid_of_pos[k] = -1;
 }
+
 
 /* per-root size, peak and bounding box; lanes of a wavefront that share a root (the
    pixel list is in raster order, so runs are long) combine their values first and one
@@ -254,11 +252,6 @@ __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ raw
     ro.t0[i] = perpix ? ord2f(rs.t0[k]) : t;
 }
 
-__global__ void __launch_bounds__(256) k_det_fill(int *a, int n, int v, const int *nd)
-{
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k < sxg_dn(nd, n)) a[k] = v;
-}
 
 /* object id of each detected pixel (or -1 when its component is too small) */
 __global__ void __launch_bounds__(256) k_det_pixkey(const int *__restrict__ fg, int nfg, int W,
@@ -392,9 +385,8 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         sxg_kt_end(SXG_KT_DET_UNION);
         sxg_kt_begin(SXG_KT_DET_FLATTEN);
         hipLaunchKernelGGL(k_det_flatten, dim3(gfg), dim3(256), 0, st, fg, nfg, W, par,
-                           rootpos, nfg_k);
+                           rootpos, rs, rawid_of_pos, nfg_k);
         sxg_kt_end(SXG_KT_DET_FLATTEN);
-        hipLaunchKernelGGL(k_det_statinit, dim3(gfg), dim3(256), 0, st, rs, nfg, nfg_k);
         sxg_kt_begin(SXG_KT_DET_STAT);
         hipLaunchKernelGGL(k_det_rootstat, dim3(gfg), dim3(256), 0, st, fg, nfg, W, lab, rootpos,
                            S.cd.as<float>(), rs, wa, nfg_k);
@@ -415,7 +407,6 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     RawObj ro;
     if (!sxg_raw_alloc(nraw, ro)) return;
     if (nfg > 0) {
-        hipLaunchKernelGGL(k_det_fill, dim3(gfg), dim3(256), 0, st, rawid_of_pos, nfg, -1, nfg_k);
         if (nraw > 0)
             hipLaunchKernelGGL(k_det_rawinfo, dim3(sxg_div_up(nraw, 256)), dim3(256), 0, st, rawpos,
                                nraw, fg, rs, rawid_of_pos, ro, dthreshold, (int)(wa.w && wa.rel),

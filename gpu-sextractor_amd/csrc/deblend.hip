@@ -75,6 +75,9 @@
 /* pixels per lane whose global-memory accesses the tree passes keep in flight together */
 #define DEB_ILP 4
 #define DEB_RUN 8
+/* list chunks of NT entries whose roots an LDS tree's thread remembers per level (bits of
+   a word); larger lists take the contiguous-run numbering */
+#define DEB_NCH 32
 /* words of the per-frame counter block (DebArgs::ccount) */
 #define DEB_NCOUNT 14
 
@@ -629,8 +632,10 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
     __shared__ float tthr[SXG_MAXNTHRESH], ttau[SXG_MAXNTHRESH];
     __shared__ int s_multi, s_nk, s_top;
     __shared__ double s_rsum[NT / 64], s_rabs[NT / 64], s_v0lo, s_v0hi;
+    __shared__ int s_ctab[G ? 1 : DEB_NCH * (NT / 64)], s_nkl;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = a.cnt[r], off = a.off[r], ls = a.lstar[r];
+    const int nch = (n + NT - 1) / NT;
     const int W = a.W;
     Comp *const C = a.comps + a.compoff[r];
     if (wid == 0) {
@@ -871,6 +876,7 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
             }
         }
         __syncthreads();
+        unsigned rbits = 0;
         /* roots (the smallest list index = first raster pixel) and sizes; a concurrent
            path-halving step can only lower par[k] towards the same root */
         if constexpr (G) {
@@ -911,18 +917,78 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
                 }
             }
         } else {
-            for (int k = tid; k < n; k += NT)
+            /* bit j: the thread's pixel j * NT + tid is a root of V_i */
+            for (int j = 0, k = tid; k < n; j++, k += NT)
                 if (m[k] >= i) {
                     const int rt = uf_find<G>(par, k);
                     par[k] = rt;
                     atomicAdd(&cnt[rt], 1);
+                    if (rt == k && j < DEB_NCH) rbits |= 1u << j;
                 }
         }
         __syncthreads();
+        int nkept = 0;
+        if (!G && nch <= DEB_NCH) {
+            /* the nodes of level i in raster order of their first pixel, from the roots
+               the thread found above (no pass over the other pixels): per chunk of NT
+               list entries and wavefront, the kept roots are counted by a ballot, one
+               wavefront turns the table into an exclusive prefix in (chunk, wavefront)
+               order, and each kept root's number is its entry plus the kept roots of
+               the lower lanes */
+            unsigned fbits = 0;
+            for (int j = 0; j < nch; j++)
+                if ((rbits >> j) & 1u)
+                    if (cnt[j * NT + tid] >= a.debarea) fbits |= 1u << j;
+            for (int j = 0; j < nch; j++) {
+                const unsigned long long b = __ballot((fbits >> j) & 1u);
+                if (lane == 0) s_ctab[j * (NT / 64) + wid] = __popcll(b);
+            }
+            __syncthreads();
+            if (wid == 0) {
+                const int ne = nch * (NT / 64);
+                int v[8], sl = 0;
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int e = lane * 8 + u;
+                    v[u] = e < ne ? s_ctab[e] : 0;
+                    sl += v[u];
+                }
+                int x = sl;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int o = __shfl_up(x, d);
+                    if (lane >= d) x += o;
+                }
+                int run = x - sl;
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int e = lane * 8 + u;
+                    if (e < ne) s_ctab[e] = run;
+                    run += v[u];
+                }
+                if (lane == 63) s_nkl = x;
+            }
+            __syncthreads();
+            nkept = s_nkl;
+            const unsigned long long below = sxw_lanes_below(lane);
+            for (int j = 0; j < nch; j++) {
+                const bool f = (fbits >> j) & 1u;
+                const unsigned long long b = __ballot(f);
+                if ((rbits >> j) & 1u) {
+                    const int k = j * NT + tid;
+                    int jk = -1;
+                    if (f) {
+                        jk = s_ctab[j * (NT / 64) + wid] + __popcll(b & below);
+                        DebNode e;
+                        e.level = i; e.parent = -1; e.count = cnt[k]; e.nchild = 0; e.nson = 0;
+                        e.flags = 0; e.kroot = k; e.kanc = -1; e.ssum = 0.0; e.sabs = 0.0;
+                        T[nc + jk] = e;
+                    }
+                    cnt[k] = jk;
+                }
+            }
+        } else {
         /* the nodes of level i in raster order of their first pixel: each thread numbers
            a contiguous run of the list after one prefix over the threads' counts */
-        int nkept = 0;
-        {
             const int per = (n + NT - 1) / NT;
             const int kb = tid * per, ke = min(kb + per, n);
             int mine = 0;
@@ -1200,7 +1266,7 @@ __device__ __forceinline__ void deb_tree(const DebArgs &a, const int r, float *c
 /* CLS names the size class in the kernel's symbol only (classes 1 and 2 share the code
    and the workgroup size; a kernel trace then times each class on its own) */
 template <int NT, int CLS>
-__global__ void __launch_bounds__(NT) k_deb_tree_lds(DebArgs a, int cls)
+__global__ void __launch_bounds__(NT, NT == 256 ? 7 : 1) k_deb_tree_lds(DebArgs a, int cls)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     /* a planned run's grid is the class's capacity */

@@ -263,11 +263,12 @@ static bool survivor_ranks(const ObjCols &o, int *&keep, int *&rank, int *&tot)
     return true;
 }
 
-/* 0: segmentation map, 1: object index map; into the device buffer "out.map" */
-static int *build_map(int mode)
+/* 0: segmentation map, 1: object index map; into dst (N ints) or, without one, the
+   device buffer "out.map" */
+static int *build_map(int mode, int *dst = nullptr)
 {
     SxgState &S = sxg_state();
-    int *map = sxg_take<int>("out.map", S.N);
+    int *map = dst ? dst : sxg_take<int>("out.map", S.N);
     if (!map) return nullptr;
     const long N = (long)S.N;
     hipLaunchKernelGGL(k_fill_i32, dim3(sxg_div_up(sxg_div_up(N, 4), 256)), dim3(256), 0,
@@ -304,7 +305,7 @@ static int *build_map(int mode)
     return map;
 }
 
-bool sxg_build_segmap() { return build_map(0) != nullptr; }
+bool sxg_build_segmap(int *dst) { return build_map(0, dst) != nullptr; }
 
 extern "C" int sxg_get_segmap(int32_t *out)
 {

@@ -177,7 +177,7 @@ struct PlanScope {
    copied to *rec (device) at the end; info is then not written. */
 static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
                          sxg_frame_info *info, const FramePlan *plan = nullptr,
-                         PlanRecord *rec = nullptr)
+                         PlanRecord *rec = nullptr, int *segdst = nullptr)
 {
     SxgState &S = sxg_state();
     int ndev = 0;
@@ -274,7 +274,7 @@ static int process_frame(const float *dframe, int W, int H, const sxg_params *p,
     if (sxg_has_error()) return -1;
     run_clean(nullptr, p->clean_param, p->clean_stacksize, nobj);
     if (sxg_has_error()) return -1;
-    if (!sxg_build_segmap()) return -1;
+    if (!sxg_build_segmap(segdst)) return -1;
     if (plan) {
         if (rec) hipLaunchKernelGGL(k_plan_end, dim3(1), dim3(64), 0, S.stream, S.pdev, rec);
         SXG_HIP_RET(hipGetLastError(), -1);
@@ -428,13 +428,11 @@ extern "C" int sxg_batch_process(const sxg_params *p, int nworkers, sxg_frame_in
     auto run = [&](int i) {
         BatchFrame &f = g_batch[i];
         sxg_frame_info fi;
-        if (process_frame(f.frame.as<float>(), f.W, f.H, p, &fi) < 0) return false;
-        SxgState &S = sxg_state();
-        int *map = sxg_take<int>("out.map", S.N);
-        if (!map || hipMemcpyAsync(f.seg.ptr, map, S.N * sizeof(int), hipMemcpyDeviceToDevice,
-                                   S.stream) != hipSuccess ||
-            hipStreamSynchronize(S.stream) != hipSuccess) {
-            sxg_set_error("sxg_batch_process: segmentation copy failed");
+        if (process_frame(f.frame.as<float>(), f.W, f.H, p, &fi, nullptr, nullptr,
+                          f.seg.as<int>()) < 0)
+            return false;
+        if (hipStreamSynchronize(sxg_state().stream) != hipSuccess) {
+            sxg_set_error("sxg_batch_process: the frame's stream failed");
             return false;
         }
         if (info) info[i] = fi;
@@ -599,20 +597,6 @@ static bool same_params(const sxg_params &a, const sxg_params *b, const std::vec
     return nm == 0 || memcmp(mask.data(), b->mask, nm * sizeof(float)) == 0;
 }
 
-/* the seg map of frame i from the context's "out.map", on the context's stream */
-static bool copy_seg(int i)
-{
-    SxgState &S = sxg_state();
-    BatchFrame &f = g_batch[i];
-    int *map = sxg_take<int>("out.map", S.N);
-    if (!map || hipMemcpyAsync(f.seg.ptr, map, S.N * sizeof(int), hipMemcpyDeviceToDevice,
-                               S.stream) != hipSuccess) {
-        sxg_set_error("graph batch: segmentation copy failed");
-        return false;
-    }
-    return true;
-}
-
 static int graph_build(const sxg_params *p, int nworkers, int margin)
 {
     GraphBatch &G = g_gb;
@@ -671,8 +655,8 @@ static int graph_build(const sxg_params *p, int nworkers, int margin)
         for (int i = 0; i < n; i++) {
             if (G.branch[i] != w) continue;
             BatchFrame &f = g_batch[i];
-            if (process_frame(f.frame.as<float>(), f.W, f.H, p, nullptr, &G.plans[i], drec + i) < 0 ||
-                !copy_seg(i))
+            if (process_frame(f.frame.as<float>(), f.W, f.H, p, nullptr, &G.plans[i], drec + i,
+                              f.seg.as<int>()) < 0)
                 return false;
         }
         return true;
@@ -783,7 +767,8 @@ extern "C" int sxg_batch_process_graph(const sxg_params *p, int nworkers, sxg_fr
         sxg_ctx_bind(g_workers[G.branch[i]]);
         const auto t0 = std::chrono::steady_clock::now();
         sxg_frame_info fi;
-        const bool ok = process_frame(f.frame.as<float>(), f.W, f.H, p, &fi) >= 0 && copy_seg(i) &&
+        const bool ok = process_frame(f.frame.as<float>(), f.W, f.H, p, &fi, nullptr, nullptr,
+                                      f.seg.as<int>()) >= 0 &&
                         hipStreamSynchronize(sxg_state().stream) == hipSuccess;
         sxg_bind_state(prev);
         if (!ok) {

@@ -299,7 +299,8 @@ void sxg_kt_begin(int id);
 void sxg_kt_end(int id);
 void sxg_kt_begin_on(int id, hipStream_t st);   /* the same, on a given stream */
 void sxg_kt_end_on(int id, hipStream_t st);
-bool sxg_build_segmap();   /* device segmentation map ("out.map") and survivor count */
+bool sxg_build_segmap(int *dst = nullptr); /* device segmentation map (into dst, else the
+                                              buffer "out.map") and survivor count */
 /* queue the catalogue rows of the survivors and their copy to host memory (after the
    segmentation map); the number of rows copied, or -1 */
 int sxg_catalog_async(sxg_isorow *out, int cap);

@@ -94,6 +94,11 @@ __device__ inline void meshstat_pass2_end(const MeshAcc &a, MeshStat &o)
  */
 #define MS_MPB 16
 #define MS_THREADS 256
+/* loads of a loader thread in flight together */
+#define MS_U 4
+/* dynamic LDS of a workgroup, the ring: more rows per batch (a larger ring) measured no
+   faster, and on a CU it leaves less room for other frames' kernels */
+#define MS_LDS_BYTES (128 * 1024)
 
 /* meshes whose rows do not fit the LDS ring: one lane per mesh reading global memory */
 __global__ void __launch_bounds__(64) k_meshstat_direct(const float *__restrict__ img,
@@ -130,9 +135,12 @@ __global__ void __launch_bounds__(64) k_meshstat_direct(const float *__restrict_
     }
     ms[mi] = o;
 }
-/* padded LDS segment of one mesh row in doubles: odd, so the 16 chain lanes of an
-   array start on distinct bank pairs */
-__host__ __device__ inline int ms_seg(int bs) { return bs | 1; }
+/* LDS segment of one mesh row in doubles: column c of the mesh sits at
+   (c & 3) * ms_q(bs) + (c >> 2), so the loader lanes, which take four adjacent columns
+   each, store to consecutive doubles; the segment length is odd, so the 16 chain lanes
+   of an array, which read the same column of 16 meshes, start on distinct bank pairs */
+__host__ __device__ inline int ms_q(int bs) { return (bs + 3) >> 2; }
+__host__ __device__ inline int ms_seg(int bs) { return (4 * ms_q(bs)) | 1; }
 
 __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict__ img,
                                                          const float *__restrict__ wmap, int W,
@@ -148,7 +156,7 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
     const int x0 = mx0 * bs, y0 = my * bs;
     const int bandw = min(W, x0 + nm * bs) - x0;
     const int bh = min(bs, H - y0);
-    const int seg = ms_seg(bs);
+    const int seg = ms_seg(bs), q = ms_q(bs);
     const int bufsz = R * mpb * seg;            /* doubles per buffer and array */
     const int nbatch = (bh + R - 1) / R;
     /* mesh of a band column: a shift when the mesh size is a power of two */
@@ -176,10 +184,10 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
             /* all the batch's loads of a thread in flight before any store */
             const int q4 = bandw >> 2, tot = nr * q4;
             const int qsh = (q4 & (q4 - 1)) == 0 ? __builtin_ctz(q4) : -1;
-            for (int e0 = lt; e0 < tot; e0 += 4 * LT) {
-                float4 v[4];
+            for (int e0 = lt; e0 < tot; e0 += MS_U * LT) {
+                float4 v[MS_U];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < MS_U; u++) {
                     const int e = e0 + u * LT;
                     if (e < tot) {
                         const int r = qsh >= 0 ? e >> qsh : e / q4, c = 4 * (e - r * q4);
@@ -197,7 +205,7 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < MS_U; u++) {
                     const int e = e0 + u * LT;
                     if (e < tot) {
                         const int r = qsh >= 0 ? e >> qsh : e / q4, c = 4 * (e - r * q4);
@@ -205,16 +213,17 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
                         const double p0 = v[u].x, p1 = v[u].y, p2 = v[u].z, p3 = v[u].w;
                         const bool i0 = incl(p0, mm), i1 = incl(p1, mm), i2 = incl(p2, mm),
                                    i3 = incl(p3, mm);
-                        const int idx = (r * mpb + mm) * seg + off;
+                        /* off is a multiple of 4: its four columns at off/4 + j*q */
+                        const int idx = (r * mpb + mm) * seg + (off >> 2);
                         double *d = bt + idx, *d2 = bt2 + idx;
                         d[0] = i0 ? p0 : 0.0;
-                        d[1] = i1 ? p1 : 0.0;
-                        d[2] = i2 ? p2 : 0.0;
-                        d[3] = i3 ? p3 : 0.0;
+                        d[q] = i1 ? p1 : 0.0;
+                        d[2 * q] = i2 ? p2 : 0.0;
+                        d[3 * q] = i3 ? p3 : 0.0;
                         d2[0] = i0 ? p0 * p0 : -0.0;
-                        d2[1] = i1 ? p1 * p1 : -0.0;
-                        d2[2] = i2 ? p2 * p2 : -0.0;
-                        d2[3] = i3 ? p3 * p3 : -0.0;
+                        d2[q] = i1 ? p1 * p1 : -0.0;
+                        d2[2 * q] = i2 ? p2 * p2 : -0.0;
+                        d2[3 * q] = i3 ? p3 * p3 : -0.0;
                         const int k = (int)i0 + (int)i1 + (int)i2 + (int)i3;
                         if (k) atomicAdd(&s_cnt[mm], k);
                     }
@@ -222,10 +231,10 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
             }
         } else {
             const int tot = nr * bandw;
-            for (int e0 = lt; e0 < tot; e0 += 4 * LT) {
-                float v[4];
+            for (int e0 = lt; e0 < tot; e0 += MS_U * LT) {
+                float v[MS_U];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < MS_U; u++) {
                     const int e = e0 + u * LT;
                     if (e < tot) {
                         const int r = e / bandw, c = e - r * bandw;
@@ -234,14 +243,14 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < MS_U; u++) {
                     const int e = e0 + u * LT;
                     if (e < tot) {
                         const int r = e / bandw, c = e - r * bandw;
                         const int mm = bsh >= 0 ? c >> bsh : c / bs, off = c - mm * bs;
                         const double pix = v[u];
                         const bool in = incl(pix, mm);
-                        const int idx = (r * mpb + mm) * seg + off;
+                        const int idx = (r * mpb + mm) * seg + (off & 3) * q + (off >> 2);
                         bt[idx] = in ? pix : 0.0;
                         bt2[idx] = in ? pix * pix : -0.0;
                         if (in) atomicAdd(&s_cnt[mm], 1);
@@ -260,32 +269,45 @@ __global__ void __launch_bounds__(MS_THREADS) k_meshstat(const float *__restrict
             if (wid != 0) {
                 if (b + 1 < nbatch) load(pass, b + 1, (b + 1) & 1);
             } else if (chain) {
-                /* the lane's chain: one double add per pixel, the reads of the next
-                   eight in flight meanwhile */
+                /* the lane's chain: one double add per pixel in column order (column c
+                   of a row at (c & 3) * q + (c >> 2)) */
                 const double *src = ring + (size_t)(2 * (b & 1) + arr) * bufsz + m * seg;
                 const int nr = min(R, bh - b * R), rstride = mpb * seg;
-                for (int r = 0; r < nr; r++) {
-                    const double *row = src + r * rstride;
-                    int x = 0;
-                    if (bw >= 8) {
-                        /* groups of eight, the next group read while this one is added;
-                           the last full group ends the row (bw = 64: no single-value
-                           tail, whose reads would each wait for the LDS) */
-                        double v[8], w[8];
+                if ((bw & 31) == 0 && ((nr * (bw >> 5)) & 1) == 0) {
+                    /* an even number of chunks of 32 columns, in two register buffers
+                       used in turn: the reads of the next chunk are in flight while this
+                       one is added.  Every load is unconditional (the pointer wraps to
+                       the batch's first chunk after its last), so no control-flow merge
+                       makes the compiler wait for both buffers at once; no copy between
+                       the buffers either, which would wait for the newer reads */
+                    const int cpr = bw >> 5, K = nr * cpr;
+                    const double *cp = src;
+                    int cc = 0, rr = 0;
+                    auto ld = [&](double (&d)[32]) {
 #pragma unroll
-                        for (int u = 0; u < 8; u++) v[u] = row[u];
-                        for (x = 8; x + 8 <= bw; x += 8) {
+                        for (int u = 0; u < 32; u++) d[u] = cp[(u & 3) * q + (u >> 2)];
+                        const bool eor = ++cc == cpr;
+                        cc = eor ? 0 : cc;
+                        rr += eor ? 1 : 0;
+                        const bool eob = rr == nr;
+                        rr = eob ? 0 : rr;
+                        cp = eob ? src : (eor ? cp + (rstride - (cpr - 1) * 8) : cp + 8);
+                    };
+                    double A[32], B[32];
+                    ld(A);
+                    for (int k = 0; k < K; k += 2) {
+                        ld(B);
 #pragma unroll
-                            for (int u = 0; u < 8; u++) w[u] = row[x + u];
+                        for (int u = 0; u < 32; u++) acc += A[u];
+                        ld(A);
 #pragma unroll
-                            for (int u = 0; u < 8; u++) acc += v[u];
-#pragma unroll
-                            for (int u = 0; u < 8; u++) v[u] = w[u];
-                        }
-#pragma unroll
-                        for (int u = 0; u < 8; u++) acc += v[u];
+                        for (int u = 0; u < 32; u++) acc += B[u];
                     }
-                    for (; x < bw; x++) acc += row[x];
+                } else {
+                    for (int r = 0; r < nr; r++) {
+                        const double *row = src + r * rstride;
+                        for (int x = 0; x < bw; x++) acc += row[(x & 3) * q + (x >> 2)];
+                    }
                 }
             }
             __syncthreads();
@@ -1108,16 +1130,16 @@ bool sxg_back_run(const BackGeom &g, const BackBufs &b, const float *src, float 
     if (timed) sxg_kt_begin(SXG_KT_MESHSTAT);
     {
         /* meshes per workgroup and rows per batch: two buffers of R rows of mpb padded
-           segments of t and t2 (doubles) within 128 KB of LDS */
+           segments of t and t2 (doubles), each within 64 KB */
         const size_t segbytes = (size_t)ms_seg(bs) * 2 * sizeof(double);
         const int mpb = (int)std::min<size_t>(MS_MPB, (64 * 1024) / segbytes);
         if (mpb >= 1) {
             /* the LDS limit is an attribute of the function: raised once, to the most
-               any launch asks for (128 KB) */
+               any launch asks for */
             static const bool attr = [] {
                 return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_meshstat),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           128 * 1024) == hipSuccess;
+                                           MS_LDS_BYTES) == hipSuccess;
             }();
             if (!attr) {
                 sxg_set_error("back_estimate: cannot raise the LDS limit of k_meshstat");

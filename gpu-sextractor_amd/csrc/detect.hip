@@ -130,7 +130,7 @@ __device__ inline int run_head(bool act, int p, int W)
 }
 
 struct RootStat {
-    int *cnt, *peak, *xmin, *xmax, *ymin, *ymax, *t0;
+    int *cnt, *peak, *xmin, *xmax, *ymax, *t0;
 };
 
 /* full path compression: every detected pixel's parent becomes its root (a valid parent
@@ -157,16 +157,18 @@ __global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg,
     rs.peak[k] = (int)0x80000000;   /* below f2ord of every float */
     rs.xmin[k] = 0x7fffffff;
     rs.xmax[k] = -1;
-    rs.ymin[k] = 0x7fffffff;
     rs.ymax[k] = -1;
     rs.t0[k] = 0x7fffffff;          /* above f2ord of every float */
     rawid_of_pos[k] = -1;
 }
 
 
-/* per-root size, peak and bounding box; lanes of a wavefront that share a root (the
-   pixel list is in raster order, so runs are long) combine their values first and one
-   lane per root issues the atomics */
+/* per-root size, peak and bounding box, in one pass: the pixel list is in raster order,
+   so a run (consecutive positions of one row, all of one root) is a contiguous range of
+   lanes; a segmented scan over the run gives its last lane the run's peak (and lowest
+   pixel threshold), and that lane issues the run's atomics (integer add, min and max:
+   their order does not matter).  The root is the component's smallest position, so the
+   smallest row is the root's and needs no atomic (k_det_rawinfo). */
 __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg, int nfg, int W,
                                                       const int *__restrict__ lab,
                                                       const int *__restrict__ rootpos,
@@ -176,48 +178,39 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool perpix = wa.w && wa.rel;
-    bool act = k < sxg_dn(nd, nfg);
-    int rk = -1, x = 0, y = 0, pv = 0, tv = 0x7fffffff;
+    const int n = sxg_dn(nd, nfg);
+    const bool act = k < n;
     const int pp = act ? fg[k] : -1;
     /* the root's position, looked up by the first pixel of each run */
     const int h = run_head(act, pp, W);
+    int rk = -1;
     if (act && h == lane) rk = rootpos[lab[pp]];
     rk = __shfl(rk, h);
-    if (act) {
-        const int p = pp;
-        y = p / W;
-        x = p - y * W;
-        pv = f2ord(cd[p]);
-        if (perpix) tv = f2ord(det_thresh(wa, 0.0f, p));
+    int pv = act ? f2ord(cd[pp]) : (int)0x80000000;
+    int tv = (act && perpix) ? f2ord(det_thresh(wa, 0.0f, pp)) : 0x7fffffff;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(pv, d);
+        if (lane - d >= h) pv = max(pv, o);
     }
-    unsigned long long pending = __ballot(act);
-    while (pending) {
-        const int leader = __builtin_ctzll(pending);
-        const int r = __shfl(rk, leader);
-        const bool mine = act && rk == r;
-        const unsigned long long grp = __ballot(mine);
-        const int pk = sxw_max(mine ? pv : (int)0x80000000);
-        const int x0 = sxw_min(mine ? x : 0x7fffffff), x1 = sxw_max(mine ? x : -1);
-        /* the list is in raster order: y grows with the lane, so the group's first and
-           last lanes hold its extreme rows */
-        const int y0 = __builtin_amdgcn_readlane(y, leader);
-        const int y1 = __builtin_amdgcn_readlane(y, 63 - __builtin_clzll(grp));
-        if (lane == leader) {
-            atomicAdd(&rs.cnt[r], __popcll(grp));
-            atomicMax(&rs.peak[r], pk);
-            atomicMin(&rs.xmin[r], x0);
-            atomicMax(&rs.xmax[r], x1);
-            atomicMin(&rs.ymin[r], y0);
-            atomicMax(&rs.ymax[r], y1);
+    if (perpix) {
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(tv, d);
+            if (lane - d >= h) tv = min(tv, o);
         }
-        if (perpix) {
-            /* the lowest pixel threshold of the object */
-            const int t0 = sxw_min(mine ? tv : 0x7fffffff);
-            if (lane == leader) atomicMin(&rs.t0[r], t0);
-        }
-        if (mine) act = false;
-        pending &= ~grp;
     }
+    /* the run's last lane: the next lane starts another run or is past the list */
+    const int hn = __shfl_down(h, 1);
+    if (!act || (lane < 63 && k + 1 < n && hn == h)) return;
+    const int y = pp / W, x1 = pp - y * W, x0 = x1 - (lane - h);
+    atomicAdd(&rs.cnt[rk], lane - h + 1);
+    atomicMax(&rs.peak[rk], pv);
+    atomicMin(&rs.xmin[rk], x0);
+    atomicMax(&rs.xmax[rk], x1);
+    atomicMax(&rs.ymax[rk], y);
+    /* the lowest pixel threshold of the object */
+    if (perpix) atomicMin(&rs.t0[rk], tv);
 }
 
 /* keep[k] = k for kept roots (size >= minarea), -1 otherwise */
@@ -233,7 +226,8 @@ __global__ void __launch_bounds__(256) k_det_rootflag(const int *__restrict__ fg
 }
 
 __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ rawpos, int nraw,
-                                                     const int *__restrict__ fg, RootStat rs,
+                                                     const int *__restrict__ fg, int W,
+                                                     RootStat rs,
                                                      int *rawid_of_pos, RawObj ro, float t,
                                                      int perpix, const int *nd, const float *td)
 {
@@ -247,7 +241,7 @@ __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ raw
     ro.fdpeak[i] = ord2f(rs.peak[k]);
     ro.xmin[i] = rs.xmin[k];
     ro.xmax[i] = rs.xmax[k];
-    ro.ymin[i] = rs.ymin[k];
+    ro.ymin[i] = fg[k] / W;   /* the root: the component's first pixel in raster order */
     ro.ymax[i] = rs.ymax[k];
     ro.t0[i] = perpix ? ord2f(rs.t0[k]) : t;
 }
@@ -370,13 +364,12 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     rs.peak = sxg_take<int>("det.peak", nfg);
     rs.xmin = sxg_take<int>("det.xmin", nfg);
     rs.xmax = sxg_take<int>("det.xmax", nfg);
-    rs.ymin = sxg_take<int>("det.ymin", nfg);
     rs.ymax = sxg_take<int>("det.ymax", nfg);
     rs.t0 = sxg_take<int>("det.t0", nfg);
     int *keep = sxg_take<int>("det.keep", nfg);
     int *rawpos = sxg_take<int>("det.rawpos", nfg);
     int *rawid_of_pos = sxg_take<int>("det.rawid", nfg);
-    if (!rs.cnt || !rs.peak || !rs.xmin || !rs.xmax || !rs.ymin || !rs.ymax || !rs.t0 || !keep ||
+    if (!rs.cnt || !rs.peak || !rs.xmin || !rs.xmax || !rs.ymax || !rs.t0 || !keep ||
         !rawpos || !rawid_of_pos)
         return;
     if (nfg > 0) {
@@ -409,7 +402,8 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     if (nfg > 0) {
         if (nraw > 0)
             hipLaunchKernelGGL(k_det_rawinfo, dim3(sxg_div_up(nraw, 256)), dim3(256), 0, st, rawpos,
-                               nraw, fg, rs, rawid_of_pos, ro, dthreshold, (int)(wa.w && wa.rel),
+                               nraw, fg, W, rs, rawid_of_pos, ro, dthreshold,
+                               (int)(wa.w && wa.rel),
                                nraw_k, td);
         /* pixel keys, kept pixels, stable sort by object */
         hipLaunchKernelGGL(k_det_pixkey, dim3(gfg), dim3(256), 0, st, fg, nfg, W, lab,

@@ -391,17 +391,24 @@ template <int PASS> __global__ void __launch_bounds__(256) k_deb_lmax(DebArgs a)
         const int root = a.lab[p];
         const int W = a.W, H = a.H;
         const int y = p / W, x = p - y * W;
-        bool top = true;
-        for (int dy = -1; dy <= 1; dy++) {
-            const int yy = y + dy;
-            if (yy < 0 || yy >= H) continue;
-            for (int dx = -1; dx <= 1; dx++) {
-                const int xx = x + dx;
-                if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
-                const int q = yy * W + xx;
-                if (a.lab[q] == root && a.cd[q] > v) top = false;
-            }
+        /* the eight neighbours' labels and values, loaded unconditionally from positions
+           clamped into the frame (all sixteen loads in flight together), then compared;
+           a position outside the frame takes no part */
+        int lq[8], inb[8];
+        float cq[8];
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            const int dd = n < 4 ? n : n + 1;   /* the 3x3 neighbourhood without its centre */
+            const int yy = y + dd / 3 - 1, xx = x + dd % 3 - 1;
+            inb[n] = (yy >= 0) & (yy < H) & (xx >= 0) & (xx < W);
+            const int q = inb[n] ? yy * W + xx : p;
+            lq[n] = a.lab[q];
+            cq[n] = a.cd[q];
         }
+        int higher = 0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) higher |= inb[n] & (lq[n] == root) & (cq[n] > v);
+        const bool top = higher == 0;
         a.lmflag[g] = top ? 1 : 0;
         if (!top) return;
     }

@@ -101,17 +101,24 @@ __global__ void __launch_bounds__(256) k_det_union(const int *__restrict__ fg, i
     if (k >= sxg_dn(nd, nfg)) return;
     const int p = fg[k];
     const int y = p / W, x = p - y * W;
-    const bool w = x > 0 && par[p - 1] >= 0;
+    /* the four neighbours' flags read together, unconditionally from positions clamped
+       into the frame (a detected pixel's parent stays >= 0 while other lanes link
+       trees, so reading them first is safe) */
+    const bool hw = x > 0, hn = y > 0, he = x < W - 1;
+    const int fw = par[hw ? p - 1 : p];
+    const int fn = par[hn ? p - W : p];
+    const int fnw = par[hn && hw ? p - W - 1 : p];
+    const int fne = par[hn && he ? p - W + 1 : p];
+    const bool w = hw && fw >= 0;
+    const bool nn = hn && fn >= 0;
+    const bool nw = hn && hw && fnw >= 0;
+    const bool ne = hn && he && fne >= 0;
     if (w && (p & 63) == 0) uf_unite(par, p, p - 1);
-    if (y > 0) {
-        const bool nn = par[p - W] >= 0;
-        const bool nw = x > 0 && par[p - W - 1] >= 0;
-        if (nn) {
-            if (!(w && nw)) uf_unite(par, p, p - W);
-        } else {
-            if (nw && !w) uf_unite(par, p, p - W - 1);
-            if (x < W - 1 && par[p - W + 1] >= 0) uf_unite(par, p, p - W + 1);
-        }
+    if (nn) {
+        if (!(w && nw)) uf_unite(par, p, p - W);
+    } else {
+        if (nw && !w) uf_unite(par, p, p - W - 1);
+        if (ne) uf_unite(par, p, p - W + 1);
     }
 }
 

@@ -36,8 +36,8 @@ __device__ inline float det_thresh(const WeightArgs &wa, float t, long p)
    plane is the label map: the root of each detected pixel, -1 elsewhere (one plane
    written per pixel, not two) */
 __global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, float t, long n,
-                                                  int W, int *par, WeightArgs wa,
-                                                  const float *td)
+                                                  int W, int *par, unsigned long long *fgm,
+                                                  WeightArgs wa, const float *td)
 {
     if (td) t = *td;
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
@@ -49,6 +49,8 @@ __global__ void __launch_bounds__(256) k_det_init(const float *__restrict__ cd, 
     const bool start = fg && (x == 0 || lane == 0 || !((F >> (lane - 1)) & 1ull));
     const unsigned long long S = __ballot(start);
     if (!live) return;
+    /* the window's detection mask (workgroups start on multiples of 64 positions) */
+    if (lane == 0) fgm[p >> 6] = F;
     if (!fg) {
         par[p] = -1;
         return;
@@ -342,11 +344,14 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     const int *npix_k = pd ? npix_d : nullptr;
     WeightArgs wa;
     if (!sxg_weight_args(wa)) return;
+    /* the detection masks of the 64-position windows, written with the parents */
+    unsigned long long *fgm = sxg_take<unsigned long long>("det.fgmask", sxg_div_up(N, 64));
+    if (!fgm) return;
     /* the parents are written already when the frame's convolution was fused with them */
     if (!S.det_init_done) {
         sxg_kt_begin(SXG_KT_DET_INIT);
         hipLaunchKernelGGL(k_det_init, dim3(sxg_div_up(N, 256)), dim3(256), 0, st,
-                           S.cd.as<float>(), dthreshold, N, W, par, wa, td);
+                           S.cd.as<float>(), dthreshold, N, W, par, fgm, wa, td);
         sxg_kt_end(SXG_KT_DET_INIT);
     }
     S.det_init_done = false;
@@ -354,7 +359,7 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     int *fg = sxg_take<int>("det.fg", S.N);
     if (!fg) return;
     sxg_kt_begin(SXG_KT_DET_FG);
-    sxg_compact_nonneg(par, fg, (int)N, nfg_d, st);
+    sxg_compact_mask(fgm, sxg_div_up(N, 64), fg, nfg_d, st);
     sxg_kt_end(SXG_KT_DET_FG);
     int nfg;
     if (pd) {

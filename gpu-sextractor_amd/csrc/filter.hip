@@ -332,7 +332,7 @@ template <int K>
 __global__ void __launch_bounds__(CV_THREADS) k_subconv_det(const float *__restrict__ src,
                                                             float *__restrict__ sub,
                                                             float *__restrict__ out, int *par,
-                                                            int W, int H,
+                                                            unsigned long long *fgm, int W, int H,
                                                             const float *__restrict__ mask,
                                                             SplineArgs sp, float t, WeightArgs wa,
                                                             const float *td)
@@ -380,6 +380,9 @@ __global__ void __launch_bounds__(CV_THREADS) k_subconv_det(const float *__restr
         const bool start = fg && (lane == 0 || !((F >> (lane - 1)) & 1ull));
         const unsigned long long S = __ballot(start);
         if (live) {
+            /* the window's detection mask (W is a multiple of 64: a tile row of a
+               wavefront is one window) */
+            if (lane == 0) fgm[p >> 6] = F;
             /* the subtracted pixel, as staged */
             sub[p] = tile[HW + r0 + r][HW + tx];
             out[p] = acc[r];
@@ -404,16 +407,19 @@ bool sxg_subconv_detect(const float *h_mask, int K, float dthresh)
     const dim3 grid(sxg_div_up(S.W, CV_TX), sxg_div_up(S.H, CV_TY)), block(CV_THREADS);
     float *sub = S.img.as<float>(), *cd = S.cd.as<float>();
     int *par = S.aux.as<int>();
+    unsigned long long *fgm =
+        sxg_take<unsigned long long>("det.fgmask", sxg_div_up((long)S.N, 64));
+    if (!fgm) return false;
     const float *src = S.sub_src;
     S.sub_src = nullptr;
     sxg_stage_begin(ST_FILTER);
     sxg_kt_begin(SXG_KT_CONV);
     switch (K) {
-    case 3: hipLaunchKernelGGL(k_subconv_det<3>, grid, block, 0, S.stream, src, sub, cd, par, S.W, S.H, dmask, sp, dthresh, wa, td); break;
-    case 5: hipLaunchKernelGGL(k_subconv_det<5>, grid, block, 0, S.stream, src, sub, cd, par, S.W, S.H, dmask, sp, dthresh, wa, td); break;
-    case 7: hipLaunchKernelGGL(k_subconv_det<7>, grid, block, 0, S.stream, src, sub, cd, par, S.W, S.H, dmask, sp, dthresh, wa, td); break;
-    case 9: hipLaunchKernelGGL(k_subconv_det<9>, grid, block, 0, S.stream, src, sub, cd, par, S.W, S.H, dmask, sp, dthresh, wa, td); break;
-    default: hipLaunchKernelGGL(k_subconv_det<11>, grid, block, 0, S.stream, src, sub, cd, par, S.W, S.H, dmask, sp, dthresh, wa, td); break;
+    case 3: hipLaunchKernelGGL(k_subconv_det<3>, grid, block, 0, S.stream, src, sub, cd, par, fgm, S.W, S.H, dmask, sp, dthresh, wa, td); break;
+    case 5: hipLaunchKernelGGL(k_subconv_det<5>, grid, block, 0, S.stream, src, sub, cd, par, fgm, S.W, S.H, dmask, sp, dthresh, wa, td); break;
+    case 7: hipLaunchKernelGGL(k_subconv_det<7>, grid, block, 0, S.stream, src, sub, cd, par, fgm, S.W, S.H, dmask, sp, dthresh, wa, td); break;
+    case 9: hipLaunchKernelGGL(k_subconv_det<9>, grid, block, 0, S.stream, src, sub, cd, par, fgm, S.W, S.H, dmask, sp, dthresh, wa, td); break;
+    default: hipLaunchKernelGGL(k_subconv_det<11>, grid, block, 0, S.stream, src, sub, cd, par, fgm, S.W, S.H, dmask, sp, dthresh, wa, td); break;
     }
     sxg_kt_end(SXG_KT_CONV);
     sxg_stage_end(ST_FILTER);

@@ -332,6 +332,9 @@ void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s,
 void sxg_compact_index(const int *flag, int *out, int n, int *count, hipStream_t s,
                        const int *nd = nullptr);
 /* Same, keeping the indices i with v[i] >= 0. */
+/* the set bits of nw 64-bit window masks as positions 64 w + b, in increasing order */
+void sxg_compact_mask(const unsigned long long *m, long nw, int *out, int *count,
+                      hipStream_t s);
 void sxg_compact_nonneg(const int *v, int *out, int n, int *count, hipStream_t s,
                         const int *nd = nullptr);
 /* Stable LSD radix sort of (key, value) pairs on the low nbits of non-negative keys;

@@ -462,6 +462,46 @@ void sxg_compact_nonneg(const int *v, int *out, int n, int *count, hipStream_t s
     compact_impl<PredNonneg>(v, out, n, count, s, nd);
 }
 
+/* The positions of the set bits of nw 64-bit window masks (bit b of mask w: position
+   64 w + b) in increasing order, and their number: one lane per window counts its bits;
+   after the scan of the workgroup sums each lane writes its window's positions in bit
+   order.  Reads the masks (8 bytes per 64 positions) instead of a flag per position. */
+__global__ void __launch_bounds__(SCAN_BLOCK) k_fgm_count(const unsigned long long *m, long nw,
+                                                          int *bsum)
+{
+    __shared__ int sm[SCAN_BLOCK / 64];
+    const long w = (long)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    int tot;
+    block_excl_scan(w < nw ? __popcll(m[w]) : 0, sm, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_fgm_write(const unsigned long long *m, long nw,
+                                                          const int *bsum, int *out)
+{
+    __shared__ int sm[SCAN_BLOCK / 64];
+    const long w = (long)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    unsigned long long b = w < nw ? m[w] : 0ull;
+    int tot;
+    int pos = block_excl_scan(__popcll(b), sm, &tot) + bsum[blockIdx.x];
+    const int base = (int)(w * 64);
+    while (b) {
+        out[pos++] = base + __builtin_ctzll(b);
+        b &= b - 1;
+    }
+}
+
+void sxg_compact_mask(const unsigned long long *m, long nw, int *out, int *count, hipStream_t s)
+{
+    SxgState &S = sxg_state();
+    const int nb = (int)sxg_div_up(nw, SCAN_BLOCK);
+    if (nb < 1 || !S.scan_tmp.ensure(sizeof(int) * (nb + 64))) return;
+    int *bsum = S.scan_tmp.as<int>();
+    hipLaunchKernelGGL(k_fgm_count, dim3(nb), dim3(SCAN_BLOCK), 0, s, m, nw, bsum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsum, nb, count);
+    hipLaunchKernelGGL(k_fgm_write, dim3(nb), dim3(SCAN_BLOCK), 0, s, m, nw, bsum, out);
+}
+
 /* ------------------------------------------------------------ plan guards ---- */
 __global__ void k_plan_guard(int *v, long long cap, int bit, PlanDev *pd, int *cc)
 {

@@ -333,31 +333,33 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const int *in, int *o
     }
 }
 
-/* n <= SCAN_SMALL: one workgroup; each thread reads its contiguous run of at most 32
-   values into registers (all its loads in flight together), adds them, takes its
-   offset from the workgroup scan and writes the run's prefixes (in == out allowed: a
-   thread writes only the elements it has read itself) */
+/* n <= SCAN_SMALL: one workgroup; the values pass through LDS (coalesced loads and
+   stores), each thread scans a contiguous run of them.  The run length is odd (at most
+   33), so the lanes of a wavefront, which read LDS words a run length apart, hit
+   distinct banks. */
 __global__ void __launch_bounds__(SCAN_SMALL_THREADS) k_scan_small(const int *in, int *out, int n,
                                                                    int *total, const int *nd)
 {
+    extern __shared__ int sv[];
     __shared__ int sm[SCAN_SMALL_THREADS / 64];
-    constexpr int PER_MAX = SCAN_SMALL / SCAN_SMALL_THREADS;
     n = sxg_dn(nd, n);
-    const int per = (n + SCAN_SMALL_THREADS - 1) / SCAN_SMALL_THREADS;
-    const int b = threadIdx.x * per;
-    int v[PER_MAX], s = 0;
-#pragma unroll
-    for (int u = 0; u < PER_MAX; u++) {
-        v[u] = (u < per && b + u < n) ? in[b + u] : 0;
-        s += v[u];
-    }
+#pragma unroll 8
+    for (int k = threadIdx.x; k < n; k += SCAN_SMALL_THREADS) sv[k] = in[k];
+    __syncthreads();
+    const int per = ((n + SCAN_SMALL_THREADS - 1) / SCAN_SMALL_THREADS) | 1;
+    const int b = threadIdx.x * per, e = min(b + per, n);
+    int s = 0;
+    for (int k = b; k < e; k++) s += sv[k];
     int tot;
     int ex = block_excl_scan_t<SCAN_SMALL_THREADS>(s, sm, &tot);
-#pragma unroll
-    for (int u = 0; u < PER_MAX; u++) {
-        if (u < per && b + u < n) out[b + u] = ex;
-        ex += v[u];
+    for (int k = b; k < e; k++) {
+        const int v = sv[k];
+        sv[k] = ex;
+        ex += v;
     }
+    __syncthreads();
+#pragma unroll 8
+    for (int k = threadIdx.x; k < n; k += SCAN_SMALL_THREADS) out[k] = sv[k];
     if (threadIdx.x == 0 && total) *total = tot;
 }
 
@@ -375,8 +377,15 @@ void sxg_scan_excl(const int *in, int *out, int n, int *total, hipStream_t s, co
         return;
     }
     if (n <= SCAN_SMALL) {
-        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SCAN_SMALL_THREADS), 0, s, in, out, n,
-                           total, nd);
+        /* the attribute is per function: set once (thread-safe static initialisation) */
+        static const bool attr = [] {
+            return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_scan_small),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       SCAN_SMALL * (int)sizeof(int)) == hipSuccess;
+        }();
+        (void)attr;
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SCAN_SMALL_THREADS),
+                           (size_t)n * sizeof(int), s, in, out, n, total, nd);
         return;
     }
     const int nb = sxg_div_up(n, SCAN_TILE);

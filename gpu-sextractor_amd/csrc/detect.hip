@@ -8,8 +8,10 @@
  *
  * Kernels:
  *   k_det_init     one lane per pixel: union-find parent = the first pixel of its run
- *                  inside its 64-pixel window (a ballot), or -1
- *   (compaction)   detected pixel indices in raster order (the foreground list)
+ *                  inside its 64-pixel window (a ballot), or -1; and the window's 64-bit
+ *                  detection mask
+ *   (compaction)   detected pixel indices in raster order (the foreground list), from
+ *                  the window masks
  *   k_det_union    one lane per detected pixel: lock-free union across window edges and
  *                  with the row above, once per overlapping stretch.  A root is only ever linked under a
  *                  smaller index (atomicMin, retried with the value found), so every
@@ -17,7 +19,8 @@
  *                  no host convergence loop (the reference loops with a host sync per
  *                  iteration, cudadetection.cu:610-638)
  *   k_det_flatten  label = root; roots remember their foreground position
- *   k_det_rootstat per root: size (atomics), peak of cd, bounding box
+ *   k_det_rootstat per root: size (atomics), peak of cd, bounding box; every entry's
+ *                  root entry (rk), which the root flags and the pixel keys read
  *   (compaction)   roots with size >= DETECT_MINAREA, in raster order = raw objects
  *   (radix sort)   pixels of raw objects grouped by object id, stable, so each object's
  *                  pixels stay in raster order
@@ -182,7 +185,7 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
                                                       const int *__restrict__ lab,
                                                       const int *__restrict__ rootpos,
                                                       const float *__restrict__ cd, RootStat rs,
-                                                      WeightArgs wa, const int *nd)
+                                                      int *rkout, WeightArgs wa, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -195,6 +198,8 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
     int rk = -1;
     if (act && h == lane) rk = rootpos[lab[pp]];
     rk = __shfl(rk, h);
+    /* every entry's root entry, for the root flags and the pixel keys */
+    if (act) rkout[k] = rk;
     int pv = act ? f2ord(cd[pp]) : (int)0x80000000;
     int tv = (act && perpix) ? f2ord(det_thresh(wa, 0.0f, pp)) : 0x7fffffff;
 #pragma unroll
@@ -222,16 +227,16 @@ __global__ void __launch_bounds__(256) k_det_rootstat(const int *__restrict__ fg
     if (perpix) atomicMin(&rs.t0[rk], tv);
 }
 
-/* keep[k] = k for kept roots (size >= minarea), -1 otherwise */
-__global__ void __launch_bounds__(256) k_det_rootflag(const int *__restrict__ fg, int nfg,
-                                                      const int *__restrict__ lab,
+/* keep[k] = k for kept roots (size >= minarea), -1 otherwise; entry k is a root when its
+   root entry rk[k] is itself */
+__global__ void __launch_bounds__(256) k_det_rootflag(const int *__restrict__ rk, int nfg,
                                                       const int *__restrict__ cnt, int minarea,
                                                       int *keep, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= sxg_dn(nd, nfg)) return;
-    const int p = fg[k];
-    keep[k] = (lab[p] == p && cnt[k] >= minarea) ? k : -1;
+    const int r = rk[k], c = cnt[k];
+    keep[k] = (r == k && c >= minarea) ? k : -1;
 }
 
 __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ rawpos, int nraw,
@@ -257,21 +262,13 @@ __global__ void __launch_bounds__(256) k_det_rawinfo(const int *__restrict__ raw
 
 
 /* object id of each detected pixel (or -1 when its component is too small) */
-__global__ void __launch_bounds__(256) k_det_pixkey(const int *__restrict__ fg, int nfg, int W,
-                                                    const int *__restrict__ lab,
-                                                    const int *__restrict__ rootpos,
+__global__ void __launch_bounds__(256) k_det_pixkey(const int *__restrict__ rk, int nfg,
                                                     const int *__restrict__ rawid_of_pos,
                                                     int *key, const int *nd)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    const bool act = k < sxg_dn(nd, nfg);
-    const int p = act ? fg[k] : -1;
-    /* one lookup chain per run of the wavefront */
-    const int h = run_head(act, p, W);
-    int v = -1;
-    if (act && h == (int)(threadIdx.x & 63)) v = rawid_of_pos[rootpos[lab[p]]];
-    v = __shfl(v, h);
-    if (act) key[k] = v;
+    if (k >= sxg_dn(nd, nfg)) return;
+    key[k] = rawid_of_pos[rk[k]];
 }
 
 __global__ void __launch_bounds__(256) k_det_gather(const int *__restrict__ idx, int n,
@@ -381,8 +378,9 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
     int *keep = sxg_take<int>("det.keep", nfg);
     int *rawpos = sxg_take<int>("det.rawpos", nfg);
     int *rawid_of_pos = sxg_take<int>("det.rawid", nfg);
+    int *rk = sxg_take<int>("det.rk", nfg);
     if (!rs.cnt || !rs.peak || !rs.xmin || !rs.xmax || !rs.ymax || !rs.t0 || !keep ||
-        !rawpos || !rawid_of_pos)
+        !rawpos || !rawid_of_pos || !rk)
         return;
     if (nfg > 0) {
         sxg_kt_begin(SXG_KT_DET_UNION);
@@ -394,9 +392,9 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
         sxg_kt_end(SXG_KT_DET_FLATTEN);
         sxg_kt_begin(SXG_KT_DET_STAT);
         hipLaunchKernelGGL(k_det_rootstat, dim3(gfg), dim3(256), 0, st, fg, nfg, W, lab, rootpos,
-                           S.cd.as<float>(), rs, wa, nfg_k);
+                           S.cd.as<float>(), rs, rk, wa, nfg_k);
         sxg_kt_end(SXG_KT_DET_STAT);
-        hipLaunchKernelGGL(k_det_rootflag, dim3(gfg), dim3(256), 0, st, fg, nfg, lab, rs.cnt,
+        hipLaunchKernelGGL(k_det_rootflag, dim3(gfg), dim3(256), 0, st, rk, nfg, rs.cnt,
                            ext_minarea, keep, nfg_k);
     }
     sxg_compact_nonneg(keep, rawpos, nfg, nraw_d, st, nfg_k);
@@ -418,8 +416,8 @@ extern "C" void run_detection(float dthreshold, float thresh1, int ext_minarea,
                                (int)(wa.w && wa.rel),
                                nraw_k, td);
         /* pixel keys, kept pixels, stable sort by object */
-        hipLaunchKernelGGL(k_det_pixkey, dim3(gfg), dim3(256), 0, st, fg, nfg, W, lab,
-                           rootpos, rawid_of_pos, keep, nfg_k);
+        hipLaunchKernelGGL(k_det_pixkey, dim3(gfg), dim3(256), 0, st, rk, nfg, rawid_of_pos,
+                           keep, nfg_k);
     }
     int *idx = rawpos;   /* reuse: indices of kept pixels in the foreground list */
     int *k0 = sxg_take<int>("det.k0", nfg), *v0 = sxg_take<int>("det.v0", nfg);

@@ -149,8 +149,8 @@ struct RootStat {
    while other lanes still walk: the root is an ancestor with the smallest index, so the
    path halving's atomicMin never moves a pointer past it), and roots remember their
    foreground position.  Only the first pixel of each run in a wavefront walks; the
-   others take its root.  The root statistics and the raw-object index of each
-   foreground position start here (no fill launches). */
+   others take its root.  The root statistics and the raw-object index of each root's
+   entry start here (no fill launches). */
 __global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg, int nfg, int W,
                                                      int *par, int *rootpos, RootStat rs,
                                                      int *rawid_of_pos, const int *nd)
@@ -164,7 +164,11 @@ __global__ void __launch_bounds__(256) k_det_flatten(const int *__restrict__ fg,
     r = __shfl(r, h);
     if (!act) return;
     par[p] = r;
-    if (r == p) rootpos[p] = k;
+    if (r != p) return;
+    /* a root: its position, and the statistics and raw-object index of its entry (only
+       root entries are ever read: the atomics and k_det_rawinfo address them, the pixel
+       keys look up the root entry, the root flags use a count only at a root) */
+    rootpos[p] = k;
     rs.cnt[k] = 0;
     rs.peak[k] = (int)0x80000000;   /* below f2ord of every float */
     rs.xmin[k] = 0x7fffffff;
